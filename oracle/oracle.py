@@ -1,0 +1,336 @@
+"""ctypes front-end of the CPU oracle (oracle/sbh_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as
+the checker, and by bench.py's cpu_baseline leg.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+LEAF3_DTYPE = np.dtype(
+    [("pos", "<f4", (3,)), ("normal", "<f4", (3,)), ("mass", "<f4"), ("alpha", "<f4")]
+)
+NODE8_DTYPE = np.dtype(
+    [
+        ("start", "<u4"),
+        ("end", "<u4"),
+        ("parent", "<u4"),
+        ("first_child", "<u4"),
+        ("child", "<u4", (8,)),
+        ("child_mask", "u1"),
+        ("depth", "u1"),
+        ("is_leaf", "u1"),
+        ("pad0", "u1"),
+        ("cx", "<f4"),
+        ("cy", "<f4"),
+        ("cz", "<f4"),
+        ("half_size", "<f4"),
+        ("diameter", "<f4"),
+        ("contrib", LEAF3_DTYPE),
+    ]
+)
+assert LEAF3_DTYPE.itemsize == 32 and NODE8_DTYPE.itemsize == 104
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_i64 = ctypes.c_int64
+_vp = ctypes.c_void_p
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    L.orc_quantize_axis.restype = ctypes.c_uint32
+    L.orc_quantize_axis.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    L.orc_morton3d.restype = ctypes.c_uint32
+    L.orc_morton3d.argtypes = [ctypes.c_uint32] * 3 + [ctypes.c_int]
+    L.orc_morton_codes.argtypes = [_f32p, _i64, _f32p, ctypes.c_float, ctypes.c_int, _u32p]
+    L.orc_default_bounds.argtypes = [_f32p, _i64, _f32p, _f32p]
+    L.orc_span.restype = ctypes.c_float
+    L.orc_span.argtypes = [_f32p, _f32p]
+    L.orc_stable_order.argtypes = [_u32p, _i64, ctypes.c_int, _u32p]
+    L.orc_octree8_total_nodes.restype = ctypes.c_uint64
+    L.orc_octree8_total_nodes.argtypes = [ctypes.c_int]
+    L.orc_build_octree8.restype = ctypes.c_int
+    L.orc_build_octree8.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _f32p, ctypes.c_int,
+                                    _u8p, _u8p, _u8p]
+    L.orc_direct_force_soft.argtypes = [_f32p, _f32p, _i64, _f32p, _i64, ctypes.c_double,
+                                        ctypes.c_double, ctypes.c_int, ctypes.c_double, _f64p]
+    L.orc_direct_force_soft_vjp.argtypes = [_f32p, _f32p, _i64, _f32p, _f32p, _i64,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, _f64p]
+    L.orc_direct_full_vjp.argtypes = [_f32p, _f32p, _i64, _f32p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_double, _f64p, _f64p]
+    L.orc_bruteforce_gravity.argtypes = [_f32p, _f32p, _i64, ctypes.c_int, _f32p, _i64,
+                                         ctypes.c_int, ctypes.c_double, _f64p]
+    L.orc_bh_octree8.argtypes = [_u8p, _u8p, _f32p, _vp, _i64, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_int, ctypes.c_float, _f64p, ctypes.POINTER(_i64)]
+    L.orc_tree_build.restype = _i64
+    L.orc_tree_build.argtypes = [_u32p, _i64, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float,
+                                 _i64, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f32p]
+    L.orc_tree_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_group_boxes.argtypes = [_f32p, _i64, ctypes.c_int, _f32p]
+    L.orc_tree_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
+                                _f64p, _f64p, _i64, _f32p, _f64p, _i64, ctypes.c_int,
+                                ctypes.c_float, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                _vp, _f64p, _vp, _vp, _vp, ctypes.POINTER(_i64)]
+    L.orc_philox4x32.argtypes = [_u32p, _u32p, _u32p]
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(np.asarray(a), dtype=dtype)
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- Morton / bounds
+def quantize_axis(v, min_v, span, bits):
+    return int(lib().orc_quantize_axis(float(np.float32(v)), float(np.float32(min_v)),
+                                       float(np.float32(span)), int(bits)))
+
+
+def morton3d(x, y, z, bits):
+    return int(lib().orc_morton3d(int(x), int(y), int(z), int(bits)))
+
+
+def default_bounds(points):
+    p = _c(points, np.float32)
+    lo = np.zeros(3, np.float32)
+    hi = np.zeros(3, np.float32)
+    lib().orc_default_bounds(p, p.shape[0], lo, hi)
+    return lo, hi
+
+
+def span(min_corner, max_corner):
+    return np.float32(lib().orc_span(_c(min_corner, np.float32), _c(max_corner, np.float32)))
+
+
+def morton_codes(points, min_corner, span_, bits):
+    p = _c(points, np.float32)
+    out = np.zeros(p.shape[0], np.uint32)
+    lib().orc_morton_codes(p, p.shape[0], _c(min_corner, np.float32), float(span_), bits, out)
+    return out
+
+
+def stable_order(codes, bits3):
+    c = _c(codes, np.uint32)
+    out = np.zeros(c.shape[0], np.uint32)
+    lib().orc_stable_order(c, c.shape[0], int(bits3), out)
+    return out
+
+
+# ---------------------------------------------------------------- octree8
+def octree8_total_nodes(max_depth):
+    return int(lib().orc_octree8_total_nodes(int(max_depth)))
+
+
+def build_octree8(points, normals, masses, min_corner, max_corner, max_depth):
+    p = _c(points, np.float32)
+    nrm = _c(normals, np.float32)
+    m = _c(masses, np.float32)
+    n = p.shape[0]
+    if not 1 <= int(max_depth) <= 10:
+        raise ValueError("max_depth must be in [1, 10]")
+    t = octree8_total_nodes(max_depth)
+    leaf = np.zeros(n * 32, np.uint8)
+    node = np.zeros(t * 104, np.uint8)
+    contrib = np.zeros(t * 32, np.uint8)
+    rc = lib().orc_build_octree8(p, nrm, m, n, _c(min_corner, np.float32),
+                                 _c(max_corner, np.float32), int(max_depth), leaf, node, contrib)
+    if rc != 0:
+        raise ValueError("invalid octree8 arguments")
+    return leaf, node, contrib
+
+
+def decode_nodes8(node_bytes):
+    return np.frombuffer(np.ascontiguousarray(node_bytes, dtype=np.uint8).tobytes(), NODE8_DTYPE)
+
+
+def decode_leaf3(leaf_bytes):
+    return np.frombuffer(np.ascontiguousarray(leaf_bytes, dtype=np.uint8).tobytes(), LEAF3_DTYPE)
+
+
+# ---------------------------------------------------------------- direct sums
+def direct_force_soft(points, masses, queries, eps, rc, periodic=False, box=1.0):
+    p, m, q = _c(points, np.float32), _c(masses, np.float32), _c(queries, np.float32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_direct_force_soft(p, m, p.shape[0], q, q.shape[0], float(eps), float(rc),
+                                int(bool(periodic)), float(box), out)
+    return out
+
+
+def direct_force_soft_vjp(points, masses, queries, cot, eps, rc, periodic=False, box=1.0):
+    p, m, q = _c(points, np.float32), _c(masses, np.float32), _c(queries, np.float32)
+    g = _c(cot, np.float32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_direct_force_soft_vjp(p, m, p.shape[0], q, g, q.shape[0], float(eps), float(rc),
+                                    int(bool(periodic)), float(box), out)
+    return out
+
+
+def direct_full_vjp(points, masses, cot, eps, periodic=False, box=1.0):
+    p, m, g = _c(points, np.float32), _c(masses, np.float32), _c(cot, np.float32)
+    gx = np.zeros((p.shape[0], 3), np.float64)
+    gm = np.zeros(p.shape[0], np.float64)
+    lib().orc_direct_full_vjp(p, m, p.shape[0], g, float(eps), int(bool(periodic)), float(box),
+                              gx, gm)
+    return gx, gm
+
+
+def bruteforce_gravity(points, masses, queries, periodic=False, box=1.0):
+    p, m, q = _c(points, np.float32), _c(masses, np.float32), _c(queries, np.float32)
+    out = np.zeros(q.shape[0], np.float64)
+    lib().orc_bruteforce_gravity(p, m, p.shape[0], p.shape[1], q, q.shape[0],
+                                 int(bool(periodic)), float(box), out)
+    return out
+
+
+# ---------------------------------------------------------------- octree8 walks
+KIND_GRAV, KIND_SOFT, KIND_SOFT_VJP, KIND_GRAV_VJP = 0, 1, 2, 3
+
+
+def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot=None,
+               prune_enabled=False, prune_r_cut_mult=4.0, periodic=False, box=1.0):
+    leaf = _c(leaf_bytes, np.uint8)
+    node = _c(node_bytes, np.uint8)
+    q = _c(queries, np.float32)
+    g = None if cot is None else _c(cot, np.float32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    cnt = _i64(0)
+    lib().orc_bh_octree8(leaf, node, q, _ptr(g), q.shape[0], int(kind), float(beta),
+                         float(eps), float(rc), int(bool(prune_enabled)),
+                         float(prune_r_cut_mult), int(bool(periodic)), float(box), out,
+                         ctypes.byref(cnt))
+    return out, int(cnt.value)
+
+
+# ---------------------------------------------------------------- adaptive tree
+class Tree:
+    """Adaptive Morton octree built by the oracle (topology in numpy arrays)."""
+
+    def __init__(self, points, masses, max_depth=10, leaf_size=32, min_corner=None,
+                 max_corner=None):
+        p = _c(points, np.float32)
+        self.n = p.shape[0]
+        if min_corner is None or max_corner is None:
+            lo, hi = default_bounds(p)
+            min_corner = lo if min_corner is None else min_corner
+            max_corner = hi if max_corner is None else max_corner
+        self.min_corner = _c(min_corner, np.float32)
+        self.max_corner = _c(max_corner, np.float32)
+        self.span = span(self.min_corner, self.max_corner)
+        self.max_depth = int(max_depth)
+        self.leaf_size = int(leaf_size)
+        codes = morton_codes(p, self.min_corner, self.span, self.max_depth)
+        self.perm = stable_order(codes, 3 * self.max_depth)
+        self.codes = codes[self.perm]
+        self.points_sorted = p[self.perm]
+        self.masses_sorted = _c(masses, np.float32)[self.perm]
+        cap = 1 + 8 * self.max_depth * (self.n // (self.leaf_size + 1) + 1)
+        cap = max(cap, 16)
+        arrs = [np.zeros(cap, np.int32) for _ in range(6)]
+        geo = np.zeros(cap * 4, np.float32)
+        nn = lib().orc_tree_build(self.codes, self.n, self.max_depth, self.leaf_size,
+                                  self.min_corner, float(self.span), cap, *arrs, geo)
+        if nn < 0:
+            raise RuntimeError("node capacity exceeded")
+        self.num_nodes = int(nn)
+        (self.start, self.count, self.first_child, self.nchild, self.parent,
+         self.level) = [a[:nn].copy() for a in arrs]
+        self.geo = geo[: 4 * nn].reshape(nn, 4).copy()
+
+    def moments(self, xs=None, ms=None):
+        xs = self.points_sorted.astype(np.float64) if xs is None else _c(xs, np.float64)
+        ms = self.masses_sorted.astype(np.float64) if ms is None else _c(ms, np.float64)
+        nn = self.num_nodes
+        mass = np.zeros(nn, np.float64)
+        com = np.zeros((nn, 3), np.float64)
+        quad = np.zeros((nn, 6), np.float64)
+        lib().orc_tree_moments(self.start, self.count, nn, _c(xs, np.float64),
+                               _c(ms, np.float64), mass, com, quad)
+        return mass, com, quad
+
+    def group_boxes(self, q_sorted_f32, group_size):
+        q = _c(q_sorted_f32, np.float32)
+        ng = (q.shape[0] + group_size - 1) // group_size
+        boxes = np.zeros(ng * 6, np.float32)
+        lib().orc_group_boxes(q, q.shape[0], int(group_size), boxes)
+        return boxes
+
+    def eval(self, *, beta=2.0, eps=0.0, use_quad=True, group_size=64, cot_sorted=None,
+             xs=None, ms=None, boxes=None):
+        """Self-gravity walk over the sorted sources (queries == sources).
+
+        xs/ms override the source positions/masses (fp64, sorted) while the
+        topology and the group boxes stay frozen -- used for finite differences.
+        Returns dict(force, gq, gsx, gsm, far, near) in sorted order.
+        """
+        xs = self.points_sorted.astype(np.float64) if xs is None else _c(xs, np.float64)
+        ms = self.masses_sorted.astype(np.float64) if ms is None else _c(ms, np.float64)
+        if boxes is None:
+            boxes = self.group_boxes(self.points_sorted, group_size)
+        mass, com, quad = self.moments(xs, ms)
+        n = self.n
+        force = np.zeros((n, 3), np.float64)
+        cot = None if cot_sorted is None else _c(cot_sorted, np.float64)
+        gq = np.zeros((n, 3), np.float64) if cot is not None else None
+        gsx = np.zeros((n, 3), np.float64) if cot is not None else None
+        gsm = np.zeros(n, np.float64) if cot is not None else None
+        counts = (_i64 * 2)()
+        lib().orc_tree_eval(self.num_nodes, self.start, self.count, self.first_child,
+                            self.nchild, _c(self.geo, np.float32).ravel(), mass,
+                            com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n,
+                            int(group_size), float(beta), int(bool(use_quad)), float(eps), 1,
+                            _ptr(cot), force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts)
+        return dict(force=force, gq=gq, gsx=gsx, gsm=gsm, far=int(counts[0]),
+                    near=int(counts[1]))
+
+
+# ---------------------------------------------------------------- Philox
+def philox4x32(ctr, key):
+    c = _c(ctr, np.uint32)
+    k = _c(key, np.uint32)
+    out = np.zeros(4, np.uint32)
+    lib().orc_philox4x32(c, k, out)
+    return out
+
+
+def rel_l2(pred, ref):
+    pred = np.asarray(pred, np.float64)
+    ref = np.asarray(ref, np.float64)
+    return float(np.linalg.norm((pred - ref).ravel()) / max(np.linalg.norm(ref.ravel()), 1e-12))
+
+
+def outlier_frac(pred, ref, thr=0.05):
+    pred = np.asarray(pred, np.float64)
+    ref = np.asarray(ref, np.float64)
+    e = np.linalg.norm(pred - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-12)
+    return float(np.mean(e > thr))

@@ -1,0 +1,959 @@
+/*
+ * sbh_oracle.c -- CPU restatement of the reference Barnes-Hut path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links or calls this file:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it,
+ * and only as the checker (or, in bench.py, as the timed CPU baseline).
+ *
+ * Every function cites the reference lines it restates.  Reference files are
+ * cited relative to the reference repository root.
+ *
+ * Arithmetic conventions
+ *   - Morton quantisation, bounds, node geometry and MAC decisions are
+ *     evaluated in IEEE fp32 with the reference's operation order (this file is
+ *     compiled with -ffp-contract=off; the one contraction the reference's
+ *     compiler performs by default -- the node-centre a+b*c -- is written as an
+ *     explicit fmaf).
+ *   - Sums (moments, forces, gradients) are accumulated in fp64: the reference
+ *     accumulates in fp32 with atomics in an undefined order, so only a
+ *     tolerance comparison is meaningful there.
+ *
+ * Pinning status (what checks each function; see tests/test_oracle_*.py)
+ *   orc_quantize_axis / orc_morton3d      hand-computed codes + integer brute force
+ *   orc_build_octree8                     invariants + independent numpy build (tiny N)
+ *   orc_direct_*                          closed forms, Newton's third law,
+ *                                         potential gradient, central differences
+ *   orc_bh_octree8_*                      beta->inf equals direct sum, beta=0 equals
+ *                                         root monopole, prune equivalence
+ *   orc_tree_*                            topology invariants, beta->inf == direct,
+ *                                         quadrupole error order, FD of the adjoint
+ *   orc_philox4x32                        Random123 known-answer vectors
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef __cplusplus
+#error "compile as C"
+#endif
+
+#define ORC_EXPORT __attribute__((visibility("default")))
+
+/* ------------------------------------------------------------------------- */
+/* Byte layouts (include/leaf_data.h:8-16; src/sbh_ffi.cu:47-65)              */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  float pos[3];
+  float normal[3];
+  float mass;
+  float alpha;
+} OrcLeaf3; /* 32 bytes */
+
+typedef struct {
+  uint32_t start, end, parent, first_child;
+  uint32_t child[8];
+  uint8_t child_mask, depth, is_leaf, pad0;
+  float cx, cy, cz, half_size, diameter;
+  OrcLeaf3 contrib;
+} OrcNode8; /* 104 bytes */
+
+_Static_assert(sizeof(OrcLeaf3) == 32, "Leaf3 layout");
+_Static_assert(sizeof(OrcNode8) == 104, "OctreeNode8 layout");
+
+ORC_EXPORT size_t orc_leaf3_size(void) { return sizeof(OrcLeaf3); }
+ORC_EXPORT size_t orc_node8_size(void) { return sizeof(OrcNode8); }
+
+/* ------------------------------------------------------------------------- */
+/* B1: quantisation and Morton interleave (src/sbh_ffi.cu:162-179)            */
+/* ------------------------------------------------------------------------- */
+ORC_EXPORT uint32_t orc_quantize_axis(float v, float min_v, float span, int bits) {
+  float t = (v - min_v) / span;
+  t = fminf(fmaxf(t, 0.0f), 0.999999f);
+  uint32_t max_bin = (1u << bits) - 1u;
+  return (uint32_t)(t * (float)(max_bin + 1u));
+}
+
+ORC_EXPORT uint32_t orc_morton3d(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t code = 0;
+  for (int i = 0; i < bits; i++) {
+    code |= ((x >> i) & 1u) << (3 * i);
+    code |= ((y >> i) & 1u) << (3 * i + 1);
+    code |= ((z >> i) & 1u) << (3 * i + 2);
+  }
+  return code;
+}
+
+ORC_EXPORT void orc_morton_codes(const float* pts, int64_t n, const float* min_corner,
+                                 float span, int bits, uint32_t* codes) {
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t qx = orc_quantize_axis(pts[3 * i + 0], min_corner[0], span, bits);
+    uint32_t qy = orc_quantize_axis(pts[3 * i + 1], min_corner[1], span, bits);
+    uint32_t qz = orc_quantize_axis(pts[3 * i + 2], min_corner[2], span, bits);
+    codes[i] = orc_morton3d(qx, qy, qz, bits);
+  }
+}
+
+/* B2: default bounds (python/jax_sbh_ffi.py:879-884: min-1e-3 / max+1e-3 in
+ * fp32) and the cube span (src/sbh_ffi.cu:1128-1130). */
+ORC_EXPORT void orc_default_bounds(const float* pts, int64_t n, float* min_corner,
+                                   float* max_corner) {
+  for (int a = 0; a < 3; a++) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int64_t i = 0; i < n; i++) {
+      float v = pts[3 * i + a];
+      lo = fminf(lo, v);
+      hi = fmaxf(hi, v);
+    }
+    min_corner[a] = lo - 1e-3f;
+    max_corner[a] = hi + 1e-3f;
+  }
+}
+
+ORC_EXPORT float orc_span(const float* min_corner, const float* max_corner) {
+  return fmaxf(fmaxf(max_corner[0] - min_corner[0], max_corner[1] - min_corner[1]),
+               max_corner[2] - min_corner[2]);
+}
+
+/* Stable refinement of the reference's counting-sort scatter
+ * (src/sbh_ffi.cu:1723-1743): points ordered by (code, original index).
+ * The reference's intra-bin order is an atomic race; this is the
+ * deterministic order the product builder must reproduce bit-exactly. */
+ORC_EXPORT void orc_stable_order(const uint32_t* codes, int64_t n, int bits3,
+                                 uint32_t* perm) {
+  /* LSD radix sort, 8-bit digits: an independent algorithm from the GPU's. */
+  uint32_t* a = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  uint32_t* b = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) a[i] = (uint32_t)i;
+  for (int shift = 0; shift < bits3; shift += 8) {
+    int64_t cnt[257];
+    memset(cnt, 0, sizeof(cnt));
+    for (int64_t i = 0; i < n; i++) cnt[((codes[a[i]] >> shift) & 255u) + 1]++;
+    for (int d = 0; d < 256; d++) cnt[d + 1] += cnt[d];
+    for (int64_t i = 0; i < n; i++) b[cnt[(codes[a[i]] >> shift) & 255u]++] = a[i];
+    uint32_t* t = a;
+    a = b;
+    b = t;
+  }
+  memcpy(perm, a, sizeof(uint32_t) * (size_t)n);
+  free(a);
+  free(b);
+}
+
+/* ------------------------------------------------------------------------- */
+/* B3-B8: dense octree8 builder (src/sbh_ffi.cu:1039-1420, 1682-1876)         */
+/* ------------------------------------------------------------------------- */
+ORC_EXPORT uint64_t orc_octree8_total_nodes(int max_depth) {
+  uint64_t total = 0, level = 1;
+  for (int l = 0; l <= max_depth; l++) {
+    total += level;
+    level <<= 3;
+  }
+  return total;
+}
+
+/* Returns 0 on success, 1 for the reference's InvalidArgument branches. */
+ORC_EXPORT int orc_build_octree8(const float* pts, const float* normals,
+                                 const float* masses, int64_t n,
+                                 const float* min_corner, const float* max_corner,
+                                 int max_depth, uint8_t* leaf_out, uint8_t* node_out,
+                                 uint8_t* contrib_out) {
+  if (max_depth < 1 || max_depth > 10) return 1;
+  float span = orc_span(min_corner, max_corner);
+  if (!(span > 0.0f)) return 1; /* "invalid bounds span" (sbh_ffi.cu:1131-1133) */
+  const uint32_t leaf_count = 1u << (3 * max_depth);
+  const uint64_t total_nodes = orc_octree8_total_nodes(max_depth);
+
+  OrcLeaf3* leaf = (OrcLeaf3*)leaf_out;
+  OrcNode8* nodes = (OrcNode8*)node_out;
+  OrcLeaf3* contrib = (OrcLeaf3*)contrib_out;
+  memset(leaf_out, 0, sizeof(OrcLeaf3) * (size_t)n);
+  memset(node_out, 0, sizeof(OrcNode8) * (size_t)total_nodes);
+  memset(contrib_out, 0, sizeof(OrcLeaf3) * (size_t)total_nodes);
+
+  uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  orc_morton_codes(pts, n, min_corner, span, max_depth, codes);
+  orc_stable_order(codes, n, 3 * max_depth, perm);
+
+  /* Per-level sums, level D first (histogram, sbh_ffi.cu:1706-1713). */
+  uint32_t** cnt = (uint32_t**)calloc((size_t)max_depth + 1, sizeof(uint32_t*));
+  double** sums = (double**)calloc((size_t)max_depth + 1, sizeof(double*)); /* 7/bin */
+  for (int l = 0; l <= max_depth; l++) {
+    size_t c = (size_t)1 << (3 * l);
+    cnt[l] = (uint32_t*)calloc(c, sizeof(uint32_t));
+    sums[l] = (double*)calloc(c * 7, sizeof(double));
+  }
+  for (int64_t i = 0; i < n; i++) {
+    uint32_t c = codes[i];
+    float m = masses[i];
+    double* s = sums[max_depth] + 7 * (size_t)c;
+    cnt[max_depth][c]++;
+    s[0] += m;
+    s[1] += (double)(m * pts[3 * i + 0]);
+    s[2] += (double)(m * pts[3 * i + 1]);
+    s[3] += (double)(m * pts[3 * i + 2]);
+    s[4] += (double)(m * normals[3 * i + 0]);
+    s[5] += (double)(m * normals[3 * i + 1]);
+    s[6] += (double)(m * normals[3 * i + 2]);
+  }
+  /* Exclusive scan of bin counts (sbh_ffi.cu:1303-1321). */
+  uint32_t* offsets = (uint32_t*)calloc((size_t)leaf_count + 1, sizeof(uint32_t));
+  for (uint32_t b = 0; b < leaf_count; b++) offsets[b + 1] = offsets[b] + cnt[max_depth][b];
+
+  /* Scatter in stable order (sbh_ffi.cu:1723-1743). */
+  for (int64_t r = 0; r < n; r++) {
+    uint32_t i = perm[r];
+    OrcLeaf3* o = &leaf[r];
+    o->pos[0] = pts[3 * i + 0];
+    o->pos[1] = pts[3 * i + 1];
+    o->pos[2] = pts[3 * i + 2];
+    o->normal[0] = normals[3 * i + 0];
+    o->normal[1] = normals[3 * i + 1];
+    o->normal[2] = normals[3 * i + 2];
+    o->mass = masses[i];
+    o->alpha = 1.0f;
+  }
+
+  /* Bottom-up reduction over 8 children (sbh_ffi.cu:1745-1786). */
+  for (int l = max_depth - 1; l >= 0; l--) {
+    size_t c = (size_t)1 << (3 * l);
+    for (size_t p = 0; p < c; p++) {
+      for (int k = 0; k < 8; k++) {
+        size_t ch = p * 8 + (size_t)k;
+        if (cnt[l + 1][ch] == 0) continue;
+        cnt[l][p] += cnt[l + 1][ch];
+        for (int q = 0; q < 7; q++) sums[l][7 * p + q] += sums[l + 1][7 * ch + q];
+      }
+    }
+  }
+
+  /* Node records (sbh_ffi.cu:1390-1416, 1788-1876). */
+  float cx = min_corner[0] + span * 0.5f;
+  float cy = min_corner[1] + span * 0.5f;
+  float cz = min_corner[2] + span * 0.5f;
+  float half_size = span * 0.5f;
+  uint32_t level_offset = 0, parent_level_offset = 0;
+  for (int l = 0; l <= max_depth; l++) {
+    uint32_t level_nodes = 1u << (3 * l);
+    uint32_t child_level_offset = level_offset + level_nodes;
+    for (uint32_t idx = 0; idx < level_nodes; idx++) {
+      uint32_t node_idx = level_offset + idx;
+      OrcNode8 node;
+      memset(&node, 0, sizeof(node));
+      node.depth = (uint8_t)l;
+      node.parent = (l == 0) ? node_idx : (parent_level_offset + (idx >> 3));
+      node.is_leaf = (l == max_depth) ? 1 : 0;
+      node.child_mask = 0;
+      node.first_child = 0xFFFFFFFFu;
+      int shift = 3 * (max_depth - l);
+      uint32_t first_leaf = idx << shift;
+      node.start = offsets[first_leaf];
+      node.end = node.start + cnt[l][idx];
+      int grid = 1 << l;
+      int ix = 0, iy = 0, iz = 0;
+      for (int b = 0; b < l; b++) {
+        ix |= (int)((idx >> (3 * b + 0)) & 1u) << b;
+        iy |= (int)((idx >> (3 * b + 1)) & 1u) << b;
+        iz |= (int)((idx >> (3 * b + 2)) & 1u) << b;
+      }
+      float step = half_size * 2.0f / (float)grid;
+      node.cx = fmaf((float)ix + 0.5f, step, cx - half_size);
+      node.cy = fmaf((float)iy + 0.5f, step, cy - half_size);
+      node.cz = fmaf((float)iz + 0.5f, step, cz - half_size);
+      node.half_size = step * 0.5f;
+      node.diameter = step * 1.7320508075688772f;
+      if (l < max_depth) {
+        uint32_t child_base = child_level_offset + (idx << 3);
+        uint8_t mask = 0;
+        for (uint32_t c = 0; c < 8; c++) {
+          node.child[c] = child_base + c;
+          if (cnt[l + 1][(idx << 3) + c] > 0) {
+            mask |= (uint8_t)(1u << c);
+            if (node.first_child == 0xFFFFFFFFu) node.first_child = child_base + c;
+          }
+        }
+        node.child_mask = mask;
+        if (mask == 0) node.is_leaf = 1;
+      } else {
+        for (int c = 0; c < 8; c++) node.child[c] = 0xFFFFFFFFu;
+      }
+      const double* s = sums[l] + 7 * (size_t)idx;
+      float mass = (float)s[0];
+      node.contrib.alpha = 1.0f;
+      node.contrib.mass = mass;
+      if (mass > 0.0f) {
+        for (int a = 0; a < 3; a++) {
+          node.contrib.pos[a] = (float)(s[1 + a] / s[0]);
+          node.contrib.normal[a] = (float)(s[4 + a] / s[0]);
+        }
+      }
+      nodes[node_idx] = node;
+      contrib[node_idx] = node.contrib;
+    }
+    parent_level_offset = level_offset;
+    level_offset = child_level_offset;
+  }
+
+  for (int l = 0; l <= max_depth; l++) {
+    free(cnt[l]);
+    free(sums[l]);
+  }
+  free(cnt);
+  free(sums);
+  free(offsets);
+  free(codes);
+  free(perm);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Pair kernels in fp64 (src/sbh_ffi.cu:142-160, 431-501, 2044-2090;          */
+/* include/kernel.h:27-39, 55-60, 73-80, 89-94)                              */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  double eps;       /* max(softening_scale, 0) */
+  double rc;        /* cutoff_scale; <= 0 or inf => attenuation 1 */
+  int periodic;     /* min-image active */
+  double box;       /* box length */
+  int nearbyint_mi; /* 1: direct-sum min-image (nearbyint), 0: kernel.h floor(x+0.5) */
+} OrcParams;
+
+static inline double orc_mi(double d, const OrcParams* P) {
+  if (!P->periodic || !(P->box > 0.0)) return d;
+  double inv_l = 1.0 / P->box;
+  if (P->nearbyint_mi) return d - P->box * nearbyint(d * inv_l);
+  return d - P->box * floor(d * inv_l + 0.5);
+}
+
+static inline int orc_has_cutoff(const OrcParams* P) {
+  return P->rc > 0.0 && isfinite(P->rc);
+}
+
+/* Softened+cutoff force of a unit... source (p, m) on q: -m*att*s^3*d. */
+static inline void orc_soft_force(const double* q, const double* p, double m,
+                                  const OrcParams* P, double* f) {
+  double d[3];
+  for (int a = 0; a < 3; a++) d[a] = orc_mi(q[a] - p[a], P);
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double s = 1.0 / sqrt(r2 + P->eps * P->eps + 1e-12);
+  double s3 = s * s * s;
+  double att = 1.0;
+  if (orc_has_cutoff(P)) {
+    double dist = sqrt(r2 + 1e-12);
+    double u = dist / (2.0 * P->rc);
+    att = erfc(u) + (2.0 / sqrt(M_PI)) * u * exp(-u * u);
+  }
+  double c = -m * att * s3;
+  for (int a = 0; a < 3; a++) f[a] += c * d[a];
+}
+
+/* J^T g of the softened force w.r.t. the query position. */
+static inline void orc_soft_force_vjp(const double* q, const double* p, double m,
+                                      const double* g, const OrcParams* P,
+                                      double* out) {
+  double d[3];
+  for (int a = 0; a < 3; a++) d[a] = orc_mi(q[a] - p[a], P);
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double s = 1.0 / sqrt(r2 + P->eps * P->eps + 1e-12);
+  double s3 = s * s * s;
+  double s5 = s3 * s * s;
+  double att = 1.0, cc = 0.0;
+  if (orc_has_cutoff(P)) {
+    double dist = sqrt(r2 + 1e-12);
+    double u = dist / (2.0 * P->rc);
+    double e = exp(-u * u);
+    att = erfc(u) + (2.0 / sqrt(M_PI)) * u * e;
+    cc = s3 * dist * e / (2.0 * sqrt(M_PI) * P->rc * P->rc * P->rc);
+  }
+  double rg = d[0] * g[0] + d[1] * g[1] + d[2] * g[2];
+  double lin = att * s3;
+  double rad = rg * (3.0 * att * s5 + cc);
+  for (int a = 0; a < 3; a++) out[a] += -m * (lin * g[a] - rad * d[a]);
+}
+
+/* gravityGrad (kernel.h:89-94): -m*d/(|d|+0.1)^3 with d = minimage(q-p). */
+static inline void orc_grav_grad(const double* q, const double* p, double m,
+                                 const OrcParams* P, double* f) {
+  double d[3];
+  for (int a = 0; a < 3; a++) d[a] = orc_mi(q[a] - p[a], P);
+  double dist = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  double den = dist + 0.1;
+  double c = -m / (den * den * den);
+  for (int a = 0; a < 3; a++) f[a] += c * d[a];
+}
+
+/* gravityGradVjp (sbh_ffi.cu:2044-2060). */
+static inline void orc_grav_grad_vjp(const double* q, const double* p, double m,
+                                     const double* g, const OrcParams* P, double* out) {
+  double d[3];
+  for (int a = 0; a < 3; a++) d[a] = orc_mi(q[a] - p[a], P);
+  double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double dist = sqrt(r2 + 1e-12);
+  double den = dist + 0.1;
+  double i3 = 1.0 / (den * den * den);
+  double i4 = i3 / den;
+  double rg = d[0] * g[0] + d[1] * g[1] + d[2] * g[2];
+  double dinv = 1.0 / (dist + 1e-12);
+  for (int a = 0; a < 3; a++) out[a] += -m * (i3 * g[a] - 3.0 * rg * dinv * i4 * d[a]);
+}
+
+static void orc_params(OrcParams* P, double eps, double rc, int periodic, double box,
+                       int nearbyint_mi) {
+  P->eps = eps > 0.0 ? eps : 0.0;
+  P->rc = rc;
+  P->periodic = periodic != 0 && box > 0.0;
+  P->box = box;
+  P->nearbyint_mi = nearbyint_mi;
+}
+
+/* D1 (sbh_ffi.cu:431-460, 503-522; python/unified_force_benchmark.py:208-230). */
+ORC_EXPORT void orc_direct_force_soft(const float* pts, const float* masses, int64_t n,
+                                      const float* queries, int64_t nq, double eps,
+                                      double rc, int periodic, double box, double* out) {
+  OrcParams P;
+  orc_params(&P, eps, rc, periodic, box, 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double f[3] = {0, 0, 0};
+    for (int64_t j = 0; j < n; j++) {
+      double p[3] = {pts[3 * j], pts[3 * j + 1], pts[3 * j + 2]};
+      orc_soft_force(q, p, masses[j], &P, f);
+    }
+    out[3 * i] = f[0];
+    out[3 * i + 1] = f[1];
+    out[3 * i + 2] = f[2];
+  }
+}
+
+/* D2 (sbh_ffi.cu:462-501, 524-546): query-side J^T g. */
+ORC_EXPORT void orc_direct_force_soft_vjp(const float* pts, const float* masses,
+                                          int64_t n, const float* queries,
+                                          const float* cot, int64_t nq, double eps,
+                                          double rc, int periodic, double box,
+                                          double* out) {
+  OrcParams P;
+  orc_params(&P, eps, rc, periodic, box, 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double g[3] = {cot[3 * i], cot[3 * i + 1], cot[3 * i + 2]};
+    double o[3] = {0, 0, 0};
+    for (int64_t j = 0; j < n; j++) {
+      double p[3] = {pts[3 * j], pts[3 * j + 1], pts[3 * j + 2]};
+      orc_soft_force_vjp(q, p, masses[j], g, &P, o);
+    }
+    out[3 * i] = o[0];
+    out[3 * i + 1] = o[1];
+    out[3 * i + 2] = o[2];
+  }
+}
+
+/* Full adjoint of L = sum_i c_i . F_i for self-gravity (queries == sources),
+ * no cutoff.  Not in the reference (its custom_vjp returns the query side
+ * only and zero mass gradients, python/jax_sbh_ffi.py:576-588).
+ *   grad_x[k] = sum_{i!=k} v_ik - sum_{j!=k} v_kj ... written per pair below.
+ *   grad_m[j] = sum_{i!=j} -s^3 (c_i . d_ij) * att-free form.            */
+ORC_EXPORT void orc_direct_full_vjp(const float* pts, const float* masses, int64_t n,
+                                    const float* cot, double eps, int periodic,
+                                    double box, double* gx, double* gm) {
+  OrcParams P;
+  orc_params(&P, eps, 0.0, periodic, box, 1);
+  double e2 = P.eps * P.eps + 1e-12;
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n; k++) {
+    double xk[3] = {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+    double ck[3] = {cot[3 * k], cot[3 * k + 1], cot[3 * k + 2]};
+    double g[3] = {0, 0, 0};
+    double gmk = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      if (j == k) continue;
+      double xj[3] = {pts[3 * j], pts[3 * j + 1], pts[3 * j + 2]};
+      double cj[3] = {cot[3 * j], cot[3 * j + 1], cot[3 * j + 2]};
+      double d[3];
+      for (int a = 0; a < 3; a++) d[a] = orc_mi(xk[a] - xj[a], &P); /* d_kj */
+      double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      double s = 1.0 / sqrt(r2 + e2);
+      double s3 = s * s * s, s5 = s3 * s * s;
+      /* k as query, j as source: v_kj = m_j (3 s5 (c_k.d) d - s3 c_k) */
+      double ckd = ck[0] * d[0] + ck[1] * d[1] + ck[2] * d[2];
+      /* j as query, k as source: d_jk = -d; contribution to x_k is -v_jk with
+       * v_jk = m_k (3 s5 (c_j.d_jk) d_jk - s3 c_j) */
+      double cjd = cj[0] * d[0] + cj[1] * d[1] + cj[2] * d[2];
+      double mj = masses[j], mk = masses[k];
+      for (int a = 0; a < 3; a++) {
+        double v_kj = mj * (3.0 * s5 * ckd * d[a] - s3 * ck[a]);
+        double v_jk = mk * (3.0 * s5 * cjd * d[a] - s3 * cj[a]); /* (-d)(-cjd) = d*cjd */
+        g[a] += v_kj - v_jk;
+      }
+      /* dL/dm_k from query j: c_j . (-s3 d_jk) = s3 (c_j . d) */
+      gmk += s3 * cjd;
+    }
+    gx[3 * k] = g[0];
+    gx[3 * k + 1] = g[1];
+    gx[3 * k + 2] = g[2];
+    gm[k] = gmk;
+  }
+}
+
+/* D3 (sbh_ffi.cu:399-429; kernel.h:55-60): potential sum -m/(|d|+0.1). */
+ORC_EXPORT void orc_bruteforce_gravity(const float* pts, const float* masses, int64_t n,
+                                       int dim, const float* queries, int64_t nq,
+                                       int periodic, double box, double* out) {
+  OrcParams P;
+  orc_params(&P, 0.0, 0.0, periodic, box, 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; i++) {
+    double acc = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      double r2 = 0.0;
+      for (int a = 0; a < dim; a++) {
+        double d = orc_mi((double)pts[dim * j + a] - (double)queries[dim * i + a], &P);
+        r2 += d * d;
+      }
+      acc += -masses[j] / (sqrt(r2) + 0.1);
+    }
+    out[i] = acc;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* W1-W4: classic walks over the dense octree8 (sbh_ffi.cu:1878-2144)         */
+/* ------------------------------------------------------------------------- */
+/* MAC (sbh_ffi.cu:1878-1893) in fp32; min-image as kernel.h:27-39. */
+static inline float orc_mi_f(float d, int periodic, float box) {
+  if (periodic && box > 0.0f) {
+    float inv_l = 1.0f / box;
+    return d - box * floorf(d * inv_l + 0.5f);
+  }
+  return d;
+}
+
+static inline int orc_use_far_field8(const OrcNode8* nd, const float* q, float beta,
+                                     int periodic, float box) {
+  if (nd->is_leaf) return 1;
+  float dx = orc_mi_f(q[0] - nd->cx, periodic, box);
+  float dy = orc_mi_f(q[1] - nd->cy, periodic, box);
+  float dz = orc_mi_f(q[2] - nd->cz, periodic, box);
+  float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+  return dist / nd->diameter >= beta;
+}
+
+static inline float orc_axis_dist(float q, float lo, float hi) {
+  if (q < lo) return lo - q;
+  if (q > hi) return q - hi;
+  return 0.0f;
+}
+
+/* Prune test (sbh_ffi.cu:1903-1948). */
+static inline int orc_should_prune8(const OrcNode8* nd, const float* q, int prune_on,
+                                    float r_cut_sq, int periodic, float box) {
+  if (!prune_on) return 0;
+  float qx = q[0], qy = q[1], qz = q[2];
+  if (periodic && box > 0.0f) {
+    float inv_l = 1.0f / box;
+    qx -= box * nearbyintf((qx - nd->cx) * inv_l);
+    qy -= box * nearbyintf((qy - nd->cy) * inv_l);
+    qz -= box * nearbyintf((qz - nd->cz) * inv_l);
+  }
+  float dx = orc_axis_dist(qx, nd->cx - nd->half_size, nd->cx + nd->half_size);
+  float dy = orc_axis_dist(qy, nd->cy - nd->half_size, nd->cy + nd->half_size);
+  float dz = orc_axis_dist(qz, nd->cz - nd->half_size, nd->cz + nd->half_size);
+  return dx * dx + dy * dy + dz * dz > r_cut_sq;
+}
+
+/* kind: 0 = gravityGrad (unsoftened, W3), 1 = softened/cutoff force (W2),
+ *       2 = softened/cutoff VJP (W4, needs cot), 3 = gravityGrad VJP. */
+ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_bytes,
+                               const float* queries, const float* cot, int64_t nq,
+                               int kind, float beta, double eps, double rc,
+                               int prune_enabled, float prune_r_cut_mult, int periodic,
+                               float box, double* out, int64_t* interactions) {
+  const OrcLeaf3* leaf = (const OrcLeaf3*)leaf_bytes;
+  const OrcNode8* nodes = (const OrcNode8*)node_bytes;
+  OrcParams P;
+  orc_params(&P, eps, rc, periodic, box, 0);
+  int per = periodic != 0 && box > 0.0f;
+  /* Prune radius (sbh_ffi.cu:3233-3237). */
+  int prune_on = (prune_enabled != 0 && rc > 0.0) ? 1 : 0;
+  float r_cut = prune_on ? prune_r_cut_mult * (float)rc : 0.0f;
+  float r_cut_sq = r_cut * r_cut;
+  int64_t total_inter = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total_inter)
+  for (int64_t i = 0; i < nq; i++) {
+    float qf[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double q[3] = {qf[0], qf[1], qf[2]};
+    double g[3] = {0, 0, 0};
+    if (cot) {
+      g[0] = cot[3 * i];
+      g[1] = cot[3 * i + 1];
+      g[2] = cot[3 * i + 2];
+    }
+    double acc[3] = {0, 0, 0};
+    int stack[128];
+    int top = 0;
+    stack[top++] = 0;
+    while (top > 0) {
+      const OrcNode8* nd = &nodes[stack[--top]];
+      if (nd->contrib.mass == 0.0f) continue;
+      if (kind != 0 && kind != 3 &&
+          orc_should_prune8(nd, qf, prune_on, r_cut_sq, per, box))
+        continue;
+      if (orc_use_far_field8(nd, qf, beta, per, box)) {
+        uint32_t s0 = nd->is_leaf ? nd->start : 0, s1 = nd->is_leaf ? nd->end : 1;
+        for (uint32_t j = s0; j < s1; j++) {
+          const OrcLeaf3* src = nd->is_leaf ? &leaf[j] : &nd->contrib;
+          double p[3] = {src->pos[0], src->pos[1], src->pos[2]};
+          switch (kind) {
+            case 0: orc_grav_grad(q, p, src->mass, &P, acc); break;
+            case 1: orc_soft_force(q, p, src->mass, &P, acc); break;
+            case 2: orc_soft_force_vjp(q, p, src->mass, g, &P, acc); break;
+            default: orc_grav_grad_vjp(q, p, src->mass, g, &P, acc); break;
+          }
+          total_inter++;
+        }
+      } else {
+        for (int c = 0; c < 8; c++)
+          if (nd->child_mask & (1u << c)) stack[top++] = (int)nd->child[c];
+      }
+    }
+    out[3 * i] = acc[0];
+    out[3 * i + 1] = acc[1];
+    out[3 * i + 2] = acc[2];
+  }
+  if (interactions) *interactions = total_inter;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Adaptive Morton octree (new; no reference counterpart beyond the octree8    */
+/* quantiser and geometry it shares).                                         */
+/*   - codes: orc_morton_codes at depth D, stable order by (code, index)      */
+/*   - node = non-empty cell whose parent holds more than leaf_size points    */
+/*   - leaf  = count <= leaf_size or level == D                               */
+/*   - nodes in breadth-first order; children of a node are contiguous        */
+/* ------------------------------------------------------------------------- */
+static inline void orc_cell_geometry(uint32_t prefix, int level, const float* min_corner,
+                                     float span, float* geo) {
+  float cx = min_corner[0] + span * 0.5f;
+  float cy = min_corner[1] + span * 0.5f;
+  float cz = min_corner[2] + span * 0.5f;
+  float half_size = span * 0.5f;
+  int ix = 0, iy = 0, iz = 0;
+  for (int b = 0; b < level; b++) {
+    ix |= (int)((prefix >> (3 * b + 0)) & 1u) << b;
+    iy |= (int)((prefix >> (3 * b + 1)) & 1u) << b;
+    iz |= (int)((prefix >> (3 * b + 2)) & 1u) << b;
+  }
+  float step = half_size * 2.0f / (float)(1 << level);
+  geo[0] = fmaf((float)ix + 0.5f, step, cx - half_size);
+  geo[1] = fmaf((float)iy + 0.5f, step, cy - half_size);
+  geo[2] = fmaf((float)iz + 0.5f, step, cz - half_size);
+  geo[3] = step * 1.7320508075688772f;
+}
+
+/* Returns the node count, or -1 if cap is too small. */
+ORC_EXPORT int64_t orc_tree_build(const uint32_t* codes_sorted, int64_t n, int max_depth,
+                                  int leaf_size, const float* min_corner, float span,
+                                  int64_t cap, int32_t* start, int32_t* count,
+                                  int32_t* first_child, int32_t* nchild, int32_t* parent,
+                                  int32_t* level, float* geo) {
+  if (cap < 1) return -1;
+  int64_t num = 1;
+  start[0] = 0;
+  count[0] = (int32_t)n;
+  parent[0] = -1;
+  level[0] = 0;
+  orc_cell_geometry(0u, 0, min_corner, span, geo);
+  int64_t head = 0; /* BFS */
+  while (head < num) {
+    int64_t nd = head++;
+    int l = level[nd];
+    first_child[nd] = -1;
+    nchild[nd] = 0;
+    if (l >= max_depth || count[nd] <= leaf_size || count[nd] == 0) continue;
+    int shift = 3 * (max_depth - l - 1);
+    int32_t s = start[nd], e = start[nd] + count[nd];
+    first_child[nd] = (int32_t)num;
+    int32_t i = s;
+    while (i < e) {
+      uint32_t pre = codes_sorted[i] >> shift;
+      int32_t j = i;
+      while (j < e && (codes_sorted[j] >> shift) == pre) j++;
+      if (num >= cap) return -1;
+      start[num] = i;
+      count[num] = j - i;
+      parent[num] = (int32_t)nd;
+      level[num] = l + 1;
+      orc_cell_geometry(pre, l + 1, min_corner, span, geo + 4 * num);
+      num++;
+      nchild[nd]++;
+      i = j;
+    }
+  }
+  return num;
+}
+
+/* Moments in fp64 over sorted sources: mass, centre of mass, and the central
+ * second moment S = sum m (x-X)(x-X)^T stored as (xx, yy, zz, xy, xz, yz). */
+ORC_EXPORT void orc_tree_moments(const int32_t* start, const int32_t* count, int64_t nn,
+                                 const double* xs, const double* ms, double* mass,
+                                 double* com, double* quad) {
+  for (int64_t k = 0; k < nn; k++) {
+    double M = 0, X[3] = {0, 0, 0};
+    for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+      M += ms[j];
+      for (int a = 0; a < 3; a++) X[a] += ms[j] * xs[3 * j + a];
+    }
+    double Q[6] = {0, 0, 0, 0, 0, 0};
+    if (M != 0.0) {
+      for (int a = 0; a < 3; a++) X[a] /= M;
+      for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+        double d0 = xs[3 * j] - X[0], d1 = xs[3 * j + 1] - X[1], d2 = xs[3 * j + 2] - X[2];
+        double m = ms[j];
+        Q[0] += m * d0 * d0;
+        Q[1] += m * d1 * d1;
+        Q[2] += m * d2 * d2;
+        Q[3] += m * d0 * d1;
+        Q[4] += m * d0 * d2;
+        Q[5] += m * d1 * d2;
+      }
+    }
+    mass[k] = M;
+    for (int a = 0; a < 3; a++) com[3 * k + a] = X[a];
+    for (int a = 0; a < 6; a++) quad[6 * k + a] = Q[a];
+  }
+}
+
+/* Group boxes over sorted queries, fp32, groups of group_size consecutive
+ * queries (the last group may be short). */
+ORC_EXPORT void orc_group_boxes(const float* q, int64_t nq, int group_size, float* boxes) {
+  int64_t ng = (nq + group_size - 1) / group_size;
+  for (int64_t g = 0; g < ng; g++) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = g * group_size; i < nq && i < (g + 1) * group_size; i++)
+      for (int a = 0; a < 3; a++) {
+        lo[a] = fminf(lo[a], q[3 * i + a]);
+        hi[a] = fmaxf(hi[a], q[3 * i + a]);
+      }
+    for (int a = 0; a < 3; a++) {
+      boxes[6 * g + a] = lo[a];
+      boxes[6 * g + 3 + a] = hi[a];
+    }
+  }
+}
+
+/* Group MAC: accept node for every query of the group iff the distance from
+ * the cell centre to the group box is at least beta * diagonal.  fp32, no
+ * contraction; the product kernel evaluates the identical expression. */
+static inline int orc_group_accept(const float* box, const float* geo, float beta) {
+  float dx = fmaxf(fmaxf(box[0] - geo[0], geo[0] - box[3]), 0.0f);
+  float dy = fmaxf(fmaxf(box[1] - geo[1], geo[1] - box[4]), 0.0f);
+  float dz = fmaxf(fmaxf(box[2] - geo[2], geo[2] - box[5]), 0.0f);
+  float d2 = dx * dx + dy * dy + dz * dz;
+  float t = beta * geo[3];
+  return d2 >= t * t;
+}
+
+/* Far-field (monopole [+ quadrupole]) force and its adjoint for one query.
+ * F = -M s^3 D + 3 s^5 (S D) + 1.5 tr(S) s^5 D - 7.5 s^7 (D.S.D) D,
+ * s = (|D|^2 + eps^2 + 1e-12)^(-1/2), D = q - X. */
+static void orc_far(const double* q, double M, const double* X, const double* Q,
+                    int use_quad, double e2, const double* c, double* F, double* gq,
+                    double* adjX, double* adjM, double* adjS) {
+  double D[3] = {q[0] - X[0], q[1] - X[1], q[2] - X[2]};
+  double r2 = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+  double s = 1.0 / sqrt(r2 + e2);
+  double s2 = s * s, s3 = s2 * s, s5 = s3 * s2, s7 = s5 * s2, s9 = s7 * s2;
+  double SD[3] = {0, 0, 0}, t = 0.0, T = 0.0;
+  if (use_quad) {
+    SD[0] = Q[0] * D[0] + Q[3] * D[1] + Q[4] * D[2];
+    SD[1] = Q[3] * D[0] + Q[1] * D[1] + Q[5] * D[2];
+    SD[2] = Q[4] * D[0] + Q[5] * D[1] + Q[2] * D[2];
+    t = D[0] * SD[0] + D[1] * SD[1] + D[2] * SD[2];
+    T = Q[0] + Q[1] + Q[2];
+  }
+  for (int a = 0; a < 3; a++)
+    F[a] += -M * s3 * D[a] + 3.0 * s5 * SD[a] + 1.5 * T * s5 * D[a] - 7.5 * s7 * t * D[a];
+  if (!c) return;
+  double u = c[0] * D[0] + c[1] * D[1] + c[2] * D[2];
+  double Sc[3] = {0, 0, 0}, cw = 0.0;
+  if (use_quad) {
+    Sc[0] = Q[0] * c[0] + Q[3] * c[1] + Q[4] * c[2];
+    Sc[1] = Q[3] * c[0] + Q[1] * c[1] + Q[5] * c[2];
+    Sc[2] = Q[4] * c[0] + Q[5] * c[1] + Q[2] * c[2];
+    cw = c[0] * SD[0] + c[1] * SD[1] + c[2] * SD[2];
+  }
+  double kD = 3.0 * M * s5 * u - 15.0 * s7 * cw - 7.5 * T * s7 * u + 52.5 * s9 * t * u;
+  double kc = -M * s3 + 1.5 * T * s5 - 7.5 * s7 * t;
+  double dD[3];
+  for (int a = 0; a < 3; a++)
+    dD[a] = kD * D[a] + kc * c[a] + 3.0 * s5 * Sc[a] - 15.0 * s7 * u * SD[a];
+  for (int a = 0; a < 3; a++) {
+    gq[a] += dD[a];
+    adjX[a] -= dD[a];
+  }
+  *adjM += -s3 * u;
+  if (use_quad) {
+    /* symmetric gradient w.r.t. S, stored (xx, yy, zz, xy, xz, yz) as the
+     * matrix entries G_jl (off-diagonal entries counted once). */
+    int jj[6] = {0, 1, 2, 0, 0, 1}, ll[6] = {0, 1, 2, 1, 2, 2};
+    for (int k = 0; k < 6; k++) {
+      int j = jj[k], l = ll[k];
+      double G = 1.5 * s5 * (c[j] * D[l] + c[l] * D[j]) - 7.5 * s7 * u * D[j] * D[l];
+      if (j == l) G += 1.5 * s5 * u;
+      adjS[k] += G;
+    }
+  }
+}
+
+/* Evaluate the adaptive-tree walk over sorted queries in groups.
+ *   self_mode: queries are the sorted sources (query i == source i); the
+ *              i == j pair is skipped.
+ *   cot == NULL: forces only.  Otherwise also
+ *     gq   (nq*3): query-side gradient,
+ *     gsx  (n*3), gsm (n): source-side position and mass gradients
+ *                 (pair terms plus the push-down of node adjoints).
+ *   counts[0] = far (node) interactions, counts[1] = near pairs.          */
+ORC_EXPORT void orc_tree_eval(int64_t nn, const int32_t* start, const int32_t* count,
+                              const int32_t* first_child, const int32_t* nchild,
+                              const float* geo, const double* mass, const double* com,
+                              const double* quad, const double* xs, const double* ms,
+                              int64_t n, const float* boxes, const double* qx,
+                              int64_t nq, int group_size, float beta, int use_quad,
+                              double eps, int self_mode, const double* cot,
+                              double* force, double* gq, double* gsx, double* gsm,
+                              int64_t* counts) {
+  double e2 = (eps > 0 ? eps * eps : 0.0) + 1e-12;
+  int64_t ng = (nq + group_size - 1) / group_size;
+  double* adjX = NULL;
+  double* adjM = NULL;
+  double* adjS = NULL;
+  if (cot) {
+    adjX = (double*)calloc((size_t)nn * 3, sizeof(double));
+    adjM = (double*)calloc((size_t)nn, sizeof(double));
+    adjS = (double*)calloc((size_t)nn * 6, sizeof(double));
+    memset(gq, 0, sizeof(double) * 3 * (size_t)nq);
+    memset(gsx, 0, sizeof(double) * 3 * (size_t)n);
+    memset(gsm, 0, sizeof(double) * (size_t)n);
+  }
+  memset(force, 0, sizeof(double) * 3 * (size_t)nq);
+  int64_t nfar = 0, nnear = 0;
+  int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 8));
+  int32_t* far = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+  int32_t* near = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+  for (int64_t g = 0; g < ng; g++) {
+    const float* box = boxes + 6 * g;
+    int64_t nf = 0, nl = 0, top = 0;
+    stack[top++] = 0;
+    while (top > 0) {
+      int32_t k = stack[--top];
+      if (mass[k] == 0.0) continue;
+      if (orc_group_accept(box, geo + 4 * k, beta)) {
+        far[nf++] = k;
+      } else if (nchild[k] == 0) {
+        near[nl++] = k;
+      } else {
+        for (int32_t c = 0; c < nchild[k]; c++) stack[top++] = first_child[k] + c;
+      }
+    }
+    for (int64_t i = g * group_size; i < nq && i < (g + 1) * group_size; i++) {
+      const double* q = qx + 3 * i;
+      const double* c = cot ? cot + 3 * i : NULL;
+      double F[3] = {0, 0, 0}, G[3] = {0, 0, 0};
+      for (int64_t f = 0; f < nf; f++) {
+        int32_t k = far[f];
+        double dummyM = 0.0;
+        orc_far(q, mass[k], com + 3 * k, quad + 6 * k, use_quad, e2, c, F, G,
+                cot ? adjX + 3 * k : NULL, cot ? adjM + k : &dummyM,
+                cot ? adjS + 6 * k : NULL);
+      }
+      nfar += nf;
+      for (int64_t b = 0; b < nl; b++) {
+        int32_t k = near[b];
+        for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+          if (self_mode && j == i) continue;
+          const double* p = xs + 3 * j;
+          double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+          double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+          double s = 1.0 / sqrt(r2 + e2);
+          double s3 = s * s * s, s5 = s3 * s * s;
+          double m = ms[j];
+          for (int a = 0; a < 3; a++) F[a] += -m * s3 * d[a];
+          nnear++;
+          if (c) {
+            double cd = c[0] * d[0] + c[1] * d[1] + c[2] * d[2];
+            for (int a = 0; a < 3; a++) {
+              double v = m * (3.0 * s5 * cd * d[a] - s3 * c[a]);
+              G[a] += v;
+              gsx[3 * j + a] -= v;
+            }
+            gsm[j] += -s3 * cd;
+          }
+        }
+      }
+      for (int a = 0; a < 3; a++) force[3 * i + a] = F[a];
+      if (c)
+        for (int a = 0; a < 3; a++) gq[3 * i + a] = G[a];
+    }
+  }
+  if (cot) {
+    /* Push node adjoints down to the sources of each node. */
+    for (int64_t k = 0; k < nn; k++) {
+      double M = mass[k];
+      if (M == 0.0) continue;
+      const double* X = com + 3 * k;
+      const double* A = adjX + 3 * k;
+      const double* S = adjS + 6 * k;
+      double B = adjM[k];
+      for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+        double dl[3] = {xs[3 * j] - X[0], xs[3 * j + 1] - X[1], xs[3 * j + 2] - X[2]};
+        double m = ms[j];
+        double Gd[3] = {S[0] * dl[0] + S[3] * dl[1] + S[4] * dl[2],
+                        S[3] * dl[0] + S[1] * dl[1] + S[5] * dl[2],
+                        S[4] * dl[0] + S[5] * dl[1] + S[2] * dl[2]};
+        for (int a = 0; a < 3; a++) gsx[3 * j + a] += (m / M) * A[a] + 2.0 * m * Gd[a];
+        gsm[j] += B + (A[0] * dl[0] + A[1] * dl[1] + A[2] * dl[2]) / M +
+                  (dl[0] * Gd[0] + dl[1] * Gd[1] + dl[2] * Gd[2]);
+      }
+    }
+    free(adjX);
+    free(adjM);
+    free(adjS);
+  }
+  free(stack);
+  free(far);
+  free(near);
+  if (counts) {
+    counts[0] = nfar;
+    counts[1] = nnear;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based RNG   */
+/* the stochastic walks use in place of cuRAND XORWOW.                        */
+/* ------------------------------------------------------------------------- */
+ORC_EXPORT void orc_philox4x32(const uint32_t* ctr_in, const uint32_t* key_in,
+                               uint32_t* out) {
+  uint32_t c0 = ctr_in[0], c1 = ctr_in[1], c2 = ctr_in[2], c3 = ctr_in[3];
+  uint32_t k0 = key_in[0], k1 = key_in[1];
+  for (int r = 0; r < 10; r++) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+    uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    uint32_t n1 = (uint32_t)p1;
+    uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    uint32_t n3 = (uint32_t)p0;
+    c0 = n0;
+    c1 = n1;
+    c2 = n2;
+    c3 = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  out[0] = c0;
+  out[1] = c1;
+  out[2] = c2;
+  out[3] = c3;
+}

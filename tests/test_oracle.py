@@ -1,0 +1,536 @@
+"""Pins for the CPU oracle (oracle/sbh_oracle.c).
+
+The reference commits no golden vectors (its tests pin behaviour through
+closed forms and invariants only), so each oracle function is pinned here by
+something independent of its own formula: hand-computed values, a library
+routine, a closed form, a conservation law, central finite differences, or a
+special case that reduces to another pinned function.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+# --------------------------------------------------------------------- Philox
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32_10.
+    assert list(O.philox4x32([0, 0, 0, 0], [0, 0])) == [
+        0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert list(O.philox4x32([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)) == [
+        0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]
+    assert list(O.philox4x32([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                             [0xA4093822, 0x299F31D0])) == [
+        0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+# --------------------------------------------------------------------- Morton
+def test_morton_hand_values():
+    assert O.morton3d(1, 0, 0, 2) == 0b001
+    assert O.morton3d(0, 1, 0, 2) == 0b010
+    assert O.morton3d(0, 0, 1, 2) == 0b100
+    assert O.morton3d(2, 0, 0, 2) == 0b001000
+    assert O.morton3d(3, 3, 3, 2) == 63
+    assert O.morton3d(1023, 0, 0, 10) == int("001" * 10, 2)
+    assert O.morton3d(0, 0, 1023, 10) == int("100" * 10, 2)
+    # bits above `bits` are ignored (the reference loops over `bits` only)
+    assert O.morton3d(4, 0, 0, 2) == 0
+
+
+def test_morton_matches_integer_bruteforce():
+    rng = _rng(0)
+    for bits in (1, 3, 7, 10):
+        xyz = rng.integers(0, 1 << bits, size=(200, 3))
+        for x, y, z in xyz:
+            expect = 0
+            for i in range(bits):
+                for a, v in enumerate((x, y, z)):
+                    if (v >> i) & 1:
+                        expect += 1 << (3 * i + a)
+            assert O.morton3d(x, y, z, bits) == expect
+
+
+def test_quantize_hand_values():
+    # t = (v - min) / span, clamped to [0, 0.999999], times 2^bits, truncated.
+    assert O.quantize_axis(0.0, 0.0, 1.0, 3) == 0
+    assert O.quantize_axis(0.5, 0.0, 1.0, 3) == 4
+    assert O.quantize_axis(0.125, 0.0, 1.0, 3) == 1
+    assert O.quantize_axis(0.12499999, 0.0, 1.0, 3) == 0
+    assert O.quantize_axis(1.0, 0.0, 1.0, 3) == 7      # clamp at 0.999999
+    assert O.quantize_axis(5.0, 0.0, 1.0, 10) == 1023
+    assert O.quantize_axis(-3.0, 0.0, 1.0, 10) == 0     # clamp at 0
+    assert O.quantize_axis(3.0, 2.0, 4.0, 2) == 1       # (3-2)/4 = 0.25 -> 1
+    # 0.999999 * 2^10 = 1023.999 -> 1023 (never 1024)
+    assert O.quantize_axis(np.float32(0.9999999), 0.0, 1.0, 10) == 1023
+
+
+def test_default_bounds_fp32():
+    p = np.array([[0.0, 1.0, -2.0], [0.5, 3.0, 4.0]], np.float32)
+    lo, hi = O.default_bounds(p)
+    assert lo.dtype == np.float32
+    np.testing.assert_array_equal(lo, np.array([0, 1, -2], np.float32) - np.float32(1e-3))
+    np.testing.assert_array_equal(hi, np.array([0.5, 3, 4], np.float32) + np.float32(1e-3))
+    assert O.span(lo, hi) == np.float32(4.0 + 1e-3) - np.float32(-2.0 - 1e-3)
+
+
+def test_stable_order_matches_numpy_stable_argsort():
+    rng = _rng(1)
+    codes = rng.integers(0, 1 << 12, size=5000).astype(np.uint32)
+    codes[::7] = codes[3]  # many ties
+    np.testing.assert_array_equal(O.stable_order(codes, 12),
+                                  np.argsort(codes, kind="stable").astype(np.uint32))
+
+
+# --------------------------------------------------------------------- octree8 builder
+def _numpy_octree8(points, normals, masses, lo, hi, D):
+    """Independent dense build: bincount per level instead of the oracle's
+    bottom-up child loop; fp64 sums."""
+    span = O.span(lo, hi)
+    codes = O.morton_codes(points, lo, span, D).astype(np.int64)
+    perm = np.argsort(codes, kind="stable")
+    levels = []
+    for l in range(D + 1):
+        pre = codes >> (3 * (D - l))
+        nb = 1 << (3 * l)
+        cnt = np.bincount(pre, minlength=nb)
+        ms = np.bincount(pre, weights=masses.astype(np.float64), minlength=nb)
+        levels.append((cnt, ms, pre))
+    return codes, perm, levels
+
+
+@pytest.mark.parametrize("D", [1, 2, 3])
+def test_octree8_matches_independent_build(D):
+    rng = _rng(10 + D)
+    n = 300
+    p = rng.uniform(-1, 1, size=(n, 3)).astype(np.float32)
+    p[:40] = p[0] + rng.normal(0, 1e-3, size=(40, 3)).astype(np.float32)  # a dense clump
+    nrm = rng.normal(size=(n, 3)).astype(np.float32)
+    m = rng.uniform(0.1, 1.0, n).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    leaf, node, contrib = O.build_octree8(p, nrm, m, lo, hi, D)
+    nodes = O.decode_nodes8(node)
+    leaves = O.decode_leaf3(leaf)
+    codes, perm, levels = _numpy_octree8(p, nrm, m, lo, hi, D)
+    # leaf buffer is the stable counting-sort order
+    np.testing.assert_array_equal(leaves["pos"], p[perm])
+    np.testing.assert_array_equal(leaves["mass"], m[perm])
+    np.testing.assert_array_equal(leaves["alpha"], 1.0)
+    off = 0
+    offsets_leaf = np.concatenate([[0], np.cumsum(levels[D][0])])
+    for l in range(D + 1):
+        cnt, ms, _ = levels[l]
+        nb = 1 << (3 * l)
+        nl = nodes[off: off + nb]
+        idx = np.arange(nb)
+        first_leaf = idx << (3 * (D - l))
+        np.testing.assert_array_equal(nl["start"], offsets_leaf[first_leaf])
+        np.testing.assert_array_equal(nl["end"] - nl["start"], cnt)
+        np.testing.assert_array_equal(nl["depth"], l)
+        if l == 0:
+            assert nl["parent"][0] == 0
+        else:
+            prev_off = off - (1 << (3 * (l - 1)))
+            np.testing.assert_array_equal(nl["parent"], prev_off + (idx >> 3))
+        if l < D:
+            ccnt = levels[l + 1][0].reshape(nb, 8)
+            mask = (ccnt > 0) @ (1 << np.arange(8))
+            np.testing.assert_array_equal(nl["child_mask"], mask)
+            np.testing.assert_array_equal(nl["child"], off + nb + (idx[:, None] << 3) + np.arange(8))
+            np.testing.assert_array_equal(nl["is_leaf"], (mask == 0).astype(np.uint8))
+            fc = np.where(mask > 0, off + nb + (idx << 3) + np.argmax(ccnt > 0, axis=1), 0xFFFFFFFF)
+            np.testing.assert_array_equal(nl["first_child"], fc.astype(np.uint32))
+        else:
+            np.testing.assert_array_equal(nl["is_leaf"], 1)
+            np.testing.assert_array_equal(nl["child"], 0xFFFFFFFF)
+        # geometry: cell of side span/2^l, centre at lo + (i+0.5)*side
+        side = np.float64(O.span(lo, hi)) / (1 << l)
+        ix = np.zeros(nb, np.int64); iy = np.zeros(nb, np.int64); iz = np.zeros(nb, np.int64)
+        for b in range(l):
+            ix |= ((idx >> (3 * b)) & 1) << b
+            iy |= ((idx >> (3 * b + 1)) & 1) << b
+            iz |= ((idx >> (3 * b + 2)) & 1) << b
+        for name, ii, a in (("cx", ix, 0), ("cy", iy, 1), ("cz", iz, 2)):
+            np.testing.assert_allclose(nl[name], lo[a] + (ii + 0.5) * side, rtol=0, atol=4e-7 * max(1, abs(lo[a]) + side * (1 << l)))
+        np.testing.assert_allclose(nl["diameter"], side * np.sqrt(3), rtol=1e-6)
+        np.testing.assert_allclose(nl["half_size"], side / 2, rtol=1e-6)
+        # moments: mass sums and centre of mass in fp64, rounded
+        np.testing.assert_allclose(nl["contrib"]["mass"], ms, rtol=1e-6)
+        off += nb
+    np.testing.assert_array_equal(contrib, node.reshape(-1, 104)[:, 72:104].ravel())
+
+
+def test_octree8_points_inside_their_leaf_cells():
+    rng = _rng(3)
+    n, D = 2000, 4
+    p = rng.uniform(-2, 3, size=(n, 3)).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    leaf, node, _ = O.build_octree8(p, np.zeros_like(p), np.ones(n, np.float32), lo, hi, D)
+    nodes = O.decode_nodes8(node)
+    leaves = O.decode_leaf3(leaf)
+    base = int(O.octree8_total_nodes(D - 1))
+    lv = nodes[base:]
+    for nd in lv[lv["end"] > lv["start"]]:
+        pts = leaves["pos"][nd["start"]:nd["end"]]
+        c = np.array([nd["cx"], nd["cy"], nd["cz"]])
+        assert np.all(np.abs(pts - c) <= nd["half_size"] * (1 + 1e-5) + 1e-6)
+
+
+def test_octree8_rejects_bad_args():
+    p = np.zeros((4, 3), np.float32)
+    with pytest.raises(ValueError):
+        O.build_octree8(p, p, np.ones(4, np.float32), np.zeros(3), np.zeros(3), 3)
+    with pytest.raises(ValueError):
+        O.build_octree8(p, p, np.ones(4, np.float32), -np.ones(3), np.ones(3), 11)
+
+
+# --------------------------------------------------------------------- direct sums
+def test_direct_single_source_closed_form():
+    p = np.array([[0.1, -0.2, 0.3]], np.float32)
+    m = np.array([2.5], np.float32)
+    q = np.array([[1.0, 0.5, -0.25], [0.1, -0.2, 0.3]], np.float32)
+    eps = 0.05
+    F = O.direct_force_soft(p, m, q, eps, np.inf)
+    d = q.astype(np.float64) - p.astype(np.float64)
+    r2 = (d * d).sum(1)
+    expect = -2.5 * d / (r2 + eps * eps + 1e-12)[:, None] ** 1.5
+    np.testing.assert_allclose(F, expect, rtol=1e-14, atol=1e-300)
+    assert np.all(F[1] == 0.0)
+
+
+def test_direct_is_gradient_of_potential():
+    rng = _rng(4)
+    n = 40
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    q = rng.uniform(-1, 1, (5, 3)).astype(np.float32)
+    eps, h = 0.07, 1e-5
+
+    def phi(x, rc):
+        d = x[None, :] - p.astype(np.float64)
+        r2 = (d * d).sum(1)
+        if np.isinf(rc):
+            return -(m / np.sqrt(r2 + eps * eps + 1e-12)).sum()
+        from scipy.special import erfc
+        r = np.sqrt(r2)
+        return -(m * erfc(r / (2 * rc)) / r).sum()
+
+    for rc, e in ((np.inf, eps), (0.6, 0.0)):
+        F = O.direct_force_soft(p, m, q, e, rc)
+        for i in range(q.shape[0]):
+            g = np.zeros(3)
+            for a in range(3):
+                x0 = q[i].astype(np.float64)
+                xp, xm = x0.copy(), x0.copy()
+                xp[a] += h
+                xm[a] -= h
+                g[a] = (phi(xp, rc) - phi(xm, rc)) / (2 * h)
+            np.testing.assert_allclose(F[i], -g, rtol=2e-6, atol=1e-8)
+
+
+def test_direct_newton_third_law():
+    rng = _rng(5)
+    n = 200
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    F = O.direct_force_soft(p, m, p, 0.01, np.inf)
+    tot = (m[:, None].astype(np.float64) * F).sum(0)
+    assert np.abs(tot).max() < 1e-9 * np.abs(m[:, None] * F).sum()
+
+
+def test_direct_matches_reference_test_closed_form():
+    # _ref_softened_direct_force (python/test_jax_ffi_tree_builder.py:8-17),
+    # evaluated in fp64.
+    from scipy.special import erfc
+    rng = _rng(23)
+    p = rng.uniform(-1, 1, (96, 3))
+    m = rng.uniform(0.1, 1.3, 96)
+    q = rng.uniform(-0.7, 0.7, (31, 3))
+    p, m, q = p.astype(np.float32), m.astype(np.float32), q.astype(np.float32)
+    eps, rc = 5e-2, 0.9
+    disp = q[:, None, :].astype(np.float64) - p[None].astype(np.float64)
+    r2 = (disp * disp).sum(-1)
+    inv_r3 = (r2 + eps * eps + 1e-12) ** -1.5
+    u = np.sqrt(r2 + 1e-12) / (2 * rc)
+    att = erfc(u) + 2 / np.sqrt(np.pi) * u * np.exp(-u * u)
+    ref = ((-m[None] * att * inv_r3)[..., None] * disp).sum(1)
+    np.testing.assert_allclose(O.direct_force_soft(p, m, q, eps, rc), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_direct_vjp_matches_central_differences():
+    rng = _rng(31)
+    n, mq = 64, 17
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.1, n).astype(np.float32)
+    q = rng.uniform(-0.5, 0.5, (mq, 3)).astype(np.float32)
+    c = rng.normal(size=(mq, 3)).astype(np.float32)
+    for eps, rc in ((7e-2, 1.4), (0.05, np.inf)):
+        G = O.direct_force_soft_vjp(p, m, q, c, eps, rc)
+        # FD of sum_i c_i . F(q_i) with a single-query restatement in fp64
+        h = 1e-5
+        for i in range(mq):
+            for a in range(3):
+                qp, qm = q.astype(np.float64).copy(), q.astype(np.float64).copy()
+                qp[i, a] += h
+                qm[i, a] -= h
+                # evaluate through the oracle with fp32 inputs would lose the
+                # perturbation; use the fp64 single-source closed form instead
+                fp = _soft_force64(p, m, qp[i], eps, rc)
+                fm = _soft_force64(p, m, qm[i], eps, rc)
+                fd = ((fp - fm) @ c[i].astype(np.float64)) / (2 * h)
+                assert abs(G[i, a] - fd) <= 1e-6 * (1 + abs(fd))
+
+
+def _soft_force64(p, m, x, eps, rc):
+    from scipy.special import erfc
+    d = x[None, :] - p.astype(np.float64)
+    r2 = (d * d).sum(1)
+    s3 = (r2 + eps * eps + 1e-12) ** -1.5
+    if np.isinf(rc):
+        att = 1.0
+    else:
+        u = np.sqrt(r2 + 1e-12) / (2 * rc)
+        att = erfc(u) + 2 / np.sqrt(np.pi) * u * np.exp(-u * u)
+    return ((-m * att * s3)[:, None] * d).sum(0)
+
+
+def test_direct_full_vjp_matches_central_differences():
+    rng = _rng(41)
+    n = 24
+    p = rng.uniform(-1, 1, (n, 3))
+    m = rng.uniform(0.2, 1.1, n)
+    c = rng.normal(size=(n, 3))
+    eps = 0.06
+    p32, m32, c32 = p.astype(np.float32), m.astype(np.float32), c.astype(np.float32)
+    gx, gm = O.direct_full_vjp(p32, m32, c32, eps)
+
+    def loss(x, mm):
+        tot = 0.0
+        for i in range(n):
+            d = x[i][None] - x
+            r2 = (d * d).sum(1)
+            s3 = (r2 + eps * eps + 1e-12) ** -1.5
+            s3[i] = 0.0
+            tot += c32[i].astype(np.float64) @ ((-mm * s3)[:, None] * d).sum(0)
+        return tot
+
+    x0 = p32.astype(np.float64)
+    m0 = m32.astype(np.float64)
+    h = 1e-6
+    for k in range(n):
+        for a in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fd = (loss(xp, m0) - loss(xm, m0)) / (2 * h)
+            assert abs(gx[k, a] - fd) <= 1e-6 * (1 + abs(fd))
+        mp, mm_ = m0.copy(), m0.copy()
+        mp[k] += h
+        mm_[k] -= h
+        fd = (loss(x0, mp) - loss(x0, mm_)) / (2 * h)
+        assert abs(gm[k] - fd) <= 1e-6 * (1 + abs(fd))
+
+
+def test_bruteforce_gravity_closed_form_and_periodic():
+    p = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], np.float32)
+    m = np.array([1.0, 3.0], np.float32)
+    q = np.array([[0.0, 2.0, 0.0]], np.float32)
+    out = O.bruteforce_gravity(p, m, q)
+    np.testing.assert_allclose(out, [-1 / 2.1 - 3 / (np.sqrt(5) + 0.1)], rtol=1e-6)
+    # periodic: a query at x=1.9 in a box of 2 sees the source at 0 at distance 0.1
+    out = O.bruteforce_gravity(p[:1], m[:1], np.array([[1.9, 0, 0]], np.float32), periodic=True, box=2.0)
+    np.testing.assert_allclose(out, [-1 / (0.1 + 0.1)], rtol=1e-5)
+    p2 = np.array([[0.0, 0.0], [0.0, 1.0]], np.float32)
+    out = O.bruteforce_gravity(p2, m, np.array([[1.0, 0.0]], np.float32))
+    np.testing.assert_allclose(out, [-1 / 1.1 - 3 / (np.sqrt(2) + 0.1)], rtol=1e-6)
+
+
+# --------------------------------------------------------------------- octree8 walks
+def _octree8_case(seed=7, n=500, D=4):
+    rng = _rng(seed)
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    leaf, node, _ = O.build_octree8(p, np.zeros_like(p), m, lo, hi, D)
+    return p, m, leaf, node
+
+
+def test_octree8_walk_huge_beta_is_direct_sum():
+    p, m, leaf, node = _octree8_case()
+    q = p[:50]
+    F, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1e30, eps=0.03, rc=np.inf)
+    np.testing.assert_allclose(F, O.direct_force_soft(p, m, q, 0.03, np.inf), rtol=1e-9, atol=1e-12)
+    F, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1e30, eps=0.03, rc=0.7)
+    np.testing.assert_allclose(F, O.direct_force_soft(p, m, q, 0.03, 0.7), rtol=1e-9, atol=1e-12)
+    c = _rng(8).normal(size=q.shape).astype(np.float32)
+    G, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT_VJP, 1e30, eps=0.03, rc=0.7, cot=c)
+    np.testing.assert_allclose(G, O.direct_force_soft_vjp(p, m, q, c, 0.03, 0.7), rtol=1e-9, atol=1e-12)
+
+
+def test_octree8_walk_beta_zero_is_root_monopole():
+    p, m, leaf, node = _octree8_case()
+    q = np.array([[5.0, -3.0, 2.0], [0.1, 0.2, 0.3]], np.float32)
+    F, cnt = O.bh_octree8(leaf, node, q, O.KIND_GRAV, 0.0)
+    assert cnt == 2
+    M = m.astype(np.float64).sum()
+    X = (m[:, None].astype(np.float64) * p).sum(0) / M
+    X32 = X.astype(np.float32).astype(np.float64)
+    d = q.astype(np.float64) - X32
+    r = np.linalg.norm(d, axis=1)
+    np.testing.assert_allclose(F, -np.float32(M) * d / (r + 0.1)[:, None] ** 3, rtol=1e-6)
+
+
+def test_octree8_prune_huge_cutoff_matches_disabled():
+    p, m, leaf, node = _octree8_case(seed=11)
+    q = p[:96]
+    a, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1.8, eps=0.05, rc=0.8, prune_enabled=False,
+                        periodic=True, box=2.0)
+    b, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1.8, eps=0.05, rc=0.8, prune_enabled=True,
+                        prune_r_cut_mult=1e6, periodic=True, box=2.0)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_octree8_periodic_shift_invariance():
+    rng = _rng(3)
+    n, L = 192, 2.0
+    p = rng.uniform(0, L, (n, 3)).astype(np.float32)
+    m = np.ones(n, np.float32)
+    lo, hi = np.zeros(3, np.float32), np.full(3, L, np.float32)
+    leaf, node, _ = O.build_octree8(p, np.zeros_like(p), m, lo, hi, 5)
+    q = p[:32]
+    f0, _ = O.bh_octree8(leaf, node, q, O.KIND_GRAV, 1.8, periodic=True, box=L)
+    f1, _ = O.bh_octree8(leaf, node, q + np.array([L, 0, 0], np.float32), O.KIND_GRAV, 1.8,
+                         periodic=True, box=L)
+    np.testing.assert_allclose(f0, f1, rtol=1e-2, atol=5e-3)
+
+
+# --------------------------------------------------------------------- adaptive tree
+def _plummer(n, seed):
+    rng = _rng(seed)
+    xmax = 20.0 ** 3 / (20.0 ** 2 + 1.0) ** 1.5
+    X = rng.uniform(0, xmax, n)
+    r = 1.0 / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return (r[:, None] * v).astype(np.float32)
+
+
+def test_tree_topology_invariants():
+    n = 5000
+    p = _plummer(n, 1)
+    m = np.full(n, 1.0 / n, np.float32)
+    T = O.Tree(p, m, max_depth=10, leaf_size=16)
+    assert T.start[0] == 0 and T.count[0] == n and T.level[0] == 0
+    leaf = T.nchild == 0
+    # leaves partition the points
+    cover = np.zeros(n, np.int64)
+    for s, c in zip(T.start[leaf], T.count[leaf]):
+        cover[s: s + c] += 1
+    assert np.all(cover == 1)
+    # leaf iff count <= leaf_size or level == D
+    np.testing.assert_array_equal(leaf, (T.count <= 16) | (T.level == 10))
+    for k in np.nonzero(~leaf)[0]:
+        ch = np.arange(T.first_child[k], T.first_child[k] + T.nchild[k])
+        assert np.all(T.parent[ch] == k) and np.all(T.level[ch] == T.level[k] + 1)
+        assert T.start[ch[0]] == T.start[k]
+        np.testing.assert_array_equal(T.start[ch[1:]], T.start[ch[:-1]] + T.count[ch[:-1]])
+        assert T.count[ch].sum() == T.count[k]
+    # every point's code carries its node's cell prefix; breadth-first order
+    assert np.all(np.diff(T.level) >= 0)
+    for k in range(T.num_nodes):
+        sh = 3 * (10 - T.level[k])
+        pre = T.codes[T.start[k]: T.start[k] + T.count[k]] >> sh
+        assert np.all(pre == pre[0])
+        # cell geometry from the prefix, fp64
+        side = np.float64(T.span) / (1 << T.level[k])
+        ijk = [0, 0, 0]
+        for b in range(T.level[k]):
+            for a in range(3):
+                ijk[a] |= ((int(pre[0]) >> (3 * b + a)) & 1) << b
+        np.testing.assert_allclose(T.geo[k, :3], T.min_corner + (np.array(ijk) + 0.5) * side, atol=1e-5)
+
+
+def test_tree_huge_beta_is_direct_sum():
+    n = 3000
+    p = _plummer(n, 2)
+    m = _rng(2).uniform(0.5, 1.5, n).astype(np.float32) / n
+    T = O.Tree(p, m, max_depth=10, leaf_size=16)
+    r = T.eval(beta=1e30, eps=0.01, use_quad=True)
+    ref = O.direct_force_soft(T.points_sorted, T.masses_sorted, T.points_sorted, 0.01, np.inf)
+    np.testing.assert_allclose(r["force"], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max())
+    assert r["far"] == 0 and r["near"] == n * (n - 1)
+
+
+def test_quadrupole_error_order():
+    # One distant cluster seen through the root multipole: the monopole error
+    # falls as (a/d)^2 and the quadrupole error as (a/d)^3.
+    rng = _rng(9)
+    n = 400
+    p = (rng.normal(size=(n, 3)) * 0.1).astype(np.float32)
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    T = O.Tree(p, m, max_depth=10, leaf_size=8)
+    mass, com, quad = T.moments()
+    errs = {False: [], True: []}
+    ds = [4.0, 8.0, 16.0]
+    for use_quad in (False, True):
+        for d in ds:
+            q = np.array([[d, 0.3 * d, -0.2 * d]])
+            F = np.zeros((1, 3))
+            import ctypes
+            # root-only evaluation: beta=0 accepts the root for the one-query group
+            boxes = np.array([q[0, 0], q[0, 1], q[0, 2], q[0, 0], q[0, 1], q[0, 2]], np.float32)
+            lib = O.lib()
+            cnt = (ctypes.c_int64 * 2)()
+            lib.orc_tree_eval(T.num_nodes, T.start, T.count, T.first_child, T.nchild,
+                              np.ascontiguousarray(T.geo.ravel()), mass, com.ravel().copy(),
+                              quad.ravel().copy(), T.points_sorted.astype(np.float64),
+                              T.masses_sorted.astype(np.float64), T.n, boxes,
+                              q.astype(np.float64), 1, 1, 0.0, int(use_quad), 0.0, 0,
+                              None, F, None, None, None, cnt)
+            assert cnt[0] == 1
+            ref = O.direct_force_soft(p, m, q.astype(np.float32), 0.0, np.inf)
+            errs[use_quad].append(np.linalg.norm(F - ref) / np.linalg.norm(ref))
+    slope_mono = np.polyfit(np.log(ds), np.log(errs[False]), 1)[0]
+    slope_quad = np.polyfit(np.log(ds), np.log(errs[True]), 1)[0]
+    assert -2.3 < slope_mono < -1.7
+    assert -3.3 < slope_quad < -2.7
+
+
+@pytest.mark.parametrize("use_quad", [False, True])
+def test_tree_adjoint_matches_central_differences(use_quad):
+    n = 700
+    p = _plummer(n, 3)
+    rng = _rng(3)
+    m = (rng.lognormal(0, 0.5, n) / n).astype(np.float32)
+    c = rng.normal(size=(n, 3))
+    T = O.Tree(p, m, max_depth=10, leaf_size=8)
+    eps, beta = 0.02, 1.2
+    boxes = T.group_boxes(T.points_sorted, 64)
+    r = T.eval(beta=beta, eps=eps, use_quad=use_quad, cot_sorted=c, boxes=boxes)
+    assert r["far"] > 0 and r["near"] > 0
+    x0 = T.points_sorted.astype(np.float64)
+    m0 = T.masses_sorted.astype(np.float64)
+
+    def loss(xs, ms):
+        f = T.eval(beta=beta, eps=eps, use_quad=use_quad, xs=xs, ms=ms, boxes=boxes)["force"]
+        return float((f * c).sum())
+
+    gx = r["gq"] + r["gsx"]
+    h = 1e-6
+    scale_x = np.abs(gx).max()
+    scale_m = np.abs(r["gsm"]).max()
+    for k in rng.choice(n, 12, replace=False):
+        for a in range(3):
+            xp, xm = x0.copy(), x0.copy()
+            xp[k, a] += h
+            xm[k, a] -= h
+            fd = (loss(xp, m0) - loss(xm, m0)) / (2 * h)
+            assert abs(gx[k, a] - fd) <= 2e-6 * scale_x, (k, a, gx[k, a], fd)
+        mp, mm = m0.copy(), m0.copy()
+        mp[k] += h * m0[k]
+        mm[k] -= h * m0[k]
+        fd = (loss(x0, mp) - loss(x0, mm)) / (2 * h * m0[k])
+        assert abs(r["gsm"][k] - fd) <= 2e-6 * scale_m, (k, r["gsm"][k], fd)
