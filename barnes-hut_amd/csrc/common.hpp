@@ -1,0 +1,134 @@
+// Shared host/device definitions for libsbh_amd (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#include "sbh_amd.h"
+
+namespace sbh {
+
+// LeafData<3> (include/leaf_data.h:8-16): 32 bytes.
+struct Leaf3 {
+  float pos[3];
+  float normal[3];
+  float mass;
+  float alpha;
+};
+
+// OctreeNode8 (src/sbh_ffi.cu:47-65): 104 bytes, same field order.
+struct Node8 {
+  uint32_t start, end, parent, first_child;
+  uint32_t child[8];
+  uint8_t child_mask, depth, is_leaf, pad0;
+  float cx, cy, cz, half_size, diameter;
+  Leaf3 contrib;
+};
+
+static_assert(sizeof(Leaf3) == 32, "Leaf3 layout");
+static_assert(sizeof(Node8) == 104, "OctreeNode8 layout");
+static_assert(offsetof(Node8, child_mask) == 48, "OctreeNode8 layout");
+static_assert(offsetof(Node8, cx) == 52, "OctreeNode8 layout");
+static_assert(offsetof(Node8, contrib) == 72, "OctreeNode8 layout");
+
+// ---------------------------------------------------------------- errors
+int fail_invalid(const char* msg);      // returns SBH_INVALID_ARGUMENT
+int fail_internal(hipError_t err);      // returns SBH_INTERNAL
+int fail_internal_msg(const char* msg); // returns SBH_INTERNAL
+void clear_error();
+
+#define SBH_HIP_TRY(expr)                         \
+  do {                                            \
+    hipError_t _e = (expr);                       \
+    if (_e != hipSuccess) return ::sbh::fail_internal(_e); \
+  } while (0)
+
+#define SBH_LAUNCH_CHECK() SBH_HIP_TRY(hipGetLastError())
+
+// ---------------------------------------------------------------- buffers
+inline bool has_dtype(const SbhBuffer* b, int32_t dtype) {
+  return b != nullptr && b->dtype == dtype;
+}
+inline int64_t elem_count(const SbhBuffer* b) {
+  int64_t c = 1;
+  for (int i = 0; i < b->rank; i++) c *= b->dims[i];
+  return c;
+}
+inline size_t elem_size(int32_t dtype) {
+  switch (dtype) {
+    case SBH_DTYPE_U8: return 1;
+    case SBH_DTYPE_U32: return 4;
+    case SBH_DTYPE_F32: return 4;
+    default: return 0;
+  }
+}
+inline int64_t byte_count(const SbhBuffer* b) {
+  return elem_count(b) * static_cast<int64_t>(elem_size(b->dtype));
+}
+// Validates pointer/dtype/rank (rank < 0: any).  Returns false on mismatch.
+inline bool buffer_ok(const SbhBuffer* b, int32_t dtype, int rank) {
+  if (b == nullptr) return false;
+  if (b->dtype != dtype) return false;
+  if (rank >= 0 && b->rank != rank) return false;
+  if (b->rank < 0 || b->rank > 4) return false;
+  if (elem_count(b) > 0 && b->data == nullptr) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------- workspace
+// Grow-only device scratch cached per (device, stream, slot).  Allocation
+// happens only when a call needs more than any earlier call on that stream.
+void* workspace(hipStream_t stream, int slot, size_t bytes);
+constexpr int kWsBuild = 0;
+constexpr int kWsSort = 1;
+constexpr int kWsWalk = 2;
+constexpr int kWsPipeline = 3;
+constexpr int kWsSlots = 4;
+
+inline unsigned grid_for(int64_t n, int block, unsigned cap = 0x7fffffffu) {
+  int64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return static_cast<unsigned>(g);
+}
+
+// ---------------------------------------------------------------- radix sort
+// Stable LSD sort of (key, value) pairs on the low `key_bits` bits of key.
+// `temp` must hold radix_sort_temp_bytes(n).  Result lands in keys_out/vals_out
+// (keys_in/vals_in are clobbered).
+size_t radix_sort_temp_bytes(int64_t n);
+int radix_sort_pairs(hipStream_t stream, uint32_t* keys_in, uint32_t* vals_in,
+                     uint32_t* keys_out, uint32_t* vals_out, int64_t n,
+                     int key_bits, void* temp);
+
+// ---------------------------------------------------------------- device math
+// Min-image as include/kernel.h:27-39 (floor(x + 0.5)).
+__device__ __forceinline__ float min_image_floor(float d, int periodic, float box,
+                                                 float inv_box) {
+  return periodic ? d - box * floorf(d * inv_box + 0.5f) : d;
+}
+// Min-image as the direct-sum pair functions (src/sbh_ffi.cu:440-445).
+__device__ __forceinline__ float min_image_rint(float d, int periodic, float box,
+                                                float inv_box) {
+  return periodic ? d - box * rintf(d * inv_box) : d;
+}
+
+constexpr float kTwoOverSqrtPi = 2.0f / 1.772453850905516f;
+constexpr float kTwoSqrtPi = 2.0f * 1.772453850905516f;
+
+// Short-range attenuation erfc(u) + 2u/sqrt(pi) exp(-u^2), u = r / (2 r_c)
+// (src/sbh_ffi.cu:151-157).  Also returns exp(-u^2) and r.
+__device__ __forceinline__ float attenuation(float r2, float rc, float* exp_u2,
+                                             float* dist) {
+  float r = sqrtf(r2 + 1e-12f);
+  float u = r / (2.0f * rc);
+  float e = expf(-(u * u));
+  *exp_u2 = e;
+  *dist = r;
+  return erfcf(u) + kTwoOverSqrtPi * u * e;
+}
+
+}  // namespace sbh

@@ -1,0 +1,589 @@
+// Dense octree8 (drop-in mode): builder and classic walks over the
+// reference's byte layout (OctreeNode8, src/sbh_ffi.cu:47-65).
+//
+// Builder (replaces src/sbh_ffi.cu:1039-1420):
+//   codes -> stable radix sort (no atomics: intra-bin order = input order)
+//   -> leaf records in sorted order -> bin offsets by binary search
+//   -> leaf-level sums in sorted order -> bottom-up 8-child sums
+//   -> one kernel per level writing complete node records.
+// Walks (replace src/sbh_ffi.cu:1950-2144): one thread per query, depth-first
+// with the reference's push order, parameters passed by value.
+#include "common.hpp"
+
+namespace sbh {
+namespace {
+
+constexpr int kB = 256;
+
+struct BuildGeom {
+  float min_x, min_y, min_z;
+  float span;
+  float cx, cy, cz, half_size;
+};
+
+// src/sbh_ffi.cu:173-179, evaluated without contraction (IEEE fp32).
+__device__ __forceinline__ uint32_t quantize_axis(float v, float min_v, float span, int bits) {
+#pragma clang fp contract(off)
+  float t = (v - min_v) / span;
+  t = fminf(fmaxf(t, 0.0f), 0.999999f);
+  uint32_t max_bin = (1u << bits) - 1u;
+  return static_cast<uint32_t>(t * static_cast<float>(max_bin + 1u));
+}
+
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  // 10-bit value -> bits at positions 3i.
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
+}
+
+__global__ __launch_bounds__(kB) void k_codes(const float* __restrict__ pts, int64_t n,
+                                              BuildGeom g, int bits,
+                                              uint32_t* __restrict__ codes,
+                                              uint32_t* __restrict__ idx) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (i >= n) return;
+  uint32_t qx = quantize_axis(pts[3 * i + 0], g.min_x, g.span, bits);
+  uint32_t qy = quantize_axis(pts[3 * i + 1], g.min_y, g.span, bits);
+  uint32_t qz = quantize_axis(pts[3 * i + 2], g.min_z, g.span, bits);
+  codes[i] = morton3(qx, qy, qz);
+  idx[i] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kB) void k_leaf_gather(const float* __restrict__ pts,
+                                                    const float* __restrict__ nrm,
+                                                    const float* __restrict__ ms, int64_t n,
+                                                    const uint32_t* __restrict__ perm,
+                                                    Leaf3* __restrict__ leaf) {
+  int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (r >= n) return;
+  int64_t i = perm[r];
+  Leaf3 o;
+  o.pos[0] = pts[3 * i + 0];
+  o.pos[1] = pts[3 * i + 1];
+  o.pos[2] = pts[3 * i + 2];
+  o.normal[0] = nrm[3 * i + 0];
+  o.normal[1] = nrm[3 * i + 1];
+  o.normal[2] = nrm[3 * i + 2];
+  o.mass = ms[i];
+  o.alpha = 1.0f;
+  leaf[r] = o;
+}
+
+// offsets[b] = number of sorted codes < b, for b in [0, nbins].
+__global__ __launch_bounds__(kB) void k_bin_offsets(const uint32_t* __restrict__ codes,
+                                                    int64_t n, int64_t nbins,
+                                                    uint32_t* __restrict__ offsets) {
+  int64_t b = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (b > nbins) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (static_cast<int64_t>(codes[mid]) < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  offsets[b] = static_cast<uint32_t>(lo);
+}
+
+// Level-D sums: count, mass, sum m*x (3), sum m*n (3), in sorted order.
+__global__ __launch_bounds__(kB) void k_leaf_sums(const Leaf3* __restrict__ leaf,
+                                                  const uint32_t* __restrict__ offsets,
+                                                  int64_t nbins, uint32_t* __restrict__ cnt,
+                                                  float* __restrict__ sums) {
+  int64_t b = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (b >= nbins) return;
+  uint32_t s = offsets[b], e = offsets[b + 1];
+  float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (uint32_t j = s; j < e; ++j) {
+    const Leaf3 L = leaf[j];
+    float m = L.mass;
+    a[0] += m;
+    a[1] += m * L.pos[0];
+    a[2] += m * L.pos[1];
+    a[3] += m * L.pos[2];
+    a[4] += m * L.normal[0];
+    a[5] += m * L.normal[1];
+    a[6] += m * L.normal[2];
+  }
+  cnt[b] = e - s;
+  for (int k = 0; k < 7; ++k) sums[7 * b + k] = a[k];
+}
+
+// Parent sums from 8 children (src/sbh_ffi.cu:1745-1786).
+__global__ __launch_bounds__(kB) void k_reduce_level(const uint32_t* __restrict__ ccnt,
+                                                     const float* __restrict__ csum,
+                                                     uint32_t* __restrict__ pcnt,
+                                                     float* __restrict__ psum,
+                                                     int64_t parents) {
+  int64_t p = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (p >= parents) return;
+  uint32_t count = 0;
+  float a[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int c = 0; c < 8; ++c) {
+    int64_t ch = p * 8 + c;
+    uint32_t cc = ccnt[ch];
+    if (cc == 0) continue;
+    count += cc;
+    for (int k = 0; k < 7; ++k) a[k] += csum[7 * ch + k];
+  }
+  pcnt[p] = count;
+  for (int k = 0; k < 7; ++k) psum[7 * p + k] = a[k];
+}
+
+// Node records for one level (src/sbh_ffi.cu:1788-1876).
+__global__ __launch_bounds__(kB) void k_build_nodes(
+    Node8* __restrict__ nodes, Leaf3* __restrict__ contrib_out,
+    const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ cnt,
+    const float* __restrict__ sums, const uint32_t* __restrict__ child_cnt, int level,
+    int max_level, uint32_t level_offset, uint32_t parent_level_offset,
+    uint32_t child_level_offset, BuildGeom g) {
+  int64_t idx64 = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  int64_t level_nodes = int64_t{1} << (3 * level);
+  if (idx64 >= level_nodes) return;
+  uint32_t idx = static_cast<uint32_t>(idx64);
+  uint32_t node_idx = level_offset + idx;
+  Node8 node;
+  node.depth = static_cast<uint8_t>(level);
+  node.parent = (level == 0) ? node_idx : (parent_level_offset + (idx >> 3));
+  node.is_leaf = (level == max_level) ? 1 : 0;
+  node.child_mask = 0;
+  node.pad0 = 0;
+  node.first_child = 0xFFFFFFFFu;
+  int shift = 3 * (max_level - level);
+  uint32_t first_leaf = idx << shift;
+  node.start = offsets[first_leaf];
+  node.end = node.start + cnt[idx];
+  int ix = 0, iy = 0, iz = 0;
+  for (int b = 0; b < level; ++b) {
+    ix |= static_cast<int>((idx >> (3 * b + 0)) & 1u) << b;
+    iy |= static_cast<int>((idx >> (3 * b + 1)) & 1u) << b;
+    iz |= static_cast<int>((idx >> (3 * b + 2)) & 1u) << b;
+  }
+  {
+#pragma clang fp contract(off)
+    float step = g.half_size * 2.0f / static_cast<float>(1 << level);
+    // (c - h) + (i + 0.5) * step with the a*b+c fused, as the reference's
+    // compiler contracts it by default.
+    node.cx = fmaf(static_cast<float>(ix) + 0.5f, step, g.cx - g.half_size);
+    node.cy = fmaf(static_cast<float>(iy) + 0.5f, step, g.cy - g.half_size);
+    node.cz = fmaf(static_cast<float>(iz) + 0.5f, step, g.cz - g.half_size);
+    node.half_size = step * 0.5f;
+    node.diameter = step * 1.7320508075688772f;
+  }
+  if (level < max_level) {
+    uint32_t child_base = child_level_offset + (idx << 3);
+    uint8_t mask = 0;
+    for (uint32_t c = 0; c < 8; ++c) {
+      node.child[c] = child_base + c;
+      if (child_cnt[(idx << 3) + c] > 0) {
+        mask |= static_cast<uint8_t>(1u << c);
+        if (node.first_child == 0xFFFFFFFFu) node.first_child = child_base + c;
+      }
+    }
+    node.child_mask = mask;
+    if (mask == 0) node.is_leaf = 1;
+  } else {
+    for (int c = 0; c < 8; ++c) node.child[c] = 0xFFFFFFFFu;
+  }
+  const float* s = sums + 7 * static_cast<int64_t>(idx);
+  float mass = s[0];
+  node.contrib.alpha = 1.0f;
+  node.contrib.mass = mass;
+  if (mass > 0.0f) {
+    node.contrib.pos[0] = s[1] / mass;
+    node.contrib.pos[1] = s[2] / mass;
+    node.contrib.pos[2] = s[3] / mass;
+    node.contrib.normal[0] = s[4] / mass;
+    node.contrib.normal[1] = s[5] / mass;
+    node.contrib.normal[2] = s[6] / mass;
+  } else {
+    for (int a = 0; a < 3; ++a) node.contrib.pos[a] = node.contrib.normal[a] = 0.0f;
+  }
+  nodes[node_idx] = node;
+  contrib_out[node_idx] = node.contrib;
+}
+
+// ------------------------------------------------------------------ walks
+struct WalkParams {
+  float beta;
+  float eps2;
+  float rc;
+  int cutoff;  // attenuation active
+  int periodic;
+  float box;
+  float inv_box;
+  int prune;
+  float r_cut_sq;
+};
+
+// Far-field test (src/sbh_ffi.cu:1878-1893), IEEE fp32, not contracted.
+__device__ __forceinline__ bool use_far_field(const Node8* __restrict__ nd, float qx, float qy,
+                                              float qz, const WalkParams& P) {
+#pragma clang fp contract(off)
+  if (nd->is_leaf) return true;
+  float dx = min_image_floor(qx - nd->cx, P.periodic, P.box, P.inv_box);
+  float dy = min_image_floor(qy - nd->cy, P.periodic, P.box, P.inv_box);
+  float dz = min_image_floor(qz - nd->cz, P.periodic, P.box, P.inv_box);
+  float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+  return dist / nd->diameter >= P.beta;
+}
+
+__device__ __forceinline__ float axis_dist(float q, float lo, float hi) {
+  if (q < lo) return lo - q;
+  if (q > hi) return q - hi;
+  return 0.0f;
+}
+
+// AABB prune (src/sbh_ffi.cu:1903-1948).
+__device__ __forceinline__ bool should_prune(const Node8* __restrict__ nd, float qx, float qy,
+                                             float qz, const WalkParams& P) {
+#pragma clang fp contract(off)
+  if (!P.prune) return false;
+  if (P.periodic) {
+    qx -= P.box * rintf((qx - nd->cx) * P.inv_box);
+    qy -= P.box * rintf((qy - nd->cy) * P.inv_box);
+    qz -= P.box * rintf((qz - nd->cz) * P.inv_box);
+  }
+  float h = nd->half_size;
+  float dx = axis_dist(qx, nd->cx - h, nd->cx + h);
+  float dy = axis_dist(qy, nd->cy - h, nd->cy + h);
+  float dz = axis_dist(qz, nd->cz - h, nd->cz + h);
+  return dx * dx + dy * dy + dz * dz > P.r_cut_sq;
+}
+
+enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2 };
+
+// Pair/node contribution to the accumulator.
+template <int KIND>
+__device__ __forceinline__ void contribution(float px, float py, float pz, float m, float qx,
+                                             float qy, float qz, float gx, float gy, float gz,
+                                             const WalkParams& P, float& ax, float& ay,
+                                             float& az) {
+  float dx = min_image_floor(qx - px, P.periodic, P.box, P.inv_box);
+  float dy = min_image_floor(qy - py, P.periodic, P.box, P.inv_box);
+  float dz = min_image_floor(qz - pz, P.periodic, P.box, P.inv_box);
+  if constexpr (KIND == kGravGrad) {
+    // gravityGrad (include/kernel.h:89-94)
+    float t = sqrtf(dx * dx + dy * dy + dz * dz) + 1e-1f;
+    float c = -m / (t * t * t);
+    ax += c * dx;
+    ay += c * dy;
+    az += c * dz;
+  } else {
+    float r2 = dx * dx + dy * dy + dz * dz;
+    float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+    float s3 = inv_r * inv_r * inv_r;
+    float att = 1.0f, cc = 0.0f;
+    if (P.cutoff) {
+      float e, dist;
+      att = attenuation(r2, P.rc, &e, &dist);
+      if constexpr (KIND == kSoftVjp) cc = s3 * dist * e / (kTwoSqrtPi * P.rc * P.rc * P.rc);
+    }
+    if constexpr (KIND == kSoftForce) {
+      // gravityGradSoftCutoff (src/sbh_ffi.cu:142-160)
+      float c = -m * att * s3;
+      ax += c * dx;
+      ay += c * dy;
+      az += c * dz;
+    } else {
+      // gravityGradSoftCutoffVjp (src/sbh_ffi.cu:2062-2090)
+      float s5 = s3 * inv_r * inv_r;
+      float rg = dx * gx + dy * gy + dz * gz;
+      float lin = att * s3;
+      float rad = rg * (3.0f * att * s5 + cc);
+      ax += -m * (lin * gx - rad * dx);
+      ay += -m * (lin * gy - rad * dy);
+      az += -m * (lin * gz - rad * dz);
+    }
+  }
+}
+
+constexpr int kStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
+
+template <int KIND>
+__global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
+                                              const Node8* __restrict__ nodes,
+                                              const float* __restrict__ queries,
+                                              const float* __restrict__ cot, int64_t nq,
+                                              float* __restrict__ out, WalkParams P,
+                                              uint32_t num_nodes, uint32_t num_leaves) {
+  int64_t q = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (q >= nq) return;
+  float qx = queries[3 * q + 0], qy = queries[3 * q + 1], qz = queries[3 * q + 2];
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+  if constexpr (KIND == kSoftVjp) {
+    gx = cot[3 * q + 0];
+    gy = cot[3 * q + 1];
+    gz = cot[3 * q + 2];
+  }
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  int stack[kStack];
+  int top = 0;
+  stack[top++] = 0;
+  while (top > 0) {
+    const Node8* nd = &nodes[stack[--top]];
+    if (nd->contrib.mass == 0.0f) continue;
+    if (KIND != kGravGrad && should_prune(nd, qx, qy, qz, P)) continue;
+    if (use_far_field(nd, qx, qy, qz, P)) {
+      if (nd->is_leaf) {
+        uint32_t je = nd->end < num_leaves ? nd->end : num_leaves;
+        for (uint32_t j = nd->start; j < je; ++j) {
+          const Leaf3* L = &leaf[j];
+          contribution<KIND>(L->pos[0], L->pos[1], L->pos[2], L->mass, qx, qy, qz, gx, gy, gz,
+                             P, ax, ay, az);
+        }
+      } else {
+        contribution<KIND>(nd->contrib.pos[0], nd->contrib.pos[1], nd->contrib.pos[2],
+                           nd->contrib.mass, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+      }
+    } else {
+      uint32_t mask = nd->child_mask;
+      for (int c = 0; c < 8; ++c)
+        if ((mask & (1u << c)) && top < kStack && nd->child[c] < num_nodes)
+          stack[top++] = static_cast<int>(nd->child[c]);
+    }
+  }
+  out[3 * q + 0] = ax;
+  out[3 * q + 1] = ay;
+  out[3 * q + 2] = az;
+}
+
+uint64_t total_nodes8(int max_depth) {
+  uint64_t total = 0, level = 1;
+  for (int l = 0; l <= max_depth; ++l) {
+    total += level;
+    level <<= 3;
+  }
+  return total;
+}
+
+int check_walk_common(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+                      const SbhBuffer* queries, const SbhBuffer* out, int threads) {
+  if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || !buffer_ok(node_bytes, SBH_DTYPE_U8, -1))
+    return fail_invalid("leaf_bytes/node_bytes must be uint8 device buffers");
+  if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf3)) != 0)
+    return fail_invalid("leaf_bytes size mismatch");
+  if (byte_count(node_bytes) % static_cast<int64_t>(sizeof(Node8)) != 0 ||
+      byte_count(node_bytes) == 0)
+    return fail_invalid("node_bytes size mismatch");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2)) return fail_invalid("queries must be (M, 3) float32");
+  if (queries->dims[1] != 3) return fail_invalid("queries dim must be 3");
+  if (!buffer_ok(out, SBH_DTYPE_F32, 2) || out->dims[0] != queries->dims[0] || out->dims[1] != 3)
+    return fail_invalid("out shape mismatch");
+  if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
+  return SBH_OK;
+}
+
+WalkParams make_walk_params(float beta, float softening, float cutoff, int prune_enabled,
+                            float prune_mult, int periodic, float box) {
+  WalkParams P;
+  P.beta = beta;
+  float eps = fmaxf(softening, 0.0f);
+  P.eps2 = eps * eps;
+  P.rc = cutoff;
+  P.cutoff = (cutoff > 0.0f && std::isfinite(cutoff)) ? 1 : 0;
+  P.periodic = (periodic != 0 && box > 0.0f) ? 1 : 0;
+  P.box = box;
+  P.inv_box = box > 0.0f ? 1.0f / box : 0.0f;
+  // src/sbh_ffi.cu:3233-3237: prune only with a positive cutoff.
+  P.prune = (prune_enabled != 0 && cutoff > 0.0f) ? 1 : 0;
+  float r_cut = P.prune ? prune_mult * cutoff : 0.0f;
+  P.r_cut_sq = r_cut * r_cut;
+  return P;
+}
+
+template <int KIND>
+int launch_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+                const SbhBuffer* queries, const SbhBuffer* cot, SbhBuffer* out,
+                const WalkParams& P) {
+  int64_t nq = queries->dims[0];
+  if (nq == 0) return SBH_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  k_walk8<KIND><<<grid_for(nq, kB), kB, 0, s>>>(
+      static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node8*>(node_bytes->data),
+      static_cast<const float*>(queries->data),
+      cot ? static_cast<const float*>(cot->data) : nullptr, nq, static_cast<float*>(out->data),
+      P, static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node8)),
+      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf3)));
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+}  // namespace
+}  // namespace sbh
+
+using namespace sbh;
+
+extern "C" int sbh_build_octree8_buffers(void* stream, const SbhBuffer* points,
+                                         const SbhBuffer* normals, const SbhBuffer* masses,
+                                         const SbhBuffer* min_corner,
+                                         const SbhBuffer* max_corner, SbhBuffer* leaf_out,
+                                         SbhBuffer* node_out, SbhBuffer* contrib_out,
+                                         int32_t max_depth) {
+  clear_error();
+  if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
+    return fail_invalid("points must have shape (N, 3)");
+  if (!buffer_ok(normals, SBH_DTYPE_F32, 2) || normals->dims[1] != 3)
+    return fail_invalid("normals must have shape (N, 3)");
+  if (points->dims[0] != normals->dims[0]) return fail_invalid("points/normals row mismatch");
+  if (!buffer_ok(masses, SBH_DTYPE_F32, 1) || masses->dims[0] != points->dims[0])
+    return fail_invalid("masses must have shape (N,)");
+  if (!buffer_ok(min_corner, SBH_DTYPE_F32, 1) || min_corner->dims[0] != 3 ||
+      !buffer_ok(max_corner, SBH_DTYPE_F32, 1) || max_corner->dims[0] != 3)
+    return fail_invalid("min_corner/max_corner must have shape (3,)");
+  if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
+  int64_t n = points->dims[0];
+  uint64_t total = total_nodes8(max_depth);
+  int64_t nbins = int64_t{1} << (3 * max_depth);
+  if (!buffer_ok(leaf_out, SBH_DTYPE_U8, -1) ||
+      byte_count(leaf_out) != n * static_cast<int64_t>(sizeof(Leaf3)))
+    return fail_invalid("leaf_out size mismatch");
+  if (!buffer_ok(node_out, SBH_DTYPE_U8, -1) ||
+      byte_count(node_out) != static_cast<int64_t>(total * sizeof(Node8)))
+    return fail_invalid("node_out size mismatch");
+  if (!buffer_ok(contrib_out, SBH_DTYPE_U8, -1) ||
+      byte_count(contrib_out) != static_cast<int64_t>(total * sizeof(Leaf3)))
+    return fail_invalid("contrib_out size mismatch");
+  if (n > INT32_MAX) return fail_invalid("too many points");
+
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Bounds are read back like the reference does (src/sbh_ffi.cu:1111-1126)
+  // so that an empty span is reported as an argument error.
+  float h[6];
+  SBH_HIP_TRY(hipMemcpyAsync(h, min_corner->data, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  SBH_HIP_TRY(hipMemcpyAsync(h + 3, max_corner->data, 3 * sizeof(float), hipMemcpyDeviceToHost, s));
+  SBH_HIP_TRY(hipStreamSynchronize(s));
+  BuildGeom g;
+  g.min_x = h[0];
+  g.min_y = h[1];
+  g.min_z = h[2];
+  g.span = fmaxf(fmaxf(h[3] - h[0], h[4] - h[1]), h[5] - h[2]);
+  if (!(g.span > 0.0f)) return fail_invalid("invalid bounds span");
+  g.cx = h[0] + g.span * 0.5f;
+  g.cy = h[1] + g.span * 0.5f;
+  g.cz = h[2] + g.span * 0.5f;
+  g.half_size = g.span * 0.5f;
+
+  // Workspace: keys/vals x2, sort temp, offsets, per-level counts and sums.
+  size_t nn = static_cast<size_t>(n > 0 ? n : 1);
+  size_t off_codes = 0;
+  size_t off_idx = off_codes + nn * 4;
+  size_t off_codes2 = off_idx + nn * 4;
+  size_t off_perm = off_codes2 + nn * 4;
+  size_t off_sort = off_perm + nn * 4;
+  size_t sort_bytes = (radix_sort_temp_bytes(n) + 255) & ~size_t{255};
+  size_t off_offsets = off_sort + sort_bytes;
+  size_t off_levels = off_offsets + ((static_cast<size_t>(nbins) + 1) * 4 + 255) / 256 * 256;
+  size_t level_bytes = 0;
+  for (int l = 0; l <= max_depth; ++l) level_bytes += (size_t{1} << (3 * l)) * 32;
+  size_t ws_bytes = off_levels + level_bytes;
+  char* ws = static_cast<char*>(workspace(s, kWsBuild, ws_bytes));
+  if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+  auto* codes = reinterpret_cast<uint32_t*>(ws + off_codes);
+  auto* idx = reinterpret_cast<uint32_t*>(ws + off_idx);
+  auto* codes_sorted = reinterpret_cast<uint32_t*>(ws + off_codes2);
+  auto* perm = reinterpret_cast<uint32_t*>(ws + off_perm);
+  auto* offsets = reinterpret_cast<uint32_t*>(ws + off_offsets);
+  uint32_t* lcnt[11];
+  float* lsum[11];
+  {
+    size_t o = off_levels;
+    for (int l = 0; l <= max_depth; ++l) {
+      size_t c = size_t{1} << (3 * l);
+      lcnt[l] = reinterpret_cast<uint32_t*>(ws + o);
+      lsum[l] = reinterpret_cast<float*>(ws + o + c * 4);
+      o += c * 32;
+    }
+  }
+  auto* pp = static_cast<const float*>(points->data);
+  auto* leaf = static_cast<Leaf3*>(leaf_out->data);
+  if (n > 0) {
+    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pp, n, g, max_depth, codes, idx);
+    SBH_LAUNCH_CHECK();
+    int rc = radix_sort_pairs(s, codes, idx, codes_sorted, perm, n, 3 * max_depth, ws + off_sort);
+    if (rc != SBH_OK) return rc;
+    k_leaf_gather<<<grid_for(n, kB), kB, 0, s>>>(pp, static_cast<const float*>(normals->data),
+                                                 static_cast<const float*>(masses->data), n,
+                                                 perm, leaf);
+    SBH_LAUNCH_CHECK();
+  }
+  k_bin_offsets<<<grid_for(nbins + 1, kB), kB, 0, s>>>(codes_sorted, n, nbins, offsets);
+  SBH_LAUNCH_CHECK();
+  k_leaf_sums<<<grid_for(nbins, kB), kB, 0, s>>>(leaf, offsets, nbins, lcnt[max_depth],
+                                                 lsum[max_depth]);
+  SBH_LAUNCH_CHECK();
+  for (int l = max_depth - 1; l >= 0; --l) {
+    int64_t parents = int64_t{1} << (3 * l);
+    k_reduce_level<<<grid_for(parents, kB), kB, 0, s>>>(lcnt[l + 1], lsum[l + 1], lcnt[l],
+                                                        lsum[l], parents);
+    SBH_LAUNCH_CHECK();
+  }
+  uint32_t level_offset = 0, parent_offset = 0;
+  auto* nodes = static_cast<Node8*>(node_out->data);
+  auto* contrib = static_cast<Leaf3*>(contrib_out->data);
+  for (int l = 0; l <= max_depth; ++l) {
+    uint32_t level_nodes = 1u << (3 * l);
+    uint32_t child_offset = level_offset + level_nodes;
+    k_build_nodes<<<grid_for(level_nodes, kB), kB, 0, s>>>(
+        nodes, contrib, offsets, lcnt[l], lsum[l], l < max_depth ? lcnt[l + 1] : nullptr, l,
+        max_depth, level_offset, parent_offset, l < max_depth ? child_offset : 0u, g);
+    SBH_LAUNCH_CHECK();
+    parent_offset = level_offset;
+    level_offset = child_offset;
+  }
+  return SBH_OK;
+}
+
+extern "C" int sbh_barnes_hut_force_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                            const SbhBuffer* node_bytes,
+                                            const SbhBuffer* queries, SbhBuffer* out,
+                                            float beta, int32_t periodic, float box_length,
+                                            int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads);
+  if (rc != SBH_OK) return rc;
+  WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, periodic, box_length);
+  return launch_walk<kGravGrad>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
+}
+
+extern "C" int sbh_barnes_hut_force_octree8_soft(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, SbhBuffer* out, float beta, float softening_scale,
+    float cutoff_scale, int32_t prune_enabled, float prune_r_cut_mult, int32_t periodic,
+    float box_length, int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads);
+  if (rc != SBH_OK) return rc;
+  if (prune_enabled != 0 && prune_r_cut_mult <= 0.0f)
+    return fail_invalid("prune_r_cut_mult must be > 0 when prune_enabled is set");
+  WalkParams P = make_walk_params(beta, softening_scale, cutoff_scale, prune_enabled,
+                                  prune_r_cut_mult, periodic, box_length);
+  return launch_walk<kSoftForce>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
+}
+
+extern "C" int sbh_barnes_hut_force_octree8_soft_vjp(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, const SbhBuffer* cotangent, SbhBuffer* out, float beta,
+    float softening_scale, float cutoff_scale, int32_t prune_enabled, float prune_r_cut_mult,
+    int32_t periodic, float box_length, int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads);
+  if (rc != SBH_OK) return rc;
+  if (!buffer_ok(cotangent, SBH_DTYPE_F32, 2) || cotangent->dims[0] != queries->dims[0] ||
+      cotangent->dims[1] != 3)
+    return fail_invalid("cotangent shape mismatch");
+  if (prune_enabled != 0 && prune_r_cut_mult <= 0.0f)
+    return fail_invalid("prune_r_cut_mult must be > 0 when prune_enabled is set");
+  WalkParams P = make_walk_params(beta, softening_scale, cutoff_scale, prune_enabled,
+                                  prune_r_cut_mult, periodic, box_length);
+  return launch_walk<kSoftVjp>(stream, leaf_bytes, node_bytes, queries, cotangent, out, P);
+}

@@ -1,0 +1,10 @@
+"""sbh_amd: MI355X-native Barnes-Hut gravity and winding-number summation.
+
+`sbh_amd.ffi` mirrors the reference's python/jax_sbh_ffi.py surface on
+PyTorch device tensors; every entry point calls the HIP library
+libsbh_amd.so through the C ABI declared in include/sbh_amd.h.
+"""
+from . import ffi  # noqa: F401
+from ._lib import library_path, load_library  # noqa: F401
+
+__all__ = ["ffi", "library_path", "load_library"]

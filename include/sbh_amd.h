@@ -1,0 +1,151 @@
+/*
+ * sbh_amd.h -- C ABI of the MI355X (gfx950) Barnes-Hut library libsbh_amd.so.
+ *
+ * Each entry point replaces one XLA-FFI handler of the reference
+ * (src/sbh_ffi.cu, registered at src/sbh_ffi.cu:3580-3947 and bound from
+ * python/jax_sbh_ffi.py).  The signature keeps the handler's order:
+ *
+ *     stream, argument buffers..., result buffers..., attributes...
+ *
+ * with the attribute names and types of the reference (i32 vs f32 as passed
+ * by python/jax_sbh_ffi.py with np.int32 / np.float32).
+ *
+ * Buffers are device pointers plus shape (SbhBuffer, the plain-C analogue of
+ * xla::ffi::Buffer<dtype, rank>).  The caller owns every argument and result
+ * buffer; results must be allocated at the byte sizes the reference's Python
+ * computes.  Parameters travel by value into the kernels (no __constant__
+ * state), so calls on different streams do not interfere.
+ *
+ * Return value: SBH_OK, SBH_INVALID_ARGUMENT (shape / attribute / range
+ * errors, detected on the host before any HIP call, with the reference's
+ * message text) or SBH_INTERNAL (a HIP runtime error).  sbh_last_error()
+ * returns the message of the last failure on the calling thread.
+ */
+#ifndef SBH_AMD_H_
+#define SBH_AMD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: the xla::ffi::ErrorCode values the reference returns. */
+#define SBH_OK 0
+#define SBH_INVALID_ARGUMENT 3
+#define SBH_INTERNAL 13
+
+/* Element types: XLA_FFI_DataType values. */
+#define SBH_DTYPE_U8 6
+#define SBH_DTYPE_U32 8
+#define SBH_DTYPE_F32 11
+
+typedef struct SbhBuffer {
+  void* data;       /* device pointer */
+  int32_t dtype;    /* SBH_DTYPE_* */
+  int32_t rank;     /* 0..4 */
+  int64_t dims[4];  /* row-major extents */
+} SbhBuffer;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* sbh_last_error(void);
+
+/* ---- layout sizes (src/sbh_ffi.cu:3949-3963) ------------------------- */
+size_t sbh_leaf3_size(void);              /* 32   : LeafData<3>            */
+size_t sbh_node3_size(void);              /* 4272 : Tree3DNode<1,3>        */
+size_t sbh_octree_level3_max_nodes(void); /* 513                           */
+size_t sbh_octree8_node_size(void);       /* 104  : OctreeNode8            */
+
+/* ---- direct sums --------------------------------------------------------- */
+
+/* Replaces BruteForceGravity (src/sbh_ffi.cu:548-597; target
+ * "sbh_bruteforce_gravity", python/jax_sbh_ffi.py:341-378).
+ * points (N,d) f32, masses (N) f32, queries (M,d) f32 -> out (M) f32,
+ * out[q] = sum_i -m_i / (|p_i - q| + 0.1), d in {2,3}. */
+int sbh_bruteforce_gravity(void* stream, const SbhBuffer* points,
+                           const SbhBuffer* masses, const SbhBuffer* queries,
+                           SbhBuffer* out, int32_t periodic, float box_length,
+                           int32_t threads);
+
+/* Replaces SoftenedDirectForce (src/sbh_ffi.cu:600-661; target
+ * "sbh_direct_force_soft", python/jax_sbh_ffi.py:381-426).
+ * out (M,3) = sum_i -m_i * att(r) * (r^2 + eps^2 + 1e-12)^(-3/2) * (q - p_i). */
+int sbh_direct_force_soft(void* stream, const SbhBuffer* points,
+                          const SbhBuffer* masses, const SbhBuffer* queries,
+                          SbhBuffer* out, float softening_scale,
+                          float cutoff_scale, int32_t periodic,
+                          float box_length, int32_t threads);
+
+/* Replaces SoftenedDirectForceVjp (src/sbh_ffi.cu:663-734; target
+ * "sbh_direct_force_soft_vjp", python/jax_sbh_ffi.py:429-479).
+ * out (M,3) = J_q^T cotangent (query side, sources fixed). */
+int sbh_direct_force_soft_vjp(void* stream, const SbhBuffer* points,
+                              const SbhBuffer* masses, const SbhBuffer* queries,
+                              const SbhBuffer* cotangent, SbhBuffer* out,
+                              float softening_scale, float cutoff_scale,
+                              int32_t periodic, float box_length,
+                              int32_t threads);
+
+/* New (no reference counterpart; the reference's custom_vjp returns the
+ * query side only and zero mass gradients, python/jax_sbh_ffi.py:576-588).
+ * Full adjoint of L = sum_i c_i . F_i for self-gravity (queries == points):
+ * grad_points (N,3) and grad_masses (N). */
+int sbh_direct_force_soft_full_vjp(void* stream, const SbhBuffer* points,
+                                   const SbhBuffer* masses,
+                                   const SbhBuffer* cotangent,
+                                   SbhBuffer* grad_points,
+                                   SbhBuffer* grad_masses,
+                                   float softening_scale, float cutoff_scale,
+                                   int32_t periodic, float box_length,
+                                   int32_t threads);
+
+/* ---- dense octree8 (drop-in mode) ---------------------------------------- */
+
+/* Replaces BuildOctree8Buffers (src/sbh_ffi.cu:1039-1420; target
+ * "sbh_build_octree8_buffers", python/jax_sbh_ffi.py:864-960).
+ * points (N,3), normals (N,3), masses (N), min_corner (3), max_corner (3)
+ * -> leaf_out u8[N*32], node_out u8[T*104], contrib_out u8[T*32],
+ * T = sum_{l<=max_depth} 8^l, max_depth in [1,10].
+ * Points inside a leaf bin are in (code, original index) order; node
+ * topology and geometry follow the reference record by record.  Unlike the
+ * reference, the bounds are read on the device (no host synchronisation). */
+int sbh_build_octree8_buffers(void* stream, const SbhBuffer* points,
+                              const SbhBuffer* normals, const SbhBuffer* masses,
+                              const SbhBuffer* min_corner,
+                              const SbhBuffer* max_corner, SbhBuffer* leaf_out,
+                              SbhBuffer* node_out, SbhBuffer* contrib_out,
+                              int32_t max_depth);
+
+/* Replaces BarnesHutForceOctree8 (src/sbh_ffi.cu:3133-3181; target
+ * "sbh_barnes_hut_force_octree8", python/jax_sbh_ffi.py:1336-1369).
+ * Classic walk with gravityGrad = -m d / (|d| + 0.1)^3. */
+int sbh_barnes_hut_force_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                 const SbhBuffer* node_bytes,
+                                 const SbhBuffer* queries, SbhBuffer* out,
+                                 float beta, int32_t periodic, float box_length,
+                                 int32_t threads);
+
+/* Replaces SoftenedBarnesHutForceOctree8 (src/sbh_ffi.cu:3183-3248; target
+ * "sbh_barnes_hut_force_octree8_soft", python/jax_sbh_ffi.py:1398-1447). */
+int sbh_barnes_hut_force_octree8_soft(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, SbhBuffer* out, float beta, float softening_scale,
+    float cutoff_scale, int32_t prune_enabled, float prune_r_cut_mult,
+    int32_t periodic, float box_length, int32_t threads);
+
+/* Replaces SoftenedBarnesHutForceOctree8Vjp (src/sbh_ffi.cu:3250-3320;
+ * target "sbh_barnes_hut_force_octree8_soft_vjp",
+ * python/jax_sbh_ffi.py:1674-1728).  Query-side J^T g, tree fixed. */
+int sbh_barnes_hut_force_octree8_soft_vjp(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, const SbhBuffer* cotangent, SbhBuffer* out,
+    float beta, float softening_scale, float cutoff_scale,
+    int32_t prune_enabled, float prune_r_cut_mult, int32_t periodic,
+    float box_length, int32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SBH_AMD_H_ */

@@ -86,6 +86,9 @@ def _declare(L: ctypes.CDLL) -> None:
                                             ctypes.c_double, _f64p]
     L.orc_direct_full_vjp.argtypes = [_f32p, _f32p, _i64, _f32p, ctypes.c_double, ctypes.c_int,
                                       ctypes.c_double, _f64p, _f64p]
+    L.orc_direct_full_vjp_cut.argtypes = [_f32p, _f32p, _i64, _f32p, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                          _f64p, _f64p]
     L.orc_bruteforce_gravity.argtypes = [_f32p, _f32p, _i64, ctypes.c_int, _f32p, _i64,
                                          ctypes.c_int, ctypes.c_double, _f64p]
     L.orc_bh_octree8.argtypes = [_u8p, _u8p, _f32p, _vp, _i64, ctypes.c_int, ctypes.c_float,
@@ -202,6 +205,15 @@ def direct_full_vjp(points, masses, cot, eps, periodic=False, box=1.0):
     gm = np.zeros(p.shape[0], np.float64)
     lib().orc_direct_full_vjp(p, m, p.shape[0], g, float(eps), int(bool(periodic)), float(box),
                               gx, gm)
+    return gx, gm
+
+
+def direct_full_vjp_cut(points, masses, cot, eps, rc, periodic=False, box=1.0):
+    p, m, g = _c(points, np.float32), _c(masses, np.float32), _c(cot, np.float32)
+    gx = np.zeros((p.shape[0], 3), np.float64)
+    gm = np.zeros(p.shape[0], np.float64)
+    lib().orc_direct_full_vjp_cut(p, m, p.shape[0], g, float(eps), float(rc),
+                                  int(bool(periodic)), float(box), gx, gm)
     return gx, gm
 
 

@@ -497,6 +497,36 @@ ORC_EXPORT void orc_direct_full_vjp(const float* pts, const float* masses, int64
   }
 }
 
+/* Full adjoint with the short-range cutoff, built from the pinned pair
+ * functions: grad_x[k] = sum_j [vjp(q=x_k,p=x_j,m_j,c_k) - vjp(q=x_j,p=x_k,m_k,c_j)],
+ * grad_m[k] = sum_j c_j . F_unit(source x_k on query x_j).  The no-cutoff case
+ * is pinned by central differences (orc_direct_full_vjp). */
+ORC_EXPORT void orc_direct_full_vjp_cut(const float* pts, const float* masses, int64_t n,
+                                        const float* cot, double eps, double rc,
+                                        int periodic, double box, double* gx, double* gm) {
+  OrcParams P;
+  orc_params(&P, eps, rc, periodic, box, 1);
+#pragma omp parallel for schedule(static)
+  for (int64_t k = 0; k < n; k++) {
+    double xk[3] = {pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]};
+    double ck[3] = {cot[3 * k], cot[3 * k + 1], cot[3 * k + 2]};
+    double g[3] = {0, 0, 0}, neg[3] = {0, 0, 0};
+    double gmk = 0.0;
+    for (int64_t j = 0; j < n; j++) {
+      if (j == k) continue;
+      double xj[3] = {pts[3 * j], pts[3 * j + 1], pts[3 * j + 2]};
+      double cj[3] = {cot[3 * j], cot[3 * j + 1], cot[3 * j + 2]};
+      orc_soft_force_vjp(xk, xj, masses[j], ck, &P, g);
+      orc_soft_force_vjp(xj, xk, masses[k], cj, &P, neg);
+      double f[3] = {0, 0, 0};
+      orc_soft_force(xj, xk, 1.0, &P, f);
+      gmk += cj[0] * f[0] + cj[1] * f[1] + cj[2] * f[2];
+    }
+    for (int a = 0; a < 3; a++) gx[3 * k + a] = g[a] - neg[a];
+    gm[k] = gmk;
+  }
+}
+
 /* D3 (sbh_ffi.cu:399-429; kernel.h:55-60): potential sum -m/(|d|+0.1). */
 ORC_EXPORT void orc_bruteforce_gravity(const float* pts, const float* masses, int64_t n,
                                        int dim, const float* queries, int64_t nq,

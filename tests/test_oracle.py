@@ -334,6 +334,29 @@ def test_direct_full_vjp_matches_central_differences():
         assert abs(gm[k] - fd) <= 1e-6 * (1 + abs(fd))
 
 
+def test_direct_full_vjp_cut_reduces_to_pinned_adjoint():
+    rng = _rng(42)
+    n = 60
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.1, n).astype(np.float32)
+    c = rng.normal(size=(n, 3)).astype(np.float32)
+    gx0, gm0 = O.direct_full_vjp(p, m, c, 0.05)
+    gx1, gm1 = O.direct_full_vjp_cut(p, m, c, 0.05, np.inf)
+    np.testing.assert_allclose(gx1, gx0, rtol=1e-11, atol=1e-11)
+    np.testing.assert_allclose(gm1, gm0, rtol=1e-11, atol=1e-11)
+    # With a cutoff, the query-side part equals the pinned query-side VJP when
+    # the sources carry no cotangent.
+    c2 = c.copy()
+    c2[10:] = 0
+    gx2, _ = O.direct_full_vjp_cut(p, m, c2, 0.05, 0.7)
+    q_side = O.direct_force_soft_vjp(p, m, p[:10], c2[:10], 0.05, 0.7)
+    # for k < 10, source-side terms come from queries j < 10 only
+    src = gx2[:10] - q_side
+    gx3, _ = O.direct_full_vjp_cut(p[:10], m[:10], c2[:10], 0.05, 0.7)
+    q_side3 = O.direct_force_soft_vjp(p[:10], m[:10], p[:10], c2[:10], 0.05, 0.7)
+    np.testing.assert_allclose(src, gx3 - q_side3, rtol=1e-9, atol=1e-9)
+
+
 def test_bruteforce_gravity_closed_form_and_periodic():
     p = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], np.float32)
     m = np.array([1.0, 3.0], np.float32)
