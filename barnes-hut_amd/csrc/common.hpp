@@ -60,8 +60,10 @@ inline int64_t elem_count(const SbhBuffer* b) {
 inline size_t elem_size(int32_t dtype) {
   switch (dtype) {
     case SBH_DTYPE_U8: return 1;
+    case SBH_DTYPE_S32: return 4;
     case SBH_DTYPE_U32: return 4;
     case SBH_DTYPE_F32: return 4;
+    case SBH_DTYPE_S64: return 8;
     default: return 0;
   }
 }
@@ -114,6 +116,30 @@ __device__ __forceinline__ float min_image_floor(float d, int periodic, float bo
 __device__ __forceinline__ float min_image_rint(float d, int periodic, float box,
                                                 float inv_box) {
   return periodic ? d - box * rintf(d * inv_box) : d;
+}
+
+// Morton quantiser (src/sbh_ffi.cu:173-179), IEEE fp32 without contraction.
+__device__ __forceinline__ uint32_t quantize_axis(float v, float min_v, float span, int bits) {
+#pragma clang fp contract(off)
+  float t = (v - min_v) / span;
+  t = fminf(fmaxf(t, 0.0f), 0.999999f);
+  uint32_t max_bin = (1u << bits) - 1u;
+  return static_cast<uint32_t>(t * static_cast<float>(max_bin + 1u));
+}
+
+// Bit interleave (src/sbh_ffi.cu:162-171): x -> bit 3i, y -> 3i+1, z -> 3i+2,
+// for coordinates of up to 10 bits.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
 }
 
 constexpr float kTwoOverSqrtPi = 2.0f / 1.772453850905516f;

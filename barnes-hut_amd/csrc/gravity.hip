@@ -1,0 +1,209 @@
+// C-ABI entry points of the adaptive-tree gravity pipeline (new targets: the
+// reference has no adaptive tree, no leaf-size parameter and no full adjoint).
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "tree.hpp"
+
+using namespace sbh;
+
+namespace {
+
+// Optional phase timing: events recorded on the call's stream around the
+// build and around the walk kernel; read back by sbh_profile_collect().
+struct ProfRec {
+  hipEvent_t ev[4];  // build start, build end, walk kernel start, walk kernel end
+  int vjp;
+};
+std::mutex g_prof_mu;
+bool g_prof_on = false;
+std::vector<ProfRec> g_prof;
+
+bool prof_begin(hipStream_t s, ProfRec* r, int vjp) {
+  {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    if (!g_prof_on) return false;
+  }
+  for (int k = 0; k < 4; ++k)
+    if (hipEventCreate(&r->ev[k]) != hipSuccess) return false;
+  r->vjp = vjp;
+  return hipEventRecord(r->ev[0], s) == hipSuccess;
+}
+
+void prof_end(const ProfRec& r) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof.push_back(r);
+}
+
+int check_common(const SbhBuffer* points, const SbhBuffer* masses, float beta, float softening,
+                 int max_depth, int leaf_size, int order) {
+  if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
+    return fail_invalid("points must have shape (N, 3)");
+  if (!buffer_ok(masses, SBH_DTYPE_F32, 1) || masses->dims[0] != points->dims[0])
+    return fail_invalid("masses must have shape (N,)");
+  if (points->dims[0] > (int64_t{1} << 30)) return fail_invalid("too many points");
+  if (!(beta >= 0.0f)) return fail_invalid("beta must be >= 0");
+  if (softening < 0.0f) return fail_invalid("softening_scale must be >= 0");
+  if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
+  if (leaf_size < 1) return fail_invalid("leaf_size must be >= 1");
+  if (order != 1 && order != 2) return fail_invalid("multipole_order must be 1 or 2");
+  return SBH_OK;
+}
+
+int resolve_workspace(hipStream_t s, SbhBuffer* workspace, const tree::Layout& L, char** ws) {
+  if (workspace != nullptr) {
+    if (!buffer_ok(workspace, SBH_DTYPE_U8, -1) ||
+        byte_count(workspace) < static_cast<int64_t>(L.total))
+      return fail_invalid("workspace too small");
+    *ws = static_cast<char*>(workspace->data);
+    return SBH_OK;
+  }
+  *ws = static_cast<char*>(::sbh::workspace(s, kWsPipeline, L.total));
+  if (*ws == nullptr) return fail_internal_msg("workspace allocation failed");
+  return SBH_OK;
+}
+
+int check_stats(const SbhBuffer* stats) {
+  if (stats == nullptr) return SBH_OK;
+  if (!buffer_ok(stats, SBH_DTYPE_S64, -1) || elem_count(stats) < 8)
+    return fail_invalid("stats must be an int64 buffer of at least 8 elements");
+  return SBH_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t sbh_gravity_workspace_bytes(int64_t n, int32_t max_depth, int32_t leaf_size) {
+  if (n < 0 || max_depth < 1 || max_depth > 10 || leaf_size < 1) return -1;
+  return static_cast<int64_t>(tree::layout(n, max_depth, leaf_size).total);
+}
+
+extern "C" int sbh_gravity_workspace_layout(int64_t n, int32_t max_depth, int32_t leaf_size,
+                                            int64_t* offsets, int32_t count) {
+  clear_error();
+  if (n < 0 || max_depth < 1 || max_depth > 10 || leaf_size < 1 || offsets == nullptr)
+    return fail_invalid("invalid layout query");
+  tree::Layout L = tree::layout(n, max_depth, leaf_size);
+  const int64_t v[] = {static_cast<int64_t>(L.header), static_cast<int64_t>(L.codes),
+                       static_cast<int64_t>(L.perm),   static_cast<int64_t>(L.sp),
+                       static_cast<int64_t>(L.sc),     static_cast<int64_t>(L.topo),
+                       static_cast<int64_t>(L.prefix), static_cast<int64_t>(L.parent),
+                       static_cast<int64_t>(L.geo),    static_cast<int64_t>(L.mp),
+                       static_cast<int64_t>(L.qd),     static_cast<int64_t>(L.adj),
+                       static_cast<int64_t>(L.adjq),   static_cast<int64_t>(L.qgrad),
+                       static_cast<int64_t>(L.sgrad),  static_cast<int64_t>(L.total),
+                       L.cap};
+  const int nv = static_cast<int>(sizeof(v) / sizeof(v[0]));
+  for (int i = 0; i < count && i < nv; ++i) offsets[i] = v[i];
+  return SBH_OK;
+}
+
+extern "C" int sbh_gravity_force(void* stream, const SbhBuffer* points, const SbhBuffer* masses,
+                                 SbhBuffer* forces, float beta, float softening_scale,
+                                 int32_t max_depth, int32_t leaf_size, int32_t multipole_order,
+                                 SbhBuffer* workspace, SbhBuffer* stats) {
+  clear_error();
+  int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
+                        multipole_order);
+  if (rc != SBH_OK) return rc;
+  int64_t n = points->dims[0];
+  if (!buffer_ok(forces, SBH_DTYPE_F32, 2) || forces->dims[0] != n || forces->dims[1] != 3)
+    return fail_invalid("forces shape mismatch");
+  if ((rc = check_stats(stats)) != SBH_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tree::Layout L = tree::layout(n, max_depth, leaf_size);
+  char* ws = nullptr;
+  if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  ProfRec pr;
+  bool prof = prof_begin(s, &pr, 0);
+  rc = tree::build(s, ws, L, static_cast<const float*>(points->data),
+                   static_cast<const float*>(masses->data), nullptr, n, max_depth, leaf_size,
+                   nullptr);
+  if (rc != SBH_OK) return rc;
+  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
+                  static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
+                  prof && n > 0 ? pr.ev + 2 : nullptr);
+  if (rc != SBH_OK) return rc;
+  if (prof && n > 0) prof_end(pr);
+  if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
+  return SBH_OK;
+}
+
+extern "C" int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
+                                     const SbhBuffer* masses, const SbhBuffer* cotangent,
+                                     SbhBuffer* forces, SbhBuffer* grad_points,
+                                     SbhBuffer* grad_masses, float beta, float softening_scale,
+                                     int32_t max_depth, int32_t leaf_size,
+                                     int32_t multipole_order, SbhBuffer* workspace,
+                                     SbhBuffer* stats) {
+  clear_error();
+  int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
+                        multipole_order);
+  if (rc != SBH_OK) return rc;
+  int64_t n = points->dims[0];
+  if (!buffer_ok(cotangent, SBH_DTYPE_F32, 2) || cotangent->dims[0] != n ||
+      cotangent->dims[1] != 3)
+    return fail_invalid("cotangent shape mismatch");
+  if (forces != nullptr &&
+      (!buffer_ok(forces, SBH_DTYPE_F32, 2) || forces->dims[0] != n || forces->dims[1] != 3))
+    return fail_invalid("forces shape mismatch");
+  if (!buffer_ok(grad_points, SBH_DTYPE_F32, 2) || grad_points->dims[0] != n ||
+      grad_points->dims[1] != 3)
+    return fail_invalid("grad_points shape mismatch");
+  if (!buffer_ok(grad_masses, SBH_DTYPE_F32, -1) || elem_count(grad_masses) != n)
+    return fail_invalid("grad_masses shape mismatch");
+  if ((rc = check_stats(stats)) != SBH_OK) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tree::Layout L = tree::layout(n, max_depth, leaf_size);
+  char* ws = nullptr;
+  if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  ProfRec pr;
+  bool prof = prof_begin(s, &pr, 1);
+  rc = tree::build(s, ws, L, static_cast<const float*>(points->data),
+                   static_cast<const float*>(masses->data),
+                   static_cast<const float*>(cotangent->data), n, max_depth, leaf_size, nullptr);
+  if (rc != SBH_OK) return rc;
+  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
+                  forces ? static_cast<float*>(forces->data) : nullptr,
+                  static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
+                  stats != nullptr, prof && n > 0 ? pr.ev + 2 : nullptr);
+  if (rc != SBH_OK) return rc;
+  if (prof && n > 0) prof_end(pr);
+  if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
+  return SBH_OK;
+}
+
+extern "C" void sbh_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> lock(g_prof_mu);
+  g_prof_on = on != 0;
+}
+
+// out[0..5] = {forward calls, forward build ms, forward walk-kernel ms,
+//              vjp calls, vjp build ms, vjp walk-kernel ms} summed over the
+// calls recorded since the last collect.  Waits for the recorded events.
+extern "C" int sbh_profile_collect(double* out, int32_t count) {
+  clear_error();
+  std::vector<ProfRec> recs;
+  {
+    std::lock_guard<std::mutex> lock(g_prof_mu);
+    recs.swap(g_prof);
+  }
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  int rc = SBH_OK;
+  for (auto& r : recs) {
+    float b = 0.f, w = 0.f;
+    if (hipEventSynchronize(r.ev[3]) != hipSuccess ||
+        hipEventElapsedTime(&b, r.ev[0], r.ev[1]) != hipSuccess ||
+        hipEventElapsedTime(&w, r.ev[2], r.ev[3]) != hipSuccess)
+      rc = fail_internal_msg("profile event query failed");
+    int o = r.vjp ? 3 : 0;
+    acc[o] += 1.0;
+    acc[o + 1] += b;
+    acc[o + 2] += w;
+    for (int k = 0; k < 4; ++k) (void)hipEventDestroy(r.ev[k]);
+  }
+  for (int i = 0; i < count && i < 6; ++i) out[i] = acc[i];
+  return rc;
+}

@@ -21,29 +21,6 @@ struct BuildGeom {
   float cx, cy, cz, half_size;
 };
 
-// src/sbh_ffi.cu:173-179, evaluated without contraction (IEEE fp32).
-__device__ __forceinline__ uint32_t quantize_axis(float v, float min_v, float span, int bits) {
-#pragma clang fp contract(off)
-  float t = (v - min_v) / span;
-  t = fminf(fmaxf(t, 0.0f), 0.999999f);
-  uint32_t max_bin = (1u << bits) - 1u;
-  return static_cast<uint32_t>(t * static_cast<float>(max_bin + 1u));
-}
-
-__device__ __forceinline__ uint32_t spread3(uint32_t v) {
-  // 10-bit value -> bits at positions 3i.
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000FFu;
-  v = (v | (v << 8)) & 0x0300F00Fu;
-  v = (v | (v << 4)) & 0x030C30C3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-
-__device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) {
-  return spread3(x) | (spread3(y) << 1) | (spread3(z) << 2);
-}
-
 __global__ __launch_bounds__(kB) void k_codes(const float* __restrict__ pts, int64_t n,
                                               BuildGeom g, int bits,
                                               uint32_t* __restrict__ codes,

@@ -1,0 +1,1132 @@
+// Adaptive Morton octree + wave-coherent group walk with full adjoint.
+//
+// Build (all on the device, no host synchronisation):
+//   k_bbox_*      data bounds (min - 1e-3, max + 1e-3, as python/jax_sbh_ffi.py:879-884)
+//   k_codes       D-level Morton codes with the reference quantiser (src/sbh_ffi.cu:162-179)
+//   radix sort    stable (code, index) order
+//   k_gather      sorted (x, y, z, m) and cotangent records
+//   per level:    k_split (children per node) -> k_scan_level -> k_emit (child records)
+//   per level:    k_moments bottom-up (mass, centre of mass, central second moment)
+// Walk (one wave = one group of 64 consecutive sorted points):
+//   a node is used as a multipole for the whole group iff the distance from its
+//   cell centre to the group's bounding box is >= beta * cell diagonal (the
+//   reference's MAC, src/sbh_ffi.cu:1878-1893, made conservative for a group);
+//   rejected leaves are summed exactly.  Lanes test 64 candidate nodes at once;
+//   accepted nodes and near leaves are batched in LDS and evaluated with all 64
+//   lanes sharing each node / source record, which is read once per wave
+//   through the scalar cache (the records are read-only during the walk).
+// Adjoint: query-side terms stay in registers; source-side terms of each
+//   interaction batch are summed across the wave with a transposed butterfly
+//   and added with one float atomic per value; node adjoints are pushed down to
+//   the points in k_finalize.
+#include "common.hpp"
+#include "tree.hpp"
+
+namespace sbh {
+namespace tree {
+namespace {
+
+constexpr int kB = 256;
+constexpr int kWavesPerBlock = kB / 64;
+constexpr int kStackSoft = 1024;
+constexpr int kStackCap = kStackSoft + 8 * 24;
+constexpr int kFarCap = 128;
+constexpr int kNearFlush = 16;
+constexpr int kNearCap = kNearFlush + 64;
+constexpr int kChunk = 8;  // nodes / sources evaluated per batch
+
+// Wave-uniform reads through the scalar data cache: the arrays are not
+// written while the walk runs, so they may be read as constant memory.
+typedef __attribute__((address_space(4))) const float cfloat;
+typedef __attribute__((address_space(4))) const int cint;
+
+__device__ __forceinline__ float4 sload4(const float4* p, int i) {
+  cfloat* q = (cfloat*)(p) + 4 * static_cast<int64_t>(i);  // generic -> constant space
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int4 sload4(const int4* p, int i) {
+  cint* q = (cint*)(p) + 4 * static_cast<int64_t>(i);
+  return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+// 1/sqrt(x) with the hardware instruction; x >= 1e-12 here, far from the
+// denormal range that rsqrtf's extra scaling handles.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// ------------------------------------------------------------------ bbox
+__global__ __launch_bounds__(kB) void k_bbox_partial(const float* __restrict__ pts, int64_t n,
+                                                     float* __restrict__ part) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    for (int a = 0; a < 3; ++a) {
+      float v = pts[3 * i + a];
+      lo[a] = fminf(lo[a], v);
+      hi[a] = fmaxf(hi[a], v);
+    }
+  }
+  __shared__ float sh[6][kB];
+  for (int a = 0; a < 3; ++a) {
+    sh[a][threadIdx.x] = lo[a];
+    sh[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int off = kB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) {
+      for (int a = 0; a < 3; ++a) {
+        sh[a][threadIdx.x] = fminf(sh[a][threadIdx.x], sh[a][threadIdx.x + off]);
+        sh[3 + a][threadIdx.x] = fmaxf(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + off]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// Reduces the partials, writes the cube geometry, initialises the root and
+// the zero-mass sentinel node used to pad interaction batches.
+__global__ __launch_bounds__(64) void k_bbox_final(const float* __restrict__ part, int nparts,
+                                                   int64_t n, Header* __restrict__ hdr,
+                                                   int4* __restrict__ topo,
+                                                   uint32_t* __restrict__ prefix,
+                                                   float4* __restrict__ geo,
+                                                   int32_t* __restrict__ parent,
+                                                   float4* __restrict__ mp,
+                                                   float4* __restrict__ qd, int sentinel,
+                                                   const float* __restrict__ user_bounds) {
+  if (threadIdx.x != 0) return;
+  float lo[3], hi[3];
+  if (user_bounds != nullptr) {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = user_bounds[a];
+      hi[a] = user_bounds[3 + a];
+    }
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = INFINITY;
+      hi[a] = -INFINITY;
+    }
+    for (int b = 0; b < nparts; ++b)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(lo[a], part[6 * b + a]);
+        hi[a] = fmaxf(hi[a], part[6 * b + 3 + a]);
+      }
+    if (n == 0)
+      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = lo[a] - 1e-3f;
+      hi[a] = hi[a] + 1e-3f;
+    }
+  }
+  CubeGeom g;
+  g.min_x = lo[0];
+  g.min_y = lo[1];
+  g.min_z = lo[2];
+  g.span = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+  g.cx = lo[0] + g.span * 0.5f;
+  g.cy = lo[1] + g.span * 0.5f;
+  g.cz = lo[2] + g.span * 0.5f;
+  g.half_size = g.span * 0.5f;
+  hdr->geom = g;
+  for (int l = 0; l < kMaxLevels; ++l) {
+    hdr->level_start[l] = 0;
+    hdr->level_count[l] = 0;
+  }
+  hdr->level_count[0] = 1;
+  hdr->num_nodes = 1;
+  hdr->overflow = 0;
+  for (int k = 0; k < 8; ++k) hdr->stats[k] = 0;
+  topo[0] = make_int4(0, static_cast<int>(n), -1, 0);
+  prefix[0] = 0;
+  parent[0] = -1;
+  geo[0] = cell_geometry(g, 0u, 0);
+  mp[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  qd[2 * sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  qd[2 * sentinel + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// ------------------------------------------------------------------ codes
+__global__ __launch_bounds__(kB) void k_codes(const float* __restrict__ pts, int64_t n,
+                                              const Header* __restrict__ hdr, int bits,
+                                              uint32_t* __restrict__ codes,
+                                              uint32_t* __restrict__ idx) {
+  int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (i >= n) return;
+  const CubeGeom g = hdr->geom;
+  uint32_t qx = quantize_axis(pts[3 * i + 0], g.min_x, g.span, bits);
+  uint32_t qy = quantize_axis(pts[3 * i + 1], g.min_y, g.span, bits);
+  uint32_t qz = quantize_axis(pts[3 * i + 2], g.min_z, g.span, bits);
+  codes[i] = morton3(qx, qy, qz);
+  idx[i] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
+                                               const float* __restrict__ ms,
+                                               const float* __restrict__ cot, int64_t n,
+                                               const uint32_t* __restrict__ perm,
+                                               float4* __restrict__ sp, float4* __restrict__ sc) {
+  int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (r >= n) return;
+  int64_t i = perm[r];
+  sp[r] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], ms[i]);
+  if (cot != nullptr) sc[r] = make_float4(cot[3 * i], cot[3 * i + 1], cot[3 * i + 2], 0.0f);
+}
+
+// ------------------------------------------------------------------ topology
+__device__ __forceinline__ int lower_bound_code(const uint32_t* __restrict__ codes, int lo,
+                                                int hi, uint32_t key) {
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (codes[mid] < key)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Octant boundaries of node k's range at level l+1: b[0..8].
+__device__ __forceinline__ void octant_bounds(const uint32_t* __restrict__ codes, int4 t,
+                                              uint32_t pre, int shift, int* b) {
+  int s = t.x, e = t.x + t.y;
+  b[0] = s;
+  b[8] = e;
+  for (int c = 1; c < 8; ++c) {
+    uint32_t key = ((pre << 3) | static_cast<uint32_t>(c)) << shift;
+    b[c] = lower_bound_code(codes, b[c - 1], e, key);
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_split(Header* __restrict__ hdr, int level, int max_depth,
+                                              int leaf_size, const uint32_t* __restrict__ codes,
+                                              int4* __restrict__ topo,
+                                              const uint32_t* __restrict__ prefix,
+                                              uint32_t* __restrict__ nch) {
+  const int start = hdr->level_start[level];
+  const int count = hdr->level_count[level];
+  const int shift = 3 * (max_depth - level - 1);
+  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
+    int k = start + u;
+    int4 t = topo[k];
+    int c = 0;
+    if (level < max_depth && t.y > leaf_size) {
+      int b[9];
+      octant_bounds(codes, t, prefix[k], shift, b);
+      for (int o = 0; o < 8; ++o) c += (b[o + 1] > b[o]) ? 1 : 0;
+    }
+    nch[u] = static_cast<uint32_t>(c);
+  }
+}
+
+// Single-block exclusive scan of the level's child counts; sets the next level.
+__global__ __launch_bounds__(1024) void k_scan_level(Header* __restrict__ hdr, int level,
+                                                     uint32_t* __restrict__ nch, int cap) {
+  __shared__ uint32_t sh[1024];
+  __shared__ uint32_t carry;
+  const int count = hdr->level_count[level];
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < count; base += 1024) {
+    int i = base + threadIdx.x;
+    uint32_t v = i < count ? nch[i] : 0u;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0u;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < count) nch[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int next_start = hdr->level_start[level] + count;
+    int total = static_cast<int>(carry);
+    if (next_start + total > cap) {
+      hdr->overflow = 1;
+      total = 0;
+    }
+    hdr->level_start[level + 1] = next_start;
+    hdr->level_count[level + 1] = total;
+    hdr->num_nodes = next_start + total;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_emit(Header* __restrict__ hdr, int level, int max_depth,
+                                             int leaf_size, const uint32_t* __restrict__ codes,
+                                             int4* __restrict__ topo, uint32_t* __restrict__ prefix,
+                                             float4* __restrict__ geo,
+                                             int32_t* __restrict__ parent,
+                                             const uint32_t* __restrict__ scan) {
+  const int start = hdr->level_start[level];
+  const int count = hdr->level_count[level];
+  const int next = hdr->level_start[level + 1];
+  const int next_count = hdr->level_count[level + 1];
+  const int shift = 3 * (max_depth - level - 1);
+  const CubeGeom g = hdr->geom;
+  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
+    int k = start + u;
+    int4 t = topo[k];
+    if (!(level < max_depth && t.y > leaf_size) || next_count == 0) {
+      t.z = -1;
+      t.w = 0;
+      topo[k] = t;
+      continue;
+    }
+    uint32_t pre = prefix[k];
+    int b[9];
+    octant_bounds(codes, t, pre, shift, b);
+    int first = next + static_cast<int>(scan[u]);
+    int r = 0;
+    for (int o = 0; o < 8; ++o) {
+      if (b[o + 1] <= b[o]) continue;
+      int ch = first + r;
+      uint32_t cp = (pre << 3) | static_cast<uint32_t>(o);
+      topo[ch] = make_int4(b[o], b[o + 1] - b[o], -1, 0);
+      prefix[ch] = cp;
+      parent[ch] = k;
+      geo[ch] = cell_geometry(g, cp, level + 1);
+      ++r;
+    }
+    t.z = first;
+    t.w = r;
+    topo[k] = t;
+  }
+}
+
+// ------------------------------------------------------------------ moments
+__global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, int level,
+                                                const int4* __restrict__ topo,
+                                                const float4* __restrict__ geo,
+                                                const float4* __restrict__ sp,
+                                                float4* __restrict__ mp, float4* __restrict__ qd) {
+  const int start = hdr->level_start[level];
+  const int count = hdr->level_count[level];
+  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
+    int k = start + u;
+    int4 t = topo[k];
+    float M = 0.f, X = 0.f, Y = 0.f, Z = 0.f;
+    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+    if (t.w == 0) {
+      for (int j = t.x; j < t.x + t.y; ++j) {
+        float4 p = sp[j];
+        M += p.w;
+        X += p.w * p.x;
+        Y += p.w * p.y;
+        Z += p.w * p.z;
+      }
+      if (M != 0.f) {
+        X /= M;
+        Y /= M;
+        Z /= M;
+        for (int j = t.x; j < t.x + t.y; ++j) {
+          float4 p = sp[j];
+          float dx = p.x - X, dy = p.y - Y, dz = p.z - Z;
+          sxx += p.w * dx * dx;
+          syy += p.w * dy * dy;
+          szz += p.w * dz * dz;
+          sxy += p.w * dx * dy;
+          sxz += p.w * dx * dz;
+          syz += p.w * dy * dz;
+        }
+      }
+    } else {
+      for (int c = t.z; c < t.z + t.w; ++c) {
+        float4 m = mp[c];
+        M += m.w;
+        X += m.w * m.x;
+        Y += m.w * m.y;
+        Z += m.w * m.z;
+      }
+      if (M != 0.f) {
+        X /= M;
+        Y /= M;
+        Z /= M;
+        for (int c = t.z; c < t.z + t.w; ++c) {
+          float4 m = mp[c];
+          float4 q0 = qd[2 * c], q1 = qd[2 * c + 1];
+          float dx = m.x - X, dy = m.y - Y, dz = m.z - Z;
+          sxx += q0.x + m.w * dx * dx;
+          syy += q0.y + m.w * dy * dy;
+          szz += q0.z + m.w * dz * dz;
+          sxy += q0.w + m.w * dx * dy;
+          sxz += q1.x + m.w * dx * dz;
+          syz += q1.y + m.w * dy * dz;
+        }
+      }
+    }
+    if (M == 0.f) {
+      float4 gg = geo[k];
+      X = gg.x;
+      Y = gg.y;
+      Z = gg.z;
+    }
+    mp[k] = make_float4(X, Y, Z, M);
+    qd[2 * k] = make_float4(sxx, syy, szz, sxy);
+    qd[2 * k + 1] = make_float4(sxz, syz, 0.f, 0.f);
+  }
+}
+
+// ------------------------------------------------------------------ query groups
+// Partition of the sorted points into contiguous groups of <= kGroupMax
+// points that stay inside one cell: for each node holding more than
+// kGroupMax points, runs of consecutive children holding <= kGroupMax points
+// each are packed greedily; an over-full leaf (depth limit) is cut into
+// chunks; a root holding <= kGroupMax points is the only group.
+template <bool EMIT>
+__device__ __forceinline__ int node_groups(const int4* __restrict__ topo, int k,
+                                           int2* __restrict__ out) {
+  const int gmax = kGroupMax;
+  const int4 t = topo[k];
+  int ng = 0;
+  if (k == 0 && t.y <= gmax) {
+    if (t.y > 0) {
+      if (EMIT) out[0] = make_int2(t.x, t.y);
+      ng = 1;
+    }
+    return ng;
+  }
+  if (t.y <= gmax) return 0;
+  if (t.w == 0) {
+    for (int s = t.x; s < t.x + t.y; s += gmax) {
+      if (EMIT) out[ng] = make_int2(s, min(gmax, t.x + t.y - s));
+      ++ng;
+    }
+    return ng;
+  }
+  int cs = 0, cc = 0;
+  for (int c = t.z; c < t.z + t.w; ++c) {
+    const int4 ct = topo[c];
+    if (ct.y > gmax) {
+      if (cc > 0) {
+        if (EMIT) out[ng] = make_int2(cs, cc);
+        ++ng;
+      }
+      cc = 0;
+      continue;
+    }
+    if (cc > 0 && cc + ct.y > gmax) {
+      if (EMIT) out[ng] = make_int2(cs, cc);
+      ++ng;
+      cc = 0;
+    }
+    if (cc == 0) cs = ct.x;
+    cc += ct.y;
+  }
+  if (cc > 0) {
+    if (EMIT) out[ng] = make_int2(cs, cc);
+    ++ng;
+  }
+  return ng;
+}
+
+__global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ hdr,
+                                                    const int4* __restrict__ topo,
+                                                    uint32_t* __restrict__ goff) {
+  const int nn = hdr->num_nodes;
+  for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
+    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, k, nullptr));
+}
+
+// Exclusive scan of the per-node group counts (one block); publishes the
+// group count and resets the walk's work queue.
+__global__ __launch_bounds__(1024) void k_group_scan(Header* __restrict__ hdr,
+                                                     uint32_t* __restrict__ goff) {
+  __shared__ uint32_t sh[1024];
+  __shared__ uint32_t carry;
+  const int count = hdr->num_nodes;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < count; base += 1024) {
+    int i = base + threadIdx.x;
+    uint32_t v = i < count ? goff[i] : 0u;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+      uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0u;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < count) goff[i] = carry + sh[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += sh[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    hdr->num_groups = static_cast<int>(carry);
+    hdr->queue = 0;
+  }
+}
+
+__global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hdr,
+                                                   const int4* __restrict__ topo,
+                                                   const uint32_t* __restrict__ goff,
+                                                   int2* __restrict__ groups) {
+  const int nn = hdr->num_nodes;
+  for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
+    node_groups<true>(topo, k, groups + goff[k]);
+}
+
+// ------------------------------------------------------------------ wave helpers
+__device__ __forceinline__ float wave_min(float v) {
+  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// One butterfly stage: lanes with bit `d` set keep the upper half of the
+// first 2h values and send the lower half to lane ^ d, and vice versa.
+template <int H, int N>
+__device__ __forceinline__ void butterfly(float (&v)[N], int lane, int d) {
+  const bool hi = lane & d;
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    float send = hi ? v[k] : v[H + k];
+    float keep = hi ? v[H + k] : v[k];
+    v[k] = keep + __shfl_xor(send, d, 64);
+  }
+}
+
+// Transposed reduction: v[0..31] per lane -> the 64-lane sum of v[lane >> 1]
+// (lanes 2j and 2j+1 both hold value j).
+__device__ __forceinline__ float reduce32(float (&v)[32], int lane) {
+  butterfly<16>(v, lane, 32);
+  butterfly<8>(v, lane, 16);
+  butterfly<4>(v, lane, 8);
+  butterfly<2>(v, lane, 4);
+  butterfly<1>(v, lane, 2);
+  return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// 64 values: returns the 64-lane sum of v[lane].
+__device__ __forceinline__ float reduce64(float (&v)[64], int lane) {
+  butterfly<32>(v, lane, 32);
+  butterfly<16>(v, lane, 16);
+  butterfly<8>(v, lane, 8);
+  butterfly<4>(v, lane, 4);
+  butterfly<2>(v, lane, 2);
+  butterfly<1>(v, lane, 1);
+  return v[0];
+}
+
+// Group MAC, fp32 without contraction (identical expression in the oracle).
+__device__ __forceinline__ bool group_accept(float lx, float ly, float lz, float hx, float hy,
+                                             float hz, float4 g, float beta) {
+#pragma clang fp contract(off)
+  float dx = fmaxf(fmaxf(lx - g.x, g.x - hx), 0.0f);
+  float dy = fmaxf(fmaxf(ly - g.y, g.y - hy), 0.0f);
+  float dz = fmaxf(fmaxf(lz - g.z, g.z - hz), 0.0f);
+  float d2 = dx * dx + dy * dy + dz * dz;
+  float t = beta * g.w;
+  return d2 >= t * t;
+}
+
+struct WalkArgs {
+  Header* hdr;
+  const int2* groups;
+  const int4* topo;
+  const float4* geo;
+  const float4* mp;
+  const float4* qd;
+  const float4* sp;
+  const float4* sc;
+  const uint32_t* perm;
+  int n;
+  int sentinel;  // zero-mass node used to pad far batches
+  float beta;
+  float e2;  // eps^2 + 1e-12
+  float* forces;
+  float4* qgrad;
+  float* sgrad;  // 4 floats per sorted point (atomics)
+  float* adj;    // 4 floats per node: dL/dX (3), dL/dM
+  float* adjq;   // 8 floats per node: dL/dS (6 used)
+  unsigned long long* stats;
+};
+
+struct Lane {
+  float x, y, z;     // query position
+  float cx, cy, cz;  // cotangent
+};
+
+// Far-field batch of kChunk nodes (sentinel-padded): forces, query-side
+// gradient and the node adjoint partials.
+template <bool VJP, bool QUAD>
+__device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, const int* far,
+                                          int lane, float& fx, float& fy, float& fz, float& gx,
+                                          float& gy, float& gz) {
+  int idx[kChunk];
+#pragma unroll
+  for (int t = 0; t < kChunk; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
+  float part[32];
+  float qpart[64];
+#pragma unroll
+  for (int t = 0; t < kChunk; ++t) {
+    const float4 m = sload4(A.mp, idx[t]);
+    const float Dx = L.x - m.x, Dy = L.y - m.y, Dz = L.z - m.z;
+    const float r2 = Dx * Dx + Dy * Dy + Dz * Dz;
+    const float s = rsq(r2 + A.e2);
+    const float s2 = s * s, s3 = s2 * s;
+    if constexpr (!QUAD) {
+      const float a = -m.w * s3;
+      fx += a * Dx;
+      fy += a * Dy;
+      fz += a * Dz;
+      if constexpr (VJP) {
+        const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
+        const float kD = -3.0f * a * s2 * u;  // 3 M s^5 u
+        const float dDx = kD * Dx + a * L.cx;
+        const float dDy = kD * Dy + a * L.cy;
+        const float dDz = kD * Dz + a * L.cz;
+        gx += dDx;
+        gy += dDy;
+        gz += dDz;
+        part[4 * t + 0] = -dDx;
+        part[4 * t + 1] = -dDy;
+        part[4 * t + 2] = -dDz;
+        part[4 * t + 3] = -s3 * u;
+      }
+    } else {
+      const float4 q0 = sload4(A.qd, 2 * idx[t]);
+      const float4 q1 = sload4(A.qd, 2 * idx[t] + 1);
+      const float s5 = s3 * s2, s7 = s5 * s2;
+      // second moment (xx, yy, zz, xy), (xz, yz)
+      const float SDx = q0.x * Dx + q0.w * Dy + q1.x * Dz;
+      const float SDy = q0.w * Dx + q0.y * Dy + q1.y * Dz;
+      const float SDz = q1.x * Dx + q1.y * Dy + q0.z * Dz;
+      const float tq = Dx * SDx + Dy * SDy + Dz * SDz;
+      const float T = q0.x + q0.y + q0.z;
+      const float a = -m.w * s3 + 1.5f * T * s5 - 7.5f * s7 * tq;
+      const float b = 3.0f * s5;
+      fx += a * Dx + b * SDx;
+      fy += a * Dy + b * SDy;
+      fz += a * Dz + b * SDz;
+      if constexpr (VJP) {
+        const float s9 = s7 * s2;
+        const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
+        const float Scx = q0.x * L.cx + q0.w * L.cy + q1.x * L.cz;
+        const float Scy = q0.w * L.cx + q0.y * L.cy + q1.y * L.cz;
+        const float Scz = q1.x * L.cx + q1.y * L.cy + q0.z * L.cz;
+        const float cw = L.cx * SDx + L.cy * SDy + L.cz * SDz;
+        const float kD = 3.0f * m.w * s5 * u - 15.0f * s7 * cw - 7.5f * T * s7 * u +
+                         52.5f * s9 * tq * u;
+        const float kc = a;  // -M s^3 + 1.5 T s^5 - 7.5 s^7 t
+        const float b15 = -15.0f * s7 * u;
+        const float dDx = kD * Dx + kc * L.cx + b * Scx + b15 * SDx;
+        const float dDy = kD * Dy + kc * L.cy + b * Scy + b15 * SDy;
+        const float dDz = kD * Dz + kc * L.cz + b * Scz + b15 * SDz;
+        gx += dDx;
+        gy += dDy;
+        gz += dDz;
+        part[4 * t + 0] = -dDx;
+        part[4 * t + 1] = -dDy;
+        part[4 * t + 2] = -dDz;
+        part[4 * t + 3] = -s3 * u;
+        // dL/dS (symmetric), (xx, yy, zz, xy, xz, yz)
+        const float h5 = 1.5f * s5, h7 = -7.5f * s7 * u, hu = h5 * u;
+        qpart[8 * t + 0] = 2.0f * h5 * L.cx * Dx + h7 * Dx * Dx + hu;
+        qpart[8 * t + 1] = 2.0f * h5 * L.cy * Dy + h7 * Dy * Dy + hu;
+        qpart[8 * t + 2] = 2.0f * h5 * L.cz * Dz + h7 * Dz * Dz + hu;
+        qpart[8 * t + 3] = h5 * (L.cx * Dy + L.cy * Dx) + h7 * Dx * Dy;
+        qpart[8 * t + 4] = h5 * (L.cx * Dz + L.cz * Dx) + h7 * Dx * Dz;
+        qpart[8 * t + 5] = h5 * (L.cy * Dz + L.cz * Dy) + h7 * Dy * Dz;
+        qpart[8 * t + 6] = 0.f;
+        qpart[8 * t + 7] = 0.f;
+      }
+    }
+  }
+  if constexpr (VJP) {
+    const float v = reduce32(part, lane);
+    const int j = lane >> 1;  // value j: node j >> 2, component j & 3
+    const int node = far[j >> 2];
+    if (!(lane & 1) && node != A.sentinel) atomicAdd(&A.adj[4 * node + (j & 3)], v);
+    if constexpr (QUAD) {
+      const float vq = reduce64(qpart, lane);
+      const int nq = far[lane >> 3];
+      const int cq = lane & 7;
+      if (cq < 6 && nq != A.sentinel) atomicAdd(&A.adjq[8 * nq + cq], vq);
+    }
+  }
+}
+
+// Near-field batch: kChunk consecutive sources of one leaf starting at j0
+// (entries at or beyond s1 carry zero weight).
+template <bool VJP>
+__device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
+                                           int s1, int lane, float& fx, float& fy, float& fz,
+                                           float& gx, float& gy, float& gz) {
+  float part[32];
+#pragma unroll
+  for (int u = 0; u < kChunk; ++u) {
+    const int j = j0 + u;
+    const float4 src = sload4(A.sp, min(j, s1 - 1));
+    const float w = (j < s1) ? src.w : 0.0f;
+    const float dx = L.x - src.x, dy = L.y - src.y, dz = L.z - src.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float s = rsq(r2 + A.e2);
+    float s3 = s * s * s;
+    s3 = (j == i) ? 0.0f : s3;  // the self pair contributes nothing
+    const float a = -w * s3;
+    fx += a * dx;
+    fy += a * dy;
+    fz += a * dz;
+    if constexpr (VJP) {
+      const float cd = L.cx * dx + L.cy * dy + L.cz * dz;
+      const float k1 = -3.0f * a * s * s * cd;  // 3 w s^5 (c.d)
+      const float vx = k1 * dx + a * L.cx;
+      const float vy = k1 * dy + a * L.cy;
+      const float vz = k1 * dz + a * L.cz;
+      gx += vx;
+      gy += vy;
+      gz += vz;
+      part[4 * u + 0] = -vx;
+      part[4 * u + 1] = -vy;
+      part[4 * u + 2] = -vz;
+      part[4 * u + 3] = -s3 * cd;
+    }
+  }
+  if constexpr (VJP) {
+    const float v = reduce32(part, lane);
+    const int jj = lane >> 1;
+    const int src = j0 + (jj >> 2);
+    if (!(lane & 1) && src < s1) atomicAdd(&A.sgrad[4 * src + (jj & 3)], v);
+  }
+}
+
+// One query group (sorted points [base, base + cnt)) processed by one wave.
+template <bool VJP, bool QUAD>
+__device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt, int lane,
+                                           int* stack, int* far, int* near,
+                                           unsigned long long& n_far,
+                                           unsigned long long& n_near) {
+  const int i = base + lane;
+  const bool valid = lane < cnt;
+  Lane L;
+  {
+    const float4 me = A.sp[valid ? i : base];
+    L.x = me.x;
+    L.y = me.y;
+    L.z = me.z;
+    L.cx = L.cy = L.cz = 0.f;
+    if constexpr (VJP) {
+      if (valid) {
+        const float4 c = A.sc[i];
+        L.cx = c.x;
+        L.cy = c.y;
+        L.cz = c.z;
+      }
+    }
+  }
+  const float lx = wave_min(L.x), ly = wave_min(L.y), lz = wave_min(L.z);
+  const float hx = wave_max(L.x), hy = wave_max(L.y), hz = wave_max(L.z);
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const float beta = A.beta;
+  const int nvalid = cnt;
+
+  float fx = 0.f, fy = 0.f, fz = 0.f;
+  float gx = 0.f, gy = 0.f, gz = 0.f;
+
+  int top = 1, nfar = 0, nnear = 0;
+  if (lane == 0) stack[0] = 0;
+  __builtin_amdgcn_wave_barrier();
+
+  auto flush_far = [&]() {
+    n_far += static_cast<unsigned long long>(nfar) * nvalid;
+    const int padded = (nfar + kChunk - 1) & ~(kChunk - 1);
+    if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
+    __builtin_amdgcn_wave_barrier();
+    for (int f0 = 0; f0 < padded; f0 += kChunk)
+      far_batch<VJP, QUAD>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
+    __builtin_amdgcn_wave_barrier();
+    nfar = 0;
+  };
+
+  auto flush_near = [&]() {
+    for (int b = 0; b < nnear; ++b) {
+      const int leaf = __builtin_amdgcn_readfirstlane(near[b]);
+      const int4 t = sload4(A.topo, leaf);
+      const int s0 = t.x, s1 = t.x + t.y;
+      // self pairs are skipped
+      n_near += static_cast<unsigned long long>(t.y) * nvalid -
+                static_cast<unsigned long long>(max(0, min(s1, base + nvalid) - max(s0, base)));
+      for (int j0 = s0; j0 < s1; j0 += kChunk)
+        near_batch<VJP>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+    }
+    nnear = 0;
+  };
+
+  // Every node is pushed at most once per group and each iteration pops at
+  // least one, so num_nodes iterations bound the loop even for a corrupt tree.
+  const int num_nodes = A.hdr->num_nodes;
+  int iters = 0;
+  while (top > 0 && iters++ <= num_nodes) {
+    int k = min(64, top);
+    k = min(k, max(1, (kStackSoft - top) / 7));
+    const int from = top - k;
+    const int node = lane < k ? stack[from + lane] : -1;
+    top = from;
+    __builtin_amdgcn_wave_barrier();
+    const bool have = node >= 0 && node < num_nodes;
+    const int nd = have ? node : 0;
+    const float4 m = A.mp[nd];
+    const float4 g = A.geo[nd];
+    const int4 t = A.topo[nd];
+    int kind = 0, nch = 0, fch = 0;
+    if (have && m.w != 0.0f) {
+      if (group_accept(lx, ly, lz, hx, hy, hz, g, beta)) {
+        kind = 1;
+      } else if (t.w == 0) {
+        kind = 2;
+      } else {
+        kind = 3;
+        nch = t.w;
+        fch = t.z;
+      }
+    }
+    const uint64_t bf = __ballot(kind == 1);
+    if (kind == 1) far[nfar + __popcll(bf & lt_mask)] = node;
+    nfar += __popcll(bf);
+    const uint64_t bn = __ballot(kind == 2);
+    if (kind == 2) near[nnear + __popcll(bn & lt_mask)] = node;
+    nnear += __popcll(bn);
+    const int incl = wave_incl_scan(nch, lane);
+    const int total = __shfl(incl, 63, 64);
+    if (top + total > kStackCap) break;  // unreachable for depth <= 10 (see kStackCap)
+    for (int c = 0; c < nch; ++c) stack[top + incl - nch + c] = fch + c;
+    top += total;
+    __builtin_amdgcn_wave_barrier();
+    if (nfar >= 64) flush_far();
+    if (nnear >= kNearFlush) flush_near();
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (nfar > 0) flush_far();
+  if (nnear > 0) flush_near();
+
+  if (valid) {
+    const int64_t o = static_cast<int64_t>(A.perm[i]);
+    if (A.forces != nullptr) {
+      A.forces[3 * o + 0] = fx;
+      A.forces[3 * o + 1] = fy;
+      A.forces[3 * o + 2] = fz;
+    }
+    if constexpr (VJP) A.qgrad[i] = make_float4(gx, gy, gz, 0.f);
+  }
+}
+
+// Persistent walk: each wave takes query groups from a device work queue
+// until none are left (heavy groups do not hold up a static schedule).
+template <bool VJP, bool QUAD>
+__global__ __launch_bounds__(kB) void k_walk(WalkArgs A) {
+  __shared__ int s_stack[kWavesPerBlock][kStackCap];
+  __shared__ int s_far[kWavesPerBlock][kFarCap];
+  __shared__ int s_near[kWavesPerBlock][kNearCap];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int num_groups = A.hdr->num_groups;
+  unsigned long long n_far = 0, n_near = 0, n_groups = 0;
+  while (true) {
+    int g = 0;
+    if (lane == 0) g = atomicAdd(&A.hdr->queue, 1);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= num_groups) break;  // every wave reaches this exit
+    const int2 grp = A.groups[g];
+    const int base = __builtin_amdgcn_readfirstlane(grp.x);
+    const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
+    if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.n) continue;  // corrupt entry
+    walk_group<VJP, QUAD>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far, n_near);
+    ++n_groups;
+  }
+  if (A.stats != nullptr && lane == 0) {
+    atomicAdd(&A.stats[0], n_far);
+    atomicAdd(&A.stats[1], n_near);
+    atomicAdd(&A.stats[2], n_groups);
+  }
+}
+
+// Zero the adjoint accumulators (fill kernel; graph-capture friendly).
+__global__ __launch_bounds__(kB) void k_zero_adjoint(const Header* __restrict__ hdr, int64_t n,
+                                                     float* __restrict__ sgrad,
+                                                     float* __restrict__ adj,
+                                                     float* __restrict__ adjq) {
+  const int64_t nn = hdr->num_nodes;
+  const int64_t total = 4 * n + 4 * nn + 8 * nn;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < total;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    if (i < 4 * n)
+      sgrad[i] = 0.f;
+    else if (i < 4 * n + 4 * nn)
+      adj[i - 4 * n] = 0.f;
+    else
+      adjq[i - 4 * n - 4 * nn] = 0.f;
+  }
+}
+
+// Push node adjoints down to the points and write gradients in input order.
+template <bool QUAD>
+__global__ __launch_bounds__(kB) void k_finalize(const Header* __restrict__ hdr, int64_t n,
+                                                 const int4* __restrict__ topo,
+                                                 const float4* __restrict__ mp,
+                                                 const float4* __restrict__ sp,
+                                                 const uint32_t* __restrict__ perm,
+                                                 const float4* __restrict__ qgrad,
+                                                 const float* __restrict__ sgrad,
+                                                 const float* __restrict__ adj,
+                                                 const float* __restrict__ adjq,
+                                                 float* __restrict__ gx_out,
+                                                 float* __restrict__ gm_out) {
+  int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (r >= n) return;
+  const float4 p = sp[r];
+  const float4 q = qgrad[r];
+  float ax = q.x + sgrad[4 * r + 0];
+  float ay = q.y + sgrad[4 * r + 1];
+  float az = q.z + sgrad[4 * r + 2];
+  float am = sgrad[4 * r + 3];
+  int node = 0;
+  const int rr = static_cast<int>(r);
+  for (int depth = 0; depth < kMaxLevels; ++depth) {
+    const float4 m = mp[node];
+    if (m.w != 0.0f) {
+      const float Ax = adj[4 * node], Ay = adj[4 * node + 1], Az = adj[4 * node + 2];
+      const float B = adj[4 * node + 3];
+      const float dx = p.x - m.x, dy = p.y - m.y, dz = p.z - m.z;
+      const float wm = p.w / m.w;
+      ax += wm * Ax;
+      ay += wm * Ay;
+      az += wm * Az;
+      am += B + (Ax * dx + Ay * dy + Az * dz) / m.w;
+      if constexpr (QUAD) {
+        const float* G = adjq + 8 * node;  // xx yy zz xy xz yz
+        const float Gdx = G[0] * dx + G[3] * dy + G[4] * dz;
+        const float Gdy = G[3] * dx + G[1] * dy + G[5] * dz;
+        const float Gdz = G[4] * dx + G[5] * dy + G[2] * dz;
+        ax += 2.0f * p.w * Gdx;
+        ay += 2.0f * p.w * Gdy;
+        az += 2.0f * p.w * Gdz;
+        am += dx * Gdx + dy * Gdy + dz * Gdz;
+      }
+    }
+    const int4 t = topo[node];
+    if (t.w == 0) break;
+    int ch = t.z;
+    for (int c = 1; c < t.w; ++c)
+      if (topo[t.z + c].x <= rr) ch = t.z + c;
+    node = ch;
+  }
+  const int64_t o = perm[r];
+  gx_out[3 * o + 0] = ax;
+  gx_out[3 * o + 1] = ay;
+  gx_out[3 * o + 2] = az;
+  gm_out[o] = am;
+}
+
+__global__ void k_copy_stats(const Header* __restrict__ hdr, int64_t* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  out[0] = static_cast<int64_t>(hdr->stats[0]);
+  out[1] = static_cast<int64_t>(hdr->stats[1]);
+  out[2] = static_cast<int64_t>(hdr->stats[2]);
+  out[3] = hdr->num_nodes;
+  int nl = 0;
+  for (int l = 0; l < kMaxLevels; ++l)
+    if (hdr->level_count[l] > 0) nl = l + 1;
+  out[4] = nl;
+  out[5] = hdr->overflow;
+  out[6] = 0;
+  out[7] = 0;
+}
+
+}  // namespace
+
+int64_t node_capacity(int64_t n, int max_depth, int leaf_size) {
+  int64_t cap = 1;
+  int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+  for (int l = 1; l <= max_depth; ++l) {
+    int64_t a = (l < 21) ? (int64_t{1} << (3 * l)) : INT64_MAX;
+    int64_t b = 8 * internal_max;
+    int64_t c = n;
+    cap += std::min(a, std::min(b, c));
+  }
+  return cap;
+}
+
+Layout layout(int64_t n, int max_depth, int leaf_size) {
+  Layout L;
+  auto align = [](size_t x) { return (x + 255) & ~size_t{255}; };
+  size_t nn = static_cast<size_t>(n > 0 ? n : 1);
+  L.cap = node_capacity(n, max_depth, leaf_size);
+  size_t cap = static_cast<size_t>(L.cap) + 1;  // + the sentinel node
+  size_t o = 0;
+  L.header = o; o = align(o + sizeof(Header));
+  L.codes = o; o = align(o + 4 * nn);
+  L.perm = o; o = align(o + 4 * nn);
+  L.keys_tmp = o; o = align(o + 4 * nn);
+  L.vals_tmp = o; o = align(o + 4 * nn);
+  L.sort_tmp = o; o = align(o + radix_sort_temp_bytes(n));
+  L.bbox_tmp = o; o = align(o + 6 * 4 * kBboxBlocks);
+  L.sp = o; o = align(o + 16 * nn);
+  L.sc = o; o = align(o + 16 * nn);
+  L.topo = o; o = align(o + 16 * cap);
+  L.prefix = o; o = align(o + 4 * cap);
+  L.parent = o; o = align(o + 4 * cap);
+  L.geo = o; o = align(o + 16 * cap);
+  L.mp = o; o = align(o + 16 * cap);
+  L.qd = o; o = align(o + 32 * cap);
+  L.nch = o; o = align(o + 4 * cap);
+  L.adj = o; o = align(o + 16 * cap);
+  L.adjq = o; o = align(o + 32 * cap);
+  L.qgrad = o; o = align(o + 16 * nn);
+  L.sgrad = o; o = align(o + 16 * nn);
+  L.groups = o; o = align(o + 8 * nn);
+  L.goff = o; o = align(o + 4 * cap);
+  L.total = o;
+  return L;
+}
+
+int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+          const float* cot, int64_t n, int max_depth, int leaf_size,
+          const float* user_bounds) {
+  auto* hdr = reinterpret_cast<Header*>(ws + L.header);
+  auto* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
+  auto* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
+  auto* keys = reinterpret_cast<uint32_t*>(ws + L.keys_tmp);
+  auto* vals = reinterpret_cast<uint32_t*>(ws + L.vals_tmp);
+  auto* topo = reinterpret_cast<int4*>(ws + L.topo);
+  auto* prefix = reinterpret_cast<uint32_t*>(ws + L.prefix);
+  auto* parent = reinterpret_cast<int32_t*>(ws + L.parent);
+  auto* geo = reinterpret_cast<float4*>(ws + L.geo);
+  auto* mp = reinterpret_cast<float4*>(ws + L.mp);
+  auto* qd = reinterpret_cast<float4*>(ws + L.qd);
+  auto* nch = reinterpret_cast<uint32_t*>(ws + L.nch);
+  auto* sp = reinterpret_cast<float4*>(ws + L.sp);
+  auto* sc = reinterpret_cast<float4*>(ws + L.sc);
+  auto* bbox = reinterpret_cast<float*>(ws + L.bbox_tmp);
+
+  unsigned nb = static_cast<unsigned>(
+      std::min<int64_t>(kBboxBlocks, std::max<int64_t>(1, (n + kB - 1) / kB)));
+  if (user_bounds == nullptr) {
+    k_bbox_partial<<<nb, kB, 0, s>>>(pts, n, bbox);
+    SBH_LAUNCH_CHECK();
+  }
+  k_bbox_final<<<1, 64, 0, s>>>(bbox, static_cast<int>(nb), n, hdr, topo, prefix, geo, parent,
+                                mp, qd, static_cast<int>(L.cap), user_bounds);
+  SBH_LAUNCH_CHECK();
+  if (n > 0) {
+    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, hdr, max_depth, keys, vals);
+    SBH_LAUNCH_CHECK();
+    int rc = radix_sort_pairs(s, keys, vals, codes, perm, n, 3 * max_depth, ws + L.sort_tmp);
+    if (rc != SBH_OK) return rc;
+    k_gather<<<grid_for(n, kB), kB, 0, s>>>(pts, ms, cot, n, perm, sp, sc);
+    SBH_LAUNCH_CHECK();
+  }
+  int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+  for (int l = 0; l < max_depth; ++l) {
+    int64_t lvl_cap = std::min<int64_t>(
+        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
+        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
+    unsigned g = grid_for(lvl_cap, kB, 4096);
+    k_split<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, nch);
+    SBH_LAUNCH_CHECK();
+    k_scan_level<<<1, 1024, 0, s>>>(hdr, l, nch, static_cast<int>(L.cap));
+    SBH_LAUNCH_CHECK();
+    k_emit<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, geo, parent, nch);
+    SBH_LAUNCH_CHECK();
+  }
+  for (int l = max_depth; l >= 0; --l) {
+    int64_t lvl_cap = std::min<int64_t>(
+        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
+        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
+    k_moments<<<grid_for(lvl_cap, kB, 4096), kB, 0, s>>>(hdr, l, topo, geo, sp, mp, qd);
+    SBH_LAUNCH_CHECK();
+  }
+  auto* goff = reinterpret_cast<uint32_t*>(ws + L.goff);
+  auto* groups = reinterpret_cast<int2*>(ws + L.groups);
+  unsigned gg = grid_for(L.cap, kB, 4096);
+  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, goff);
+  SBH_LAUNCH_CHECK();
+  k_group_scan<<<1, 1024, 0, s>>>(hdr, goff);
+  SBH_LAUNCH_CHECK();
+  k_group_emit<<<gg, kB, 0, s>>>(hdr, topo, goff, groups);
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
+         bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
+         bool want_stats, hipEvent_t* ev_walk) {
+  if (n == 0) return SBH_OK;
+  auto* hdr = reinterpret_cast<Header*>(ws + L.header);
+  WalkArgs A;
+  A.hdr = hdr;
+  A.groups = reinterpret_cast<const int2*>(ws + L.groups);
+  A.topo = reinterpret_cast<const int4*>(ws + L.topo);
+  A.geo = reinterpret_cast<const float4*>(ws + L.geo);
+  A.mp = reinterpret_cast<const float4*>(ws + L.mp);
+  A.qd = reinterpret_cast<const float4*>(ws + L.qd);
+  A.sp = reinterpret_cast<const float4*>(ws + L.sp);
+  A.sc = reinterpret_cast<const float4*>(ws + L.sc);
+  A.perm = reinterpret_cast<const uint32_t*>(ws + L.perm);
+  A.n = static_cast<int>(n);
+  A.sentinel = static_cast<int>(L.cap);
+  A.beta = beta;
+  float eps = fmaxf(softening, 0.0f);
+  A.e2 = eps * eps + 1e-12f;
+  A.forces = forces;
+  A.qgrad = reinterpret_cast<float4*>(ws + L.qgrad);
+  A.sgrad = reinterpret_cast<float*>(ws + L.sgrad);
+  A.adj = reinterpret_cast<float*>(ws + L.adj);
+  A.adjq = reinterpret_cast<float*>(ws + L.adjq);
+  A.stats = want_stats ? hdr->stats : nullptr;
+  if (vjp) {
+    k_zero_adjoint<<<grid_for(4 * n + 12 * L.cap, kB, 8192), kB, 0, s>>>(hdr, n, A.sgrad, A.adj,
+                                                                        A.adjq);
+    SBH_LAUNCH_CHECK();
+  }
+  // Persistent waves drain the group queue: enough blocks to fill every CU
+  // several times over, never more than one wave per point.
+  unsigned grid = grid_for(n, 64 * kWavesPerBlock, 256 * 8);
+  if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[0], s));
+  if (vjp && quad)
+    k_walk<true, true><<<grid, kB, 0, s>>>(A);
+  else if (vjp)
+    k_walk<true, false><<<grid, kB, 0, s>>>(A);
+  else if (quad)
+    k_walk<false, true><<<grid, kB, 0, s>>>(A);
+  else
+    k_walk<false, false><<<grid, kB, 0, s>>>(A);
+  SBH_LAUNCH_CHECK();
+  if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[1], s));
+  if (vjp) {
+    if (quad)
+      k_finalize<true><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.topo, A.mp, A.sp, A.perm,
+                                                       A.qgrad, A.sgrad, A.adj, A.adjq,
+                                                       grad_points, grad_masses);
+    else
+      k_finalize<false><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.topo, A.mp, A.sp, A.perm,
+                                                        A.qgrad, A.sgrad, A.adj, A.adjq,
+                                                        grad_points, grad_masses);
+    SBH_LAUNCH_CHECK();
+  }
+  return SBH_OK;
+}
+
+int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out) {
+  k_copy_stats<<<1, 64, 0, s>>>(reinterpret_cast<const Header*>(ws + L.header), out);
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+}  // namespace tree
+}  // namespace sbh

@@ -1,0 +1,88 @@
+// Adaptive Morton octree: device-side header, workspace layout and the build
+// / walk drivers shared by the C-ABI entry points.
+#pragma once
+
+#include <algorithm>
+
+#include "common.hpp"
+
+namespace sbh {
+namespace tree {
+
+constexpr int kMaxLevels = 24;
+constexpr int kBboxBlocks = 1024;
+
+struct CubeGeom {
+  float min_x, min_y, min_z, span;
+  float cx, cy, cz, half_size;
+};
+
+// Lives at offset 0 of the workspace; written only by device kernels.
+struct Header {
+  CubeGeom geom;
+  int32_t level_start[kMaxLevels];
+  int32_t level_count[kMaxLevels];
+  int32_t num_nodes;
+  int32_t overflow;
+  int32_t num_groups;  // query groups of the walk
+  int32_t queue;       // next group to hand out (walk work queue)
+  unsigned long long stats[8];  // far interactions, near pairs, groups
+};
+
+constexpr int kGroupMax = 64;  // one wavefront per query group
+
+// Byte offsets of every array inside the workspace.  Node arrays hold `cap`
+// entries (an upper bound on the node count, see node_capacity()).
+struct Layout {
+  int64_t cap;
+  size_t header, codes, perm, keys_tmp, vals_tmp, sort_tmp, bbox_tmp;
+  size_t sp, sc;                       // sorted (x,y,z,m) and cotangent records
+  size_t topo, prefix, parent, geo;    // int4 {start,count,first_child,nchild}, cell prefix
+  size_t mp, qd, nch;                  // float4 {com, mass}, 2 x float4 second moment
+  size_t adj, adjq, qgrad, sgrad;      // adjoint accumulators
+  size_t groups, goff;                 // int2 {start, count} per group; per-node offsets
+  size_t total;
+};
+
+int64_t node_capacity(int64_t n, int max_depth, int leaf_size);
+Layout layout(int64_t n, int max_depth, int leaf_size);
+
+// Build topology and moments.  `cot` may be null (forces only).  `user_bounds`
+// (device, 6 floats: min xyz, max xyz) may be null to use the data bounds
+// expanded by 1e-3.
+int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+          const float* cot, int64_t n, int max_depth, int leaf_size,
+          const float* user_bounds);
+
+// Walk; forces (input order, N x 3) may be null; with vjp, writes grad_points
+// (N x 3) and grad_masses (N) in input order.
+// ev_walk (may be null): two events recorded immediately before and after the
+// walk kernel on stream s.
+int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
+         bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
+         bool want_stats, hipEvent_t* ev_walk = nullptr);
+
+// Copies {far, near, groups, nodes, levels, overflow, 0, 0} into a device int64[8].
+int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out);
+
+// Cell geometry from a level-l prefix (same expression as the octree8 node
+// centres, src/sbh_ffi.cu:1828-1834): {cx, cy, cz, diagonal}.
+__device__ __forceinline__ float4 cell_geometry(const CubeGeom& g, uint32_t prefix, int level) {
+#pragma clang fp contract(off)
+  int ix = 0, iy = 0, iz = 0;
+  for (int b = 0; b < level; ++b) {
+    ix |= static_cast<int>((prefix >> (3 * b + 0)) & 1u) << b;
+    iy |= static_cast<int>((prefix >> (3 * b + 1)) & 1u) << b;
+    iz |= static_cast<int>((prefix >> (3 * b + 2)) & 1u) << b;
+  }
+  float step = g.half_size * 2.0f / static_cast<float>(1 << level);
+  float4 r;
+  r.x = fmaf(static_cast<float>(ix) + 0.5f, step, g.cx - g.half_size);
+  r.y = fmaf(static_cast<float>(iy) + 0.5f, step, g.cy - g.half_size);
+  r.z = fmaf(static_cast<float>(iz) + 0.5f, step, g.cz - g.half_size);
+  r.w = step * 1.7320508075688772f;
+  return r;
+}
+
+}  // namespace tree
+}  // namespace sbh

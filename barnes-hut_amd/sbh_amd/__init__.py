@@ -8,3 +8,6 @@ from . import ffi  # noqa: F401
 from ._lib import library_path, load_library  # noqa: F401
 
 __all__ = ["ffi", "library_path", "load_library"]
+from . import gravity  # noqa: E402,F401
+
+__all__.append("gravity")

@@ -22,7 +22,7 @@ SBH_OK = 0
 SBH_INVALID_ARGUMENT = 3
 SBH_INTERNAL = 13
 
-_DTYPE_CODES = {torch.uint8: 6, torch.int32: 8, torch.float32: 11}
+_DTYPE_CODES = {torch.uint8: 6, torch.int32: 4, torch.int64: 5, torch.float32: 11}
 
 
 class SbhBuffer(ctypes.Structure):

@@ -1,0 +1,261 @@
+#!/usr/bin/env python3
+"""Headline benchmark: Barnes-Hut forces/s at N = 2^20, theta = 0.5, fwd + VJP.
+
+Workload (the metric's configuration): N = 1,048,576 fp32 points from a
+Plummer sphere (scale radius 1, truncated at r = 20), masses 2^-20, Plummer
+softening 0.005, theta = 0.5 (beta = 1/theta = 2 on the cell diagonal),
+octree max depth 10, leaf size 32, quadrupole moments.
+
+One step = what a `jax.grad`-style training step does through the recompute
+custom VJP: forward (device bounds, Morton codes, radix sort, tree build,
+moments, walk) and backward (rebuild + one adjoint walk returning position and
+mass gradients).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below).  The CPU baseline is the oracle's
+restatement of the same walk, timed on a bounded sample of query groups on
+one host core and extrapolated to the full workload.
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "barnes-hut_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "BH forces/sec at N=1M θ=0.5 (rel_l2≤1e-3), fwd+VJP; batched vs per-set"
+FP32_PEAK_TFLOPS = 157.3
+# Algorithmic flops per lane interaction (one query x one node or source).
+FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
+FLOPS_FAR_FWD = {1: 20, 2: 54}
+FLOPS_FAR_VJP = {1: 48, 2: 200}
+
+
+def plummer(n, seed):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    xmax = 20.0 ** 3 / (20.0 ** 2 + 1.0) ** 1.5
+    X = rng.uniform(0, xmax, n)
+    r = 1.0 / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return (r[:, None] * v).astype(np.float32)
+
+
+def direct_fp64(P, M, Q, eps, chunk=16):
+    import torch
+
+    out = []
+    e2 = eps * eps + 1e-12
+    for q in Q.split(chunk):
+        d = q[:, None, :] - P[None]
+        s3 = ((d * d).sum(-1) + e2) ** -1.5
+        out.append(-((M[None] * s3)[..., None] * d).sum(1))
+    return torch.cat(out)
+
+
+def rel_l2(a, b):
+    import torch
+
+    a = a.double()
+    b = b.double()
+    return float(torch.linalg.norm(a - b) / torch.clamp(torch.linalg.norm(b), min=1e-30))
+
+
+def cpu_baseline(p, m, c, args, sample_groups):
+    """Oracle restatement of the same fwd + VJP step on one host core, on
+    `sample_groups` query groups, extrapolated to all groups."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    T = O.Tree(p, m, max_depth=args.depth, leaf_size=args.leaf)
+    t_build = time.perf_counter() - t0
+    ng = (T.n + 63) // 64
+    gsel = min(sample_groups, ng)
+    g0 = (ng - gsel) // 2
+    quad = args.order == 2
+    cs = c[T.perm]
+    t0 = time.perf_counter()
+    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=quad)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    T.eval_groups(0, 0, beta=args.beta, eps=args.eps, use_quad=quad, cot_sorted=cs)
+    t_push = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=quad, cot_sorted=cs)
+    t_vjp = time.perf_counter() - t0
+    scale = ng / gsel
+    t_step = 2 * t_build + t_fwd * scale + t_push + max(t_vjp - t_push, 0.0) * scale
+    return {
+        "value": T.n / t_step,
+        "unit": "forces/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle adaptive-tree walk (fp64, 1 core): tree build x2 timed in full; "
+                   f"forward and adjoint walks timed on {gsel} of {ng} groups of 64 "
+                   f"queries and scaled by {scale:.1f}"),
+        "seconds_measured": round(2 * t_build + t_fwd + t_vjp, 2),
+        "host_cpu": platform.processor() or platform.machine(),
+        "host_threads_available": os.cpu_count(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--order", type=int, default=2, choices=(1, 2))
+    ap.add_argument("--depth", type=int, default=10)
+    ap.add_argument("--leaf", type=int, default=32)
+    ap.add_argument("--eps", type=float, default=0.005)
+    ap.add_argument("--beta", type=float, default=2.0)
+    ap.add_argument("--accuracy-queries", type=int, default=1024)
+    ap.add_argument("--cpu-sample-groups", type=int, default=192)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
+                                   "it (queries would shard by Morton range, see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import sbh_amd
+    from sbh_amd import gravity as G
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    n = args.n
+    p = plummer(n, 2000)
+    m = np.full(n, 1.0 / n, np.float32)
+    c = np.random.default_rng(2002).normal(size=(n, 3)).astype(np.float32)
+    P = torch.tensor(p, device=dev)
+    M = torch.tensor(m, device=dev)
+    C = torch.tensor(c, device=dev)
+    kw = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth,
+              leaf_size=args.leaf, multipole_order=args.order)
+    ws = G.new_workspace(n, args.depth, args.leaf, dev)
+    F = torch.empty_like(P)
+
+    def step():
+        G.gravity_force(P, M, workspace=ws, out=F, **kw)
+        return G.gravity_force_vjp(P, M, C, workspace=ws, **kw)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    lib = sbh_amd.load_library()
+    lib.sbh_profile_enable(ctypes.c_int32(1))
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        gx, gm = step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    lib.sbh_profile_enable(ctypes.c_int32(0))
+    prof = (ctypes.c_double * 6)()
+    lib.sbh_profile_collect(prof, ctypes.c_int32(6))
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+    value = n / (ms_step * 1e-3)
+
+    # interaction counts (deterministic: identical every step)
+    sf, sv = G.new_stats(dev), G.new_stats(dev)
+    G.gravity_force(P, M, workspace=ws, stats=sf, **kw)
+    G.gravity_force_vjp(P, M, C, workspace=ws, stats=sv, **kw)
+    sf, sv = G.stats_dict(sf), G.stats_dict(sv)
+    walk_vjp_ms = prof[5] / max(prof[3], 1)
+    walk_fwd_ms = prof[2] / max(prof[0], 1)
+    flops_vjp = FLOPS_NEAR_VJP * sv["near"] + FLOPS_FAR_VJP[args.order] * sv["far"]
+    flops_fwd = FLOPS_NEAR_FWD * sf["near"] + FLOPS_FAR_FWD[args.order] * sf["far"]
+    achieved = flops_vjp / (walk_vjp_ms * 1e-3) / 1e12
+
+    # accuracy against the FP64 direct sum on a query sample
+    idx = torch.tensor(np.random.default_rng(7).choice(n, args.accuracy_queries, replace=False),
+                       device=dev)
+    ref = direct_fp64(P.double(), M.double(), P[idx].double(), args.eps)
+    acc = {"force_rel_l2_vs_fp64_direct": rel_l2(F[idx], ref),
+           "force_sample_queries": args.accuracy_queries}
+    direct = None
+    if not args.no_direct:
+        from sbh_amd import ffi
+
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t2 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        Fd = ffi.softened_direct_force(P, M, P, softening_scale=args.eps,
+                                       cutoff_scale=float("inf"))
+        t1.record(stream)
+        gxd, gmd = ffi.softened_direct_force_full_vjp(P, M, C, softening_scale=args.eps,
+                                                      cutoff_scale=float("inf"))
+        t2.record(stream)
+        torch.cuda.synchronize()
+        d_ms = t0.elapsed_time(t2)
+        direct = {"fwd_ms": t0.elapsed_time(t1), "vjp_ms": t1.elapsed_time(t2),
+                  "fwd_plus_vjp_ms": d_ms, "speedup_bh_vs_direct": d_ms / ms_step,
+                  "direct_fwd_tflops": 20.0 * n * n / (t0.elapsed_time(t1) * 1e-3) / 1e12,
+                  "direct_fwd_rel_l2_vs_fp64": rel_l2(Fd[idx], ref)}
+        acc["grad_points_rel_l2_vs_fp32_direct"] = rel_l2(gx, gxd)
+        acc["grad_masses_rel_l2_vs_fp32_direct"] = rel_l2(gm, gmd)
+        acc["force_full_rel_l2_vs_fp32_direct"] = rel_l2(F, Fd)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, m, c, args, args.cpu_sample_groups)
+
+    rec = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "forces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"plummer_sphere_n{n}_theta0.5_depth{args.depth}_leaf{args.leaf}"
+                        f"_eps{args.eps}_fwd+vjp",
+            "n": n, "theta": 1.0 / args.beta, "beta_diag": args.beta, "max_depth": args.depth,
+            "leaf_size": args.leaf, "softening": args.eps, "multipole_order": args.order,
+            "step": "forward (build+walk) + recompute VJP (rebuild+adjoint walk), full "
+                    "position and mass gradients",
+            "batched": "not measured (single set)",
+        },
+        "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
+                      "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
+        "interactions": {"fwd": sf, "vjp": sv, "fwd_walk_tflops":
+                         flops_fwd / (walk_fwd_ms * 1e-3) / 1e12},
+        "accuracy": acc,
+        "direct_sum": direct,
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
+                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)}>",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops_vjp},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(rec))
+
+
+if __name__ == "__main__":
+    main()

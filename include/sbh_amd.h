@@ -37,6 +37,8 @@ extern "C" {
 #define SBH_INTERNAL 13
 
 /* Element types: XLA_FFI_DataType values. */
+#define SBH_DTYPE_S32 4
+#define SBH_DTYPE_S64 5
 #define SBH_DTYPE_U8 6
 #define SBH_DTYPE_U32 8
 #define SBH_DTYPE_F32 11
@@ -143,6 +145,63 @@ int sbh_barnes_hut_force_octree8_soft_vjp(
     float beta, float softening_scale, float cutoff_scale,
     int32_t prune_enabled, float prune_r_cut_mult, int32_t periodic,
     float box_length, int32_t threads);
+
+/* ---- adaptive Morton octree pipeline (new targets) ----------------------
+ *
+ * No reference counterpart: the reference's octree8 is dense (8^D cells,
+ * infeasible at D = 10), has no leaf-size parameter, and its custom_vjp
+ * returns the query-side gradient only (python/jax_sbh_ffi.py:1855-1891).
+ * These targets build an adaptive octree on the device (bounds = data bounds
+ * -/+ 1e-3, codes with the reference quantiser at depth max_depth, a cell
+ * splits while it holds more than leaf_size points), then walk it with one
+ * 64-lane wavefront per group of 64 Morton-consecutive points.  A cell is used
+ * as a multipole for a group iff the distance from its centre to the group's
+ * bounding box is >= beta * cell diagonal (the reference MAC,
+ * src/sbh_ffi.cu:1878-1893, applied conservatively to the whole group);
+ * otherwise it is opened, and rejected leaves are summed exactly.
+ *
+ * Self-gravity only (queries are the points; the i == j pair is skipped).
+ * Kernel: F_i = sum -m (r^2 + eps^2 + 1e-12)^(-3/2) d, no cutoff, not periodic.
+ * multipole_order: 1 = monopole, 2 = monopole + quadrupole.
+ * workspace: u8 device buffer of sbh_gravity_workspace_bytes() bytes, or NULL
+ *            for a library-owned cached buffer (allocated when it must grow).
+ * stats:     NULL or int64[8] device buffer receiving {far (lane x node)
+ *            interactions, near pairs, groups, nodes, levels, overflow, 0, 0}. */
+int64_t sbh_gravity_workspace_bytes(int64_t n, int32_t max_depth, int32_t leaf_size);
+
+/* Byte offsets of the workspace arrays, in this order: header, codes, perm,
+ * sorted points, sorted cotangents, topo (int4 start,count,first_child,
+ * nchild), prefix, parent, geo (float4 cx,cy,cz,diagonal), mp (float4 com,mass),
+ * qd (2 x float4 second moment), adj, adjq, qgrad, sgrad, total, node capacity. */
+int sbh_gravity_workspace_layout(int64_t n, int32_t max_depth, int32_t leaf_size,
+                                 int64_t* offsets, int32_t count);
+
+/* Forces only: points (N,3), masses (N) -> forces (N,3). */
+int sbh_gravity_force(void* stream, const SbhBuffer* points,
+                      const SbhBuffer* masses, SbhBuffer* forces, float beta,
+                      float softening_scale, int32_t max_depth,
+                      int32_t leaf_size, int32_t multipole_order,
+                      SbhBuffer* workspace, SbhBuffer* stats);
+
+/* Recompute-style VJP: rebuilds the tree and walks it once, returning the
+ * full adjoint of L = sum_i c_i . F_i with respect to positions and masses
+ * (grad_points (N,3), grad_masses (N)); forces (N,3) may be NULL. */
+int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
+                          const SbhBuffer* masses, const SbhBuffer* cotangent,
+                          SbhBuffer* forces, SbhBuffer* grad_points,
+                          SbhBuffer* grad_masses, float beta,
+                          float softening_scale, int32_t max_depth,
+                          int32_t leaf_size, int32_t multipole_order,
+                          SbhBuffer* workspace, SbhBuffer* stats);
+
+/* Phase timing for the sbh_gravity_* targets (no reference counterpart; the
+ * reference times from Python around dispatch).  When enabled, each call
+ * records HIP events on its stream around the tree build and around the walk
+ * kernel.  sbh_profile_collect waits for the recorded events and writes
+ * {forward calls, forward build ms, forward walk ms, vjp calls, vjp build ms,
+ * vjp walk ms} summed since the previous collect. */
+void sbh_profile_enable(int32_t on);
+int sbh_profile_collect(double* out, int32_t count);
 
 #ifdef __cplusplus
 }

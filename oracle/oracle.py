@@ -98,11 +98,15 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_tree_build.argtypes = [_u32p, _i64, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float,
                                  _i64, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f32p]
     L.orc_tree_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p]
-    L.orc_group_boxes.argtypes = [_f32p, _i64, ctypes.c_int, _f32p]
+    L.orc_tree_groups.restype = _i64
+    L.orc_tree_groups.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, ctypes.c_int, _i32p, _i32p]
+    L.orc_group_boxes.argtypes = [_f32p, _i64, _i32p, _i32p, _f32p]
     L.orc_tree_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
-                                _f64p, _f64p, _i64, _f32p, _f64p, _i64, ctypes.c_int,
+                                _f64p, _f64p, _i64, _f32p, _f64p, _i64, _i64, _i32p, _i32p,
                                 ctypes.c_float, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                 _vp, _f64p, _vp, _vp, _vp, ctypes.POINTER(_i64)]
+    L.orc_tree_eval_range.argtypes = (list(L.orc_tree_eval.argtypes[:15]) +
+                                      list(L.orc_tree_eval.argtypes[16:]) + [_i64, _i64])
     L.orc_philox4x32.argtypes = [_u32p, _u32p, _u32p]
 
 
@@ -290,23 +294,33 @@ class Tree:
                                _c(ms, np.float64), mass, com, quad)
         return mass, com, quad
 
-    def group_boxes(self, q_sorted_f32, group_size):
+    def groups(self, gmax=64):
+        """Query groups of the walk as (start, count) arrays (orc_tree_groups)."""
+        gs = np.zeros(max(self.n, 1), np.int32)
+        gc = np.zeros(max(self.n, 1), np.int32)
+        ng = lib().orc_tree_groups(self.num_nodes, self.start, self.count, self.first_child,
+                                   self.nchild, int(gmax), gs, gc)
+        return gs[:ng].copy(), gc[:ng].copy()
+
+    def group_boxes(self, q_sorted_f32, gmax=64):
         q = _c(q_sorted_f32, np.float32)
-        ng = (q.shape[0] + group_size - 1) // group_size
-        boxes = np.zeros(ng * 6, np.float32)
-        lib().orc_group_boxes(q, q.shape[0], int(group_size), boxes)
+        gs, gc = self.groups(gmax)
+        boxes = np.zeros(max(len(gs), 1) * 6, np.float32)
+        lib().orc_group_boxes(q, len(gs), gs, gc, boxes)
         return boxes
 
     def eval(self, *, beta=2.0, eps=0.0, use_quad=True, group_size=64, cot_sorted=None,
              xs=None, ms=None, boxes=None):
         """Self-gravity walk over the sorted sources (queries == sources).
 
-        xs/ms override the source positions/masses (fp64, sorted) while the
+        group_size is the largest query group (see orc_tree_groups).  xs/ms
+        override the source positions/masses (fp64, sorted) while the
         topology and the group boxes stay frozen -- used for finite differences.
-        Returns dict(force, gq, gsx, gsm, far, near) in sorted order.
+        Returns dict(force, gq, gsx, gsm, far, near, groups) in sorted order.
         """
         xs = self.points_sorted.astype(np.float64) if xs is None else _c(xs, np.float64)
         ms = self.masses_sorted.astype(np.float64) if ms is None else _c(ms, np.float64)
+        gs, gc = self.groups(group_size)
         if boxes is None:
             boxes = self.group_boxes(self.points_sorted, group_size)
         mass, com, quad = self.moments(xs, ms)
@@ -319,11 +333,35 @@ class Tree:
         counts = (_i64 * 2)()
         lib().orc_tree_eval(self.num_nodes, self.start, self.count, self.first_child,
                             self.nchild, _c(self.geo, np.float32).ravel(), mass,
-                            com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n,
-                            int(group_size), float(beta), int(bool(use_quad)), float(eps), 1,
+                            com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n, len(gs),
+                            gs, gc, float(beta), int(bool(use_quad)), float(eps), 1,
                             _ptr(cot), force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts)
         return dict(force=force, gq=gq, gsx=gsx, gsm=gsm, far=int(counts[0]),
-                    near=int(counts[1]))
+                    near=int(counts[1]), groups=len(gs))
+
+    def eval_groups(self, g_begin, g_end, *, beta=2.0, eps=0.0, use_quad=True, group_size=64,
+                    cot_sorted=None):
+        """Walk only groups [g_begin, g_end) (the node adjoints are still
+        pushed down over the whole tree).  Used to time a bounded sample."""
+        xs = self.points_sorted.astype(np.float64)
+        ms = self.masses_sorted.astype(np.float64)
+        gs, gc = self.groups(group_size)
+        boxes = self.group_boxes(self.points_sorted, group_size)
+        mass, com, quad = self.moments(xs, ms)
+        n = self.n
+        force = np.zeros((n, 3), np.float64)
+        cot = None if cot_sorted is None else _c(cot_sorted, np.float64)
+        gq = np.zeros((n, 3), np.float64) if cot is not None else None
+        gsx = np.zeros((n, 3), np.float64) if cot is not None else None
+        gsm = np.zeros(n, np.float64) if cot is not None else None
+        counts = (_i64 * 2)()
+        lib().orc_tree_eval_range(self.num_nodes, self.start, self.count, self.first_child,
+                                  self.nchild, _c(self.geo, np.float32).ravel(), mass,
+                                  com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n, gs, gc,
+                                  float(beta), int(bool(use_quad)), float(eps), 1, _ptr(cot),
+                                  force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts,
+                                  int(g_begin), int(g_end))
+        return dict(force=force, far=int(counts[0]), near=int(counts[1]), groups=len(gs))
 
 
 # ---------------------------------------------------------------- Philox

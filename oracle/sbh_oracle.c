@@ -755,13 +755,70 @@ ORC_EXPORT void orc_tree_moments(const int32_t* start, const int32_t* count, int
   }
 }
 
-/* Group boxes over sorted queries, fp32, groups of group_size consecutive
- * queries (the last group may be short). */
-ORC_EXPORT void orc_group_boxes(const float* q, int64_t nq, int group_size, float* boxes) {
-  int64_t ng = (nq + group_size - 1) / group_size;
+/* Query groups of the walk: a partition of the sorted points into contiguous
+ * ranges of at most gmax points that stay inside one cell.  For every node
+ * holding more than gmax points, its children are scanned in order and runs
+ * of consecutive children with <= gmax points each are packed greedily into
+ * groups of <= gmax points (children above gmax are nodes of their own); a
+ * leaf holding more than gmax points (possible at the depth limit) is cut
+ * into consecutive chunks of gmax.  If the root holds <= gmax points it is
+ * the only group.  Order: by node index, then child order.  Returns the
+ * number of groups written to g_start/g_count. */
+ORC_EXPORT int64_t orc_tree_groups(int64_t nn, const int32_t* start, const int32_t* count,
+                                   const int32_t* first_child, const int32_t* nchild,
+                                   int gmax, int32_t* g_start, int32_t* g_count) {
+  int64_t ng = 0;
+  if (nn < 1 || count[0] == 0) return 0;
+  if (count[0] <= gmax) {
+    g_start[0] = start[0];
+    g_count[0] = count[0];
+    return 1;
+  }
+  for (int64_t k = 0; k < nn; k++) {
+    if (count[k] <= gmax) continue;
+    if (nchild[k] == 0) {
+      for (int32_t s = start[k]; s < start[k] + count[k]; s += gmax) {
+        g_start[ng] = s;
+        g_count[ng] = (start[k] + count[k] - s) < gmax ? (start[k] + count[k] - s) : gmax;
+        ng++;
+      }
+      continue;
+    }
+    int32_t cs = -1, cc = 0;
+    for (int32_t c = first_child[k]; c < first_child[k] + nchild[k]; c++) {
+      if (count[c] > gmax) {
+        if (cc > 0) {
+          g_start[ng] = cs;
+          g_count[ng] = cc;
+          ng++;
+        }
+        cc = 0;
+        continue;
+      }
+      if (cc > 0 && cc + count[c] > gmax) {
+        g_start[ng] = cs;
+        g_count[ng] = cc;
+        ng++;
+        cc = 0;
+      }
+      if (cc == 0) cs = start[c];
+      cc += count[c];
+    }
+    if (cc > 0) {
+      g_start[ng] = cs;
+      g_count[ng] = cc;
+      ng++;
+    }
+  }
+  return ng;
+}
+
+/* Group boxes (fp32 min / max of the group's sorted query positions). */
+ORC_EXPORT void orc_group_boxes(const float* q, int64_t ng, const int32_t* g_start,
+                                const int32_t* g_count, float* boxes) {
   for (int64_t g = 0; g < ng; g++) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    for (int64_t i = g * group_size; i < nq && i < (g + 1) * group_size; i++)
+    for (int64_t i = g_start[g]; i < g_start[g] + g_count[g]; i++)
       for (int a = 0; a < 3; a++) {
         lo[a] = fminf(lo[a], q[3 * i + a]);
         hi[a] = fmaxf(hi[a], q[3 * i + a]);
@@ -837,7 +894,8 @@ static void orc_far(const double* q, double M, const double* X, const double* Q,
   }
 }
 
-/* Evaluate the adaptive-tree walk over sorted queries in groups.
+/* Evaluate the adaptive-tree walk over sorted queries in groups (ranges
+ * g_start/g_count, bounding boxes `boxes`, see orc_tree_groups).
  *   self_mode: queries are the sorted sources (query i == source i); the
  *              i == j pair is skipped.
  *   cot == NULL: forces only.  Otherwise also
@@ -845,17 +903,45 @@ static void orc_far(const double* q, double M, const double* X, const double* Q,
  *     gsx  (n*3), gsm (n): source-side position and mass gradients
  *                 (pair terms plus the push-down of node adjoints).
  *   counts[0] = far (node) interactions, counts[1] = near pairs.          */
+ORC_EXPORT void orc_tree_eval_range(int64_t nn, const int32_t* start, const int32_t* count,
+                                    const int32_t* first_child, const int32_t* nchild,
+                                    const float* geo, const double* mass, const double* com,
+                                    const double* quad, const double* xs, const double* ms,
+                                    int64_t n, const float* boxes, const double* qx,
+                                    int64_t nq, const int32_t* g_start,
+                                    const int32_t* g_count, float beta, int use_quad,
+                                    double eps, int self_mode, const double* cot,
+                                    double* force, double* gq, double* gsx, double* gsm,
+                                    int64_t* counts, int64_t g_begin, int64_t g_end);
+
 ORC_EXPORT void orc_tree_eval(int64_t nn, const int32_t* start, const int32_t* count,
                               const int32_t* first_child, const int32_t* nchild,
                               const float* geo, const double* mass, const double* com,
                               const double* quad, const double* xs, const double* ms,
                               int64_t n, const float* boxes, const double* qx,
-                              int64_t nq, int group_size, float beta, int use_quad,
+                              int64_t nq, int64_t ng, const int32_t* g_start,
+                              const int32_t* g_count, float beta, int use_quad,
                               double eps, int self_mode, const double* cot,
                               double* force, double* gq, double* gsx, double* gsm,
                               int64_t* counts) {
+  orc_tree_eval_range(nn, start, count, first_child, nchild, geo, mass, com, quad, xs, ms, n,
+                      boxes, qx, nq, g_start, g_count, beta, use_quad, eps, self_mode, cot,
+                      force, gq, gsx, gsm, counts, 0, ng);
+}
+
+/* Same walk restricted to groups [g_begin, g_end) (used to time a bounded
+ * sample of the full evaluation). */
+ORC_EXPORT void orc_tree_eval_range(int64_t nn, const int32_t* start, const int32_t* count,
+                                    const int32_t* first_child, const int32_t* nchild,
+                                    const float* geo, const double* mass, const double* com,
+                                    const double* quad, const double* xs, const double* ms,
+                                    int64_t n, const float* boxes, const double* qx,
+                                    int64_t nq, const int32_t* g_start,
+                                    const int32_t* g_count, float beta, int use_quad,
+                                    double eps, int self_mode, const double* cot,
+                                    double* force, double* gq, double* gsx, double* gsm,
+                                    int64_t* counts, int64_t g_begin, int64_t g_end) {
   double e2 = (eps > 0 ? eps * eps : 0.0) + 1e-12;
-  int64_t ng = (nq + group_size - 1) / group_size;
   double* adjX = NULL;
   double* adjM = NULL;
   double* adjS = NULL;
@@ -872,7 +958,7 @@ ORC_EXPORT void orc_tree_eval(int64_t nn, const int32_t* start, const int32_t* c
   int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 8));
   int32_t* far = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
   int32_t* near = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
-  for (int64_t g = 0; g < ng; g++) {
+  for (int64_t g = g_begin; g < g_end; g++) {
     const float* box = boxes + 6 * g;
     int64_t nf = 0, nl = 0, top = 0;
     stack[top++] = 0;
@@ -887,7 +973,7 @@ ORC_EXPORT void orc_tree_eval(int64_t nn, const int32_t* start, const int32_t* c
         for (int32_t c = 0; c < nchild[k]; c++) stack[top++] = first_child[k] + c;
       }
     }
-    for (int64_t i = g * group_size; i < nq && i < (g + 1) * group_size; i++) {
+    for (int64_t i = g_start[g]; i < g_start[g] + g_count[g] && i < nq; i++) {
       const double* q = qx + 3 * i;
       const double* c = cot ? cot + 3 * i : NULL;
       double F[3] = {0, 0, 0}, G[3] = {0, 0, 0};

@@ -1,0 +1,210 @@
+"""GPU parity: adaptive Morton octree + group walk + full adjoint
+(sbh_gravity_force / sbh_gravity_force_vjp) against the oracle.
+
+- Topology (perm, codes, node ranges, children, parents) and node geometry
+  must equal the oracle's adaptive tree bit for bit.
+- Interaction counts (far lane-node pairs, near pairs) must be identical,
+  i.e. the same MAC decisions; forces and gradients then differ from the
+  oracle's FP64 evaluation of the same interaction set only by fp32 rounding
+  (tolerance rel_l2 <= 2e-5 forces, <= 1e-4 gradients, stated per test).
+- Accuracy against the FP64 direct sum (the metric's bar, rel_l2 <= 1e-3).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _G():
+    from sbh_amd import gravity
+
+    return gravity
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def plummer(n, seed):
+    rng = np.random.default_rng(seed)
+    xmax = 20.0 ** 3 / (20.0 ** 2 + 1.0) ** 1.5
+    X = rng.uniform(0, xmax, n)
+    r = 1.0 / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
+    v = rng.normal(size=(n, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    return (r[:, None] * v).astype(np.float32)
+
+
+def blobs(n, k, seed):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 1, (k, 3))
+    lab = rng.integers(0, k, n)
+    return (c[lab] + rng.normal(0, 0.02, (n, 3))).astype(np.float32)
+
+
+def _case(kind, n, seed):
+    rng = np.random.default_rng(seed + 1)
+    if kind == "plummer":
+        p = plummer(n, seed)
+    elif kind == "blobs":
+        p = blobs(n, 64, seed)
+    elif kind == "uniform":
+        p = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+    elif kind == "same":
+        p = np.tile(np.array([[0.25, -0.5, 1.0]], np.float32), (n, 1))
+    else:
+        raise ValueError(kind)
+    m = (rng.lognormal(0, 0.5, n) / n).astype(np.float32)
+    c = rng.normal(size=(n, 3)).astype(np.float32)
+    return p, m, c
+
+
+@pytest.mark.parametrize("kind,n,depth,leaf", [
+    ("plummer", 20000, 10, 32), ("blobs", 12000, 10, 16), ("uniform", 5000, 6, 8),
+    ("same", 300, 10, 32), ("uniform", 1, 10, 32), ("uniform", 40, 10, 32),
+])
+def test_tree_matches_oracle(gpu, kind, n, depth, leaf):
+    G = _G()
+    p, m, _ = _case(kind, n, 7)
+    ws = G.new_workspace(n, depth, leaf)
+    G.gravity_force(p, m, max_depth=depth, leaf_size=leaf, workspace=ws)
+    tv = G.read_tree(ws, n, depth, leaf)
+    T = O.Tree(p, m, max_depth=depth, leaf_size=leaf)
+    assert tv.overflow == 0
+    np.testing.assert_array_equal(tv.perm, T.perm)
+    np.testing.assert_array_equal(tv.codes, T.codes)
+    assert tv.num_nodes == T.num_nodes
+    np.testing.assert_array_equal(tv.topo[:, 0], T.start)
+    np.testing.assert_array_equal(tv.topo[:, 1], T.count)
+    np.testing.assert_array_equal(tv.topo[:, 2], T.first_child)
+    np.testing.assert_array_equal(tv.topo[:, 3], T.nchild)
+    np.testing.assert_array_equal(tv.parent, T.parent)
+    np.testing.assert_array_equal(tv.geo, T.geo)
+    np.testing.assert_array_equal(tv.sorted_points[:, :3], T.points_sorted)
+    mass, com, quad = T.moments()
+    np.testing.assert_allclose(tv.mp[:, 3], mass, rtol=2e-5)
+    scale = np.abs(T.points_sorted).max()
+    np.testing.assert_allclose(tv.mp[:, :3], com, rtol=0, atol=2e-6 * scale)
+    qs = np.abs(quad).max() + 1e-30
+    gq = tv.qd[:, [0, 1, 2, 3, 4, 5]]
+    np.testing.assert_allclose(gq, quad, rtol=0, atol=2e-4 * qs)
+
+
+@pytest.mark.parametrize("kind,n", [("plummer", 20000), ("blobs", 16384), ("uniform", 3001)])
+@pytest.mark.parametrize("order", [1, 2])
+def test_forces_and_adjoint_match_oracle(gpu, kind, n, order):
+    G = _G()
+    p, m, c = _case(kind, n, 11)
+    eps = 0.005
+    stats = G.new_stats()
+    gx, gm, f = G.gravity_force_vjp(p, m, c, beta=2.0, softening_scale=eps, max_depth=10,
+                                    leaf_size=32, multipole_order=order, stats=stats,
+                                    return_forces=True)
+    gx, gm, f = _np(gx), _np(gm), _np(f)
+    st = G.stats_dict(stats)
+    T = O.Tree(p, m, max_depth=10, leaf_size=32)
+    r = T.eval(beta=2.0, eps=eps, use_quad=order == 2, cot_sorted=c[T.perm])
+    # identical interaction sets
+    assert st["far"] == r["far"] and st["near"] == r["near"], (st, r["far"], r["near"])
+    assert st["groups"] == r["groups"] and st["overflow"] == 0
+    perm = T.perm
+    assert O.rel_l2(f[perm], r["force"]) < 2e-5
+    assert O.rel_l2(gx[perm], r["gq"] + r["gsx"]) < 1e-4
+    assert O.rel_l2(gm[perm], r["gsm"]) < 1e-4
+    # forward-only entry point gives the same forces
+    f2 = _np(G.gravity_force(p, m, beta=2.0, softening_scale=eps, multipole_order=order))
+    np.testing.assert_allclose(f2, f, rtol=1e-6, atol=1e-6 * np.abs(f).max())
+
+
+def test_accuracy_vs_fp64_direct(gpu):
+    G = _G()
+    n = 65536
+    p, m, c = _case("plummer", n, 3)
+    m = np.full(n, 1.0 / n, np.float32)
+    eps = 0.005
+    idx = np.random.default_rng(0).choice(n, 2048, replace=False)
+    ref = O.direct_force_soft(p, m, p[idx], eps, np.inf)
+    for order in (1, 2):
+        f = _np(G.gravity_force(p, m, beta=2.0, softening_scale=eps, multipole_order=order))
+        err = O.rel_l2(f[idx], ref)
+        print(f"order={order} rel_l2={err:.3e} outlier={O.outlier_frac(f[idx], ref):.4f}")
+        assert err < 1e-3
+
+
+def test_gradients_vs_fp64_direct_adjoint(gpu):
+    G = _G()
+    n = 8192
+    p, m, c = _case("blobs", n, 5)
+    eps = 0.002
+    rgx, rgm = O.direct_full_vjp(p, m, c, eps)
+    for order in (1, 2):
+        gx, gm = G.gravity_force_vjp(p, m, c, beta=2.0, softening_scale=eps,
+                                     multipole_order=order)
+        ex, em = O.rel_l2(_np(gx), rgx), O.rel_l2(_np(gm), rgm)
+        print(f"order={order} grad_x rel_l2={ex:.3e} grad_m rel_l2={em:.3e}")
+        assert ex < 1e-3 and em < (1e-3 if order == 2 else 3e-3)
+
+
+def test_huge_beta_is_direct_sum(gpu):
+    G = _G()
+    n = 3000
+    p, m, c = _case("plummer", n, 2)
+    f = _np(G.gravity_force(p, m, beta=1e30, softening_scale=0.01))
+    ref = O.direct_force_soft(p, m, p, 0.01, np.inf)
+    assert O.rel_l2(f, ref) < 1e-5
+    gx, gm = G.gravity_force_vjp(p, m, c, beta=1e30, softening_scale=0.01)
+    rgx, rgm = O.direct_full_vjp(p, m, c, 0.01)
+    assert O.rel_l2(_np(gx), rgx) < 1e-5 and O.rel_l2(_np(gm), rgm) < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129])
+def test_small_and_ragged(gpu, n):
+    G = _G()
+    p, m, c = _case("uniform", n, n)
+    f = _np(G.gravity_force(p, m, softening_scale=0.05, leaf_size=4))
+    ref = O.direct_force_soft(p, m, p, 0.05, np.inf)
+    assert np.isfinite(f).all()
+    if n > 1:
+        assert O.rel_l2(f, ref) < 1e-3
+    gx, gm = G.gravity_force_vjp(p, m, c, softening_scale=0.05, leaf_size=4)
+    rgx, rgm = O.direct_full_vjp(p, m, c, 0.05)
+    if n > 1:
+        assert O.rel_l2(_np(gx), rgx) < 2e-3 and O.rel_l2(_np(gm), rgm) < 5e-3
+    else:
+        assert np.all(_np(gx) == 0) and np.all(_np(gm) == 0)
+
+
+def test_identical_points_deep_leaf(gpu):
+    G = _G()
+    n = 500
+    p, m, c = _case("same", n, 1)
+    p[-1] += np.float32(0.5)
+    f = _np(G.gravity_force(p, m, softening_scale=0.01))
+    ref = O.direct_force_soft(p, m, p, 0.01, np.inf)
+    assert O.rel_l2(f, ref) < 1e-5
+
+
+def test_recompute_custom_vjp(gpu):
+    G = _G()
+    n = 4096
+    p, m, c = _case("blobs", n, 9)
+    x = torch.tensor(p, device="cuda", requires_grad=True)
+    mt = torch.tensor(m, device="cuda", requires_grad=True)
+    ct = torch.tensor(c, device="cuda")
+    kw = dict(softening_scale=0.002, multipole_order=2)
+    (G.gravity_force_recompute_vjp(x, mt, **kw) * ct).sum().backward()
+    gx, gm = G.gravity_force_vjp(p, m, c, **kw)
+    np.testing.assert_allclose(_np(x.grad), _np(gx), rtol=1e-4, atol=1e-4 * np.abs(_np(gx)).max())
+    np.testing.assert_allclose(_np(mt.grad), _np(gm), rtol=1e-4, atol=1e-4 * np.abs(_np(gm)).max())
+
+
+def test_determinism_of_forces(gpu):
+    G = _G()
+    p, m, _ = _case("plummer", 30000, 4)
+    a = _np(G.gravity_force(p, m, softening_scale=0.005))
+    b = _np(G.gravity_force(p, m, softening_scale=0.005))
+    np.testing.assert_array_equal(a, b)

@@ -476,6 +476,33 @@ def test_tree_topology_invariants():
         np.testing.assert_allclose(T.geo[k, :3], T.min_corner + (np.array(ijk) + 0.5) * side, atol=1e-5)
 
 
+@pytest.mark.parametrize("gmax", [64, 32])
+def test_tree_groups_partition_and_stay_in_one_cell(gmax):
+    n = 20000
+    p = _plummer(n, 4)
+    p[:300] = p[0]  # a depth-limited leaf with more than gmax points
+    m = np.full(n, 1.0 / n, np.float32)
+    T = O.Tree(p, m, max_depth=10, leaf_size=16)
+    gs, gc = T.groups(gmax)
+    # contiguous partition of [0, n)
+    order = np.argsort(gs)
+    gs, gc = gs[order], gc[order]
+    assert gs[0] == 0 and np.all(gs[1:] == gs[:-1] + gc[:-1]) and gs[-1] + gc[-1] == n
+    assert gc.min() >= 1 and gc.max() <= gmax
+    # every group lies inside the cell of one node holding more than gmax
+    # points (its parent), or inside one over-full leaf
+    for s, c in zip(gs, gc):
+        inside = (T.start <= s) & (T.start + T.count >= s + c) & (T.count > gmax)
+        assert inside.any()
+    # groups are not split needlessly: two consecutive groups under the same
+    # parent could not have been merged
+    boxes = T.group_boxes(T.points_sorted, gmax).reshape(-1, 6)
+    diag = np.linalg.norm(boxes[:, 3:] - boxes[:, :3], axis=1)
+    consecutive = T.groups(gmax)
+    assert diag.max() <= np.float32(T.span) * np.sqrt(3) * 0.5 + 1e-3
+    assert len(consecutive[0]) == len(gs)
+
+
 def test_tree_huge_beta_is_direct_sum():
     n = 3000
     p = _plummer(n, 2)
@@ -511,7 +538,8 @@ def test_quadrupole_error_order():
                               np.ascontiguousarray(T.geo.ravel()), mass, com.ravel().copy(),
                               quad.ravel().copy(), T.points_sorted.astype(np.float64),
                               T.masses_sorted.astype(np.float64), T.n, boxes,
-                              q.astype(np.float64), 1, 1, 0.0, int(use_quad), 0.0, 0,
+                              q.astype(np.float64), 1, 1, np.zeros(1, np.int32),
+                              np.ones(1, np.int32), 0.0, int(use_quad), 0.0, 0,
                               None, F, None, None, None, cnt)
             assert cnt[0] == 1
             ref = O.direct_force_soft(p, m, q.astype(np.float32), 0.0, np.inf)
