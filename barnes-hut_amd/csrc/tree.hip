@@ -488,8 +488,9 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
-// One butterfly stage: lanes with bit `d` set keep the upper half of the
-// first 2h values and send the lower half to lane ^ d, and vice versa.
+// One butterfly stage of a transposed wave reduction: lanes with bit `d` set
+// keep the upper half of the first 2H values (adding the partner's copy),
+// the others keep the lower half.
 template <int H, int N>
 __device__ __forceinline__ void butterfly(float (&v)[N], int lane, int d) {
   const bool hi = lane & d;
@@ -501,25 +502,49 @@ __device__ __forceinline__ void butterfly(float (&v)[N], int lane, int d) {
   }
 }
 
-// Transposed reduction: v[0..31] per lane -> the 64-lane sum of v[lane >> 1]
-// (lanes 2j and 2j+1 both hold value j).
-__device__ __forceinline__ float reduce32(float (&v)[32], int lane) {
-  butterfly<16>(v, lane, 32);
-  butterfly<8>(v, lane, 16);
-  butterfly<4>(v, lane, 8);
-  butterfly<2>(v, lane, 4);
-  butterfly<1>(v, lane, 2);
-  return v[0] + __shfl_xor(v[0], 1, 64);
+// The d = 32 and d = 16 stages with the gfx950 half-wave / row swaps:
+// v_permlane32_swap exchanges lanes 32-63 of its first operand with lanes
+// 0-31 of the second (v_permlane16_swap: odd 16-lane rows of the first with
+// even rows of the second), so the sum of the two results is the butterfly
+// output with no select or LDS permute.
+template <int H, int N>
+__device__ __forceinline__ void butterfly_swap32(float (&v)[N]) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]), __float_as_uint(v[H + k]),
+                                              false, false);
+    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+}
+template <int H, int N>
+__device__ __forceinline__ void butterfly_swap16(float (&v)[N]) {
+#pragma unroll
+  for (int k = 0; k < H; ++k) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[k]), __float_as_uint(v[H + k]),
+                                              false, false);
+    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
 }
 
-// 64 values: returns the 64-lane sum of v[lane].
-__device__ __forceinline__ float reduce64(float (&v)[64], int lane) {
-  butterfly<32>(v, lane, 32);
-  butterfly<16>(v, lane, 16);
-  butterfly<8>(v, lane, 8);
-  butterfly<4>(v, lane, 4);
-  butterfly<2>(v, lane, 2);
-  butterfly<1>(v, lane, 1);
+// Transposed reduction of V = 2^k values per lane over the 64 lanes: returns
+// the 64-lane sum of v[lane >> (6 - k)] (each value is held by 64/V lanes).
+template <int V>
+__device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
+  static_assert(V == 16 || V == 32 || V == 64, "unsupported width");
+  butterfly_swap32<V / 2>(v);
+  butterfly_swap16<V / 4>(v);
+  butterfly<V / 8>(v, lane, 8);
+  butterfly<V / 16>(v, lane, 4);
+  if constexpr (V >= 32) {
+    butterfly<V / 32>(v, lane, 2);
+  } else {
+    v[0] += __shfl_xor(v[0], 2, 64);
+  }
+  if constexpr (V >= 64) {
+    butterfly<1>(v, lane, 1);
+  } else {
+    v[0] += __shfl_xor(v[0], 1, 64);
+  }
   return v[0];
 }
 
@@ -562,19 +587,38 @@ struct Lane {
   float cx, cy, cz;  // cotangent
 };
 
-// Far-field batch of kChunk nodes (sentinel-padded): forces, query-side
-// gradient and the node adjoint partials.
-template <bool VJP, bool QUAD>
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// Adds the wave-summed batch partials v[0..V) (PER values per entry, the
+// first USED of them meaningful) to dst: value j goes to
+// dst[stride * e + j % PER] with e = entry[j / PER] (or base + j / PER when
+// entry is null); e == skip and e >= limit are dropped.  One lane per value
+// issues the atomic.
+template <int V, int PER, int USED>
+__device__ __forceinline__ void scatter_batch(float (&v)[V], int lane, float* dst, int stride,
+                                              const int* entry, int skip, int base, int limit) {
+  constexpr int sh = 6 - ilog2(V);
+  const float s = wave_transpose_sum<V>(v, lane);
+  const int j = lane >> sh;
+  if ((lane & ((1 << sh) - 1)) == 0 && (j % PER) < USED) {
+    const int e = entry != nullptr ? entry[j / PER] : base + j / PER;
+    if (e != skip && e < limit) atomicAdd(&dst[stride * e + (j % PER)], s);
+  }
+}
+
+// Far-field batch of C nodes (sentinel-padded): forces (if FORCES),
+// query-side gradient and the node adjoint partials (if VJP).
+template <bool VJP, bool QUAD, bool FORCES, int C>
 __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, const int* far,
                                           int lane, float& fx, float& fy, float& fz, float& gx,
                                           float& gy, float& gz) {
-  int idx[kChunk];
+  int idx[C];
 #pragma unroll
-  for (int t = 0; t < kChunk; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
-  float part[32];
-  float qpart[64];
+  for (int t = 0; t < C; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
+  float part[4 * C];
+  float qpart[8 * C];
 #pragma unroll
-  for (int t = 0; t < kChunk; ++t) {
+  for (int t = 0; t < C; ++t) {
     const float4 m = sload4(A.mp, idx[t]);
     const float Dx = L.x - m.x, Dy = L.y - m.y, Dz = L.z - m.z;
     const float r2 = Dx * Dx + Dy * Dy + Dz * Dz;
@@ -582,9 +626,11 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
     const float s2 = s * s, s3 = s2 * s;
     if constexpr (!QUAD) {
       const float a = -m.w * s3;
-      fx += a * Dx;
-      fy += a * Dy;
-      fz += a * Dz;
+      if constexpr (FORCES) {
+        fx += a * Dx;
+        fy += a * Dy;
+        fz += a * Dz;
+      }
       if constexpr (VJP) {
         const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
         const float kD = -3.0f * a * s2 * u;  // 3 M s^5 u
@@ -611,9 +657,11 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       const float T = q0.x + q0.y + q0.z;
       const float a = -m.w * s3 + 1.5f * T * s5 - 7.5f * s7 * tq;
       const float b = 3.0f * s5;
-      fx += a * Dx + b * SDx;
-      fy += a * Dy + b * SDy;
-      fz += a * Dz + b * SDz;
+      if constexpr (FORCES) {
+        fx += a * Dx + b * SDx;
+        fy += a * Dy + b * SDy;
+        fz += a * Dz + b * SDz;
+      }
       if constexpr (VJP) {
         const float s9 = s7 * s2;
         const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
@@ -649,26 +697,20 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
     }
   }
   if constexpr (VJP) {
-    const float v = reduce32(part, lane);
-    const int j = lane >> 1;  // value j: node j >> 2, component j & 3
-    const int node = far[j >> 2];
-    if (!(lane & 1) && node != A.sentinel) atomicAdd(&A.adj[4 * node + (j & 3)], v);
-    if constexpr (QUAD) {
-      const float vq = reduce64(qpart, lane);
-      const int nq = far[lane >> 3];
-      const int cq = lane & 7;
-      if (cq < 6 && nq != A.sentinel) atomicAdd(&A.adjq[8 * nq + cq], vq);
-    }
+    // node adjoints: (dL/dX, dL/dM) and the six dL/dS entries
+    scatter_batch<4 * C, 4, 4>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
+    if constexpr (QUAD)
+      scatter_batch<8 * C, 8, 6>(qpart, lane, A.adjq, 8, far, A.sentinel, 0, INT_MAX);
   }
 }
 
 // Near-field batch: kChunk consecutive sources of one leaf starting at j0
 // (entries at or beyond s1 carry zero weight).
-template <bool VJP>
+template <bool VJP, bool FORCES>
 __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
                                            int s1, int lane, float& fx, float& fy, float& fz,
                                            float& gx, float& gy, float& gz) {
-  float part[32];
+  float part[4 * kChunk];
 #pragma unroll
   for (int u = 0; u < kChunk; ++u) {
     const int j = j0 + u;
@@ -680,9 +722,11 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
     float s3 = s * s * s;
     s3 = (j == i) ? 0.0f : s3;  // the self pair contributes nothing
     const float a = -w * s3;
-    fx += a * dx;
-    fy += a * dy;
-    fz += a * dz;
+    if constexpr (FORCES) {
+      fx += a * dx;
+      fy += a * dy;
+      fz += a * dz;
+    }
     if constexpr (VJP) {
       const float cd = L.cx * dx + L.cy * dy + L.cz * dz;
       const float k1 = -3.0f * a * s * s * cd;  // 3 w s^5 (c.d)
@@ -698,16 +742,12 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
       part[4 * u + 3] = -s3 * cd;
     }
   }
-  if constexpr (VJP) {
-    const float v = reduce32(part, lane);
-    const int jj = lane >> 1;
-    const int src = j0 + (jj >> 2);
-    if (!(lane & 1) && src < s1) atomicAdd(&A.sgrad[4 * src + (jj & 3)], v);
-  }
+  if constexpr (VJP)
+    scatter_batch<4 * kChunk, 4, 4>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
-template <bool VJP, bool QUAD>
+template <bool VJP, bool QUAD, bool FORCES>
 __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt, int lane,
                                            int* stack, int* far, int* near,
                                            unsigned long long& n_far,
@@ -743,13 +783,15 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   if (lane == 0) stack[0] = 0;
   __builtin_amdgcn_wave_barrier();
 
+  // The quadrupole adjoint holds 12 partials per node: batch 4 nodes, not 8.
+  constexpr int C = (VJP && QUAD) ? kChunk / 2 : kChunk;
   auto flush_far = [&]() {
     n_far += static_cast<unsigned long long>(nfar) * nvalid;
-    const int padded = (nfar + kChunk - 1) & ~(kChunk - 1);
+    const int padded = (nfar + C - 1) & ~(C - 1);
     if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
     __builtin_amdgcn_wave_barrier();
-    for (int f0 = 0; f0 < padded; f0 += kChunk)
-      far_batch<VJP, QUAD>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
+    for (int f0 = 0; f0 < padded; f0 += C)
+      far_batch<VJP, QUAD, FORCES, C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
     __builtin_amdgcn_wave_barrier();
     nfar = 0;
   };
@@ -763,7 +805,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       n_near += static_cast<unsigned long long>(t.y) * nvalid -
                 static_cast<unsigned long long>(max(0, min(s1, base + nvalid) - max(s0, base)));
       for (int j0 = s0; j0 < s1; j0 += kChunk)
-        near_batch<VJP>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+        near_batch<VJP, FORCES>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
     }
     nnear = 0;
   };
@@ -817,7 +859,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 
   if (valid) {
     const int64_t o = static_cast<int64_t>(A.perm[i]);
-    if (A.forces != nullptr) {
+    if constexpr (FORCES) {
       A.forces[3 * o + 0] = fx;
       A.forces[3 * o + 1] = fy;
       A.forces[3 * o + 2] = fz;
@@ -828,7 +870,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 
 // Persistent walk: each wave takes query groups from a device work queue
 // until none are left (heavy groups do not hold up a static schedule).
-template <bool VJP, bool QUAD>
+template <bool VJP, bool QUAD, bool FORCES>
 __global__ __launch_bounds__(kB) void k_walk(WalkArgs A) {
   __shared__ int s_stack[kWavesPerBlock][kStackCap];
   __shared__ int s_far[kWavesPerBlock][kFarCap];
@@ -846,7 +888,8 @@ __global__ __launch_bounds__(kB) void k_walk(WalkArgs A) {
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
     if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.n) continue;  // corrupt entry
-    walk_group<VJP, QUAD>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far, n_near);
+    walk_group<VJP, QUAD, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far,
+                                  n_near);
     ++n_groups;
   }
   if (A.stats != nullptr && lane == 0) {
@@ -1098,14 +1141,19 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   // several times over, never more than one wave per point.
   unsigned grid = grid_for(n, 64 * kWavesPerBlock, 256 * 8);
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[0], s));
-  if (vjp && quad)
-    k_walk<true, true><<<grid, kB, 0, s>>>(A);
+  const bool f = forces != nullptr;
+  if (vjp && quad && f)
+    k_walk<true, true, true><<<grid, kB, 0, s>>>(A);
+  else if (vjp && quad)
+    k_walk<true, true, false><<<grid, kB, 0, s>>>(A);
+  else if (vjp && f)
+    k_walk<true, false, true><<<grid, kB, 0, s>>>(A);
   else if (vjp)
-    k_walk<true, false><<<grid, kB, 0, s>>>(A);
+    k_walk<true, false, false><<<grid, kB, 0, s>>>(A);
   else if (quad)
-    k_walk<false, true><<<grid, kB, 0, s>>>(A);
+    k_walk<false, true, true><<<grid, kB, 0, s>>>(A);
   else
-    k_walk<false, false><<<grid, kB, 0, s>>>(A);
+    k_walk<false, false, true><<<grid, kB, 0, s>>>(A);
   SBH_LAUNCH_CHECK();
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[1], s));
   if (vjp) {
