@@ -37,6 +37,23 @@ FLOPS_FAR_FWD = {1: 20, 2: 54}
 FLOPS_FAR_VJP = {1: 48, 2: 200}
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
+                               "gravity_walk_plummer_n1048576_quad_hbm_traffic.md")
+
+
+def profiled_traffic(kernel_key):
+    """HBM bytes per launch of `kernel_key` measured by the two rocprofv3
+    --pmc passes recorded in profiles/ (None if not profiled)."""
+    try:
+        with open(TRAFFIC_PROFILE) as fh:
+            for line in fh:
+                if line.startswith(f"traffic_bytes_per_launch {kernel_key}:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def plummer(n, seed):
     import numpy as np
 
@@ -260,9 +277,12 @@ def main():
         "accuracy": acc,
         "direct_sum": direct,
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
-                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)}>",
+                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0>",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": (profiled_traffic("k_walk<true,true,false>")
+                                 if args.order == 2 and n == 1 << 20 else None),
+                     "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
                      "flops_per_launch": flops_vjp},
         "cpu_baseline": cpu,
     }
