@@ -107,15 +107,41 @@ int radix_sort_pairs(hipStream_t stream, uint32_t* keys_in, uint32_t* vals_in,
                      int key_bits, void* temp);
 
 // ---------------------------------------------------------------- device math
-// Min-image as include/kernel.h:27-39 (floor(x + 0.5)).
+// Min-image as include/kernel.h:27-39 (floor(x + 0.5)); not contracted so
+// that MAC decisions that depend on it equal the IEEE restatement.
 __device__ __forceinline__ float min_image_floor(float d, int periodic, float box,
                                                  float inv_box) {
+#pragma clang fp contract(off)
   return periodic ? d - box * floorf(d * inv_box + 0.5f) : d;
 }
 // Min-image as the direct-sum pair functions (src/sbh_ffi.cu:440-445).
 __device__ __forceinline__ float min_image_rint(float d, int periodic, float box,
                                                 float inv_box) {
+#pragma clang fp contract(off)
   return periodic ? d - box * rintf(d * inv_box) : d;
+}
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): counter-based RNG used
+// by the stochastic walks in place of cuRAND XORWOW.
+__device__ __forceinline__ uint4 philox4x32_10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
+    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+// Uniform in (0, 1] indexed by (query, node, sample, step): 24 random bits,
+// so P(u > t) = 1 - t exactly for t on the 2^-24 grid (the reference draws
+// curand_uniform, also in (0, 1], src/sbh_ffi.cu:2213, 2235).
+__device__ __forceinline__ float sbh_uniform(uint32_t seed, uint32_t q, uint32_t node,
+                                             uint32_t sample, uint32_t step) {
+  const uint4 r = philox4x32_10(make_uint4(q, node, sample, step), seed, 0x5BD1E995u);
+  return static_cast<float>((r.x >> 8) + 1u) * (1.0f / 16777216.0f);
 }
 
 // Morton quantiser (src/sbh_ffi.cu:173-179), IEEE fp32 without contraction.

@@ -9,6 +9,7 @@
 // Walks (replace src/sbh_ffi.cu:1950-2144): one thread per query, depth-first
 // with the reference's push order, parameters passed by value.
 #include "common.hpp"
+#include "o8walk.hpp"
 
 namespace sbh {
 namespace {
@@ -190,99 +191,13 @@ __global__ __launch_bounds__(kB) void k_build_nodes(
 }
 
 // ------------------------------------------------------------------ walks
-struct WalkParams {
-  float beta;
-  float eps2;
-  float rc;
-  int cutoff;  // attenuation active
-  int periodic;
-  float box;
-  float inv_box;
-  int prune;
-  float r_cut_sq;
-};
-
-// Far-field test (src/sbh_ffi.cu:1878-1893), IEEE fp32, not contracted.
-__device__ __forceinline__ bool use_far_field(const Node8* __restrict__ nd, float qx, float qy,
-                                              float qz, const WalkParams& P) {
-#pragma clang fp contract(off)
-  if (nd->is_leaf) return true;
-  float dx = min_image_floor(qx - nd->cx, P.periodic, P.box, P.inv_box);
-  float dy = min_image_floor(qy - nd->cy, P.periodic, P.box, P.inv_box);
-  float dz = min_image_floor(qz - nd->cz, P.periodic, P.box, P.inv_box);
-  float dist = sqrtf(dx * dx + dy * dy + dz * dz);
-  return dist / nd->diameter >= P.beta;
-}
-
-__device__ __forceinline__ float axis_dist(float q, float lo, float hi) {
-  if (q < lo) return lo - q;
-  if (q > hi) return q - hi;
-  return 0.0f;
-}
-
-// AABB prune (src/sbh_ffi.cu:1903-1948).
-__device__ __forceinline__ bool should_prune(const Node8* __restrict__ nd, float qx, float qy,
-                                             float qz, const WalkParams& P) {
-#pragma clang fp contract(off)
-  if (!P.prune) return false;
-  if (P.periodic) {
-    qx -= P.box * rintf((qx - nd->cx) * P.inv_box);
-    qy -= P.box * rintf((qy - nd->cy) * P.inv_box);
-    qz -= P.box * rintf((qz - nd->cz) * P.inv_box);
-  }
-  float h = nd->half_size;
-  float dx = axis_dist(qx, nd->cx - h, nd->cx + h);
-  float dy = axis_dist(qy, nd->cy - h, nd->cy + h);
-  float dz = axis_dist(qz, nd->cz - h, nd->cz + h);
-  return dx * dx + dy * dy + dz * dz > P.r_cut_sq;
-}
-
-enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2 };
-
-// Pair/node contribution to the accumulator.
-template <int KIND>
-__device__ __forceinline__ void contribution(float px, float py, float pz, float m, float qx,
-                                             float qy, float qz, float gx, float gy, float gz,
-                                             const WalkParams& P, float& ax, float& ay,
-                                             float& az) {
-  float dx = min_image_floor(qx - px, P.periodic, P.box, P.inv_box);
-  float dy = min_image_floor(qy - py, P.periodic, P.box, P.inv_box);
-  float dz = min_image_floor(qz - pz, P.periodic, P.box, P.inv_box);
-  if constexpr (KIND == kGravGrad) {
-    // gravityGrad (include/kernel.h:89-94)
-    float t = sqrtf(dx * dx + dy * dy + dz * dz) + 1e-1f;
-    float c = -m / (t * t * t);
-    ax += c * dx;
-    ay += c * dy;
-    az += c * dz;
-  } else {
-    float r2 = dx * dx + dy * dy + dz * dz;
-    float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
-    float s3 = inv_r * inv_r * inv_r;
-    float att = 1.0f, cc = 0.0f;
-    if (P.cutoff) {
-      float e, dist;
-      att = attenuation(r2, P.rc, &e, &dist);
-      if constexpr (KIND == kSoftVjp) cc = s3 * dist * e / (kTwoSqrtPi * P.rc * P.rc * P.rc);
-    }
-    if constexpr (KIND == kSoftForce) {
-      // gravityGradSoftCutoff (src/sbh_ffi.cu:142-160)
-      float c = -m * att * s3;
-      ax += c * dx;
-      ay += c * dy;
-      az += c * dz;
-    } else {
-      // gravityGradSoftCutoffVjp (src/sbh_ffi.cu:2062-2090)
-      float s5 = s3 * inv_r * inv_r;
-      float rg = dx * gx + dy * gy + dz * gz;
-      float lin = att * s3;
-      float rad = rg * (3.0f * att * s5 + cc);
-      ax += -m * (lin * gx - rad * dx);
-      ay += -m * (lin * gy - rad * dy);
-      az += -m * (lin * gz - rad * dz);
-    }
-  }
-}
+using o8::WalkParams;
+using o8::use_far_field;
+using o8::should_prune;
+using o8::contribution;
+using o8::kGravGrad;
+using o8::kSoftForce;
+using o8::kSoftVjp;
 
 constexpr int kStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
 
@@ -343,40 +258,8 @@ uint64_t total_nodes8(int max_depth) {
   return total;
 }
 
-int check_walk_common(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
-                      const SbhBuffer* queries, const SbhBuffer* out, int threads) {
-  if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || !buffer_ok(node_bytes, SBH_DTYPE_U8, -1))
-    return fail_invalid("leaf_bytes/node_bytes must be uint8 device buffers");
-  if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf3)) != 0)
-    return fail_invalid("leaf_bytes size mismatch");
-  if (byte_count(node_bytes) % static_cast<int64_t>(sizeof(Node8)) != 0 ||
-      byte_count(node_bytes) == 0)
-    return fail_invalid("node_bytes size mismatch");
-  if (!buffer_ok(queries, SBH_DTYPE_F32, 2)) return fail_invalid("queries must be (M, 3) float32");
-  if (queries->dims[1] != 3) return fail_invalid("queries dim must be 3");
-  if (!buffer_ok(out, SBH_DTYPE_F32, 2) || out->dims[0] != queries->dims[0] || out->dims[1] != 3)
-    return fail_invalid("out shape mismatch");
-  if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
-  return SBH_OK;
-}
-
-WalkParams make_walk_params(float beta, float softening, float cutoff, int prune_enabled,
-                            float prune_mult, int periodic, float box) {
-  WalkParams P;
-  P.beta = beta;
-  float eps = fmaxf(softening, 0.0f);
-  P.eps2 = eps * eps;
-  P.rc = cutoff;
-  P.cutoff = (cutoff > 0.0f && std::isfinite(cutoff)) ? 1 : 0;
-  P.periodic = (periodic != 0 && box > 0.0f) ? 1 : 0;
-  P.box = box;
-  P.inv_box = box > 0.0f ? 1.0f / box : 0.0f;
-  // src/sbh_ffi.cu:3233-3237: prune only with a positive cutoff.
-  P.prune = (prune_enabled != 0 && cutoff > 0.0f) ? 1 : 0;
-  float r_cut = P.prune ? prune_mult * cutoff : 0.0f;
-  P.r_cut_sq = r_cut * r_cut;
-  return P;
-}
+using o8::check_walk_common;
+using o8::make_walk_params;
 
 template <int KIND>
 int launch_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,

@@ -446,3 +446,168 @@ def softened_barnes_hut_force_octree8_recompute_vjp(points, masses, *, beta: flo
               prune_r_cut_mult=float(prune_r_cut_mult), periodic=bool(periodic),
               box_length=float(box_length), threads=int(threads))
     return _Octree8SoftRecompute.apply(p, m, kw, int(max_depth))
+
+
+# ------------------------------------------------------------------ stochastic octree8
+register_stochastic_barnes_hut_force_octree8 = _register
+register_stochastic_barnes_hut_force_octree8_vjp = _register
+register_softened_stochastic_barnes_hut_force_octree8 = _register
+register_softened_stochastic_barnes_hut_force_octree8_vjp = _register
+
+
+def stochastic_barnes_hut_force_octree8(leaf_bytes, node_bytes, queries, *, beta: float = 1.0,
+                                        samples_per_subdomain: int = 1, seed: int = 0,
+                                        periodic: bool = False, box_length: float = 1.0,
+                                        threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1450-1487."""
+    leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _f32(queries)
+    _check_walk(queries, periodic, box_length, threads)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_octree8", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(queries), buf(out), i32(samples_per_subdomain), i32(seed), f32(beta),
+         i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def stochastic_barnes_hut_force_octree8_vjp(leaf_bytes, node_bytes, queries, cotangent, *,
+                                            beta: float = 1.0, samples_per_subdomain: int = 1,
+                                            seed: int = 0, periodic: bool = False,
+                                            box_length: float = 1.0,
+                                            threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1490-1534."""
+    leaf_bytes, node_bytes = _u8(leaf_bytes), _u8(node_bytes)
+    queries, cotangent = _f32(queries), _f32(cotangent)
+    _check_walk(queries, periodic, box_length, threads)
+    if cotangent.shape != queries.shape:
+        raise ValueError("cotangent must have same shape as queries")
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_octree8_vjp", stream_handle(), buf(leaf_bytes),
+         buf(node_bytes), buf(queries), buf(cotangent), buf(out), i32(samples_per_subdomain),
+         i32(seed), f32(beta), i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def softened_stochastic_barnes_hut_force_octree8(leaf_bytes, node_bytes, queries, *,
+                                                 beta: float = 1.0,
+                                                 samples_per_subdomain: int = 1, seed: int = 0,
+                                                 softening_scale: float = 1e-2,
+                                                 cutoff_scale: float = 1.0,
+                                                 prune_enabled: bool = True,
+                                                 prune_r_cut_mult: float = 4.0,
+                                                 periodic: bool = False,
+                                                 box_length: float = 1.0,
+                                                 threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1537-1588."""
+    leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _f32(queries)
+    _check_walk(queries, periodic, box_length, threads)
+    _check_soft(softening_scale, cutoff_scale, prune_enabled, prune_r_cut_mult)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_octree8_soft", stream_handle(), buf(leaf_bytes),
+         buf(node_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
+         f32(beta), f32(softening_scale), f32(cutoff_scale), i32(1 if prune_enabled else 0),
+         f32(prune_r_cut_mult), i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def softened_stochastic_barnes_hut_force_octree8_vjp(leaf_bytes, node_bytes, queries,
+                                                     cotangent, *, beta: float = 1.0,
+                                                     samples_per_subdomain: int = 1,
+                                                     seed: int = 0,
+                                                     softening_scale: float = 1e-2,
+                                                     cutoff_scale: float = 1.0,
+                                                     prune_enabled: bool = True,
+                                                     prune_r_cut_mult: float = 4.0,
+                                                     periodic: bool = False,
+                                                     box_length: float = 1.0,
+                                                     threads: int = 256,
+                                                     use_ffi: bool = False) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1591-1671: use_ffi=False is the reference's
+    elementwise central difference of the forward call (6*M calls)."""
+    kw = dict(beta=beta, samples_per_subdomain=samples_per_subdomain, seed=seed,
+              softening_scale=softening_scale, cutoff_scale=cutoff_scale,
+              prune_enabled=prune_enabled, prune_r_cut_mult=prune_r_cut_mult,
+              periodic=periodic, box_length=box_length, threads=threads)
+    if not use_ffi:
+        return _force_vjp_finite_difference_elementwise(
+            lambda q: softened_stochastic_barnes_hut_force_octree8(leaf_bytes, node_bytes, q,
+                                                                   **kw),
+            queries, cotangent)
+    leaf_bytes, node_bytes = _u8(leaf_bytes), _u8(node_bytes)
+    queries, cotangent = _f32(queries), _f32(cotangent)
+    _check_walk(queries, periodic, box_length, threads)
+    if cotangent.shape != queries.shape:
+        raise ValueError("cotangent must have same shape as queries")
+    _check_soft(softening_scale, cutoff_scale, prune_enabled, prune_r_cut_mult)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_octree8_soft_vjp", stream_handle(), buf(leaf_bytes),
+         buf(node_bytes), buf(queries), buf(cotangent), buf(out), i32(samples_per_subdomain),
+         i32(seed), f32(beta), f32(softening_scale), f32(cutoff_scale),
+         i32(1 if prune_enabled else 0), f32(prune_r_cut_mult), i32(1 if periodic else 0),
+         f32(box_length), i32(threads))
+    return out
+
+
+class _StochasticRecompute(torch.autograd.Function):
+    """custom_vjp of python/jax_sbh_ffi.py:1900-2046 and 2049-2248: forward
+    builds and samples; backward rebuilds and runs the VJP walk with the same
+    seed (query side; zero mass gradient, as the reference)."""
+
+    @staticmethod
+    def forward(ctx, points, masses, kw, max_depth, softened):
+        ctx.save_for_backward(points, masses)
+        ctx.kw, ctx.max_depth, ctx.softened = kw, max_depth, softened
+        leaf, node = build_octree8_buffers(points, torch.zeros_like(points), masses,
+                                           max_depth=max_depth)
+        if softened:
+            return softened_stochastic_barnes_hut_force_octree8(leaf, node, points, **kw)
+        return stochastic_barnes_hut_force_octree8(leaf, node, points, **kw)
+
+    @staticmethod
+    def backward(ctx, cot):
+        points, masses = ctx.saved_tensors
+        leaf, node = build_octree8_buffers(points, torch.zeros_like(points), masses,
+                                           max_depth=ctx.max_depth)
+        if ctx.softened:
+            gx = softened_stochastic_barnes_hut_force_octree8_vjp(
+                leaf, node, points, cot.contiguous(), use_ffi=True, **ctx.kw)
+        else:
+            gx = stochastic_barnes_hut_force_octree8_vjp(leaf, node, points, cot.contiguous(),
+                                                         **ctx.kw)
+        return gx, torch.zeros_like(masses), None, None, None
+
+
+def stochastic_barnes_hut_force_octree8_recompute_vjp(points, masses, *, beta: float = 1.0,
+                                                      samples_per_subdomain: int = 1,
+                                                      seed: int = 0, max_depth: int = 7,
+                                                      periodic: bool = False,
+                                                      box_length: float = 1.0,
+                                                      threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1926-1946."""
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    p = points if isinstance(points, torch.Tensor) and points.is_cuda else _f32(points)
+    m = masses if isinstance(masses, torch.Tensor) and masses.is_cuda else _f32(masses)
+    kw = dict(beta=float(beta), samples_per_subdomain=int(samples_per_subdomain),
+              seed=int(seed), periodic=bool(periodic), box_length=float(box_length),
+              threads=int(threads))
+    return _StochasticRecompute.apply(p, m, kw, int(max_depth), False)
+
+
+def softened_stochastic_barnes_hut_force_octree8_recompute_vjp(
+        points, masses, *, beta: float = 1.0, samples_per_subdomain: int = 1, seed: int = 0,
+        softening_scale: float = 1e-2, cutoff_scale: float = 1.0, max_depth: int = 7,
+        prune_enabled: bool = True, prune_r_cut_mult: float = 4.0, periodic: bool = False,
+        box_length: float = 1.0, threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:2079-2113."""
+    if prune_enabled and prune_r_cut_mult <= 0:
+        raise ValueError("prune_r_cut_mult must be > 0 when prune_enabled=True")
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    p = points if isinstance(points, torch.Tensor) and points.is_cuda else _f32(points)
+    m = masses if isinstance(masses, torch.Tensor) and masses.is_cuda else _f32(masses)
+    kw = dict(beta=float(beta), samples_per_subdomain=int(samples_per_subdomain),
+              seed=int(seed), softening_scale=float(softening_scale),
+              cutoff_scale=float(cutoff_scale), prune_enabled=bool(prune_enabled),
+              prune_r_cut_mult=float(prune_r_cut_mult), periodic=bool(periodic),
+              box_length=float(box_length), threads=int(threads))
+    return _StochasticRecompute.apply(p, m, kw, int(max_depth), True)

@@ -146,6 +146,60 @@ int sbh_barnes_hut_force_octree8_soft_vjp(
     int32_t prune_enabled, float prune_r_cut_mult, int32_t periodic,
     float box_length, int32_t threads);
 
+/* ---- stochastic walks over the dense octree8 ------------------------------
+ * Multi-level prefix control variate (src/sbh_ffi.cu:2146-2442): subdomains
+ * are the nodes at depth 3 (or shallower leaves); each contributes its
+ * monopole plus samples_per_subdomain Russian-roulette correction paths.
+ * Random numbers: Philox4x32-10 keyed by seed and indexed by (query,
+ * subdomain node, sample, step), in place of the reference's cuRAND XORWOW
+ * streams (src/sbh_ffi.cu:2160-2163); results are therefore statistically,
+ * not bitwise, equivalent to the reference, and deterministic per seed. */
+
+/* Replaces StochasticBarnesHutForceOctree8 (src/sbh_ffi.cu:3322-3373; target
+ * "sbh_stochastic_bh_force_octree8", python/jax_sbh_ffi.py:1450-1487). */
+int sbh_stochastic_bh_force_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                    const SbhBuffer* node_bytes,
+                                    const SbhBuffer* queries, SbhBuffer* out,
+                                    int32_t samples_per_subdomain, int32_t seed,
+                                    float beta, int32_t periodic,
+                                    float box_length, int32_t threads);
+
+/* Replaces SoftenedStochasticBarnesHutForceOctree8 (src/sbh_ffi.cu:3375-3443;
+ * target "sbh_stochastic_bh_force_octree8_soft",
+ * python/jax_sbh_ffi.py:1537-1588). */
+int sbh_stochastic_bh_force_octree8_soft(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, SbhBuffer* out, int32_t samples_per_subdomain,
+    int32_t seed, float beta, float softening_scale, float cutoff_scale,
+    int32_t prune_enabled, float prune_r_cut_mult, int32_t periodic,
+    float box_length, int32_t threads);
+
+/* Replaces StochasticBarnesHutForceOctree8Vjp (src/sbh_ffi.cu:3445-3503;
+ * target "sbh_stochastic_bh_force_octree8_vjp",
+ * python/jax_sbh_ffi.py:1490-1534).  Query-side J^T g along the same sampled
+ * paths as the forward call with the same seed. */
+int sbh_stochastic_bh_force_octree8_vjp(void* stream,
+                                        const SbhBuffer* leaf_bytes,
+                                        const SbhBuffer* node_bytes,
+                                        const SbhBuffer* queries,
+                                        const SbhBuffer* cotangent,
+                                        SbhBuffer* out,
+                                        int32_t samples_per_subdomain,
+                                        int32_t seed, float beta,
+                                        int32_t periodic, float box_length,
+                                        int32_t threads);
+
+/* Replaces SoftenedStochasticBarnesHutForceOctree8Vjp
+ * (src/sbh_ffi.cu:3505-3576; target "sbh_stochastic_bh_force_octree8_soft_vjp",
+ * python/jax_sbh_ffi.py:1591-1671 with use_ffi=True). */
+int sbh_stochastic_bh_force_octree8_soft_vjp(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* queries, const SbhBuffer* cotangent, SbhBuffer* out,
+    int32_t samples_per_subdomain, int32_t seed, float beta,
+    float softening_scale, float cutoff_scale, int32_t prune_enabled,
+    float prune_r_cut_mult, int32_t periodic, float box_length,
+    int32_t threads);
+
 /* ---- adaptive Morton octree pipeline (new targets) ----------------------
  *
  * No reference counterpart: the reference's octree8 is dense (8^D cells,

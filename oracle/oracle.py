@@ -94,6 +94,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_bh_octree8.argtypes = [_u8p, _u8p, _f32p, _vp, _i64, ctypes.c_int, ctypes.c_float,
                                  ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_float,
                                  ctypes.c_int, ctypes.c_float, _f64p, ctypes.POINTER(_i64)]
+    L.orc_sbh_octree8.argtypes = [_u8p, _u8p, _f32p, _vp, _i64, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                  ctypes.c_float, _f64p]
     L.orc_tree_build.restype = _i64
     L.orc_tree_build.argtypes = [_u32p, _i64, ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_float,
                                  _i64, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f32p]
@@ -246,6 +250,20 @@ def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot
                          float(prune_r_cut_mult), int(bool(periodic)), float(box), out,
                          ctypes.byref(cnt))
     return out, int(cnt.value)
+
+
+def sbh_octree8(leaf_bytes, node_bytes, queries, kind, samples, seed, beta, eps=0.0, rc=0.0,
+                cot=None, prune_enabled=False, prune_r_cut_mult=4.0, periodic=False, box=1.0):
+    """Stochastic walk (same Philox sample paths as the GPU kernels)."""
+    leaf = _c(leaf_bytes, np.uint8)
+    node = _c(node_bytes, np.uint8)
+    q = _c(queries, np.float32)
+    g = None if cot is None else _c(cot, np.float32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_sbh_octree8(leaf, node, q, _ptr(g), q.shape[0], int(kind), int(samples), int(seed),
+                          float(beta), float(eps), float(rc), int(bool(prune_enabled)),
+                          float(prune_r_cut_mult), int(bool(periodic)), float(box), out)
+    return out
 
 
 # ---------------------------------------------------------------- adaptive tree

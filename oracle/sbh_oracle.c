@@ -656,6 +656,150 @@ ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_by
 }
 
 /* ------------------------------------------------------------------------- */
+/* S1/S2: stochastic multi-level control-variate walk (src/sbh_ffi.cu:2146-2442) */
+/* ------------------------------------------------------------------------- */
+ORC_EXPORT void orc_philox4x32(const uint32_t* ctr_in, const uint32_t* key_in,
+                               uint32_t* out);
+
+/* Uniform in (0, 1] from one Philox draw indexed by (query, node, sample,
+ * step); replaces curand_uniform on the XORWOW streams seeded with seed + q
+ * (src/sbh_ffi.cu:2160-2163).  P(u > t) = 1 - t for t a multiple of 2^-24. */
+static inline float orc_sbh_uniform(uint32_t seed, uint32_t q, uint32_t node, uint32_t sample,
+                                    uint32_t step) {
+  uint32_t ctr[4] = {q, node, sample, step};
+  uint32_t key[2] = {seed, 0x5BD1E995u};
+  uint32_t out[4];
+  orc_philox4x32(ctr, key, out);
+  return (float)((out[0] >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+static inline uint32_t orc_child_slot8(const OrcNode8* nd, const float* p) {
+  uint32_t ix = p[0] >= nd->cx ? 1u : 0u;
+  uint32_t iy = p[1] >= nd->cy ? 1u : 0u;
+  uint32_t iz = p[2] >= nd->cz ? 1u : 0u;
+  return ix | (iy << 1) | (iz << 2);
+}
+
+/* Distance from q to the cell centre (min-image, kernel.h:27-39), fp32. */
+static inline float orc_centre_dist8(const OrcNode8* nd, const float* q, int per, float box) {
+  float dx = orc_mi_f(q[0] - nd->cx, per, box);
+  float dy = orc_mi_f(q[1] - nd->cy, per, box);
+  float dz = orc_mi_f(q[2] - nd->cz, per, box);
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+/* kind as orc_bh_octree8 (0 grav, 1 soft, 2 soft vjp, 3 grav vjp); the
+ * softened kinds apply the prune test like the reference (softened = true).
+ * Subdomains: nodes at depth 3 or leaves above it (kControlDepth = 3,
+ * src/sbh_ffi.cu:2167-2192), in the reference's push/pop order. */
+ORC_EXPORT void orc_sbh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_bytes,
+                                const float* queries, const float* cot, int64_t nq, int kind,
+                                int samples, int seed, float beta, double eps, double rc,
+                                int prune_enabled, float prune_r_cut_mult, int periodic,
+                                float box, double* out) {
+  const OrcLeaf3* leaf = (const OrcLeaf3*)leaf_bytes;
+  const OrcNode8* nodes = (const OrcNode8*)node_bytes;
+  OrcParams P;
+  orc_params(&P, eps, rc, periodic, box, 0);
+  int per = periodic != 0 && box > 0.0f;
+  int softened = (kind == 1 || kind == 2);
+  int prune_on = (softened && prune_enabled != 0 && rc > 0.0) ? 1 : 0;
+  float r_cut = prune_on ? prune_r_cut_mult * (float)rc : 0.0f;
+  float r_cut_sq = r_cut * r_cut;
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < nq; i++) {
+    float qf[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double q[3] = {qf[0], qf[1], qf[2]};
+    double g[3] = {0, 0, 0};
+    if (cot) {
+      g[0] = cot[3 * i];
+      g[1] = cot[3 * i + 1];
+      g[2] = cot[3 * i + 2];
+    }
+#define ORC_TERM(src, acc)                                                 \
+  do {                                                                     \
+    double p_[3] = {(src)->pos[0], (src)->pos[1], (src)->pos[2]};          \
+    switch (kind) {                                                        \
+      case 0: orc_grav_grad(q, p_, (src)->mass, &P, acc); break;           \
+      case 1: orc_soft_force(q, p_, (src)->mass, &P, acc); break;          \
+      case 2: orc_soft_force_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
+      default: orc_grav_grad_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
+    }                                                                      \
+  } while (0)
+    double total[3] = {0, 0, 0};
+    int stack[64];
+    int top = 0;
+    stack[top++] = 0;
+    while (top > 0) {
+      int idx = stack[--top];
+      const OrcNode8* nd = &nodes[idx];
+      if (nd->contrib.mass == 0.0f) continue;
+      if (!(nd->is_leaf || nd->depth >= 3)) {
+        for (int c = 0; c < 8; c++)
+          if (nd->child_mask & (1u << c)) stack[top++] = (int)nd->child[c];
+        continue;
+      }
+      const OrcNode8* sub = nd;
+      if (prune_on && orc_should_prune8(sub, qf, 1, r_cut_sq, per, box)) continue;
+      if (sub->is_leaf) {
+        for (uint32_t j = sub->start; j < sub->end; j++) ORC_TERM(&leaf[j], total);
+      } else {
+        ORC_TERM(&sub->contrib, total);
+      }
+      uint32_t num_el = sub->end - sub->start;
+      double sub_total[3] = {0, 0, 0};
+      for (int ss = 0; ss < samples; ss++) {
+        float u = orc_sbh_uniform((uint32_t)seed, (uint32_t)i, (uint32_t)idx, (uint32_t)ss, 0);
+        uint32_t sidx = (uint32_t)((float)num_el * u) + sub->start;
+        if (sidx > sub->end - 1) sidx = sub->end - 1;
+        const float* spos = leaf[sidx].pos;
+        const OrcNode8* prev = sub;
+        const OrcNode8* cur = sub;
+        float event_prob = 1.0f;
+        double wsum[3] = {0, 0, 0};
+        float cur_beta = fmaxf(beta, orc_centre_dist8(cur, qf, per, box) / cur->diameter);
+        uint32_t step = 1;
+        while (!cur->is_leaf) {
+          if (prune_on && orc_should_prune8(cur, qf, 1, r_cut_sq, per, box)) break;
+          float ratio = orc_centre_dist8(cur, qf, per, box) / cur->diameter;
+          float tp = fminf(cur_beta / ratio, 1.0f);
+          float pu = orc_sbh_uniform((uint32_t)seed, (uint32_t)i, (uint32_t)idx, (uint32_t)ss,
+                                     step++);
+          if (pu > tp) break;
+          event_prob *= tp;
+          prev = cur;
+          uint32_t slot = orc_child_slot8(cur, spos);
+          if (!(cur->child_mask & (1u << slot))) break;
+          cur = &nodes[cur->child[slot]];
+          cur_beta = fmaxf(cur_beta, ratio);
+          double prev_r[3] = {0, 0, 0}, cur_r[3] = {0, 0, 0};
+          ORC_TERM(&prev->contrib, prev_r);
+          for (int cc = 0; cc < 8; cc++) {
+            if (!(prev->child_mask & (1u << cc))) continue;
+            const OrcNode8* ch = &nodes[prev->child[cc]];
+            if (prune_on && orc_should_prune8(ch, qf, 1, r_cut_sq, per, box)) continue;
+            if (ch->is_leaf) {
+              for (uint32_t j = ch->start; j < ch->end; j++) ORC_TERM(&leaf[j], cur_r);
+            } else {
+              ORC_TERM(&ch->contrib, cur_r);
+            }
+          }
+          float prob_inv = (float)num_el / (event_prob * (float)(prev->end - prev->start));
+          for (int a = 0; a < 3; a++) wsum[a] += (double)prob_inv * (cur_r[a] - prev_r[a]);
+        }
+        for (int a = 0; a < 3; a++) sub_total[a] += wsum[a];
+      }
+      if (samples > 0)
+        for (int a = 0; a < 3; a++) total[a] += sub_total[a] / (double)samples;
+    }
+#undef ORC_TERM
+    out[3 * i] = total[0];
+    out[3 * i + 1] = total[1];
+    out[3 * i + 2] = total[2];
+  }
+}
+
+/* ------------------------------------------------------------------------- */
 /* Adaptive Morton octree (new; no reference counterpart beyond the octree8    */
 /* quantiser and geometry it shares).                                         */
 /*   - codes: orc_morton_codes at depth D, stable order by (code, index)      */

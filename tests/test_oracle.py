@@ -585,3 +585,98 @@ def test_tree_adjoint_matches_central_differences(use_quad):
         mm[k] -= h * m0[k]
         fd = (loss(x0, mp) - loss(x0, mm)) / (2 * h * m0[k])
         assert abs(r["gsm"][k] - fd) <= 2e-6 * scale_m, (k, r["gsm"][k], fd)
+
+
+# --------------------------------------------------------------------- stochastic octree8
+def _subdomain_proxies(leaf, node):
+    """Sources the samples=0 estimator sees: depth-3 cell monopoles and the
+    points of leaves at depth <= 3 (src/sbh_ffi.cu:2167-2200)."""
+    nd, lf = O.decode_nodes8(node), O.decode_leaf3(leaf)
+    pos, mass = [], []
+    for k in range(nd.shape[0]):
+        c = nd[k]["contrib"]
+        if c["mass"] == 0.0 or nd[k]["depth"] > 3:
+            continue
+        if nd[k]["is_leaf"]:
+            s, e = nd[k]["start"], nd[k]["end"]
+            pos.append(lf["pos"][s:e])
+            mass.append(lf["mass"][s:e])
+        elif nd[k]["depth"] == 3:
+            pos.append(c["pos"][None])
+            mass.append(np.array([c["mass"]]))
+    return np.concatenate(pos).astype(np.float32), np.concatenate(mass).astype(np.float32)
+
+
+def test_sbh_shallow_tree_is_direct_sum():
+    # max_depth 3: every subdomain is a leaf, so no sample path has a step
+    p, m, leaf, node = _octree8_case(seed=21, n=300, D=3)
+    q = p[:40]
+    for seed in (0, 9):
+        F = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 4, seed, 1.0, eps=0.03, rc=0.6)
+        np.testing.assert_allclose(F, O.direct_force_soft(p, m, q, 0.03, 0.6), rtol=1e-10,
+                                   atol=1e-13)
+
+
+def test_sbh_zero_samples_is_subdomain_monopole_sum():
+    p, m, leaf, node = _octree8_case(seed=22, n=600, D=6)
+    q = p[:30]
+    pp, pm = _subdomain_proxies(leaf, node)
+    assert pm.shape[0] < p.shape[0]
+    F = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 0, 5, 1.0, eps=0.04, rc=np.inf)
+    np.testing.assert_allclose(F, O.direct_force_soft(pp, pm, q, 0.04, np.inf), rtol=1e-9,
+                               atol=1e-12)
+
+
+@pytest.mark.parametrize("kind", [O.KIND_SOFT, O.KIND_GRAV, O.KIND_SOFT_VJP, O.KIND_GRAV_VJP])
+def test_sbh_estimator_is_unbiased(kind):
+    """Mean over independent seeds converges to the exact sum: a wrong path
+    probability, child slot or weight biases it by many standard errors."""
+    p, m, leaf, node = _octree8_case(seed=5, n=400, D=6)
+    q = p[:24]
+    c = _rng(6).normal(size=q.shape).astype(np.float32)
+    eps, rc = (0.03, 0.9) if kind in (O.KIND_SOFT, O.KIND_SOFT_VJP) else (0.0, 0.0)
+    vjp = kind in (O.KIND_SOFT_VJP, O.KIND_GRAV_VJP)
+    exact, _ = O.bh_octree8(leaf, node, q, kind, 1e30, eps=eps, rc=rc, cot=c if vjp else None)
+    runs = np.stack([O.sbh_octree8(leaf, node, q, kind, 48, s, 1.0, eps=eps, rc=rc,
+                                   cot=c if vjp else None) for s in range(48)])
+    mean = runs.mean(0)
+    se = runs.std(0, ddof=1) / np.sqrt(runs.shape[0])
+    z = (mean - exact) / se
+    assert (z ** 2).mean() < 2.0
+    assert np.abs(z).max() < 5.0
+    # and a single run is a genuine estimate (not the exact sum)
+    assert np.abs(runs[0] - exact).max() > 0
+
+
+def test_sbh_vjp_is_derivative_of_same_sample_paths():
+    # beta huge: every path descends to its leaf with probability 1, so the
+    # weights are constant in q and the VJP is the exact derivative of the
+    # forward on the same Philox draws.
+    p, m, leaf, node = _octree8_case(seed=23, n=300, D=5)
+    q = p[:12].astype(np.float64) + 0.01
+    c = _rng(24).normal(size=q.shape).astype(np.float32)
+    G = O.sbh_octree8(leaf, node, q.astype(np.float32), O.KIND_SOFT_VJP, 2, 3, 1e30, eps=0.1,
+                      rc=np.inf, cot=c)
+    h = 1e-4
+    for i in range(q.shape[0]):
+        for a in range(3):
+            qp, qm = q.copy(), q.copy()
+            qp[i, a] += h
+            qm[i, a] -= h
+            fp = O.sbh_octree8(leaf, node, qp.astype(np.float32), O.KIND_SOFT, 2, 3, 1e30,
+                               eps=0.1, rc=np.inf)[i]
+            fm = O.sbh_octree8(leaf, node, qm.astype(np.float32), O.KIND_SOFT, 2, 3, 1e30,
+                               eps=0.1, rc=np.inf)[i]
+            step = float(qp.astype(np.float32)[i, a]) - float(qm.astype(np.float32)[i, a])
+            fd = (fp - fm) @ c[i].astype(np.float64) / step
+            assert abs(G[i, a] - fd) <= 1e-3 * (1 + abs(fd)), (i, a, G[i, a], fd)
+
+
+def test_sbh_prune_huge_cutoff_matches_disabled():
+    p, m, leaf, node = _octree8_case(seed=11, n=320)
+    q = p[:96]
+    a = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 3, 11, 1.0, eps=0.05, rc=0.8,
+                      prune_enabled=False, periodic=True, box=2.0)
+    b = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 3, 11, 1.0, eps=0.05, rc=0.8,
+                      prune_enabled=True, prune_r_cut_mult=1e6, periodic=True, box=2.0)
+    np.testing.assert_array_equal(a, b)
