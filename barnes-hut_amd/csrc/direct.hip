@@ -256,6 +256,48 @@ __global__ __launch_bounds__(kBlock) void k_bruteforce_gravity(
   if (i < nq) out[i] = acc;
 }
 
+// Winding-number brute force (src/sbh_gpu.cu:53-64 with brute_force_gpu.h:18-66):
+// per query, the compensated (Kahan) sum of signedSolidAngle<3> over all
+// oriented area elements, staged through LDS one tile at a time.
+__global__ __launch_bounds__(kBlock) void k_bruteforce_wn(
+    const float* __restrict__ pts, const float* __restrict__ nrm,
+    const float* __restrict__ area, int n, const float* __restrict__ qs, int nq,
+    float* __restrict__ out) {
+  __shared__ float4 tp[kBlock];
+  __shared__ float4 tn[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  float qx = 0.f, qy = 0.f, qz = 0.f;
+  if (i < nq) {
+    qx = qs[3 * static_cast<int64_t>(i)];
+    qy = qs[3 * static_cast<int64_t>(i) + 1];
+    qz = qs[3 * static_cast<int64_t>(i) + 2];
+  }
+  float total = 0.0f, comp = 0.0f;
+  for (int base = 0; base < n; base += kBlock) {
+    const int j = base + threadIdx.x;
+    if (j < n) {
+      const int64_t o = 3 * static_cast<int64_t>(j);
+      tp[threadIdx.x] = make_float4(pts[o], pts[o + 1], pts[o + 2], area[j]);
+      tn[threadIdx.x] = make_float4(nrm[o], nrm[o + 1], nrm[o + 2], 0.f);
+    }
+    __syncthreads();
+    const int cnt = min(kBlock, n - base);
+    for (int k = 0; k < cnt; ++k) {
+      const float4 p = tp[k], nv = tn[k];
+      const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+      const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      const float v = p.w * (nv.x * dx + nv.y * dy + nv.z * dz) * 0.07957747154594767f /
+                      (r * r * r + 1e-5f);
+      const float y = v - comp;
+      const float t = total + y;
+      comp = (t - total) - y;
+      total = t;
+    }
+    __syncthreads();
+  }
+  if (i < nq) out[i] = total;
+}
+
 PairParams make_params(float softening, float cutoff, int periodic, float box) {
   PairParams P;
   float eps = fmaxf(softening, 0.0f);
@@ -318,6 +360,34 @@ extern "C" int sbh_bruteforce_gravity(void* stream, const SbhBuffer* points,
       static_cast<const float*>(points->data), static_cast<const float*>(masses->data),
       static_cast<int>(n), static_cast<const float*>(queries->data), static_cast<int>(nq),
       static_cast<int>(dim), static_cast<float*>(out->data), per, box_length);
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+extern "C" int sbh_bruteforce_wn(void* stream, const SbhBuffer* points,
+                                 const SbhBuffer* normals, const SbhBuffer* areas,
+                                 const SbhBuffer* queries, SbhBuffer* out, int32_t threads) {
+  clear_error();
+  if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
+    return fail_invalid("points must have shape (N, 3)");
+  const int64_t n = points->dims[0];
+  if (!buffer_ok(normals, SBH_DTYPE_F32, 2) || normals->dims[0] != n || normals->dims[1] != 3)
+    return fail_invalid("normals must have shape (N, 3)");
+  if (!buffer_ok(areas, SBH_DTYPE_F32, 1) || areas->dims[0] != n)
+    return fail_invalid("areas must have shape (N,)");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2) || queries->dims[1] != 3)
+    return fail_invalid("queries must have shape (M, 3)");
+  const int64_t nq = queries->dims[0];
+  if (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != nq)
+    return fail_invalid("out must have shape (M,)");
+  if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
+  if (n > INT32_MAX || nq > INT32_MAX) return fail_invalid("too many points");
+  if (nq == 0) return SBH_OK;
+  k_bruteforce_wn<<<grid_for(nq, kBlock), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const float*>(points->data), static_cast<const float*>(normals->data),
+      static_cast<const float*>(areas->data), static_cast<int>(n),
+      static_cast<const float*>(queries->data), static_cast<int>(nq),
+      static_cast<float*>(out->data));
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }

@@ -1,0 +1,154 @@
+// Wave-coherent group traversal of the adaptive octree, shared by the
+// gravity walk (tree.hip) and the winding-number walk (wn.hip).
+//
+// One wavefront owns a group of up to 64 queries with bounding box
+// [lo, hi].  A node is accepted as a multipole for the whole group iff the
+// distance from its cell centre to the box is >= beta * cell diagonal (the
+// reference's MAC, src/sbh_ffi.cu:1878-1893, made conservative for a group);
+// rejected leaves are summed exactly.  Lanes test up to 64 stack entries at
+// once; accepted nodes and near leaves are appended to per-wave LDS lists and
+// handed to the caller's batch evaluators.
+#pragma once
+
+#include "common.hpp"
+#include "tree.hpp"
+
+namespace sbh {
+namespace tree {
+namespace gw {
+
+constexpr int kB = 256;
+constexpr int kWavesPerBlock = kB / 64;
+constexpr int kStackSoft = 1024;
+constexpr int kStackCap = kStackSoft + 8 * 24;
+constexpr int kFarCap = 128;
+constexpr int kNearFlush = 16;
+constexpr int kNearCap = kNearFlush + 64;
+constexpr int kChunk = 8;  // nodes / sources evaluated per batch
+
+// Wave-uniform reads through the scalar data cache: the arrays are not
+// written while a walk runs, so they may be read as constant memory.
+typedef __attribute__((address_space(4))) const float cfloat;
+typedef __attribute__((address_space(4))) const int cint;
+
+__device__ __forceinline__ float4 sload4(const float4* p, int i) {
+  cfloat* q = (cfloat*)(p) + 4 * static_cast<int64_t>(i);  // generic -> constant space
+  return make_float4(q[0], q[1], q[2], q[3]);
+}
+__device__ __forceinline__ int4 sload4(const int4* p, int i) {
+  cint* q = (cint*)(p) + 4 * static_cast<int64_t>(i);
+  return make_int4(q[0], q[1], q[2], q[3]);
+}
+
+// 1/sqrt(x) with the hardware instruction; x >= 1e-12 wherever it is used,
+// far from the denormal range that rsqrtf's extra scaling handles.
+__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+
+__device__ __forceinline__ float wave_min(float v) {
+  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d, 64);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+// Group MAC, fp32 without contraction (identical expression in the oracle).
+__device__ __forceinline__ bool group_accept(float lx, float ly, float lz, float hx, float hy,
+                                             float hz, float4 g, float beta) {
+#pragma clang fp contract(off)
+  float dx = fmaxf(fmaxf(lx - g.x, g.x - hx), 0.0f);
+  float dy = fmaxf(fmaxf(ly - g.y, g.y - hy), 0.0f);
+  float dz = fmaxf(fmaxf(lz - g.z, g.z - hz), 0.0f);
+  float d2 = dx * dx + dy * dy + dz * dz;
+  float t = beta * g.w;
+  return d2 >= t * t;
+}
+
+struct Box {
+  float lx, ly, lz, hx, hy, hz;
+};
+
+// Wave-wide bounding box of the lanes' positions (every lane must hold a
+// position of the group; idle lanes repeat a valid one).
+__device__ __forceinline__ Box wave_box(float x, float y, float z) {
+  return Box{wave_min(x), wave_min(y), wave_min(z), wave_max(x), wave_max(y), wave_max(z)};
+}
+
+// Depth-first group traversal.  far[0..nfar) holds accepted non-empty nodes
+// when flush_far(nfar) is called (at 64 entries and at the end); near[0..nnear)
+// holds rejected non-empty leaves when flush_near(nnear) is called (at
+// kNearFlush entries and at the end).  Both callbacks run on the whole wave.
+template <class FarFn, class NearFn>
+__device__ __forceinline__ void traverse(const Header* hdr, const int4* topo, const float4* geo,
+                                         const float4* mp, float beta, const Box& b, int lane,
+                                         int* stack, int* far, int* near, FarFn&& flush_far,
+                                         NearFn&& flush_near) {
+  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int top = 1, nfar = 0, nnear = 0;
+  if (lane == 0) stack[0] = 0;
+  __builtin_amdgcn_wave_barrier();
+  // Every node is pushed at most once per group and each iteration pops at
+  // least one, so num_nodes iterations bound the loop even for a corrupt tree.
+  const int num_nodes = hdr->num_nodes;
+  int iters = 0;
+  while (top > 0 && iters++ <= num_nodes) {
+    int k = min(64, top);
+    k = min(k, max(1, (kStackSoft - top) / 7));
+    const int from = top - k;
+    const int node = lane < k ? stack[from + lane] : -1;
+    top = from;
+    __builtin_amdgcn_wave_barrier();
+    const bool have = node >= 0 && node < num_nodes;
+    const int nd = have ? node : 0;
+    const float4 m = mp[nd];
+    const float4 g = geo[nd];
+    const int4 t = topo[nd];
+    int kind = 0, nch = 0, fch = 0;
+    if (have && m.w != 0.0f) {
+      if (group_accept(b.lx, b.ly, b.lz, b.hx, b.hy, b.hz, g, beta)) {
+        kind = 1;
+      } else if (t.w == 0) {
+        kind = 2;
+      } else {
+        kind = 3;
+        nch = t.w;
+        fch = t.z;
+      }
+    }
+    const uint64_t bf = __ballot(kind == 1);
+    if (kind == 1) far[nfar + __popcll(bf & lt_mask)] = node;
+    nfar += __popcll(bf);
+    const uint64_t bn = __ballot(kind == 2);
+    if (kind == 2) near[nnear + __popcll(bn & lt_mask)] = node;
+    nnear += __popcll(bn);
+    const int incl = wave_incl_scan(nch, lane);
+    const int total = __shfl(incl, 63, 64);
+    if (top + total > kStackCap) break;  // unreachable for depth <= 10 (see kStackCap)
+    for (int c = 0; c < nch; ++c) stack[top + incl - nch + c] = fch + c;
+    top += total;
+    __builtin_amdgcn_wave_barrier();
+    if (nfar >= 64) {
+      flush_far(nfar);
+      nfar = 0;
+    }
+    if (nnear >= kNearFlush) {
+      flush_near(nnear);
+      nnear = 0;
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (nfar > 0) flush_far(nfar);
+  if (nnear > 0) flush_near(nnear);
+}
+
+}  // namespace gw
+}  // namespace tree
+}  // namespace sbh

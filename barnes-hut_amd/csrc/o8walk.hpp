@@ -72,7 +72,20 @@ __device__ __forceinline__ uint32_t child_slot(const Node8* __restrict__ nd, flo
   return ix | (iy << 1) | (iz << 2);
 }
 
-enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2, kGravVjp = 3 };
+enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2, kGravVjp = 3, kWinding = 4 };
+
+constexpr float kInvFourPi = 0.07957747154594767f;
+
+// signedSolidAngle<3> (include/kernel.h:73-80): a n.(p - q) / (4 pi (|p - q|^3 + 1e-5)).
+__device__ __forceinline__ float wn_pair(const Leaf3& L, float qx, float qy, float qz,
+                                         const WalkParams& P) {
+  float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
+  float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
+  float dz = min_image_floor(L.pos[2] - qz, P.periodic, P.box, P.inv_box);
+  float r = sqrtf(dx * dx + dy * dy + dz * dz);
+  float nd = L.normal[0] * dx + L.normal[1] * dy + L.normal[2] * dz;
+  return L.mass * nd * kInvFourPi / (r * r * r + 1e-5f);
+}
 
 // Pair/node contribution to the accumulator.
 template <int KIND>
@@ -135,8 +148,12 @@ template <int KIND>
 __device__ __forceinline__ void contribution(const Leaf3& L, float qx, float qy, float qz,
                                              float gx, float gy, float gz, const WalkParams& P,
                                              float& ax, float& ay, float& az) {
-  contribution<KIND>(L.pos[0], L.pos[1], L.pos[2], L.mass, qx, qy, qz, gx, gy, gz, P, ax, ay,
-                     az);
+  if constexpr (KIND == kWinding) {
+    ax += wn_pair(L, qx, qy, qz, P);
+  } else {
+    contribution<KIND>(L.pos[0], L.pos[1], L.pos[2], L.mass, qx, qy, qz, gx, gy, gz, P, ax, ay,
+                       az);
+  }
 }
 
 // Host: walk parameters from the handler attributes (prune radius as
@@ -159,8 +176,10 @@ inline WalkParams make_walk_params(float beta, float softening, float cutoff, in
 }
 
 // Host: shared validation of the walk handlers (src/sbh_ffi.cu:3142-3159).
+// `scalar_out`: the result is one value per query, shape (M,), instead of (M, 3).
 inline int check_walk_common(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
-                             const SbhBuffer* queries, const SbhBuffer* out, int threads) {
+                             const SbhBuffer* queries, const SbhBuffer* out, int threads,
+                             bool scalar_out = false) {
   if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || !buffer_ok(node_bytes, SBH_DTYPE_U8, -1))
     return fail_invalid("leaf_bytes/node_bytes must be uint8 device buffers");
   if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf3)) != 0)
@@ -170,7 +189,9 @@ inline int check_walk_common(const SbhBuffer* leaf_bytes, const SbhBuffer* node_
     return fail_invalid("node_bytes size mismatch");
   if (!buffer_ok(queries, SBH_DTYPE_F32, 2)) return fail_invalid("queries must be (M, 3) float32");
   if (queries->dims[1] != 3) return fail_invalid("queries dim must be 3");
-  if (!buffer_ok(out, SBH_DTYPE_F32, 2) || out->dims[0] != queries->dims[0] || out->dims[1] != 3)
+  if (scalar_out ? (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != queries->dims[0])
+                 : (!buffer_ok(out, SBH_DTYPE_F32, 2) || out->dims[0] != queries->dims[0] ||
+                    out->dims[1] != 3))
     return fail_invalid("out shape mismatch");
   if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
   return SBH_OK;

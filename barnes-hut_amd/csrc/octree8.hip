@@ -198,6 +198,7 @@ using o8::contribution;
 using o8::kGravGrad;
 using o8::kSoftForce;
 using o8::kSoftVjp;
+using o8::kWinding;
 
 constexpr int kStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
 
@@ -224,18 +225,14 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
   while (top > 0) {
     const Node8* nd = &nodes[stack[--top]];
     if (nd->contrib.mass == 0.0f) continue;
-    if (KIND != kGravGrad && should_prune(nd, qx, qy, qz, P)) continue;
+    if (KIND != kGravGrad && KIND != kWinding && should_prune(nd, qx, qy, qz, P)) continue;
     if (use_far_field(nd, qx, qy, qz, P)) {
       if (nd->is_leaf) {
         uint32_t je = nd->end < num_leaves ? nd->end : num_leaves;
-        for (uint32_t j = nd->start; j < je; ++j) {
-          const Leaf3* L = &leaf[j];
-          contribution<KIND>(L->pos[0], L->pos[1], L->pos[2], L->mass, qx, qy, qz, gx, gy, gz,
-                             P, ax, ay, az);
-        }
+        for (uint32_t j = nd->start; j < je; ++j)
+          contribution<KIND>(leaf[j], qx, qy, qz, gx, gy, gz, P, ax, ay, az);
       } else {
-        contribution<KIND>(nd->contrib.pos[0], nd->contrib.pos[1], nd->contrib.pos[2],
-                           nd->contrib.mass, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+        contribution<KIND>(nd->contrib, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
       }
     } else {
       uint32_t mask = nd->child_mask;
@@ -244,9 +241,13 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
           stack[top++] = static_cast<int>(nd->child[c]);
     }
   }
-  out[3 * q + 0] = ax;
-  out[3 * q + 1] = ay;
-  out[3 * q + 2] = az;
+  if constexpr (KIND == kWinding) {
+    out[q] = ax;
+  } else {
+    out[3 * q + 0] = ax;
+    out[3 * q + 1] = ay;
+    out[3 * q + 2] = az;
+  }
 }
 
 uint64_t total_nodes8(int max_depth) {
@@ -413,6 +414,20 @@ extern "C" int sbh_barnes_hut_force_octree8(void* stream, const SbhBuffer* leaf_
   if (rc != SBH_OK) return rc;
   WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, periodic, box_length);
   return launch_walk<kGravGrad>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
+}
+
+// Winding numbers over octree8 records built from oriented area elements
+// (points = element centroids, normals = unit normals, masses = areas): the
+// classic vote walk of src/sbh_gpu.cu:100-120 with signedSolidAngle<3>,
+// include/kernel.h:73-80.  out: (M,).
+extern "C" int sbh_barnes_hut_wn_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                         const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                         SbhBuffer* out, float beta, int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, 0, 1.0f);
+  return launch_walk<o8::kWinding>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
 }
 
 extern "C" int sbh_barnes_hut_force_octree8_soft(

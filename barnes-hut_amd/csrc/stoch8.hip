@@ -129,9 +129,13 @@ __global__ __launch_bounds__(kB) void k_stoch8(const Leaf3* __restrict__ leaf,
       tz += sz * inv;
     }
   }
-  out[3 * q + 0] = tx;
-  out[3 * q + 1] = ty;
-  out[3 * q + 2] = tz;
+  if constexpr (KIND == kWinding) {
+    out[q] = tx;
+  } else {
+    out[3 * q + 0] = tx;
+    out[3 * q + 1] = ty;
+    out[3 * q + 2] = tz;
+  }
 }
 
 template <int KIND>
@@ -227,5 +231,20 @@ extern "C" int sbh_stochastic_bh_force_octree8_soft_vjp(
   o8::WalkParams P = o8::make_walk_params(beta, softening_scale, cutoff_scale, prune_enabled,
                                           prune_r_cut_mult, periodic, box_length);
   return launch<o8::kSoftVjp>(stream, leaf_bytes, node_bytes, queries, cotangent, out, P,
+                              samples_per_subdomain, seed);
+}
+
+// Stochastic winding numbers (MLPCV of src/sbh_gpu.cu:166-187 with
+// signedSolidAngle<3>) over octree8 records of oriented area elements.
+extern "C" int sbh_stochastic_bh_wn_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                            const SbhBuffer* node_bytes,
+                                            const SbhBuffer* queries, SbhBuffer* out,
+                                            int32_t samples_per_subdomain, int32_t seed,
+                                            float beta, int32_t threads) {
+  clear_error();
+  int rc = o8::check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  o8::WalkParams P = o8::make_walk_params(beta, 0.f, 0.f, 0, 0.f, 0, 1.0f);
+  return launch<o8::kWinding>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P,
                               samples_per_subdomain, seed);
 }

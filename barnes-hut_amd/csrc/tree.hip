@@ -21,37 +21,13 @@
 //   the points in k_finalize.
 #include "common.hpp"
 #include "tree.hpp"
+#include "groupwalk.hpp"
 
 namespace sbh {
 namespace tree {
 namespace {
 
-constexpr int kB = 256;
-constexpr int kWavesPerBlock = kB / 64;
-constexpr int kStackSoft = 1024;
-constexpr int kStackCap = kStackSoft + 8 * 24;
-constexpr int kFarCap = 128;
-constexpr int kNearFlush = 16;
-constexpr int kNearCap = kNearFlush + 64;
-constexpr int kChunk = 8;  // nodes / sources evaluated per batch
-
-// Wave-uniform reads through the scalar data cache: the arrays are not
-// written while the walk runs, so they may be read as constant memory.
-typedef __attribute__((address_space(4))) const float cfloat;
-typedef __attribute__((address_space(4))) const int cint;
-
-__device__ __forceinline__ float4 sload4(const float4* p, int i) {
-  cfloat* q = (cfloat*)(p) + 4 * static_cast<int64_t>(i);  // generic -> constant space
-  return make_float4(q[0], q[1], q[2], q[3]);
-}
-__device__ __forceinline__ int4 sload4(const int4* p, int i) {
-  cint* q = (cint*)(p) + 4 * static_cast<int64_t>(i);
-  return make_int4(q[0], q[1], q[2], q[3]);
-}
-
-// 1/sqrt(x) with the hardware instruction; x >= 1e-12 here, far from the
-// denormal range that rsqrtf's extra scaling handles.
-__device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+using namespace gw;
 
 // ------------------------------------------------------------------ bbox
 __global__ __launch_bounds__(kB) void k_bbox_partial(const float* __restrict__ pts, int64_t n,
@@ -83,40 +59,53 @@ __global__ __launch_bounds__(kB) void k_bbox_partial(const float* __restrict__ p
   if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-// Reduces the partials, writes the cube geometry, initialises the root and
-// the zero-mass sentinel node used to pad interaction batches.
-__global__ __launch_bounds__(64) void k_bbox_final(const float* __restrict__ part, int nparts,
-                                                   int64_t n, Header* __restrict__ hdr,
+// Reduces the per-block partials: bounds[0..3) = min - 1e-3, bounds[3..6) =
+// max + 1e-3 (python/jax_sbh_ffi.py:879-884); an empty set gives +-1e-3.
+__global__ __launch_bounds__(kB) void k_bbox_reduce(const float* __restrict__ part, int nparts,
+                                                    int64_t n, float* __restrict__ bounds) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int b = threadIdx.x; b < nparts; b += kB)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], part[6 * b + a]);
+      hi[a] = fmaxf(hi[a], part[6 * b + 3 + a]);
+    }
+  __shared__ float sh[6][kB];
+  for (int a = 0; a < 3; ++a) {
+    sh[a][threadIdx.x] = lo[a];
+    sh[3 + a][threadIdx.x] = hi[a];
+  }
+  __syncthreads();
+  for (int off = kB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(off)) {
+      for (int a = 0; a < 3; ++a) {
+        sh[a][threadIdx.x] = fminf(sh[a][threadIdx.x], sh[a][threadIdx.x + off]);
+        sh[3 + a][threadIdx.x] = fmaxf(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + off]);
+      }
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    float v = n == 0 ? 0.0f : sh[a][0];
+    bounds[a] = a < 3 ? v - 1e-3f : v + 1e-3f;
+  }
+}
+
+// Writes the cube geometry from the bounds (min xyz, max xyz), initialises the
+// root and the zero-mass sentinel node used to pad interaction batches.
+__global__ __launch_bounds__(64) void k_bbox_final(int64_t n, Header* __restrict__ hdr,
                                                    int4* __restrict__ topo,
                                                    uint32_t* __restrict__ prefix,
                                                    float4* __restrict__ geo,
                                                    int32_t* __restrict__ parent,
                                                    float4* __restrict__ mp,
                                                    float4* __restrict__ qd, int sentinel,
-                                                   const float* __restrict__ user_bounds) {
+                                                   const float* __restrict__ bounds) {
   if (threadIdx.x != 0) return;
   float lo[3], hi[3];
-  if (user_bounds != nullptr) {
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = user_bounds[a];
-      hi[a] = user_bounds[3 + a];
-    }
-  } else {
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = INFINITY;
-      hi[a] = -INFINITY;
-    }
-    for (int b = 0; b < nparts; ++b)
-      for (int a = 0; a < 3; ++a) {
-        lo[a] = fminf(lo[a], part[6 * b + a]);
-        hi[a] = fmaxf(hi[a], part[6 * b + 3 + a]);
-      }
-    if (n == 0)
-      for (int a = 0; a < 3; ++a) lo[a] = hi[a] = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-      lo[a] = lo[a] - 1e-3f;
-      hi[a] = hi[a] + 1e-3f;
-    }
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = bounds[a];
+    hi[a] = bounds[3 + a];
   }
   CubeGeom g;
   g.min_x = lo[0];
@@ -472,22 +461,6 @@ __global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hd
 }
 
 // ------------------------------------------------------------------ wave helpers
-__device__ __forceinline__ float wave_min(float v) {
-  for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
-  return v;
-}
-__device__ __forceinline__ float wave_max(float v) {
-  for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
-  return v;
-}
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
-  return v;
-}
-
 // One butterfly stage of a transposed wave reduction: lanes with bit `d` set
 // keep the upper half of the first 2H values (adding the partner's copy),
 // the others keep the lower half.
@@ -546,18 +519,6 @@ __device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
     v[0] += __shfl_xor(v[0], 1, 64);
   }
   return v[0];
-}
-
-// Group MAC, fp32 without contraction (identical expression in the oracle).
-__device__ __forceinline__ bool group_accept(float lx, float ly, float lz, float hx, float hy,
-                                             float hz, float4 g, float beta) {
-#pragma clang fp contract(off)
-  float dx = fmaxf(fmaxf(lx - g.x, g.x - hx), 0.0f);
-  float dy = fmaxf(fmaxf(ly - g.y, g.y - hy), 0.0f);
-  float dz = fmaxf(fmaxf(lz - g.z, g.z - hz), 0.0f);
-  float d2 = dx * dx + dy * dy + dz * dz;
-  float t = beta * g.w;
-  return d2 >= t * t;
 }
 
 struct WalkArgs {
@@ -770,22 +731,15 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       }
     }
   }
-  const float lx = wave_min(L.x), ly = wave_min(L.y), lz = wave_min(L.z);
-  const float hx = wave_max(L.x), hy = wave_max(L.y), hz = wave_max(L.z);
-  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const float beta = A.beta;
+  const Box box = wave_box(L.x, L.y, L.z);
   const int nvalid = cnt;
 
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float gx = 0.f, gy = 0.f, gz = 0.f;
 
-  int top = 1, nfar = 0, nnear = 0;
-  if (lane == 0) stack[0] = 0;
-  __builtin_amdgcn_wave_barrier();
-
   // The quadrupole adjoint holds 12 partials per node: batch 4 nodes, not 8.
   constexpr int C = (VJP && QUAD) ? kChunk / 2 : kChunk;
-  auto flush_far = [&]() {
+  auto flush_far = [&](int nfar) {
     n_far += static_cast<unsigned long long>(nfar) * nvalid;
     const int padded = (nfar + C - 1) & ~(C - 1);
     if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
@@ -793,10 +747,9 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
     for (int f0 = 0; f0 < padded; f0 += C)
       far_batch<VJP, QUAD, FORCES, C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
     __builtin_amdgcn_wave_barrier();
-    nfar = 0;
   };
 
-  auto flush_near = [&]() {
+  auto flush_near = [&](int nnear) {
     for (int b = 0; b < nnear; ++b) {
       const int leaf = __builtin_amdgcn_readfirstlane(near[b]);
       const int4 t = sload4(A.topo, leaf);
@@ -807,55 +760,10 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       for (int j0 = s0; j0 < s1; j0 += kChunk)
         near_batch<VJP, FORCES>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
     }
-    nnear = 0;
   };
 
-  // Every node is pushed at most once per group and each iteration pops at
-  // least one, so num_nodes iterations bound the loop even for a corrupt tree.
-  const int num_nodes = A.hdr->num_nodes;
-  int iters = 0;
-  while (top > 0 && iters++ <= num_nodes) {
-    int k = min(64, top);
-    k = min(k, max(1, (kStackSoft - top) / 7));
-    const int from = top - k;
-    const int node = lane < k ? stack[from + lane] : -1;
-    top = from;
-    __builtin_amdgcn_wave_barrier();
-    const bool have = node >= 0 && node < num_nodes;
-    const int nd = have ? node : 0;
-    const float4 m = A.mp[nd];
-    const float4 g = A.geo[nd];
-    const int4 t = A.topo[nd];
-    int kind = 0, nch = 0, fch = 0;
-    if (have && m.w != 0.0f) {
-      if (group_accept(lx, ly, lz, hx, hy, hz, g, beta)) {
-        kind = 1;
-      } else if (t.w == 0) {
-        kind = 2;
-      } else {
-        kind = 3;
-        nch = t.w;
-        fch = t.z;
-      }
-    }
-    const uint64_t bf = __ballot(kind == 1);
-    if (kind == 1) far[nfar + __popcll(bf & lt_mask)] = node;
-    nfar += __popcll(bf);
-    const uint64_t bn = __ballot(kind == 2);
-    if (kind == 2) near[nnear + __popcll(bn & lt_mask)] = node;
-    nnear += __popcll(bn);
-    const int incl = wave_incl_scan(nch, lane);
-    const int total = __shfl(incl, 63, 64);
-    if (top + total > kStackCap) break;  // unreachable for depth <= 10 (see kStackCap)
-    for (int c = 0; c < nch; ++c) stack[top + incl - nch + c] = fch + c;
-    top += total;
-    __builtin_amdgcn_wave_barrier();
-    if (nfar >= 64) flush_far();
-    if (nnear >= kNearFlush) flush_near();
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (nfar > 0) flush_far();
-  if (nnear > 0) flush_near();
+  traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, stack, far, near, flush_far,
+           flush_near);
 
   if (valid) {
     const int64_t o = static_cast<int64_t>(A.perm[i]);
@@ -1018,7 +926,7 @@ Layout layout(int64_t n, int max_depth, int leaf_size) {
   L.keys_tmp = o; o = align(o + 4 * nn);
   L.vals_tmp = o; o = align(o + 4 * nn);
   L.sort_tmp = o; o = align(o + radix_sort_temp_bytes(n));
-  L.bbox_tmp = o; o = align(o + 6 * 4 * kBboxBlocks);
+  L.bbox_tmp = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
   L.sp = o; o = align(o + 16 * nn);
   L.sc = o; o = align(o + 16 * nn);
   L.topo = o; o = align(o + 16 * cap);
@@ -1036,6 +944,17 @@ Layout layout(int64_t n, int max_depth, int leaf_size) {
   L.goff = o; o = align(o + 4 * cap);
   L.total = o;
   return L;
+}
+
+int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** bounds) {
+  const unsigned nb = static_cast<unsigned>(
+      std::min<int64_t>(kBboxBlocks, std::max<int64_t>(1, (n + kB - 1) / kB)));
+  k_bbox_partial<<<nb, kB, 0, s>>>(pts, n, tmp);
+  SBH_LAUNCH_CHECK();
+  *bounds = tmp + 6 * kBboxBlocks;
+  k_bbox_reduce<<<1, kB, 0, s>>>(tmp, static_cast<int>(nb), n, *bounds);
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
 }
 
 int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
@@ -1057,14 +976,14 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
   auto* sc = reinterpret_cast<float4*>(ws + L.sc);
   auto* bbox = reinterpret_cast<float*>(ws + L.bbox_tmp);
 
-  unsigned nb = static_cast<unsigned>(
-      std::min<int64_t>(kBboxBlocks, std::max<int64_t>(1, (n + kB - 1) / kB)));
   if (user_bounds == nullptr) {
-    k_bbox_partial<<<nb, kB, 0, s>>>(pts, n, bbox);
-    SBH_LAUNCH_CHECK();
+    float* b = nullptr;
+    int rc = data_bounds(s, pts, n, bbox, &b);
+    if (rc != SBH_OK) return rc;
+    user_bounds = b;
   }
-  k_bbox_final<<<1, 64, 0, s>>>(bbox, static_cast<int>(nb), n, hdr, topo, prefix, geo, parent,
-                                mp, qd, static_cast<int>(L.cap), user_bounds);
+  k_bbox_final<<<1, 64, 0, s>>>(n, hdr, topo, prefix, geo, parent, mp, qd,
+                                static_cast<int>(L.cap), user_bounds);
   SBH_LAUNCH_CHECK();
   if (n > 0) {
     k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, hdr, max_depth, keys, vals);

@@ -47,6 +47,11 @@ struct Layout {
 int64_t node_capacity(int64_t n, int max_depth, int leaf_size);
 Layout layout(int64_t n, int max_depth, int leaf_size);
 
+// Data bounds of n points on the device (no host synchronisation): min - 1e-3
+// (3 floats) then max + 1e-3 (3 floats), written to tmp + 6 * kBboxBlocks and
+// returned in *bounds; tmp holds 6 * (kBboxBlocks + 1) floats.
+int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** bounds);
+
 // Build topology and moments.  `cot` may be null (forces only).  `user_bounds`
 // (device, 6 floats: min xyz, max xyz) may be null to use the data bounds
 // expanded by 1e-3.

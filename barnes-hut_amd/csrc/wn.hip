@@ -1,0 +1,386 @@
+// Winding numbers of an oriented-area (dipole) cloud on the adaptive octree.
+//
+// Sources are area elements (centroid p, unit normal n, area a); the value at
+// q is sum a n.(p - q) / (4 pi (|p - q|^3 + 1e-5)) (signedSolidAngle<3>,
+// include/kernel.h:73-80).  The tree is built over the centroids with the
+// areas as weights (tree::build: area-weighted centroid c and area A per
+// node), then per node the dipole D = sum a n, so an accepted node counts as
+// the dipole D at c -- the reference's node record (A, D / A) at c
+// (src/sbh_gpu.cu:100-120 with src/sbh_ffi.cu:1840-1868).  A first-moment
+// (sum a n (p - c)^T) correction was tried and rejected: for smooth surface
+// patches at the MAC distance the next Taylor term is larger and of opposite
+// sign, so the truncated series is less accurate than the dipole alone.
+//
+// Queries are a separate cloud: Morton-sorted in their own cube, cut into
+// runs of 64 (one wavefront each) and walked with the same group MAC and
+// traversal as the gravity walk.
+#include "common.hpp"
+#include "groupwalk.hpp"
+#include "tree.hpp"
+
+namespace sbh {
+namespace tree {
+namespace {
+
+using namespace gw;
+
+constexpr int kQueryBits = 10;  // 30-bit query Morton codes
+constexpr float kInvFourPi = 0.07957747154594767f;
+
+struct WnHeader {
+  int32_t queue;
+  int32_t pad[3];
+  unsigned long long stats[4];  // far interactions, near pairs, groups
+};
+
+struct WnLayout {
+  Layout t;
+  size_t wd;                             // per node: dipole (float4)
+  size_t qhdr, qbbox, qkeys, qvals, qcodes, qperm, qsort, qs;
+  size_t total;
+};
+
+WnLayout wn_layout(int64_t n, int64_t m, int max_depth, int leaf_size) {
+  auto align = [](size_t x) { return (x + 255) & ~size_t{255}; };
+  WnLayout W;
+  W.t = layout(n, max_depth, leaf_size);
+  const size_t cap = static_cast<size_t>(W.t.cap) + 1;
+  const size_t mm = static_cast<size_t>(m > 0 ? m : 1);
+  size_t o = W.t.total;
+  W.wd = o; o = align(o + 16 * cap);
+  W.qhdr = o; o = align(o + sizeof(WnHeader));
+  W.qbbox = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
+  W.qkeys = o; o = align(o + 4 * mm);
+  W.qvals = o; o = align(o + 4 * mm);
+  W.qcodes = o; o = align(o + 4 * mm);
+  W.qperm = o; o = align(o + 4 * mm);
+  W.qsort = o; o = align(o + radix_sort_temp_bytes(m));
+  W.qs = o; o = align(o + 16 * mm);
+  W.total = o;
+  return W;
+}
+
+// ------------------------------------------------------------------ moments
+// Bottom-up, one level per launch (children are finished before parents).
+// The level-0 launch also clears the sentinel node's dipole.
+__global__ __launch_bounds__(kB) void k_wn_moments(const Header* __restrict__ hdr, int level,
+                                                   const int4* __restrict__ topo,
+                                                   const float4* __restrict__ sp,
+                                                   const float4* __restrict__ sn,
+                                                   float4* __restrict__ wd, int sentinel) {
+  if (level == 0 && blockIdx.x == 0 && threadIdx.x == 0)
+    wd[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int start = hdr->level_start[level];
+  const int count = hdr->level_count[level];
+  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
+    const int k = start + u;
+    const int4 t = topo[k];
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    if (t.w == 0) {
+      for (int j = t.x; j < t.x + t.y; ++j) {
+        const float a = sp[j].w;
+        const float4 nv = sn[j];
+        dx += a * nv.x;
+        dy += a * nv.y;
+        dz += a * nv.z;
+      }
+    } else {
+      for (int ch = t.z; ch < t.z + t.w; ++ch) {
+        const float4 dc = wd[ch];
+        dx += dc.x;
+        dy += dc.y;
+        dz += dc.z;
+      }
+    }
+    wd[k] = make_float4(dx, dy, dz, 0.f);
+  }
+}
+
+// ------------------------------------------------------------------ queries
+// Morton codes of the queries in their own cube (bounds from data_bounds).
+__global__ __launch_bounds__(kB) void k_qcodes(const float* __restrict__ q, int64_t m,
+                                               const float* __restrict__ bounds,
+                                               uint32_t* __restrict__ codes,
+                                               uint32_t* __restrict__ idx,
+                                               WnHeader* __restrict__ wh) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (i == 0) {
+    wh->queue = 0;
+    for (int k = 0; k < 4; ++k) wh->stats[k] = 0;
+  }
+  if (i >= m) return;
+  float span;
+  {
+#pragma clang fp contract(off)
+    span = fmaxf(fmaxf(bounds[3] - bounds[0], bounds[4] - bounds[1]), bounds[5] - bounds[2]);
+  }
+  const uint32_t qx = quantize_axis(q[3 * i + 0], bounds[0], span, kQueryBits);
+  const uint32_t qy = quantize_axis(q[3 * i + 1], bounds[1], span, kQueryBits);
+  const uint32_t qz = quantize_axis(q[3 * i + 2], bounds[2], span, kQueryBits);
+  codes[i] = morton3(qx, qy, qz);
+  idx[i] = static_cast<uint32_t>(i);
+}
+
+__global__ __launch_bounds__(kB) void k_qgather(const float* __restrict__ q, int64_t m,
+                                                const uint32_t* __restrict__ perm,
+                                                float4* __restrict__ qs) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (r >= m) return;
+  const int64_t i = perm[r];
+  qs[r] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], 0.f);
+}
+
+// ------------------------------------------------------------------ walk
+struct WnArgs {
+  const Header* hdr;
+  WnHeader* wh;
+  const int4* topo;
+  const float4* geo;
+  const float4* mp;   // centroid, area
+  const float4* wd;   // dipole
+  const float4* sp;   // sorted centroid, area
+  const float4* sn;   // sorted normal
+  const float4* qs;   // sorted queries
+  const uint32_t* qperm;
+  int64_t m;
+  int sentinel;
+  float beta;
+  float* out;
+  bool stats;
+};
+
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// C far nodes (sentinel-padded) against the lane's query.
+template <int C>
+__device__ __forceinline__ float wn_far_batch(const WnArgs& A, float qx, float qy, float qz,
+                                              const int* far) {
+  float w = 0.f;
+#pragma unroll
+  for (int t = 0; t < C; ++t) {
+    const int k = __builtin_amdgcn_readfirstlane(far[t]);
+    const float4 c = sload4(A.mp, k);
+    const float4 d = sload4(A.wd, k);
+    const float Xx = c.x - qx, Xy = c.y - qy, Xz = c.z - qz;
+    const float R2 = Xx * Xx + Xy * Xy + Xz * Xz;
+    const float R = __builtin_sqrtf(R2);
+    w += (d.x * Xx + d.y * Xy + d.z * Xz) * kInvFourPi * rcp(R * R2 + 1e-5f);
+  }
+  return w;
+}
+
+// kChunk consecutive elements of one leaf from j0 (entries at or beyond s1
+// carry zero area).
+__device__ __forceinline__ float wn_near_batch(const WnArgs& A, float qx, float qy, float qz,
+                                               int j0, int s1) {
+  float w = 0.f;
+#pragma unroll
+  for (int u = 0; u < kChunk; ++u) {
+    const int j = j0 + u;
+    const int jj = min(j, s1 - 1);
+    const float4 p = sload4(A.sp, jj);
+    const float4 nv = sload4(A.sn, jj);
+    const float a = (j < s1) ? p.w : 0.0f;
+    const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float r = __builtin_sqrtf(r2);
+    w += a * (nv.x * dx + nv.y * dy + nv.z * dz) * kInvFourPi * rcp(r * r2 + 1e-5f);
+  }
+  return w;
+}
+
+__global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
+  __shared__ int s_stack[kWavesPerBlock][kStackCap];
+  __shared__ int s_far[kWavesPerBlock][kFarCap];
+  __shared__ int s_near[kWavesPerBlock][kNearCap];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  int* stack = s_stack[wv];
+  int* far = s_far[wv];
+  int* near = s_near[wv];
+  const int num_groups = static_cast<int>((A.m + 63) / 64);
+  unsigned long long n_far = 0, n_near = 0, n_groups = 0;
+  while (true) {
+    int g = 0;
+    if (lane == 0) g = atomicAdd(&A.wh->queue, 1);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= num_groups) break;  // every wave reaches this exit
+    const int64_t base = static_cast<int64_t>(g) * 64;
+    const int cnt = static_cast<int>(min<int64_t>(64, A.m - base));
+    const bool valid = lane < cnt;
+    const int64_t i = base + (valid ? lane : 0);
+    const float4 q = A.qs[i];
+    const Box box = wave_box(q.x, q.y, q.z);
+    float w = 0.f;
+    constexpr int C = kChunk;
+    auto flush_far = [&](int nfar) {
+      n_far += static_cast<unsigned long long>(nfar) * cnt;
+      const int padded = (nfar + C - 1) & ~(C - 1);
+      if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
+      __builtin_amdgcn_wave_barrier();
+      for (int f0 = 0; f0 < padded; f0 += C) w += wn_far_batch<C>(A, q.x, q.y, q.z, far + f0);
+      __builtin_amdgcn_wave_barrier();
+    };
+    auto flush_near = [&](int nnear) {
+      for (int b = 0; b < nnear; ++b) {
+        const int leaf = __builtin_amdgcn_readfirstlane(near[b]);
+        const int4 t = sload4(A.topo, leaf);
+        n_near += static_cast<unsigned long long>(t.y) * cnt;
+        for (int j0 = t.x; j0 < t.x + t.y; j0 += kChunk)
+          w += wn_near_batch(A, q.x, q.y, q.z, j0, t.x + t.y);
+      }
+    };
+    traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, stack, far, near, flush_far,
+             flush_near);
+    if (valid) A.out[A.qperm[i]] = w;
+    ++n_groups;
+  }
+  if (A.stats && lane == 0) {
+    atomicAdd(&A.wh->stats[0], n_far);
+    atomicAdd(&A.wh->stats[1], n_near);
+    atomicAdd(&A.wh->stats[2], n_groups);
+  }
+}
+
+__global__ void k_wn_copy_stats(const Header* __restrict__ hdr, const WnHeader* __restrict__ wh,
+                                int64_t* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  out[0] = static_cast<int64_t>(wh->stats[0]);
+  out[1] = static_cast<int64_t>(wh->stats[1]);
+  out[2] = static_cast<int64_t>(wh->stats[2]);
+  out[3] = hdr->num_nodes;
+  int levels = 0;
+  for (int l = 0; l < kMaxLevels; ++l)
+    if (hdr->level_count[l] > 0) levels = l + 1;
+  out[4] = levels;
+  out[5] = hdr->overflow;
+  out[6] = 0;
+  out[7] = 0;
+}
+
+int check_wn(const SbhBuffer* points, const SbhBuffer* normals, const SbhBuffer* areas,
+             const SbhBuffer* queries, const SbhBuffer* out, float beta, int max_depth,
+             int leaf_size) {
+  if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
+    return fail_invalid("points must have shape (N, 3)");
+  const int64_t n = points->dims[0];
+  if (!buffer_ok(normals, SBH_DTYPE_F32, 2) || normals->dims[0] != n || normals->dims[1] != 3)
+    return fail_invalid("normals must have shape (N, 3)");
+  if (!buffer_ok(areas, SBH_DTYPE_F32, 1) || areas->dims[0] != n)
+    return fail_invalid("areas must have shape (N,)");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2) || queries->dims[1] != 3)
+    return fail_invalid("queries must have shape (M, 3)");
+  if (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != queries->dims[0])
+    return fail_invalid("out must have shape (M,)");
+  if (n > (int64_t{1} << 30) || queries->dims[0] > (int64_t{1} << 31) - 64)
+    return fail_invalid("too many points");
+  if (!(beta >= 0.0f)) return fail_invalid("beta must be >= 0");
+  if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
+  if (leaf_size < 1) return fail_invalid("leaf_size must be >= 1");
+  return SBH_OK;
+}
+
+}  // namespace
+}  // namespace tree
+}  // namespace sbh
+
+using namespace sbh;
+
+extern "C" int64_t sbh_winding_number_workspace_bytes(int64_t num_elements, int64_t num_queries,
+                                                      int32_t max_depth, int32_t leaf_size) {
+  if (num_elements < 0 || num_queries < 0 || max_depth < 1 || max_depth > 10 || leaf_size < 1)
+    return -1;
+  return static_cast<int64_t>(
+      tree::wn_layout(num_elements, num_queries, max_depth, leaf_size).total);
+}
+
+extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const SbhBuffer* normals,
+                                  const SbhBuffer* areas, const SbhBuffer* queries,
+                                  SbhBuffer* out, float beta, int32_t max_depth,
+                                  int32_t leaf_size, SbhBuffer* workspace, SbhBuffer* stats) {
+  clear_error();
+  int rc = tree::check_wn(points, normals, areas, queries, out, beta, max_depth, leaf_size);
+  if (rc != SBH_OK) return rc;
+  if (stats != nullptr &&
+      (!buffer_ok(stats, SBH_DTYPE_S64, -1) || elem_count(stats) < 8))
+    return fail_invalid("stats must be an int64 buffer of at least 8 elements");
+  const int64_t n = points->dims[0], m = queries->dims[0];
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const tree::WnLayout W = tree::wn_layout(n, m, max_depth, leaf_size);
+  char* ws = nullptr;
+  if (workspace != nullptr) {
+    if (!buffer_ok(workspace, SBH_DTYPE_U8, -1) ||
+        byte_count(workspace) < static_cast<int64_t>(W.total))
+      return fail_invalid("workspace too small");
+    ws = static_cast<char*>(workspace->data);
+  } else {
+    ws = static_cast<char*>(::sbh::workspace(s, kWsPipeline, W.total));
+    if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+  }
+  const auto* pts = static_cast<const float*>(points->data);
+  const auto* q = static_cast<const float*>(queries->data);
+  // Source tree over the centroids with areas as weights; the normals ride
+  // along as the gathered per-point vector.
+  rc = tree::build(s, ws, W.t, pts, static_cast<const float*>(areas->data),
+                   static_cast<const float*>(normals->data), n, max_depth, leaf_size, nullptr);
+  if (rc != SBH_OK) return rc;
+  auto* hdr = reinterpret_cast<tree::Header*>(ws + W.t.header);
+  auto* wh = reinterpret_cast<tree::WnHeader*>(ws + W.qhdr);
+  auto* wd = reinterpret_cast<float4*>(ws + W.wd);
+  const auto* topo = reinterpret_cast<const int4*>(ws + W.t.topo);
+  const auto* mp = reinterpret_cast<const float4*>(ws + W.t.mp);
+  const auto* sp = reinterpret_cast<const float4*>(ws + W.t.sp);
+  const auto* sn = reinterpret_cast<const float4*>(ws + W.t.sc);
+  const int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+  for (int l = max_depth; l >= 0; --l) {
+    const int64_t lvl_cap = std::min<int64_t>(
+        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
+        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
+    tree::k_wn_moments<<<grid_for(lvl_cap, tree::gw::kB, 4096), tree::gw::kB, 0, s>>>(
+        hdr, l, topo, sp, sn, wd, static_cast<int>(W.t.cap));
+    SBH_LAUNCH_CHECK();
+  }
+  // Query order and groups.
+  float* qb = nullptr;
+  rc = tree::data_bounds(s, q, m, reinterpret_cast<float*>(ws + W.qbbox), &qb);
+  if (rc != SBH_OK) return rc;
+  auto* qkeys = reinterpret_cast<uint32_t*>(ws + W.qkeys);
+  auto* qvals = reinterpret_cast<uint32_t*>(ws + W.qvals);
+  auto* qcodes = reinterpret_cast<uint32_t*>(ws + W.qcodes);
+  auto* qperm = reinterpret_cast<uint32_t*>(ws + W.qperm);
+  auto* qs = reinterpret_cast<float4*>(ws + W.qs);
+  tree::k_qcodes<<<grid_for(std::max<int64_t>(m, 1), tree::gw::kB), tree::gw::kB, 0, s>>>(
+      q, m, qb, qkeys, qvals, wh);
+  SBH_LAUNCH_CHECK();
+  if (m > 0) {
+    rc = radix_sort_pairs(s, qkeys, qvals, qcodes, qperm, m, 3 * tree::kQueryBits,
+                          ws + W.qsort);
+    if (rc != SBH_OK) return rc;
+    tree::k_qgather<<<grid_for(m, tree::gw::kB), tree::gw::kB, 0, s>>>(q, m, qperm, qs);
+    SBH_LAUNCH_CHECK();
+    tree::WnArgs A;
+    A.hdr = hdr;
+    A.wh = wh;
+    A.topo = topo;
+    A.geo = reinterpret_cast<const float4*>(ws + W.t.geo);
+    A.mp = mp;
+    A.wd = wd;
+    A.sp = sp;
+    A.sn = sn;
+    A.qs = qs;
+    A.qperm = qperm;
+    A.m = m;
+    A.sentinel = static_cast<int>(W.t.cap);
+    A.beta = beta;
+    A.out = static_cast<float*>(out->data);
+    A.stats = stats != nullptr;
+    const unsigned grid = grid_for(m, 64 * tree::gw::kWavesPerBlock, 256 * 8);
+    tree::k_wn_walk<<<grid, tree::gw::kB, 0, s>>>(A);
+    SBH_LAUNCH_CHECK();
+  }
+  if (stats != nullptr) {
+    tree::k_wn_copy_stats<<<1, 64, 0, s>>>(hdr, wh, static_cast<int64_t*>(stats->data));
+    SBH_LAUNCH_CHECK();
+  }
+  return SBH_OK;
+}

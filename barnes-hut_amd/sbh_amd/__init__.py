@@ -9,5 +9,7 @@ from ._lib import library_path, load_library  # noqa: F401
 
 __all__ = ["ffi", "library_path", "load_library"]
 from . import gravity  # noqa: E402,F401
+from . import synthetic  # noqa: E402,F401
+from . import wn  # noqa: E402,F401
 
-__all__.append("gravity")
+__all__ += ["gravity", "synthetic", "wn"]

@@ -200,6 +200,59 @@ int sbh_stochastic_bh_force_octree8_soft_vjp(
     float prune_r_cut_mult, int32_t periodic, float box_length,
     int32_t threads);
 
+/* ---- winding numbers (K1) -------------------------------------------------
+ *
+ * Sources are oriented area elements: centroid p, unit normal n, area a.  The
+ * value at q is sum a n.(p - q) / (4 pi (|p - q|^3 + 1e-5))
+ * (signedSolidAngle<3>, include/kernel.h:73-80).  The reference exposes these
+ * only through its pybind11 module over CPU-built trees (src/sbh_gpu.cu:
+ * eval_brute_force_wn 53-64, eval_barnes_hut_wn 100-120,
+ * multi_level_prefix_control_variate_wn 166-187); there is no XLA target, so
+ * these are new targets with the reference's arithmetic.  Results: (M,). */
+
+/* Replaces evalBruteForce_wn / evalBruteForceGPU_wn (src/sbh_gpu.cu:53-64,
+ * 250-282; compensated summation of include/brute_force_gpu.h:18-66).
+ * points (N,3), normals (N,3), areas (N,), queries (M,3). */
+int sbh_bruteforce_wn(void* stream, const SbhBuffer* points,
+                      const SbhBuffer* normals, const SbhBuffer* areas,
+                      const SbhBuffer* queries, SbhBuffer* out, int32_t threads);
+
+/* Replaces evalBarnesHut_wn (src/sbh_gpu.cu:100-120, 363-397) on octree8
+ * records built by sbh_build_octree8_buffers from (centroids, normals,
+ * areas): the octree8 MAC of src/sbh_ffi.cu:1878-1893, node record =
+ * area A and area-weighted normal at the area-weighted centroid. */
+int sbh_barnes_hut_wn_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                              const SbhBuffer* node_bytes,
+                              const SbhBuffer* queries, SbhBuffer* out,
+                              float beta, int32_t threads);
+
+/* Replaces multiLevelPrefixControlVariateSampler_wn (src/sbh_gpu.cu:166-187,
+ * 474-510) with the octree8 stochastic walk of src/sbh_ffi.cu:2146-2291
+ * (depth-3 subdomains, Philox4x32-10 sample paths as
+ * sbh_stochastic_bh_force_octree8). */
+int sbh_stochastic_bh_wn_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                 const SbhBuffer* node_bytes,
+                                 const SbhBuffer* queries, SbhBuffer* out,
+                                 int32_t samples_per_subdomain, int32_t seed,
+                                 float beta, int32_t threads);
+
+/* Adaptive-tree winding numbers (no reference counterpart: the reference
+ * walks a CPU-built 512-ary tree).  The tree is built over the centroids
+ * with the areas as weights; queries are Morton-sorted in their own cube and
+ * walked in groups of 64 with the group MAC of the gravity pipeline; an
+ * accepted node is the dipole sum(a n) at the area-weighted centroid (the
+ * reference's node record).
+ * workspace: u8 buffer of sbh_winding_number_workspace_bytes() bytes, or NULL.
+ * stats: NULL or int64[8] {far interactions, near pairs, groups, nodes,
+ * levels, overflow, 0, 0}. */
+int64_t sbh_winding_number_workspace_bytes(int64_t num_elements, int64_t num_queries,
+                                           int32_t max_depth, int32_t leaf_size);
+int sbh_winding_number(void* stream, const SbhBuffer* points,
+                       const SbhBuffer* normals, const SbhBuffer* areas,
+                       const SbhBuffer* queries, SbhBuffer* out, float beta,
+                       int32_t max_depth, int32_t leaf_size,
+                       SbhBuffer* workspace, SbhBuffer* stats);
+
 /* ---- adaptive Morton octree pipeline (new targets) ----------------------
  *
  * No reference counterpart: the reference's octree8 is dense (8^D cells,

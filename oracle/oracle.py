@@ -112,6 +112,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_tree_eval_range.argtypes = (list(L.orc_tree_eval.argtypes[:15]) +
                                       list(L.orc_tree_eval.argtypes[16:]) + [_i64, _i64])
     L.orc_philox4x32.argtypes = [_u32p, _u32p, _u32p]
+    L.orc_bruteforce_wn.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _i64, _f64p]
+    L.orc_wn_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
+    L.orc_wn_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
+                              _f64p, _f64p, _f64p, _f32p, _f64p, _i64, _i64, _i32p, _i32p,
+                              ctypes.c_float, _f64p, ctypes.POINTER(_i64)]
 
 
 def _c(a, dtype):
@@ -234,7 +239,7 @@ def bruteforce_gravity(points, masses, queries, periodic=False, box=1.0):
 
 
 # ---------------------------------------------------------------- octree8 walks
-KIND_GRAV, KIND_SOFT, KIND_SOFT_VJP, KIND_GRAV_VJP = 0, 1, 2, 3
+KIND_GRAV, KIND_SOFT, KIND_SOFT_VJP, KIND_GRAV_VJP, KIND_WN = 0, 1, 2, 3, 4
 
 
 def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot=None,
@@ -249,6 +254,8 @@ def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot
                          float(eps), float(rc), int(bool(prune_enabled)),
                          float(prune_r_cut_mult), int(bool(periodic)), float(box), out,
                          ctypes.byref(cnt))
+    if kind == KIND_WN:
+        return out[:, 0].copy(), int(cnt.value)
     return out, int(cnt.value)
 
 
@@ -263,7 +270,51 @@ def sbh_octree8(leaf_bytes, node_bytes, queries, kind, samples, seed, beta, eps=
     lib().orc_sbh_octree8(leaf, node, q, _ptr(g), q.shape[0], int(kind), int(samples), int(seed),
                           float(beta), float(eps), float(rc), int(bool(prune_enabled)),
                           float(prune_r_cut_mult), int(bool(periodic)), float(box), out)
+    if kind == KIND_WN:
+        return out[:, 0].copy()
     return out
+
+
+def bruteforce_wn(points, normals, areas, queries):
+    """Exact fp64 winding-number sum of dipoles (signedSolidAngle<3>)."""
+    p = _c(points, np.float32)
+    nv = _c(normals, np.float32)
+    a = _c(areas, np.float32)
+    q = _c(queries, np.float32)
+    out = np.zeros(q.shape[0], np.float64)
+    lib().orc_bruteforce_wn(p, nv, a, p.shape[0], q, q.shape[0], out)
+    return out
+
+
+def triangle_wn(verts, faces, queries, chunk=256):
+    """Exact winding number of a closed triangle mesh at each query: the sum
+    of triangle solid angles (Van Oosterom & Strackee, IEEE TBME 1983:
+    tan(omega/2) = a.(b x c) / (|a||b||c| + (a.b)|c| + (a.c)|b| + (b.c)|a|))
+    over 4 pi, in fp64."""
+    V = np.asarray(verts, np.float64)
+    F = np.asarray(faces, np.int64)
+    Q = np.asarray(queries, np.float64)
+    A0, B0, C0 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    out = np.zeros(Q.shape[0], np.float64)
+    for s0 in range(0, Q.shape[0], chunk):
+        q = Q[s0:s0 + chunk, None, :]
+        a, b, c = A0[None] - q, B0[None] - q, C0[None] - q
+        la, lb, lc = (np.linalg.norm(x, axis=2) for x in (a, b, c))
+        det = np.einsum("qfk,qfk->qf", a, np.cross(b, c))
+        den = (la * lb * lc + np.einsum("qfk,qfk->qf", a, b) * lc +
+               np.einsum("qfk,qfk->qf", a, c) * lb + np.einsum("qfk,qfk->qf", b, c) * la)
+        out[s0:s0 + chunk] = (2.0 * np.arctan2(det, den)).sum(axis=1) / (4.0 * np.pi)
+    return out
+
+
+def query_order(queries, bits=10):
+    """Stable Morton order of a query cloud in its own cube (default bounds)."""
+    q = _c(queries, np.float32)
+    if q.shape[0] == 0:
+        return np.zeros(0, np.uint32)
+    lo, hi = default_bounds(q)
+    codes = morton_codes(q, lo, span(lo, hi), bits)
+    return stable_order(codes, 3 * bits)
 
 
 # ---------------------------------------------------------------- adaptive tree
@@ -380,6 +431,45 @@ class Tree:
                                   force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts,
                                   int(g_begin), int(g_end))
         return dict(force=force, far=int(counts[0]), near=int(counts[1]), groups=len(gs))
+
+
+    def wn_moments(self, normals_sorted):
+        nn = self.num_nodes
+        area = np.zeros(nn, np.float64)
+        cen = np.zeros((nn, 3), np.float64)
+        dip = np.zeros((nn, 3), np.float64)
+        lib().orc_wn_moments(self.start, self.count, nn,
+                             self.points_sorted.astype(np.float64),
+                             _c(normals_sorted, np.float64),
+                             self.masses_sorted.astype(np.float64), area, cen, dip)
+        return area, cen, dip
+
+    def wn_eval(self, normals, queries, *, beta=2.0, group_size=64):
+        """Winding numbers at `queries` (input order) with the group walk: the
+        tree's points are dipole positions, its masses the areas, `normals`
+        (input order) the unit normals.  Queries are grouped in runs of
+        group_size in their own Morton order (query_order)."""
+        ns = _c(normals, np.float64)[self.perm]
+        area, cen, dip = self.wn_moments(ns)
+        q = _c(queries, np.float32)
+        nq = q.shape[0]
+        qperm = query_order(q)
+        qs = q[qperm]
+        ng = (nq + group_size - 1) // group_size
+        gs = (np.arange(ng) * group_size).astype(np.int32)
+        gc = np.minimum(group_size, nq - gs).astype(np.int32)
+        boxes = np.zeros(max(ng, 1) * 6, np.float32)
+        lib().orc_group_boxes(qs, ng, gs, gc, boxes)
+        out = np.zeros(nq, np.float64)
+        counts = (_i64 * 2)()
+        lib().orc_wn_eval(self.num_nodes, self.start, self.count, self.first_child, self.nchild,
+                          _c(self.geo, np.float32).ravel(), area, cen.ravel(), dip.ravel(),
+                          self.points_sorted.astype(np.float64), ns,
+                          self.masses_sorted.astype(np.float64), boxes,
+                          qs.astype(np.float64), nq, ng, gs, gc, float(beta), out, counts)
+        res = np.zeros(nq, np.float64)
+        res[qperm] = out
+        return dict(wn=res, far=int(counts[0]), near=int(counts[1]), groups=int(ng))
 
 
 # ---------------------------------------------------------------- Philox

@@ -593,8 +593,43 @@ static inline int orc_should_prune8(const OrcNode8* nd, const float* q, int prun
   return dx * dx + dy * dy + dz * dz > r_cut_sq;
 }
 
+/* signedSolidAngle<3> (include/kernel.h:73-80) in fp64:
+ * a n.(p - q) / (4 pi (|p - q|^3 + 1e-5)), added to acc[0]. */
+static inline void orc_wn_pair(const double* q, const OrcLeaf3* src, const OrcParams* P,
+                               double* acc) {
+  double d0 = orc_mi((double)src->pos[0] - q[0], P);
+  double d1 = orc_mi((double)src->pos[1] - q[1], P);
+  double d2 = orc_mi((double)src->pos[2] - q[2], P);
+  double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  double nd = src->normal[0] * d0 + src->normal[1] * d1 + src->normal[2] * d2;
+  acc[0] += (double)src->mass * nd / (4.0 * M_PI * (r * r * r + 1e-5));
+}
+
+/* Exact winding-number sum of dipoles (the brute_force_gpu.h:18-66 summation
+ * with signedSolidAngle, in fp64). */
+ORC_EXPORT void orc_bruteforce_wn(const float* pts, const float* nrm, const float* area, int64_t n,
+                                  const float* queries, int64_t nq, double* out) {
+  OrcParams P;
+  orc_params(&P, 0.0, 0.0, 0, 1.0f, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double acc[3] = {0, 0, 0};
+    for (int64_t j = 0; j < n; j++) {
+      OrcLeaf3 s;
+      s.pos[0] = pts[3 * j]; s.pos[1] = pts[3 * j + 1]; s.pos[2] = pts[3 * j + 2];
+      s.normal[0] = nrm[3 * j]; s.normal[1] = nrm[3 * j + 1]; s.normal[2] = nrm[3 * j + 2];
+      s.mass = area[j];
+      s.alpha = 1.0f;
+      orc_wn_pair(q, &s, &P, acc);
+    }
+    out[i] = acc[0];
+  }
+}
+
 /* kind: 0 = gravityGrad (unsoftened, W3), 1 = softened/cutoff force (W2),
- *       2 = softened/cutoff VJP (W4, needs cot), 3 = gravityGrad VJP. */
+ *       2 = softened/cutoff VJP (W4, needs cot), 3 = gravityGrad VJP,
+ *       4 = winding number (signedSolidAngle, scalar in out[3 i]). */
 ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_bytes,
                                const float* queries, const float* cot, int64_t nq,
                                int kind, float beta, double eps, double rc,
@@ -639,6 +674,7 @@ ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_by
             case 0: orc_grav_grad(q, p, src->mass, &P, acc); break;
             case 1: orc_soft_force(q, p, src->mass, &P, acc); break;
             case 2: orc_soft_force_vjp(q, p, src->mass, g, &P, acc); break;
+            case 4: orc_wn_pair(q, src, &P, acc); break;
             default: orc_grav_grad_vjp(q, p, src->mass, g, &P, acc); break;
           }
           total_inter++;
@@ -688,7 +724,7 @@ static inline float orc_centre_dist8(const OrcNode8* nd, const float* q, int per
   return sqrtf(dx * dx + dy * dy + dz * dz);
 }
 
-/* kind as orc_bh_octree8 (0 grav, 1 soft, 2 soft vjp, 3 grav vjp); the
+/* kind as orc_bh_octree8 (0 grav, 1 soft, 2 soft vjp, 3 grav vjp, 4 wn); the
  * softened kinds apply the prune test like the reference (softened = true).
  * Subdomains: nodes at depth 3 or leaves above it (kControlDepth = 3,
  * src/sbh_ffi.cu:2167-2192), in the reference's push/pop order. */
@@ -723,6 +759,7 @@ ORC_EXPORT void orc_sbh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_b
       case 0: orc_grav_grad(q, p_, (src)->mass, &P, acc); break;           \
       case 1: orc_soft_force(q, p_, (src)->mass, &P, acc); break;          \
       case 2: orc_soft_force_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
+      case 4: orc_wn_pair(q, (src), &P, acc); break;                       \
       default: orc_grav_grad_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
     }                                                                      \
   } while (0)
@@ -1184,6 +1221,109 @@ ORC_EXPORT void orc_tree_eval_range(int64_t nn, const int32_t* start, const int3
   free(stack);
   free(far);
   free(near);
+  if (counts) {
+    counts[0] = nfar;
+    counts[1] = nnear;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* K1 on the adaptive tree: winding numbers of a dipole (oriented-area) cloud */
+/* ------------------------------------------------------------------------- */
+/* Per node (sorted sources xs, unit normals ns, areas as): area A,
+ * area-weighted centroid c = sum a p / A and dipole D = sum a n -- the
+ * reference's node record (A, D / A) at c (src/sbh_ffi.cu:1840-1868 with
+ * normals), in fp64. */
+ORC_EXPORT void orc_wn_moments(const int32_t* start, const int32_t* count, int64_t nn,
+                               const double* xs, const double* ns, const double* as,
+                               double* area, double* cen, double* dip) {
+  for (int64_t k = 0; k < nn; k++) {
+    double A = 0, C[3] = {0, 0, 0}, D[3] = {0, 0, 0};
+    for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+      A += as[j];
+      for (int a = 0; a < 3; a++) {
+        C[a] += as[j] * xs[3 * j + a];
+        D[a] += as[j] * ns[3 * j + a];
+      }
+    }
+    if (A != 0.0)
+      for (int a = 0; a < 3; a++) C[a] /= A;
+    area[k] = A;
+    for (int a = 0; a < 3; a++) {
+      cen[3 * k + a] = C[a];
+      dip[3 * k + a] = D[a];
+    }
+  }
+}
+
+/* Far field of one node at query q: the dipole D at the centroid c,
+ * (D.(c - q)) / (4 pi (|c - q|^3 + 1e-5)). */
+static inline double orc_wn_far(const double* q, const double* c, const double* D) {
+  double X[3] = {c[0] - q[0], c[1] - q[1], c[2] - q[2]};
+  double R = sqrt(X[0] * X[0] + X[1] * X[1] + X[2] * X[2]);
+  return (D[0] * X[0] + D[1] * X[1] + D[2] * X[2]) / (4.0 * M_PI * (R * R * R + 1e-5));
+}
+
+/* Group walk for winding numbers: queries (sorted, fp64 copies of fp32
+ * values) in groups g_start/g_count with boxes; same MAC and traversal as
+ * orc_tree_eval_range.  out[i] in sorted query order; counts as there. */
+ORC_EXPORT void orc_wn_eval(int64_t nn, const int32_t* start, const int32_t* count,
+                            const int32_t* first_child, const int32_t* nchild, const float* geo,
+                            const double* area, const double* cen, const double* dip,
+                            const double* xs, const double* ns, const double* as,
+                            const float* boxes, const double* qx, int64_t nq, int64_t ng,
+                            const int32_t* g_start, const int32_t* g_count, float beta,
+                            double* out, int64_t* counts) {
+  int64_t nfar = 0, nnear = 0;
+  memset(out, 0, sizeof(double) * (size_t)nq);
+#pragma omp parallel reduction(+ : nfar, nnear)
+  {
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 8));
+    int32_t* far = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+    int32_t* near = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+#pragma omp for schedule(dynamic, 4)
+    for (int64_t g = 0; g < ng; g++) {
+      const float* box = boxes + 6 * g;
+      int64_t nf = 0, nl = 0, top = 0;
+      stack[top++] = 0;
+      while (top > 0) {
+        int32_t k = stack[--top];
+        if (area[k] == 0.0) continue;
+        if (orc_group_accept(box, geo + 4 * k, beta)) {
+          far[nf++] = k;
+        } else if (nchild[k] == 0) {
+          near[nl++] = k;
+        } else {
+          for (int32_t c = 0; c < nchild[k]; c++) stack[top++] = first_child[k] + c;
+        }
+      }
+      for (int64_t i = g_start[g]; i < g_start[g] + g_count[g] && i < nq; i++) {
+        const double* q = qx + 3 * i;
+        double w = 0.0;
+        for (int64_t f = 0; f < nf; f++) {
+          int32_t k = far[f];
+          w += orc_wn_far(q, cen + 3 * k, dip + 3 * k);
+        }
+        nfar += nf;
+        for (int64_t b = 0; b < nl; b++) {
+          int32_t k = near[b];
+          for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+            const double* p = xs + 3 * j;
+            const double* nv = ns + 3 * j;
+            double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+            double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+            w += as[j] * (nv[0] * d[0] + nv[1] * d[1] + nv[2] * d[2]) /
+                 (4.0 * M_PI * (r * r * r + 1e-5));
+            nnear++;
+          }
+        }
+        out[i] = w;
+      }
+    }
+    free(stack);
+    free(far);
+    free(near);
+  }
   if (counts) {
     counts[0] = nfar;
     counts[1] = nnear;

@@ -680,3 +680,90 @@ def test_sbh_prune_huge_cutoff_matches_disabled():
     b = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 3, 11, 1.0, eps=0.05, rc=0.8,
                       prune_enabled=True, prune_r_cut_mult=1e6, periodic=True, box=2.0)
     np.testing.assert_array_equal(a, b)
+
+
+# --------------------------------------------------------------------- winding numbers
+def _sphere_dipoles(freq=8, seed=None):
+    from sbh_amd import synthetic as S
+
+    V, F = S.geodesic_sphere(freq) if seed is None else S.perturbed_sphere(freq, seed)
+    c, n, a = S.face_dipoles(V, F)
+    return V, F, c, n, a
+
+
+def test_triangle_wn_closed_mesh_is_indicator():
+    # Gauss: the solid angles of a closed oriented surface sum to 4 pi inside, 0 outside
+    V, F, *_ = _sphere_dipoles(6, seed=3)
+    q = np.array([[0, 0, 0], [0.2, -0.1, 0.3], [3, 0, 0], [0, -2.5, 1]], np.float64)
+    np.testing.assert_allclose(O.triangle_wn(V, F, q), [1, 1, 0, 0], atol=1e-12)
+    # a single triangle: the octant triangle seen from the origin subtends pi/2
+    tri = np.eye(3)[None]
+    w = O.triangle_wn(tri.reshape(3, 3), np.array([[0, 1, 2]]), np.zeros((1, 3)))
+    np.testing.assert_allclose(w, [1.0 / 8.0], rtol=1e-14)
+
+
+def test_bruteforce_wn_closed_form_and_discretisation():
+    # one dipole: a n.(p - q) / (4 pi (r^3 + 1e-5))
+    p = np.array([[0.3, -0.2, 0.5]], np.float32)
+    n = np.array([[0.0, 0.6, 0.8]], np.float32)
+    a = np.array([0.7], np.float32)
+    q = np.array([[1.0, 1.0, -1.0]], np.float32)
+    d = p[0].astype(np.float64) - q[0]
+    r = np.linalg.norm(d)
+    ref = 0.7 * float(n[0] @ d) / (4 * np.pi * (r ** 3 + 1e-5))
+    np.testing.assert_allclose(O.bruteforce_wn(p, n, a, q), [ref], rtol=1e-6)
+    # dipoles of a fine sphere approximate the indicator away from the surface
+    V, F, c, n, a = _sphere_dipoles(16)
+    q = np.array([[0, 0, 0], [0.3, 0.2, -0.1], [2, 1, 0], [0, 0, -3]], np.float32)
+    np.testing.assert_allclose(O.bruteforce_wn(c, n, a, q), [1, 1, 0, 0], atol=2e-3)
+
+
+def test_octree8_wn_huge_beta_is_bruteforce():
+    _, _, c, n, a = _sphere_dipoles(8, seed=5)
+    lo, hi = O.default_bounds(c)
+    leaf, node, _ = O.build_octree8(c, n, a, lo, hi, 5)
+    q = np.random.default_rng(2).uniform(-1.5, 1.5, (200, 3)).astype(np.float32)
+    w, _ = O.bh_octree8(leaf, node, q, O.KIND_WN, 1e30)
+    np.testing.assert_allclose(w, O.bruteforce_wn(c, n, a, q), rtol=1e-9, atol=1e-12)
+    w2, cnt = O.bh_octree8(leaf, node, q, O.KIND_WN, 2.0)
+    assert cnt < 200 * c.shape[0]
+    assert O.rel_l2(w2, w) < 2e-2
+
+
+def test_tree_wn_huge_beta_is_bruteforce_and_beta2_is_close():
+    _, _, c, n, a = _sphere_dipoles(12, seed=6)
+    q = np.random.default_rng(3).uniform(-1.5, 1.5, (700, 3)).astype(np.float32)
+    ref = O.bruteforce_wn(c, n, a, q)
+    T = O.Tree(c, a, max_depth=10, leaf_size=16)
+    r = T.wn_eval(n, q, beta=1e30)
+    np.testing.assert_allclose(r["wn"], ref, rtol=1e-9, atol=1e-12)
+    assert r["far"] == 0 and r["near"] == q.shape[0] * c.shape[0]
+    r = T.wn_eval(n, q, beta=2.0)
+    assert r["far"] > 0 and r["near"] < q.shape[0] * c.shape[0]
+    assert O.rel_l2(r["wn"], ref) < 2e-3
+
+
+def test_query_order_is_stable_morton_order():
+    q = np.random.default_rng(4).uniform(-2, 3, (500, 3)).astype(np.float32)
+    q[10] = q[20]  # equal codes keep input order
+    perm = O.query_order(q)
+    lo, hi = O.default_bounds(q)
+    codes = O.morton_codes(q, lo, O.span(lo, hi), 10)
+    np.testing.assert_array_equal(perm, np.argsort(codes, kind="stable"))
+
+
+def test_sbh_wn_unbiased_and_shallow_exact():
+    _, _, c, n, a = _sphere_dipoles(8, seed=7)
+    lo, hi = O.default_bounds(c)
+    q = np.random.default_rng(5).uniform(-1.2, 1.2, (24, 3)).astype(np.float32)
+    exact = O.bruteforce_wn(c, n, a, q)
+    leaf3, node3, _ = O.build_octree8(c, n, a, lo, hi, 3)
+    np.testing.assert_allclose(O.sbh_octree8(leaf3, node3, q, O.KIND_WN, 4, 1, 1.0), exact,
+                               rtol=1e-9, atol=1e-12)
+    leaf, node, _ = O.build_octree8(c, n, a, lo, hi, 6)
+    runs = np.stack([O.sbh_octree8(leaf, node, q, O.KIND_WN, 32, s, 1.0) for s in range(48)])
+    mean = runs.mean(0)
+    se = runs.std(0, ddof=1) / np.sqrt(runs.shape[0])
+    z = (mean - exact) / se
+    assert (z ** 2).mean() < 2.0
+    assert np.abs(z).max() < 5.0
