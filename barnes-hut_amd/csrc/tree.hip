@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
   int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (r >= n) return;
   int64_t i = perm[r];
-  sp[r] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], ms[i]);
+  sp[r] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], ms != nullptr ? ms[i] : 0.0f);
   if (cot != nullptr) sc[r] = make_float4(cot[3 * i], cot[3 * i + 1], cot[3 * i + 2], 0.0f);
 }
 
@@ -959,7 +959,7 @@ int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** 
 
 int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
           const float* cot, int64_t n, int max_depth, int leaf_size,
-          const float* user_bounds) {
+          const float* user_bounds, int parts) {
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
   auto* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
   auto* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
@@ -1006,13 +1006,14 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
     k_emit<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, geo, parent, nch);
     SBH_LAUNCH_CHECK();
   }
-  for (int l = max_depth; l >= 0; --l) {
+  for (int l = max_depth; l >= 0 && (parts & kBuildMoments); --l) {
     int64_t lvl_cap = std::min<int64_t>(
         l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
         std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
     k_moments<<<grid_for(lvl_cap, kB, 4096), kB, 0, s>>>(hdr, l, topo, geo, sp, mp, qd);
     SBH_LAUNCH_CHECK();
   }
+  if (!(parts & kBuildGroups)) return SBH_OK;
   auto* goff = reinterpret_cast<uint32_t*>(ws + L.goff);
   auto* groups = reinterpret_cast<int2*>(ws + L.groups);
   unsigned gg = grid_for(L.cap, kB, 4096);

@@ -52,12 +52,18 @@ Layout layout(int64_t n, int max_depth, int leaf_size);
 // returned in *bounds; tmp holds 6 * (kBboxBlocks + 1) floats.
 int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** bounds);
 
-// Build topology and moments.  `cot` may be null (forces only).  `user_bounds`
-// (device, 6 floats: min xyz, max xyz) may be null to use the data bounds
-// expanded by 1e-3.
+// Parts of build() beyond the topology (sorted records, nodes, geometry).
+constexpr int kBuildMoments = 1;  // mass, centre of mass, second moment per node
+constexpr int kBuildGroups = 2;   // query groups of the walk (cells of <= kGroupMax points)
+constexpr int kBuildAll = kBuildMoments | kBuildGroups;
+
+// Build topology, moments and groups.  `cot` may be null (forces only).
+// `user_bounds` (device, 6 floats: min xyz, max xyz) may be null to use the
+// data bounds expanded by 1e-3.  Without kBuildMoments the node masses
+// stay unset (zero only for the sentinel).
 int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
           const float* cot, int64_t n, int max_depth, int leaf_size,
-          const float* user_bounds);
+          const float* user_bounds, int parts = kBuildAll);
 
 // Walk; forces (input order, N x 3) may be null; with vjp, writes grad_points
 // (N x 3) and grad_masses (N) in input order.

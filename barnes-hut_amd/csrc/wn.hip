@@ -11,9 +11,12 @@
 // patches at the MAC distance the next Taylor term is larger and of opposite
 // sign, so the truncated series is less accurate than the dipole alone.
 //
-// Queries are a separate cloud: Morton-sorted in their own cube, cut into
-// runs of 64 (one wavefront each) and walked with the same group MAC and
-// traversal as the gravity walk.
+// Queries are a separate cloud with a tree of their own (topology only, in
+// their own cube); its cell groups of <= 64 queries (one wavefront each) are
+// walked against the element tree with the group MAC and traversal of the
+// gravity walk.  (Plain runs of 64 Morton-sorted queries are not used: runs
+// that cross a coarse cell boundary have boxes spanning much of the domain
+// and accept almost nothing.)
 #include "common.hpp"
 #include "groupwalk.hpp"
 #include "tree.hpp"
@@ -24,19 +27,15 @@ namespace {
 
 using namespace gw;
 
-constexpr int kQueryBits = 10;  // 30-bit query Morton codes
 constexpr float kInvFourPi = 0.07957747154594767f;
-
-struct WnHeader {
-  int32_t queue;
-  int32_t pad[3];
-  unsigned long long stats[4];  // far interactions, near pairs, groups
-};
+constexpr int kQueryDepth = 10;
+constexpr int kQueryLeaf = kGroupMax;
 
 struct WnLayout {
-  Layout t;
-  size_t wd;                             // per node: dipole (float4)
-  size_t qhdr, qbbox, qkeys, qvals, qcodes, qperm, qsort, qs;
+  Layout t;      // element tree, at the workspace base
+  size_t wd;     // per element-tree node: dipole (float4)
+  size_t qbase;  // query tree, at workspace + qbase
+  Layout qt;
   size_t total;
 };
 
@@ -45,18 +44,11 @@ WnLayout wn_layout(int64_t n, int64_t m, int max_depth, int leaf_size) {
   WnLayout W;
   W.t = layout(n, max_depth, leaf_size);
   const size_t cap = static_cast<size_t>(W.t.cap) + 1;
-  const size_t mm = static_cast<size_t>(m > 0 ? m : 1);
   size_t o = W.t.total;
   W.wd = o; o = align(o + 16 * cap);
-  W.qhdr = o; o = align(o + sizeof(WnHeader));
-  W.qbbox = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
-  W.qkeys = o; o = align(o + 4 * mm);
-  W.qvals = o; o = align(o + 4 * mm);
-  W.qcodes = o; o = align(o + 4 * mm);
-  W.qperm = o; o = align(o + 4 * mm);
-  W.qsort = o; o = align(o + radix_sort_temp_bytes(m));
-  W.qs = o; o = align(o + 16 * mm);
-  W.total = o;
+  W.qbase = o;
+  W.qt = layout(m, kQueryDepth, kQueryLeaf);
+  W.total = W.qbase + W.qt.total;
   return W;
 }
 
@@ -96,44 +88,11 @@ __global__ __launch_bounds__(kB) void k_wn_moments(const Header* __restrict__ hd
   }
 }
 
-// ------------------------------------------------------------------ queries
-// Morton codes of the queries in their own cube (bounds from data_bounds).
-__global__ __launch_bounds__(kB) void k_qcodes(const float* __restrict__ q, int64_t m,
-                                               const float* __restrict__ bounds,
-                                               uint32_t* __restrict__ codes,
-                                               uint32_t* __restrict__ idx,
-                                               WnHeader* __restrict__ wh) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (i == 0) {
-    wh->queue = 0;
-    for (int k = 0; k < 4; ++k) wh->stats[k] = 0;
-  }
-  if (i >= m) return;
-  float span;
-  {
-#pragma clang fp contract(off)
-    span = fmaxf(fmaxf(bounds[3] - bounds[0], bounds[4] - bounds[1]), bounds[5] - bounds[2]);
-  }
-  const uint32_t qx = quantize_axis(q[3 * i + 0], bounds[0], span, kQueryBits);
-  const uint32_t qy = quantize_axis(q[3 * i + 1], bounds[1], span, kQueryBits);
-  const uint32_t qz = quantize_axis(q[3 * i + 2], bounds[2], span, kQueryBits);
-  codes[i] = morton3(qx, qy, qz);
-  idx[i] = static_cast<uint32_t>(i);
-}
-
-__global__ __launch_bounds__(kB) void k_qgather(const float* __restrict__ q, int64_t m,
-                                                const uint32_t* __restrict__ perm,
-                                                float4* __restrict__ qs) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (r >= m) return;
-  const int64_t i = perm[r];
-  qs[r] = make_float4(q[3 * i], q[3 * i + 1], q[3 * i + 2], 0.f);
-}
-
 // ------------------------------------------------------------------ walk
 struct WnArgs {
-  const Header* hdr;
-  WnHeader* wh;
+  const Header* hdr;  // element tree
+  Header* qhdr;       // query tree: group count, work queue, statistics
+  const int2* groups; // query groups (start, count) in sorted query order
   const int4* topo;
   const float4* geo;
   const float4* mp;   // centroid, area
@@ -142,7 +101,7 @@ struct WnArgs {
   const float4* sn;   // sorted normal
   const float4* qs;   // sorted queries
   const uint32_t* qperm;
-  int64_t m;
+  int m;
   int sentinel;
   float beta;
   float* out;
@@ -198,17 +157,19 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
   int* stack = s_stack[wv];
   int* far = s_far[wv];
   int* near = s_near[wv];
-  const int num_groups = static_cast<int>((A.m + 63) / 64);
+  const int num_groups = A.qhdr->num_groups;
   unsigned long long n_far = 0, n_near = 0, n_groups = 0;
   while (true) {
     int g = 0;
-    if (lane == 0) g = atomicAdd(&A.wh->queue, 1);
+    if (lane == 0) g = atomicAdd(&A.qhdr->queue, 1);
     g = __builtin_amdgcn_readfirstlane(g);
     if (g >= num_groups) break;  // every wave reaches this exit
-    const int64_t base = static_cast<int64_t>(g) * 64;
-    const int cnt = static_cast<int>(min<int64_t>(64, A.m - base));
+    const int2 grp = A.groups[g];
+    const int base = __builtin_amdgcn_readfirstlane(grp.x);
+    const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
+    if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.m) continue;  // corrupt entry
     const bool valid = lane < cnt;
-    const int64_t i = base + (valid ? lane : 0);
+    const int i = base + (valid ? lane : 0);
     const float4 q = A.qs[i];
     const Box box = wave_box(q.x, q.y, q.z);
     float w = 0.f;
@@ -236,18 +197,18 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
     ++n_groups;
   }
   if (A.stats && lane == 0) {
-    atomicAdd(&A.wh->stats[0], n_far);
-    atomicAdd(&A.wh->stats[1], n_near);
-    atomicAdd(&A.wh->stats[2], n_groups);
+    atomicAdd(&A.qhdr->stats[0], n_far);
+    atomicAdd(&A.qhdr->stats[1], n_near);
+    atomicAdd(&A.qhdr->stats[2], n_groups);
   }
 }
 
-__global__ void k_wn_copy_stats(const Header* __restrict__ hdr, const WnHeader* __restrict__ wh,
+__global__ void k_wn_copy_stats(const Header* __restrict__ hdr, const Header* __restrict__ qhdr,
                                 int64_t* __restrict__ out) {
   if (threadIdx.x != 0) return;
-  out[0] = static_cast<int64_t>(wh->stats[0]);
-  out[1] = static_cast<int64_t>(wh->stats[1]);
-  out[2] = static_cast<int64_t>(wh->stats[2]);
+  out[0] = static_cast<int64_t>(qhdr->stats[0]);
+  out[1] = static_cast<int64_t>(qhdr->stats[1]);
+  out[2] = static_cast<int64_t>(qhdr->stats[2]);
   out[3] = hdr->num_nodes;
   int levels = 0;
   for (int l = 0; l < kMaxLevels; ++l)
@@ -272,7 +233,7 @@ int check_wn(const SbhBuffer* points, const SbhBuffer* normals, const SbhBuffer*
     return fail_invalid("queries must have shape (M, 3)");
   if (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != queries->dims[0])
     return fail_invalid("out must have shape (M,)");
-  if (n > (int64_t{1} << 30) || queries->dims[0] > (int64_t{1} << 31) - 64)
+  if (n > (int64_t{1} << 30) || queries->dims[0] > (int64_t{1} << 30))
     return fail_invalid("too many points");
   if (!(beta >= 0.0f)) return fail_invalid("beta must be >= 0");
   if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
@@ -325,7 +286,6 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
                    static_cast<const float*>(normals->data), n, max_depth, leaf_size, nullptr);
   if (rc != SBH_OK) return rc;
   auto* hdr = reinterpret_cast<tree::Header*>(ws + W.t.header);
-  auto* wh = reinterpret_cast<tree::WnHeader*>(ws + W.qhdr);
   auto* wd = reinterpret_cast<float4*>(ws + W.wd);
   const auto* topo = reinterpret_cast<const int4*>(ws + W.t.topo);
   const auto* mp = reinterpret_cast<const float4*>(ws + W.t.mp);
@@ -340,36 +300,26 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
         hdr, l, topo, sp, sn, wd, static_cast<int>(W.t.cap));
     SBH_LAUNCH_CHECK();
   }
-  // Query order and groups.
-  float* qb = nullptr;
-  rc = tree::data_bounds(s, q, m, reinterpret_cast<float*>(ws + W.qbbox), &qb);
+  // Query tree (topology and groups only).
+  char* qws = ws + W.qbase;
+  rc = tree::build(s, qws, W.qt, q, nullptr, nullptr, m, tree::kQueryDepth, tree::kQueryLeaf,
+                   nullptr, tree::kBuildGroups);
   if (rc != SBH_OK) return rc;
-  auto* qkeys = reinterpret_cast<uint32_t*>(ws + W.qkeys);
-  auto* qvals = reinterpret_cast<uint32_t*>(ws + W.qvals);
-  auto* qcodes = reinterpret_cast<uint32_t*>(ws + W.qcodes);
-  auto* qperm = reinterpret_cast<uint32_t*>(ws + W.qperm);
-  auto* qs = reinterpret_cast<float4*>(ws + W.qs);
-  tree::k_qcodes<<<grid_for(std::max<int64_t>(m, 1), tree::gw::kB), tree::gw::kB, 0, s>>>(
-      q, m, qb, qkeys, qvals, wh);
-  SBH_LAUNCH_CHECK();
+  auto* qhdr = reinterpret_cast<tree::Header*>(qws + W.qt.header);
   if (m > 0) {
-    rc = radix_sort_pairs(s, qkeys, qvals, qcodes, qperm, m, 3 * tree::kQueryBits,
-                          ws + W.qsort);
-    if (rc != SBH_OK) return rc;
-    tree::k_qgather<<<grid_for(m, tree::gw::kB), tree::gw::kB, 0, s>>>(q, m, qperm, qs);
-    SBH_LAUNCH_CHECK();
     tree::WnArgs A;
     A.hdr = hdr;
-    A.wh = wh;
+    A.qhdr = qhdr;
+    A.groups = reinterpret_cast<const int2*>(qws + W.qt.groups);
     A.topo = topo;
     A.geo = reinterpret_cast<const float4*>(ws + W.t.geo);
     A.mp = mp;
     A.wd = wd;
     A.sp = sp;
     A.sn = sn;
-    A.qs = qs;
-    A.qperm = qperm;
-    A.m = m;
+    A.qs = reinterpret_cast<const float4*>(qws + W.qt.sp);
+    A.qperm = reinterpret_cast<const uint32_t*>(qws + W.qt.perm);
+    A.m = static_cast<int>(m);
     A.sentinel = static_cast<int>(W.t.cap);
     A.beta = beta;
     A.out = static_cast<float*>(out->data);
@@ -379,7 +329,7 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
     SBH_LAUNCH_CHECK();
   }
   if (stats != nullptr) {
-    tree::k_wn_copy_stats<<<1, 64, 0, s>>>(hdr, wh, static_cast<int64_t*>(stats->data));
+    tree::k_wn_copy_stats<<<1, 64, 0, s>>>(hdr, qhdr, static_cast<int64_t*>(stats->data));
     SBH_LAUNCH_CHECK();
   }
   return SBH_OK;

@@ -54,18 +54,6 @@ def profiled_traffic(kernel_key):
     return None
 
 
-def plummer(n, seed):
-    import numpy as np
-
-    rng = np.random.default_rng(seed)
-    xmax = 20.0 ** 3 / (20.0 ** 2 + 1.0) ** 1.5
-    X = rng.uniform(0, xmax, n)
-    r = 1.0 / np.sqrt(X ** (-2.0 / 3.0) - 1.0)
-    v = rng.normal(size=(n, 3))
-    v /= np.linalg.norm(v, axis=1, keepdims=True)
-    return (r[:, None] * v).astype(np.float32)
-
-
 def direct_fp64(P, M, Q, eps, chunk=16):
     import torch
 
@@ -146,23 +134,34 @@ def main():
     ap.add_argument("--cpu-sample-groups", type=int, default=1200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--workload", default="gravity", choices=("gravity", "wn"),
+                    help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
+                         "204,020-face perturbed geodesic sphere at 2^20 queries")
+    ap.add_argument("--wn-freq", type=int, default=101,
+                    help="geodesic frequency of the wn mesh (20 f^2 faces)")
+    ap.add_argument("--wn-samples", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
                                    "it (queries would shard by Morton range, see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    rec = bench_wn(args) if args.workload == "wn" else bench_gravity(args)
+    print(json.dumps(rec))
 
+
+def bench_gravity(args):
     import numpy as np
     import torch
 
     import sbh_amd
     from sbh_amd import gravity as G
+    from sbh_amd import synthetic
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     n = args.n
-    p = plummer(n, 2000)
+    p = synthetic.plummer(n, 2000)
     m = np.full(n, 1.0 / n, np.float32)
     c = np.random.default_rng(2002).normal(size=(n, 3)).astype(np.float32)
     P = torch.tensor(p, device=dev)
@@ -286,7 +285,175 @@ def main():
                      "flops_per_launch": flops_vjp},
         "cpu_baseline": cpu,
     }
-    print(json.dumps(rec))
+    return rec
+
+FLOPS_WN = 20  # per lane interaction (element or node dipole), fp32
+
+
+def wn_fp64(Pc, Nn, A, Q, chunk=8):
+    """FP64 dipole sum (signedSolidAngle<3>) on the device for a query sample."""
+    import math
+
+    import torch
+
+    out = []
+    for q in Q.split(chunk):
+        d = Pc[None] - q[:, None, :]
+        r = torch.linalg.norm(d, dim=-1)
+        out.append((A[None] * (Nn[None] * d).sum(-1) / (4 * math.pi * (r ** 3 + 1e-5))).sum(1))
+    return torch.cat(out)
+
+
+def triangle_wn_fp64(V, F, Q, chunk=8):
+    """FP64 exact solid-angle sum of the triangle mesh (Van Oosterom-Strackee)
+    on the device for a query sample."""
+    import math
+
+    import torch
+
+    A0, B0, C0 = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    out = []
+    for q in Q.split(chunk):
+        a, b, c = A0[None] - q[:, None], B0[None] - q[:, None], C0[None] - q[:, None]
+        la, lb, lc = a.norm(dim=-1), b.norm(dim=-1), c.norm(dim=-1)
+        det = (a * torch.cross(b, c, dim=-1)).sum(-1)
+        den = (la * lb * lc + (a * b).sum(-1) * lc + (a * c).sum(-1) * lb +
+               (b * c).sum(-1) * la)
+        out.append(2.0 * torch.atan2(det, den).sum(1) / (4 * math.pi))
+    return torch.cat(out)
+
+
+def cpu_baseline_wn(c, nrm, a, q, leaf, sample):
+    """Oracle restatement of the adaptive winding-number walk on the host
+    (OpenMP threads), on `sample` queries, scaled to the full query count."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    t0 = time.perf_counter()
+    T = O.Tree(c, a, max_depth=10, leaf_size=leaf)
+    t_build = time.perf_counter() - t0
+    sel = np.random.default_rng(11).choice(q.shape[0], sample, replace=False)
+    qs = q[np.sort(sel)]
+    t0 = time.perf_counter()
+    T.wn_eval(nrm, qs, beta=2.0)
+    t_walk = time.perf_counter() - t0
+    t_all = t_build + t_walk * q.shape[0] / sample
+    return {
+        "value": q.shape[0] / t_all, "unit": "winding numbers/s",
+        "cores": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+        "kind": "port",
+        "sample": (f"oracle adaptive-tree winding-number walk (fp64, OpenMP): element tree "
+                   f"build, query tree and walk timed on {sample} of {q.shape[0]} queries"
+                   + (" and scaled" if sample < q.shape[0] else "")),
+        "seconds_measured": round(t_build + t_walk, 2),
+        "host_cpu": platform.processor() or platform.machine(),
+        "host_threads_available": os.cpu_count(),
+    }
+
+
+def bench_wn(args):
+    """Winding numbers: 20 f^2-face geodesic sphere (f = 101: 204,020 faces)
+    with its radius perturbed by real spherical harmonics l = 1..6, one dipole
+    per face, 2^20 queries uniform in [-1.5, 1.5]^3; classic theta = 0.5."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import ffi, synthetic, wn
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    V, F = synthetic.perturbed_sphere(args.wn_freq, 4000)
+    c, nrm, a = synthetic.face_dipoles(V, F)
+    m = args.n
+    q = np.random.default_rng(4003).uniform(-1.5, 1.5, (m, 3)).astype(np.float32)
+    Pc, Nn, A, Q = (torch.tensor(x, device=dev) for x in (c, nrm, a, q))
+    leaf = args.leaf if args.leaf != 32 else 16
+    ws = wn.new_workspace(c.shape[0], m, 10, leaf, dev)
+    out = torch.empty(m, dtype=torch.float32, device=dev)
+
+    def step():
+        return wn.winding_number(Pc, Nn, A, Q, beta=args.beta, max_depth=10, leaf_size=leaf,
+                                 workspace=ws, out=out)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record(stream)
+    for _ in range(args.steps):
+        step()
+    ev[1].record(stream)
+    torch.cuda.synchronize()
+    ms_step = ev[0].elapsed_time(ev[1]) / args.steps
+    st = torch.zeros(8, dtype=torch.int64, device=dev)
+    wn.winding_number(Pc, Nn, A, Q, beta=args.beta, max_depth=10, leaf_size=leaf,
+                      workspace=ws, stats=st)
+    s = [int(x) for x in st.cpu()]
+    flops = FLOPS_WN * (s[0] + s[1])
+
+    # accuracy on a query sample: FP64 dipole sum and FP64 exact triangles
+    idx = torch.tensor(np.random.default_rng(7).choice(m, args.accuracy_queries, replace=False),
+                       device=dev)
+    ref = wn_fp64(Pc.double(), Nn.double(), A.double(), Q[idx].double())
+    exact = triangle_wn_fp64(torch.tensor(V, device=dev), torch.tensor(F, device=dev),
+                             Q[idx].double())
+    acc = {"wn_rel_l2_vs_fp64_dipole_sum": rel_l2(out[idx], ref),
+           "dipole_sum_rel_l2_vs_exact_triangles": rel_l2(ref, exact),
+           "sample_queries": args.accuracy_queries}
+
+    def timed(fn, reps=3):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return r, e0.elapsed_time(e1) / reps
+
+    others = {}
+    if not args.no_direct:
+        bf, bf_ms = timed(lambda: wn.eval_brute_force_wn(Pc, Nn, A, Q), reps=1)
+        others["brute_force"] = {"ms": bf_ms, "speedup_tree_vs_brute": bf_ms / ms_step,
+                                 "rel_l2_vs_fp64": rel_l2(bf[idx], ref),
+                                 "tflops": FLOPS_WN * m * c.shape[0] / (bf_ms * 1e-3) / 1e12}
+    lb, nb = ffi.build_octree8_buffers(Pc, Nn, A, max_depth=7)
+    cl, cl_ms = timed(lambda: wn.eval_barnes_hut_wn(lb, nb, Q, beta=args.beta))
+    others["octree8_classic_depth7"] = {"ms": cl_ms, "rel_l2_vs_fp64": rel_l2(cl[idx], ref)}
+    sb, sb_ms = timed(lambda: wn.multi_level_prefix_control_variate_wn(
+        lb, nb, Q, samples_per_subdomain=args.wn_samples, seed=0, beta=1.0), reps=1)
+    others["octree8_stochastic_depth7"] = {"ms": sb_ms, "samples_per_subdomain":
+                                           args.wn_samples,
+                                           "rel_l2_vs_fp64": rel_l2(sb[idx], ref)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline_wn(c, nrm, a, q, leaf, m)
+    achieved = flops / (ms_step * 1e-3) / 1e12
+    return {
+        "metric": "winding numbers/sec (classic BH, theta=0.5)",
+        "value": m / (ms_step * 1e-3),
+        "unit": "winding numbers/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"perturbed_geodesic_sphere_faces{F.shape[0]}_queries{m}"
+                               f"_theta0.5_leaf{leaf}",
+                   "faces": int(F.shape[0]), "queries": m, "beta_diag": args.beta,
+                   "max_depth": 10, "leaf_size": leaf,
+                   "step": "element tree build + dipole moments + query tree + group walk"},
+        "interactions": {"far": s[0], "near": s[1], "groups": s[2], "nodes": s[3]},
+        "accuracy": acc,
+        "others": others,
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
+                     "kernel": "k_wn_walk (step time, build included)", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+    }
 
 
 if __name__ == "__main__":

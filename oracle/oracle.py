@@ -307,16 +307,6 @@ def triangle_wn(verts, faces, queries, chunk=256):
     return out
 
 
-def query_order(queries, bits=10):
-    """Stable Morton order of a query cloud in its own cube (default bounds)."""
-    q = _c(queries, np.float32)
-    if q.shape[0] == 0:
-        return np.zeros(0, np.uint32)
-    lo, hi = default_bounds(q)
-    codes = morton_codes(q, lo, span(lo, hi), bits)
-    return stable_order(codes, 3 * bits)
-
-
 # ---------------------------------------------------------------- adaptive tree
 class Tree:
     """Adaptive Morton octree built by the oracle (topology in numpy arrays)."""
@@ -447,28 +437,29 @@ class Tree:
     def wn_eval(self, normals, queries, *, beta=2.0, group_size=64):
         """Winding numbers at `queries` (input order) with the group walk: the
         tree's points are dipole positions, its masses the areas, `normals`
-        (input order) the unit normals.  Queries are grouped in runs of
-        group_size in their own Morton order (query_order)."""
+        (input order) the unit normals.  The queries get a tree of their own
+        (depth 10, leaf size group_size, own cube) whose cell groups
+        (orc_tree_groups) are walked against this tree."""
         ns = _c(normals, np.float64)[self.perm]
         area, cen, dip = self.wn_moments(ns)
         q = _c(queries, np.float32)
         nq = q.shape[0]
-        qperm = query_order(q)
-        qs = q[qperm]
-        ng = (nq + group_size - 1) // group_size
-        gs = (np.arange(ng) * group_size).astype(np.int32)
-        gc = np.minimum(group_size, nq - gs).astype(np.int32)
-        boxes = np.zeros(max(ng, 1) * 6, np.float32)
-        lib().orc_group_boxes(qs, ng, gs, gc, boxes)
         out = np.zeros(nq, np.float64)
+        if nq == 0:
+            return dict(wn=out, far=0, near=0, groups=0)
+        QT = Tree(q, np.zeros(nq, np.float32), max_depth=10, leaf_size=group_size)
+        gs, gc = QT.groups(group_size)
+        boxes = QT.group_boxes(QT.points_sorted, group_size)
+        ng = len(gs)
         counts = (_i64 * 2)()
         lib().orc_wn_eval(self.num_nodes, self.start, self.count, self.first_child, self.nchild,
                           _c(self.geo, np.float32).ravel(), area, cen.ravel(), dip.ravel(),
                           self.points_sorted.astype(np.float64), ns,
                           self.masses_sorted.astype(np.float64), boxes,
-                          qs.astype(np.float64), nq, ng, gs, gc, float(beta), out, counts)
+                          QT.points_sorted.astype(np.float64), nq, ng, gs, gc, float(beta), out,
+                          counts)
         res = np.zeros(nq, np.float64)
-        res[qperm] = out
+        res[QT.perm] = out
         return dict(wn=res, far=int(counts[0]), near=int(counts[1]), groups=int(ng))
 
 

@@ -743,15 +743,6 @@ def test_tree_wn_huge_beta_is_bruteforce_and_beta2_is_close():
     assert O.rel_l2(r["wn"], ref) < 2e-3
 
 
-def test_query_order_is_stable_morton_order():
-    q = np.random.default_rng(4).uniform(-2, 3, (500, 3)).astype(np.float32)
-    q[10] = q[20]  # equal codes keep input order
-    perm = O.query_order(q)
-    lo, hi = O.default_bounds(q)
-    codes = O.morton_codes(q, lo, O.span(lo, hi), 10)
-    np.testing.assert_array_equal(perm, np.argsort(codes, kind="stable"))
-
-
 def test_sbh_wn_unbiased_and_shallow_exact():
     _, _, c, n, a = _sphere_dipoles(8, seed=7)
     lo, hi = O.default_bounds(c)
