@@ -207,3 +207,142 @@ extern "C" int sbh_profile_collect(double* out, int32_t count) {
   for (int i = 0; i < count && i < 6; ++i) out[i] = acc[i];
   return rc;
 }
+
+// ---------------------------------------------------------------- batches
+namespace {
+
+int check_batch(const SbhBuffer* points, const SbhBuffer* set_offsets, int* nsets) {
+  if (!buffer_ok(set_offsets, SBH_DTYPE_S32, 1) || set_offsets->dims[0] < 2)
+    return fail_invalid("set_offsets must be an int32 vector of num_sets + 1 entries");
+  const int64_t s = set_offsets->dims[0] - 1;
+  if (s > (int64_t{1} << 20)) return fail_invalid("too many sets");
+  if (points->dims[0] > INT32_MAX) return fail_invalid("too many points");
+  *nsets = static_cast<int>(s);
+  return SBH_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t sbh_gravity_batch_workspace_bytes(int64_t total_points, int32_t num_sets,
+                                                     int32_t max_depth, int32_t leaf_size) {
+  if (total_points < 0 || num_sets < 1 || max_depth < 1 || max_depth > 10 || leaf_size < 1)
+    return -1;
+  return static_cast<int64_t>(tree::layout(total_points, max_depth, leaf_size, num_sets).total);
+}
+
+extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
+                                       const SbhBuffer* masses, const SbhBuffer* set_offsets,
+                                       SbhBuffer* forces, float beta, float softening_scale,
+                                       int32_t max_depth, int32_t leaf_size,
+                                       int32_t multipole_order, int32_t walk_blocks,
+                                       SbhBuffer* workspace, SbhBuffer* stats) {
+  clear_error();
+  int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
+                        multipole_order);
+  if (rc != SBH_OK) return rc;
+  int nsets = 0;
+  if ((rc = check_batch(points, set_offsets, &nsets)) != SBH_OK) return rc;
+  const int64_t n = points->dims[0];
+  if (!buffer_ok(forces, SBH_DTYPE_F32, 2) || forces->dims[0] != n || forces->dims[1] != 3)
+    return fail_invalid("forces shape mismatch");
+  if ((rc = check_stats(stats)) != SBH_OK) return rc;
+  if (walk_blocks < 0) return fail_invalid("walk_blocks must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
+  char* ws = nullptr;
+  if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
+                          static_cast<const float*>(masses->data), nullptr, n,
+                          static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
+  if (rc != SBH_OK) return rc;
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
+                  static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr, nullptr,
+                  walk_blocks);
+  if (rc != SBH_OK) return rc;
+  if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
+  return SBH_OK;
+}
+
+extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points,
+                                           const SbhBuffer* masses, const SbhBuffer* cotangent,
+                                           const SbhBuffer* set_offsets, SbhBuffer* forces,
+                                           SbhBuffer* grad_points, SbhBuffer* grad_masses,
+                                           float beta, float softening_scale, int32_t max_depth,
+                                           int32_t leaf_size, int32_t multipole_order,
+                                           int32_t walk_blocks, SbhBuffer* workspace,
+                                           SbhBuffer* stats) {
+  clear_error();
+  int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
+                        multipole_order);
+  if (rc != SBH_OK) return rc;
+  int nsets = 0;
+  if ((rc = check_batch(points, set_offsets, &nsets)) != SBH_OK) return rc;
+  const int64_t n = points->dims[0];
+  if (!buffer_ok(cotangent, SBH_DTYPE_F32, 2) || cotangent->dims[0] != n ||
+      cotangent->dims[1] != 3)
+    return fail_invalid("cotangent shape mismatch");
+  if (forces != nullptr &&
+      (!buffer_ok(forces, SBH_DTYPE_F32, 2) || forces->dims[0] != n || forces->dims[1] != 3))
+    return fail_invalid("forces shape mismatch");
+  if (!buffer_ok(grad_points, SBH_DTYPE_F32, 2) || grad_points->dims[0] != n ||
+      grad_points->dims[1] != 3)
+    return fail_invalid("grad_points shape mismatch");
+  if (!buffer_ok(grad_masses, SBH_DTYPE_F32, -1) || elem_count(grad_masses) != n)
+    return fail_invalid("grad_masses shape mismatch");
+  if ((rc = check_stats(stats)) != SBH_OK) return rc;
+  if (walk_blocks < 0) return fail_invalid("walk_blocks must be >= 0");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
+  char* ws = nullptr;
+  if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
+                          static_cast<const float*>(masses->data),
+                          static_cast<const float*>(cotangent->data), n,
+                          static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
+  if (rc != SBH_OK) return rc;
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
+                  forces ? static_cast<float*>(forces->data) : nullptr,
+                  static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
+                  stats != nullptr, nullptr, walk_blocks);
+  if (rc != SBH_OK) return rc;
+  if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
+  return SBH_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+// Stream capture of any sequence of library calls into a HIP graph.  Calls
+// captured this way must use caller workspaces (no library-owned buffer may
+// need to grow while capturing).
+extern "C" int sbh_graph_begin(void* stream) {
+  clear_error();
+  SBH_HIP_TRY(hipStreamBeginCapture(static_cast<hipStream_t>(stream),
+                                    hipStreamCaptureModeThreadLocal));
+  return SBH_OK;
+}
+
+extern "C" int sbh_graph_end(void* stream, void** graph_exec) {
+  clear_error();
+  if (graph_exec == nullptr) return fail_invalid("graph_exec must not be NULL");
+  hipGraph_t g = nullptr;
+  SBH_HIP_TRY(hipStreamEndCapture(static_cast<hipStream_t>(stream), &g));
+  hipGraphExec_t ex = nullptr;
+  hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return fail_internal(e);
+  *graph_exec = ex;
+  return SBH_OK;
+}
+
+extern "C" int sbh_graph_launch(void* graph_exec, void* stream) {
+  clear_error();
+  if (graph_exec == nullptr) return fail_invalid("graph_exec must not be NULL");
+  SBH_HIP_TRY(hipGraphLaunch(static_cast<hipGraphExec_t>(graph_exec),
+                             static_cast<hipStream_t>(stream)));
+  return SBH_OK;
+}
+
+extern "C" int sbh_graph_destroy(void* graph_exec) {
+  clear_error();
+  if (graph_exec != nullptr) SBH_HIP_TRY(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph_exec)));
+  return SBH_OK;
+}

@@ -82,18 +82,19 @@ __device__ __forceinline__ Box wave_box(float x, float y, float z) {
   return Box{wave_min(x), wave_min(y), wave_min(z), wave_max(x), wave_max(y), wave_max(z)};
 }
 
-// Depth-first group traversal.  far[0..nfar) holds accepted non-empty nodes
-// when flush_far(nfar) is called (at 64 entries and at the end); near[0..nnear)
-// holds rejected non-empty leaves when flush_near(nnear) is called (at
-// kNearFlush entries and at the end).  Both callbacks run on the whole wave.
+// Depth-first group traversal from `root`.  far[0..nfar) holds accepted
+// non-empty nodes when flush_far(nfar) is called (at 64 entries and at the
+// end); near[0..nnear) holds rejected non-empty leaves when
+// flush_near(nnear) is called (at kNearFlush entries and at the end).  Both
+// callbacks run on the whole wave.
 template <class FarFn, class NearFn>
 __device__ __forceinline__ void traverse(const Header* hdr, const int4* topo, const float4* geo,
                                          const float4* mp, float beta, const Box& b, int lane,
-                                         int* stack, int* far, int* near, FarFn&& flush_far,
-                                         NearFn&& flush_near) {
+                                         int root, int* stack, int* far, int* near,
+                                         FarFn&& flush_far, NearFn&& flush_near) {
   const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int top = 1, nfar = 0, nnear = 0;
-  if (lane == 0) stack[0] = 0;
+  if (lane == 0) stack[0] = root;
   __builtin_amdgcn_wave_barrier();
   // Every node is pushed at most once per group and each iteration pops at
   // least one, so num_nodes iterations bound the loop even for a corrupt tree.

@@ -91,72 +91,210 @@ __global__ __launch_bounds__(kB) void k_bbox_reduce(const float* __restrict__ pa
   }
 }
 
-// Writes the cube geometry from the bounds (min xyz, max xyz), initialises the
-// root and the zero-mass sentinel node used to pad interaction batches.
-__global__ __launch_bounds__(64) void k_bbox_final(int64_t n, Header* __restrict__ hdr,
-                                                   int4* __restrict__ topo,
-                                                   uint32_t* __restrict__ prefix,
-                                                   float4* __restrict__ geo,
-                                                   int32_t* __restrict__ parent,
-                                                   float4* __restrict__ mp,
-                                                   float4* __restrict__ qd, int sentinel,
-                                                   const float* __restrict__ bounds) {
-  if (threadIdx.x != 0) return;
-  float lo[3], hi[3];
-  for (int a = 0; a < 3; ++a) {
-    lo[a] = bounds[a];
-    hi[a] = bounds[3 + a];
+// Order-preserving map of floats to unsigned ints (for atomic min / max).
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ __launch_bounds__(kB) void k_setbox_init(uint32_t* __restrict__ box, int nsets) {
+  for (int i = blockIdx.x * kB + threadIdx.x; i < 6 * nsets; i += gridDim.x * kB)
+    box[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+}
+
+// Per-set data bounds of the input rows (sets are contiguous row ranges):
+// a block whose rows all belong to one set reduces in LDS and issues six
+// atomics; otherwise every wave with a single set reduces across lanes, and
+// mixed waves fall back to per-lane atomics.
+__global__ __launch_bounds__(kB) void k_set_bbox(const float* __restrict__ pts, int64_t n,
+                                                 const int32_t* __restrict__ offs, int nsets,
+                                                 uint32_t* __restrict__ box) {
+  __shared__ float sh[6][kB];
+  const int lane = threadIdx.x & 63;
+  for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kB; b0 < n;
+       b0 += static_cast<int64_t>(gridDim.x) * kB) {
+    const int64_t i = b0 + threadIdx.x;
+    const bool ok = i < n;
+    const int64_t last = min<int64_t>(b0 + kB, n) - 1;
+    const int s_first = set_of(offs, nsets, static_cast<int>(b0));
+    const int s_last = set_of(offs, nsets, static_cast<int>(last));
+    float v[6];
+    for (int a = 0; a < 3; ++a) {
+      const float x = ok ? pts[3 * i + a] : 0.0f;
+      v[a] = ok ? x : INFINITY;
+      v[3 + a] = ok ? x : -INFINITY;
+    }
+    if (s_first == s_last) {
+      for (int a = 0; a < 6; ++a) sh[a][threadIdx.x] = v[a];
+      __syncthreads();
+      for (int off = kB / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < static_cast<unsigned>(off))
+          for (int a = 0; a < 3; ++a) {
+            sh[a][threadIdx.x] = fminf(sh[a][threadIdx.x], sh[a][threadIdx.x + off]);
+            sh[3 + a][threadIdx.x] = fmaxf(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + off]);
+          }
+        __syncthreads();
+      }
+      if (threadIdx.x < 6) {
+        const uint32_t u = f2ord(sh[threadIdx.x][0]);
+        if (threadIdx.x < 3)
+          atomicMin(&box[6 * s_first + threadIdx.x], u);
+        else
+          atomicMax(&box[6 * s_first + threadIdx.x], u);
+      }
+      __syncthreads();
+      continue;
+    }
+    const int my = ok ? set_of(offs, nsets, static_cast<int>(i)) : -1;
+    const int w0 = __shfl(my, 0, 64);
+    const bool uniform = __all(my == w0 || my < 0) && w0 >= 0;
+    if (uniform) {
+      for (int d = 32; d > 0; d >>= 1)
+        for (int a = 0; a < 3; ++a) {
+          v[a] = fminf(v[a], __shfl_xor(v[a], d, 64));
+          v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], d, 64));
+        }
+      if (lane < 6) {
+        float val = v[0];
+        for (int a = 1; a < 6; ++a) val = (lane == a) ? v[a] : val;
+        const uint32_t u = f2ord(val);
+        if (lane < 3)
+          atomicMin(&box[6 * w0 + lane], u);
+        else
+          atomicMax(&box[6 * w0 + lane], u);
+      }
+    } else if (ok) {
+      for (int a = 0; a < 3; ++a) {
+        atomicMin(&box[6 * my + a], f2ord(v[a]));
+        atomicMax(&box[6 * my + 3 + a], f2ord(v[3 + a]));
+      }
+    }
   }
-  CubeGeom g;
-  g.min_x = lo[0];
-  g.min_y = lo[1];
-  g.min_z = lo[2];
-  g.span = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
-  g.cx = lo[0] + g.span * 0.5f;
-  g.cy = lo[1] + g.span * 0.5f;
-  g.cz = lo[2] + g.span * 0.5f;
-  g.half_size = g.span * 0.5f;
-  hdr->geom = g;
-  for (int l = 0; l < kMaxLevels; ++l) {
-    hdr->level_start[l] = 0;
-    hdr->level_count[l] = 0;
+}
+
+// Initialises the forest: per-set cube geometry, one root per set (node k is
+// the root of set k), the level table and the zero-mass sentinel node.  The
+// bounds come either as six final floats (one set: min xyz, max xyz) or as
+// the per-set atomics of k_set_bbox (expanded by 1e-3 here; an empty set
+// gets the cube of the origin).
+__global__ __launch_bounds__(kB) void k_forest_init(
+    int64_t n, int nsets, const float* __restrict__ bounds6,
+    const uint32_t* __restrict__ setbox, int32_t* __restrict__ offs, CubeGeom* __restrict__ geoms,
+    Header* __restrict__ hdr, int4* __restrict__ topo, uint32_t* __restrict__ prefix,
+    float4* __restrict__ geo, int32_t* __restrict__ parent, float4* __restrict__ mp,
+    float4* __restrict__ qd, int sentinel) {
+  __shared__ int bad;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  if (setbox == nullptr) {
+    if (threadIdx.x == 0) {
+      offs[0] = 0;
+      offs[1] = static_cast<int32_t>(n);
+    }
+  } else {
+    // offsets must run monotonically from 0 to n; otherwise every set is
+    // left empty and the header reports overflow = 2
+    for (int k = threadIdx.x; k < nsets; k += kB)
+      if (offs[k] < 0 || offs[k] > offs[k + 1] || offs[k + 1] > n) atomicOr(&bad, 1);
+    if (threadIdx.x == 0 && (offs[0] != 0 || offs[nsets] != n)) atomicOr(&bad, 1);
   }
-  hdr->level_count[0] = 1;
-  hdr->num_nodes = 1;
-  hdr->overflow = 0;
-  for (int k = 0; k < 8; ++k) hdr->stats[k] = 0;
-  topo[0] = make_int4(0, static_cast<int>(n), -1, 0);
-  prefix[0] = 0;
-  parent[0] = -1;
-  geo[0] = cell_geometry(g, 0u, 0);
-  mp[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
-  qd[2 * sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
-  qd[2 * sentinel + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nsets; k += kB) {
+    float lo[3], hi[3];
+    int32_t s0, cnt;
+    if (setbox == nullptr) {
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = bounds6[a];
+        hi[a] = bounds6[3 + a];
+      }
+      s0 = 0;
+      cnt = static_cast<int32_t>(n);
+    } else {
+      s0 = bad ? 0 : offs[k];
+      cnt = bad ? 0 : offs[k + 1] - s0;
+      for (int a = 0; a < 3; ++a) {
+        const float l = cnt > 0 ? ord2f(setbox[6 * k + a]) : 0.0f;
+        const float h = cnt > 0 ? ord2f(setbox[6 * k + 3 + a]) : 0.0f;
+        lo[a] = l - 1e-3f;
+        hi[a] = h + 1e-3f;
+      }
+    }
+    CubeGeom g;
+    g.min_x = lo[0];
+    g.min_y = lo[1];
+    g.min_z = lo[2];
+    g.span = fmaxf(fmaxf(hi[0] - lo[0], hi[1] - lo[1]), hi[2] - lo[2]);
+    g.cx = lo[0] + g.span * 0.5f;
+    g.cy = lo[1] + g.span * 0.5f;
+    g.cz = lo[2] + g.span * 0.5f;
+    g.half_size = g.span * 0.5f;
+    geoms[k] = g;
+    topo[k] = make_int4(s0, cnt, -1, 0);
+    prefix[k] = 0;
+    parent[k] = -1;
+    geo[k] = cell_geometry(g, 0u, 0);
+    if (k == 0) hdr->geom = g;
+  }
+  if (threadIdx.x == 0) {
+    for (int l = 0; l < kMaxLevels; ++l) {
+      hdr->level_start[l] = 0;
+      hdr->level_count[l] = 0;
+    }
+    hdr->level_count[0] = nsets;
+    hdr->num_nodes = nsets;
+    hdr->overflow = bad ? 2 : 0;
+    for (int k = 0; k < 8; ++k) hdr->stats[k] = 0;
+    mp[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+    qd[2 * sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+    qd[2 * sentinel + 1] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 }
 
 // ------------------------------------------------------------------ codes
 __global__ __launch_bounds__(kB) void k_codes(const float* __restrict__ pts, int64_t n,
-                                              const Header* __restrict__ hdr, int bits,
-                                              uint32_t* __restrict__ codes,
+                                              const CubeGeom* __restrict__ geoms,
+                                              const int32_t* __restrict__ offs, int nsets,
+                                              int bits, uint32_t* __restrict__ codes_in,
+                                              uint32_t* __restrict__ keys,
                                               uint32_t* __restrict__ idx) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (i >= n) return;
-  const CubeGeom g = hdr->geom;
+  const CubeGeom g = geoms[nsets > 1 ? set_of(offs, nsets, static_cast<int>(i)) : 0];
   uint32_t qx = quantize_axis(pts[3 * i + 0], g.min_x, g.span, bits);
   uint32_t qy = quantize_axis(pts[3 * i + 1], g.min_y, g.span, bits);
   uint32_t qz = quantize_axis(pts[3 * i + 2], g.min_z, g.span, bits);
-  codes[i] = morton3(qx, qy, qz);
+  const uint32_t c = morton3(qx, qy, qz);
+  codes_in[i] = c;
+  keys[i] = c;
   idx[i] = static_cast<uint32_t>(i);
+}
+
+// Second sort pass of a forest: (set of the point, current order).
+__global__ __launch_bounds__(kB) void k_setkeys(const uint32_t* __restrict__ perm, int64_t n,
+                                                const int32_t* __restrict__ offs, int nsets,
+                                                uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t i = perm[r];
+  keys[r] = static_cast<uint32_t>(set_of(offs, nsets, static_cast<int>(i)));
+  vals[r] = i;
 }
 
 __global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
                                                const float* __restrict__ ms,
                                                const float* __restrict__ cot, int64_t n,
                                                const uint32_t* __restrict__ perm,
+                                               const uint32_t* __restrict__ codes_in,
+                                               uint32_t* __restrict__ codes,
                                                float4* __restrict__ sp, float4* __restrict__ sc) {
   int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (r >= n) return;
   int64_t i = perm[r];
+  codes[r] = codes_in[i];
   sp[r] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], ms != nullptr ? ms[i] : 0.0f);
   if (cot != nullptr) sc[r] = make_float4(cot[3 * i], cot[3 * i + 1], cot[3 * i + 2], 0.0f);
 }
@@ -249,13 +387,14 @@ __global__ __launch_bounds__(kB) void k_emit(Header* __restrict__ hdr, int level
                                              int4* __restrict__ topo, uint32_t* __restrict__ prefix,
                                              float4* __restrict__ geo,
                                              int32_t* __restrict__ parent,
-                                             const uint32_t* __restrict__ scan) {
+                                             const uint32_t* __restrict__ scan,
+                                             const CubeGeom* __restrict__ geoms,
+                                             const int32_t* __restrict__ offs, int nsets) {
   const int start = hdr->level_start[level];
   const int count = hdr->level_count[level];
   const int next = hdr->level_start[level + 1];
   const int next_count = hdr->level_count[level + 1];
   const int shift = 3 * (max_depth - level - 1);
-  const CubeGeom g = hdr->geom;
   for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
     int k = start + u;
     int4 t = topo[k];
@@ -266,6 +405,7 @@ __global__ __launch_bounds__(kB) void k_emit(Header* __restrict__ hdr, int level
       continue;
     }
     uint32_t pre = prefix[k];
+    const CubeGeom g = geoms[nsets > 1 ? set_of(offs, nsets, t.x) : 0];
     int b[9];
     octant_bounds(codes, t, pre, shift, b);
     int first = next + static_cast<int>(scan[u]);
@@ -364,14 +504,14 @@ __global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, 
 // points that stay inside one cell: for each node holding more than
 // kGroupMax points, runs of consecutive children holding <= kGroupMax points
 // each are packed greedily; an over-full leaf (depth limit) is cut into
-// chunks; a root holding <= kGroupMax points is the only group.
+// chunks; a root holding <= kGroupMax points is its set's only group.
 template <bool EMIT>
-__device__ __forceinline__ int node_groups(const int4* __restrict__ topo, int k,
+__device__ __forceinline__ int node_groups(const int4* __restrict__ topo, int k, int nroots,
                                            int2* __restrict__ out) {
   const int gmax = kGroupMax;
   const int4 t = topo[k];
   int ng = 0;
-  if (k == 0 && t.y <= gmax) {
+  if (k < nroots && t.y <= gmax) {
     if (t.y > 0) {
       if (EMIT) out[0] = make_int2(t.x, t.y);
       ng = 1;
@@ -416,8 +556,9 @@ __global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ h
                                                     const int4* __restrict__ topo,
                                                     uint32_t* __restrict__ goff) {
   const int nn = hdr->num_nodes;
+  const int nroots = hdr->level_count[0];
   for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, k, nullptr));
+    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, k, nroots, nullptr));
 }
 
 // Exclusive scan of the per-node group counts (one block); publishes the
@@ -456,8 +597,9 @@ __global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hd
                                                    const uint32_t* __restrict__ goff,
                                                    int2* __restrict__ groups) {
   const int nn = hdr->num_nodes;
+  const int nroots = hdr->level_count[0];
   for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    node_groups<true>(topo, k, groups + goff[k]);
+    node_groups<true>(topo, k, nroots, groups + goff[k]);
 }
 
 // ------------------------------------------------------------------ wave helpers
@@ -531,6 +673,8 @@ struct WalkArgs {
   const float4* sp;
   const float4* sc;
   const uint32_t* perm;
+  const int32_t* offs;  // set offsets (sorted order), nsets + 1 entries
+  int nsets;
   int n;
   int sentinel;  // zero-mass node used to pad far batches
   float beta;
@@ -762,7 +906,8 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
     }
   };
 
-  traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, stack, far, near, flush_far,
+  const int root = A.nsets > 1 ? set_of(A.offs, A.nsets, base) : 0;
+  traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, root, stack, far, near, flush_far,
            flush_near);
 
   if (valid) {
@@ -828,6 +973,7 @@ __global__ __launch_bounds__(kB) void k_zero_adjoint(const Header* __restrict__ 
 // Push node adjoints down to the points and write gradients in input order.
 template <bool QUAD>
 __global__ __launch_bounds__(kB) void k_finalize(const Header* __restrict__ hdr, int64_t n,
+                                                 const int32_t* __restrict__ offs, int nsets,
                                                  const int4* __restrict__ topo,
                                                  const float4* __restrict__ mp,
                                                  const float4* __restrict__ sp,
@@ -846,8 +992,8 @@ __global__ __launch_bounds__(kB) void k_finalize(const Header* __restrict__ hdr,
   float ay = q.y + sgrad[4 * r + 1];
   float az = q.z + sgrad[4 * r + 2];
   float am = sgrad[4 * r + 3];
-  int node = 0;
   const int rr = static_cast<int>(r);
+  int node = nsets > 1 ? set_of(offs, nsets, rr) : 0;  // the root of the point's set
   for (int depth = 0; depth < kMaxLevels; ++depth) {
     const float4 m = mp[node];
     if (m.w != 0.0f) {
@@ -901,11 +1047,14 @@ __global__ void k_copy_stats(const Header* __restrict__ hdr, int64_t* __restrict
 
 }  // namespace
 
-int64_t node_capacity(int64_t n, int max_depth, int leaf_size) {
-  int64_t cap = 1;
+int64_t node_capacity(int64_t n, int max_depth, int leaf_size, int nsets) {
+  // Level 0 holds the roots; every deeper level holds at most 8 children per
+  // split node, nodes of one level are disjoint, and a split node holds more
+  // than leaf_size points.
+  int64_t cap = nsets;
   int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
   for (int l = 1; l <= max_depth; ++l) {
-    int64_t a = (l < 21) ? (int64_t{1} << (3 * l)) : INT64_MAX;
+    int64_t a = (l < 18) ? static_cast<int64_t>(nsets) << (3 * l) : INT64_MAX;
     int64_t b = 8 * internal_max;
     int64_t c = n;
     cap += std::min(a, std::min(b, c));
@@ -913,11 +1062,12 @@ int64_t node_capacity(int64_t n, int max_depth, int leaf_size) {
   return cap;
 }
 
-Layout layout(int64_t n, int max_depth, int leaf_size) {
+Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   Layout L;
   auto align = [](size_t x) { return (x + 255) & ~size_t{255}; };
   size_t nn = static_cast<size_t>(n > 0 ? n : 1);
-  L.cap = node_capacity(n, max_depth, leaf_size);
+  L.nsets = nsets;
+  L.cap = node_capacity(n, max_depth, leaf_size, nsets);
   size_t cap = static_cast<size_t>(L.cap) + 1;  // + the sentinel node
   size_t o = 0;
   L.header = o; o = align(o + sizeof(Header));
@@ -942,6 +1092,10 @@ Layout layout(int64_t n, int max_depth, int leaf_size) {
   L.sgrad = o; o = align(o + 16 * nn);
   L.groups = o; o = align(o + 8 * nn);
   L.goff = o; o = align(o + 4 * cap);
+  L.geoms = o; o = align(o + sizeof(CubeGeom) * static_cast<size_t>(nsets));
+  L.offs = o; o = align(o + 4 * (static_cast<size_t>(nsets) + 1));
+  L.setbox = o; o = align(o + 24 * static_cast<size_t>(nsets));
+  L.codes_in = o; o = align(o + 4 * nn);
   L.total = o;
   return L;
 }
@@ -957,11 +1111,15 @@ int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** 
   return SBH_OK;
 }
 
-int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
-          const float* cot, int64_t n, int max_depth, int leaf_size,
-          const float* user_bounds, int parts) {
+namespace {
+
+// Everything after the roots exist: codes, sort, gather, levels, moments, groups.
+int build_rest(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+               const float* cot, int64_t n, const int32_t* offs, int max_depth,
+               int leaf_size, int parts) {
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
   auto* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
+  auto* codes_in = reinterpret_cast<uint32_t*>(ws + L.codes_in);
   auto* perm = reinterpret_cast<uint32_t*>(ws + L.perm);
   auto* keys = reinterpret_cast<uint32_t*>(ws + L.keys_tmp);
   auto* vals = reinterpret_cast<uint32_t*>(ws + L.vals_tmp);
@@ -974,43 +1132,44 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
   auto* nch = reinterpret_cast<uint32_t*>(ws + L.nch);
   auto* sp = reinterpret_cast<float4*>(ws + L.sp);
   auto* sc = reinterpret_cast<float4*>(ws + L.sc);
-  auto* bbox = reinterpret_cast<float*>(ws + L.bbox_tmp);
-
-  if (user_bounds == nullptr) {
-    float* b = nullptr;
-    int rc = data_bounds(s, pts, n, bbox, &b);
-    if (rc != SBH_OK) return rc;
-    user_bounds = b;
-  }
-  k_bbox_final<<<1, 64, 0, s>>>(n, hdr, topo, prefix, geo, parent, mp, qd,
-                                static_cast<int>(L.cap), user_bounds);
-  SBH_LAUNCH_CHECK();
+  auto* geoms = reinterpret_cast<CubeGeom*>(ws + L.geoms);
+  const int nsets = L.nsets;
   if (n > 0) {
-    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, hdr, max_depth, keys, vals);
+    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, geoms, offs, nsets, max_depth, codes_in, keys,
+                                           vals);
     SBH_LAUNCH_CHECK();
     int rc = radix_sort_pairs(s, keys, vals, codes, perm, n, 3 * max_depth, ws + L.sort_tmp);
     if (rc != SBH_OK) return rc;
-    k_gather<<<grid_for(n, kB), kB, 0, s>>>(pts, ms, cot, n, perm, sp, sc);
+    if (nsets > 1) {
+      // stable pass on the set id: order (set, code, input index)
+      int set_bits = 0;
+      while ((1 << set_bits) < nsets) ++set_bits;
+      k_setkeys<<<grid_for(n, kB), kB, 0, s>>>(perm, n, offs, nsets, keys, vals);
+      SBH_LAUNCH_CHECK();
+      rc = radix_sort_pairs(s, keys, vals, codes, perm, n, set_bits, ws + L.sort_tmp);
+      if (rc != SBH_OK) return rc;
+    }
+    k_gather<<<grid_for(n, kB), kB, 0, s>>>(pts, ms, cot, n, perm, codes_in, codes, sp, sc);
     SBH_LAUNCH_CHECK();
   }
   int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+  auto level_cap = [&](int l) {
+    return std::min<int64_t>(
+        l < 18 ? static_cast<int64_t>(nsets) << (3 * l) : INT64_MAX,
+        std::max<int64_t>(nsets, std::min<int64_t>(n, 8 * internal_max)));
+  };
   for (int l = 0; l < max_depth; ++l) {
-    int64_t lvl_cap = std::min<int64_t>(
-        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
-        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
-    unsigned g = grid_for(lvl_cap, kB, 4096);
+    unsigned g = grid_for(level_cap(l), kB, 4096);
     k_split<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, nch);
     SBH_LAUNCH_CHECK();
     k_scan_level<<<1, 1024, 0, s>>>(hdr, l, nch, static_cast<int>(L.cap));
     SBH_LAUNCH_CHECK();
-    k_emit<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, geo, parent, nch);
+    k_emit<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, geo, parent, nch,
+                            geoms, offs, nsets);
     SBH_LAUNCH_CHECK();
   }
   for (int l = max_depth; l >= 0 && (parts & kBuildMoments); --l) {
-    int64_t lvl_cap = std::min<int64_t>(
-        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
-        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
-    k_moments<<<grid_for(lvl_cap, kB, 4096), kB, 0, s>>>(hdr, l, topo, geo, sp, mp, qd);
+    k_moments<<<grid_for(level_cap(l), kB, 4096), kB, 0, s>>>(hdr, l, topo, geo, sp, mp, qd);
     SBH_LAUNCH_CHECK();
   }
   if (!(parts & kBuildGroups)) return SBH_OK;
@@ -1026,9 +1185,58 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
   return SBH_OK;
 }
 
+int init_roots(hipStream_t s, char* ws, const Layout& L, int64_t n, const float* bounds6,
+               const uint32_t* setbox) {
+  k_forest_init<<<1, kB, 0, s>>>(
+      n, L.nsets, bounds6, setbox, reinterpret_cast<int32_t*>(ws + L.offs),
+      reinterpret_cast<CubeGeom*>(ws + L.geoms), reinterpret_cast<Header*>(ws + L.header),
+      reinterpret_cast<int4*>(ws + L.topo), reinterpret_cast<uint32_t*>(ws + L.prefix),
+      reinterpret_cast<float4*>(ws + L.geo), reinterpret_cast<int32_t*>(ws + L.parent),
+      reinterpret_cast<float4*>(ws + L.mp), reinterpret_cast<float4*>(ws + L.qd),
+      static_cast<int>(L.cap));
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+}  // namespace
+
+int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+          const float* cot, int64_t n, int max_depth, int leaf_size,
+          const float* user_bounds, int parts) {
+  if (L.nsets != 1) return fail_internal_msg("build: layout is a forest");
+  if (user_bounds == nullptr) {
+    float* b = nullptr;
+    int rc = data_bounds(s, pts, n, reinterpret_cast<float*>(ws + L.bbox_tmp), &b);
+    if (rc != SBH_OK) return rc;
+    user_bounds = b;
+  }
+  int rc = init_roots(s, ws, L, n, user_bounds, nullptr);
+  if (rc != SBH_OK) return rc;
+  return build_rest(s, ws, L, pts, ms, cot, n, reinterpret_cast<const int32_t*>(ws + L.offs),
+                    max_depth, leaf_size, parts);
+}
+
+int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+                 const float* cot, int64_t n, const int32_t* offs, int max_depth,
+                 int leaf_size, int parts) {
+  auto* box = reinterpret_cast<uint32_t*>(ws + L.setbox);
+  auto* ofs = reinterpret_cast<int32_t*>(ws + L.offs);
+  SBH_HIP_TRY(hipMemcpyAsync(ofs, offs, 4 * (static_cast<size_t>(L.nsets) + 1),
+                             hipMemcpyDeviceToDevice, s));
+  k_setbox_init<<<grid_for(6 * L.nsets, kB, 64), kB, 0, s>>>(box, L.nsets);
+  SBH_LAUNCH_CHECK();
+  if (n > 0) {
+    k_set_bbox<<<grid_for(n, kB, 1024), kB, 0, s>>>(pts, n, ofs, L.nsets, box);
+    SBH_LAUNCH_CHECK();
+  }
+  int rc = init_roots(s, ws, L, n, nullptr, box);
+  if (rc != SBH_OK) return rc;
+  return build_rest(s, ws, L, pts, ms, cot, n, ofs, max_depth, leaf_size, parts);
+}
+
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
          bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
-         bool want_stats, hipEvent_t* ev_walk) {
+         bool want_stats, hipEvent_t* ev_walk, int walk_blocks) {
   if (n == 0) return SBH_OK;
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
   WalkArgs A;
@@ -1041,6 +1249,8 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   A.sp = reinterpret_cast<const float4*>(ws + L.sp);
   A.sc = reinterpret_cast<const float4*>(ws + L.sc);
   A.perm = reinterpret_cast<const uint32_t*>(ws + L.perm);
+  A.offs = reinterpret_cast<const int32_t*>(ws + L.offs);
+  A.nsets = L.nsets;
   A.n = static_cast<int>(n);
   A.sentinel = static_cast<int>(L.cap);
   A.beta = beta;
@@ -1059,7 +1269,7 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   }
   // Persistent waves drain the group queue: enough blocks to fill every CU
   // several times over, never more than one wave per point.
-  unsigned grid = grid_for(n, 64 * kWavesPerBlock, 256 * 8);
+  unsigned grid = grid_for(n, 64 * kWavesPerBlock, walk_blocks > 0 ? walk_blocks : 256 * 8);
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[0], s));
   const bool f = forces != nullptr;
   if (vjp && quad && f)
@@ -1078,13 +1288,13 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[1], s));
   if (vjp) {
     if (quad)
-      k_finalize<true><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.topo, A.mp, A.sp, A.perm,
-                                                       A.qgrad, A.sgrad, A.adj, A.adjq,
-                                                       grad_points, grad_masses);
+      k_finalize<true><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.offs, A.nsets, A.topo, A.mp,
+                                                       A.sp, A.perm, A.qgrad, A.sgrad, A.adj,
+                                                       A.adjq, grad_points, grad_masses);
     else
-      k_finalize<false><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.topo, A.mp, A.sp, A.perm,
-                                                        A.qgrad, A.sgrad, A.adj, A.adjq,
-                                                        grad_points, grad_masses);
+      k_finalize<false><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.offs, A.nsets, A.topo, A.mp,
+                                                        A.sp, A.perm, A.qgrad, A.sgrad, A.adj,
+                                                        A.adjq, grad_points, grad_masses);
     SBH_LAUNCH_CHECK();
   }
   return SBH_OK;

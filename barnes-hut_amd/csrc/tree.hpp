@@ -32,9 +32,13 @@ struct Header {
 constexpr int kGroupMax = 64;  // one wavefront per query group
 
 // Byte offsets of every array inside the workspace.  Node arrays hold `cap`
-// entries (an upper bound on the node count, see node_capacity()).
+// entries (an upper bound on the node count, see node_capacity()).  A
+// workspace holds a forest of `nsets` independent trees (one per point set;
+// node k < nsets is the root of set k), one tree in the plain case.
 struct Layout {
   int64_t cap;
+  int nsets;
+  size_t geoms, offs, setbox, codes_in;  // per-set cubes, set offsets, bbox atomics
   size_t header, codes, perm, keys_tmp, vals_tmp, sort_tmp, bbox_tmp;
   size_t sp, sc;                       // sorted (x,y,z,m) and cotangent records
   size_t topo, prefix, parent, geo;    // int4 {start,count,first_child,nchild}, cell prefix
@@ -44,8 +48,22 @@ struct Layout {
   size_t total;
 };
 
-int64_t node_capacity(int64_t n, int max_depth, int leaf_size);
-Layout layout(int64_t n, int max_depth, int leaf_size);
+int64_t node_capacity(int64_t n, int max_depth, int leaf_size, int nsets = 1);
+Layout layout(int64_t n, int max_depth, int leaf_size, int nsets = 1);
+
+// Set of sorted (or input) point i given the set offsets offs[0..nsets]:
+// the largest s with offs[s] <= i.
+__device__ __forceinline__ int set_of(const int32_t* __restrict__ offs, int nsets, int i) {
+  int lo = 0, hi = nsets - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (offs[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
 
 // Data bounds of n points on the device (no host synchronisation): min - 1e-3
 // (3 floats) then max + 1e-3 (3 floats), written to tmp + 6 * kBboxBlocks and
@@ -65,13 +83,24 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
           const float* cot, int64_t n, int max_depth, int leaf_size,
           const float* user_bounds, int parts = kBuildAll);
 
+// Forest build: points are the concatenation of L.nsets sets, set k holding
+// input rows [offs[k], offs[k+1]) (device int32 array of nsets + 1 entries,
+// offs[0] = 0, offs[nsets] = n).  Every set gets its own cube (its data bounds
+// expanded by 1e-3), its own root and its own groups; no node mixes sets.
+// With one set this is build() without user bounds.
+int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
+                 const float* cot, int64_t n, const int32_t* offs, int max_depth,
+                 int leaf_size, int parts = kBuildAll);
+
 // Walk; forces (input order, N x 3) may be null; with vjp, writes grad_points
 // (N x 3) and grad_masses (N) in input order.
 // ev_walk (may be null): two events recorded immediately before and after the
 // walk kernel on stream s.
+// Works on forests: each group walks the tree of its own set.  walk_blocks
+// (0: default) caps the persistent grid.
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
          bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
-         bool want_stats, hipEvent_t* ev_walk = nullptr);
+         bool want_stats, hipEvent_t* ev_walk = nullptr, int walk_blocks = 0);
 
 // Copies {far, near, groups, nodes, levels, overflow, 0, 0} into a device int64[8].
 int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out);

@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
           w += wn_near_batch(A, q.x, q.y, q.z, j0, t.x + t.y);
       }
     };
-    traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, stack, far, near, flush_far,
+    traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack, far, near, flush_far,
              flush_near);
     if (valid) A.out[A.qperm[i]] = w;
     ++n_groups;

@@ -9,7 +9,8 @@ from ._lib import library_path, load_library  # noqa: F401
 
 __all__ = ["ffi", "library_path", "load_library"]
 from . import gravity  # noqa: E402,F401
+from . import graph  # noqa: E402,F401
 from . import synthetic  # noqa: E402,F401
 from . import wn  # noqa: E402,F401
 
-__all__ += ["gravity", "synthetic", "wn"]
+__all__ += ["gravity", "graph", "synthetic", "wn"]

@@ -131,6 +131,97 @@ def gravity_force_recompute_vjp(points, masses, *, beta: float = 2.0,
     return _GravityRecompute.apply(p, m, kw)
 
 
+# ------------------------------------------------------------------ batches
+def set_offsets(sizes, device=None) -> torch.Tensor:
+    """int32 device vector [0, n_0, n_0 + n_1, ...] for sets of the given sizes."""
+    sizes = np.asarray(sizes, np.int64)
+    if sizes.ndim != 1 or sizes.size < 1 or np.any(sizes < 0):
+        raise ValueError("sizes must be a non-empty vector of non-negative counts")
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    if offs[-1] > np.iinfo(np.int32).max:
+        raise ValueError("too many points")
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    return torch.tensor(offs.astype(np.int32), device=dev)
+
+
+def batch_workspace_bytes(total_points: int, num_sets: int, max_depth: int = 10,
+                          leaf_size: int = 32) -> int:
+    lib = _lib.load_library()
+    lib.sbh_gravity_batch_workspace_bytes.restype = ctypes.c_int64
+    v = int(lib.sbh_gravity_batch_workspace_bytes(ctypes.c_int64(total_points), i32(num_sets),
+                                                  i32(max_depth), i32(leaf_size)))
+    if v < 0:
+        raise ValueError("invalid workspace query")
+    return v
+
+
+def new_batch_workspace(total_points: int, num_sets: int, max_depth: int = 10,
+                        leaf_size: int = 32, device: Optional[torch.device] = None):
+    dev = device or torch.device("cuda", torch.cuda.current_device())
+    return torch.empty(batch_workspace_bytes(total_points, num_sets, max_depth, leaf_size),
+                       dtype=torch.uint8, device=dev)
+
+
+def _check_batch(points, masses, offsets, check_offsets):
+    if points.dim() != 2 or points.shape[1] != 3:
+        raise ValueError("points must have shape (N, 3)")
+    if masses.dim() != 1 or masses.shape[0] != points.shape[0]:
+        raise ValueError("masses must have shape (N,)")
+    if not isinstance(offsets, torch.Tensor) or offsets.dtype != torch.int32 or \
+            offsets.dim() != 1 or offsets.shape[0] < 2:
+        raise ValueError("offsets must be an int32 tensor of num_sets + 1 entries")
+    if check_offsets:
+        o = offsets.cpu().numpy()
+        if o[0] != 0 or o[-1] != points.shape[0] or np.any(np.diff(o) < 0):
+            raise ValueError("offsets must rise from 0 to N")
+
+
+def gravity_force_batch(points, masses, offsets, *, beta: float = 2.0,
+                        softening_scale: float = 1e-2, max_depth: int = 10,
+                        leaf_size: int = 32, multipole_order: int = 2, walk_blocks: int = 0,
+                        workspace: Optional[torch.Tensor] = None,
+                        stats: Optional[torch.Tensor] = None,
+                        out: Optional[torch.Tensor] = None,
+                        check_offsets: bool = True) -> torch.Tensor:
+    """Self-gravity of independent sets in one pipeline: set k is rows
+    offsets[k]:offsets[k+1] of points/masses; no set sees another.  Inside
+    a graph capture pass preallocated `out` and `workspace` and
+    check_offsets=False (the check reads the offsets back to the host)."""
+    _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
+    points, masses = _f32(points), _f32(masses)
+    _check_batch(points, masses, offsets, check_offsets)
+    forces = out if out is not None else torch.empty_like(points)
+    call("sbh_gravity_batch_force", stream_handle(), buf(points), buf(masses), buf(offsets),
+         buf(forces), f32(beta), f32(softening_scale), i32(max_depth), i32(leaf_size),
+         i32(multipole_order), i32(walk_blocks), buf(workspace), buf(stats))
+    return forces
+
+
+def gravity_force_vjp_batch(points, masses, cotangent, offsets, *, beta: float = 2.0,
+                            softening_scale: float = 1e-2, max_depth: int = 10,
+                            leaf_size: int = 32, multipole_order: int = 2,
+                            walk_blocks: int = 0, workspace: Optional[torch.Tensor] = None,
+                            stats: Optional[torch.Tensor] = None,
+                            forces_out: Optional[torch.Tensor] = None,
+                            grad_points_out: Optional[torch.Tensor] = None,
+                            grad_masses_out: Optional[torch.Tensor] = None,
+                            check_offsets: bool = True):
+    """(grad_points, grad_masses) of sum_i c_i . F_i over all sets; also fills
+    forces_out when given."""
+    _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
+    points, masses, cotangent = _f32(points), _f32(masses), _f32(cotangent)
+    _check_batch(points, masses, offsets, check_offsets)
+    if cotangent.shape != points.shape:
+        raise ValueError("cotangent must have same shape as points")
+    gx = grad_points_out if grad_points_out is not None else torch.empty_like(points)
+    gm = grad_masses_out if grad_masses_out is not None else torch.empty_like(masses)
+    call("sbh_gravity_batch_force_vjp", stream_handle(), buf(points), buf(masses),
+         buf(cotangent), buf(offsets), buf(forces_out), buf(gx), buf(gm), f32(beta),
+         f32(softening_scale), i32(max_depth), i32(leaf_size), i32(multipole_order),
+         i32(walk_blocks), buf(workspace), buf(stats))
+    return gx, gm
+
+
 def new_stats(device=None) -> torch.Tensor:
     dev = device or torch.device("cuda", torch.cuda.current_device())
     return torch.zeros(8, dtype=torch.int64, device=dev)

@@ -134,9 +134,13 @@ def main():
     ap.add_argument("--cpu-sample-groups", type=int, default=1200)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
-    ap.add_argument("--workload", default="gravity", choices=("gravity", "wn"),
+    ap.add_argument("--workload", default="gravity", choices=("gravity", "wn", "batched"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
-                         "204,020-face perturbed geodesic sphere at 2^20 queries")
+                         "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
+                         "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
+                         "graph-captured forest pipeline vs 64 per-set calls")
+    ap.add_argument("--sets", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--wn-freq", type=int, default=101,
                     help="geodesic frequency of the wn mesh (20 f^2 faces)")
     ap.add_argument("--wn-samples", type=int, default=32)
@@ -146,7 +150,8 @@ def main():
                                    "it (queries would shard by Morton range, see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    rec = bench_wn(args) if args.workload == "wn" else bench_gravity(args)
+    bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched}[args.workload]
+    rec = bench(args)
     print(json.dumps(rec))
 
 
@@ -453,6 +458,142 @@ def bench_wn(args):
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
+    }
+
+
+def batched_sets(num_sets, seed=5000):
+    """Set sizes round(exp(U(ln 1024, ln 524288))), each a blob mixture of
+    U{8..64} blobs (sigma 0.02, log-normal masses normalised to sum 1) with
+    cotangents N(0, 1)."""
+    import math
+
+    import numpy as np
+
+    from sbh_amd import synthetic
+
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(1024), math.log(524288), num_sets)))
+    sizes = sizes.astype(np.int64)
+    ps, ms, cs = [], [], []
+    for k, n in enumerate(sizes):
+        p, m = synthetic.blobs(int(n), int(rng.integers(8, 65)), seed + 1 + k)
+        ps.append(p)
+        ms.append(m)
+        cs.append(rng.normal(size=(int(n), 3)).astype(np.float32))
+    return sizes, ps, ms, cs
+
+
+def bench_batched(args):
+    import numpy as np
+    import torch
+
+    from sbh_amd import autotune
+    from sbh_amd import gravity as G
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    sizes, ps, ms, cs = batched_sets(args.sets)
+    P = torch.tensor(np.concatenate(ps), device=dev)
+    M = torch.tensor(np.concatenate(ms), device=dev)
+    C = torch.tensor(np.concatenate(cs), device=dev)
+    offs = G.set_offsets(sizes, dev)
+    n_total = int(P.shape[0])
+    kw = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth,
+              multipole_order=args.order)
+    if args.no_autotune:
+        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "table": None}
+    else:
+        tuned = autotune.tune_batch(P, M, C, offs, reps=2, **kw)
+    leaf, blocks = tuned["leaf_size"], tuned["walk_blocks"]
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    # batched: one forest pipeline, captured as a HIP graph
+    step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw)
+    from sbh_amd.graph import Graph
+
+    step()
+    g = Graph()
+    g.capture(step)
+    ms_graph = timed(g.replay, args.steps, args.warmup)
+    ms_eager = timed(step, args.steps, args.warmup)
+    # per set: 64 separate calls of the single-set pipeline on one stream
+    Ps = [P[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
+    Ms = [M[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
+    Cs = [C[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
+    wss = G.new_workspace(int(sizes.max()), args.depth, leaf, dev)
+    Fs = [torch.empty_like(x) for x in Ps]
+
+    def per_set():
+        for k in range(len(Ps)):
+            G.gravity_force(Ps[k], Ms[k], leaf_size=leaf, workspace=wss, out=Fs[k], **kw)
+            G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=leaf, workspace=wss, **kw)
+
+    ms_per_set = timed(per_set, max(1, args.steps // 2), 1)
+    per_set_default = None
+    if leaf != args.leaf:
+        wsd = G.new_workspace(int(sizes.max()), args.depth, args.leaf, dev)
+
+        def per_set_d():
+            for k in range(len(Ps)):
+                G.gravity_force(Ps[k], Ms[k], leaf_size=args.leaf, workspace=wsd, out=Fs[k], **kw)
+                G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=args.leaf, workspace=wsd,
+                                    **kw)
+        per_set_default = timed(per_set_d, max(1, args.steps // 2), 1)
+    # parity of the batch with the per-set calls (forces bit-exact)
+    g.replay()
+    per_set()
+    torch.cuda.synchronize()
+    same = all(torch.equal(F[o0:o1], Fs[k]) for k, (o0, o1) in
+               enumerate(zip(offs[:-1].tolist(), offs[1:].tolist())))
+    # accuracy of the largest set against the FP64 direct sum
+    kbig = int(np.argmax(sizes))
+    o0 = int(offs[kbig])
+    idx = torch.tensor(np.random.default_rng(7).choice(int(sizes[kbig]), args.accuracy_queries,
+                                                       replace=False), device=dev)
+    Pb, Mb = Ps[kbig].double(), Ms[kbig].double()
+    ref = direct_fp64(Pb, Mb, Pb[idx], args.eps)
+    st = G.new_stats(dev)
+    G.gravity_force_vjp_batch(P, M, C, offs, stats=st, leaf_size=leaf, walk_blocks=blocks,
+                              workspace=ws, **kw)
+    sv = G.stats_dict(st)
+    g.close()
+    return {
+        "metric": "batched fwd+VJP forces/sec, 64 sets (graph-captured forest vs per-set calls)",
+        "value": n_total / (ms_graph * 1e-3),
+        "unit": "forces/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_graph,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"blob_sets{args.sets}_total{n_total}_theta0.5_fwd+vjp",
+                   "sets": args.sets, "total_points": n_total, "min_set": int(sizes.min()),
+                   "max_set": int(sizes.max()), "softening": args.eps, "beta_diag": args.beta,
+                   "max_depth": args.depth, "multipole_order": args.order,
+                   "leaf_size": leaf, "walk_blocks": blocks},
+        "batched_graph_ms": ms_graph,
+        "batched_eager_ms": ms_eager,
+        "per_set_ms": ms_per_set,
+        "per_set_default_leaf_ms": per_set_default,
+        "speedup_batched_vs_per_set": ms_per_set / ms_graph,
+        "batch_forces_equal_per_set": bool(same),
+        "autotune": tuned.get("table"),
+        "interactions_vjp": sv,
+        "accuracy": {"largest_set_force_rel_l2_vs_fp64_direct":
+                     rel_l2(F[o0:o0 + int(sizes[kbig])][idx], ref),
+                     "largest_set": int(sizes[kbig])},
+        "roofline": None,
+        "cpu_baseline": None,
     }
 
 

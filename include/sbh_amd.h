@@ -301,6 +301,44 @@ int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
                           int32_t leaf_size, int32_t multipole_order,
                           SbhBuffer* workspace, SbhBuffer* stats);
 
+/* ---- batched sets (new targets) ------------------------------------------
+ *
+ * Many independent self-gravity problems in one pipeline: set k is input rows
+ * set_offsets[k] .. set_offsets[k+1]-1 (int32 device vector of num_sets + 1
+ * entries rising from 0 to N).  Every set gets its own cube, tree and query
+ * groups inside one forest, built and walked by the same launches as one
+ * set (the persistent walk drains the groups of all sets from one queue).
+ * Per set, results equal sbh_gravity_force / sbh_gravity_force_vjp on that
+ * set alone.  Offsets that do not rise from 0 to N leave every set empty and
+ * set stats[5] (overflow) to 2.  walk_blocks: persistent walk grid (0 =
+ * default).  No reference counterpart (the reference runs one launch per
+ * call, python/benchmark_octree8.py). */
+int64_t sbh_gravity_batch_workspace_bytes(int64_t total_points, int32_t num_sets,
+                                          int32_t max_depth, int32_t leaf_size);
+int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
+                            const SbhBuffer* masses, const SbhBuffer* set_offsets,
+                            SbhBuffer* forces, float beta, float softening_scale,
+                            int32_t max_depth, int32_t leaf_size,
+                            int32_t multipole_order, int32_t walk_blocks,
+                            SbhBuffer* workspace, SbhBuffer* stats);
+int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points,
+                                const SbhBuffer* masses, const SbhBuffer* cotangent,
+                                const SbhBuffer* set_offsets, SbhBuffer* forces,
+                                SbhBuffer* grad_points, SbhBuffer* grad_masses,
+                                float beta, float softening_scale, int32_t max_depth,
+                                int32_t leaf_size, int32_t multipole_order,
+                                int32_t walk_blocks, SbhBuffer* workspace,
+                                SbhBuffer* stats);
+
+/* HIP graph capture of library calls on `stream` (no reference counterpart;
+ * XLA captures its own graphs).  Between sbh_graph_begin and sbh_graph_end
+ * only calls with caller-provided workspaces may be made.  sbh_graph_end
+ * instantiates the graph into *graph_exec; sbh_graph_launch replays it. */
+int sbh_graph_begin(void* stream);
+int sbh_graph_end(void* stream, void** graph_exec);
+int sbh_graph_launch(void* graph_exec, void* stream);
+int sbh_graph_destroy(void* graph_exec);
+
 /* Phase timing for the sbh_gravity_* targets (no reference counterpart; the
  * reference times from Python around dispatch).  When enabled, each call
  * records HIP events on its stream around the tree build and around the walk
