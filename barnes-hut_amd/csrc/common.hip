@@ -66,7 +66,7 @@ extern "C" {
 const char* sbh_last_error(void) { return sbh::g_err; }
 
 size_t sbh_leaf3_size(void) { return sizeof(sbh::Leaf3); }
-size_t sbh_node3_size(void) { return 4272; }
+size_t sbh_node3_size(void) { return sizeof(sbh::Node3); }
 size_t sbh_octree_level3_max_nodes(void) { return 1 + 8 * 8 * 8; }
 size_t sbh_octree8_node_size(void) { return sizeof(sbh::Node8); }
 

@@ -28,11 +28,30 @@ struct Node8 {
   Leaf3 contrib;
 };
 
+// Tree3DNode<1, 3> (include/tree3d.h:212-225; the reference's Node3Layout,
+// src/sbh_ffi.cu:123-140): 4272 bytes, same field order and padding.
+struct Node3 {
+  uint64_t start, end, buffer_idx, parent_idx;
+  float bmin[3], bmax[3];  // AABB<3>
+  float diameter;
+  Leaf3 contrib;
+  uint64_t bitmask[8];     // BitMask3D<3>: bit b <-> child cell b = x + 8 (y + 8 z)
+  uint64_t child[512];     // buffer index of the node of each present cell
+  int32_t num_children, depth, max_depth;
+  uint8_t is_leaf, pad[3];
+};
+
 static_assert(sizeof(Leaf3) == 32, "Leaf3 layout");
 static_assert(sizeof(Node8) == 104, "OctreeNode8 layout");
 static_assert(offsetof(Node8, child_mask) == 48, "OctreeNode8 layout");
 static_assert(offsetof(Node8, cx) == 52, "OctreeNode8 layout");
 static_assert(offsetof(Node8, contrib) == 72, "OctreeNode8 layout");
+static_assert(sizeof(Node3) == 4272, "Tree3DNode<1,3> layout");
+static_assert(offsetof(Node3, bmin) == 32 && offsetof(Node3, diameter) == 56 &&
+                  offsetof(Node3, contrib) == 60 && offsetof(Node3, bitmask) == 96 &&
+                  offsetof(Node3, child) == 160 && offsetof(Node3, num_children) == 4256 &&
+                  offsetof(Node3, is_leaf) == 4268,
+              "Tree3DNode<1,3> layout");
 
 // ---------------------------------------------------------------- errors
 int fail_invalid(const char* msg);      // returns SBH_INVALID_ARGUMENT

@@ -611,3 +611,203 @@ def softened_stochastic_barnes_hut_force_octree8_recompute_vjp(
               prune_r_cut_mult=float(prune_r_cut_mult), periodic=bool(periodic),
               box_length=float(box_length), threads=int(threads))
     return _StochasticRecompute.apply(p, m, kw, int(max_depth), True)
+
+
+# ------------------------------------------------------------------ level-3 tree (drop-in mode)
+register_build_octree_level3_buffers = _register
+register_barnes_hut_gravity = _register
+register_barnes_hut_force = _register
+register_softened_barnes_hut_force = _register
+register_stochastic_barnes_hut_gravity = _register
+register_stochastic_barnes_hut_force = _register
+register_softened_stochastic_barnes_hut_force = _register
+
+
+def _l3_queries(queries):
+    queries = _f32(queries)
+    if queries.dim() != 2 or queries.shape[1] not in (2, 3):
+        raise ValueError("queries must be (M, 2) or (M, 3)")
+    if queries.shape[1] == 2:
+        raise ValueError("2-D quadtree targets are not part of this library (3-D only)")
+    return queries
+
+
+def build_octree_buffers_with_contrib(points, normals, masses, min_corner=None,
+                                      max_corner=None, *, alpha: float = 1.0, level: int = 3,
+                                      backend: str = "ffi"):
+    """python/jax_sbh_ffi.py:805-861 (level-3 Tree3DNode<1,3> buffers)."""
+    del alpha
+    if level != 3:
+        raise ValueError("level must be 3 (FFI expects Tree3DNode<1, 3>)")
+    if backend != "ffi":
+        raise ValueError("backend must be 'ffi' (the CPU tree module is not part of this "
+                         "library)")
+    leaf_size, node_size, max_nodes = _get_octree_level3_layout()
+    points, normals, masses = _f32(points), _f32(normals), _f32(masses)
+    mn, mx = _default_corners(points, min_corner, max_corner)
+    if points.dim() != 2 or points.shape[1] != 3:
+        raise ValueError("points must have shape (N, 3)")
+    if normals.shape != points.shape:
+        raise ValueError("normals shape mismatch")
+    if masses.dim() != 1 or masses.shape[0] != points.shape[0]:
+        raise ValueError("masses must have shape (N,)")
+    dev = points.device
+    leaf = torch.empty((points.shape[0] * leaf_size,), dtype=torch.uint8, device=dev)
+    node = torch.empty((max_nodes * node_size,), dtype=torch.uint8, device=dev)
+    contrib = torch.empty((max_nodes * leaf_size,), dtype=torch.uint8, device=dev)
+    call("sbh_build_octree_level3_buffers", stream_handle(), buf(points), buf(normals),
+         buf(masses), buf(mn), buf(mx), buf(leaf), buf(node), buf(contrib))
+    return leaf, node, contrib
+
+
+def build_octree_buffers(points, normals, masses, min_corner=None, max_corner=None, *,
+                         alpha: float = 1.0, level: int = 3, backend: str = "ffi"):
+    """python/jax_sbh_ffi.py:744-802."""
+    leaf, node, _ = build_octree_buffers_with_contrib(points, normals, masses, min_corner,
+                                                      max_corner, alpha=alpha, level=level,
+                                                      backend=backend)
+    return leaf, node
+
+
+def barnes_hut_gravity(leaf_bytes, node_bytes, queries, *, beta: float = 2.0,
+                       periodic: bool = False, box_length: float = 1.0,
+                       threads: int = 64) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:963-997 (potential, (M,))."""
+    leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0],), dtype=torch.float32, device=queries.device)
+    call("sbh_barnes_hut_gravity_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(queries), buf(out), f32(beta), i32(1 if periodic else 0), f32(box_length),
+         i32(threads))
+    return out
+
+
+def barnes_hut_force(leaf_bytes, node_bytes, queries, *, beta: float = 2.0,
+                     periodic: bool = False, box_length: float = 1.0,
+                     threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1000-1034."""
+    leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_barnes_hut_force_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(queries), buf(out), f32(beta), i32(1 if periodic else 0), f32(box_length),
+         i32(threads))
+    return out
+
+
+def softened_barnes_hut_force(leaf_bytes, node_bytes, queries, *, beta: float = 2.0,
+                              softening_scale: float = 1e-2, cutoff_scale: float = 1.0,
+                              periodic: bool = False, box_length: float = 1.0,
+                              threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1063-1105."""
+    leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_barnes_hut_force_soft_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(queries), buf(out), f32(beta), f32(softening_scale), f32(cutoff_scale),
+         i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def stochastic_barnes_hut_gravity(leaf_bytes, node_bytes, contrib_bytes, queries, *,
+                                  samples_per_subdomain: int = 1, seed: int = 0,
+                                  periodic: bool = False, box_length: float = 1.0,
+                                  threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1138-1177."""
+    leaf_bytes, node_bytes = _u8(leaf_bytes), _u8(node_bytes)
+    contrib_bytes, queries = _u8(contrib_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0],), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_gravity_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
+         i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def stochastic_barnes_hut_force(leaf_bytes, node_bytes, contrib_bytes, queries, *,
+                                samples_per_subdomain: int = 1, seed: int = 0,
+                                periodic: bool = False, box_length: float = 1.0,
+                                threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1180-1219."""
+    leaf_bytes, node_bytes = _u8(leaf_bytes), _u8(node_bytes)
+    contrib_bytes, queries = _u8(contrib_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
+         i32(1 if periodic else 0), f32(box_length), i32(threads))
+    return out
+
+
+def softened_stochastic_barnes_hut_force(leaf_bytes, node_bytes, contrib_bytes, queries, *,
+                                         samples_per_subdomain: int = 1, seed: int = 0,
+                                         softening_scale: float = 1e-2,
+                                         cutoff_scale: float = 1.0, periodic: bool = False,
+                                         box_length: float = 1.0,
+                                         threads: int = 256) -> torch.Tensor:
+    """python/jax_sbh_ffi.py:1252-1299."""
+    leaf_bytes, node_bytes = _u8(leaf_bytes), _u8(node_bytes)
+    contrib_bytes, queries = _u8(contrib_bytes), _l3_queries(queries)
+    _check_periodic(periodic, box_length)
+    _check_threads(threads)
+    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
+    call("sbh_stochastic_bh_force_soft_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+         buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
+         f32(softening_scale), f32(cutoff_scale), i32(1 if periodic else 0), f32(box_length),
+         i32(threads))
+    return out
+
+
+def barnes_hut_force_vjp(leaf_bytes, node_bytes, queries, cotangent, *, beta: float = 2.0,
+                         periodic: bool = False, box_length: float = 1.0, threads: int = 256):
+    """python/jax_sbh_ffi.py:1037-1060 (central differences of the forward)."""
+    return _force_vjp_finite_difference(
+        lambda q: barnes_hut_force(leaf_bytes, node_bytes, q, beta=beta, periodic=periodic,
+                                   box_length=box_length, threads=threads),
+        queries, cotangent)
+
+
+def softened_barnes_hut_force_vjp(leaf_bytes, node_bytes, queries, cotangent, *,
+                                  beta: float = 2.0, softening_scale: float = 1e-2,
+                                  cutoff_scale: float = 1.0, periodic: bool = False,
+                                  box_length: float = 1.0, threads: int = 256):
+    """python/jax_sbh_ffi.py:1108-1135."""
+    return _force_vjp_finite_difference(
+        lambda q: softened_barnes_hut_force(leaf_bytes, node_bytes, q, beta=beta,
+                                            softening_scale=softening_scale,
+                                            cutoff_scale=cutoff_scale, periodic=periodic,
+                                            box_length=box_length, threads=threads),
+        queries, cotangent)
+
+
+def stochastic_barnes_hut_force_vjp(leaf_bytes, node_bytes, contrib_bytes, queries, cotangent,
+                                    *, samples_per_subdomain: int = 1, seed: int = 0,
+                                    periodic: bool = False, box_length: float = 1.0,
+                                    threads: int = 256):
+    """python/jax_sbh_ffi.py:1222-1249."""
+    return _force_vjp_finite_difference(
+        lambda q: stochastic_barnes_hut_force(leaf_bytes, node_bytes, contrib_bytes, q,
+                                              samples_per_subdomain=samples_per_subdomain,
+                                              seed=seed, periodic=periodic,
+                                              box_length=box_length, threads=threads),
+        queries, cotangent)
+
+
+def softened_stochastic_barnes_hut_force_vjp(leaf_bytes, node_bytes, contrib_bytes, queries,
+                                             cotangent, *, samples_per_subdomain: int = 1,
+                                             seed: int = 0, softening_scale: float = 1e-2,
+                                             cutoff_scale: float = 1.0, periodic: bool = False,
+                                             box_length: float = 1.0, threads: int = 256):
+    """python/jax_sbh_ffi.py:1302-1333."""
+    return _force_vjp_finite_difference(
+        lambda q: softened_stochastic_barnes_hut_force(
+            leaf_bytes, node_bytes, contrib_bytes, q,
+            samples_per_subdomain=samples_per_subdomain, seed=seed,
+            softening_scale=softening_scale, cutoff_scale=cutoff_scale, periodic=periodic,
+            box_length=box_length, threads=threads),
+        queries, cotangent)

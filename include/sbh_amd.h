@@ -200,6 +200,77 @@ int sbh_stochastic_bh_force_octree8_soft_vjp(
     float prune_r_cut_mult, int32_t periodic, float box_length,
     int32_t threads);
 
+/* ---- level-3 tree (Tree3DNode<1,3> buffers) -------------------------------
+ *
+ * One 8 x 8 x 8 split under the root (cells on each axis of the given box);
+ * leaf N*32, node 513*4272 and contrib 513*32 bytes.  The walks are the
+ * reference's warp-voted traversal (a node is a far field only if every
+ * active query of a 32-query warp of a `threads`-sized block accepts it);
+ * leaves always accept, so a warp uses either the root monopole or the exact
+ * sum.  The stochastic variants sum every (leaf) subdomain exactly and divide
+ * the zero correction by samples_per_subdomain (0 samples give NaN, as the
+ * reference); `seed` is accepted and unused because no sample path has a
+ * step in level-3 buffers. */
+
+/* Replaces BuildOctreeLevel3Buffers (src/sbh_ffi.cu:736-1037; target
+ * "sbh_build_octree_level3_buffers", python/jax_sbh_ffi.py:744-861).  Points
+ * inside a cell are stored in input order (the reference's atomic cursor
+ * leaves that order undefined). */
+int sbh_build_octree_level3_buffers(void* stream, const SbhBuffer* points,
+                                    const SbhBuffer* normals, const SbhBuffer* masses,
+                                    const SbhBuffer* min_corner,
+                                    const SbhBuffer* max_corner, SbhBuffer* leaf_out,
+                                    SbhBuffer* node_out, SbhBuffer* contrib_out);
+
+/* Replaces BarnesHutGravity3D (src/sbh_ffi.cu:2490-2534; target
+ * "sbh_barnes_hut_gravity_3d", python/jax_sbh_ffi.py:963-997): potential
+ * -m / (|p - q| + 0.1), out (M,). */
+int sbh_barnes_hut_gravity_3d(void* stream, const SbhBuffer* leaf_bytes,
+                              const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                              SbhBuffer* out, float beta, int32_t periodic,
+                              float box_length, int32_t threads);
+
+/* Replaces BarnesHutForce3D (src/sbh_ffi.cu:2729-2777; target
+ * "sbh_barnes_hut_force_3d", python/jax_sbh_ffi.py:1000-1034). */
+int sbh_barnes_hut_force_3d(void* stream, const SbhBuffer* leaf_bytes,
+                            const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                            SbhBuffer* out, float beta, int32_t periodic,
+                            float box_length, int32_t threads);
+
+/* Replaces SoftenedBarnesHutForce3D (src/sbh_ffi.cu:2671-2727; target
+ * "sbh_barnes_hut_force_soft_3d", python/jax_sbh_ffi.py:1063-1105). */
+int sbh_barnes_hut_force_soft_3d(void* stream, const SbhBuffer* leaf_bytes,
+                                 const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                 SbhBuffer* out, float beta, float softening_scale,
+                                 float cutoff_scale, int32_t periodic, float box_length,
+                                 int32_t threads);
+
+/* Replaces StochasticBarnesHutGravity3D (src/sbh_ffi.cu:2834-2887; target
+ * "sbh_stochastic_bh_gravity_3d", python/jax_sbh_ffi.py:1138-1177). */
+int sbh_stochastic_bh_gravity_3d(void* stream, const SbhBuffer* leaf_bytes,
+                                 const SbhBuffer* node_bytes, const SbhBuffer* contrib_bytes,
+                                 const SbhBuffer* queries, SbhBuffer* out,
+                                 int32_t samples_per_subdomain, int32_t seed,
+                                 int32_t periodic, float box_length, int32_t threads);
+
+/* Replaces StochasticBarnesHutForce3D (src/sbh_ffi.cu:2944-2997; target
+ * "sbh_stochastic_bh_force_3d", python/jax_sbh_ffi.py:1180-1219). */
+int sbh_stochastic_bh_force_3d(void* stream, const SbhBuffer* leaf_bytes,
+                               const SbhBuffer* node_bytes, const SbhBuffer* contrib_bytes,
+                               const SbhBuffer* queries, SbhBuffer* out,
+                               int32_t samples_per_subdomain, int32_t seed,
+                               int32_t periodic, float box_length, int32_t threads);
+
+/* Replaces SoftenedStochasticBarnesHutForce3D (src/sbh_ffi.cu:3066-3131;
+ * target "sbh_stochastic_bh_force_soft_3d", python/jax_sbh_ffi.py:1252-1299). */
+int sbh_stochastic_bh_force_soft_3d(void* stream, const SbhBuffer* leaf_bytes,
+                                    const SbhBuffer* node_bytes,
+                                    const SbhBuffer* contrib_bytes,
+                                    const SbhBuffer* queries, SbhBuffer* out,
+                                    int32_t samples_per_subdomain, int32_t seed,
+                                    float softening_scale, float cutoff_scale,
+                                    int32_t periodic, float box_length, int32_t threads);
+
 /* ---- winding numbers (K1) -------------------------------------------------
  *
  * Sources are oriented area elements: centroid p, unit normal n, area a.  The

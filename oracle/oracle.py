@@ -112,6 +112,11 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_tree_eval_range.argtypes = (list(L.orc_tree_eval.argtypes[:15]) +
                                       list(L.orc_tree_eval.argtypes[16:]) + [_i64, _i64])
     L.orc_philox4x32.argtypes = [_u32p, _u32p, _u32p]
+    L.orc_node3_size.restype = ctypes.c_size_t
+    L.orc_build_level3.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _f32p, _u8p, _u8p, _u8p]
+    L.orc_l3_eval.argtypes = [_u8p, _i64, _u8p, _i64, _f32p, _i64, ctypes.c_int, ctypes.c_float,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_int, ctypes.c_float, _f64p]
     L.orc_bruteforce_wn.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _i64, _f64p]
     L.orc_wn_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.orc_wn_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
@@ -273,6 +278,48 @@ def sbh_octree8(leaf_bytes, node_bytes, queries, kind, samples, seed, beta, eps=
     if kind == KIND_WN:
         return out[:, 0].copy()
     return out
+
+
+NODE3_DTYPE = np.dtype({
+    "names": ["start", "end", "buffer_idx", "parent_idx", "bmin", "bmax", "diameter", "contrib",
+              "bitmask", "child", "num_children", "depth", "max_depth", "is_leaf"],
+    "formats": ["<u8", "<u8", "<u8", "<u8", ("<f4", 3), ("<f4", 3), "<f4", LEAF3_DTYPE,
+                ("<u8", 8), ("<u8", 512), "<i4", "<i4", "<i4", "u1"],
+    "offsets": [0, 8, 16, 24, 32, 44, 56, 60, 96, 160, 4256, 4260, 4264, 4268],
+    "itemsize": 4272})
+L3_MAX_NODES = 513
+L3_POTENTIAL, L3_FORCE, L3_SOFT = 0, 1, 2
+
+
+def build_level3(points, normals, masses, min_corner, max_corner):
+    """Level-3 buffers (leaf N*32, node 513*4272, contrib 513*32 bytes)."""
+    p = _c(points, np.float32)
+    nv = _c(normals, np.float32)
+    m = _c(masses, np.float32)
+    n = p.shape[0]
+    leaf = np.zeros(n * 32, np.uint8)
+    node = np.zeros(L3_MAX_NODES * 4272, np.uint8)
+    contrib = np.zeros(L3_MAX_NODES * 32, np.uint8)
+    lib().orc_build_level3(p, nv, m, n, _c(min_corner, np.float32), _c(max_corner, np.float32),
+                           leaf, node, contrib)
+    return leaf, node, contrib
+
+
+def decode_nodes3(node_bytes):
+    return np.frombuffer(np.ascontiguousarray(node_bytes, np.uint8).tobytes(), NODE3_DTYPE)
+
+
+def l3_eval(leaf_bytes, node_bytes, queries, kind, *, beta=2.0, threads=256, samples=-1,
+            eps=0.0, rc=0.0, periodic=False, box=1.0):
+    """Level-3 walks: classic vote walk (samples < 0) or stochastic."""
+    leaf = _c(leaf_bytes, np.uint8)
+    node = _c(node_bytes, np.uint8)
+    q = _c(queries, np.float32)
+    out = np.zeros((q.shape[0], 3), np.float64)
+    lib().orc_l3_eval(leaf, leaf.size // 32, node, node.size // 4272, q, q.shape[0], int(kind),
+                      float(beta), int(threads), int(samples), float(eps), float(rc),
+                      int(bool(periodic)), float(box), out)
+    return out[:, 0].copy() if kind == L3_POTENTIAL else out
 
 
 def bruteforce_wn(points, normals, areas, queries):

@@ -692,6 +692,230 @@ ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_by
 }
 
 /* ------------------------------------------------------------------------- */
+/* L1/L2: level-3 tree (src/sbh_ffi.cu:181-397, 736-1037) and its walks       */
+/* (src/sbh_ffi.cu:1441-1680, include/barnes_hut_gpu.h:19-72,                 */
+/* include/tree_sampler_gpu.h:25-140)                                         */
+/* ------------------------------------------------------------------------- */
+typedef struct {
+  uint64_t start, end, buffer_idx, parent_idx;
+  float bmin[3], bmax[3];
+  float diameter;
+  OrcLeaf3 contrib;
+  uint64_t bitmask[8];
+  uint64_t child[512];
+  int32_t num_children, depth, max_depth;
+  uint8_t is_leaf, pad[3];
+} OrcNode3; /* 4272 bytes: Tree3DNode<1, 3> (include/tree3d.h:212-225) */
+
+_Static_assert(sizeof(OrcNode3) == 4272, "Tree3DNode<1,3> layout");
+
+ORC_EXPORT size_t orc_node3_size(void) { return sizeof(OrcNode3); }
+
+/* quantize_axis_level3 (src/sbh_ffi.cu:181-190). */
+static inline int orc_quantize3(float x, float lo, float hi) {
+  float span = hi - lo;
+  if (span <= 1e-12f) return 0;
+  float t = (x - lo) / span;
+  t = fminf(fmaxf(t, 0.0f), 0.999999f);
+  return (int)(t * 8.0f);
+}
+
+/* Level-3 builder: cells by the per-axis quantiser, points in stable (cell,
+ * index) order, fp64 cell sums, child records at 1 + rank in cell order.
+ * The child minimum corner is fmaf(d, q, min) (nvcc contracts
+ * min + d * q); diameters are not contracted. */
+ORC_EXPORT void orc_build_level3(const float* pts, const float* normals, const float* masses,
+                                 int64_t n, const float* lo, const float* hi,
+                                 uint8_t* leaf_out, uint8_t* node_out, uint8_t* contrib_out) {
+  OrcLeaf3* leaf = (OrcLeaf3*)leaf_out;
+  OrcNode3* nodes = (OrcNode3*)node_out;
+  OrcLeaf3* contrib = (OrcLeaf3*)contrib_out;
+  memset(node_out, 0, sizeof(OrcNode3) * 513);
+  memset(contrib_out, 0, sizeof(OrcLeaf3) * 513);
+  uint32_t* cell = (uint32_t*)calloc((size_t)(n > 0 ? n : 1), sizeof(uint32_t));
+  uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)(n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) {
+    int qx = orc_quantize3(pts[3 * i], lo[0], hi[0]);
+    int qy = orc_quantize3(pts[3 * i + 1], lo[1], hi[1]);
+    int qz = orc_quantize3(pts[3 * i + 2], lo[2], hi[2]);
+    cell[i] = (uint32_t)(qx + 8 * (qy + 8 * qz));
+  }
+  orc_stable_order(cell, n, 9, perm);
+  int64_t cnt[512], off[513];
+  double sum[512][7], tot[7] = {0, 0, 0, 0, 0, 0, 0};
+  memset(cnt, 0, sizeof(cnt));
+  memset(sum, 0, sizeof(sum));
+  for (int64_t r = 0; r < n; r++) {
+    uint32_t i = perm[r];
+    OrcLeaf3* o = &leaf[r];
+    for (int a = 0; a < 3; a++) {
+      o->pos[a] = pts[3 * i + a];
+      o->normal[a] = normals[3 * i + a];
+    }
+    o->mass = masses[i];
+    o->alpha = 1.0f;
+    uint32_t c = cell[i];
+    cnt[c]++;
+    double m = masses[i];
+    double v[7] = {m, m * pts[3 * i], m * pts[3 * i + 1], m * pts[3 * i + 2],
+                   m * normals[3 * i], m * normals[3 * i + 1], m * normals[3 * i + 2]};
+    for (int a = 0; a < 7; a++) {
+      sum[c][a] += v[a];
+      tot[a] += v[a];
+    }
+  }
+  off[0] = 0;
+  for (int c = 0; c < 512; c++) off[c + 1] = off[c] + cnt[c];
+  float d[3];
+  for (int a = 0; a < 3; a++) d[a] = (hi[a] - lo[a]) / 8.0f;
+  OrcNode3* root = &nodes[0];
+  int rank = 0;
+  for (int c = 0; c < 512; c++) {
+    if (cnt[c] == 0) continue;
+    int idx = 1 + rank++;
+    OrcNode3* ch = &nodes[idx];
+    int q[3] = {c & 7, (c >> 3) & 7, (c >> 6) & 7};
+    ch->start = (uint64_t)off[c];
+    ch->end = (uint64_t)(off[c] + cnt[c]);
+    ch->buffer_idx = (uint64_t)idx;
+    ch->parent_idx = 0;
+    for (int a = 0; a < 3; a++) {
+      ch->bmin[a] = fmaf(d[a], (float)q[a], lo[a]);
+      ch->bmax[a] = ch->bmin[a] + d[a];
+    }
+    ch->diameter = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    float m = (float)sum[c][0];
+    ch->contrib.mass = m;
+    ch->contrib.alpha = 1.0f;
+    if (m > 0.0f)
+      for (int a = 0; a < 3; a++) {
+        ch->contrib.pos[a] = (float)(sum[c][1 + a] / sum[c][0]);
+        ch->contrib.normal[a] = (float)(sum[c][4 + a] / sum[c][0]);
+      }
+    ch->depth = 1;
+    ch->max_depth = 1;
+    ch->is_leaf = 1;
+    contrib[idx] = ch->contrib;
+    root->bitmask[c >> 6] |= 1ull << (c & 63);
+    root->child[c] = (uint64_t)idx;
+  }
+  root->start = 0;
+  root->end = (uint64_t)n;
+  float dd[3];
+  for (int a = 0; a < 3; a++) {
+    root->bmin[a] = lo[a];
+    root->bmax[a] = hi[a];
+    dd[a] = hi[a] - lo[a];
+  }
+  root->diameter = sqrtf(dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]);
+  float M = (float)tot[0];
+  root->contrib.mass = M;
+  root->contrib.alpha = 1.0f;
+  if (M > 0.0f)
+    for (int a = 0; a < 3; a++) {
+      root->contrib.pos[a] = (float)(tot[1 + a] / tot[0]);
+      root->contrib.normal[a] = (float)(tot[4 + a] / tot[0]);
+    }
+  root->num_children = rank;
+  root->max_depth = 1;
+  root->is_leaf = n <= 1 ? 1 : 0;
+  contrib[0] = root->contrib;
+  free(cell);
+  free(perm);
+}
+
+/* kind: 0 = gravityPotential (scalar in out[3 i]), 1 = gravityGrad,
+ * 2 = gravityGradSoftCutoff<3> (eps, rc; attenuation for any rc > 0). */
+static void orc_l3_term(int kind, const double* q, const OrcLeaf3* src, const OrcParams* P,
+                        double* acc) {
+  double p[3] = {src->pos[0], src->pos[1], src->pos[2]};
+  if (kind == 0) {
+    double d0 = orc_mi(p[0] - q[0], P), d1 = orc_mi(p[1] - q[1], P), d2 = orc_mi(p[2] - q[2], P);
+    acc[0] += -(double)src->mass / (sqrt(d0 * d0 + d1 * d1 + d2 * d2) + 0.1);
+  } else if (kind == 1) {
+    orc_grav_grad(q, p, src->mass, P, acc);
+  } else {
+    orc_soft_force(q, p, src->mass, P, acc);
+  }
+}
+
+/* Recursive vote walk for one reference warp (queries g0..g1-1): a node is a
+ * far field only if every query of the warp has |q - COM| / diameter >= beta
+ * (leaves: always); the path is common to the warp. */
+static void orc_l3_vote(const OrcLeaf3* leaf, const OrcNode3* nodes, int64_t nn, int64_t nl,
+                        int64_t node, const float* qf, int64_t g0, int64_t g1, int kind,
+                        float beta, const OrcParams* P, double* out) {
+  const OrcNode3* nd = &nodes[node];
+  int all = 1;
+  for (int64_t i = g0; i < g1 && all; i++) {
+    if (nd->is_leaf) break;
+    float dx = qf[3 * i] - nd->contrib.pos[0];
+    float dy = qf[3 * i + 1] - nd->contrib.pos[1];
+    float dz = qf[3 * i + 2] - nd->contrib.pos[2];
+    float ratio = sqrtf(dx * dx + dy * dy + dz * dz) / nd->diameter;
+    if (!(ratio >= beta)) all = 0;
+  }
+  if (all) {
+    for (int64_t i = g0; i < g1; i++) {
+      double q[3] = {qf[3 * i], qf[3 * i + 1], qf[3 * i + 2]};
+      if (nd->is_leaf) {
+        uint64_t je = nd->end < (uint64_t)nl ? nd->end : (uint64_t)nl;
+        for (uint64_t j = nd->start; j < je; j++) orc_l3_term(kind, q, &leaf[j], P, out + 3 * i);
+      } else {
+        orc_l3_term(kind, q, &nd->contrib, P, out + 3 * i);
+      }
+    }
+    return;
+  }
+  for (int c = 0; c < 512; c++)
+    if ((nd->bitmask[c >> 6] >> (c & 63)) & 1ull)
+      if (nd->child[c] < (uint64_t)nn)
+        orc_l3_vote(leaf, nodes, nn, nl, (int64_t)nd->child[c], qf, g0, g1, kind, beta, P, out);
+}
+
+/* Classic level-3 walk (samples < 0) or the stochastic one (samples >= 0:
+ * every subdomain is a leaf in level-3 buffers; 0 samples give 0 / 0).
+ * Reference warps: 32 consecutive queries of a block of `threads`. */
+ORC_EXPORT void orc_l3_eval(const uint8_t* leaf_bytes, int64_t nl, const uint8_t* node_bytes,
+                            int64_t nn, const float* queries, int64_t nq, int kind, float beta,
+                            int threads, int samples, double eps, double rc, int periodic,
+                            float box, double* out) {
+  const OrcLeaf3* leaf = (const OrcLeaf3*)leaf_bytes;
+  const OrcNode3* nodes = (const OrcNode3*)node_bytes;
+  OrcParams P;
+  orc_params(&P, eps, rc > 0.0 ? rc : 0.0, periodic, box, 0);
+  memset(out, 0, sizeof(double) * 3 * (size_t)nq);
+  if (samples < 0) {
+    for (int64_t b0 = 0; b0 < nq; b0 += threads)
+      for (int64_t w0 = b0; w0 < b0 + threads && w0 < nq; w0 += 32) {
+        int64_t w1 = w0 + 32;
+        if (w1 > b0 + threads) w1 = b0 + threads;
+        if (w1 > nq) w1 = nq;
+        orc_l3_vote(leaf, nodes, nn, nl, 0, queries, w0, w1, kind, beta, &P, out);
+      }
+    return;
+  }
+  const OrcNode3* root = &nodes[0];
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double extra = 0.0;
+    for (int c = 0; c < 512; c++) {
+      if (!((root->bitmask[c >> 6] >> (c & 63)) & 1ull) || root->child[c] >= (uint64_t)nn)
+        continue;
+      const OrcNode3* sub = &nodes[root->child[c]];
+      if (sub->is_leaf) {
+        uint64_t je = sub->end < (uint64_t)nl ? sub->end : (uint64_t)nl;
+        for (uint64_t j = sub->start; j < je; j++) orc_l3_term(kind, q, &leaf[j], &P, out + 3 * i);
+      } else {
+        orc_l3_term(kind, q, &sub->contrib, &P, out + 3 * i);
+      }
+      extra += 0.0 / (double)samples;
+    }
+    for (int a = 0; a < 3; a++) out[3 * i + a] += extra;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
 /* S1/S2: stochastic multi-level control-variate walk (src/sbh_ffi.cu:2146-2442) */
 /* ------------------------------------------------------------------------- */
 ORC_EXPORT void orc_philox4x32(const uint32_t* ctr_in, const uint32_t* key_in,

@@ -758,3 +758,97 @@ def test_sbh_wn_unbiased_and_shallow_exact():
     z = (mean - exact) / se
     assert (z ** 2).mean() < 2.0
     assert np.abs(z).max() < 5.0
+
+
+# --------------------------------------------------------------------- level-3 tree
+def _l3_case(seed=31, n=700):
+    rng = _rng(seed)
+    p = rng.uniform(-1, 1, (n, 3)).astype(np.float32)
+    nv = rng.normal(size=(n, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    return p, nv, m, lo, hi
+
+
+def test_level3_matches_independent_numpy_build():
+    p, nv, m, lo, hi = _l3_case()
+    leaf, node, contrib = O.build_level3(p, nv, m, lo, hi)
+    span = (hi - lo).astype(np.float32)
+    t = np.clip(((p - lo) / span).astype(np.float32), np.float32(0), np.float32(0.999999))
+    q = (t * np.float32(8)).astype(np.int64)
+    cell = q[:, 0] + 8 * (q[:, 1] + 8 * q[:, 2])
+    perm = np.argsort(cell, kind="stable")
+    lf = O.decode_leaf3(leaf)
+    np.testing.assert_array_equal(lf["pos"], p[perm])
+    np.testing.assert_array_equal(lf["mass"], m[perm])
+    nd = O.decode_nodes3(node)
+    assert nd.shape == (513,)
+    cnt = np.bincount(cell, minlength=512)
+    present = np.nonzero(cnt)[0]
+    root = nd[0]
+    assert root["num_children"] == present.size and root["end"] == p.shape[0]
+    bits = np.zeros(512, bool)
+    for w in range(8):
+        for b in range(64):
+            bits[64 * w + b] = (int(root["bitmask"][w]) >> b) & 1
+    np.testing.assert_array_equal(np.nonzero(bits)[0], present)
+    np.testing.assert_array_equal(root["child"][present], 1 + np.arange(present.size))
+    d = ((hi - lo) / np.float32(8)).astype(np.float32)
+    off = np.concatenate([[0], np.cumsum(cnt)])
+    for r, c in enumerate(present):
+        ch = nd[1 + r]
+        qc = np.array([c & 7, (c >> 3) & 7, (c >> 6) & 7], np.float64)
+        bmin = (d.astype(np.float64) * qc + lo.astype(np.float64)).astype(np.float32)
+        np.testing.assert_array_equal(ch["bmin"], bmin)
+        np.testing.assert_array_equal(ch["bmax"], (bmin + d).astype(np.float32))
+        assert ch["start"] == off[c] and ch["end"] == off[c + 1] and ch["is_leaf"] == 1
+        assert ch["diameter"] == np.float32(np.sqrt(np.float32((d * d).sum(dtype=np.float32))))
+        sel = cell == c
+        M = m[sel].astype(np.float64).sum()
+        np.testing.assert_allclose(ch["contrib"]["mass"], M, rtol=1e-6)
+        np.testing.assert_allclose(ch["contrib"]["pos"], (m[sel, None] * p[sel]).sum(0) / M,
+                                   rtol=1e-5, atol=1e-6)
+    assert nd[1 + present.size:]["num_children"].sum() == 0
+    assert (nd[1 + present.size:]["end"] == 0).all()
+    cb = O.decode_leaf3(contrib)
+    np.testing.assert_array_equal(cb["mass"][: 1 + present.size],
+                                  nd["contrib"]["mass"][: 1 + present.size])
+
+
+def test_level3_walks_reduce_to_direct_sums():
+    p, nv, m, lo, hi = _l3_case(seed=32, n=500)
+    leaf, node, contrib = O.build_level3(p, nv, m, lo, hi)
+    q = p[:70]
+    # every warp has a query inside the set, so no warp accepts the root
+    pot = O.l3_eval(leaf, node, q, O.L3_POTENTIAL, beta=2.0, threads=64)
+    d = p[None].astype(np.float64) - q[:, None].astype(np.float64)
+    ref = (-m / (np.linalg.norm(d, axis=2) + 0.1)).sum(1)
+    np.testing.assert_allclose(pot, ref, rtol=1e-10)
+    f = O.l3_eval(leaf, node, q, O.L3_SOFT, beta=2.0, threads=64, eps=0.05, rc=0.7)
+    np.testing.assert_allclose(f, O.direct_force_soft(p, m, q, 0.05, 0.7), rtol=1e-9, atol=1e-12)
+    # stochastic: leaf subdomains only, the exact sum; zero samples give 0/0
+    s = O.l3_eval(leaf, node, q, O.L3_SOFT, samples=2, eps=0.05, rc=0.7)
+    np.testing.assert_allclose(s, O.direct_force_soft(p, m, q, 0.05, 0.7), rtol=1e-9, atol=1e-12)
+    assert np.isnan(O.l3_eval(leaf, node, q, O.L3_FORCE, samples=0)).all()
+
+
+def test_level3_vote_uses_root_monopole_only_for_all_far_warps():
+    p, nv, m, lo, hi = _l3_case(seed=33, n=400)
+    leaf, node, contrib = O.build_level3(p, nv, m, lo, hi)
+    root = O.decode_nodes3(node)[0]
+    com = root["contrib"]["pos"].astype(np.float64)
+    far = (com + np.array([30.0, 0, 0]) + _rng(1).normal(0, 0.1, (64, 3))).astype(np.float32)
+    q = far.copy()
+    q[40] = p[0]  # one near query in the second warp of 32
+    pot = O.l3_eval(leaf, node, q, O.L3_POTENTIAL, beta=2.0, threads=64)
+    mono = -float(root["contrib"]["mass"]) / (np.linalg.norm(q.astype(np.float64) - com, axis=1)
+                                              + 0.1)
+    d = p[None].astype(np.float64) - q[:, None].astype(np.float64)
+    exact = (-m / (np.linalg.norm(d, axis=2) + 0.1)).sum(1)
+    np.testing.assert_allclose(pot[:32], mono[:32], rtol=1e-6)
+    np.testing.assert_allclose(pot[32:], exact[32:], rtol=1e-9)
+    # with 40-query blocks the warps are [0,32), [32,40), [40,64): only the
+    # warp holding query 40 sums exactly
+    pot = O.l3_eval(leaf, node, q, O.L3_POTENTIAL, beta=2.0, threads=40)
+    np.testing.assert_allclose(pot[:40], mono[:40], rtol=1e-6)
+    np.testing.assert_allclose(pot[40:], exact[40:], rtol=1e-9)
