@@ -13,8 +13,9 @@ moments, walk) and backward (rebuild + one adjoint walk returning position and
 mass gradients).  Inputs are resident in HBM before the timed region.
 
 Prints one JSON line (see the keys below).  The CPU baseline is the oracle's
-restatement of the same walk, timed on a bounded sample of query groups on
-one host core and extrapolated to the full workload.
+restatement of the same walk on the host cores (OpenMP), timed on a bounded
+sample of query groups and scaled to the full workload; beside it, the
+reference's CPU tree path (restated) and its numpy direct sum, forward only.
 """
 import argparse
 import ctypes
@@ -75,10 +76,9 @@ def rel_l2(a, b):
 
 
 def cpu_baseline(p, m, c, args, sample_groups):
-    """Oracle restatement of the same fwd + VJP step on one host core, on
-    `sample_groups` query groups, extrapolated to all groups."""
-    import numpy as np
-
+    """Oracle restatement of the same fwd + VJP step on the host cores
+    (OpenMP over query groups), on `sample_groups` query groups, scaled to all
+    groups."""
     from oracle import oracle as O
 
     t0 = time.perf_counter()
@@ -103,15 +103,62 @@ def cpu_baseline(p, m, c, args, sample_groups):
     return {
         "value": T.n / t_step,
         "unit": "forces/s",
-        "cores": 1,
+        "cores": O.num_threads(),
         "kind": "port",
-        "sample": (f"oracle adaptive-tree walk (fp64, 1 core): tree build x2 timed in full; "
-                   f"forward and adjoint walks timed on {gsel} of {ng} query groups (<= 64 "
-                   f"queries each) and scaled by {scale:.1f}"),
+        "sample": (f"oracle adaptive-tree walk with the same MAC and expansions (fp64, OpenMP "
+                   f"over query groups): tree build x2 timed in full; forward and adjoint "
+                   f"walks timed on {gsel} of {ng} query groups (<= 64 queries each) and "
+                   f"scaled by {scale:.2f}"),
         "seconds_measured": round(2 * t_build + t_fwd + t_vjp, 2),
         "host_cpu": platform.processor() or platform.machine(),
         "host_threads_available": os.cpu_count(),
     }
+
+
+def cpu_reference_path(p, m, args, ref_idx, ref, nsample=1 << 15):
+    """The reference's CPU path (src/sbh.cpp evalBarnesHut over Tree3D),
+    restated in oracle/sbh_cpu.c, like-for-like: a true octree (Tree3D<L,1>)
+    with the benchmark's leaf size, the softened force, forward only; the
+    walk is timed on `nsample` queries (plus the accuracy sample) and scaled."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    n = p.shape[0]
+    t0 = time.perf_counter()
+    T = O.CpuTree(p, m, leaf_size=args.leaf, dim_split_log=1, max_depth=args.depth)
+    t_build = time.perf_counter() - t0
+    sel = np.arange(0, n, max(1, n // nsample))[:nsample]
+    t0 = time.perf_counter()
+    _, n_int = T.eval(p[sel], O.CPU_SOFT_FORCE, beta=args.beta, eps=args.eps)
+    t_eval = (time.perf_counter() - t0) * n / sel.size
+    f, _ = T.eval(p[ref_idx], O.CPU_SOFT_FORCE, beta=args.beta, eps=args.eps)
+    err = float(np.linalg.norm(f - ref) / np.linalg.norm(ref))
+    return {"value": n / (t_build + t_eval), "unit": "forces/s (forward only)",
+            "threads": O.num_threads(), "build_s": round(t_build, 3),
+            "eval_s_scaled": round(t_eval, 3), "sample_queries": int(sel.size),
+            "interactions_per_query": n_int / sel.size,
+            "force_rel_l2_vs_fp64_direct": err,
+            "variant": f"Tree3D<{args.leaf},1> restatement, MAC |q-com|/diag >= {args.beta}, "
+                       "softened force, single-threaded build, strided OpenMP queries"}
+
+
+def cpu_numpy_direct(p, m, args, nsample=64):
+    """The reference benchmark's numpy direct sum (unified_force_benchmark.py:
+    208-230, fp32, one query at a time), timed on `nsample` queries and scaled."""
+    import numpy as np
+
+    eps2 = np.float32(args.eps * args.eps)
+    t0 = time.perf_counter()
+    for i in range(nsample):
+        disp = p[i][None, :] - p
+        r2 = np.sum(disp * disp, axis=1)
+        inv_r = np.float32(1.0) / np.sqrt(r2 + eps2 + np.float32(1e-12))
+        coeff = -m * inv_r * inv_r * inv_r
+        np.sum(coeff[:, None] * disp, axis=0, dtype=np.float32)
+    t = (time.perf_counter() - t0) / nsample
+    return {"value": 1.0 / t, "unit": "forces/s (forward only)", "cores": 1,
+            "sample": f"{nsample} of {p.shape[0]} queries, fp32 numpy, scaled"}
 
 
 def main():
@@ -131,7 +178,7 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--beta", type=float, default=2.0)
     ap.add_argument("--accuracy-queries", type=int, default=1024)
-    ap.add_argument("--cpu-sample-groups", type=int, default=1200)
+    ap.add_argument("--cpu-sample-groups", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--workload", default="gravity", choices=("gravity", "wn", "batched"),
@@ -248,9 +295,11 @@ def bench_gravity(args):
         acc["grad_masses_rel_l2_vs_fp32_direct"] = rel_l2(gm, gmd)
         acc["force_full_rel_l2_vs_fp32_direct"] = rel_l2(F, Fd)
 
-    cpu = None
+    cpu = cpu_ref = cpu_np = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, m, c, args, args.cpu_sample_groups)
+        cpu_ref = cpu_reference_path(p, m, args, idx.cpu().numpy(), ref.cpu().numpy())
+        cpu_np = cpu_numpy_direct(p, m, args)
 
     rec = {
         "metric": METRIC,
@@ -289,6 +338,8 @@ def bench_gravity(args):
                      "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
                      "flops_per_launch": flops_vjp},
         "cpu_baseline": cpu,
+        "cpu_reference_path": cpu_ref,
+        "cpu_numpy_direct": cpu_np,
     }
     return rec
 

@@ -57,6 +57,7 @@ def lib() -> ctypes.CDLL:
 
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
@@ -117,6 +118,21 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_l3_eval.argtypes = [_u8p, _i64, _u8p, _i64, _f32p, _i64, ctypes.c_int, ctypes.c_float,
                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, ctypes.c_float, _f64p]
+    L.orc_num_threads.restype = ctypes.c_int
+    L.orc_cpu_tree_build.restype = ctypes.c_void_p
+    L.orc_cpu_tree_build.argtypes = [_f32p, _vp, _f32p, _i64, _f32p, _f32p, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int]
+    L.orc_cpu_tree_free.argtypes = [_vp]
+    L.orc_cpu_tree_nodes.restype = _i64
+    L.orc_cpu_tree_nodes.argtypes = [_vp]
+    L.orc_cpu_tree_arrays.argtypes = [_vp, _i64p, _i64p, _i64p, _i32p,
+                                      np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS"),
+                                      _f32p, _f32p, _f32p]
+    L.orc_cpu_bh_eval.restype = _i64
+    L.orc_cpu_bh_eval.argtypes = [_vp, _f32p, _i64, ctypes.c_int, ctypes.c_float,
+                                  ctypes.c_double, ctypes.c_int, _f64p]
+    L.orc_cpu_mlpcv.argtypes = [_vp, _f32p, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
+                                ctypes.c_double, ctypes.c_int, _f64p]
     L.orc_bruteforce_wn.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _i64, _f64p]
     L.orc_wn_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.orc_wn_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
@@ -320,6 +336,70 @@ def l3_eval(leaf_bytes, node_bytes, queries, kind, *, beta=2.0, threads=256, sam
                       float(beta), int(threads), int(samples), float(eps), float(rc),
                       int(bool(periodic)), float(box), out)
     return out[:, 0].copy() if kind == L3_POTENTIAL else out
+
+
+CPU_POTENTIAL, CPU_WN, CPU_SMOOTHDIST, CPU_SOFT_FORCE = 0, 1, 2, 3
+
+
+def num_threads():
+    """Threads the oracle's OpenMP loops use."""
+    return int(lib().orc_num_threads())
+
+
+class CpuTree:
+    """Restatement of the reference's CPU Tree3D<LEAF_SIZE, DIM_SPLIT_LOG>
+    (see oracle/sbh_cpu.c): dim_split_log 1 is a true octree, 3 the
+    512-ary Octree3."""
+
+    def __init__(self, points, masses, normals=None, min_corner=None, max_corner=None,
+                 leaf_size=1, dim_split_log=3, max_depth=100):
+        p = _c(points, np.float32)
+        self.n = p.shape[0]
+        if min_corner is None or max_corner is None:
+            lo, hi = default_bounds(p)
+            min_corner = lo if min_corner is None else min_corner
+            max_corner = hi if max_corner is None else max_corner
+        nv = None if normals is None else _c(normals, np.float32)
+        self._nv = nv
+        self._h = lib().orc_cpu_tree_build(p, _ptr(nv), _c(masses, np.float32), self.n,
+                                           _c(min_corner, np.float32),
+                                           _c(max_corner, np.float32), int(leaf_size),
+                                           int(dim_split_log), int(max_depth))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().orc_cpu_tree_free(h)
+            self._h = None
+
+    @property
+    def num_nodes(self):
+        return int(lib().orc_cpu_tree_nodes(self._h))
+
+    def arrays(self):
+        nn = self.num_nodes
+        a = dict(start=np.zeros(nn, np.int64), end=np.zeros(nn, np.int64),
+                 first_child=np.zeros(nn, np.int64), nchild=np.zeros(nn, np.int32),
+                 is_leaf=np.zeros(nn, np.uint8), mass=np.zeros(nn, np.float32),
+                 com=np.zeros((nn, 3), np.float32), leaf_pos=np.zeros((self.n, 3), np.float32))
+        lib().orc_cpu_tree_arrays(self._h, a["start"], a["end"], a["first_child"], a["nchild"],
+                                  a["is_leaf"], a["mass"], a["com"], a["leaf_pos"])
+        return a
+
+    def eval(self, queries, kind=CPU_POTENTIAL, beta=2.0, eps=0.0, threads=0):
+        q = _c(queries, np.float32)
+        out = np.zeros((q.shape[0], 3), np.float64)
+        n_int = lib().orc_cpu_bh_eval(self._h, q, q.shape[0], int(kind), float(beta),
+                                      float(eps), int(threads), out)
+        res = out if kind == CPU_SOFT_FORCE else out[:, 0].copy()
+        return res, int(n_int)
+
+    def mlpcv(self, queries, kind=CPU_POTENTIAL, samples=1, seed=0, eps=0.0, threads=0):
+        q = _c(queries, np.float32)
+        out = np.zeros((q.shape[0], 3), np.float64)
+        lib().orc_cpu_mlpcv(self._h, q, q.shape[0], int(kind), int(samples), int(seed),
+                            float(eps), int(threads), out)
+        return out if kind == CPU_SOFT_FORCE else out[:, 0].copy()
 
 
 def bruteforce_wn(points, normals, areas, queries):

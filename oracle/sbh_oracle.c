@@ -33,6 +33,9 @@
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifdef __cplusplus
 #error "compile as C"
@@ -1347,78 +1350,116 @@ ORC_EXPORT void orc_tree_eval_range(int64_t nn, const int32_t* start, const int3
                                     double* force, double* gq, double* gsx, double* gsm,
                                     int64_t* counts, int64_t g_begin, int64_t g_end) {
   double e2 = (eps > 0 ? eps * eps : 0.0) + 1e-12;
+  /* Groups run on OpenMP threads; source-side and node adjoints are summed in
+   * per-thread buffers and reduced at the end (fp64, order-independent up to
+   * rounding). */
+  int nthreads = 1;
+#ifdef _OPENMP
+  nthreads = omp_get_max_threads();
+#endif
   double* adjX = NULL;
   double* adjM = NULL;
   double* adjS = NULL;
+  double* t_adj = NULL; /* per thread: 10 per node, then 4 per source */
+  const size_t per_thread = (size_t)nn * 10 + (size_t)n * 4;
   if (cot) {
     adjX = (double*)calloc((size_t)nn * 3, sizeof(double));
     adjM = (double*)calloc((size_t)nn, sizeof(double));
     adjS = (double*)calloc((size_t)nn * 6, sizeof(double));
+    t_adj = (double*)calloc(per_thread * (size_t)nthreads, sizeof(double));
     memset(gq, 0, sizeof(double) * 3 * (size_t)nq);
     memset(gsx, 0, sizeof(double) * 3 * (size_t)n);
     memset(gsm, 0, sizeof(double) * (size_t)n);
   }
   memset(force, 0, sizeof(double) * 3 * (size_t)nq);
   int64_t nfar = 0, nnear = 0;
-  int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 8));
-  int32_t* far = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
-  int32_t* near = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
-  for (int64_t g = g_begin; g < g_end; g++) {
-    const float* box = boxes + 6 * g;
-    int64_t nf = 0, nl = 0, top = 0;
-    stack[top++] = 0;
-    while (top > 0) {
-      int32_t k = stack[--top];
-      if (mass[k] == 0.0) continue;
-      if (orc_group_accept(box, geo + 4 * k, beta)) {
-        far[nf++] = k;
-      } else if (nchild[k] == 0) {
-        near[nl++] = k;
-      } else {
-        for (int32_t c = 0; c < nchild[k]; c++) stack[top++] = first_child[k] + c;
-      }
-    }
-    for (int64_t i = g_start[g]; i < g_start[g] + g_count[g] && i < nq; i++) {
-      const double* q = qx + 3 * i;
-      const double* c = cot ? cot + 3 * i : NULL;
-      double F[3] = {0, 0, 0}, G[3] = {0, 0, 0};
-      for (int64_t f = 0; f < nf; f++) {
-        int32_t k = far[f];
-        double dummyM = 0.0;
-        orc_far(q, mass[k], com + 3 * k, quad + 6 * k, use_quad, e2, c, F, G,
-                cot ? adjX + 3 * k : NULL, cot ? adjM + k : &dummyM,
-                cot ? adjS + 6 * k : NULL);
-      }
-      nfar += nf;
-      for (int64_t b = 0; b < nl; b++) {
-        int32_t k = near[b];
-        for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
-          if (self_mode && j == i) continue;
-          const double* p = xs + 3 * j;
-          double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
-          double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-          double s = 1.0 / sqrt(r2 + e2);
-          double s3 = s * s * s, s5 = s3 * s * s;
-          double m = ms[j];
-          for (int a = 0; a < 3; a++) F[a] += -m * s3 * d[a];
-          nnear++;
-          if (c) {
-            double cd = c[0] * d[0] + c[1] * d[1] + c[2] * d[2];
-            for (int a = 0; a < 3; a++) {
-              double v = m * (3.0 * s5 * cd * d[a] - s3 * c[a]);
-              G[a] += v;
-              gsx[3 * j + a] -= v;
-            }
-            gsm[j] += -s3 * cd;
-          }
+#pragma omp parallel num_threads(nthreads) reduction(+ : nfar, nnear)
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    double* my = cot ? t_adj + per_thread * (size_t)tid : NULL;
+    double* myX = my;                        /* 3 per node */
+    double* myM = my ? my + 3 * nn : NULL;   /* 1 per node */
+    double* myS = my ? my + 4 * nn : NULL;   /* 6 per node */
+    double* myGx = my ? my + 10 * nn : NULL; /* 3 per source */
+    double* myGm = my ? my + 10 * nn + 3 * n : NULL;
+    int32_t* stack = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 8));
+    int32_t* far = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+    int32_t* near = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+#pragma omp for schedule(dynamic, 8)
+    for (int64_t g = g_begin; g < g_end; g++) {
+      const float* box = boxes + 6 * g;
+      int64_t nf = 0, nl = 0, top = 0;
+      stack[top++] = 0;
+      while (top > 0) {
+        int32_t k = stack[--top];
+        if (mass[k] == 0.0) continue;
+        if (orc_group_accept(box, geo + 4 * k, beta)) {
+          far[nf++] = k;
+        } else if (nchild[k] == 0) {
+          near[nl++] = k;
+        } else {
+          for (int32_t c = 0; c < nchild[k]; c++) stack[top++] = first_child[k] + c;
         }
       }
-      for (int a = 0; a < 3; a++) force[3 * i + a] = F[a];
-      if (c)
-        for (int a = 0; a < 3; a++) gq[3 * i + a] = G[a];
+      for (int64_t i = g_start[g]; i < g_start[g] + g_count[g] && i < nq; i++) {
+        const double* q = qx + 3 * i;
+        const double* c = cot ? cot + 3 * i : NULL;
+        double F[3] = {0, 0, 0}, G[3] = {0, 0, 0};
+        for (int64_t f = 0; f < nf; f++) {
+          int32_t k = far[f];
+          double dummyM = 0.0;
+          orc_far(q, mass[k], com + 3 * k, quad + 6 * k, use_quad, e2, c, F, G,
+                  cot ? myX + 3 * k : NULL, cot ? myM + k : &dummyM, cot ? myS + 6 * k : NULL);
+        }
+        nfar += nf;
+        for (int64_t b = 0; b < nl; b++) {
+          int32_t k = near[b];
+          for (int32_t j = start[k]; j < start[k] + count[k]; j++) {
+            if (self_mode && j == i) continue;
+            const double* p = xs + 3 * j;
+            double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+            double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+            double s = 1.0 / sqrt(r2 + e2);
+            double s3 = s * s * s, s5 = s3 * s * s;
+            double m = ms[j];
+            for (int a = 0; a < 3; a++) F[a] += -m * s3 * d[a];
+            nnear++;
+            if (c) {
+              double cd = c[0] * d[0] + c[1] * d[1] + c[2] * d[2];
+              for (int a = 0; a < 3; a++) {
+                double v = m * (3.0 * s5 * cd * d[a] - s3 * c[a]);
+                G[a] += v;
+                myGx[3 * j + a] -= v;
+              }
+              myGm[j] += -s3 * cd;
+            }
+          }
+        }
+        for (int a = 0; a < 3; a++) force[3 * i + a] = F[a];
+        if (c)
+          for (int a = 0; a < 3; a++) gq[3 * i + a] = G[a];
+      }
     }
+    free(stack);
+    free(far);
+    free(near);
   }
   if (cot) {
+    for (int t = 0; t < nthreads; t++) {
+      const double* my = t_adj + per_thread * (size_t)t;
+      for (int64_t k = 0; k < nn; k++) {
+        for (int a = 0; a < 3; a++) adjX[3 * k + a] += my[3 * k + a];
+        adjM[k] += my[3 * nn + k];
+        for (int a = 0; a < 6; a++) adjS[6 * k + a] += my[4 * nn + 6 * k + a];
+      }
+      for (int64_t j = 0; j < n; j++) {
+        for (int a = 0; a < 3; a++) gsx[3 * j + a] += my[10 * nn + 3 * j + a];
+        gsm[j] += my[10 * nn + 3 * n + j];
+      }
+    }
     /* Push node adjoints down to the sources of each node. */
     for (int64_t k = 0; k < nn; k++) {
       double M = mass[k];
@@ -1441,10 +1482,8 @@ ORC_EXPORT void orc_tree_eval_range(int64_t nn, const int32_t* start, const int3
     free(adjX);
     free(adjM);
     free(adjS);
+    free(t_adj);
   }
-  free(stack);
-  free(far);
-  free(near);
   if (counts) {
     counts[0] = nfar;
     counts[1] = nnear;

@@ -852,3 +852,68 @@ def test_level3_vote_uses_root_monopole_only_for_all_far_warps():
     pot = O.l3_eval(leaf, node, q, O.L3_POTENTIAL, beta=2.0, threads=40)
     np.testing.assert_allclose(pot[:40], mono[:40], rtol=1e-6)
     np.testing.assert_allclose(pot[40:], exact[40:], rtol=1e-9)
+
+
+# --------------------------------------------------------------------- CPU path restatement
+@pytest.mark.parametrize("dsl,leaf", [(3, 1), (1, 16), (2, 4)])
+def test_cpu_tree_invariants_and_limits(dsl, leaf):
+    rng = _rng(41)
+    p = rng.uniform(-1, 1, (1500, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, 1500).astype(np.float32)
+    T = O.CpuTree(p, m, leaf_size=leaf, dim_split_log=dsl, max_depth=100)
+    a = T.arrays()
+    lf = a["is_leaf"] == 1
+    # leaves partition the points; children partition their parent
+    cover = np.zeros(1500, int)
+    for s0, e0 in zip(a["start"][lf], a["end"][lf]):
+        cover[s0:e0] += 1
+    assert (cover == 1).all()
+    assert ((a["end"] - a["start"])[lf] <= leaf).all()
+    for k in np.nonzero(~lf)[0]:
+        c0, c1 = a["first_child"][k], a["first_child"][k] + a["nchild"][k]
+        assert a["start"][c0] == a["start"][k] and a["end"][c1 - 1] == a["end"][k]
+        np.testing.assert_allclose(a["mass"][c0:c1].sum(dtype=np.float64), a["mass"][k],
+                                   rtol=1e-5)
+    np.testing.assert_array_equal(np.sort(a["leaf_pos"].view("V12").ravel()),
+                                  np.sort(p.view("V12").ravel()))
+    q = p[:200]
+    d = p[None].astype(np.float64) - q[:, None].astype(np.float64)
+    exact = (-m / (np.linalg.norm(d, axis=2) + 0.1)).sum(1)
+    pot, n_int = T.eval(q, O.CPU_POTENTIAL, beta=1e30)
+    np.testing.assert_allclose(pot, exact, rtol=1e-10)
+    assert n_int == 200 * 1500
+    # beta = 0 accepts the root: its monopole
+    pot0, n0 = T.eval(q, O.CPU_POTENTIAL, beta=0.0)
+    com = a["com"][0].astype(np.float64)
+    np.testing.assert_allclose(pot0, -a["mass"][0] / (np.linalg.norm(q - com, axis=1) + 0.1),
+                               rtol=1e-6)
+    assert n0 == 200
+    pot2, n2 = T.eval(q, O.CPU_POTENTIAL, beta=2.0)
+    assert n2 < n_int and O.rel_l2(pot2, exact) < 2e-3
+
+
+def test_cpu_tree_other_kernels_reduce_to_direct_sums():
+    rng = _rng(42)
+    p = rng.uniform(-1, 1, (800, 3)).astype(np.float32)
+    nv = rng.normal(size=(800, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, 800).astype(np.float32)
+    T = O.CpuTree(p, m, normals=nv, leaf_size=8, dim_split_log=1)
+    q = p[:100]
+    f, _ = T.eval(q, O.CPU_SOFT_FORCE, beta=1e30, eps=0.05)
+    ref = O.direct_force_soft(p, m, q, 0.05, np.inf)  # carries an extra 1e-12 in r^2
+    np.testing.assert_allclose(f, ref, rtol=1e-6, atol=1e-9 * np.abs(ref).max())
+    w, _ = T.eval(q + 0.01, O.CPU_WN, beta=1e30)
+    np.testing.assert_allclose(w, O.bruteforce_wn(p, nv, m, q + 0.01), rtol=1e-9, atol=1e-12)
+
+
+def test_cpu_mlpcv_is_unbiased():
+    rng = _rng(43)
+    p = rng.uniform(-1, 1, (600, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, 600).astype(np.float32)
+    T = O.CpuTree(p, m, leaf_size=4, dim_split_log=1, max_depth=100)
+    q = p[:16]
+    exact, _ = T.eval(q, O.CPU_POTENTIAL, beta=1e30)
+    runs = np.stack([T.mlpcv(q, O.CPU_POTENTIAL, samples=16, seed=s) for s in range(40)])
+    z = (runs.mean(0) - exact) / (runs.std(0, ddof=1) / np.sqrt(runs.shape[0]))
+    assert (z ** 2).mean() < 2.5 and np.abs(z).max() < 5.0
+    assert np.abs(runs[0] - exact).max() > 0
