@@ -501,52 +501,25 @@ __global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, 
 
 // ------------------------------------------------------------------ query groups
 // Partition of the sorted points into contiguous groups of <= kGroupMax
-// points that stay inside one cell: for each node holding more than
-// kGroupMax points, runs of consecutive children holding <= kGroupMax points
-// each are packed greedily; an over-full leaf (depth limit) is cut into
-// chunks; a root holding <= kGroupMax points is its set's only group.
+// points that stay inside one cell: every maximal node holding at most
+// kGroupCell points (its parent holds more, or it is a root), and every
+// over-full leaf at the depth limit, is cut into consecutive runs of
+// kGroupMax points (the last run shorter).  Runs cross the child cells of
+// such a node, which keeps wavefronts ~95% full, while the node bounds the
+// group's box.
 template <bool EMIT>
-__device__ __forceinline__ int node_groups(const int4* __restrict__ topo, int k, int nroots,
+__device__ __forceinline__ int node_groups(const int4* __restrict__ topo,
+                                           const int32_t* __restrict__ parent, int k,
                                            int2* __restrict__ out) {
-  const int gmax = kGroupMax;
   const int4 t = topo[k];
+  if (t.y == 0) return 0;
+  const int p = parent[k];
+  const bool maximal = t.y <= kGroupCell && (p < 0 || topo[p].y > kGroupCell);
+  const bool overfull_leaf = t.y > kGroupCell && t.w == 0;
+  if (!maximal && !overfull_leaf) return 0;
   int ng = 0;
-  if (k < nroots && t.y <= gmax) {
-    if (t.y > 0) {
-      if (EMIT) out[0] = make_int2(t.x, t.y);
-      ng = 1;
-    }
-    return ng;
-  }
-  if (t.y <= gmax) return 0;
-  if (t.w == 0) {
-    for (int s = t.x; s < t.x + t.y; s += gmax) {
-      if (EMIT) out[ng] = make_int2(s, min(gmax, t.x + t.y - s));
-      ++ng;
-    }
-    return ng;
-  }
-  int cs = 0, cc = 0;
-  for (int c = t.z; c < t.z + t.w; ++c) {
-    const int4 ct = topo[c];
-    if (ct.y > gmax) {
-      if (cc > 0) {
-        if (EMIT) out[ng] = make_int2(cs, cc);
-        ++ng;
-      }
-      cc = 0;
-      continue;
-    }
-    if (cc > 0 && cc + ct.y > gmax) {
-      if (EMIT) out[ng] = make_int2(cs, cc);
-      ++ng;
-      cc = 0;
-    }
-    if (cc == 0) cs = ct.x;
-    cc += ct.y;
-  }
-  if (cc > 0) {
-    if (EMIT) out[ng] = make_int2(cs, cc);
+  for (int s0 = t.x; s0 < t.x + t.y; s0 += kGroupMax) {
+    if (EMIT) out[ng] = make_int2(s0, min(kGroupMax, t.x + t.y - s0));
     ++ng;
   }
   return ng;
@@ -554,11 +527,11 @@ __device__ __forceinline__ int node_groups(const int4* __restrict__ topo, int k,
 
 __global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ hdr,
                                                     const int4* __restrict__ topo,
+                                                    const int32_t* __restrict__ parent,
                                                     uint32_t* __restrict__ goff) {
   const int nn = hdr->num_nodes;
-  const int nroots = hdr->level_count[0];
   for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, k, nroots, nullptr));
+    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, parent, k, nullptr));
 }
 
 // Exclusive scan of the per-node group counts (one block); publishes the
@@ -594,12 +567,12 @@ __global__ __launch_bounds__(1024) void k_group_scan(Header* __restrict__ hdr,
 
 __global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hdr,
                                                    const int4* __restrict__ topo,
+                                                   const int32_t* __restrict__ parent,
                                                    const uint32_t* __restrict__ goff,
                                                    int2* __restrict__ groups) {
   const int nn = hdr->num_nodes;
-  const int nroots = hdr->level_count[0];
   for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    node_groups<true>(topo, k, nroots, groups + goff[k]);
+    node_groups<true>(topo, parent, k, groups + goff[k]);
 }
 
 // ------------------------------------------------------------------ wave helpers
@@ -695,15 +668,16 @@ struct Lane {
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
 // Adds the wave-summed batch partials v[0..V) (PER values per entry, the
-// first USED of them meaningful) to dst: value j goes to
+// first USED of them meaningful; negated when NEG) to dst: value j goes to
 // dst[stride * e + j % PER] with e = entry[j / PER] (or base + j / PER when
 // entry is null); e == skip and e >= limit are dropped.  One lane per value
 // issues the atomic.
-template <int V, int PER, int USED>
+template <int V, int PER, int USED, bool NEG>
 __device__ __forceinline__ void scatter_batch(float (&v)[V], int lane, float* dst, int stride,
                                               const int* entry, int skip, int base, int limit) {
   constexpr int sh = 6 - ilog2(V);
-  const float s = wave_transpose_sum<V>(v, lane);
+  const float sum = wave_transpose_sum<V>(v, lane);
+  const float s = NEG ? -sum : sum;
   const int j = lane >> sh;
   if ((lane & ((1 << sh) - 1)) == 0 && (j % PER) < USED) {
     const int e = entry != nullptr ? entry[j / PER] : base + j / PER;
@@ -720,14 +694,13 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   int idx[C];
 #pragma unroll
   for (int t = 0; t < C; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
-  float part[4 * C];
+  float part[4 * C];   // minus the node-side partials (negated in the scatter)
   float qpart[8 * C];
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     const float4 m = sload4(A.mp, idx[t]);
     const float Dx = L.x - m.x, Dy = L.y - m.y, Dz = L.z - m.z;
-    const float r2 = Dx * Dx + Dy * Dy + Dz * Dz;
-    const float s = rsq(r2 + A.e2);
+    const float s = rsq(fmaf(Dz, Dz, fmaf(Dy, Dy, fmaf(Dx, Dx, A.e2))));
     const float s2 = s * s, s3 = s2 * s;
     if constexpr (!QUAD) {
       const float a = -m.w * s3;
@@ -745,10 +718,10 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         gx += dDx;
         gy += dDy;
         gz += dDz;
-        part[4 * t + 0] = -dDx;
-        part[4 * t + 1] = -dDy;
-        part[4 * t + 2] = -dDz;
-        part[4 * t + 3] = -s3 * u;
+        part[4 * t + 0] = dDx;
+        part[4 * t + 1] = dDy;
+        part[4 * t + 2] = dDz;
+        part[4 * t + 3] = s3 * u;
       }
     } else {
       const float4 q0 = sload4(A.qd, 2 * idx[t]);
@@ -784,18 +757,23 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         gx += dDx;
         gy += dDy;
         gz += dDz;
-        part[4 * t + 0] = -dDx;
-        part[4 * t + 1] = -dDy;
-        part[4 * t + 2] = -dDz;
-        part[4 * t + 3] = -s3 * u;
-        // dL/dS (symmetric), (xx, yy, zz, xy, xz, yz)
-        const float h5 = 1.5f * s5, h7 = -7.5f * s7 * u, hu = h5 * u;
-        qpart[8 * t + 0] = 2.0f * h5 * L.cx * Dx + h7 * Dx * Dx + hu;
-        qpart[8 * t + 1] = 2.0f * h5 * L.cy * Dy + h7 * Dy * Dy + hu;
-        qpart[8 * t + 2] = 2.0f * h5 * L.cz * Dz + h7 * Dz * Dz + hu;
-        qpart[8 * t + 3] = h5 * (L.cx * Dy + L.cy * Dx) + h7 * Dx * Dy;
-        qpart[8 * t + 4] = h5 * (L.cx * Dz + L.cz * Dx) + h7 * Dx * Dz;
-        qpart[8 * t + 5] = h5 * (L.cy * Dz + L.cz * Dy) + h7 * Dy * Dz;
+        part[4 * t + 0] = dDx;
+        part[4 * t + 1] = dDy;
+        part[4 * t + 2] = dDz;
+        part[4 * t + 3] = s3 * u;
+        // dL/dS (symmetric), (xx, yy, zz, xy, xz, yz):
+        //   G = h5 (c D^T + D c^T) + h7 D D^T + h5 u I,  h5 = 1.5 s^5, h7 = -7.5 s^7 u
+        //     = P D^T + D P^T + h5 u I  with  P = h5 c + (h7 / 2) D
+        const float h5 = 1.5f * s5, hh7 = -3.75f * s7 * u, hu = h5 * u;
+        const float Px = fmaf(hh7, Dx, h5 * L.cx);
+        const float Py = fmaf(hh7, Dy, h5 * L.cy);
+        const float Pz = fmaf(hh7, Dz, h5 * L.cz);
+        qpart[8 * t + 0] = fmaf(2.0f * Px, Dx, hu);
+        qpart[8 * t + 1] = fmaf(2.0f * Py, Dy, hu);
+        qpart[8 * t + 2] = fmaf(2.0f * Pz, Dz, hu);
+        qpart[8 * t + 3] = fmaf(Px, Dy, Dx * Py);
+        qpart[8 * t + 4] = fmaf(Px, Dz, Dx * Pz);
+        qpart[8 * t + 5] = fmaf(Py, Dz, Dy * Pz);
         qpart[8 * t + 6] = 0.f;
         qpart[8 * t + 7] = 0.f;
       }
@@ -803,9 +781,9 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   }
   if constexpr (VJP) {
     // node adjoints: (dL/dX, dL/dM) and the six dL/dS entries
-    scatter_batch<4 * C, 4, 4>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
+    scatter_batch<4 * C, 4, 4, true>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
     if constexpr (QUAD)
-      scatter_batch<8 * C, 8, 6>(qpart, lane, A.adjq, 8, far, A.sentinel, 0, INT_MAX);
+      scatter_batch<8 * C, 8, 6, false>(qpart, lane, A.adjq, 8, far, A.sentinel, 0, INT_MAX);
   }
 }
 
@@ -815,40 +793,43 @@ template <bool VJP, bool FORCES>
 __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
                                            int s1, int lane, float& fx, float& fy, float& fz,
                                            float& gx, float& gy, float& gz) {
-  float part[4 * kChunk];
+  float part[4 * kChunk];  // minus the source-side partials (negated in the scatter)
 #pragma unroll
   for (int u = 0; u < kChunk; ++u) {
     const int j = j0 + u;
     const float4 src = sload4(A.sp, min(j, s1 - 1));
-    const float w = (j < s1) ? src.w : 0.0f;
+    // zero weight past the leaf end; j, s1 and the record are wave-uniform, so
+    // the mask stays on the scalar unit
+    const float w = __uint_as_float(
+        __builtin_amdgcn_readfirstlane(__float_as_uint(src.w) & (j < s1 ? ~0u : 0u)));
     const float dx = L.x - src.x, dy = L.y - src.y, dz = L.z - src.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    const float s = rsq(r2 + A.e2);
-    float s3 = s * s * s;
-    s3 = (j == i) ? 0.0f : s3;  // the self pair contributes nothing
-    const float a = -w * s3;
+    const float s = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, A.e2))));
+    const float s2 = s * s;
+    float ws3 = (w * s) * s2;  // w s^3
+    // the self pair: its force term vanishes with d = 0, its adjoint terms do not
+    if constexpr (VJP) ws3 = (j == i) ? 0.0f : ws3;
     if constexpr (FORCES) {
-      fx += a * dx;
-      fy += a * dy;
-      fz += a * dz;
+      fx = fmaf(-ws3, dx, fx);
+      fy = fmaf(-ws3, dy, fy);
+      fz = fmaf(-ws3, dz, fz);
     }
     if constexpr (VJP) {
       const float cd = L.cx * dx + L.cy * dy + L.cz * dz;
-      const float k1 = -3.0f * a * s * s * cd;  // 3 w s^5 (c.d)
-      const float vx = k1 * dx + a * L.cx;
-      const float vy = k1 * dy + a * L.cy;
-      const float vz = k1 * dz + a * L.cz;
+      const float k1 = 3.0f * ws3 * s2 * cd;  // 3 w s^5 (c.d)
+      const float vx = k1 * dx - ws3 * L.cx;
+      const float vy = k1 * dy - ws3 * L.cy;
+      const float vz = k1 * dz - ws3 * L.cz;
       gx += vx;
       gy += vy;
       gz += vz;
-      part[4 * u + 0] = -vx;
-      part[4 * u + 1] = -vy;
-      part[4 * u + 2] = -vz;
-      part[4 * u + 3] = -s3 * cd;
+      part[4 * u + 0] = vx;
+      part[4 * u + 1] = vy;
+      part[4 * u + 2] = vz;
+      part[4 * u + 3] = ((j == i) ? 0.0f : s * s2) * cd;  // s^3 (c.d)
     }
   }
   if constexpr (VJP)
-    scatter_batch<4 * kChunk, 4, 4>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
+    scatter_batch<4 * kChunk, 4, 4, true>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
@@ -1176,11 +1157,11 @@ int build_rest(hipStream_t s, char* ws, const Layout& L, const float* pts, const
   auto* goff = reinterpret_cast<uint32_t*>(ws + L.goff);
   auto* groups = reinterpret_cast<int2*>(ws + L.groups);
   unsigned gg = grid_for(L.cap, kB, 4096);
-  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, goff);
+  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, parent, goff);
   SBH_LAUNCH_CHECK();
   k_group_scan<<<1, 1024, 0, s>>>(hdr, goff);
   SBH_LAUNCH_CHECK();
-  k_group_emit<<<gg, kB, 0, s>>>(hdr, topo, goff, groups);
+  k_group_emit<<<gg, kB, 0, s>>>(hdr, topo, parent, goff, groups);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }

@@ -29,7 +29,8 @@ struct Header {
   unsigned long long stats[8];  // far interactions, near pairs, groups
 };
 
-constexpr int kGroupMax = 64;  // one wavefront per query group
+constexpr int kGroupMax = 64;     // one wavefront per query group
+constexpr int kGroupCell = 2048;  // groups are runs inside cells of at most this many points
 
 // Byte offsets of every array inside the workspace.  Node arrays hold `cap`
 // entries (an upper bound on the node count, see node_capacity()).  A

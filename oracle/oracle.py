@@ -104,7 +104,8 @@ def _declare(L: ctypes.CDLL) -> None:
                                  _i64, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, _f32p]
     L.orc_tree_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.orc_tree_groups.restype = _i64
-    L.orc_tree_groups.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, ctypes.c_int, _i32p, _i32p]
+    L.orc_tree_groups.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _i32p, ctypes.c_int,
+                                  ctypes.c_int, _i32p, _i32p]
     L.orc_group_boxes.argtypes = [_f32p, _i64, _i32p, _i32p, _f32p]
     L.orc_tree_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
                                 _f64p, _f64p, _i64, _f32p, _f64p, _i64, _i64, _i32p, _i32p,
@@ -435,6 +436,9 @@ def triangle_wn(verts, faces, queries, chunk=256):
 
 
 # ---------------------------------------------------------------- adaptive tree
+GROUP_CELL = 2048  # query groups are runs of <= 64 points inside cells of <= 2048 points
+
+
 class Tree:
     """Adaptive Morton octree built by the oracle (topology in numpy arrays)."""
 
@@ -480,12 +484,12 @@ class Tree:
                                _c(ms, np.float64), mass, com, quad)
         return mass, com, quad
 
-    def groups(self, gmax=64):
+    def groups(self, gmax=64, cell_max=GROUP_CELL):
         """Query groups of the walk as (start, count) arrays (orc_tree_groups)."""
         gs = np.zeros(max(self.n, 1), np.int32)
         gc = np.zeros(max(self.n, 1), np.int32)
         ng = lib().orc_tree_groups(self.num_nodes, self.start, self.count, self.first_child,
-                                   self.nchild, int(gmax), gs, gc)
+                                   self.nchild, self.parent, int(gmax), int(cell_max), gs, gc)
         return gs[:ng].copy(), gc[:ng].copy()
 
     def group_boxes(self, q_sorted_f32, gmax=64):

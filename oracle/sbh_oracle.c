@@ -1164,57 +1164,26 @@ ORC_EXPORT void orc_tree_moments(const int32_t* start, const int32_t* count, int
 }
 
 /* Query groups of the walk: a partition of the sorted points into contiguous
- * ranges of at most gmax points that stay inside one cell.  For every node
- * holding more than gmax points, its children are scanned in order and runs
- * of consecutive children with <= gmax points each are packed greedily into
- * groups of <= gmax points (children above gmax are nodes of their own); a
- * leaf holding more than gmax points (possible at the depth limit) is cut
- * into consecutive chunks of gmax.  If the root holds <= gmax points it is
- * the only group.  Order: by node index, then child order.  Returns the
- * number of groups written to g_start/g_count. */
+ * ranges of at most gmax points that stay inside one cell.  Every maximal
+ * node holding at most cell_max points (its parent holds more, or it has no
+ * parent) and every leaf holding more than cell_max points (possible at the
+ * depth limit) is cut into consecutive runs of gmax points, the last run
+ * shorter.  Order: by node index.  Returns the number of groups. */
 ORC_EXPORT int64_t orc_tree_groups(int64_t nn, const int32_t* start, const int32_t* count,
                                    const int32_t* first_child, const int32_t* nchild,
-                                   int gmax, int32_t* g_start, int32_t* g_count) {
+                                   const int32_t* parent, int gmax, int cell_max,
+                                   int32_t* g_start, int32_t* g_count) {
+  (void)first_child;
   int64_t ng = 0;
-  if (nn < 1 || count[0] == 0) return 0;
-  if (count[0] <= gmax) {
-    g_start[0] = start[0];
-    g_count[0] = count[0];
-    return 1;
-  }
   for (int64_t k = 0; k < nn; k++) {
-    if (count[k] <= gmax) continue;
-    if (nchild[k] == 0) {
-      for (int32_t s = start[k]; s < start[k] + count[k]; s += gmax) {
-        g_start[ng] = s;
-        g_count[ng] = (start[k] + count[k] - s) < gmax ? (start[k] + count[k] - s) : gmax;
-        ng++;
-      }
-      continue;
-    }
-    int32_t cs = -1, cc = 0;
-    for (int32_t c = first_child[k]; c < first_child[k] + nchild[k]; c++) {
-      if (count[c] > gmax) {
-        if (cc > 0) {
-          g_start[ng] = cs;
-          g_count[ng] = cc;
-          ng++;
-        }
-        cc = 0;
-        continue;
-      }
-      if (cc > 0 && cc + count[c] > gmax) {
-        g_start[ng] = cs;
-        g_count[ng] = cc;
-        ng++;
-        cc = 0;
-      }
-      if (cc == 0) cs = start[c];
-      cc += count[c];
-    }
-    if (cc > 0) {
-      g_start[ng] = cs;
-      g_count[ng] = cc;
+    if (count[k] == 0) continue;
+    int32_t p = parent[k];
+    int maximal = count[k] <= cell_max && (p < 0 || count[p] > cell_max);
+    int overfull_leaf = count[k] > cell_max && nchild[k] == 0;
+    if (!maximal && !overfull_leaf) continue;
+    for (int32_t s = start[k]; s < start[k] + count[k]; s += gmax) {
+      g_start[ng] = s;
+      g_count[ng] = (start[k] + count[k] - s) < gmax ? (start[k] + count[k] - s) : gmax;
       ng++;
     }
   }

@@ -480,27 +480,34 @@ def test_tree_topology_invariants():
 def test_tree_groups_partition_and_stay_in_one_cell(gmax):
     n = 20000
     p = _plummer(n, 4)
-    p[:300] = p[0]  # a depth-limited leaf with more than gmax points
+    p[:300] = p[0]  # a depth-limited leaf with more points than a group cell
     m = np.full(n, 1.0 / n, np.float32)
     T = O.Tree(p, m, max_depth=10, leaf_size=16)
-    gs, gc = T.groups(gmax)
-    # contiguous partition of [0, n)
+    cell_max = 256
+    gs, gc = T.groups(gmax, cell_max)
+    # contiguous partition of [0, n) into runs of <= gmax
     order = np.argsort(gs)
-    gs, gc = gs[order], gc[order]
-    assert gs[0] == 0 and np.all(gs[1:] == gs[:-1] + gc[:-1]) and gs[-1] + gc[-1] == n
+    gs_s, gc_s = gs[order], gc[order]
+    assert gs_s[0] == 0 and np.all(gs_s[1:] == gs_s[:-1] + gc_s[:-1])
+    assert gs_s[-1] + gc_s[-1] == n
     assert gc.min() >= 1 and gc.max() <= gmax
-    # every group lies inside the cell of one node holding more than gmax
-    # points (its parent), or inside one over-full leaf
-    for s, c in zip(gs, gc):
-        inside = (T.start <= s) & (T.start + T.count >= s + c) & (T.count > gmax)
-        assert inside.any()
-    # groups are not split needlessly: two consecutive groups under the same
-    # parent could not have been merged
+    # independent restatement: maximal nodes of <= cell_max points, and
+    # over-full leaves, cut into runs of gmax from their start
+    cnt, par = T.count, T.parent
+    maximal = (cnt > 0) & (cnt <= cell_max) & ((par < 0) | (cnt[np.maximum(par, 0)] > cell_max))
+    overfull = (cnt > cell_max) & (T.nchild == 0)
+    assert overfull.any()
+    expect = []
+    for k in np.nonzero(maximal | overfull)[0]:
+        for s0 in range(T.start[k], T.start[k] + cnt[k], gmax):
+            expect.append((s0, min(gmax, T.start[k] + cnt[k] - s0)))
+    assert sorted(expect) == sorted(zip(gs.tolist(), gc.tolist()))
+    # each group's box lies inside the cell of its node
     boxes = T.group_boxes(T.points_sorted, gmax).reshape(-1, 6)
     diag = np.linalg.norm(boxes[:, 3:] - boxes[:, :3], axis=1)
-    consecutive = T.groups(gmax)
-    assert diag.max() <= np.float32(T.span) * np.sqrt(3) * 0.5 + 1e-3
-    assert len(consecutive[0]) == len(gs)
+    assert diag.max() <= np.float32(T.span) * np.sqrt(3) + 1e-3
+    default = T.groups(gmax)
+    assert (default[1] <= gmax).all() and default[1].sum() == n
 
 
 def test_tree_huge_beta_is_direct_sum():
