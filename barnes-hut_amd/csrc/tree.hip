@@ -346,29 +346,44 @@ __global__ __launch_bounds__(kB) void k_split(Header* __restrict__ hdr, int leve
 }
 
 // Single-block exclusive scan of the level's child counts; sets the next level.
+// In-place exclusive scan of count u32 values by one block of 1024 threads
+// (tiles of 4096: 4 consecutive values per thread, wave scans, a scan of the
+// 16 wave totals); returns the total to every thread.
+__device__ uint32_t block_exclusive_scan(uint32_t* __restrict__ data, int count) {
+  __shared__ uint32_t wsum[16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  for (int base = 0; base < count; base += 4096) {
+    const int i0 = base + 4 * static_cast<int>(threadIdx.x);
+    uint32_t v[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = i0 + e < count ? data[i0 + e] : 0u;
+    const uint32_t local = v[0] + v[1] + v[2] + v[3];
+    const uint32_t incl = static_cast<uint32_t>(wave_incl_scan(static_cast<int>(local), lane));
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+      const int w = lane < 16 ? static_cast<int>(wsum[lane]) : 0;
+      const int wi = wave_incl_scan(w, lane);
+      if (lane < 16) wsum[lane] = static_cast<uint32_t>(wi);
+    }
+    __syncthreads();
+    uint32_t run = carry + (wid > 0 ? wsum[wid - 1] : 0u) + incl - local;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      if (i0 + e < count) data[i0 + e] = run;
+      run += v[e];
+    }
+    carry += wsum[15];
+    __syncthreads();  // wsum is rewritten by the next tile
+  }
+  return carry;
+}
+
 __global__ __launch_bounds__(1024) void k_scan_level(Header* __restrict__ hdr, int level,
                                                      uint32_t* __restrict__ nch, int cap) {
-  __shared__ uint32_t sh[1024];
-  __shared__ uint32_t carry;
   const int count = hdr->level_count[level];
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < count; base += 1024) {
-    int i = base + threadIdx.x;
-    uint32_t v = i < count ? nch[i] : 0u;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0u;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < count) nch[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += sh[1023];
-    __syncthreads();
-  }
+  const uint32_t carry = block_exclusive_scan(nch, count);
   if (threadIdx.x == 0) {
     int next_start = hdr->level_start[level] + count;
     int total = static_cast<int>(carry);
@@ -495,7 +510,7 @@ __global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, 
     }
     mp[k] = make_float4(X, Y, Z, M);
     qd[2 * k] = make_float4(sxx, syy, szz, sxy);
-    qd[2 * k + 1] = make_float4(sxz, syz, 0.f, 0.f);
+    qd[2 * k + 1] = make_float4(sxz, syz, 1.5f * (sxx + syy + szz), 0.f);  // + 1.5 tr(S)
   }
 }
 
@@ -538,27 +553,7 @@ __global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ h
 // group count and resets the walk's work queue.
 __global__ __launch_bounds__(1024) void k_group_scan(Header* __restrict__ hdr,
                                                      uint32_t* __restrict__ goff) {
-  __shared__ uint32_t sh[1024];
-  __shared__ uint32_t carry;
-  const int count = hdr->num_nodes;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (int base = 0; base < count; base += 1024) {
-    int i = base + threadIdx.x;
-    uint32_t v = i < count ? goff[i] : 0u;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-      uint32_t t = threadIdx.x >= static_cast<unsigned>(off) ? sh[threadIdx.x - off] : 0u;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (i < count) goff[i] = carry + sh[threadIdx.x] - v;
-    __syncthreads();
-    if (threadIdx.x == 0) carry += sh[1023];
-    __syncthreads();
-  }
+  const uint32_t carry = block_exclusive_scan(goff, hdr->num_nodes);
   if (threadIdx.x == 0) {
     hdr->num_groups = static_cast<int>(carry);
     hdr->queue = 0;
@@ -576,17 +571,33 @@ __global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hd
 }
 
 // ------------------------------------------------------------------ wave helpers
-// One butterfly stage of a transposed wave reduction: lanes with bit `d` set
+// Partner value across lanes by a DPP move (no LDS crossbar):
+//   kRowRor8     lane l <- (l + 8) mod 16 within its row of 16, i.e. l ^ 8
+//   kHalfMirror  lane l <- 7 - l within its half-row of 8 (partner bit 2 flipped)
+//   kQuadXor2 / kQuadXor1  quad permutations l ^ 2 and l ^ 1
+constexpr int kRowRor8 = 0x128;
+constexpr int kHalfMirror = 0x141;
+constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int kQuadXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_partner(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+
+// One butterfly stage of a transposed wave reduction: lanes with bit `D` set
 // keep the upper half of the first 2H values (adding the partner's copy),
-// the others keep the lower half.
-template <int H, int N>
-__device__ __forceinline__ void butterfly(float (&v)[N], int lane, int d) {
-  const bool hi = lane & d;
+// the others keep the lower half.  The partner is any lane whose bit D
+// differs while the higher bits agree (so it holds the same value indices
+// summed over a disjoint set of lanes); the DPP patterns above qualify.
+template <int H, int N, int D, int CTRL>
+__device__ __forceinline__ void butterfly(float (&v)[N], int lane) {
+  const bool hi = lane & D;
 #pragma unroll
   for (int k = 0; k < H; ++k) {
     float send = hi ? v[k] : v[H + k];
     float keep = hi ? v[H + k] : v[k];
-    v[k] = keep + __shfl_xor(send, d, 64);
+    v[k] = keep + dpp_partner<CTRL>(send);
   }
 }
 
@@ -621,17 +632,17 @@ __device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
   static_assert(V == 16 || V == 32 || V == 64, "unsupported width");
   butterfly_swap32<V / 2>(v);
   butterfly_swap16<V / 4>(v);
-  butterfly<V / 8>(v, lane, 8);
-  butterfly<V / 16>(v, lane, 4);
+  butterfly<V / 8, V, 8, kRowRor8>(v, lane);
+  butterfly<V / 16, V, 4, kHalfMirror>(v, lane);
   if constexpr (V >= 32) {
-    butterfly<V / 32>(v, lane, 2);
+    butterfly<V / 32, V, 2, kQuadXor2>(v, lane);
   } else {
-    v[0] += __shfl_xor(v[0], 2, 64);
+    v[0] += dpp_partner<kQuadXor2>(v[0]);
   }
   if constexpr (V >= 64) {
-    butterfly<1>(v, lane, 1);
+    butterfly<1, V, 1, kQuadXor1>(v, lane);
   } else {
-    v[0] += __shfl_xor(v[0], 1, 64);
+    v[0] += dpp_partner<kQuadXor1>(v[0]);
   }
   return v[0];
 }
@@ -732,8 +743,8 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       const float SDy = q0.w * Dx + q0.y * Dy + q1.y * Dz;
       const float SDz = q1.x * Dx + q1.y * Dy + q0.z * Dz;
       const float tq = Dx * SDx + Dy * SDy + Dz * SDz;
-      const float T = q0.x + q0.y + q0.z;
-      const float a = -m.w * s3 + 1.5f * T * s5 - 7.5f * s7 * tq;
+      const float T15 = q1.z;  // 1.5 tr(S), stored with the moments
+      const float a = -m.w * s3 + T15 * s5 - 7.5f * s7 * tq;
       const float b = 3.0f * s5;
       if constexpr (FORCES) {
         fx += a * Dx + b * SDx;
@@ -747,9 +758,9 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         const float Scy = q0.w * L.cx + q0.y * L.cy + q1.y * L.cz;
         const float Scz = q1.x * L.cx + q1.y * L.cy + q0.z * L.cz;
         const float cw = L.cx * SDx + L.cy * SDy + L.cz * SDz;
-        const float kD = 3.0f * m.w * s5 * u - 15.0f * s7 * cw - 7.5f * T * s7 * u +
+        const float kD = 3.0f * m.w * s5 * u - 15.0f * s7 * cw - 5.0f * T15 * s7 * u +
                          52.5f * s9 * tq * u;
-        const float kc = a;  // -M s^3 + 1.5 T s^5 - 7.5 s^7 t
+        const float kc = a;  // -M s^3 + 1.5 tr(S) s^5 - 7.5 s^7 t
         const float b15 = -15.0f * s7 * u;
         const float dDx = kD * Dx + kc * L.cx + b * Scx + b15 * SDx;
         const float dDy = kD * Dy + kc * L.cy + b * Scy + b15 * SDy;

@@ -81,8 +81,13 @@ def gravity_force(points, masses, *, beta: float = 2.0, softening_scale: float =
 def gravity_force_vjp(points, masses, cotangent, *, beta: float = 2.0,
                       softening_scale: float = 1e-2, max_depth: int = 10, leaf_size: int = 32,
                       multipole_order: int = 2, workspace: Optional[torch.Tensor] = None,
-                      stats: Optional[torch.Tensor] = None, return_forces: bool = False):
-    """(grad_points, grad_masses) of L = sum_i c_i . F_i [, forces]."""
+                      stats: Optional[torch.Tensor] = None, return_forces: bool = False,
+                      grad_points_out: Optional[torch.Tensor] = None,
+                      grad_masses_out: Optional[torch.Tensor] = None):
+    """(grad_points, grad_masses) of L = sum_i c_i . F_i [, forces].
+
+    A graph capture passes a preallocated `workspace`, `grad_points_out` and
+    `grad_masses_out` (return_forces allocates)."""
     _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
     points, masses, cotangent = _f32(points), _f32(masses), _f32(cotangent)
     if points.dim() != 2 or points.shape[1] != 3:
@@ -91,8 +96,8 @@ def gravity_force_vjp(points, masses, cotangent, *, beta: float = 2.0,
         raise ValueError("masses must have shape (N,)")
     if cotangent.shape != points.shape:
         raise ValueError("cotangent must have same shape as points")
-    gx = torch.empty_like(points)
-    gm = torch.empty_like(masses)
+    gx = grad_points_out if grad_points_out is not None else torch.empty_like(points)
+    gm = grad_masses_out if grad_masses_out is not None else torch.empty_like(masses)
     forces = torch.empty_like(points) if return_forces else None
     call("sbh_gravity_force_vjp", stream_handle(), buf(points), buf(masses), buf(cotangent),
          buf(forces), buf(gx), buf(gm), f32(beta), f32(softening_scale), i32(max_depth),

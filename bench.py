@@ -32,6 +32,7 @@ for _p in (ROOT, os.path.join(ROOT, "barnes-hut_amd")):
 
 METRIC = "BH forces/sec at N=1M θ=0.5 (rel_l2≤1e-3), fwd+VJP; batched vs per-set"
 FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
 # Algorithmic flops per lane interaction (one query x one node or source).
 FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
 FLOPS_FAR_FWD = {1: 20, 2: 54}
@@ -143,6 +144,31 @@ def cpu_reference_path(p, m, args, ref_idx, ref, nsample=1 << 15):
                        "softened force, single-threaded build, strided OpenMP queries"}
 
 
+def cpu_reference_literal(p, m, args, nsample=1 << 14):
+    """The reference's CPU path as it ships (src/sbh.cpp:49-88): 512-ary
+    Octree3 (Tree3D<1,3>), leaf size 1, max depth 100, potential
+    -m/(r + 0.1), MAC |q - COM| / diagonal >= beta; walk timed on `nsample`
+    strided queries and scaled."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    n = p.shape[0]
+    t0 = time.perf_counter()
+    T = O.CpuTree(p, m, leaf_size=1, dim_split_log=3, max_depth=100)
+    t_build = time.perf_counter() - t0
+    sel = np.arange(0, n, max(1, n // nsample))[:nsample]
+    t0 = time.perf_counter()
+    _, n_int = T.eval(p[sel], O.CPU_POTENTIAL, beta=args.beta)
+    t_eval = (time.perf_counter() - t0) * n / sel.size
+    return {"value": n / (t_build + t_eval), "unit": "potentials/s (forward only)",
+            "threads": O.num_threads(), "build_s": round(t_build, 3),
+            "eval_s_scaled": round(t_eval, 3), "sample_queries": int(sel.size),
+            "interactions_per_query": n_int / sel.size,
+            "variant": "Tree3D<1,3> (512-ary Octree3), leaf 1, depth 100, potential "
+                       "-m/(r+0.1), single-threaded build, strided OpenMP queries"}
+
+
 def cpu_numpy_direct(p, m, args, nsample=64):
     """The reference benchmark's numpy direct sum (unified_force_benchmark.py:
     208-230, fp32, one query at a time), timed on `nsample` queries and scaled."""
@@ -178,14 +204,23 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--beta", type=float, default=2.0)
     ap.add_argument("--accuracy-queries", type=int, default=1024)
-    ap.add_argument("--cpu-sample-groups", type=int, default=4000)
+    ap.add_argument("--cpu-sample-groups", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
-    ap.add_argument("--workload", default="gravity", choices=("gravity", "wn", "batched"),
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time eager launches only (default: the step is also captured as a "
+                         "HIP graph and the replay is the reported value)")
+    ap.add_argument("--workload", default="gravity",
+                    choices=("gravity", "wn", "batched", "stochastic"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
-                         "graph-captured forest pipeline vs 64 per-set calls")
+                         "graph-captured forest pipeline vs 64 per-set calls; stochastic: "
+                         "classic and stochastic BH on the dense octree8 at the headline "
+                         "Plummer sphere, with the seed-mean unbiasedness check")
+    ap.add_argument("--o8-depth", type=int, default=8)
+    ap.add_argument("--seeds", type=int, default=256)
+    ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--sets", type=int, default=64)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--wn-freq", type=int, default=101,
@@ -197,7 +232,8 @@ def main():
                                    "it (queries would shard by Morton range, see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
-    bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched}[args.workload]
+    bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
+             "stochastic": bench_stochastic}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -223,10 +259,12 @@ def bench_gravity(args):
               leaf_size=args.leaf, multipole_order=args.order)
     ws = G.new_workspace(n, args.depth, args.leaf, dev)
     F = torch.empty_like(P)
+    GX, GM = torch.empty_like(P), torch.empty_like(M)
 
     def step():
         G.gravity_force(P, M, workspace=ws, out=F, **kw)
-        return G.gravity_force_vjp(P, M, C, workspace=ws, **kw)
+        return G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX,
+                                   grad_masses_out=GM, **kw)
 
     if args.copy_calibration_mb > 0:
         src = torch.ones(args.copy_calibration_mb << 18, dtype=torch.float32, device=dev)
@@ -250,7 +288,28 @@ def bench_gravity(args):
     lib.sbh_profile_enable(ctypes.c_int32(0))
     prof = (ctypes.c_double * 6)()
     lib.sbh_profile_collect(prof, ctypes.c_int32(6))
-    ms_step = ev0.elapsed_time(ev1) / args.steps
+    eager_ms = ev0.elapsed_time(ev1) / args.steps
+
+    # the same step captured once as a HIP graph and replayed (no per-launch
+    # host work); every replay recomputes everything
+    graph_ms = None
+    if not args.no_graph:
+        from sbh_amd.graph import Graph
+
+        g = Graph()
+        g.capture(step)
+        for _ in range(max(args.warmup, 1)):
+            g.replay()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            g.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        graph_ms = ev0.elapsed_time(ev1) / args.steps
+        g.close()
+        gx, gm = GX, GM
+    ms_step = graph_ms if graph_ms is not None else eager_ms
     value = n / (ms_step * 1e-3)
 
     # interaction counts (deterministic: identical every step)
@@ -295,11 +354,12 @@ def bench_gravity(args):
         acc["grad_masses_rel_l2_vs_fp32_direct"] = rel_l2(gm, gmd)
         acc["force_full_rel_l2_vs_fp32_direct"] = rel_l2(F, Fd)
 
-    cpu = cpu_ref = cpu_np = None
+    cpu = cpu_ref = cpu_np = cpu_lit = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, m, c, args, args.cpu_sample_groups)
         cpu_ref = cpu_reference_path(p, m, args, idx.cpu().numpy(), ref.cpu().numpy())
         cpu_np = cpu_numpy_direct(p, m, args)
+        cpu_lit = cpu_reference_literal(p, m, args)
 
     rec = {
         "metric": METRIC,
@@ -322,7 +382,10 @@ def bench_gravity(args):
             "step": "forward (build+walk) + recompute VJP (rebuild+adjoint walk), full "
                     "position and mass gradients",
             "batched": "not measured (single set)",
+            "launch": "HIP graph replay" if graph_ms is not None else "eager",
         },
+        "eager_ms_per_step": eager_ms,
+        "graph_ms_per_step": graph_ms,
         "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
                       "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
         "interactions": {"fwd": sf, "vjp": sv, "fwd_walk_tflops":
@@ -337,11 +400,71 @@ def bench_gravity(args):
                                  if args.order == 2 and n == 1 << 20 else None),
                      "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
                      "flops_per_launch": flops_vjp},
+        "other_rooflines": other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct),
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
         "cpu_numpy_direct": cpu_np,
+        "cpu_reference_literal": cpu_lit,
     }
     return rec
+
+
+def build_bytes(n, nodes, vjp):
+    """Algorithmic (minimal) HBM bytes of one tree build of n points with
+    `nodes` nodes: read position + mass (16 B), write the sorted record
+    (16 B), permutation (4 B) and key (4 B) per point, and 48 B of node
+    record (topology + monopole) per node; the adjoint build also reads and
+    writes the cotangent (12 + 16 B per point)."""
+    return (40 + (28 if vjp else 0)) * n + 48 * nodes
+
+
+def build_bytes_design(n, nodes, vjp):
+    """HBM bytes the build as designed must move: bounds (read 12 B/pt),
+    Morton codes (read 12, write 8), 4-pass radix sort (per pass: histogram
+    reads 4, scatter reads 8 and writes 8), gather (read perm 4 + position 12
+    + mass 4 + code 4, write sorted record 16 + code 4; adjoint +12 read +16
+    written), leaf moments (read 16), and 84 B written per node (topology 16,
+    geometry 16, parent 4, monopole 16, second moment 32)."""
+    per_point = 12 + 20 + 4 * 20 + 44 + 16 + (28 if vjp else 0)
+    return per_point * n + 84 * nodes
+
+
+def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct):
+    """Build (HBM-bound), forward walk (FP32) and direct sum (FP32) against the
+    chip's peaks; the VJP walk's HBM rate from the profiled traffic."""
+    fwd_build_ms = prof[1] / max(prof[0], 1)
+    vjp_build_ms = prof[4] / max(prof[3], 1)
+    vjp_walk_ms = prof[5] / max(prof[3], 1)
+    out = {}
+    for tag, ms, vjp in (("tree_build_fwd", fwd_build_ms, False),
+                         ("tree_build_vjp", vjp_build_ms, True)):
+        b = build_bytes(n, sf["nodes"], vjp)
+        gbs = b / (ms * 1e-3) / 1e9
+        out[tag] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "bytes": b, "ms": ms,
+                    "design_bytes": build_bytes_design(n, sf["nodes"], vjp),
+                    "note": "about 70 dependent launches per build; launch latency, not "
+                            "bandwidth, sets the time"}
+    t = flops_fwd / (walk_fwd_ms * 1e-3) / 1e12
+    out["walk_fwd"] = {"bound": "mfma", "achieved": t, "peak": FP32_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": t / FP32_PEAK_TFLOPS, "ms": walk_fwd_ms}
+    # walks against HBM: profiled traffic over the kernel time, beside the
+    # minimal bytes (40 B/pt + 48 B/node; the adjoint +36 B/pt)
+    for tag, key, ms, extra in (("walk_fwd_hbm", "k_walk<false,true,true>", walk_fwd_ms, 0),
+                                ("walk_vjp_hbm", "k_walk<true,true,false>", vjp_walk_ms, 36)):
+        traffic = profiled_traffic(key)
+        if traffic is None or args.order != 2 or n != 1 << 20:
+            continue
+        gbs = traffic / (ms * 1e-3) / 1e9
+        out[tag] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                    "minimal_bytes": (40 + extra) * n + 48 * sf["nodes"]}
+    if direct is not None:
+        t = direct["direct_fwd_tflops"]
+        out["direct_sum_fwd"] = {"bound": "mfma", "compute": "fp32 VALU, LDS-tiled",
+                                 "achieved": t, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                 "frac": t / FP32_PEAK_TFLOPS, "flops_per_pair": 20}
+    return out
 
 FLOPS_WN = 20  # per lane interaction (element or node dipole), fp32
 
@@ -509,6 +632,115 @@ def bench_wn(args):
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "flops_per_launch": flops},
         "cpu_baseline": cpu,
+    }
+
+
+def bench_stochastic(args):
+    """Classic and stochastic BH gravity on the reference's dense octree8 at
+    the headline's Plummer sphere (the octree8 handlers of src/sbh_ffi.cu:
+    3183-3248 and 3375-3443): one classic launch, one stochastic launch with
+    one sample per subdomain, and the unbiasedness check over `--seeds` seeds
+    on a query sample, against classic BH and the FP64 direct sum."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import ffi, synthetic
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n, depth = args.n, args.o8_depth
+    p = synthetic.plummer(n, 2000)
+    m = np.full(n, 1.0 / n, np.float32)
+    P = torch.tensor(p, device=dev)
+    M = torch.tensor(m, device=dev)
+    Z = torch.zeros_like(P)
+    soft = dict(softening_scale=args.eps, cutoff_scale=float("inf"), prune_enabled=False)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return r, e0.elapsed_time(e1) / reps
+
+    (leaf, node), build_ms = timed(lambda: ffi.build_octree8_buffers(P, Z, M, max_depth=depth),
+                                   args.steps)
+    cl, cl_ms = timed(lambda: ffi.softened_barnes_hut_force_octree8(
+        leaf, node, P, beta=args.beta, **soft), max(1, args.steps // 2))
+    st, st_ms = timed(lambda: ffi.softened_stochastic_barnes_hut_force_octree8(
+        leaf, node, P, beta=args.beta, samples_per_subdomain=1, seed=1, **soft), args.steps)
+
+    # unbiasedness: `seeds` independent estimates on a query sample
+    rng = np.random.default_rng(7)
+    idx = torch.tensor(np.sort(rng.choice(n, args.stoch_queries, replace=False)), device=dev)
+    Qs = P[idx].contiguous()
+    runs = torch.stack([ffi.softened_stochastic_barnes_hut_force_octree8(
+        leaf, node, Qs, beta=args.beta, samples_per_subdomain=1, seed=s, **soft).double()
+        for s in range(args.seeds)])
+    mean = runs.mean(0)
+    se = runs.std(0) / args.seeds ** 0.5
+    ref = direct_fp64(P.double(), M.double(), Qs.double(), args.eps)
+    clq = cl[idx].double()
+
+    def ztest(target):
+        z = ((mean - target) / se.clamp(min=1e-30)).flatten()
+        # per component z, and the l2 norm of (mean - target) over that of the
+        # standard error (about 1 for an unbiased estimate of `target`)
+        return {"frac_abs_z_lt_3": float((z.abs() < 3).double().mean()),
+                "mean_z2": float((z * z).mean()), "max_abs_z": float(z.abs().max()),
+                "l2_dev_over_l2_se": float(torch.linalg.norm(mean - target) /
+                                           torch.linalg.norm(se))}
+
+    acc = {"classic_rel_l2_vs_fp64_direct": rel_l2(clq, ref),
+           "stochastic_one_seed_rel_l2_vs_fp64_direct": rel_l2(st[idx], ref),
+           "stochastic_mean_rel_l2_vs_fp64_direct": rel_l2(mean, ref),
+           "unbiasedness_vs_classic_bh": ztest(clq),
+           "unbiasedness_vs_fp64_direct": ztest(ref),
+           "seeds": args.seeds, "sample_queries": args.stoch_queries}
+
+    cpu = cpu_ref = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        lb, nb = leaf.cpu().numpy(), node.cpu().numpy()
+        ns = 4096
+        qs = p[rng.choice(n, ns, replace=False)]
+        t0 = time.perf_counter()
+        O.sbh_octree8(lb, nb, qs, O.KIND_SOFT, 1, 1, args.beta, eps=args.eps, rc=float("inf"))
+        t = (time.perf_counter() - t0) / ns
+        cpu = {"value": 1.0 / t, "unit": "forces/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"oracle octree8 stochastic walk (same sample paths, fp64, OpenMP "
+                         f"over queries) on {ns} of {n} queries, scaled"}
+        T = O.CpuTree(p, m, leaf_size=args.leaf, dim_split_log=1, max_depth=10)
+        ns = 1 << 16
+        t0 = time.perf_counter()
+        T.mlpcv(p[:ns], O.CPU_SOFT_FORCE, samples=1, seed=1, eps=args.eps)
+        t = (time.perf_counter() - t0) / ns
+        cpu_ref = {"value": 1.0 / t, "unit": "forces/s (stochastic, 1 sample)",
+                   "threads": O.num_threads(),
+                   "variant": f"reference CPU MLPCV (src/sbh.cpp) restated over Tree3D<"
+                              f"{args.leaf},1>, {ns} queries, tree build excluded"}
+    return {
+        "metric": "stochastic BH forces/sec at N=1M (octree8, 1 sample per subdomain)",
+        "value": n / (st_ms * 1e-3),
+        "unit": "forces/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": st_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plummer_sphere_n{n}_octree8_depth{depth}_eps{args.eps}"
+                               f"_beta{args.beta}_stochastic_s1",
+                   "n": n, "max_depth": depth, "beta": args.beta, "softening": args.eps,
+                   "samples_per_subdomain": 1},
+        "phases_ms": {"octree8_build": build_ms, "classic_walk": cl_ms,
+                      "stochastic_walk": st_ms},
+        "accuracy": acc,
+        "cpu_baseline": cpu,
+        "cpu_reference_path": cpu_ref,
     }
 
 

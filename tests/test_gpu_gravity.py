@@ -208,3 +208,39 @@ def test_determinism_of_forces(gpu):
     a = _np(G.gravity_force(p, m, softening_scale=0.005))
     b = _np(G.gravity_force(p, m, softening_scale=0.005))
     np.testing.assert_array_equal(a, b)
+
+
+def test_graph_replay_single_set_matches_eager(gpu):
+    # one set through the single-set entry points, captured once and replayed
+    # with unrelated eager work in between (the bench's timed step)
+    from sbh_amd.graph import Graph
+
+    G = _G()
+    n = 50000
+    p, m, c = _case("plummer", n, 12)
+    P, M, C = (torch.tensor(a, device="cuda") for a in (p, m, c))
+    kw = dict(softening_scale=0.005, max_depth=10, leaf_size=32, multipole_order=2)
+    ws = G.new_workspace(n, 10, 32)
+    Fe = G.gravity_force(P, M, workspace=ws, **kw).clone()
+    gxe, gme = (t.clone() for t in G.gravity_force_vjp(P, M, C, workspace=ws, **kw))
+    F, gx, gm = torch.empty_like(P), torch.empty_like(P), torch.empty_like(M)
+
+    def step():
+        G.gravity_force(P, M, workspace=ws, out=F, **kw)
+        G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=gx, grad_masses_out=gm,
+                            **kw)
+
+    g = Graph()
+    g.capture(step)
+    for r in range(3):
+        F.zero_()
+        gx.zero_()
+        gm.zero_()
+        tmp = torch.randn(1 << 16, device="cuda").sum()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.isfinite(tmp)
+        assert torch.equal(F, Fe), r
+        torch.testing.assert_close(gx, gxe, rtol=1e-4, atol=1e-5 * float(gxe.abs().max()))
+        torch.testing.assert_close(gm, gme, rtol=1e-4, atol=1e-5 * float(gme.abs().max()))
+    g.close()
