@@ -510,7 +510,9 @@ __global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, 
     }
     mp[k] = make_float4(X, Y, Z, M);
     qd[2 * k] = make_float4(sxx, syy, szz, sxy);
-    qd[2 * k + 1] = make_float4(sxz, syz, 1.5f * (sxx + syy + szz), 0.f);  // + 1.5 tr(S)
+    // + 1.5 tr(S) and -2.5 tr(S), constants of the walk's far-field terms
+    const float tr = sxx + syy + szz;
+    qd[2 * k + 1] = make_float4(sxz, syz, 1.5f * tr, -2.5f * tr);
   }
 }
 
@@ -629,11 +631,15 @@ __device__ __forceinline__ void butterfly_swap16(float (&v)[N]) {
 // the 64-lane sum of v[lane >> (6 - k)] (each value is held by 64/V lanes).
 template <int V>
 __device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
-  static_assert(V == 16 || V == 32 || V == 64, "unsupported width");
+  static_assert(V == 8 || V == 16 || V == 32 || V == 64, "unsupported width");
   butterfly_swap32<V / 2>(v);
   butterfly_swap16<V / 4>(v);
   butterfly<V / 8, V, 8, kRowRor8>(v, lane);
-  butterfly<V / 16, V, 4, kHalfMirror>(v, lane);
+  if constexpr (V >= 16) {
+    butterfly<V / 16, V, 4, kHalfMirror>(v, lane);
+  } else {
+    v[0] += dpp_partner<kHalfMirror>(v[0]);
+  }
   if constexpr (V >= 32) {
     butterfly<V / 32, V, 2, kQuadXor2>(v, lane);
   } else {
@@ -706,7 +712,10 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
 #pragma unroll
   for (int t = 0; t < C; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
   float part[4 * C];   // minus the node-side partials (negated in the scatter)
-  float qpart[8 * C];
+  // quadrupole adjoint, 10 values per node packed as 8 + 2: per node
+  // (dL/dS xx yy zz xy xz yz, minus dL/dM, minus dL/dX x) and (minus dL/dX y, z)
+  float q8[8 * C];
+  float q2[2 * C];
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     const float4 m = sload4(A.mp, idx[t]);
@@ -737,14 +746,15 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
     } else {
       const float4 q0 = sload4(A.qd, 2 * idx[t]);
       const float4 q1 = sload4(A.qd, 2 * idx[t] + 1);
-      const float s5 = s3 * s2, s7 = s5 * s2;
-      // second moment (xx, yy, zz, xy), (xz, yz)
+      // powers of s through w = s^2; second moment (xx, yy, zz, xy), (xz, yz)
+      const float w = s2, s5 = s3 * w;
       const float SDx = q0.x * Dx + q0.w * Dy + q1.x * Dz;
       const float SDy = q0.w * Dx + q0.y * Dy + q1.y * Dz;
       const float SDz = q1.x * Dx + q1.y * Dy + q0.z * Dz;
       const float tq = Dx * SDx + Dy * SDy + Dz * SDz;
-      const float T15 = q1.z;  // 1.5 tr(S), stored with the moments
-      const float a = -m.w * s3 + T15 * s5 - 7.5f * s7 * tq;
+      const float wtq = w * tq;
+      // a = -M s^3 + 1.5 tr(S) s^5 - 7.5 s^7 t,  b = 3 s^5
+      const float a = s3 * fmaf(w, fmaf(-7.5f, wtq, q1.z), -m.w);
       const float b = 3.0f * s5;
       if constexpr (FORCES) {
         fx += a * Dx + b * SDx;
@@ -752,49 +762,72 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         fz += a * Dz + b * SDz;
       }
       if constexpr (VJP) {
-        const float s9 = s7 * s2;
         const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
         const float Scx = q0.x * L.cx + q0.w * L.cy + q1.x * L.cz;
         const float Scy = q0.w * L.cx + q0.y * L.cy + q1.y * L.cz;
         const float Scz = q1.x * L.cx + q1.y * L.cy + q0.z * L.cz;
-        const float cw = L.cx * SDx + L.cy * SDy + L.cz * SDz;
-        const float kD = 3.0f * m.w * s5 * u - 15.0f * s7 * cw - 5.0f * T15 * s7 * u +
-                         52.5f * s9 * tq * u;
-        const float kc = a;  // -M s^3 + 1.5 tr(S) s^5 - 7.5 s^7 t
-        const float b15 = -15.0f * s7 * u;
-        const float dDx = kD * Dx + kc * L.cx + b * Scx + b15 * SDx;
-        const float dDy = kD * Dy + kc * L.cy + b * Scy + b15 * SDy;
-        const float dDz = kD * Dz + kc * L.cz + b * Scz + b15 * SDz;
+        const float cw = Scx * Dx + Scy * Dy + Scz * Dz;  // c . S D
+        // d(c.F)/dD = kD D + a c + b S c + b15 S D with
+        //   kD = u (3 M s^5 - 5 (1.5 tr S) s^7 + 52.5 s^9 t) - 15 s^7 (c.S D),
+        //   b15 = -15 s^7 u; the bracket is b (M + w (17.5 w t - 2.5 tr S))
+        const float A1 = b * fmaf(w, fmaf(17.5f, wtq, q1.w), m.w);
+        const float m15s7 = -15.0f * (s5 * w);
+        const float kD = fmaf(m15s7, cw, u * A1);
+        const float b15 = m15s7 * u;
+        const float dDx = kD * Dx + a * L.cx + b * Scx + b15 * SDx;
+        const float dDy = kD * Dy + a * L.cy + b * Scy + b15 * SDy;
+        const float dDz = kD * Dz + a * L.cz + b * Scz + b15 * SDz;
         gx += dDx;
         gy += dDy;
         gz += dDz;
-        part[4 * t + 0] = dDx;
-        part[4 * t + 1] = dDy;
-        part[4 * t + 2] = dDz;
-        part[4 * t + 3] = s3 * u;
-        // dL/dS (symmetric), (xx, yy, zz, xy, xz, yz):
-        //   G = h5 (c D^T + D c^T) + h7 D D^T + h5 u I,  h5 = 1.5 s^5, h7 = -7.5 s^7 u
-        //     = P D^T + D P^T + h5 u I  with  P = h5 c + (h7 / 2) D
-        const float h5 = 1.5f * s5, hh7 = -3.75f * s7 * u, hu = h5 * u;
-        const float Px = fmaf(hh7, Dx, h5 * L.cx);
-        const float Py = fmaf(hh7, Dy, h5 * L.cy);
-        const float Pz = fmaf(hh7, Dz, h5 * L.cz);
-        qpart[8 * t + 0] = fmaf(2.0f * Px, Dx, hu);
-        qpart[8 * t + 1] = fmaf(2.0f * Py, Dy, hu);
-        qpart[8 * t + 2] = fmaf(2.0f * Pz, Dz, hu);
-        qpart[8 * t + 3] = fmaf(Px, Dy, Dx * Py);
-        qpart[8 * t + 4] = fmaf(Px, Dz, Dx * Pz);
-        qpart[8 * t + 5] = fmaf(Py, Dz, Dy * Pz);
-        qpart[8 * t + 6] = 0.f;
-        qpart[8 * t + 7] = 0.f;
+        q8[8 * t + 6] = s3 * u;
+        q8[8 * t + 7] = dDx;
+        q2[2 * t + 0] = dDy;
+        q2[2 * t + 1] = dDz;
+        // dL/dS (symmetric) = P D^T + D P^T + 1.5 s^5 u I with
+        //   P = 1.5 s^5 c - 3.75 s^7 u D = (b c + (b15 / 2) D) / 2 =: P2 / 2;
+        //   diagonal: P2_a D_a + 1.5 s^5 u, off-diagonal stored doubled:
+        //   P2_a D_b + D_a P2_b (halved in k_finalize)
+        const float hb15 = 0.5f * b15, hu = (0.5f * b) * u;
+        const float Px = fmaf(hb15, Dx, b * L.cx);
+        const float Py = fmaf(hb15, Dy, b * L.cy);
+        const float Pz = fmaf(hb15, Dz, b * L.cz);
+        q8[8 * t + 0] = fmaf(Px, Dx, hu);
+        q8[8 * t + 1] = fmaf(Py, Dy, hu);
+        q8[8 * t + 2] = fmaf(Pz, Dz, hu);
+        q8[8 * t + 3] = fmaf(Px, Dy, Dx * Py);
+        q8[8 * t + 4] = fmaf(Px, Dz, Dx * Pz);
+        q8[8 * t + 5] = fmaf(Py, Dz, Dy * Pz);
       }
     }
   }
-  if constexpr (VJP) {
-    // node adjoints: (dL/dX, dL/dM) and the six dL/dS entries
+  if constexpr (VJP && !QUAD) {
+    // node adjoints (dL/dX, dL/dM)
     scatter_batch<4 * C, 4, 4, true>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
-    if constexpr (QUAD)
-      scatter_batch<8 * C, 8, 6, false>(qpart, lane, A.adjq, 8, far, A.sentinel, 0, INT_MAX);
+  }
+  if constexpr (VJP && QUAD) {
+    static_assert(C == 4, "the packed quadrupole adjoint assumes 4 nodes per batch");
+    {
+      // q8: value j = lane >> 1 is slot j & 7 of node j >> 3
+      const float sum = wave_transpose_sum<8 * C>(q8, lane);
+      const int j = lane >> 1, slot = j & 7;
+      if ((lane & 1) == 0) {
+        const int e = far[j >> 3];
+        if (e != A.sentinel) {
+          float* dst = slot < 6 ? A.adjq + 8 * e + slot : A.adj + 4 * e + (slot == 6 ? 3 : 0);
+          atomicAdd(dst, slot < 6 ? sum : -sum);
+        }
+      }
+    }
+    {
+      // q2: value j = lane >> 3 is component 1 + (j & 1) of node j >> 1's dL/dX
+      const float sum = wave_transpose_sum<2 * C>(q2, lane);
+      const int j = lane >> 3;
+      if ((lane & 7) == 0) {
+        const int e = far[j >> 1];
+        if (e != A.sentinel) atomicAdd(A.adj + 4 * e + 1 + (j & 1), -sum);
+      }
+    }
   }
 }
 
@@ -815,8 +848,8 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
         __builtin_amdgcn_readfirstlane(__float_as_uint(src.w) & (j < s1 ? ~0u : 0u)));
     const float dx = L.x - src.x, dy = L.y - src.y, dz = L.z - src.z;
     const float s = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, A.e2))));
-    const float s2 = s * s;
-    float ws3 = (w * s) * s2;  // w s^3
+    const float s2 = s * s, s3 = s * s2;
+    float ws3 = w * s3;  // w s^3
     // the self pair: its force term vanishes with d = 0, its adjoint terms do not
     if constexpr (VJP) ws3 = (j == i) ? 0.0f : ws3;
     if constexpr (FORCES) {
@@ -826,17 +859,18 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
     }
     if constexpr (VJP) {
       const float cd = L.cx * dx + L.cy * dy + L.cz * dz;
-      const float k1 = 3.0f * ws3 * s2 * cd;  // 3 w s^5 (c.d)
-      const float vx = k1 * dx - ws3 * L.cx;
-      const float vy = k1 * dy - ws3 * L.cy;
-      const float vz = k1 * dz - ws3 * L.cz;
+      // v = 3 w s^5 (c.d) d - w s^3 c = w s^3 (3 s^2 (c.d) d - c)
+      const float k = (3.0f * s2) * cd;
+      const float vx = ws3 * fmaf(k, dx, -L.cx);
+      const float vy = ws3 * fmaf(k, dy, -L.cy);
+      const float vz = ws3 * fmaf(k, dz, -L.cz);
       gx += vx;
       gy += vy;
       gz += vz;
       part[4 * u + 0] = vx;
       part[4 * u + 1] = vy;
       part[4 * u + 2] = vz;
-      part[4 * u + 3] = ((j == i) ? 0.0f : s * s2) * cd;  // s^3 (c.d)
+      part[4 * u + 3] = s3 * cd;  // s^3 (c.d); exactly 0 for the self pair (d = 0)
     }
   }
   if constexpr (VJP)
@@ -915,8 +949,11 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 
 // Persistent walk: each wave takes query groups from a device work queue
 // until none are left (heavy groups do not hold up a static schedule).
+// Registers are capped at 128 VGPRs (4 waves per SIMD): the adjoint variant
+// would take ~143 and drop to 3 waves; the cap costs a few spills outside
+// the interaction batches and is ~4% faster.
 template <bool VJP, bool QUAD, bool FORCES>
-__global__ __launch_bounds__(kB) void k_walk(WalkArgs A) {
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(WalkArgs A) {
   __shared__ int s_stack[kWavesPerBlock][kStackCap];
   __shared__ int s_far[kWavesPerBlock][kFarCap];
   __shared__ int s_near[kWavesPerBlock][kNearCap];
@@ -998,10 +1035,12 @@ __global__ __launch_bounds__(kB) void k_finalize(const Header* __restrict__ hdr,
       az += wm * Az;
       am += B + (Ax * dx + Ay * dy + Az * dz) / m.w;
       if constexpr (QUAD) {
-        const float* G = adjq + 8 * node;  // xx yy zz xy xz yz
-        const float Gdx = G[0] * dx + G[3] * dy + G[4] * dz;
-        const float Gdy = G[3] * dx + G[1] * dy + G[5] * dz;
-        const float Gdz = G[4] * dx + G[5] * dy + G[2] * dz;
+        // xx yy zz, then xy xz yz stored doubled by the walk
+        const float* G = adjq + 8 * node;
+        const float Gxy = 0.5f * G[3], Gxz = 0.5f * G[4], Gyz = 0.5f * G[5];
+        const float Gdx = G[0] * dx + Gxy * dy + Gxz * dz;
+        const float Gdy = Gxy * dx + G[1] * dy + Gyz * dz;
+        const float Gdz = Gxz * dx + Gyz * dy + G[2] * dz;
         ax += 2.0f * p.w * Gdx;
         ay += 2.0f * p.w * Gdy;
         az += 2.0f * p.w * Gdz;
