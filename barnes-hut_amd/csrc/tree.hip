@@ -725,9 +725,9 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
     if constexpr (!QUAD) {
       const float a = -m.w * s3;
       if constexpr (FORCES) {
-        fx += a * Dx;
-        fy += a * Dy;
-        fz += a * Dz;
+        fx = fmaf(a, Dx, fx);
+        fy = fmaf(a, Dy, fy);
+        fz = fmaf(a, Dz, fz);
       }
       if constexpr (VJP) {
         const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
@@ -757,9 +757,9 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       const float a = s3 * fmaf(w, fmaf(-7.5f, wtq, q1.z), -m.w);
       const float b = 3.0f * s5;
       if constexpr (FORCES) {
-        fx += a * Dx + b * SDx;
-        fy += a * Dy + b * SDy;
-        fz += a * Dz + b * SDz;
+        fx = fmaf(a, Dx, fmaf(b, SDx, fx));
+        fy = fmaf(a, Dy, fmaf(b, SDy, fy));
+        fz = fmaf(a, Dz, fmaf(b, SDz, fz));
       }
       if constexpr (VJP) {
         const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
@@ -919,13 +919,24 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
     __builtin_amdgcn_wave_barrier();
   };
 
+  // Near leaves whose points directly follow each other in sorted order
+  // (sibling leaves, appended in order by the traversal) are merged into one
+  // source range, so only a range end pads its last batch of kChunk.
   auto flush_near = [&](int nnear) {
-    for (int b = 0; b < nnear; ++b) {
-      const int leaf = __builtin_amdgcn_readfirstlane(near[b]);
-      const int4 t = sload4(A.topo, leaf);
-      const int s0 = t.x, s1 = t.x + t.y;
+    int b = 0;
+    while (b < nnear) {
+      const int4 t = sload4(A.topo, __builtin_amdgcn_readfirstlane(near[b++]));
+      const int s0 = t.x;
+      int s1 = t.x + t.y;
+      while (b < nnear) {
+        const int4 tn = sload4(A.topo, __builtin_amdgcn_readfirstlane(near[b]));
+        if (tn.x != s1) break;
+        s1 += tn.y;
+        ++b;
+      }
+      s1 = __builtin_amdgcn_readfirstlane(s1);  // wave-uniform: keeps source loads scalar
       // self pairs are skipped
-      n_near += static_cast<unsigned long long>(t.y) * nvalid -
+      n_near += static_cast<unsigned long long>(s1 - s0) * nvalid -
                 static_cast<unsigned long long>(max(0, min(s1, base + nvalid) - max(s0, base)));
       for (int j0 = s0; j0 < s1; j0 += kChunk)
         near_batch<VJP, FORCES>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);

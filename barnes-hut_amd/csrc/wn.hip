@@ -182,13 +182,22 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
       for (int f0 = 0; f0 < padded; f0 += C) w += wn_far_batch<C>(A, q.x, q.y, q.z, far + f0);
       __builtin_amdgcn_wave_barrier();
     };
+    // consecutive near leaves with adjacent element ranges are merged
     auto flush_near = [&](int nnear) {
-      for (int b = 0; b < nnear; ++b) {
-        const int leaf = __builtin_amdgcn_readfirstlane(near[b]);
-        const int4 t = sload4(A.topo, leaf);
-        n_near += static_cast<unsigned long long>(t.y) * cnt;
-        for (int j0 = t.x; j0 < t.x + t.y; j0 += kChunk)
-          w += wn_near_batch(A, q.x, q.y, q.z, j0, t.x + t.y);
+      int b = 0;
+      while (b < nnear) {
+        const int4 t = sload4(A.topo, __builtin_amdgcn_readfirstlane(near[b++]));
+        const int s0 = t.x;
+        int s1 = t.x + t.y;
+        while (b < nnear) {
+          const int4 tn = sload4(A.topo, __builtin_amdgcn_readfirstlane(near[b]));
+          if (tn.x != s1) break;
+          s1 += tn.y;
+          ++b;
+        }
+        s1 = __builtin_amdgcn_readfirstlane(s1);  // wave-uniform: keeps element loads scalar
+        n_near += static_cast<unsigned long long>(s1 - s0) * cnt;
+        for (int j0 = s0; j0 < s1; j0 += kChunk) w += wn_near_batch(A, q.x, q.y, q.z, j0, s1);
       }
     };
     traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack, far, near, flush_far,
