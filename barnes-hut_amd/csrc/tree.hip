@@ -631,10 +631,14 @@ __device__ __forceinline__ void butterfly_swap16(float (&v)[N]) {
 // the 64-lane sum of v[lane >> (6 - k)] (each value is held by 64/V lanes).
 template <int V>
 __device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
-  static_assert(V == 8 || V == 16 || V == 32 || V == 64, "unsupported width");
+  static_assert(V == 4 || V == 8 || V == 16 || V == 32 || V == 64, "unsupported width");
   butterfly_swap32<V / 2>(v);
   butterfly_swap16<V / 4>(v);
-  butterfly<V / 8, V, 8, kRowRor8>(v, lane);
+  if constexpr (V >= 8) {
+    butterfly<V / 8, V, 8, kRowRor8>(v, lane);
+  } else {
+    v[0] += dpp_partner<kRowRor8>(v[0]);
+  }
   if constexpr (V >= 16) {
     butterfly<V / 16, V, 4, kHalfMirror>(v, lane);
   } else {
@@ -831,19 +835,19 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   }
 }
 
-// Near-field batch: kChunk consecutive sources of one leaf starting at j0
-// (entries at or beyond s1 carry zero weight).
-template <bool VJP, bool FORCES>
+// Near-field batch: CH consecutive sources starting at j0 (entries at or
+// beyond s1 carry zero weight).
+template <bool VJP, bool FORCES, int CH>
 __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
                                            int s1, int lane, float& fx, float& fy, float& fz,
                                            float& gx, float& gy, float& gz) {
-  float part[4 * kChunk];  // minus the source-side partials (negated in the scatter)
+  float part[4 * CH];  // minus the source-side partials (negated in the scatter)
 #pragma unroll
-  for (int u = 0; u < kChunk; ++u) {
+  for (int u = 0; u < CH; ++u) {
     const int j = j0 + u;
     const float4 src = sload4(A.sp, min(j, s1 - 1));
-    // zero weight past the leaf end; j, s1 and the record are wave-uniform, so
-    // the mask stays on the scalar unit
+    // zero weight past the range end; j, s1 and the record are wave-uniform,
+    // so the mask stays on the scalar unit
     const float w = __uint_as_float(
         __builtin_amdgcn_readfirstlane(__float_as_uint(src.w) & (j < s1 ? ~0u : 0u)));
     const float dx = L.x - src.x, dy = L.y - src.y, dz = L.z - src.z;
@@ -874,7 +878,7 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
     }
   }
   if constexpr (VJP)
-    scatter_batch<4 * kChunk, 4, 4, true>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
+    scatter_batch<4 * CH, 4, 4, true>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
@@ -938,8 +942,26 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       // self pairs are skipped
       n_near += static_cast<unsigned long long>(s1 - s0) * nvalid -
                 static_cast<unsigned long long>(max(0, min(s1, base + nvalid) - max(s0, base)));
-      for (int j0 = s0; j0 < s1; j0 += kChunk)
-        near_batch<VJP, FORCES>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+      // batches of 8, then in the forward walk the rest exactly (4, 2, 1); the
+      // adjoint pads its last batch of 8 (tail variants there push the kernel
+      // past its register budget and cost more than the padding saves)
+      if constexpr (VJP) {
+        for (int j0 = s0; j0 < s1; j0 += kChunk)
+          near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+      } else {
+        int j0 = s0;
+        for (; j0 + kChunk <= s1; j0 += kChunk)
+          near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+        if (j0 + 4 <= s1) {
+          near_batch<VJP, FORCES, 4>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+          j0 += 4;
+        }
+        if (j0 + 2 <= s1) {
+          near_batch<VJP, FORCES, 2>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+          j0 += 2;
+        }
+        if (j0 < s1) near_batch<VJP, FORCES, 1>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+      }
     }
   };
 
