@@ -4,37 +4,43 @@
 #include <vector>
 
 #include "common.hpp"
+#include "profile.hpp"
 #include "tree.hpp"
 
 using namespace sbh;
 
+namespace sbh {
+namespace prof {
 namespace {
+std::mutex g_mu;
+bool g_on = false;
+std::vector<Rec> g_recs;
+}  // namespace
 
-// Optional phase timing: events recorded on the call's stream around the
-// build and around the walk kernel; read back by sbh_profile_collect().
-struct ProfRec {
-  hipEvent_t ev[4];  // build start, build end, walk kernel start, walk kernel end
-  int vjp;
-};
-std::mutex g_prof_mu;
-bool g_prof_on = false;
-std::vector<ProfRec> g_prof;
-
-bool prof_begin(hipStream_t s, ProfRec* r, int vjp) {
+bool begin(hipStream_t s, Rec* r, int kind) {
   {
-    std::lock_guard<std::mutex> lock(g_prof_mu);
-    if (!g_prof_on) return false;
+    std::lock_guard<std::mutex> lock(g_mu);
+    if (!g_on) return false;
   }
   for (int k = 0; k < 4; ++k)
     if (hipEventCreate(&r->ev[k]) != hipSuccess) return false;
-  r->vjp = vjp;
+  r->kind = kind;
   return hipEventRecord(r->ev[0], s) == hipSuccess;
 }
 
-void prof_end(const ProfRec& r) {
-  std::lock_guard<std::mutex> lock(g_prof_mu);
-  g_prof.push_back(r);
+void end(const Rec& r) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  g_recs.push_back(r);
 }
+
+}  // namespace prof
+}  // namespace sbh
+
+namespace {
+
+using ProfRec = prof::Rec;
+inline bool prof_begin(hipStream_t s, ProfRec* r, int kind) { return prof::begin(s, r, kind); }
+inline void prof_end(const ProfRec& r) { prof::end(r); }
 
 int check_common(const SbhBuffer* points, const SbhBuffer* masses, float beta, float softening,
                  int max_depth, int leaf_size, int order) {
@@ -176,21 +182,21 @@ extern "C" int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
 }
 
 extern "C" void sbh_profile_enable(int32_t on) {
-  std::lock_guard<std::mutex> lock(g_prof_mu);
-  g_prof_on = on != 0;
+  std::lock_guard<std::mutex> lock(prof::g_mu);
+  prof::g_on = on != 0;
 }
 
-// out[0..5] = {forward calls, forward build ms, forward walk-kernel ms,
-//              vjp calls, vjp build ms, vjp walk-kernel ms} summed over the
+// out[3 k .. 3 k + 2] = {calls, build ms, walk-kernel ms} of kind k (0
+// gravity forward, 1 gravity adjoint, 2 winding numbers) summed over the
 // calls recorded since the last collect.  Waits for the recorded events.
 extern "C" int sbh_profile_collect(double* out, int32_t count) {
   clear_error();
   std::vector<ProfRec> recs;
   {
-    std::lock_guard<std::mutex> lock(g_prof_mu);
-    recs.swap(g_prof);
+    std::lock_guard<std::mutex> lock(prof::g_mu);
+    recs.swap(prof::g_recs);
   }
-  double acc[6] = {0, 0, 0, 0, 0, 0};
+  double acc[3 * prof::kKinds] = {};
   int rc = SBH_OK;
   for (auto& r : recs) {
     float b = 0.f, w = 0.f;
@@ -198,13 +204,13 @@ extern "C" int sbh_profile_collect(double* out, int32_t count) {
         hipEventElapsedTime(&b, r.ev[0], r.ev[1]) != hipSuccess ||
         hipEventElapsedTime(&w, r.ev[2], r.ev[3]) != hipSuccess)
       rc = fail_internal_msg("profile event query failed");
-    int o = r.vjp ? 3 : 0;
+    const int o = 3 * r.kind;
     acc[o] += 1.0;
     acc[o + 1] += b;
     acc[o + 2] += w;
     for (int k = 0; k < 4; ++k) (void)hipEventDestroy(r.ev[k]);
   }
-  for (int i = 0; i < count && i < 6; ++i) out[i] = acc[i];
+  for (int i = 0; i < count && i < 3 * prof::kKinds; ++i) out[i] = acc[i];
   return rc;
 }
 
@@ -251,14 +257,18 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  ProfRec pr;
+  bool prof = prof_begin(s, &pr, 0);
   rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
                           static_cast<const float*>(masses->data), nullptr, n,
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
+  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
-                  static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr, nullptr,
-                  walk_blocks);
+                  static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
+                  prof && n > 0 ? pr.ev + 2 : nullptr, walk_blocks);
   if (rc != SBH_OK) return rc;
+  if (prof && n > 0) prof_end(pr);
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }
@@ -295,16 +305,20 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
+  ProfRec pr;
+  bool prof = prof_begin(s, &pr, 1);
   rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
                           static_cast<const float*>(masses->data),
                           static_cast<const float*>(cotangent->data), n,
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
+  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
-                  stats != nullptr, nullptr, walk_blocks);
+                  stats != nullptr, prof && n > 0 ? pr.ev + 2 : nullptr, walk_blocks);
   if (rc != SBH_OK) return rc;
+  if (prof && n > 0) prof_end(pr);
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }

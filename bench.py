@@ -566,6 +566,20 @@ def bench_wn(args):
     ev[1].record(stream)
     torch.cuda.synchronize()
     ms_step = ev[0].elapsed_time(ev[1]) / args.steps
+    # phase times from the library's HIP events (build = element tree,
+    # dipoles and query tree; walk = the k_wn_walk launch)
+    import sbh_amd
+
+    lib = sbh_amd.load_library()
+    lib.sbh_profile_enable(ctypes.c_int32(1))
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    lib.sbh_profile_enable(ctypes.c_int32(0))
+    prof = (ctypes.c_double * 9)()
+    lib.sbh_profile_collect(prof, ctypes.c_int32(9))
+    wn_build_ms = prof[7] / max(prof[6], 1)
+    wn_walk_ms = prof[8] / max(prof[6], 1)
     st = torch.zeros(8, dtype=torch.int64, device=dev)
     wn.winding_number(Pc, Nn, A, Q, beta=args.beta, max_depth=10, leaf_size=leaf,
                       workspace=ws, stats=st)
@@ -610,7 +624,7 @@ def bench_wn(args):
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_wn(c, nrm, a, q, leaf, m)
-    achieved = flops / (ms_step * 1e-3) / 1e12
+    achieved = flops / (wn_walk_ms * 1e-3) / 1e12
     return {
         "metric": "winding numbers/sec (classic BH, theta=0.5)",
         "value": m / (ms_step * 1e-3),
@@ -624,10 +638,11 @@ def bench_wn(args):
                    "max_depth": 10, "leaf_size": leaf,
                    "step": "element tree build + dipole moments + query tree + group walk"},
         "interactions": {"far": s[0], "near": s[1], "groups": s[2], "nodes": s[3]},
+        "phases_ms": {"build": wn_build_ms, "walk_kernel": wn_walk_ms},
         "accuracy": acc,
         "others": others,
         "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
-                     "kernel": "k_wn_walk (step time, build included)", "achieved": achieved,
+                     "kernel": "k_wn_walk", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                      "flops_per_launch": flops},
@@ -811,6 +826,17 @@ def bench_batched(args):
     g.capture(step)
     ms_graph = timed(g.replay, args.steps, args.warmup)
     ms_eager = timed(step, args.steps, args.warmup)
+    # walk-kernel times of the batched step (library HIP events, eager calls)
+    import sbh_amd
+
+    lib = sbh_amd.load_library()
+    lib.sbh_profile_enable(ctypes.c_int32(1))
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    lib.sbh_profile_enable(ctypes.c_int32(0))
+    prof = (ctypes.c_double * 6)()
+    lib.sbh_profile_collect(prof, ctypes.c_int32(6))
     # per set: 64 separate calls of the single-set pipeline on one stream
     Ps = [P[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
     Ms = [M[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
@@ -851,7 +877,23 @@ def bench_batched(args):
     G.gravity_force_vjp_batch(P, M, C, offs, stats=st, leaf_size=leaf, walk_blocks=blocks,
                               workspace=ws, **kw)
     sv = G.stats_dict(st)
+    stf = G.new_stats(dev)
+    G.gravity_force_batch(P, M, offs, stats=stf, leaf_size=leaf, walk_blocks=blocks,
+                          workspace=ws, out=F, **kw)
+    sf = G.stats_dict(stf)
     g.close()
+    walk_vjp_ms = prof[5] / max(prof[3], 1)
+    walk_fwd_ms = prof[2] / max(prof[0], 1)
+    flops_vjp = FLOPS_NEAR_VJP * sv["near"] + FLOPS_FAR_VJP[args.order] * sv["far"]
+    flops_fwd = FLOPS_NEAR_FWD * sf["near"] + FLOPS_FAR_FWD[args.order] * sf["far"]
+    achieved = flops_vjp / (walk_vjp_ms * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        big_args = argparse.Namespace(**vars(args))
+        big_args.leaf = leaf
+        cpu = cpu_baseline(ps[kbig], ms[kbig], cs[kbig], big_args, args.cpu_sample_groups)
+        cpu["sample"] = (f"largest set ({int(sizes[kbig])} points) only; its forces/s stands "
+                         f"for the batch. " + cpu["sample"])
     return {
         "metric": "batched fwd+VJP forces/sec, 64 sets (graph-captured forest vs per-set calls)",
         "value": n_total / (ms_graph * 1e-3),
@@ -875,8 +917,15 @@ def bench_batched(args):
         "accuracy": {"largest_set_force_rel_l2_vs_fp64_direct":
                      rel_l2(F[o0:o0 + int(sizes[kbig])][idx], ref),
                      "largest_set": int(sizes[kbig])},
-        "roofline": None,
-        "cpu_baseline": None,
+        "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
+                      "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
+        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
+                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0> (all sets)",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "flops_per_launch": flops_vjp,
+                     "fwd_walk_tflops": flops_fwd / (walk_fwd_ms * 1e-3) / 1e12},
+        "cpu_baseline": cpu,
     }
 
 

@@ -410,12 +410,14 @@ int sbh_graph_end(void* stream, void** graph_exec);
 int sbh_graph_launch(void* graph_exec, void* stream);
 int sbh_graph_destroy(void* graph_exec);
 
-/* Phase timing for the sbh_gravity_* targets (no reference counterpart; the
- * reference times from Python around dispatch).  When enabled, each call
- * records HIP events on its stream around the tree build and around the walk
- * kernel.  sbh_profile_collect waits for the recorded events and writes
- * {forward calls, forward build ms, forward walk ms, vjp calls, vjp build ms,
- * vjp walk ms} summed since the previous collect. */
+/* Phase timing for the sbh_gravity_* (single set and batch) and
+ * sbh_winding_number targets (no reference counterpart; the reference times
+ * from Python around dispatch).  When enabled, each call records HIP events
+ * on its stream around the tree build(s) and around the walk kernel.
+ * sbh_profile_collect waits for the recorded events and writes up to 9
+ * values, {calls, build ms, walk ms} for gravity forward, gravity adjoint and
+ * winding numbers in turn, summed since the previous collect.  Not for use
+ * while a graph is being captured. */
 void sbh_profile_enable(int32_t on);
 int sbh_profile_collect(double* out, int32_t count);
 
