@@ -60,6 +60,35 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
   return v;
 }
 
+// XCD-aware work queue of a persistent walk.  The groups (in Morton order)
+// are split into 8 contiguous eighths, one per XCD; workgroups are dispatched
+// to the 8 XCDs round-robin, so block b starts on its home eighth b mod 8 and
+// each XCD's private L2 serves a compact region of the tree (measured: the
+// forward walk 3% faster than with one global queue; chunks of 16 groups
+// dealt round-robin to the XCDs gained half of that).  A wave that finds its
+// current eighth exhausted moves on to the next one (work stealing); after 8
+// exhausted eighths it gets -1.  `tries` is the calling wave's state (used by
+// lane 0 only), starting at 0.  Each wave makes at most 8 failing claims, so
+// every wave reaches the -1 exit.
+__device__ __forceinline__ int next_group(Header* hdr, int num_groups, int lane, int& tries) {
+  int g = -1;
+  if (lane == 0) {
+    const int home = static_cast<int>(blockIdx.x & 7u);
+    while (tries < 8) {
+      const int q = (home + tries) & 7;
+      const int lo = static_cast<int>(static_cast<int64_t>(num_groups) * q / 8);
+      const int hi = static_cast<int>(static_cast<int64_t>(num_groups) * (q + 1) / 8);
+      const int t = lo + atomicAdd(&hdr->xq[q], 1);
+      if (t < hi) {
+        g = t;
+        break;
+      }
+      ++tries;
+    }
+  }
+  return __builtin_amdgcn_readfirstlane(g);
+}
+
 // Group MAC, fp32 without contraction (identical expression in the oracle).
 __device__ __forceinline__ bool group_accept(float lx, float ly, float lz, float hx, float hy,
                                              float hz, float4 g, float beta) {

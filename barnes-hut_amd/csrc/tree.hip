@@ -559,6 +559,7 @@ __global__ __launch_bounds__(1024) void k_group_scan(Header* __restrict__ hdr,
   if (threadIdx.x == 0) {
     hdr->num_groups = static_cast<int>(carry);
     hdr->queue = 0;
+    for (int x = 0; x < 8; ++x) hdr->xq[x] = 0;
   }
 }
 
@@ -994,11 +995,10 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const int w = threadIdx.x >> 6;
   const int num_groups = A.hdr->num_groups;
   unsigned long long n_far = 0, n_near = 0, n_groups = 0;
+  int tries = 0;
   while (true) {
-    int g = 0;
-    if (lane == 0) g = atomicAdd(&A.hdr->queue, 1);
-    g = __builtin_amdgcn_readfirstlane(g);
-    if (g >= num_groups) break;  // every wave reaches this exit
+    const int g = next_group(A.hdr, num_groups, lane, tries);
+    if (g < 0) break;  // every wave reaches this exit
     const int2 grp = A.groups[g];
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);

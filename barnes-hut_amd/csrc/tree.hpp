@@ -25,7 +25,8 @@ struct Header {
   int32_t num_nodes;
   int32_t overflow;
   int32_t num_groups;  // query groups of the walk
-  int32_t queue;       // next group to hand out (walk work queue)
+  int32_t queue;       // unused by the walks (kept zero)
+  int32_t xq[8];       // per-XCD work queues: next group of each eighth of the groups
   unsigned long long stats[8];  // far interactions, near pairs, groups
 };
 

@@ -160,11 +160,10 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
   int* near = s_near[wv];
   const int num_groups = A.qhdr->num_groups;
   unsigned long long n_far = 0, n_near = 0, n_groups = 0;
+  int tries = 0;
   while (true) {
-    int g = 0;
-    if (lane == 0) g = atomicAdd(&A.qhdr->queue, 1);
-    g = __builtin_amdgcn_readfirstlane(g);
-    if (g >= num_groups) break;  // every wave reaches this exit
+    const int g = next_group(A.qhdr, num_groups, lane, tries);
+    if (g < 0) break;  // every wave reaches this exit
     const int2 grp = A.groups[g];
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
