@@ -121,6 +121,11 @@ inline unsigned grid_for(int64_t n, int block, unsigned cap = 0x7fffffffu) {
 // `temp` must hold radix_sort_temp_bytes(n).  Result lands in keys_out/vals_out
 // (keys_in/vals_in are clobbered).
 size_t radix_sort_temp_bytes(int64_t n);
+// In-place exclusive prefix sum of n uint32 values (three launches: tile
+// scans, a scan of the tile sums, the fix-up); `temp` holds
+// exclusive_scan_temp_bytes(n).
+size_t exclusive_scan_temp_bytes(int64_t n);
+int exclusive_scan(hipStream_t stream, uint32_t* data, int64_t n, void* temp);
 int radix_sort_pairs(hipStream_t stream, uint32_t* keys_in, uint32_t* vals_in,
                      uint32_t* keys_out, uint32_t* vals_out, int64_t n,
                      int key_bits, void* temp);

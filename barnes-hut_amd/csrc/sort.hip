@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kSortBlock) void k_radix_scatter(
   }
 }
 
-int exclusive_scan(hipStream_t s, uint32_t* data, int64_t n, uint32_t* sums) {
+int scan_impl(hipStream_t s, uint32_t* data, int64_t n, uint32_t* sums) {
   int64_t tiles = (n + kScanTile - 1) / kScanTile;
   k_scan_tiles<<<static_cast<unsigned>(tiles), kScanBlock, 0, s>>>(data, n, sums);
   SBH_LAUNCH_CHECK();
@@ -160,6 +160,15 @@ int exclusive_scan(hipStream_t s, uint32_t* data, int64_t n, uint32_t* sums) {
 }
 
 }  // namespace
+
+size_t exclusive_scan_temp_bytes(int64_t n) {
+  return static_cast<size_t>((n + kScanTile - 1) / kScanTile + 64) * sizeof(uint32_t);
+}
+
+int exclusive_scan(hipStream_t s, uint32_t* data, int64_t n, void* temp) {
+  if (n <= 0) return SBH_OK;
+  return scan_impl(s, data, n, static_cast<uint32_t*>(temp));
+}
 
 size_t radix_sort_temp_bytes(int64_t n) {
   int64_t tiles = (n + kTile - 1) / kTile;
@@ -186,7 +195,7 @@ int radix_sort_pairs(hipStream_t s, uint32_t* keys_in, uint32_t* vals_in, uint32
     int shift = 8 * p;
     k_radix_hist<<<static_cast<unsigned>(tiles), kSortBlock, 0, s>>>(ki, n, shift, hist, tiles);
     SBH_LAUNCH_CHECK();
-    int rc = exclusive_scan(s, hist, tiles * kRadix, sums);
+    int rc = scan_impl(s, hist, tiles * kRadix, sums);
     if (rc != SBH_OK) return rc;
     k_radix_scatter<<<static_cast<unsigned>(tiles), kSortBlock, 0, s>>>(ki, vi, n, shift, hist,
                                                                         tiles, ko, vo);

@@ -300,28 +300,40 @@ __global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
 }
 
 // ------------------------------------------------------------------ topology
-__device__ __forceinline__ int lower_bound_code(const uint32_t* __restrict__ codes, int lo,
-                                                int hi, uint32_t key) {
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (codes[mid] < key)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-// Octant boundaries of node k's range at level l+1: b[0..8].
+// Octant boundaries of node k's range at level l+1: b[0..8].  The seven
+// lower-bound searches run in lockstep (seven independent loads per step),
+// so the dependent-load chain is log2(range) long instead of seven times it.
 __device__ __forceinline__ void octant_bounds(const uint32_t* __restrict__ codes, int4 t,
                                               uint32_t pre, int shift, int* b) {
-  int s = t.x, e = t.x + t.y;
+  const int s = t.x, e = t.x + t.y;
+  int lo[7], hi[7];
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    lo[c] = s;
+    hi[c] = e;
+  }
+  for (int len = e - s; len > 0; len >>= 1) {
+    // all seven probes are issued before any is compared (a finished search
+    // re-reads a valid element and ignores it)
+    int mid[7];
+    uint32_t v[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) mid[c] = min((lo[c] + hi[c]) >> 1, e - 1);
+#pragma unroll
+    for (int c = 0; c < 7; ++c) v[c] = codes[mid[c]];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      const uint32_t key = ((pre << 3) | static_cast<uint32_t>(c + 1)) << shift;
+      const bool active = lo[c] < hi[c];
+      const bool below = v[c] < key;
+      lo[c] = (active && below) ? mid[c] + 1 : lo[c];
+      hi[c] = (active && !below) ? mid[c] : hi[c];
+    }
+  }
   b[0] = s;
   b[8] = e;
-  for (int c = 1; c < 8; ++c) {
-    uint32_t key = ((pre << 3) | static_cast<uint32_t>(c)) << shift;
-    b[c] = lower_bound_code(codes, b[c - 1], e, key);
-  }
+#pragma unroll
+  for (int c = 0; c < 7; ++c) b[c + 1] = lo[c];
 }
 
 __global__ __launch_bounds__(kB) void k_split(Header* __restrict__ hdr, int level, int max_depth,
@@ -345,20 +357,22 @@ __global__ __launch_bounds__(kB) void k_split(Header* __restrict__ hdr, int leve
   }
 }
 
-// Single-block exclusive scan of the level's child counts; sets the next level.
 // In-place exclusive scan of count u32 values by one block of 1024 threads
 // (tiles of 4096: 4 consecutive values per thread, wave scans, a scan of the
 // 16 wave totals); returns the total to every thread.
 __device__ uint32_t block_exclusive_scan(uint32_t* __restrict__ data, int count) {
+  constexpr int kPer = 4;
   __shared__ uint32_t wsum[16];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   uint32_t carry = 0;
-  for (int base = 0; base < count; base += 4096) {
-    const int i0 = base + 4 * static_cast<int>(threadIdx.x);
-    uint32_t v[4];
+  for (int base = 0; base < count; base += 1024 * kPer) {
+    const int i0 = base + kPer * static_cast<int>(threadIdx.x);
+    uint32_t v[kPer];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) v[e] = i0 + e < count ? data[i0 + e] : 0u;
-    const uint32_t local = v[0] + v[1] + v[2] + v[3];
+    for (int e = 0; e < kPer; ++e) v[e] = i0 + e < count ? data[i0 + e] : 0u;
+    uint32_t local = 0;
+#pragma unroll
+    for (int e = 0; e < kPer; ++e) local += v[e];
     const uint32_t incl = static_cast<uint32_t>(wave_incl_scan(static_cast<int>(local), lane));
     if (lane == 63) wsum[wid] = incl;
     __syncthreads();
@@ -370,7 +384,7 @@ __device__ uint32_t block_exclusive_scan(uint32_t* __restrict__ data, int count)
     __syncthreads();
     uint32_t run = carry + (wid > 0 ? wsum[wid - 1] : 0u) + incl - local;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
+    for (int e = 0; e < kPer; ++e) {
       if (i0 + e < count) data[i0 + e] = run;
       run += v[e];
     }
@@ -380,6 +394,7 @@ __device__ uint32_t block_exclusive_scan(uint32_t* __restrict__ data, int count)
   return carry;
 }
 
+// Single-block exclusive scan of the level's child counts; sets the next level.
 __global__ __launch_bounds__(1024) void k_scan_level(Header* __restrict__ hdr, int level,
                                                      uint32_t* __restrict__ nch, int cap) {
   const int count = hdr->level_count[level];
@@ -542,33 +557,30 @@ __device__ __forceinline__ int node_groups(const int4* __restrict__ topo,
   return ng;
 }
 
+// Per-node group counts over the whole capacity `cap` (zero past the node
+// count), for a device-wide exclusive scan of host-known length.
 __global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ hdr,
                                                     const int4* __restrict__ topo,
                                                     const int32_t* __restrict__ parent,
-                                                    uint32_t* __restrict__ goff) {
+                                                    uint32_t* __restrict__ goff, int cap) {
   const int nn = hdr->num_nodes;
-  for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    goff[k] = static_cast<uint32_t>(node_groups<false>(topo, parent, k, nullptr));
+  for (int k = blockIdx.x * kB + threadIdx.x; k < cap; k += gridDim.x * kB)
+    goff[k] = k < nn ? static_cast<uint32_t>(node_groups<false>(topo, parent, k, nullptr)) : 0u;
 }
 
-// Exclusive scan of the per-node group counts (one block); publishes the
-// group count and resets the walk's work queue.
-__global__ __launch_bounds__(1024) void k_group_scan(Header* __restrict__ hdr,
-                                                     uint32_t* __restrict__ goff) {
-  const uint32_t carry = block_exclusive_scan(goff, hdr->num_nodes);
-  if (threadIdx.x == 0) {
-    hdr->num_groups = static_cast<int>(carry);
-    hdr->queue = 0;
-    for (int x = 0; x < 8; ++x) hdr->xq[x] = 0;
-  }
-}
-
-__global__ __launch_bounds__(kB) void k_group_emit(const Header* __restrict__ hdr,
+// Writes the groups at their scanned offsets; publishes the group count
+// (the exclusive sum at the node count) and resets the walk's work queues.
+__global__ __launch_bounds__(kB) void k_group_emit(Header* __restrict__ hdr,
                                                    const int4* __restrict__ topo,
                                                    const int32_t* __restrict__ parent,
                                                    const uint32_t* __restrict__ goff,
                                                    int2* __restrict__ groups) {
   const int nn = hdr->num_nodes;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    hdr->num_groups = static_cast<int>(goff[nn]);
+    hdr->queue = 0;
+    for (int x = 0; x < 8; ++x) hdr->xq[x] = 0;
+  }
   for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
     node_groups<true>(topo, parent, k, groups + goff[k]);
 }
@@ -1139,7 +1151,9 @@ Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   L.perm = o; o = align(o + 4 * nn);
   L.keys_tmp = o; o = align(o + 4 * nn);
   L.vals_tmp = o; o = align(o + 4 * nn);
-  L.sort_tmp = o; o = align(o + radix_sort_temp_bytes(n));
+  // radix sort temporaries, reused by the scan of the group counts
+  L.sort_tmp = o;
+  o = align(o + std::max(radix_sort_temp_bytes(n), exclusive_scan_temp_bytes(L.cap + 1)));
   L.bbox_tmp = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
   L.sp = o; o = align(o + 16 * nn);
   L.sc = o; o = align(o + 16 * nn);
@@ -1239,11 +1253,14 @@ int build_rest(hipStream_t s, char* ws, const Layout& L, const float* pts, const
   if (!(parts & kBuildGroups)) return SBH_OK;
   auto* goff = reinterpret_cast<uint32_t*>(ws + L.goff);
   auto* groups = reinterpret_cast<int2*>(ws + L.groups);
-  unsigned gg = grid_for(L.cap, kB, 4096);
-  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, parent, goff);
+  // counts over cap + 1 entries: the node count is at most cap, so the
+  // exclusive sum at index num_nodes is the group total
+  const int64_t span = L.cap + 1;
+  unsigned gg = grid_for(span, kB, 4096);
+  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, parent, goff, static_cast<int>(span));
   SBH_LAUNCH_CHECK();
-  k_group_scan<<<1, 1024, 0, s>>>(hdr, goff);
-  SBH_LAUNCH_CHECK();
+  int rc = exclusive_scan(s, goff, span, ws + L.sort_tmp);
+  if (rc != SBH_OK) return rc;
   k_group_emit<<<gg, kB, 0, s>>>(hdr, topo, parent, goff, groups);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
