@@ -284,6 +284,23 @@ __device__ __forceinline__ int next_bit(const Node3* nd, int from) {
   return kCells;
 }
 
+// Exact sum over the point records [j0, j1) in index order; records are
+// fetched 8 at a time so that their loads overlap.
+template <int KIND, class Params>
+__device__ __forceinline__ void l3_leaf_sum(const Leaf3* __restrict__ leaf, uint64_t j0,
+                                            uint64_t j1, float qx, float qy, float qz,
+                                            const Params& P, float* acc) {
+  uint64_t j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    Leaf3 r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = leaf[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) l3_term<KIND>(r[u], qx, qy, qz, P, acc);
+  }
+  for (; j < j1; ++j) l3_term<KIND>(leaf[j], qx, qy, qz, P, acc);
+}
+
 template <int KIND>
 __global__ void k_l3_walk(const Leaf3* __restrict__ leaf, const Node3* __restrict__ nodes,
                           const float* __restrict__ queries, int nq, float* __restrict__ out,
@@ -310,7 +327,7 @@ __global__ void k_l3_walk(const Leaf3* __restrict__ leaf, const Node3* __restric
       if (all) {
         if (nd->is_leaf) {
           const uint64_t je = nd->end < num_leaves ? nd->end : num_leaves;
-          for (uint64_t j = nd->start; j < je; ++j) l3_term<KIND>(leaf[j], qx, qy, qz, A.P, acc);
+          l3_leaf_sum<KIND>(leaf, nd->start, je, qx, qy, qz, A.P, acc);
         } else {
           l3_term<KIND>(nd->contrib, qx, qy, qz, A.P, acc);
         }
@@ -374,7 +391,7 @@ __global__ __launch_bounds__(kB) void k_l3_stoch(const Leaf3* __restrict__ leaf,
     const Node3* sub = &nodes[ci];
     if (sub->is_leaf) {
       const uint64_t je = sub->end < num_leaves ? sub->end : num_leaves;
-      for (uint64_t j = sub->start; j < je; ++j) l3_term<KIND>(leaf[j], qx, qy, qz, P, acc);
+      l3_leaf_sum<KIND>(leaf, sub->start, je, qx, qy, qz, P, acc);
     } else {
       l3_term<KIND>(ci < num_contrib ? contrib[ci] : sub->contrib, qx, qy, qz, P, acc);
     }

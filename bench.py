@@ -711,6 +711,18 @@ def bench_stochastic(args):
                 "l2_dev_over_l2_se": float(torch.linalg.norm(mean - target) /
                                            torch.linalg.norm(se))}
 
+    # the reference's level-3 (Tree3DNode<1,3>) targets on the same points:
+    # build, and the warp-voted softened walk on the query sample (interior
+    # queries sum every point exactly, so it is a direct sum in cost)
+    (l3leaf, l3node), l3_build_ms = timed(
+        lambda: ffi.build_octree_buffers(P, Z, M), args.steps)
+    l3f, l3_walk_ms = timed(lambda: ffi.softened_barnes_hut_force(
+        l3leaf, l3node, Qs, beta=args.beta, softening_scale=args.eps,
+        cutoff_scale=float("inf")), 3)
+    level3 = {"build_ms": l3_build_ms, "walk_ms": l3_walk_ms, "walk_queries": int(Qs.shape[0]),
+              "walk_pairs_per_s": Qs.shape[0] * n / (l3_walk_ms * 1e-3),
+              "rel_l2_vs_fp64_direct": rel_l2(l3f, ref)}
+
     acc = {"classic_rel_l2_vs_fp64_direct": rel_l2(clq, ref),
            "stochastic_one_seed_rel_l2_vs_fp64_direct": rel_l2(st[idx], ref),
            "stochastic_mean_rel_l2_vs_fp64_direct": rel_l2(mean, ref),
@@ -753,6 +765,7 @@ def bench_stochastic(args):
                    "samples_per_subdomain": 1},
         "phases_ms": {"octree8_build": build_ms, "classic_walk": cl_ms,
                       "stochastic_walk": st_ms},
+        "level3": level3,
         "accuracy": acc,
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
