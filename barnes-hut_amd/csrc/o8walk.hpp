@@ -76,6 +76,10 @@ enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2, kGravVjp = 3, kWind
 
 constexpr float kInvFourPi = 0.07957747154594767f;
 
+// Query count from which the per-query walks run in Morton order of the
+// queries (below it the ordering costs more than it saves).
+constexpr int64_t kOrderMin = 4096;
+
 // signedSolidAngle<3> (include/kernel.h:73-80): a n.(p - q) / (4 pi (|p - q|^3 + 1e-5)).
 __device__ __forceinline__ float wn_pair(const Leaf3& L, float qx, float qy, float qz,
                                          const WalkParams& P) {

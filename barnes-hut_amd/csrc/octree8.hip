@@ -10,6 +10,7 @@
 // with the reference's push order, parameters passed by value.
 #include "common.hpp"
 #include "o8walk.hpp"
+#include "tree.hpp"
 
 namespace sbh {
 namespace {
@@ -207,10 +208,15 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
                                               const Node8* __restrict__ nodes,
                                               const float* __restrict__ queries,
                                               const float* __restrict__ cot, int64_t nq,
+                                              const uint32_t* __restrict__ order,
                                               float* __restrict__ out, WalkParams P,
                                               uint32_t num_nodes, uint32_t num_leaves) {
-  int64_t q = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (q >= nq) return;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (t >= nq) return;
+  // thread t walks query order[t] (Morton order: the lanes of a wave are
+  // neighbours and follow nearly the same paths); each query's result does
+  // not depend on which thread computes it
+  const int64_t q = order != nullptr ? static_cast<int64_t>(order[t]) : t;
   float qx = queries[3 * q + 0], qy = queries[3 * q + 1], qz = queries[3 * q + 2];
   float gx = 0.f, gy = 0.f, gz = 0.f;
   if constexpr (KIND == kSoftVjp) {
@@ -269,10 +275,18 @@ int launch_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node
   int64_t nq = queries->dims[0];
   if (nq == 0) return SBH_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t* order = nullptr;
+  if (nq >= o8::kOrderMin) {
+    char* ws = static_cast<char*>(workspace(s, kWsWalk, tree::query_order_bytes(nq)));
+    if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+    int rc = tree::query_order(s, static_cast<const float*>(queries->data), nq, ws, &order);
+    if (rc != SBH_OK) return rc;
+  }
   k_walk8<KIND><<<grid_for(nq, kB), kB, 0, s>>>(
       static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node8*>(node_bytes->data),
       static_cast<const float*>(queries->data),
-      cot ? static_cast<const float*>(cot->data) : nullptr, nq, static_cast<float*>(out->data),
+      cot ? static_cast<const float*>(cot->data) : nullptr, nq, order,
+      static_cast<float*>(out->data),
       P, static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node8)),
       static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf3)));
   SBH_LAUNCH_CHECK();

@@ -11,6 +11,7 @@
 // sample path is reproducible independently of thread scheduling.
 #include "common.hpp"
 #include "o8walk.hpp"
+#include "tree.hpp"
 
 namespace sbh {
 namespace {
@@ -25,12 +26,16 @@ __global__ __launch_bounds__(kB) void k_stoch8(const Leaf3* __restrict__ leaf,
                                                const Node8* __restrict__ nodes,
                                                const float* __restrict__ queries,
                                                const float* __restrict__ cot, int64_t nq,
+                                               const uint32_t* __restrict__ order,
                                                float* __restrict__ out, WalkParams P,
                                                int samples, uint32_t seed, uint32_t num_nodes,
                                                uint32_t num_leaves) {
   constexpr bool kSoft = (KIND == kSoftForce || KIND == kSoftVjp);
-  const int64_t q = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (q >= nq) return;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
+  if (t >= nq) return;
+  // thread t estimates query order[t] (Morton order); the random numbers are
+  // keyed by the query index q, so the estimate does not depend on the thread
+  const int64_t q = order != nullptr ? static_cast<int64_t>(order[t]) : t;
   const float qx = queries[3 * q], qy = queries[3 * q + 1], qz = queries[3 * q + 2];
   float gx = 0.f, gy = 0.f, gz = 0.f;
   if constexpr (KIND == kSoftVjp || KIND == kGravVjp) {
@@ -145,10 +150,18 @@ int launch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_byte
   const int64_t nq = queries->dims[0];
   if (nq == 0) return SBH_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t* order = nullptr;
+  if (nq >= kOrderMin) {
+    char* ws = static_cast<char*>(workspace(s, kWsWalk, tree::query_order_bytes(nq)));
+    if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+    int rc = tree::query_order(s, static_cast<const float*>(queries->data), nq, ws, &order);
+    if (rc != SBH_OK) return rc;
+  }
   k_stoch8<KIND><<<grid_for(nq, kB), kB, 0, s>>>(
       static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node8*>(node_bytes->data),
       static_cast<const float*>(queries->data),
-      cot ? static_cast<const float*>(cot->data) : nullptr, nq, static_cast<float*>(out->data),
+      cot ? static_cast<const float*>(cot->data) : nullptr, nq, order,
+      static_cast<float*>(out->data),
       P, samples, static_cast<uint32_t>(seed),
       static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node8)),
       static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf3)));

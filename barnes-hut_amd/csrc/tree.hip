@@ -1387,5 +1387,63 @@ int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out) {
   return SBH_OK;
 }
 
+// ------------------------------------------------------------------ query order
+namespace {
+
+// 30-bit Morton key of each query in the queries' own bounding box.
+__global__ __launch_bounds__(kB) void k_query_keys(const float* __restrict__ q, int64_t nq,
+                                                   const float* __restrict__ bounds,
+                                                   uint32_t* __restrict__ keys,
+                                                   uint32_t* __restrict__ vals) {
+  const float lx = bounds[0], ly = bounds[1], lz = bounds[2];
+  const float span = fmaxf(fmaxf(bounds[3] - lx, bounds[4] - ly), fmaxf(bounds[5] - lz, 1e-30f));
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x; i < nq;
+       i += static_cast<int64_t>(gridDim.x) * kB) {
+    keys[i] = morton3(quantize_axis(q[3 * i], lx, span, 10), quantize_axis(q[3 * i + 1], ly, span, 10),
+                      quantize_axis(q[3 * i + 2], lz, span, 10));
+    vals[i] = static_cast<uint32_t>(i);
+  }
+}
+
+struct QueryOrderLayout {
+  size_t tmp, keys, vals, keys_out, perm, sort, total;
+};
+
+QueryOrderLayout query_order_layout(int64_t nq) {
+  auto align = [](size_t x) { return (x + 255) & ~size_t{255}; };
+  const size_t n4 = 4 * static_cast<size_t>(nq > 0 ? nq : 1);
+  QueryOrderLayout L;
+  size_t o = 0;
+  L.tmp = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
+  L.keys = o; o = align(o + n4);
+  L.vals = o; o = align(o + n4);
+  L.keys_out = o; o = align(o + n4);
+  L.perm = o; o = align(o + n4);
+  L.sort = o; o = align(o + radix_sort_temp_bytes(nq));
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+size_t query_order_bytes(int64_t nq) { return query_order_layout(nq).total; }
+
+int query_order(hipStream_t s, const float* q, int64_t nq, char* ws, const uint32_t** perm) {
+  const QueryOrderLayout L = query_order_layout(nq);
+  float* bounds = nullptr;
+  int rc = data_bounds(s, q, nq, reinterpret_cast<float*>(ws + L.tmp), &bounds);
+  if (rc != SBH_OK) return rc;
+  auto* keys = reinterpret_cast<uint32_t*>(ws + L.keys);
+  auto* vals = reinterpret_cast<uint32_t*>(ws + L.vals);
+  auto* out = reinterpret_cast<uint32_t*>(ws + L.perm);
+  k_query_keys<<<grid_for(nq, kB, 4096), kB, 0, s>>>(q, nq, bounds, keys, vals);
+  SBH_LAUNCH_CHECK();
+  rc = radix_sort_pairs(s, keys, vals, reinterpret_cast<uint32_t*>(ws + L.keys_out), out, nq, 30,
+                        ws + L.sort);
+  if (rc != SBH_OK) return rc;
+  *perm = out;
+  return SBH_OK;
+}
+
 }  // namespace tree
 }  // namespace sbh

@@ -72,6 +72,13 @@ __device__ __forceinline__ int set_of(const int32_t* __restrict__ offs, int nset
 // returned in *bounds; tmp holds 6 * (kBboxBlocks + 1) floats.
 int data_bounds(hipStream_t s, const float* pts, int64_t n, float* tmp, float** bounds);
 
+// Morton order of nq query points (30-bit keys in their own bounding box):
+// *perm (inside ws, nq entries) lists the query indices in that order.  Used
+// to schedule per-query walks so that the lanes of a wave are neighbours;
+// ws holds query_order_bytes(nq).
+size_t query_order_bytes(int64_t nq);
+int query_order(hipStream_t s, const float* q, int64_t nq, char* ws, const uint32_t** perm);
+
 // Parts of build() beyond the topology (sorted records, nodes, geometry).
 constexpr int kBuildMoments = 1;  // mass, centre of mass, second moment per node
 constexpr int kBuildGroups = 2;   // query groups of the walk (cells of <= kGroupMax points)

@@ -239,3 +239,33 @@ def test_reference_recompute_custom_vjp_matches_fixed_tree_vjp(gpu):
     leaf, node = ffi.build_octree8_buffers(p, np.zeros_like(p), m, max_depth=6)
     ref = ffi.softened_barnes_hut_force_octree8_vjp(leaf, node, p, c, **kw)
     np.testing.assert_allclose(_np(x.grad), _np(ref), rtol=1e-5, atol=1e-6)
+
+
+def test_morton_scheduled_walks_equal_unordered_chunks(gpu):
+    # From 4096 queries the walks run in Morton order of the queries; each
+    # query's result must not change: compare with chunked calls below the
+    # threshold (input order), bit for bit, and with the oracle.
+    ffi = _ffi()
+    p, m, leaf, node = _tree(n=20000, depth=6, seed=5, kind="plummer")
+    rng = np.random.default_rng(6)
+    q = np.concatenate([p[rng.permutation(p.shape[0])[:9000]],
+                        rng.normal(0, 3, (1000, 3)).astype(np.float32)])
+    c = rng.normal(size=q.shape).astype(np.float32)
+    lt, nt = torch.tensor(leaf, device="cuda"), torch.tensor(node, device="cuda")
+    qt, ct = torch.tensor(q, device="cuda"), torch.tensor(c, device="cuda")
+    kw = dict(beta=1.5, softening_scale=0.02, cutoff_scale=0.5, prune_enabled=True,
+              prune_r_cut_mult=2.0)
+    calls = [
+        lambda a, b: ffi.barnes_hut_force_octree8(lt, nt, a, beta=1.5),
+        lambda a, b: ffi.softened_barnes_hut_force_octree8(lt, nt, a, **kw),
+        lambda a, b: ffi.softened_barnes_hut_force_octree8_vjp(lt, nt, a, b, **kw),
+    ]
+    for f in calls:
+        whole = _np(f(qt, ct))
+        parts = np.concatenate([_np(f(qt[k:k + 2500].contiguous(), ct[k:k + 2500].contiguous()))
+                                for k in range(0, q.shape[0], 2500)])
+        np.testing.assert_array_equal(whole, parts)
+    out = _np(ffi.softened_barnes_hut_force_octree8(lt, nt, qt, **kw))
+    ref, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1.5, eps=0.02, rc=0.5,
+                          prune_enabled=True, prune_r_cut_mult=2.0)
+    assert O.rel_l2(out, ref) < 1e-5

@@ -263,3 +263,18 @@ def test_stochastic_recompute_vjp_matches_explicit_vjp(gpu, softened):
     np.testing.assert_array_equal(_np(f), _np(ref_f))
     (f * c).sum().backward()
     np.testing.assert_array_equal(_np(x.grad), _np(ref_g))
+
+
+def test_stochastic_morton_scheduled_matches_oracle(gpu):
+    # 6000 queries: the walk runs in Morton order of the queries; the random
+    # numbers are keyed by the query index, so the paths are the oracle's
+    ffi = _ffi()
+    p, m, leaf, node = _case("plummer", 12000, 6, seed=31)
+    rng = np.random.default_rng(32)
+    q = p[rng.permutation(p.shape[0])[:6000]].copy()
+    lt, nt, qt = _dev(leaf, node, q)
+    out = _np(ffi.softened_stochastic_barnes_hut_force_octree8(
+        lt, nt, qt, beta=1.0, samples_per_subdomain=2, seed=77, **SOFT))
+    ref = O.sbh_octree8(leaf, node, q, O.KIND_SOFT, 2, 77, 1.0, eps=0.03, rc=0.6,
+                        prune_enabled=True, prune_r_cut_mult=2.5)
+    assert O.rel_l2(out, ref) < 1e-5
