@@ -160,6 +160,24 @@ __device__ __forceinline__ void contribution(const Leaf3& L, float qx, float qy,
   }
 }
 
+// Exact sum of the point records [j0, j1) in index order; records are
+// fetched 8 at a time so that their loads overlap.
+template <int KIND>
+__device__ __forceinline__ void leaf_sum(const Leaf3* __restrict__ leaf, uint32_t j0,
+                                         uint32_t j1, float qx, float qy, float qz, float gx,
+                                         float gy, float gz, const WalkParams& P, float& ax,
+                                         float& ay, float& az) {
+  uint32_t j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    Leaf3 r[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) r[u] = leaf[j + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) contribution<KIND>(r[u], qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+  }
+  for (; j < j1; ++j) contribution<KIND>(leaf[j], qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+}
+
 // Host: walk parameters from the handler attributes (prune radius as
 // src/sbh_ffi.cu:3233-3237: pruning only with a positive cutoff).
 inline WalkParams make_walk_params(float beta, float softening, float cutoff, int prune_enabled,

@@ -235,8 +235,7 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
     if (use_far_field(nd, qx, qy, qz, P)) {
       if (nd->is_leaf) {
         uint32_t je = nd->end < num_leaves ? nd->end : num_leaves;
-        for (uint32_t j = nd->start; j < je; ++j)
-          contribution<KIND>(leaf[j], qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+        o8::leaf_sum<KIND>(leaf, nd->start, je, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
       } else {
         contribution<KIND>(nd->contrib, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
       }
