@@ -109,69 +109,89 @@ __global__ __launch_bounds__(kB) void k_setbox_init(uint32_t* __restrict__ box, 
 // a block whose rows all belong to one set reduces in LDS and issues six
 // atomics; otherwise every wave with a single set reduces across lanes, and
 // mixed waves fall back to per-lane atomics.
-__global__ __launch_bounds__(kB) void k_set_bbox(const float* __restrict__ pts, int64_t n,
-                                                 const int32_t* __restrict__ offs, int nsets,
-                                                 uint32_t* __restrict__ box) {
-  __shared__ float sh[6][kB];
-  const int lane = threadIdx.x & 63;
-  for (int64_t b0 = static_cast<int64_t>(blockIdx.x) * kB; b0 < n;
-       b0 += static_cast<int64_t>(gridDim.x) * kB) {
-    const int64_t i = b0 + threadIdx.x;
-    const bool ok = i < n;
-    const int64_t last = min<int64_t>(b0 + kB, n) - 1;
-    const int s_first = set_of(offs, nsets, static_cast<int>(b0));
-    const int s_last = set_of(offs, nsets, static_cast<int>(last));
-    float v[6];
-    for (int a = 0; a < 3; ++a) {
-      const float x = ok ? pts[3 * i + a] : 0.0f;
-      v[a] = ok ? x : INFINITY;
-      v[3 + a] = ok ? x : -INFINITY;
-    }
-    if (s_first == s_last) {
-      for (int a = 0; a < 6; ++a) sh[a][threadIdx.x] = v[a];
-      __syncthreads();
-      for (int off = kB / 2; off > 0; off >>= 1) {
-        if (threadIdx.x < static_cast<unsigned>(off))
-          for (int a = 0; a < 3; ++a) {
-            sh[a][threadIdx.x] = fminf(sh[a][threadIdx.x], sh[a][threadIdx.x + off]);
-            sh[3 + a][threadIdx.x] = fmaxf(sh[3 + a][threadIdx.x], sh[3 + a][threadIdx.x + off]);
-          }
-        __syncthreads();
-      }
-      if (threadIdx.x < 6) {
-        const uint32_t u = f2ord(sh[threadIdx.x][0]);
-        if (threadIdx.x < 3)
-          atomicMin(&box[6 * s_first + threadIdx.x], u);
-        else
-          atomicMax(&box[6 * s_first + threadIdx.x], u);
-      }
-      __syncthreads();
-      continue;
-    }
-    const int my = ok ? set_of(offs, nsets, static_cast<int>(i)) : -1;
-    const int w0 = __shfl(my, 0, 64);
-    const bool uniform = __all(my == w0 || my < 0) && w0 >= 0;
-    if (uniform) {
-      for (int d = 32; d > 0; d >>= 1)
-        for (int a = 0; a < 3; ++a) {
-          v[a] = fminf(v[a], __shfl_xor(v[a], d, 64));
-          v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], d, 64));
-        }
-      if (lane < 6) {
-        float val = v[0];
-        for (int a = 1; a < 6; ++a) val = (lane == a) ? v[a] : val;
-        const uint32_t u = f2ord(val);
-        if (lane < 3)
-          atomicMin(&box[6 * w0 + lane], u);
-        else
-          atomicMax(&box[6 * w0 + lane], u);
-      }
-    } else if (ok) {
+// Per-set boxes, pass 1: one block per tile of kB consecutive points.  A tile
+// inside one set stores its box in part[6 t ..]; a tile that straddles set
+// boundaries (at most nsets - 1 of them) stores the neutral box and adds its
+// points to the set boxes with ordered-integer atomics.
+__global__ __launch_bounds__(kB) void k_set_bbox_tiles(const float* __restrict__ pts, int64_t n,
+                                                       const int32_t* __restrict__ offs,
+                                                       int nsets, uint32_t* __restrict__ box,
+                                                       float* __restrict__ part) {
+  __shared__ float sh[kB / 64][6];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t t = blockIdx.x;
+  const int64_t b0 = t * kB;
+  const int64_t i = b0 + threadIdx.x;
+  const bool ok = i < n;
+  const int64_t last = min<int64_t>(b0 + kB, n) - 1;
+  const int s_first = set_of(offs, nsets, static_cast<int>(b0));
+  const int s_last = set_of(offs, nsets, static_cast<int>(last));
+  float v[6];
+  for (int a = 0; a < 3; ++a) {
+    const float x = ok ? pts[3 * i + a] : 0.0f;
+    v[a] = ok ? x : INFINITY;
+    v[3 + a] = ok ? x : -INFINITY;
+  }
+  if (s_first != s_last) {
+    if (threadIdx.x < 6) part[6 * t + threadIdx.x] = threadIdx.x < 3 ? INFINITY : -INFINITY;
+    if (ok) {
+      const int my = set_of(offs, nsets, static_cast<int>(i));
       for (int a = 0; a < 3; ++a) {
         atomicMin(&box[6 * my + a], f2ord(v[a]));
         atomicMax(&box[6 * my + 3 + a], f2ord(v[3 + a]));
       }
     }
+    return;
+  }
+  for (int d = 32; d > 0; d >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = fminf(v[a], __shfl_xor(v[a], d, 64));
+      v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], d, 64));
+    }
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) sh[w][a] = v[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    float r = sh[0][a];
+    for (int k = 1; k < kB / 64; ++k) r = a < 3 ? fminf(r, sh[k][a]) : fmaxf(r, sh[k][a]);
+    part[6 * t + a] = r;
+  }
+}
+
+// Pass 2: one block per set reduces the partial boxes of the tiles that
+// overlap it (straddling tiles hold the neutral box) into the set's box.
+__global__ __launch_bounds__(kB) void k_set_bbox_sets(const int32_t* __restrict__ offs,
+                                                      const float* __restrict__ part,
+                                                      uint32_t* __restrict__ box) {
+  __shared__ float sh[kB / 64][6];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k = blockIdx.x;
+  const int o0 = offs[k], o1 = offs[k + 1];
+  if (o1 <= o0) return;
+  const int t0 = o0 / kB, t1 = (o1 - 1) / kB + 1;
+  float v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int t = t0 + static_cast<int>(threadIdx.x); t < t1; t += kB)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = fminf(v[a], part[6 * static_cast<int64_t>(t) + a]);
+      v[3 + a] = fmaxf(v[3 + a], part[6 * static_cast<int64_t>(t) + 3 + a]);
+    }
+  for (int d = 32; d > 0; d >>= 1)
+    for (int a = 0; a < 3; ++a) {
+      v[a] = fminf(v[a], __shfl_xor(v[a], d, 64));
+      v[3 + a] = fmaxf(v[3 + a], __shfl_xor(v[3 + a], d, 64));
+    }
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) sh[w][a] = v[a];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int a = threadIdx.x;
+    float r = sh[0][a];
+    for (int j = 1; j < kB / 64; ++j) r = a < 3 ? fminf(r, sh[j][a]) : fmaxf(r, sh[j][a]);
+    if (a < 3)
+      atomicMin(&box[6 * k + a], f2ord(r));
+    else
+      atomicMax(&box[6 * k + a], f2ord(r));
   }
 }
 
@@ -1307,7 +1327,13 @@ int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, con
   k_setbox_init<<<grid_for(6 * L.nsets, kB, 64), kB, 0, s>>>(box, L.nsets);
   SBH_LAUNCH_CHECK();
   if (n > 0) {
-    k_set_bbox<<<grid_for(n, kB, 1024), kB, 0, s>>>(pts, n, ofs, L.nsets, box);
+    // tile partials in the sort keys' scratch (free until the codes are made)
+    auto* part = reinterpret_cast<float*>(ws + L.keys_tmp);
+    const int64_t tiles = (n + kB - 1) / kB;
+    k_set_bbox_tiles<<<static_cast<unsigned>(tiles), kB, 0, s>>>(pts, n, ofs, L.nsets, box,
+                                                                part);
+    SBH_LAUNCH_CHECK();
+    k_set_bbox_sets<<<static_cast<unsigned>(L.nsets), kB, 0, s>>>(ofs, part, box);
     SBH_LAUNCH_CHECK();
   }
   int rc = init_roots(s, ws, L, n, nullptr, box);
