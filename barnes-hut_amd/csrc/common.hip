@@ -59,6 +59,12 @@ void* workspace(hipStream_t stream, int slot, size_t bytes) {
   return p;
 }
 
+bool stream_capturing(hipStream_t stream) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(stream, &st) != hipSuccess) return false;
+  return st != hipStreamCaptureStatusNone;
+}
+
 }  // namespace sbh
 
 extern "C" {

@@ -103,6 +103,8 @@ inline bool buffer_ok(const SbhBuffer* b, int32_t dtype, int rank) {
 // Grow-only device scratch cached per (device, stream, slot).  Allocation
 // happens only when a call needs more than any earlier call on that stream.
 void* workspace(hipStream_t stream, int slot, size_t bytes);
+// True while `stream` is being captured into a graph (no allocation allowed).
+bool stream_capturing(hipStream_t stream);
 constexpr int kWsBuild = 0;
 constexpr int kWsSort = 1;
 constexpr int kWsWalk = 2;

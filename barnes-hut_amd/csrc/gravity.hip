@@ -22,10 +22,20 @@ bool begin(hipStream_t s, Rec* r, int kind) {
     std::lock_guard<std::mutex> lock(g_mu);
     if (!g_on) return false;
   }
-  for (int k = 0; k < 4; ++k)
-    if (hipEventCreate(&r->ev[k]) != hipSuccess) return false;
+  for (int k = 0; k < 4; ++k) r->ev[k] = nullptr;
   r->kind = kind;
-  return hipEventRecord(r->ev[0], s) == hipSuccess;
+  for (int k = 0; k < 4; ++k) {
+    if (hipEventCreate(&r->ev[k]) != hipSuccess) {
+      r->ev[k] = nullptr;
+      destroy(*r);
+      return false;
+    }
+  }
+  if (hipEventRecord(r->ev[0], s) != hipSuccess) {
+    destroy(*r);
+    return false;
+  }
+  return true;
 }
 
 void end(const Rec& r) {
@@ -33,14 +43,17 @@ void end(const Rec& r) {
   g_recs.push_back(r);
 }
 
+void destroy(Rec& r) {
+  for (int k = 0; k < 4; ++k) {
+    if (r.ev[k] != nullptr) (void)hipEventDestroy(r.ev[k]);
+    r.ev[k] = nullptr;
+  }
+}
+
 }  // namespace prof
 }  // namespace sbh
 
 namespace {
-
-using ProfRec = prof::Rec;
-inline bool prof_begin(hipStream_t s, ProfRec* r, int kind) { return prof::begin(s, r, kind); }
-inline void prof_end(const ProfRec& r) { prof::end(r); }
 
 int check_common(const SbhBuffer* points, const SbhBuffer* masses, float beta, float softening,
                  int max_depth, int leaf_size, int order) {
@@ -120,18 +133,17 @@ extern "C" int sbh_gravity_force(void* stream, const SbhBuffer* points, const Sb
   tree::Layout L = tree::layout(n, max_depth, leaf_size);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
-  ProfRec pr;
-  bool prof = prof_begin(s, &pr, 0);
+  prof::Scope pr(s, 0);
   rc = tree::build(s, ws, L, static_cast<const float*>(points->data),
                    static_cast<const float*>(masses->data), nullptr, n, max_depth, leaf_size,
                    nullptr);
   if (rc != SBH_OK) return rc;
-  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
                   static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
-                  prof && n > 0 ? pr.ev + 2 : nullptr);
+                  pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr);
   if (rc != SBH_OK) return rc;
-  if (prof && n > 0) prof_end(pr);
+  if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }
@@ -164,19 +176,18 @@ extern "C" int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
   tree::Layout L = tree::layout(n, max_depth, leaf_size);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
-  ProfRec pr;
-  bool prof = prof_begin(s, &pr, 1);
+  prof::Scope pr(s, 1);
   rc = tree::build(s, ws, L, static_cast<const float*>(points->data),
                    static_cast<const float*>(masses->data),
                    static_cast<const float*>(cotangent->data), n, max_depth, leaf_size, nullptr);
   if (rc != SBH_OK) return rc;
-  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
-                  stats != nullptr, prof && n > 0 ? pr.ev + 2 : nullptr);
+                  stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr);
   if (rc != SBH_OK) return rc;
-  if (prof && n > 0) prof_end(pr);
+  if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }
@@ -191,7 +202,7 @@ extern "C" void sbh_profile_enable(int32_t on) {
 // calls recorded since the last collect.  Waits for the recorded events.
 extern "C" int sbh_profile_collect(double* out, int32_t count) {
   clear_error();
-  std::vector<ProfRec> recs;
+  std::vector<prof::Rec> recs;
   {
     std::lock_guard<std::mutex> lock(prof::g_mu);
     recs.swap(prof::g_recs);
@@ -257,18 +268,17 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
-  ProfRec pr;
-  bool prof = prof_begin(s, &pr, 0);
+  prof::Scope pr(s, 0);
   rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
                           static_cast<const float*>(masses->data), nullptr, n,
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
-  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
                   static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
-                  prof && n > 0 ? pr.ev + 2 : nullptr, walk_blocks);
+                  pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks);
   if (rc != SBH_OK) return rc;
-  if (prof && n > 0) prof_end(pr);
+  if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }
@@ -305,20 +315,19 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
   if ((rc = resolve_workspace(s, workspace, L, &ws)) != SBH_OK) return rc;
-  ProfRec pr;
-  bool prof = prof_begin(s, &pr, 1);
+  prof::Scope pr(s, 1);
   rc = tree::build_forest(s, ws, L, static_cast<const float*>(points->data),
                           static_cast<const float*>(masses->data),
                           static_cast<const float*>(cotangent->data), n,
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
-  if (prof) SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
+  if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
-                  stats != nullptr, prof && n > 0 ? pr.ev + 2 : nullptr, walk_blocks);
+                  stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks);
   if (rc != SBH_OK) return rc;
-  if (prof && n > 0) prof_end(pr);
+  if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
   return SBH_OK;
 }

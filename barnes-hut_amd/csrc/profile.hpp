@@ -17,10 +17,34 @@ struct Rec {
 };
 
 // Creates the events and records the first one when profiling is enabled;
-// false (and nothing recorded) otherwise.
+// false (and no event left alive) otherwise.
 bool begin(hipStream_t s, Rec* r, int kind);
-// Queues a completed record for sbh_profile_collect.
+// Queues a completed record for sbh_profile_collect (which destroys it).
 void end(const Rec& r);
+// Destroys the events of a record that will not be queued.
+void destroy(Rec& r);
+
+// Owns one record's events from begin() until commit(); a call that returns
+// early (an error, an empty input) drops them in the destructor.
+class Scope {
+ public:
+  Scope(hipStream_t s, int kind) { on_ = begin(s, &r_, kind); }
+  ~Scope() {
+    if (on_) destroy(r_);
+  }
+  Scope(const Scope&) = delete;
+  Scope& operator=(const Scope&) = delete;
+  bool on() const { return on_; }
+  Rec& rec() { return r_; }
+  void commit() {
+    if (on_) end(r_);
+    on_ = false;
+  }
+
+ private:
+  Rec r_{};
+  bool on_ = false;
+};
 
 }  // namespace prof
 }  // namespace sbh

@@ -151,7 +151,9 @@ int launch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_byte
   if (nq == 0) return SBH_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint32_t* order = nullptr;
-  if (nq >= kOrderMin) {
+  // (not while the stream is captured: the ordering scratch may need an
+  // allocation, and the unordered walk gives the same results)
+  if (nq >= kOrderMin && !stream_capturing(s)) {
     char* ws = static_cast<char*>(workspace(s, kWsWalk, tree::query_order_bytes(nq)));
     if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
     int rc = tree::query_order(s, static_cast<const float*>(queries->data), nq, ws, &order);

@@ -289,8 +289,8 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
   }
   const auto* pts = static_cast<const float*>(points->data);
   const auto* q = static_cast<const float*>(queries->data);
-  prof::Rec pr;
-  const bool profiled = prof::begin(s, &pr, 2) && m > 0;
+  prof::Scope pr(s, 2);
+  const bool profiled = pr.on() && m > 0;
   // Source tree over the centroids with areas as weights; the normals ride
   // along as the gathered per-point vector.
   rc = tree::build(s, ws, W.t, pts, static_cast<const float*>(areas->data),
@@ -337,14 +337,14 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
     A.stats = stats != nullptr;
     const unsigned grid = grid_for(m, 64 * tree::gw::kWavesPerBlock, 256 * 8);
     if (profiled) {
-      SBH_HIP_TRY(hipEventRecord(pr.ev[1], s));
-      SBH_HIP_TRY(hipEventRecord(pr.ev[2], s));
+      SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
+      SBH_HIP_TRY(hipEventRecord(pr.rec().ev[2], s));
     }
     tree::k_wn_walk<<<grid, tree::gw::kB, 0, s>>>(A);
     SBH_LAUNCH_CHECK();
     if (profiled) {
-      SBH_HIP_TRY(hipEventRecord(pr.ev[3], s));
-      prof::end(pr);
+      SBH_HIP_TRY(hipEventRecord(pr.rec().ev[3], s));
+      pr.commit();
     }
   }
   if (stats != nullptr) {

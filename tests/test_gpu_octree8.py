@@ -269,3 +269,35 @@ def test_morton_scheduled_walks_equal_unordered_chunks(gpu):
     ref, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, 1.5, eps=0.02, rc=0.5,
                           prune_enabled=True, prune_r_cut_mult=2.0)
     assert O.rel_l2(out, ref) < 1e-5
+
+
+def test_walk_captured_in_graph_equals_eager(gpu):
+    # under stream capture the walk skips the query ordering (no scratch
+    # allocation while capturing); results are the same bit for bit
+    from sbh_amd.graph import Graph
+
+    ffi = _ffi()
+    p, m, leaf, node = _tree(n=12000, depth=5, seed=8, kind="plummer")
+    q = p[np.random.default_rng(9).permutation(p.shape[0])[:6000]].copy()
+    lt, nt = torch.tensor(leaf, device="cuda"), torch.tensor(node, device="cuda")
+    qt = torch.tensor(q, device="cuda")
+    kw = dict(beta=1.5, softening_scale=0.02, cutoff_scale=0.5, prune_enabled=True,
+              prune_r_cut_mult=2.0)
+    eager = _np(ffi.softened_barnes_hut_force_octree8(lt, nt, qt, **kw))
+    from sbh_amd._lib import buf, call, f32, i32, stream_handle
+
+    out = torch.empty((q.shape[0], 3), dtype=torch.float32, device="cuda")
+
+    def step():
+        # the C-ABI call with a preallocated output (nothing allocates)
+        call("sbh_barnes_hut_force_octree8_soft", stream_handle(), buf(lt), buf(nt), buf(qt),
+             buf(out), f32(1.5), f32(0.02), f32(0.5), i32(1), f32(2.0), i32(0), f32(1.0),
+             i32(256))
+
+    g = Graph()
+    g.capture(step)
+    for _ in range(2):
+        out.zero_()
+        g.replay()
+        np.testing.assert_array_equal(_np(out), eager)
+    g.close()
