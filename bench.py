@@ -211,14 +211,19 @@ def main():
                     help="time eager launches only (default: the step is also captured as a "
                          "HIP graph and the replay is the reported value)")
     ap.add_argument("--workload", default="gravity",
-                    choices=("gravity", "wn", "batched", "stochastic"),
+                    choices=("gravity", "wn", "batched", "stochastic", "treepm"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
                          "graph-captured forest pipeline vs 64 per-set calls; stochastic: "
                          "classic and stochastic BH on the dense octree8 at the headline "
-                         "Plummer sphere, with the seed-mean unbiasedness check")
+                         "Plummer sphere, with the seed-mean unbiasedness check; treepm: the "
+                         "short-range TreePM force (periodic, erfc cutoff, prune) on 128^3 "
+                         "clustered particles")
     ap.add_argument("--o8-depth", type=int, default=8)
+    ap.add_argument("--treepm-n1d", type=int, default=128,
+                    help="treepm: particles per side (n1d^3 particles, box of n1d cells)")
+    ap.add_argument("--treepm-depth", type=int, default=8)
     ap.add_argument("--seeds", type=int, default=256)
     ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--sets", type=int, default=64)
@@ -233,7 +238,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
-             "stochastic": bench_stochastic}[args.workload]
+             "stochastic": bench_stochastic, "treepm": bench_treepm}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -769,6 +774,126 @@ def bench_stochastic(args):
         "accuracy": acc,
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
+    }
+
+
+def treepm_particles(n1d, seed=6000):
+    """n1d^3 particles in a periodic box of n1d grid cells: a displaced grid
+    (sigma 0.3 cells) with 30% of the particles moved into 2000 Gaussian
+    halos (sigma 0.5 cells); masses 1."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    g = np.arange(n1d, dtype=np.float64)
+    p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    p += rng.normal(0, 0.3, p.shape)
+    k = int(0.3 * p.shape[0])
+    idx = rng.choice(p.shape[0], k, replace=False)
+    centres = rng.uniform(0, n1d, (2000, 3))
+    p[idx] = centres[rng.integers(0, 2000, k)] + rng.normal(0, 0.5, (k, 3))
+    p = np.mod(p, n1d).astype(np.float32)
+    p[p >= n1d] = 0.0
+    return p, np.ones(p.shape[0], np.float32)
+
+
+def treepm_direct_fp64(P, M, Q, eps, rc, box, chunk=8):
+    """FP64 min-image direct sum of the erfc-attenuated softened force
+    (src/sbh_ffi.cu:142-160, 431-460)."""
+    import math
+
+    import torch
+
+    out = []
+    for q in Q.split(chunk):
+        d = q[:, None, :] - P[None]
+        d = d - box * torch.round(d / box)
+        r2 = (d * d).sum(-1)
+        s3 = (r2 + eps * eps + 1e-12) ** -1.5
+        u = torch.sqrt(r2 + 1e-12) / (2.0 * rc)
+        att = torch.special.erfc(u) + (2.0 / math.sqrt(math.pi)) * u * torch.exp(-u * u)
+        out.append(-((M[None] * att * s3)[..., None] * d).sum(1))
+    return torch.cat(out)
+
+
+def bench_treepm(args):
+    """Short-range TreePM force (the cosmology use of the octree8 path,
+    python/cosmology/treepm_jaxpm_demo.py:51-79): periodic box, erfc cutoff
+    at the PM split scale, AABB prune, Plummer softening."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import ffi
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n1d = args.treepm_n1d
+    box = float(n1d)
+    p, m = treepm_particles(n1d)
+    n = p.shape[0]
+    P, M = torch.tensor(p, device=dev), torch.tensor(m, device=dev)
+    Z = torch.zeros_like(P)
+    lo = torch.zeros(3, device=dev)
+    hi = torch.full((3,), box, device=dev)
+    kw = dict(beta=1.7, softening_scale=0.1, cutoff_scale=1.0, prune_enabled=True,
+              prune_r_cut_mult=4.0, periodic=True, box_length=box)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return r, e0.elapsed_time(e1) / reps
+
+    depth = args.treepm_depth
+    (leaf, node), build_ms = timed(
+        lambda: ffi.build_octree8_buffers(P, Z, M, lo, hi, max_depth=depth), args.steps)
+    F, walk_ms = timed(lambda: ffi.softened_barnes_hut_force_octree8(leaf, node, P, **kw),
+                       args.steps)
+    C = torch.tensor(np.random.default_rng(6001).normal(size=(n, 3)).astype(np.float32),
+                     device=dev)
+    _, vjp_ms = timed(lambda: ffi.softened_barnes_hut_force_octree8_vjp(leaf, node, P, C, **kw),
+                      max(1, args.steps // 2))
+    ms_step = build_ms + walk_ms
+    idx = torch.tensor(np.random.default_rng(7).choice(n, args.accuracy_queries, replace=False),
+                       device=dev)
+    ref = treepm_direct_fp64(P.double(), M.double(), P[idx].double(), 0.1, 1.0, box)
+    acc = {"force_rel_l2_vs_fp64_direct": rel_l2(F[idx], ref),
+           "sample_queries": args.accuracy_queries}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        lb, nb = leaf.cpu().numpy(), node.cpu().numpy()
+        ns = 1 << 19
+        qs = p[np.random.default_rng(8).choice(n, ns, replace=False)]
+        t0 = time.perf_counter()
+        O.bh_octree8(lb, nb, qs, O.KIND_SOFT, 1.7, eps=0.1, rc=1.0, prune_enabled=True,
+                     prune_r_cut_mult=4.0, periodic=True, box=box)
+        t = (time.perf_counter() - t0) / ns
+        cpu = {"value": 1.0 / t, "unit": "forces/s (walk only)", "cores": O.num_threads(),
+               "kind": "port",
+               "sample": f"oracle octree8 walk (same MAC, prune and min-image, fp64, OpenMP "
+                         f"over queries) on {ns} of {n} queries, scaled"}
+    return {
+        "metric": "short-range TreePM forces/sec (octree8, periodic, erfc cutoff, prune)",
+        "value": n / (ms_step * 1e-3),
+        "unit": "forces/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"treepm_short_range_n{n1d}^3_box{n1d}_depth{depth}_rsplit1",
+                   "n": n, "box_length": box, "max_depth": depth, "beta": 1.7,
+                   "softening": 0.1, "cutoff_scale": 1.0, "prune_r_cut_mult": 4.0,
+                   "step": "octree8 build + softened periodic walk (forward)"},
+        "phases_ms": {"octree8_build": build_ms, "walk": walk_ms, "vjp_walk": vjp_ms},
+        "accuracy": acc,
+        "roofline": None,
+        "cpu_baseline": cpu,
     }
 
 
