@@ -201,8 +201,43 @@ using o8::kSoftForce;
 using o8::kSoftVjp;
 using o8::kWinding;
 
-constexpr int kStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
 
+// Node record read once per wave through the scalar data cache (the node
+// buffer is read-only during a walk; records are 4-byte aligned).
+typedef __attribute__((address_space(4))) const uint32_t cu32;
+
+__device__ __forceinline__ Node8 sload_node8(const Node8* nodes, uint32_t id) {
+  static_assert(sizeof(Node8) == 26 * 4, "Node8 is 26 dwords");
+  cu32* src = (cu32*)(nodes) + 26 * static_cast<uint64_t>(id);
+  Node8 n;
+  uint32_t* d = reinterpret_cast<uint32_t*>(&n);
+#pragma unroll
+  for (int k = 0; k < 26; ++k) d[k] = src[k];
+  return n;
+}
+
+__device__ __forceinline__ Leaf3 sload_leaf3(const Leaf3* leaf, uint32_t j) {
+  cu32* src = (cu32*)(leaf) + 8 * static_cast<uint64_t>(j);
+  Leaf3 r;
+  uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) d[k] = src[k];
+  return r;
+}
+
+constexpr int kWStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
+
+// Wave-coherent form of the reference's per-query DFS
+// (src/sbh_ffi.cu:1950-2144): the 64 queries of a wave share one stack of
+// (node, lanes) entries.  A popped node is evaluated by the lanes listed with
+// it; each of them skips it (empty, pruned), uses it (far field; a leaf is
+// summed exactly) or opens it, and the children are pushed, in the
+// reference's order, with the set of lanes that opened it.  Each lane thus
+// visits exactly the nodes of its own DFS and in the same order (a DFS
+// restricted to an ancestor-closed subset keeps its order), so its result is
+// bit-identical to the per-query walk, while every node and point record is
+// read once per wave through the scalar cache.  Queries run in Morton order
+// (order[t]) so that the lanes of a wave share most of their paths.
 template <int KIND>
 __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
                                               const Node8* __restrict__ nodes,
@@ -211,41 +246,73 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
                                               const uint32_t* __restrict__ order,
                                               float* __restrict__ out, WalkParams P,
                                               uint32_t num_nodes, uint32_t num_leaves) {
+  __shared__ uint32_t s_node[kB / 64][kWStack];
+  __shared__ uint64_t s_lanes[kB / 64][kWStack];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (t >= nq) return;
-  // thread t walks query order[t] (Morton order: the lanes of a wave are
-  // neighbours and follow nearly the same paths); each query's result does
-  // not depend on which thread computes it
-  const int64_t q = order != nullptr ? static_cast<int64_t>(order[t]) : t;
-  float qx = queries[3 * q + 0], qy = queries[3 * q + 1], qz = queries[3 * q + 2];
-  float gx = 0.f, gy = 0.f, gz = 0.f;
-  if constexpr (KIND == kSoftVjp) {
-    gx = cot[3 * q + 0];
-    gy = cot[3 * q + 1];
-    gz = cot[3 * q + 2];
-  }
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  int stack[kStack];
-  int top = 0;
-  stack[top++] = 0;
-  while (top > 0) {
-    const Node8* nd = &nodes[stack[--top]];
-    if (nd->contrib.mass == 0.0f) continue;
-    if (KIND != kGravGrad && KIND != kWinding && should_prune(nd, qx, qy, qz, P)) continue;
-    if (use_far_field(nd, qx, qy, qz, P)) {
-      if (nd->is_leaf) {
-        uint32_t je = nd->end < num_leaves ? nd->end : num_leaves;
-        o8::leaf_sum<KIND>(leaf, nd->start, je, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
-      } else {
-        contribution<KIND>(nd->contrib, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
-      }
-    } else {
-      uint32_t mask = nd->child_mask;
-      for (int c = 0; c < 8; ++c)
-        if ((mask & (1u << c)) && top < kStack && nd->child[c] < num_nodes)
-          stack[top++] = static_cast<int>(nd->child[c]);
+  const bool valid = t < nq;
+  const int64_t q = !valid ? 0 : (order != nullptr ? static_cast<int64_t>(order[t]) : t);
+  float qx = 0.f, qy = 0.f, qz = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+  if (valid) {
+    qx = queries[3 * q + 0];
+    qy = queries[3 * q + 1];
+    qz = queries[3 * q + 2];
+    if constexpr (KIND == kSoftVjp) {
+      gx = cot[3 * q + 0];
+      gy = cot[3 * q + 1];
+      gz = cot[3 * q + 2];
     }
   }
+  const uint64_t all = __ballot(valid);
+  if (all == 0) return;  // the whole wave is past the end
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  uint32_t* stk_node = s_node[w];
+  uint64_t* stk_lanes = s_lanes[w];
+  if (lane == 0) {
+    stk_node[0] = 0;
+    stk_lanes[0] = all;
+  }
+  int top = 1;
+  __builtin_amdgcn_wave_barrier();
+  while (top > 0) {
+    --top;
+    const uint32_t id = __builtin_amdgcn_readfirstlane(stk_node[top]);
+    const uint64_t lanes = stk_lanes[top];
+    __builtin_amdgcn_wave_barrier();
+    if (id >= num_nodes) continue;
+    const Node8 nd = sload_node8(nodes, id);
+    if (nd.contrib.mass == 0.0f) continue;
+    const bool mine = (lanes >> lane) & 1ull;
+    bool skip = !mine, far = false;
+    if (mine) {
+      if (KIND != kGravGrad && KIND != kWinding) skip = should_prune(&nd, qx, qy, qz, P);
+      if (!skip) far = use_far_field(&nd, qx, qy, qz, P);
+    }
+    if (nd.is_leaf) {
+      // a leaf always takes the far branch: the exact sum of its points
+      if (far) {
+        const uint32_t je = nd.end < num_leaves ? nd.end : num_leaves;
+        for (uint32_t j = nd.start; j < je; ++j)
+          contribution<KIND>(sload_leaf3(leaf, j), qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+      }
+      continue;
+    }
+    if (far) contribution<KIND>(nd.contrib, qx, qy, qz, gx, gy, gz, P, ax, ay, az);
+    const uint64_t open = __ballot(!skip && !far);
+    if (open == 0) continue;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if ((nd.child_mask & (1u << c)) && top < kWStack && nd.child[c] < num_nodes) {
+        if (lane == 0) {
+          stk_node[top] = nd.child[c];
+          stk_lanes[top] = open;
+        }
+        ++top;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (!valid) return;
   if constexpr (KIND == kWinding) {
     out[q] = ax;
   } else {
