@@ -441,8 +441,9 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   if (nq == 0) return SBH_OK;
   L3Params A;
   A.P = o8::make_walk_params(beta, softening, cutoff, 0, 0.f, periodic, box);
-  // gravityGradSoftCutoff applies the attenuation for any cutoff > 0
-  A.P.cutoff = cutoff > 0.0f ? 1 : 0;
+  // gravityGradSoftCutoff applies the attenuation for any cutoff > 0; at
+  // +inf it is exactly 1 (u = 0), so it is skipped there
+  A.P.cutoff = (cutoff > 0.0f && std::isfinite(cutoff)) ? 1 : 0;
   A.beta = beta;
   A.threads = threads;
   const unsigned blocks = static_cast<unsigned>((nq + threads - 1) / threads);
