@@ -298,6 +298,43 @@ __global__ __launch_bounds__(kBlock) void k_bruteforce_wn(
   if (i < nq) out[i] = total;
 }
 
+// smoothDistExp brute force (src/sbh_gpu.cu:66-77, 287-322): sum over all
+// points of exp(-alpha_j |p_j - q|), Kahan-compensated as
+// include/brute_force_gpu.h:18-66, points staged through LDS tiles.
+__global__ __launch_bounds__(kBlock) void k_bruteforce_smoothdist(
+    const float* __restrict__ pts, const float* __restrict__ alpha, int n,
+    const float* __restrict__ qs, int nq, float* __restrict__ out) {
+  __shared__ float4 tp[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  float qx = 0.f, qy = 0.f, qz = 0.f;
+  if (i < nq) {
+    qx = qs[3 * static_cast<int64_t>(i)];
+    qy = qs[3 * static_cast<int64_t>(i) + 1];
+    qz = qs[3 * static_cast<int64_t>(i) + 2];
+  }
+  float total = 0.0f, comp = 0.0f;
+  for (int base = 0; base < n; base += kBlock) {
+    const int j = base + threadIdx.x;
+    if (j < n) {
+      const int64_t o = 3 * static_cast<int64_t>(j);
+      tp[threadIdx.x] = make_float4(pts[o], pts[o + 1], pts[o + 2], alpha[j]);
+    }
+    __syncthreads();
+    const int cnt = min(kBlock, n - base);
+    for (int k = 0; k < cnt; ++k) {
+      const float4 p = tp[k];
+      const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+      const float v = expf(-p.w * sqrtf(dx * dx + dy * dy + dz * dz));
+      const float y = v - comp;
+      const float t = total + y;
+      comp = (t - total) - y;
+      total = t;
+    }
+    __syncthreads();
+  }
+  if (i < nq) out[i] = total;
+}
+
 PairParams make_params(float softening, float cutoff, int periodic, float box) {
   PairParams P;
   float eps = fmaxf(softening, 0.0f);
@@ -387,6 +424,31 @@ extern "C" int sbh_bruteforce_wn(void* stream, const SbhBuffer* points,
       static_cast<const float*>(points->data), static_cast<const float*>(normals->data),
       static_cast<const float*>(areas->data), static_cast<int>(n),
       static_cast<const float*>(queries->data), static_cast<int>(nq),
+      static_cast<float*>(out->data));
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+extern "C" int sbh_bruteforce_smoothdist(void* stream, const SbhBuffer* points,
+                                         const SbhBuffer* alphas, const SbhBuffer* queries,
+                                         SbhBuffer* out, int32_t threads) {
+  clear_error();
+  if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
+    return fail_invalid("points must have shape (N, 3)");
+  const int64_t n = points->dims[0];
+  if (!buffer_ok(alphas, SBH_DTYPE_F32, 1) || alphas->dims[0] != n)
+    return fail_invalid("alphas must have shape (N,)");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2) || queries->dims[1] != 3)
+    return fail_invalid("queries must have shape (M, 3)");
+  const int64_t nq = queries->dims[0];
+  if (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != nq)
+    return fail_invalid("out must have shape (M,)");
+  if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
+  if (n > INT32_MAX || nq > INT32_MAX) return fail_invalid("too many points");
+  if (nq == 0) return SBH_OK;
+  k_bruteforce_smoothdist<<<grid_for(nq, kBlock), kBlock, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const float*>(points->data), static_cast<const float*>(alphas->data),
+      static_cast<int>(n), static_cast<const float*>(queries->data), static_cast<int>(nq),
       static_cast<float*>(out->data));
   SBH_LAUNCH_CHECK();
   return SBH_OK;

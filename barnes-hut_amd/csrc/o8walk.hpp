@@ -72,7 +72,22 @@ __device__ __forceinline__ uint32_t child_slot(const Node8* __restrict__ nd, flo
   return ix | (iy << 1) | (iz << 2);
 }
 
-enum WalkKind { kGravGrad = 0, kSoftForce = 1, kSoftVjp = 2, kGravVjp = 3, kWinding = 4 };
+enum WalkKind {
+  kGravGrad = 0,
+  kSoftForce = 1,
+  kSoftVjp = 2,
+  kGravVjp = 3,
+  kWinding = 4,
+  kPotential = 5,   // gravityPotential (include/kernel.h:55-60)
+  kSmoothDist = 6,  // smoothDistExp (include/kernel.h:82-87)
+};
+
+// Kinds with one value per query (the others give a 3-vector).
+template <int KIND>
+constexpr bool kScalarKind = KIND == kWinding || KIND == kPotential || KIND == kSmoothDist;
+// Kinds that use the softened cutoff force and hence the AABB prune.
+template <int KIND>
+constexpr bool kPrunedKind = KIND == kSoftForce || KIND == kSoftVjp;
 
 constexpr float kInvFourPi = 0.07957747154594767f;
 
@@ -89,6 +104,24 @@ __device__ __forceinline__ float wn_pair(const Leaf3& L, float qx, float qy, flo
   float r = sqrtf(dx * dx + dy * dy + dz * dz);
   float nd = L.normal[0] * dx + L.normal[1] * dy + L.normal[2] * dz;
   return L.mass * nd * kInvFourPi / (r * r * r + 1e-5f);
+}
+
+// gravityPotential (include/kernel.h:55-60): -m / (|p - q| + 0.1).
+__device__ __forceinline__ float potential_pair(const Leaf3& L, float qx, float qy, float qz,
+                                                const WalkParams& P) {
+  float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
+  float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
+  float dz = min_image_floor(L.pos[2] - qz, P.periodic, P.box, P.inv_box);
+  return -L.mass / (sqrtf(dx * dx + dy * dy + dz * dz) + 1e-1f);
+}
+
+// smoothDistExp (include/kernel.h:82-87): exp(-alpha |p - q|).
+__device__ __forceinline__ float smooth_pair(const Leaf3& L, float qx, float qy, float qz,
+                                             const WalkParams& P) {
+  float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
+  float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
+  float dz = min_image_floor(L.pos[2] - qz, P.periodic, P.box, P.inv_box);
+  return expf(-L.alpha * sqrtf(dx * dx + dy * dy + dz * dz));
 }
 
 // Pair/node contribution to the accumulator.
@@ -154,6 +187,10 @@ __device__ __forceinline__ void contribution(const Leaf3& L, float qx, float qy,
                                              float& ax, float& ay, float& az) {
   if constexpr (KIND == kWinding) {
     ax += wn_pair(L, qx, qy, qz, P);
+  } else if constexpr (KIND == kPotential) {
+    ax += potential_pair(L, qx, qy, qz, P);
+  } else if constexpr (KIND == kSmoothDist) {
+    ax += smooth_pair(L, qx, qy, qz, P);
   } else {
     contribution<KIND>(L.pos[0], L.pos[1], L.pos[2], L.mass, qx, qy, qz, gx, gy, gz, P, ax, ay,
                        az);

@@ -285,7 +285,7 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
     const bool mine = (lanes >> lane) & 1ull;
     bool skip = !mine, far = false;
     if (mine) {
-      if (KIND != kGravGrad && KIND != kWinding) skip = should_prune(&nd, qx, qy, qz, P);
+      if constexpr (o8::kPrunedKind<KIND>) skip = should_prune(&nd, qx, qy, qz, P);
       if (!skip) far = use_far_field(&nd, qx, qy, qz, P);
     }
     if (nd.is_leaf) {
@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
     __builtin_amdgcn_wave_barrier();
   }
   if (!valid) return;
-  if constexpr (KIND == kWinding) {
+  if constexpr (o8::kScalarKind<KIND>) {
     out[q] = ax;
   } else {
     out[3 * q + 0] = ax;
@@ -510,6 +510,35 @@ extern "C" int sbh_barnes_hut_wn_octree8(void* stream, const SbhBuffer* leaf_byt
   if (rc != SBH_OK) return rc;
   WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, 0, 1.0f);
   return launch_walk<o8::kWinding>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
+}
+
+// Potential -m / (r + 0.1) (gravityPotential, include/kernel.h:55-60) over
+// octree8 records: the classic walk of src/sbh_gpu.cu:79-98, 324-361
+// (evalBarnesHut_gravity).  out: (M,).
+extern "C" int sbh_barnes_hut_potential_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                                const SbhBuffer* node_bytes,
+                                                const SbhBuffer* queries, SbhBuffer* out,
+                                                float beta, int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, 0, 1.0f);
+  return launch_walk<o8::kPotential>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
+}
+
+// Smooth distance field sum exp(-alpha r) (smoothDistExp, include/kernel.h:
+// 82-87) over octree8 records (alpha = 1 in every record, as the builder of
+// src/sbh_ffi.cu:1741, 1860 writes): src/sbh_gpu.cu:122-141, 400-434
+// (evalBarnesHut_smoothdist).  out: (M,).
+extern "C" int sbh_barnes_hut_smoothdist_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                                 const SbhBuffer* node_bytes,
+                                                 const SbhBuffer* queries, SbhBuffer* out,
+                                                 float beta, int32_t threads) {
+  clear_error();
+  int rc = check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  WalkParams P = make_walk_params(beta, 0.0f, 0.0f, 0, 0.0f, 0, 1.0f);
+  return launch_walk<o8::kSmoothDist>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P);
 }
 
 extern "C" int sbh_barnes_hut_force_octree8_soft(

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kB) void k_stoch8(const Leaf3* __restrict__ leaf,
       tz += sz * inv;
     }
   }
-  if constexpr (KIND == kWinding) {
+  if constexpr (kScalarKind<KIND>) {
     out[q] = tx;
   } else {
     out[3 * q + 0] = tx;
@@ -262,4 +262,33 @@ extern "C" int sbh_stochastic_bh_wn_octree8(void* stream, const SbhBuffer* leaf_
   o8::WalkParams P = o8::make_walk_params(beta, 0.f, 0.f, 0, 0.f, 0, 1.0f);
   return launch<o8::kWinding>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P,
                               samples_per_subdomain, seed);
+}
+
+// Stochastic potential and smooth-distance sums (MLPCV of src/sbh_gpu.cu:
+// 143-164, 436-472 and 189-211, 512-550 with gravityPotential and
+// smoothDistExp) over octree8 records.  out: (M,).
+extern "C" int sbh_stochastic_bh_potential_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                                   const SbhBuffer* node_bytes,
+                                                   const SbhBuffer* queries, SbhBuffer* out,
+                                                   int32_t samples_per_subdomain, int32_t seed,
+                                                   float beta, int32_t threads) {
+  clear_error();
+  int rc = o8::check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  o8::WalkParams P = o8::make_walk_params(beta, 0.f, 0.f, 0, 0.f, 0, 1.0f);
+  return launch<o8::kPotential>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P,
+                                samples_per_subdomain, seed);
+}
+
+extern "C" int sbh_stochastic_bh_smoothdist_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                                    const SbhBuffer* node_bytes,
+                                                    const SbhBuffer* queries, SbhBuffer* out,
+                                                    int32_t samples_per_subdomain, int32_t seed,
+                                                    float beta, int32_t threads) {
+  clear_error();
+  int rc = o8::check_walk_common(leaf_bytes, node_bytes, queries, out, threads, true);
+  if (rc != SBH_OK) return rc;
+  o8::WalkParams P = o8::make_walk_params(beta, 0.f, 0.f, 0, 0.f, 0, 1.0f);
+  return launch<o8::kSmoothDist>(stream, leaf_bytes, node_bytes, queries, nullptr, out, P,
+                                 samples_per_subdomain, seed);
 }

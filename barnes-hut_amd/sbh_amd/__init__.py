@@ -12,5 +12,6 @@ from . import gravity  # noqa: E402,F401
 from . import graph  # noqa: E402,F401
 from . import synthetic  # noqa: E402,F401
 from . import wn  # noqa: E402,F401
+from . import fields  # noqa: E402,F401
 
-__all__ += ["gravity", "graph", "synthetic", "wn"]
+__all__ += ["gravity", "graph", "synthetic", "wn", "fields"]

@@ -307,6 +307,41 @@ int sbh_stochastic_bh_wn_octree8(void* stream, const SbhBuffer* leaf_bytes,
                                  int32_t samples_per_subdomain, int32_t seed,
                                  float beta, int32_t threads);
 
+/* Potential and smooth-distance fields (the reference's pybind11 module
+ * src/sbh_gpu.cu, which evaluates them over CPU-built trees; here over
+ * octree8 records, alpha = 1 in every record as src/sbh_ffi.cu:1741, 1860
+ * writes).  Each output has shape (M,).
+ *   sbh_barnes_hut_potential_octree8      replaces evalBarnesHut_gravity
+ *                                         (src/sbh_gpu.cu:79-98, 324-361)
+ *   sbh_stochastic_bh_potential_octree8   multiLevelPrefixControlVariateSampler_gravity
+ *                                         (src/sbh_gpu.cu:143-164, 436-472)
+ *   sbh_bruteforce_smoothdist             evalBruteForce_smoothdist
+ *                                         (src/sbh_gpu.cu:66-77, 287-322; Kahan sum,
+ *                                         include/brute_force_gpu.h:18-66)
+ *   sbh_barnes_hut_smoothdist_octree8     evalBarnesHut_smoothdist
+ *                                         (src/sbh_gpu.cu:122-141, 400-434)
+ *   sbh_stochastic_bh_smoothdist_octree8  multiLevelPrefixControlVariateSampler_smoothdist
+ *                                         (src/sbh_gpu.cu:189-211, 512-550)
+ * The potential brute force (evalBruteForce_gravity, src/sbh_gpu.cu:39-51)
+ * is sbh_bruteforce_gravity above. */
+int sbh_barnes_hut_potential_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                     const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                     SbhBuffer* out, float beta, int32_t threads);
+int sbh_stochastic_bh_potential_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                        const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                        SbhBuffer* out, int32_t samples_per_subdomain,
+                                        int32_t seed, float beta, int32_t threads);
+int sbh_bruteforce_smoothdist(void* stream, const SbhBuffer* points, const SbhBuffer* alphas,
+                              const SbhBuffer* queries, SbhBuffer* out, int32_t threads);
+int sbh_barnes_hut_smoothdist_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                      const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                      SbhBuffer* out, float beta, int32_t threads);
+int sbh_stochastic_bh_smoothdist_octree8(void* stream, const SbhBuffer* leaf_bytes,
+                                         const SbhBuffer* node_bytes,
+                                         const SbhBuffer* queries, SbhBuffer* out,
+                                         int32_t samples_per_subdomain, int32_t seed,
+                                         float beta, int32_t threads);
+
 /* Adaptive-tree winding numbers (no reference counterpart: the reference
  * walks a CPU-built 512-ary tree).  The tree is built over the centroids
  * with the areas as weights; queries are Morton-sorted in their own cube and

@@ -135,6 +135,7 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_cpu_mlpcv.argtypes = [_vp, _f32p, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_uint32,
                                 ctypes.c_double, ctypes.c_int, _f64p]
     L.orc_bruteforce_wn.argtypes = [_f32p, _f32p, _f32p, _i64, _f32p, _i64, _f64p]
+    L.orc_bruteforce_smoothdist.argtypes = [_f32p, _f32p, _i64, _f32p, _i64, _f64p]
     L.orc_wn_moments.argtypes = [_i32p, _i32p, _i64, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p]
     L.orc_wn_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
                               _f64p, _f64p, _f64p, _f32p, _f64p, _i64, _i64, _i32p, _i32p,
@@ -262,6 +263,8 @@ def bruteforce_gravity(points, masses, queries, periodic=False, box=1.0):
 
 # ---------------------------------------------------------------- octree8 walks
 KIND_GRAV, KIND_SOFT, KIND_SOFT_VJP, KIND_GRAV_VJP, KIND_WN = 0, 1, 2, 3, 4
+KIND_POTENTIAL, KIND_SMOOTH = 5, 6
+_SCALAR_KINDS = (KIND_WN, KIND_POTENTIAL, KIND_SMOOTH)
 
 
 def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot=None,
@@ -276,7 +279,7 @@ def bh_octree8(leaf_bytes, node_bytes, queries, kind, beta, eps=0.0, rc=0.0, cot
                          float(eps), float(rc), int(bool(prune_enabled)),
                          float(prune_r_cut_mult), int(bool(periodic)), float(box), out,
                          ctypes.byref(cnt))
-    if kind == KIND_WN:
+    if kind in _SCALAR_KINDS:
         return out[:, 0].copy(), int(cnt.value)
     return out, int(cnt.value)
 
@@ -292,7 +295,7 @@ def sbh_octree8(leaf_bytes, node_bytes, queries, kind, samples, seed, beta, eps=
     lib().orc_sbh_octree8(leaf, node, q, _ptr(g), q.shape[0], int(kind), int(samples), int(seed),
                           float(beta), float(eps), float(rc), int(bool(prune_enabled)),
                           float(prune_r_cut_mult), int(bool(periodic)), float(box), out)
-    if kind == KIND_WN:
+    if kind in _SCALAR_KINDS:
         return out[:, 0].copy()
     return out
 
@@ -401,6 +404,16 @@ class CpuTree:
         lib().orc_cpu_mlpcv(self._h, q, q.shape[0], int(kind), int(samples), int(seed),
                             float(eps), int(threads), out)
         return out if kind == CPU_SOFT_FORCE else out[:, 0].copy()
+
+
+def bruteforce_smoothdist(points, alphas, queries):
+    """Exact fp64 sum of exp(-alpha_j |p_j - q|) (smoothDistExp)."""
+    p = _c(points, np.float32)
+    a = _c(alphas, np.float32)
+    q = _c(queries, np.float32)
+    out = np.zeros(q.shape[0], np.float64)
+    lib().orc_bruteforce_smoothdist(p, a, p.shape[0], q, q.shape[0], out)
+    return out
 
 
 def bruteforce_wn(points, normals, areas, queries):

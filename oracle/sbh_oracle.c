@@ -608,6 +608,45 @@ static inline void orc_wn_pair(const double* q, const OrcLeaf3* src, const OrcPa
   acc[0] += (double)src->mass * nd / (4.0 * M_PI * (r * r * r + 1e-5));
 }
 
+/* gravityPotential (include/kernel.h:55-60): -m / (|p - q| + 0.1). */
+static inline void orc_potential_pair(const double* q, const OrcLeaf3* src, const OrcParams* P,
+                                      double* acc) {
+  double d0 = orc_mi((double)src->pos[0] - q[0], P);
+  double d1 = orc_mi((double)src->pos[1] - q[1], P);
+  double d2 = orc_mi((double)src->pos[2] - q[2], P);
+  acc[0] += -(double)src->mass / (sqrt(d0 * d0 + d1 * d1 + d2 * d2) + 0.1);
+}
+
+/* smoothDistExp (include/kernel.h:82-87): exp(-alpha |p - q|). */
+static inline void orc_smooth_pair(const double* q, const OrcLeaf3* src, const OrcParams* P,
+                                   double* acc) {
+  double d0 = orc_mi((double)src->pos[0] - q[0], P);
+  double d1 = orc_mi((double)src->pos[1] - q[1], P);
+  double d2 = orc_mi((double)src->pos[2] - q[2], P);
+  acc[0] += exp(-(double)src->alpha * sqrt(d0 * d0 + d1 * d1 + d2 * d2));
+}
+
+/* Exact smooth-distance sum (src/sbh_gpu.cu:66-77, 287-322), fp64. */
+ORC_EXPORT void orc_bruteforce_smoothdist(const float* pts, const float* alpha, int64_t n,
+                                          const float* queries, int64_t nq, double* out) {
+  OrcParams P;
+  orc_params(&P, 0.0, 0.0, 0, 1.0f, 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {queries[3 * i], queries[3 * i + 1], queries[3 * i + 2]};
+    double acc[3] = {0, 0, 0};
+    for (int64_t j = 0; j < n; j++) {
+      OrcLeaf3 s;
+      s.pos[0] = pts[3 * j]; s.pos[1] = pts[3 * j + 1]; s.pos[2] = pts[3 * j + 2];
+      s.normal[0] = s.normal[1] = s.normal[2] = 0.0f;
+      s.mass = 0.0f;
+      s.alpha = alpha[j];
+      orc_smooth_pair(q, &s, &P, acc);
+    }
+    out[i] = acc[0];
+  }
+}
+
 /* Exact winding-number sum of dipoles (the brute_force_gpu.h:18-66 summation
  * with signedSolidAngle, in fp64). */
 ORC_EXPORT void orc_bruteforce_wn(const float* pts, const float* nrm, const float* area, int64_t n,
@@ -632,7 +671,8 @@ ORC_EXPORT void orc_bruteforce_wn(const float* pts, const float* nrm, const floa
 
 /* kind: 0 = gravityGrad (unsoftened, W3), 1 = softened/cutoff force (W2),
  *       2 = softened/cutoff VJP (W4, needs cot), 3 = gravityGrad VJP,
- *       4 = winding number (signedSolidAngle, scalar in out[3 i]). */
+ *       4 = winding number (signedSolidAngle, scalar in out[3 i]),
+ *       5 = gravityPotential, 6 = smoothDistExp (scalars in out[3 i]). */
 ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_bytes,
                                const float* queries, const float* cot, int64_t nq,
                                int kind, float beta, double eps, double rc,
@@ -665,7 +705,7 @@ ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_by
     while (top > 0) {
       const OrcNode8* nd = &nodes[stack[--top]];
       if (nd->contrib.mass == 0.0f) continue;
-      if (kind != 0 && kind != 3 &&
+      if ((kind == 1 || kind == 2) &&
           orc_should_prune8(nd, qf, prune_on, r_cut_sq, per, box))
         continue;
       if (orc_use_far_field8(nd, qf, beta, per, box)) {
@@ -678,6 +718,8 @@ ORC_EXPORT void orc_bh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_by
             case 1: orc_soft_force(q, p, src->mass, &P, acc); break;
             case 2: orc_soft_force_vjp(q, p, src->mass, g, &P, acc); break;
             case 4: orc_wn_pair(q, src, &P, acc); break;
+            case 5: orc_potential_pair(q, src, &P, acc); break;
+            case 6: orc_smooth_pair(q, src, &P, acc); break;
             default: orc_grav_grad_vjp(q, p, src->mass, g, &P, acc); break;
           }
           total_inter++;
@@ -987,6 +1029,8 @@ ORC_EXPORT void orc_sbh_octree8(const uint8_t* leaf_bytes, const uint8_t* node_b
       case 1: orc_soft_force(q, p_, (src)->mass, &P, acc); break;          \
       case 2: orc_soft_force_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
       case 4: orc_wn_pair(q, (src), &P, acc); break;                       \
+      case 5: orc_potential_pair(q, (src), &P, acc); break;                \
+      case 6: orc_smooth_pair(q, (src), &P, acc); break;                   \
       default: orc_grav_grad_vjp(q, p_, (src)->mass, g, &P, acc); break;   \
     }                                                                      \
   } while (0)

@@ -767,6 +767,70 @@ def test_sbh_wn_unbiased_and_shallow_exact():
     assert np.abs(z).max() < 5.0
 
 
+# --------------------------------------------------------------------- potential / smooth distance
+def test_bruteforce_smoothdist_closed_form():
+    rng = _rng(41)
+    p = rng.uniform(-1, 1, (37, 3)).astype(np.float32)
+    alpha = rng.uniform(0.5, 3.0, 37).astype(np.float32)
+    q = rng.uniform(-2, 2, (11, 3)).astype(np.float32)
+    d = np.linalg.norm(p[None].astype(np.float64) - q[:, None].astype(np.float64), axis=2)
+    ref = np.exp(-alpha.astype(np.float64)[None] * d).sum(1)
+    np.testing.assert_allclose(O.bruteforce_smoothdist(p, alpha, q), ref, rtol=1e-12)
+    # one point at the query: exp(0) = 1
+    np.testing.assert_allclose(O.bruteforce_smoothdist(p[:1], alpha[:1], p[:1]), [1.0])
+
+
+@pytest.mark.parametrize("kind", ["potential", "smooth"])
+def test_octree8_scalar_fields_huge_beta_are_exact_sums(kind):
+    rng = _rng(42)
+    p = rng.uniform(-1, 1, (900, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, 900).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    leaf, node, _ = O.build_octree8(p, np.zeros_like(p), m, lo, hi, 5)
+    q = rng.uniform(-1.5, 1.5, (120, 3)).astype(np.float32)
+    d = np.linalg.norm(p[None].astype(np.float64) - q[:, None].astype(np.float64), axis=2)
+    if kind == "potential":
+        k, exact = O.KIND_POTENTIAL, (-m.astype(np.float64)[None] / (d + 0.1)).sum(1)
+    else:
+        # every leaf record carries alpha = 1 (src/sbh_ffi.cu:1741)
+        k, exact = O.KIND_SMOOTH, np.exp(-d).sum(1)
+    w, cnt = O.bh_octree8(leaf, node, q, k, 1e30)
+    np.testing.assert_allclose(w, exact, rtol=1e-9)
+    assert cnt == 120 * 900
+    w2, cnt2 = O.bh_octree8(leaf, node, q, k, 2.0)
+    assert cnt2 < cnt
+    if kind == "potential":
+        assert O.rel_l2(w2, exact) < 5e-2
+    else:
+        # smoothDistExp ignores the mass, so an accepted node counts as one
+        # point at its centre of mass (the reference's semantics): the far
+        # field under-counts and the sum is not an approximation of the exact
+        # one; it stays positive and below it
+        assert (w2 > 0).all() and (w2 < exact).all()
+
+
+@pytest.mark.parametrize("kind", ["potential", "smooth"])
+def test_sbh_scalar_fields_unbiased_and_shallow_exact(kind):
+    rng = _rng(43)
+    p = rng.uniform(-1, 1, (1200, 3)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, 1200).astype(np.float32)
+    lo, hi = O.default_bounds(p)
+    q = rng.uniform(-1.2, 1.2, (20, 3)).astype(np.float32)
+    k = O.KIND_POTENTIAL if kind == "potential" else O.KIND_SMOOTH
+    leaf3, node3, _ = O.build_octree8(p, np.zeros_like(p), m, lo, hi, 3)
+    exact, _ = O.bh_octree8(leaf3, node3, q, k, 1e30)
+    # depth 3: every subdomain is a leaf, so the estimator is the exact sum
+    np.testing.assert_allclose(O.sbh_octree8(leaf3, node3, q, k, 4, 1, 1.0), exact, rtol=1e-9)
+    leaf, node, _ = O.build_octree8(p, np.zeros_like(p), m, lo, hi, 6)
+    exact, _ = O.bh_octree8(leaf, node, q, k, 1e30)
+    runs = np.stack([O.sbh_octree8(leaf, node, q, k, 8, s, 1.0) for s in range(64)])
+    mean = runs.mean(0)
+    se = runs.std(0, ddof=1) / np.sqrt(runs.shape[0])
+    z = (mean - exact) / np.maximum(se, 1e-300)
+    assert (z ** 2).mean() < 2.5
+    assert np.abs(z).max() < 5.0
+
+
 # --------------------------------------------------------------------- level-3 tree
 def _l3_case(seed=31, n=700):
     rng = _rng(seed)
