@@ -41,6 +41,34 @@ struct Node3 {
   uint8_t is_leaf, pad[3];
 };
 
+// LeafData<2> (include/leaf_data.h:8-16): 24 bytes.
+struct Leaf2 {
+  float pos[2];
+  float normal[2];
+  float mass;
+  float alpha;
+};
+
+// Tree2DNode<1, 3> (include/tree2d.h:212-225): 616 bytes, same field order
+// and padding; 8 x 8 cells, cell b <-> x + 8 y.
+struct Node2 {
+  uint64_t start, end, buffer_idx, parent_idx;
+  float bmin[2], bmax[2];  // AABB<2>
+  float diameter;
+  Leaf2 contrib;
+  uint64_t bitmask[1];     // BitMask2D<3>
+  uint64_t child[64];
+  int32_t num_children, depth, max_depth;
+  uint8_t is_leaf, pad[3];
+};
+
+static_assert(sizeof(Leaf2) == 24, "Leaf2 layout");
+static_assert(sizeof(Node2) == 616, "Tree2DNode<1,3> layout");
+static_assert(offsetof(Node2, bmin) == 32 && offsetof(Node2, diameter) == 48 &&
+                  offsetof(Node2, contrib) == 52 && offsetof(Node2, bitmask) == 80 &&
+                  offsetof(Node2, child) == 88 && offsetof(Node2, num_children) == 600 &&
+                  offsetof(Node2, is_leaf) == 612,
+              "Tree2DNode<1,3> layout");
 static_assert(sizeof(Leaf3) == 32, "Leaf3 layout");
 static_assert(sizeof(Node8) == 104, "OctreeNode8 layout");
 static_assert(offsetof(Node8, child_mask) == 48, "OctreeNode8 layout");

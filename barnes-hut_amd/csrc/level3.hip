@@ -1,7 +1,7 @@
 // Level-3 targets: the single-split 8 x 8 x 8 tree in the Tree3DNode<1, 3>
-// layout (src/sbh_ffi.cu:181-397, 736-1037) and the walks over it
-// (src/sbh_ffi.cu:1441-1680 via include/barnes_hut_gpu.h:19-72 and
-// include/tree_sampler_gpu.h:25-140).
+// layout (src/sbh_ffi.cu:181-397, 736-1037) and the walks over
+// Tree3DNode<1, 3> and Tree2DNode<1, 3> buffers (src/sbh_ffi.cu:1422-1680 via
+// include/barnes_hut_gpu.h:19-72 and include/tree_sampler_gpu.h:25-140).
 //
 // Build: per-axis cell index of every point (quantize_axis_level3), a stable
 // radix sort on the 9-bit cell id (the reference scatters with an atomic
@@ -10,14 +10,18 @@
 // fixed order, and one 512-thread block writing the root and the child
 // records (ranks of present cells by an LDS scan, the root's bit mask by one
 // wave ballot per 64 cells).  No float atomics: the build is deterministic.
+// Multi-level buffers in the same layouts (the reference's CPU trees) come
+// from ndtree.hip.
 //
 // Classic walk: the reference's warp-voted traversal: a node is used as a far
 // field only if every active query of a 32-query warp accepts it, so all
-// queries of that warp take the same path.  Launched with the caller's block
-// size, every reference warp is one half of a 64-wide wavefront here, and the
-// vote is a ballot restricted to that half.  Leaves always accept, so for
-// level-3 buffers a warp either uses the root monopole (all its queries far
-// from the whole set) or sums every point exactly.
+// queries of that warp take the same path.  For level-3 buffers (every child
+// of the root a leaf) a warp either uses the root monopole or sums every
+// point exactly.
+//
+// Stochastic walk: per subdomain (child of the root) its exact sum or
+// monopole plus the control-variate corrections along roulette paths to a
+// sample point; level-3 buffers have no paths (their subdomains are leaves).
 #include "common.hpp"
 #include "o8walk.hpp"
 
@@ -224,7 +228,26 @@ __global__ __launch_bounds__(kCells) void k_l3_nodes(const int32_t* __restrict__
 }
 
 // ------------------------------------------------------------------ walks
+// The walks read Tree3DNode<1, 3> records (level-3 buffers, or CPU-layout
+// octrees) and Tree2DNode<1, 3> records (quadtrees) alike.
 enum L3Kind { kPotential = 0, kForce = 1, kSoftForce3 = 2 };
+
+template <int DIM>
+struct L3T;
+template <>
+struct L3T<3> {
+  using Node = Node3;
+  using Leaf = Leaf3;
+  static constexpr int kCellsN = 512;
+  static constexpr int kWords = 8;
+};
+template <>
+struct L3T<2> {
+  using Node = Node2;
+  using Leaf = Leaf2;
+  static constexpr int kCellsN = 64;
+  static constexpr int kWords = 1;
+};
 
 struct L3Params {
   o8::WalkParams P;
@@ -233,10 +256,11 @@ struct L3Params {
 };
 
 // Contribution of one record to a query (gravityPotential, gravityGrad,
-// gravityGradSoftCutoff<3>: include/kernel.h:55-60, 89-94; src/sbh_ffi.cu:142-160).
+// gravityGradSoftCutoff<N>: include/kernel.h:55-60, 89-94; src/sbh_ffi.cu:142-160).
 template <int KIND>
-__device__ __forceinline__ void l3_term(const Leaf3& L, float qx, float qy, float qz,
+__device__ __forceinline__ void l3_term(const Leaf3& L, const float* q,
                                         const o8::WalkParams& P, float* acc) {
+  const float qx = q[0], qy = q[1], qz = q[2];
   if constexpr (KIND == kPotential) {
     const float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
     const float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
@@ -249,91 +273,153 @@ __device__ __forceinline__ void l3_term(const Leaf3& L, float qx, float qy, floa
   }
 }
 
-// farFieldRatio (include/tree3d.h:134-141): |q - centre of mass| / diameter,
-// infinite for a leaf.
-__device__ __forceinline__ float far_ratio(const Node3* nd, float qx, float qy, float qz) {
+template <int KIND>
+__device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
+                                        const o8::WalkParams& P, float* acc) {
+  if constexpr (KIND == kPotential) {
+    // gravityPotential<2>: d = p - q
+    const float dx = min_image_floor(L.pos[0] - q[0], P.periodic, P.box, P.inv_box);
+    const float dy = min_image_floor(L.pos[1] - q[1], P.periodic, P.box, P.inv_box);
+    acc[0] += -L.mass / (sqrtf(dx * dx + dy * dy) + 1e-1f);
+  } else {
+    // gravityGrad<2> and gravityGradSoftCutoff<2>: d = q - p
+    const float dx = min_image_floor(q[0] - L.pos[0], P.periodic, P.box, P.inv_box);
+    const float dy = min_image_floor(q[1] - L.pos[1], P.periodic, P.box, P.inv_box);
+    float c;
+    if constexpr (KIND == kForce) {
+      const float t = sqrtf(dx * dx + dy * dy) + 1e-1f;
+      c = -L.mass / (t * t * t);
+    } else {
+      const float r2 = dx * dx + dy * dy;
+      const float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+      float att = 1.0f;
+      if (P.cutoff) {
+        float e, dist;
+        att = attenuation(r2, P.rc, &e, &dist);
+      }
+      c = -L.mass * att * (inv_r * inv_r * inv_r);
+    }
+    acc[0] += c * dx;
+    acc[1] += c * dy;
+  }
+}
+
+// farFieldRatio (include/tree3d.h:134-141, include/tree2d.h:132-138):
+// |q - centre of mass| / diameter, infinite for a leaf.
+template <int DIM>
+__device__ __forceinline__ float far_ratio(const typename L3T<DIM>::Node* nd, const float* q) {
 #pragma clang fp contract(off)
   if (nd->is_leaf) return INFINITY;
-  const float dx = qx - nd->contrib.pos[0];
-  const float dy = qy - nd->contrib.pos[1];
-  const float dz = qz - nd->contrib.pos[2];
-  return sqrtf(dx * dx + dy * dy + dz * dz) / nd->diameter;
+  float s = 0.0f;
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
+    const float d = q[a] - nd->contrib.pos[a];
+    s = a == 0 ? d * d : s + d * d;
+  }
+  return sqrtf(s) / nd->diameter;
 }
 
 // Cell of point p in `parent` (AABB::quantize, include/aabb.h:36-42), or -1
 // when p is outside parent's bounds widened by 1e-4 (AABB::contains).
-__device__ __forceinline__ int cell_in(const Node3* parent, const float* p) {
+template <int DIM>
+__device__ __forceinline__ int cell_in(const typename L3T<DIM>::Node* parent, const float* p) {
 #pragma clang fp contract(off)
-  int q[3];
-  for (int a = 0; a < 3; ++a) {
+  int q[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) {
     if (!(parent->bmin[a] - 1e-4f <= p[a] && p[a] <= parent->bmax[a] + 1e-4f)) return -1;
     const float t =
         static_cast<float>(kSplit) * (p[a] - parent->bmin[a]) / (parent->bmax[a] - parent->bmin[a]);
     q[a] = min(max(static_cast<int>(t), 0), kSplit - 1);
   }
-  return q[0] + kSplit * (q[1] + kSplit * q[2]);
+  int c = 0;
+#pragma unroll
+  for (int a = DIM - 1; a >= 0; --a) c = c * kSplit + q[a];
+  return c;
 }
 
-// First set bit of the node's mask at or after `from` (kCells if none).
-__device__ __forceinline__ int next_bit(const Node3* nd, int from) {
-  for (int w = from >> 6; w < kCells / 64; ++w) {
+// First set bit of the node's mask at or after `from` (kCellsN if none).
+template <int DIM>
+__device__ __forceinline__ int next_bit(const typename L3T<DIM>::Node* nd, int from) {
+  constexpr int kN = L3T<DIM>::kCellsN;
+  for (int w = from >> 6; w < L3T<DIM>::kWords; ++w) {
     uint64_t m = nd->bitmask[w];
     if (w == (from >> 6)) m &= ~0ull << (from & 63);
     if (m) return 64 * w + __builtin_ctzll(m);
   }
-  return kCells;
+  return kN;
 }
 
 // Exact sum over the point records [j0, j1) in index order; records are
 // fetched 8 at a time so that their loads overlap.
-template <int KIND, class Params>
-__device__ __forceinline__ void l3_leaf_sum(const Leaf3* __restrict__ leaf, uint64_t j0,
-                                            uint64_t j1, float qx, float qy, float qz,
-                                            const Params& P, float* acc) {
+template <int KIND, class Leaf, class Params>
+__device__ __forceinline__ void l3_leaf_sum(const Leaf* __restrict__ leaf, uint64_t j0,
+                                            uint64_t j1, const float* q, const Params& P,
+                                            float* acc) {
   uint64_t j = j0;
   for (; j + 8 <= j1; j += 8) {
-    Leaf3 r[8];
+    Leaf r[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) r[u] = leaf[j + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) l3_term<KIND>(r[u], qx, qy, qz, P, acc);
+    for (int u = 0; u < 8; ++u) l3_term<KIND>(r[u], q, P, acc);
   }
-  for (; j < j1; ++j) l3_term<KIND>(leaf[j], qx, qy, qz, P, acc);
+  for (; j < j1; ++j) l3_term<KIND>(leaf[j], q, P, acc);
 }
 
-template <int KIND>
-__global__ void k_l3_walk(const Leaf3* __restrict__ leaf, const Node3* __restrict__ nodes,
+template <int DIM, int KIND>
+__device__ __forceinline__ void store_out(float* __restrict__ out, int q, const float* acc) {
+  if constexpr (KIND == kPotential) {
+    out[q] = acc[0];
+  } else {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) out[static_cast<int64_t>(DIM) * q + a] = acc[a];
+  }
+}
+
+// Vote walk (include/barnes_hut_gpu.h:19-72): stackless traversal in child
+// order; a node is used as a far field only if every active query of its
+// 32-query reference warp accepts it, so a warp's queries share one path.
+// Launched with the caller's block size, every reference warp is one half
+// of a 64-wide wavefront here, and the vote is a ballot restricted to that
+// half.
+template <int DIM, int KIND>
+__global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
+                          const typename L3T<DIM>::Node* __restrict__ nodes,
                           const float* __restrict__ queries, int nq, float* __restrict__ out,
                           L3Params A, uint32_t num_nodes, uint32_t num_leaves) {
+  using Node = typename L3T<DIM>::Node;
+  constexpr int kN = L3T<DIM>::kCellsN;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = q < nq;
   const int lane = __lane_id();
-  // the reference warp of this query: lanes [32 h, 32 h + 32) of the wave
   const uint64_t half = (lane < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
   const uint64_t members = __ballot(active) & half;
   if (!active) return;
-  const float qx = queries[3 * static_cast<int64_t>(q)];
-  const float qy = queries[3 * static_cast<int64_t>(q) + 1];
-  const float qz = queries[3 * static_cast<int64_t>(q) + 2];
-  float acc[3] = {0.f, 0.f, 0.f};
-  // stackless traversal (include/barnes_hut_gpu.h:19-72)
+  float qv[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
+  float acc[DIM] = {};
   uint32_t node = 0;
   bool from_child = false;
-  for (int guard = 0; guard < 4 * kMaxNodes3 * 64; ++guard) {
-    const Node3* nd = &nodes[node];
+  // every node is entered once (from its parent or its previous sibling) and
+  // left once upwards, so 2 num_nodes + 1 steps end any walk of a valid tree
+  const uint64_t guard_max = 4ull * num_nodes + 64ull;
+  for (uint64_t guard = 0; guard < guard_max; ++guard) {
+    const Node* nd = &nodes[node];
     if (!from_child) {
-      const bool far = far_ratio(nd, qx, qy, qz) >= A.beta;
+      const bool far = far_ratio<DIM>(nd, qv) >= A.beta;
       const bool all = (__ballot(far) & members) == members;
       if (all) {
         if (nd->is_leaf) {
           const uint64_t je = nd->end < num_leaves ? nd->end : num_leaves;
-          l3_leaf_sum<KIND>(leaf, nd->start, je, qx, qy, qz, A.P, acc);
+          l3_leaf_sum<KIND>(leaf, nd->start, je, qv, A.P, acc);
         } else {
-          l3_term<KIND>(nd->contrib, qx, qy, qz, A.P, acc);
+          l3_term<KIND>(nd->contrib, qv, A.P, acc);
         }
       } else {
-        const int b = next_bit(nd, 0);
-        if (b < kCells && nd->child[b] < num_nodes) {
+        const int b = next_bit<DIM>(nd, 0);
+        if (b < kN && nd->child[b] < num_nodes) {
           node = static_cast<uint32_t>(nd->child[b]);
           continue;
         }
@@ -341,12 +427,13 @@ __global__ void k_l3_walk(const Leaf3* __restrict__ leaf, const Node3* __restric
     }
     if (node == 0) break;
     // next sibling: the parent's next present cell after this node's cell
-    const Node3* parent = &nodes[nd->parent_idx < num_nodes ? nd->parent_idx : 0];
-    float mid[3];
-    for (int a = 0; a < 3; ++a) mid[a] = (nd->bmin[a] + nd->bmax[a]) / 2.0f;
-    const int cell = cell_in(parent, mid);
-    const int b = cell < 0 ? kCells : next_bit(parent, cell + 1);
-    if (b >= kCells || parent->child[b] >= num_nodes) {
+    const Node* parent = &nodes[nd->parent_idx < num_nodes ? nd->parent_idx : 0];
+    float mid[DIM];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) mid[a] = (nd->bmin[a] + nd->bmax[a]) / 2.0f;
+    const int cell = cell_in<DIM>(parent, mid);
+    const int b = cell < 0 ? kN : next_bit<DIM>(parent, cell + 1);
+    if (b >= kN || parent->child[b] >= num_nodes) {
       node = static_cast<uint32_t>(nd->parent_idx < num_nodes ? nd->parent_idx : 0);
       from_child = true;
     } else {
@@ -354,88 +441,165 @@ __global__ void k_l3_walk(const Leaf3* __restrict__ leaf, const Node3* __restric
       from_child = false;
     }
   }
-  if constexpr (KIND == kPotential) {
-    out[q] = acc[0];
-  } else {
-    out[3 * static_cast<int64_t>(q) + 0] = acc[0];
-    out[3 * static_cast<int64_t>(q) + 1] = acc[1];
-    out[3 * static_cast<int64_t>(q) + 2] = acc[2];
-  }
+  store_out<DIM, KIND>(out, q, acc);
 }
 
-// Stochastic walk over level-3 buffers (include/tree_sampler_gpu.h:25-140):
-// the subdomains are the root's children, which are all leaves in buffers
-// from the level-3 builder, so every subdomain is summed exactly and no
-// sample path has a step; the estimate is divided by samples_per_subdomain
-// as in the reference (0 samples give 0 / 0).  A non-leaf subdomain
-// contributes its monopole only.
-template <int KIND>
-__global__ __launch_bounds__(kB) void k_l3_stoch(const Leaf3* __restrict__ leaf,
-                                                 const Node3* __restrict__ nodes,
-                                                 const Leaf3* __restrict__ contrib,
-                                                 const float* __restrict__ queries, int nq,
-                                                 float* __restrict__ out, o8::WalkParams P,
-                                                 int samples, uint32_t num_nodes,
-                                                 uint32_t num_leaves, uint32_t num_contrib) {
-  const int q = blockIdx.x * kB + threadIdx.x;
+// Node of the child cell of `node` that holds p, skipping children that have
+// exactly one child (Tree2DNode / Tree3DNode::containedInChild(p),
+// include/tree2d.h:168-179); -1 when p is outside a box on the way (the
+// reference returns nullptr there).  An absent cell reads child id 0, as the
+// reference's zero-filled child table does.
+template <int DIM>
+__device__ __forceinline__ int64_t contained_child(const typename L3T<DIM>::Node* __restrict__ nodes,
+                                                   uint64_t node, const float* p,
+                                                   uint32_t num_nodes) {
+  for (int hop = 0; hop < 128; ++hop) {
+    const int cell = cell_in<DIM>(&nodes[node], p);
+    if (cell < 0) return -1;
+    const uint64_t ch = nodes[node].child[cell];
+    if (ch >= num_nodes) return -1;
+    if (nodes[ch].num_children != 1) return static_cast<int64_t>(ch);
+    node = ch;
+  }
+  return -1;
+}
+
+// Multi-level prefix control-variate walk (include/tree_sampler_gpu.h:25-140).
+// Subdomains are the root's children: each adds its exact sum (leaf) or its
+// monopole, plus, per sample, a sample point's roulette path whose steps add
+// n / (P count(prev)) (sum over prev's children - prev's monopole).
+// Random numbers: Philox4x32-10 in place of cuRAND XORWOW.  As in the
+// reference, the sample point is shared by all queries of a block (its
+// stream depends only on the block index) and the roulette stream on
+// seed + query.
+template <int DIM, int KIND>
+__global__ void k_l3_stoch(const typename L3T<DIM>::Leaf* __restrict__ leaf,
+                           const typename L3T<DIM>::Node* __restrict__ nodes,
+                           const typename L3T<DIM>::Leaf* __restrict__ contrib,
+                           const float* __restrict__ queries, int nq, float* __restrict__ out,
+                           o8::WalkParams P, int samples, int seed, uint32_t num_nodes,
+                           uint32_t num_leaves, uint32_t num_contrib) {
+  using Node = typename L3T<DIM>::Node;
+  constexpr int kN = L3T<DIM>::kCellsN;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
-  const float qx = queries[3 * static_cast<int64_t>(q)];
-  const float qy = queries[3 * static_cast<int64_t>(q) + 1];
-  const float qz = queries[3 * static_cast<int64_t>(q) + 2];
-  float acc[3] = {0.f, 0.f, 0.f};
-  float mlcv[3] = {0.f, 0.f, 0.f};
-  const Node3* root = &nodes[0];
-  for (int b = next_bit(root, 0); b < kCells; b = next_bit(root, b + 1)) {
+  float qv[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
+  float acc[DIM] = {};
+  float mlcv[DIM] = {};
+  const uint32_t rseed = static_cast<uint32_t>(seed) + static_cast<uint32_t>(q);
+  const Node* root = &nodes[0];
+  for (int b = next_bit<DIM>(root, 0); b < kN; b = next_bit<DIM>(root, b + 1)) {
     const uint64_t ci = root->child[b];
     if (ci >= num_nodes) continue;
-    const Node3* sub = &nodes[ci];
+    const Node* sub = &nodes[ci];
     if (sub->is_leaf) {
       const uint64_t je = sub->end < num_leaves ? sub->end : num_leaves;
-      l3_leaf_sum<KIND>(leaf, sub->start, je, qx, qy, qz, P, acc);
+      l3_leaf_sum<KIND>(leaf, sub->start, je, qv, P, acc);
     } else {
-      l3_term<KIND>(ci < num_contrib ? contrib[ci] : sub->contrib, qx, qy, qz, P, acc);
+      l3_term<KIND>(sub->contrib, qv, P, acc);
     }
-    const float inv_s = 1.0f / static_cast<float>(samples);
-    for (int a = 0; a < 3; ++a) mlcv[a] += 0.0f * inv_s;  // 0 / samples, as the reference
+    const uint64_t count = sub->end - sub->start;
+    float total[DIM] = {};
+    for (int ss = 0; ss < samples; ++ss) {
+      const float u2 = sbh_uniform(blockIdx.x, 1u, static_cast<uint32_t>(ci),
+                                   static_cast<uint32_t>(ss), 0u);
+      uint64_t idx = static_cast<uint64_t>(
+          fmaf(static_cast<float>(count), u2, static_cast<float>(sub->start)));
+      if (idx > sub->end - 1) idx = sub->end - 1;
+      if (count == 0 || idx >= num_leaves) continue;
+      float sp[DIM];
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) sp[a] = leaf[idx].pos[a];
+      uint64_t cur = ci;
+      float event_prob = 1.0f;
+      float wsum[DIM] = {};
+      float cur_beta = fmaxf(1.0f, far_ratio<DIM>(sub, qv));
+      // a path ends at a leaf; depth <= 100 in trees from the builders
+      for (uint32_t step = 0; step < 1024u && !nodes[cur].is_leaf; ++step) {
+        const Node* cn = &nodes[cur];
+        const float ffr = far_ratio<DIM>(cn, qv);
+        const float tp = fminf(cur_beta / ffr, 1.0f);
+        const float pu = sbh_uniform(rseed, 0u, static_cast<uint32_t>(ci),
+                                     static_cast<uint32_t>(ss), step);
+        if (pu > tp) break;
+        event_prob *= tp;
+        const int64_t nxt = contained_child<DIM>(nodes, cur, sp, num_nodes);
+        cur_beta = fmaxf(cur_beta, ffr);
+        float prev_res[DIM] = {}, cur_res[DIM] = {};
+        l3_term<KIND>(cn->contrib, qv, P, prev_res);
+        const int fb = next_bit<DIM>(cn, 0);
+        const uint64_t first = fb < kN ? cn->child[fb] : 0ull;
+        for (int32_t o = 0; o < cn->num_children; ++o) {
+          const uint64_t c = first + static_cast<uint64_t>(o);
+          if (c >= num_nodes) break;
+          const Node* ch = &nodes[c];
+          if (ch->is_leaf) {
+            const uint64_t je = ch->end < num_leaves ? ch->end : num_leaves;
+            l3_leaf_sum<KIND>(leaf, ch->start, je, qv, P, cur_res);
+          } else {
+            l3_term<KIND>(c < num_contrib ? contrib[c] : ch->contrib, qv, P, cur_res);
+          }
+        }
+        const float w = static_cast<float>(count) /
+                        (event_prob * static_cast<float>(cn->end - cn->start));
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) wsum[a] += w * (cur_res[a] - prev_res[a]);
+        if (nxt < 0) break;  // the reference would dereference nullptr here
+        cur = static_cast<uint64_t>(nxt);
+      }
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) total[a] += wsum[a];
+    }
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) mlcv[a] += total[a] / static_cast<float>(samples);
   }
-  if constexpr (KIND == kPotential) {
-    out[q] = acc[0] + mlcv[0];
-  } else {
-    for (int a = 0; a < 3; ++a) out[3 * static_cast<int64_t>(q) + a] = acc[a] + mlcv[a];
-  }
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) acc[a] += mlcv[a];
+  store_out<DIM, KIND>(out, q, acc);
 }
 
+template <int DIM>
 int check_l3_walk(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                   const SbhBuffer* contrib_bytes, const SbhBuffer* queries,
                   const SbhBuffer* out, int threads, bool vec_out) {
+  using Node = typename L3T<DIM>::Node;
+  using Leaf = typename L3T<DIM>::Leaf;
   if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || !buffer_ok(node_bytes, SBH_DTYPE_U8, -1))
     return fail_invalid("leaf_bytes/node_bytes must be uint8 device buffers");
-  if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf3)) != 0)
+  if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf)) != 0)
     return fail_invalid("leaf_bytes size mismatch");
-  if (byte_count(node_bytes) % static_cast<int64_t>(sizeof(Node3)) != 0 ||
+  if (byte_count(node_bytes) % static_cast<int64_t>(sizeof(Node)) != 0 ||
       byte_count(node_bytes) == 0)
     return fail_invalid("node_bytes size mismatch");
+  if (byte_count(node_bytes) / static_cast<int64_t>(sizeof(Node)) > INT32_MAX)
+    return fail_invalid("too many nodes");
   if (contrib_bytes != nullptr &&
       (!buffer_ok(contrib_bytes, SBH_DTYPE_U8, -1) ||
-       byte_count(contrib_bytes) % static_cast<int64_t>(sizeof(Leaf3)) != 0))
+       byte_count(contrib_bytes) % static_cast<int64_t>(sizeof(Leaf)) != 0))
     return fail_invalid("contrib_bytes size mismatch");
-  if (!buffer_ok(queries, SBH_DTYPE_F32, 2)) return fail_invalid("queries must be (M, 3) float32");
-  if (queries->dims[1] != 3) return fail_invalid("queries dim must be 3");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2))
+    return fail_invalid(DIM == 2 ? "queries must be (M, 2) float32" : "queries must be (M, 3) float32");
+  if (queries->dims[1] != DIM)
+    return fail_invalid(DIM == 2 ? "queries dim must be 2" : "queries dim must be 3");
   if (queries->dims[0] > INT32_MAX) return fail_invalid("too many queries");
   if (vec_out ? (!buffer_ok(out, SBH_DTYPE_F32, 2) || out->dims[0] != queries->dims[0] ||
-                 out->dims[1] != 3)
+                 out->dims[1] != DIM)
               : (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != queries->dims[0]))
     return fail_invalid("out shape mismatch");
   if (threads <= 0 || threads > 1024) return fail_invalid("threads must be in [1, 1024]");
   return SBH_OK;
 }
 
-template <int KIND>
+template <int DIM, int KIND>
 int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                    const SbhBuffer* queries, SbhBuffer* out, float beta, float softening,
                    float cutoff, int periodic, float box, int threads) {
-  int rc = check_l3_walk(leaf_bytes, node_bytes, nullptr, queries, out, threads,
-                         KIND != kPotential);
+  using Node = typename L3T<DIM>::Node;
+  using Leaf = typename L3T<DIM>::Leaf;
+  int rc = check_l3_walk<DIM>(leaf_bytes, node_bytes, nullptr, queries, out, threads,
+                              KIND != kPotential);
   if (rc != SBH_OK) return rc;
   const int nq = static_cast<int>(queries->dims[0]);
   if (nq == 0) return SBH_OK;
@@ -447,34 +611,38 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   A.beta = beta;
   A.threads = threads;
   const unsigned blocks = static_cast<unsigned>((nq + threads - 1) / threads);
-  k_l3_walk<KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
-      static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node3*>(node_bytes->data),
+  k_l3_walk<DIM, KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const Leaf*>(leaf_bytes->data), static_cast<const Node*>(node_bytes->data),
       static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
-      static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node3)),
-      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf3)));
+      static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node)),
+      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf)));
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }
 
-template <int KIND>
+template <int DIM, int KIND>
 int launch_l3_stoch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                     const SbhBuffer* contrib_bytes, const SbhBuffer* queries, SbhBuffer* out,
-                    int samples, float softening, float cutoff, int periodic, float box,
-                    int threads) {
-  int rc = check_l3_walk(leaf_bytes, node_bytes, contrib_bytes, queries, out, threads,
-                         KIND != kPotential);
+                    int samples, int seed, float softening, float cutoff, int periodic,
+                    float box, int threads) {
+  using Node = typename L3T<DIM>::Node;
+  using Leaf = typename L3T<DIM>::Leaf;
+  int rc = check_l3_walk<DIM>(leaf_bytes, node_bytes, contrib_bytes, queries, out, threads,
+                              KIND != kPotential);
   if (rc != SBH_OK) return rc;
+  if (contrib_bytes == nullptr) return fail_invalid("contrib_bytes must be given");
   const int nq = static_cast<int>(queries->dims[0]);
   if (nq == 0) return SBH_OK;
   o8::WalkParams P = o8::make_walk_params(1.0f, softening, cutoff, 0, 0.f, periodic, box);
-  P.cutoff = cutoff > 0.0f ? 1 : 0;
-  k_l3_stoch<KIND><<<grid_for(nq, kB), kB, 0, static_cast<hipStream_t>(stream)>>>(
-      static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node3*>(node_bytes->data),
-      static_cast<const Leaf3*>(contrib_bytes->data), static_cast<const float*>(queries->data),
-      nq, static_cast<float*>(out->data), P, samples,
-      static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node3)),
-      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf3)),
-      static_cast<uint32_t>(byte_count(contrib_bytes) / sizeof(Leaf3)));
+  P.cutoff = (cutoff > 0.0f && std::isfinite(cutoff)) ? 1 : 0;
+  const unsigned blocks = static_cast<unsigned>((nq + threads - 1) / threads);
+  k_l3_stoch<DIM, KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const Leaf*>(leaf_bytes->data), static_cast<const Node*>(node_bytes->data),
+      static_cast<const Leaf*>(contrib_bytes->data), static_cast<const float*>(queries->data),
+      nq, static_cast<float*>(out->data), P, samples, seed,
+      static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node)),
+      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf)),
+      static_cast<uint32_t>(byte_count(contrib_bytes) / sizeof(Leaf)));
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }
@@ -567,8 +735,8 @@ extern "C" int sbh_barnes_hut_gravity_3d(void* stream, const SbhBuffer* leaf_byt
                                          SbhBuffer* out, float beta, int32_t periodic,
                                          float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
-                                    0.f, periodic, box_length, threads);
+  return launch_l3_walk<3, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+                                       0.f, periodic, box_length, threads);
 }
 
 extern "C" int sbh_barnes_hut_force_3d(void* stream, const SbhBuffer* leaf_bytes,
@@ -576,8 +744,8 @@ extern "C" int sbh_barnes_hut_force_3d(void* stream, const SbhBuffer* leaf_bytes
                                        SbhBuffer* out, float beta, int32_t periodic,
                                        float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f, 0.f,
-                                periodic, box_length, threads);
+  return launch_l3_walk<3, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+                                   0.f, periodic, box_length, threads);
 }
 
 extern "C" int sbh_barnes_hut_force_soft_3d(void* stream, const SbhBuffer* leaf_bytes,
@@ -587,9 +755,9 @@ extern "C" int sbh_barnes_hut_force_soft_3d(void* stream, const SbhBuffer* leaf_
                                             float cutoff_scale, int32_t periodic,
                                             float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
-                                     softening_scale, cutoff_scale, periodic, box_length,
-                                     threads);
+  return launch_l3_walk<3, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
+                                        softening_scale, cutoff_scale, periodic, box_length,
+                                        threads);
 }
 
 extern "C" int sbh_stochastic_bh_gravity_3d(void* stream, const SbhBuffer* leaf_bytes,
@@ -600,10 +768,9 @@ extern "C" int sbh_stochastic_bh_gravity_3d(void* stream, const SbhBuffer* leaf_
                                             int32_t periodic, float box_length,
                                             int32_t threads) {
   clear_error();
-  (void)seed;
-  return launch_l3_stoch<kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
-                                     out, samples_per_subdomain, 0.f, 0.f, periodic, box_length,
-                                     threads);
+  return launch_l3_stoch<3, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+                                        out, samples_per_subdomain, seed, 0.f, 0.f, periodic,
+                                        box_length, threads);
 }
 
 extern "C" int sbh_stochastic_bh_force_3d(void* stream, const SbhBuffer* leaf_bytes,
@@ -613,10 +780,9 @@ extern "C" int sbh_stochastic_bh_force_3d(void* stream, const SbhBuffer* leaf_by
                                           int32_t samples_per_subdomain, int32_t seed,
                                           int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  (void)seed;
-  return launch_l3_stoch<kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
-                                 samples_per_subdomain, 0.f, 0.f, periodic, box_length,
-                                 threads);
+  return launch_l3_stoch<3, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
+                                    samples_per_subdomain, seed, 0.f, 0.f, periodic, box_length,
+                                    threads);
 }
 
 extern "C" int sbh_stochastic_bh_force_soft_3d(
@@ -625,8 +791,73 @@ extern "C" int sbh_stochastic_bh_force_soft_3d(
     int32_t samples_per_subdomain, int32_t seed, float softening_scale, float cutoff_scale,
     int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  (void)seed;
-  return launch_l3_stoch<kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
-                                      out, samples_per_subdomain, softening_scale, cutoff_scale,
-                                      periodic, box_length, threads);
+  return launch_l3_stoch<3, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+                                         out, samples_per_subdomain, seed, softening_scale,
+                                         cutoff_scale, periodic, box_length, threads);
+}
+
+// 2-D targets over Tree2DNode<1, 3> buffers (src/sbh_ffi.cu:2444-2669,
+// 2779-3064): the same walks with two coordinates.
+extern "C" int sbh_barnes_hut_gravity(void* stream, const SbhBuffer* leaf_bytes,
+                                      const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                      SbhBuffer* out, float beta, int32_t periodic,
+                                      float box_length, int32_t threads) {
+  clear_error();
+  return launch_l3_walk<2, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+                                       0.f, periodic, box_length, threads);
+}
+
+extern "C" int sbh_barnes_hut_force(void* stream, const SbhBuffer* leaf_bytes,
+                                    const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                    SbhBuffer* out, float beta, int32_t periodic,
+                                    float box_length, int32_t threads) {
+  clear_error();
+  return launch_l3_walk<2, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+                                   0.f, periodic, box_length, threads);
+}
+
+extern "C" int sbh_barnes_hut_force_soft(void* stream, const SbhBuffer* leaf_bytes,
+                                         const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                                         SbhBuffer* out, float beta, float softening_scale,
+                                         float cutoff_scale, int32_t periodic,
+                                         float box_length, int32_t threads) {
+  clear_error();
+  return launch_l3_walk<2, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
+                                        softening_scale, cutoff_scale, periodic, box_length,
+                                        threads);
+}
+
+extern "C" int sbh_stochastic_bh_gravity(void* stream, const SbhBuffer* leaf_bytes,
+                                         const SbhBuffer* node_bytes,
+                                         const SbhBuffer* contrib_bytes,
+                                         const SbhBuffer* queries, SbhBuffer* out,
+                                         int32_t samples_per_subdomain, int32_t seed,
+                                         int32_t periodic, float box_length, int32_t threads) {
+  clear_error();
+  return launch_l3_stoch<2, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+                                        out, samples_per_subdomain, seed, 0.f, 0.f, periodic,
+                                        box_length, threads);
+}
+
+extern "C" int sbh_stochastic_bh_force(void* stream, const SbhBuffer* leaf_bytes,
+                                       const SbhBuffer* node_bytes,
+                                       const SbhBuffer* contrib_bytes, const SbhBuffer* queries,
+                                       SbhBuffer* out, int32_t samples_per_subdomain,
+                                       int32_t seed, int32_t periodic, float box_length,
+                                       int32_t threads) {
+  clear_error();
+  return launch_l3_stoch<2, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
+                                    samples_per_subdomain, seed, 0.f, 0.f, periodic, box_length,
+                                    threads);
+}
+
+extern "C" int sbh_stochastic_bh_force_soft(
+    void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+    const SbhBuffer* contrib_bytes, const SbhBuffer* queries, SbhBuffer* out,
+    int32_t samples_per_subdomain, int32_t seed, float softening_scale, float cutoff_scale,
+    int32_t periodic, float box_length, int32_t threads) {
+  clear_error();
+  return launch_l3_stoch<2, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+                                         out, samples_per_subdomain, seed, softening_scale,
+                                         cutoff_scale, periodic, box_length, threads);
 }

@@ -55,7 +55,7 @@ def load_library(path: Optional[str] = None) -> ctypes.CDLL:
     lib = ctypes.CDLL(p)
     lib.sbh_last_error.restype = ctypes.c_char_p
     for name in ("sbh_leaf3_size", "sbh_node3_size", "sbh_octree_level3_max_nodes",
-                 "sbh_octree8_node_size"):
+                 "sbh_octree8_node_size", "sbh_leaf2_size", "sbh_node2_size"):
         getattr(lib, name).restype = ctypes.c_size_t
     if path is None:
         _lib = lib

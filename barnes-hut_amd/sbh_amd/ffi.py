@@ -14,6 +14,7 @@ The reference's `jax.custom_vjp` recompute wrappers are mirrored as
 """
 from __future__ import annotations
 
+import ctypes
 from typing import Callable, Optional
 
 import numpy as np
@@ -627,21 +628,65 @@ def _l3_queries(queries):
     queries = _f32(queries)
     if queries.dim() != 2 or queries.shape[1] not in (2, 3):
         raise ValueError("queries must be (M, 2) or (M, 3)")
-    if queries.shape[1] == 2:
-        raise ValueError("2-D quadtree targets are not part of this library (3-D only)")
     return queries
+
+
+def _tree_buffers(points, normals, masses, min_corner, max_corner, alpha, dim):
+    """Tree2D<1,3> / Tree3D<1,3> bytes (the reference's sbh.Quadtree3 /
+    sbh.Octree3, include/tree2d.h:237-407, include/tree3d.h:238-413), built on
+    the device: (leaf, node, contrib) uint8 tensors."""
+    points, normals, masses = _f32(points), _f32(normals), _f32(masses)
+    if points.dim() != 2 or points.shape[1] != dim:
+        raise ValueError(f"points must have shape (N, {dim})")
+    if normals.shape != points.shape:
+        raise ValueError("normals shape mismatch")
+    if masses.dim() != 1 or masses.shape[0] != points.shape[0]:
+        raise ValueError("masses must have shape (N,)")
+    mn, mx = _default_corners(points, min_corner, max_corner)
+    nodes = ctypes.c_int64(0)
+    s = stream_handle()
+    call("sbh_tree_build", s, i32(dim), buf(points), buf(normals), buf(masses), buf(mn),
+         buf(mx), f32(alpha), ctypes.byref(nodes))
+    lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+    dev = points.device
+    leaf = torch.empty((points.shape[0] * lsz,), dtype=torch.uint8, device=dev)
+    node = torch.empty((nodes.value * nsz,), dtype=torch.uint8, device=dev)
+    contrib = torch.empty((nodes.value * lsz,), dtype=torch.uint8, device=dev)
+    call("sbh_tree_emit", s, buf(leaf), buf(node), buf(contrib))
+    return leaf, node, contrib
+
+
+def build_quadtree_buffers_with_contrib(points, normals, masses, min_corner=None,
+                                        max_corner=None, *, alpha: float = 1.0, level: int = 3):
+    """python/jax_sbh_ffi.py:692-724 (Tree2DNode<1,3> buffers of sbh.Quadtree3,
+    built on the device)."""
+    if level != 3:
+        raise ValueError("level must be 3 (FFI expects Tree2DNode<1, 3>)")
+    return _tree_buffers(points, normals, masses, min_corner, max_corner, alpha, 2)
+
+
+def build_quadtree_buffers(points, normals, masses, min_corner=None, max_corner=None, *,
+                           alpha: float = 1.0, level: int = 3):
+    """python/jax_sbh_ffi.py:658-689."""
+    leaf, node, _ = build_quadtree_buffers_with_contrib(points, normals, masses, min_corner,
+                                                        max_corner, alpha=alpha, level=level)
+    return leaf, node
 
 
 def build_octree_buffers_with_contrib(points, normals, masses, min_corner=None,
                                       max_corner=None, *, alpha: float = 1.0, level: int = 3,
                                       backend: str = "ffi"):
-    """python/jax_sbh_ffi.py:805-861 (level-3 Tree3DNode<1,3> buffers)."""
+    """python/jax_sbh_ffi.py:805-861: backend "ffi" gives the level-3 buffers
+    (leaf N*32, node 513*4272, contrib 513*32 bytes); backend "cpu" the
+    multi-level Tree3D<1,3> buffers of sbh.Octree3 (alpha 1, as the reference
+    passes), here built on the device."""
     del alpha
     if level != 3:
         raise ValueError("level must be 3 (FFI expects Tree3DNode<1, 3>)")
+    if backend == "cpu":
+        return _tree_buffers(points, normals, masses, min_corner, max_corner, 1.0, 3)
     if backend != "ffi":
-        raise ValueError("backend must be 'ffi' (the CPU tree module is not part of this "
-                         "library)")
+        raise ValueError("backend must be 'ffi' or 'cpu'")
     leaf_size, node_size, max_nodes = _get_octree_level3_layout()
     points, normals, masses = _f32(points), _f32(normals), _f32(masses)
     mn, mx = _default_corners(points, min_corner, max_corner)
@@ -677,7 +722,8 @@ def barnes_hut_gravity(leaf_bytes, node_bytes, queries, *, beta: float = 2.0,
     _check_periodic(periodic, box_length)
     _check_threads(threads)
     out = torch.empty((queries.shape[0],), dtype=torch.float32, device=queries.device)
-    call("sbh_barnes_hut_gravity_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    target = "sbh_barnes_hut_gravity_3d" if queries.shape[1] == 3 else "sbh_barnes_hut_gravity"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(queries), buf(out), f32(beta), i32(1 if periodic else 0), f32(box_length),
          i32(threads))
     return out
@@ -690,8 +736,9 @@ def barnes_hut_force(leaf_bytes, node_bytes, queries, *, beta: float = 2.0,
     leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _l3_queries(queries)
     _check_periodic(periodic, box_length)
     _check_threads(threads)
-    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
-    call("sbh_barnes_hut_force_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    out = torch.empty(tuple(queries.shape), dtype=torch.float32, device=queries.device)
+    target = "sbh_barnes_hut_force_3d" if queries.shape[1] == 3 else "sbh_barnes_hut_force"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(queries), buf(out), f32(beta), i32(1 if periodic else 0), f32(box_length),
          i32(threads))
     return out
@@ -705,8 +752,10 @@ def softened_barnes_hut_force(leaf_bytes, node_bytes, queries, *, beta: float = 
     leaf_bytes, node_bytes, queries = _u8(leaf_bytes), _u8(node_bytes), _l3_queries(queries)
     _check_periodic(periodic, box_length)
     _check_threads(threads)
-    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
-    call("sbh_barnes_hut_force_soft_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    out = torch.empty(tuple(queries.shape), dtype=torch.float32, device=queries.device)
+    target = "sbh_barnes_hut_force_soft_3d" if queries.shape[1] == 3 else \
+        "sbh_barnes_hut_force_soft"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(queries), buf(out), f32(beta), f32(softening_scale), f32(cutoff_scale),
          i32(1 if periodic else 0), f32(box_length), i32(threads))
     return out
@@ -722,7 +771,9 @@ def stochastic_barnes_hut_gravity(leaf_bytes, node_bytes, contrib_bytes, queries
     _check_periodic(periodic, box_length)
     _check_threads(threads)
     out = torch.empty((queries.shape[0],), dtype=torch.float32, device=queries.device)
-    call("sbh_stochastic_bh_gravity_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    target = "sbh_stochastic_bh_gravity_3d" if queries.shape[1] == 3 else \
+        "sbh_stochastic_bh_gravity"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
          i32(1 if periodic else 0), f32(box_length), i32(threads))
     return out
@@ -737,8 +788,10 @@ def stochastic_barnes_hut_force(leaf_bytes, node_bytes, contrib_bytes, queries, 
     contrib_bytes, queries = _u8(contrib_bytes), _l3_queries(queries)
     _check_periodic(periodic, box_length)
     _check_threads(threads)
-    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
-    call("sbh_stochastic_bh_force_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    out = torch.empty(tuple(queries.shape), dtype=torch.float32, device=queries.device)
+    target = "sbh_stochastic_bh_force_3d" if queries.shape[1] == 3 else \
+        "sbh_stochastic_bh_force"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
          i32(1 if periodic else 0), f32(box_length), i32(threads))
     return out
@@ -755,8 +808,10 @@ def softened_stochastic_barnes_hut_force(leaf_bytes, node_bytes, contrib_bytes, 
     contrib_bytes, queries = _u8(contrib_bytes), _l3_queries(queries)
     _check_periodic(periodic, box_length)
     _check_threads(threads)
-    out = torch.empty((queries.shape[0], 3), dtype=torch.float32, device=queries.device)
-    call("sbh_stochastic_bh_force_soft_3d", stream_handle(), buf(leaf_bytes), buf(node_bytes),
+    out = torch.empty(tuple(queries.shape), dtype=torch.float32, device=queries.device)
+    target = "sbh_stochastic_bh_force_soft_3d" if queries.shape[1] == 3 else \
+        "sbh_stochastic_bh_force_soft"
+    call(target, stream_handle(), buf(leaf_bytes), buf(node_bytes),
          buf(contrib_bytes), buf(queries), buf(out), i32(samples_per_subdomain), i32(seed),
          f32(softening_scale), f32(cutoff_scale), i32(1 if periodic else 0), f32(box_length),
          i32(threads))

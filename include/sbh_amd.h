@@ -205,12 +205,16 @@ int sbh_stochastic_bh_force_octree8_soft_vjp(
  * One 8 x 8 x 8 split under the root (cells on each axis of the given box);
  * leaf N*32, node 513*4272 and contrib 513*32 bytes.  The walks are the
  * reference's warp-voted traversal (a node is a far field only if every
- * active query of a 32-query warp of a `threads`-sized block accepts it);
- * leaves always accept, so a warp uses either the root monopole or the exact
- * sum.  The stochastic variants sum every (leaf) subdomain exactly and divide
- * the zero correction by samples_per_subdomain (0 samples give NaN, as the
- * reference); `seed` is accepted and unused because no sample path has a
- * step in level-3 buffers. */
+ * active query of a 32-query warp of a `threads`-sized block accepts it) and
+ * its multi-level control-variate sampler; they take any Tree3DNode<1,3>
+ * buffer, also the multi-level trees of sbh_tree_build (dim 3).  For level-3
+ * buffers leaves always accept, so a warp uses either the root monopole or
+ * the exact sum, and the stochastic variants sum every (leaf) subdomain
+ * exactly and divide the zero correction by samples_per_subdomain (0 samples
+ * give NaN, as the reference).  Sample paths draw Philox4x32-10 numbers in
+ * place of cuRAND XORWOW: the sample point from a stream of the block index
+ * (shared by a block's queries, as the reference's), the roulette from a
+ * stream of seed + query. */
 
 /* Replaces BuildOctreeLevel3Buffers (src/sbh_ffi.cu:736-1037; target
  * "sbh_build_octree_level3_buffers", python/jax_sbh_ffi.py:744-861).  Points
@@ -270,6 +274,92 @@ int sbh_stochastic_bh_force_soft_3d(void* stream, const SbhBuffer* leaf_bytes,
                                     int32_t samples_per_subdomain, int32_t seed,
                                     float softening_scale, float cutoff_scale,
                                     int32_t periodic, float box_length, int32_t threads);
+
+/* ---- 2-D quadtree targets (Tree2DNode<1,3> buffers) -----------------------
+ *
+ * Leaf records LeafData<2> (24 B: position[2], normal[2], mass, alpha), node
+ * records Tree2DNode<1,3> (616 B: 8 x 8 cells, cell = x + 8 y), contrib one
+ * LeafData<2> per node.  The walks are the level-3 walks above with two
+ * coordinates: queries (M, 2), forces (M, 2), potentials (M,). */
+
+/* Replaces BarnesHutGravity (src/sbh_ffi.cu:2444-2488; target
+ * "sbh_barnes_hut_gravity", python/jax_sbh_ffi.py:963-997). */
+int sbh_barnes_hut_gravity(void* stream, const SbhBuffer* leaf_bytes,
+                           const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                           SbhBuffer* out, float beta, int32_t periodic, float box_length,
+                           int32_t threads);
+
+/* Replaces BarnesHutForce (src/sbh_ffi.cu:2563-2611; target
+ * "sbh_barnes_hut_force", python/jax_sbh_ffi.py:1000-1034). */
+int sbh_barnes_hut_force(void* stream, const SbhBuffer* leaf_bytes,
+                         const SbhBuffer* node_bytes, const SbhBuffer* queries, SbhBuffer* out,
+                         float beta, int32_t periodic, float box_length, int32_t threads);
+
+/* Replaces SoftenedBarnesHutForce (src/sbh_ffi.cu:2613-2669; target
+ * "sbh_barnes_hut_force_soft", python/jax_sbh_ffi.py:1063-1105). */
+int sbh_barnes_hut_force_soft(void* stream, const SbhBuffer* leaf_bytes,
+                              const SbhBuffer* node_bytes, const SbhBuffer* queries,
+                              SbhBuffer* out, float beta, float softening_scale,
+                              float cutoff_scale, int32_t periodic, float box_length,
+                              int32_t threads);
+
+/* Replaces StochasticBarnesHutGravity (src/sbh_ffi.cu:2779-2832; target
+ * "sbh_stochastic_bh_gravity", python/jax_sbh_ffi.py:1138-1177). */
+int sbh_stochastic_bh_gravity(void* stream, const SbhBuffer* leaf_bytes,
+                              const SbhBuffer* node_bytes, const SbhBuffer* contrib_bytes,
+                              const SbhBuffer* queries, SbhBuffer* out,
+                              int32_t samples_per_subdomain, int32_t seed, int32_t periodic,
+                              float box_length, int32_t threads);
+
+/* Replaces StochasticBarnesHutForce (src/sbh_ffi.cu:2889-2942; target
+ * "sbh_stochastic_bh_force", python/jax_sbh_ffi.py:1180-1219). */
+int sbh_stochastic_bh_force(void* stream, const SbhBuffer* leaf_bytes,
+                            const SbhBuffer* node_bytes, const SbhBuffer* contrib_bytes,
+                            const SbhBuffer* queries, SbhBuffer* out,
+                            int32_t samples_per_subdomain, int32_t seed, int32_t periodic,
+                            float box_length, int32_t threads);
+
+/* Replaces SoftenedStochasticBarnesHutForce (src/sbh_ffi.cu:2999-3064; target
+ * "sbh_stochastic_bh_force_soft", python/jax_sbh_ffi.py:1252-1299). */
+int sbh_stochastic_bh_force_soft(void* stream, const SbhBuffer* leaf_bytes,
+                                 const SbhBuffer* node_bytes, const SbhBuffer* contrib_bytes,
+                                 const SbhBuffer* queries, SbhBuffer* out,
+                                 int32_t samples_per_subdomain, int32_t seed,
+                                 float softening_scale, float cutoff_scale, int32_t periodic,
+                                 float box_length, int32_t threads);
+
+/* ---- CPU-layout trees: Tree2D<1,3> and Tree3D<1,3> -------------------------
+ *
+ * Replaces the reference's host builders sbh.Quadtree3 / sbh.Octree3
+ * (include/tree2d.h:237-407, include/tree3d.h:238-413, include/py_tree.h:
+ * 20-73; their byte exports src/sbh.cpp:227-235), used by
+ * build_quadtree_buffers(_with_contrib) and build_octree_buffers(backend=
+ * "cpu") (python/jax_sbh_ffi.py:658-742).  Leaf size 1, maximum depth 100,
+ * 8^dim cells per split, nodes in breadth-first order with each node's
+ * children contiguous in cell order, moments aggregated bottom-up in fp32.
+ * Built on the GPU in two calls, because the node count sizes the caller's
+ * node buffer:
+ *   sbh_tree_build: points (N, dim), normals (N, dim), masses (N,),
+ *     min_corner / max_corner (dim,), all device float32; `alpha` is every
+ *     point's alpha.  Builds into state kept per (device, stream),
+ *     synchronizes the stream and returns the node count.  Not capturable.
+ *   sbh_tree_emit: writes the last tree built on `stream`: leaf_out N*24 /
+ *     N*32 bytes, node_out num_nodes*616 / num_nodes*4272 bytes, contrib_out
+ *     num_nodes*24 / num_nodes*32 bytes (may be NULL).
+ * Points that share a cell for 100 levels form one leaf; their order there
+ * is the input order (the reference's std::partition leaves it unspecified);
+ * elsewhere every leaf holds one point and the bytes equal the reference's.
+ * Limits: N <= 2^26 (dim 2) or 2^23 (dim 3). */
+int sbh_tree_build(void* stream, int32_t dim, const SbhBuffer* points,
+                   const SbhBuffer* normals, const SbhBuffer* masses,
+                   const SbhBuffer* min_corner, const SbhBuffer* max_corner, float alpha,
+                   int64_t* num_nodes);
+int sbh_tree_emit(void* stream, SbhBuffer* leaf_out, SbhBuffer* node_out,
+                  SbhBuffer* contrib_out);
+
+/* sizeof(LeafData<2>) = 24 and sizeof(Tree2DNode<1,3>) = 616. */
+size_t sbh_leaf2_size(void);
+size_t sbh_node2_size(void);
 
 /* ---- winding numbers (K1) -------------------------------------------------
  *

@@ -120,6 +120,18 @@ def _declare(L: ctypes.CDLL) -> None:
                               ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int, ctypes.c_float, _f64p]
     L.orc_num_threads.restype = ctypes.c_int
+    L.orc_node2_size.restype = ctypes.c_size_t
+    L.orc_nd_build.restype = ctypes.c_void_p
+    L.orc_nd_build.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, _i64, _f32p, _f32p,
+                               ctypes.c_float]
+    L.orc_nd_num_nodes.restype = _i64
+    L.orc_nd_num_nodes.argtypes = [_vp]
+    L.orc_nd_emit.argtypes = [_vp, _u8p, _u8p, _u8p]
+    L.orc_nd_free.argtypes = [_vp]
+    L.orc_nd_eval.argtypes = [ctypes.c_int, _u8p, _i64, _u8p, _i64, _u8p, _i64, _f32p, _i64,
+                              ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                              ctypes.c_double, _f64p]
     L.orc_cpu_tree_build.restype = ctypes.c_void_p
     L.orc_cpu_tree_build.argtypes = [_f32p, _vp, _f32p, _i64, _f32p, _f32p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
@@ -340,6 +352,68 @@ def l3_eval(leaf_bytes, node_bytes, queries, kind, *, beta=2.0, threads=256, sam
                       float(beta), int(threads), int(samples), float(eps), float(rc),
                       int(bool(periodic)), float(box), out)
     return out[:, 0].copy() if kind == L3_POTENTIAL else out
+
+
+LEAF2_DTYPE = np.dtype(
+    [("pos", "<f4", (2,)), ("normal", "<f4", (2,)), ("mass", "<f4"), ("alpha", "<f4")]
+)
+NODE2_DTYPE = np.dtype({
+    "names": ["start", "end", "buffer_idx", "parent_idx", "bmin", "bmax", "diameter", "contrib",
+              "bitmask", "child", "num_children", "depth", "max_depth", "is_leaf"],
+    "formats": ["<u8", "<u8", "<u8", "<u8", ("<f4", 2), ("<f4", 2), "<f4", LEAF2_DTYPE,
+                ("<u8", 1), ("<u8", 64), "<i4", "<i4", "<i4", "u1"],
+    "offsets": [0, 8, 16, 24, 32, 40, 48, 52, 80, 88, 600, 604, 608, 612],
+    "itemsize": 616})
+assert LEAF2_DTYPE.itemsize == 24
+
+
+def nd_build(points, normals, masses, min_corner, max_corner, alpha=1.0):
+    """Tree2D<1,3> (points (N, 2)) or Tree3D<1,3> (points (N, 3)) bytes:
+    (leaf, node, contrib) uint8 arrays, as the reference's sbh.Quadtree3 /
+    sbh.Octree3 leaf_data_bytes / node_buffer_bytes / contrib_data_bytes."""
+    p = _c(points, np.float32)
+    dim = p.shape[1]
+    nv = _c(normals, np.float32)
+    m = _c(masses, np.float32)
+    n = p.shape[0]
+    L = lib()
+    t = L.orc_nd_build(dim, p, nv, m, n, _c(min_corner, np.float32),
+                       _c(max_corner, np.float32), float(alpha))
+    try:
+        nn = int(L.orc_nd_num_nodes(t))
+        lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+        leaf = np.zeros(max(n, 1) * lsz, np.uint8)
+        node = np.zeros(nn * nsz, np.uint8)
+        contrib = np.zeros(nn * lsz, np.uint8)
+        L.orc_nd_emit(t, leaf, node, contrib)
+    finally:
+        L.orc_nd_free(t)
+    return leaf[: n * lsz].copy(), node, contrib
+
+
+def decode_nodes2(node_bytes):
+    return np.frombuffer(np.ascontiguousarray(node_bytes, np.uint8).tobytes(), NODE2_DTYPE)
+
+
+def nd_eval(leaf_bytes, node_bytes, contrib_bytes, queries, kind, *, beta=2.0, threads=256,
+            samples=-1, seed=0, eps=0.0, rc=0.0, periodic=False, box=1.0):
+    """Vote walk (samples < 0) or control-variate sampler (samples >= 0) over
+    Tree2DNode<1,3> or Tree3DNode<1,3> buffers; the layout follows queries' width."""
+    q = _c(queries, np.float32)
+    dim = q.shape[1]
+    lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+    leaf = _c(leaf_bytes, np.uint8)
+    node = _c(node_bytes, np.uint8)
+    contrib = _c(contrib_bytes if contrib_bytes is not None else np.zeros(0, np.uint8), np.uint8)
+    out = np.zeros((q.shape[0], dim), np.float64)
+    lib().orc_nd_eval(dim, leaf if leaf.size else np.zeros(1, np.uint8), leaf.size // lsz, node,
+                      node.size // nsz, contrib if contrib.size else np.zeros(1, np.uint8),
+                      contrib.size // lsz, q, q.shape[0], int(kind), float(beta), int(threads),
+                      int(samples), int(seed), float(eps), float(rc), int(bool(periodic)),
+                      float(box), out)
+    if kind == L3_POTENTIAL:  # written as a flat (M,) array
+        return out.reshape(-1)[: q.shape[0]].copy()
+    return out
 
 
 CPU_POTENTIAL, CPU_WN, CPU_SMOOTHDIST, CPU_SOFT_FORCE = 0, 1, 2, 3

@@ -39,6 +39,9 @@ def test_layout_sizes_match_reference():
     assert lib.sbh_node3_size() == 4272
     assert lib.sbh_octree_level3_max_nodes() == 513
     assert lib.sbh_octree8_node_size() == 104
+    # LeafData<2>, Tree2DNode<1,3> (include/leaf_data.h:8-16, include/tree2d.h:212-225)
+    assert lib.sbh_leaf2_size() == 24
+    assert lib.sbh_node2_size() == 616
 
 
 def _buf(dtype, *dims, ptr=None):
@@ -169,3 +172,37 @@ def test_python_mirror_validates_like_reference():
     p = np.zeros((4, 3), np.float32)
     with pytest.raises((ValueError, RuntimeError)):
         ffi.softened_direct_force(p, np.ones(3, np.float32), p)
+
+
+@pytest.mark.parametrize("dim,pts,nrm,ms,lo,msg", [
+    (4, (F32, 5, 4), (F32, 5, 4), (F32, 5), (F32, 4), "dim must be 2 or 3"),
+    (2, (F32, 5, 3), (F32, 5, 3), (F32, 5), (F32, 2), "points must have shape (N, 2)"),
+    (3, (F32, 5, 3), (F32, 4, 3), (F32, 5), (F32, 3), "normals shape mismatch"),
+    (2, (F32, 5, 2), (F32, 5, 2), (F32, 6), (F32, 2), "masses must have shape (N,)"),
+    (2, (F32, 5, 2), (F32, 5, 2), (F32, 5), (F32, 3), "min_corner/max_corner"),
+])
+def test_tree_builder_rejects(dim, pts, nrm, ms, lo, msg):
+    bufs = [_buf(*x, ptr=FAKE) for x in (pts, nrm, ms, lo, lo)]
+    nodes = ctypes.c_int64(-1)
+    rc, err = _call("sbh_tree_build", None, _i(dim), *map(ctypes.byref, bufs), _f(1.0),
+                    ctypes.byref(nodes))
+    assert rc == 3 and msg in err and nodes.value == -1
+
+
+def test_quadtree_walks_reject_wrong_widths():
+    # 2-D targets: LeafData<2> / Tree2DNode<1,3> byte multiples, (M, 2) queries
+    leaf, node = _buf(U8, 24 * 5, ptr=FAKE), _buf(U8, 616 * 3, ptr=FAKE)
+    q3, out = _buf(F32, 4, 3, ptr=FAKE), _buf(F32, 4, 2, ptr=FAKE)
+    rc, err = _call("sbh_barnes_hut_force", None, *map(ctypes.byref, (leaf, node, q3, out)),
+                    _f(2.0), _i(0), _f(1.0), _i(256))
+    assert rc == 3 and "queries dim must be 2" in err
+    bad_node = _buf(U8, 616 * 3 + 8, ptr=FAKE)
+    q2 = _buf(F32, 4, 2, ptr=FAKE)
+    rc, err = _call("sbh_barnes_hut_force", None, *map(ctypes.byref, (leaf, bad_node, q2, out)),
+                    _f(2.0), _i(0), _f(1.0), _i(256))
+    assert rc == 3 and "node_bytes size mismatch" in err
+    bad_leaf = _buf(U8, 32 * 5, ptr=FAKE)  # 160 is not a multiple of 24
+    rc, err = _call("sbh_barnes_hut_gravity", None,
+                    *map(ctypes.byref, (bad_leaf, node, q2, _buf(F32, 4, ptr=FAKE))),
+                    _f(2.0), _i(0), _f(1.0), _i(64))
+    assert rc == 3 and "leaf_bytes size mismatch" in err

@@ -1,0 +1,225 @@
+"""GPU parity: the reference's CPU-layout trees (Tree2D<1,3> quadtrees for the
+2-D targets, Tree3D<1,3> octrees of build_octree_buffers(backend="cpu")),
+built on the device, and the vote / control-variate walks over them.
+
+Builder: leaf, node and contrib bytes equal the oracle's (oracle/sbh_ndtree.c,
+fp32 in the reference's operation order) word for word; the only words
+allowed to differ are NaN moments of massless nodes (the NaN's sign bit is
+the hardware's).  Walks: the same warp votes and the same sample paths as the
+oracle, so results differ only by fp32 against fp64 accumulation.  The
+reference's 2-D examples (python/example_jax_bh_compare.py,
+python/sbh_jax_examples/stochastic_2d.py) close the file.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _ffi():
+    from sbh_amd import ffi
+
+    return ffi
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def _case(n, dim, seed, kind="uniform"):
+    rng = np.random.default_rng(seed)
+    p = rng.uniform(-1, 1, (n, dim))
+    if kind == "clump":
+        p[: n // 2] = 0.3 + rng.normal(0, 2e-3, (n // 2, dim))
+    if kind == "dups":
+        p[: n // 4] = p[0]
+    p = p.astype(np.float32)
+    nv = rng.normal(size=(n, dim)).astype(np.float32)
+    m = rng.uniform(0.2, 1.0, n).astype(np.float32)
+    return p, nv, m
+
+
+def _corners(p, dim):
+    if p.shape[0] == 0:
+        return np.zeros(dim, np.float32), np.ones(dim, np.float32)
+    return ((p.min(0) - np.float32(1e-3)).astype(np.float32),
+            (p.max(0) + np.float32(1e-3)).astype(np.float32))
+
+
+def _same_words(a, b):
+    a = np.ascontiguousarray(a, np.uint8).view(np.uint32)
+    b = np.ascontiguousarray(b, np.uint8).view(np.uint32)
+    assert a.shape == b.shape
+    diff = np.nonzero(a != b)[0]
+    if diff.size:
+        fa, fb = a[diff].view(np.float32), b[diff].view(np.float32)
+        assert (np.isnan(fa) & np.isnan(fb)).all(), (diff[:8], a[diff[:8]], b[diff[:8]])
+
+
+def _build(ffi, dim, p, nv, m, lo, hi, alpha=1.0):
+    if dim == 2:
+        return ffi.build_quadtree_buffers_with_contrib(p, nv, m, lo, hi, alpha=alpha)
+    return ffi.build_octree_buffers_with_contrib(p, nv, m, lo, hi, backend="cpu")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("n,seed,kind", [(3000, 1, "uniform"), (2000, 2, "clump"),
+                                         (500, 3, "dups"), (1, 4, "uniform"),
+                                         (0, 5, "uniform"), (2, 6, "uniform"),
+                                         (40000, 7, "clump")])
+def test_tree_builder_bytes_match_oracle(gpu, dim, n, seed, kind):
+    ffi = _ffi()
+    p, nv, m = _case(n, dim, seed, kind)
+    lo, hi = _corners(p, dim)
+    alpha = 0.5 if dim == 2 else 1.0
+    leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi, alpha)
+    ol, on, oc = O.nd_build(p, nv, m, lo, hi, alpha=alpha)
+    assert node.numel() == on.size and contrib.numel() == oc.size
+    _same_words(_np(leaf), ol)
+    _same_words(_np(node), on)
+    _same_words(_np(contrib), oc)
+
+
+def test_tree_builder_alternating_dims_and_growth(gpu):
+    ffi = _ffi()
+    for dim, n, seed in [(2, 60000, 11), (3, 500, 12), (3, 30000, 13), (2, 700, 14)]:
+        p, nv, m = _case(n, dim, seed, "clump")
+        lo, hi = _corners(p, dim)
+        leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi)
+        ol, on, oc = O.nd_build(p, nv, m, lo, hi)
+        _same_words(_np(node), on)
+        _same_words(_np(leaf), ol)
+
+
+def test_tree_builder_points_outside_corners(gpu):
+    # quantize clamps points outside the box into the edge cells at every level
+    ffi = _ffi()
+    p, nv, m = _case(2000, 2, 21)
+    lo, hi = np.array([-0.5, -0.5], np.float32), np.array([0.5, 0.5], np.float32)
+    leaf, node, contrib = _build(ffi, 2, p, nv, m, lo, hi)
+    ol, on, oc = O.nd_build(p, nv, m, lo, hi)
+    _same_words(_np(node), on)
+    _same_words(_np(leaf), ol)
+
+
+def _walk(ffi, name, leaf, node, contrib, q, **kw):
+    fn = getattr(ffi, name)
+    if "stochastic" in name:
+        return _np(fn(leaf, node, contrib, q, **kw))
+    return _np(fn(leaf, node, q, **kw))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("threads", [32, 96, 256])
+def test_tree_vote_walks_match_oracle(gpu, dim, threads):
+    ffi = _ffi()
+    p, nv, m = _case(20000, dim, 31, "clump")
+    lo, hi = _corners(p, dim)
+    leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi)
+    lb, nb, cb = _np(leaf), _np(node), _np(contrib)
+    rng = np.random.default_rng(3)
+    q = np.concatenate([p[:3000], rng.uniform(-1.5, 1.5, (1000, dim))]).astype(np.float32)
+    cases = [
+        ("barnes_hut_gravity", O.L3_POTENTIAL, dict(beta=2.0), {}),
+        ("barnes_hut_force", O.L3_FORCE, dict(beta=2.0), {}),
+        ("softened_barnes_hut_force", O.L3_SOFT,
+         dict(beta=1.5, softening_scale=0.02, cutoff_scale=0.5), dict(eps=0.02, rc=0.5)),
+        ("softened_barnes_hut_force", O.L3_SOFT,
+         dict(beta=2.0, softening_scale=0.01, cutoff_scale=float("inf")), dict(eps=0.01)),
+        ("barnes_hut_force", O.L3_FORCE, dict(beta=2.0, periodic=True, box_length=2.0),
+         dict(periodic=True, box=2.0)),
+    ]
+    for name, kind, kw, okw in cases:
+        out = _walk(ffi, name, leaf, node, contrib, q, threads=threads, **kw)
+        ref = O.nd_eval(lb, nb, cb, q, kind, beta=kw["beta"], threads=threads, **okw)
+        assert O.rel_l2(out, ref) < 2e-6, (name, kw)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tree_stochastic_walks_match_oracle(gpu, dim):
+    ffi = _ffi()
+    p, nv, m = _case(20000, dim, 41, "clump")
+    lo, hi = _corners(p, dim)
+    leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi)
+    lb, nb, cb = _np(leaf), _np(node), _np(contrib)
+    q = p[:2048]
+    for threads in (64, 256):
+        for name, kind, kw, okw in [
+            ("stochastic_barnes_hut_gravity", O.L3_POTENTIAL, {}, {}),
+            ("stochastic_barnes_hut_force", O.L3_FORCE, {}, {}),
+            ("softened_stochastic_barnes_hut_force", O.L3_SOFT,
+             dict(softening_scale=0.02, cutoff_scale=0.5), dict(eps=0.02, rc=0.5)),
+        ]:
+            out = _walk(ffi, name, leaf, node, contrib, q, samples_per_subdomain=2, seed=7,
+                        threads=threads, **kw)
+            ref = O.nd_eval(lb, nb, cb, q, kind, samples=2, seed=7, threads=threads, **okw)
+            assert O.rel_l2(out, ref) < 1e-4, (name, threads)
+    a = _walk(ffi, "stochastic_barnes_hut_force", leaf, node, contrib, q, seed=3)
+    b = _walk(ffi, "stochastic_barnes_hut_force", leaf, node, contrib, q, seed=3)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_tree_stochastic_unbiased_over_blocks(gpu):
+    # one query repeated, one query per block: independent sample points and
+    # paths; their mean is the direct sum within its standard error
+    ffi = _ffi()
+    p, nv, m = _case(3000, 2, 51, "clump")
+    lo, hi = _corners(p, 2)
+    leaf, node, contrib = _build(ffi, 2, p, nv, m, lo, hi)
+    K = 20000
+    q = np.full((K, 2), 0.1, np.float32)
+    est = _walk(ffi, "stochastic_barnes_hut_force", leaf, node, contrib, q, seed=9, threads=1)
+    d = (q[:1, None] - p[None]).astype(np.float64)
+    exact = (-m[None, :, None] * d / (np.linalg.norm(d, axis=2)[..., None] + 0.1) ** 3).sum(1)[0]
+    se = est.std(0, ddof=1) / np.sqrt(K)
+    z = np.abs(est.mean(0) - exact) / se
+    assert (se > 0).all() and (z < 4.5).all(), (est.mean(0), exact, se)
+
+
+def test_quadtree_reference_examples(gpu):
+    # python/example_jax_bh_compare.py:20-60 and
+    # python/sbh_jax_examples/stochastic_2d.py:19-45
+    ffi = _ffi()
+    rng = np.random.default_rng(0)
+    n = 300
+    points = rng.uniform(-1.0, 1.0, (n, 2)).astype(np.float32)
+    normals = np.zeros_like(points)
+    masses = np.full(n, 1.0 / n, np.float32)
+    leaf, node = ffi.build_quadtree_buffers(points, normals, masses, level=3)
+    assert leaf.dtype == node.dtype == torch.uint8 and leaf.numel() == 24 * n
+    f = _np(ffi.barnes_hut_force(leaf, node, points, beta=2.0))
+    d = (points[:, None] - points[None]).astype(np.float64)
+    exact = (-masses[None, :, None] * d /
+             (np.linalg.norm(d, axis=2)[..., None] + 0.1) ** 3).sum(1)
+    assert f.shape == (n, 2) and O.rel_l2(f, exact) < 1e-3
+    leaf, node, contrib = ffi.build_quadtree_buffers_with_contrib(points, normals, masses,
+                                                                  level=3)
+    s = _np(ffi.stochastic_barnes_hut_gravity(leaf, node, contrib, points,
+                                              samples_per_subdomain=2, seed=123))
+    bf = (-masses[None] / (np.linalg.norm(d, axis=2) + 0.1)).sum(1)
+    assert s.shape == (n,) and np.isfinite(s).all() and O.rel_l2(s, bf) < 0.05
+    g = _np(ffi.barnes_hut_gravity(leaf, node, points, beta=2.0))
+    assert O.rel_l2(g, bf) < 1e-3
+
+
+def test_quadtree_rejects(gpu):
+    ffi = _ffi()
+    p, nv, m = _case(100, 2, 61)
+    with pytest.raises(ValueError):
+        ffi.build_quadtree_buffers(p, nv, m, level=2)
+    with pytest.raises(ValueError):
+        ffi.build_quadtree_buffers(np.zeros((5, 3), np.float32), np.zeros((5, 3), np.float32),
+                                   np.ones(5, np.float32))
+    leaf, node, contrib = ffi.build_quadtree_buffers_with_contrib(p, nv, m)
+    q3 = np.zeros((4, 3), np.float32)
+    with pytest.raises(ValueError):
+        ffi.barnes_hut_force(leaf, node, q3)  # 2-D buffers read as 3-D records
+    with pytest.raises(ValueError):
+        ffi.stochastic_barnes_hut_force(leaf, node[:-3], contrib, p)
+    with pytest.raises(ValueError):
+        ffi.build_octree_buffers(np.zeros((5, 3), np.float32), np.zeros((5, 3), np.float32),
+                                 np.ones(5, np.float32), backend="gpu")
