@@ -137,7 +137,8 @@ constexpr int kWsBuild = 0;
 constexpr int kWsSort = 1;
 constexpr int kWsWalk = 2;
 constexpr int kWsPipeline = 3;
-constexpr int kWsSlots = 4;
+constexpr int kWsRope = 4;
+constexpr int kWsSlots = 5;
 
 inline unsigned grid_for(int64_t n, int block, unsigned cap = 0x7fffffffu) {
   int64_t g = (n + block - 1) / block;

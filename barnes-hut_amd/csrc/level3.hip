@@ -444,6 +444,140 @@ __global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
   store_out<DIM, KIND>(out, q, acc);
 }
 
+// ------------------------------------------------------------- rope walk
+// For trees of many nodes the vote walk runs over a compact copy of the
+// node records ("rope"): centre of mass, mass, diameter, point range with
+// the leaf flag, first child, and the escape link = the node the reference's
+// stackless traversal moves to after finishing this node (its next sibling by
+// AABB::quantize of the midpoint in the parent, else the parent's escape;
+// none past the root).  Visit order, votes and sums are those of k_l3_walk;
+// each step is one dependent load instead of the node, its parent and the
+// child tables.
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kRopeMin = 2048;  // fewer nodes: the plain walk
+
+template <int DIM>
+struct Rope;
+template <>
+struct Rope<2> {
+  float c[2];
+  float m, diam;
+  uint32_t start, end, first, escape;
+};
+template <>
+struct Rope<3> {
+  float c[3];
+  float m, diam;
+  uint32_t start, end, first, escape, pad[3];
+};
+static_assert(sizeof(Rope<2>) == 32 && sizeof(Rope<3>) == 48, "rope records");
+
+template <int DIM>
+__device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM>::Node* __restrict__ nodes,
+                                                 uint32_t c, uint32_t num_nodes) {
+  using Node = typename L3T<DIM>::Node;
+  constexpr int kN = L3T<DIM>::kCellsN;
+  const Node* nd = &nodes[c];
+  const Node* parent = &nodes[nd->parent_idx < num_nodes ? nd->parent_idx : 0];
+  float mid[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) mid[a] = (nd->bmin[a] + nd->bmax[a]) / 2.0f;
+  const int cell = cell_in<DIM>(parent, mid);
+  const int b = cell < 0 ? kN : next_bit<DIM>(parent, cell + 1);
+  if (b >= kN || parent->child[b] >= num_nodes) return kNone;
+  return static_cast<uint32_t>(parent->child[b]);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM>::Node* __restrict__ nodes,
+                                                uint32_t num_nodes, uint32_t num_leaves,
+                                                Rope<DIM>* __restrict__ rope) {
+  using Node = typename L3T<DIM>::Node;
+  constexpr int kN = L3T<DIM>::kCellsN;
+  const uint32_t c = blockIdx.x * kB + threadIdx.x;
+  if (c >= num_nodes) return;
+  const Node* nd = &nodes[c];
+  Rope<DIM> r = {};
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) r.c[a] = nd->contrib.pos[a];
+  r.m = nd->contrib.mass;
+  r.diam = nd->diameter;
+  const uint64_t je = nd->end < num_leaves ? nd->end : num_leaves;
+  const uint64_t js = nd->start < je ? nd->start : je;
+  r.start = static_cast<uint32_t>(js) | (nd->is_leaf ? kLeafBit : 0u);
+  r.end = static_cast<uint32_t>(je);
+  const int b = next_bit<DIM>(nd, 0);
+  r.first = (b < kN && nd->child[b] < num_nodes) ? static_cast<uint32_t>(nd->child[b]) : kNone;
+  // finishing node c: next sibling, else climb (the reference's FromChild
+  // state) until a node with a next sibling; the root ends the walk
+  uint32_t esc = kNone, cur = c;
+  for (int hop = 0; hop < 256 && cur != 0; ++hop) {
+    const uint32_t s = next_sibling<DIM>(nodes, cur, num_nodes);
+    if (s != kNone) {
+      esc = s;
+      break;
+    }
+    const uint64_t pi = nodes[cur].parent_idx;
+    cur = static_cast<uint32_t>(pi < num_nodes ? pi : 0);
+  }
+  r.escape = esc;
+  rope[c] = r;
+}
+
+template <int DIM, int KIND>
+__global__ void k_l3_walk_rope(const typename L3T<DIM>::Leaf* __restrict__ leaf,
+                               const Rope<DIM>* __restrict__ rope,
+                               const float* __restrict__ queries, int nq,
+                               float* __restrict__ out, L3Params A, uint32_t num_nodes) {
+  using Leaf = typename L3T<DIM>::Leaf;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = q < nq;
+  const int lane = __lane_id();
+  const uint64_t half = (lane < 32) ? 0x00000000ffffffffull : 0xffffffff00000000ull;
+  const uint64_t members = __ballot(active) & half;
+  if (!active) return;
+  float qv[DIM];
+#pragma unroll
+  for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
+  float acc[DIM] = {};
+  uint32_t c = 0;
+  const uint64_t guard_max = 4ull * num_nodes + 64ull;
+  for (uint64_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
+    const Rope<DIM> r = rope[c];
+    const bool is_leaf = (r.start & kLeafBit) != 0u;
+    float ratio;
+    {
+#pragma clang fp contract(off)
+      // farFieldRatio, as far_ratio()
+      float s = 0.0f;
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) {
+        const float d = qv[a] - r.c[a];
+        s = a == 0 ? d * d : s + d * d;
+      }
+      ratio = is_leaf ? INFINITY : sqrtf(s) / r.diam;
+    }
+    const bool far = ratio >= A.beta;
+    const bool all = (__ballot(far) & members) == members;
+    if (all) {
+      if (is_leaf) {
+        l3_leaf_sum<KIND>(leaf, r.start & ~kLeafBit, r.end, qv, A.P, acc);
+      } else {
+        Leaf L = {};
+#pragma unroll
+        for (int a = 0; a < DIM; ++a) L.pos[a] = r.c[a];
+        L.mass = r.m;
+        l3_term<KIND>(L, qv, A.P, acc);
+      }
+      c = r.escape;
+    } else {
+      c = r.first != kNone ? r.first : r.escape;
+    }
+  }
+  store_out<DIM, KIND>(out, q, acc);
+}
+
 // Node of the child cell of `node` that holds p, skipping children that have
 // exactly one child (Tree2DNode / Tree3DNode::containedInChild(p),
 // include/tree2d.h:168-179); -1 when p is outside a box on the way (the
@@ -611,11 +745,28 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   A.beta = beta;
   A.threads = threads;
   const unsigned blocks = static_cast<unsigned>((nq + threads - 1) / threads);
-  k_l3_walk<DIM, KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
-      static_cast<const Leaf*>(leaf_bytes->data), static_cast<const Node*>(node_bytes->data),
-      static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
-      static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node)),
-      static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf)));
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t num_nodes = static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node));
+  const uint32_t num_leaves = static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf));
+  const auto* nodes = static_cast<const Node*>(node_bytes->data);
+  // the rope needs a workspace: not under stream capture (no allocation there)
+  Rope<DIM>* rope = nullptr;
+  if (num_nodes >= kRopeMin && !stream_capturing(s))
+    rope = static_cast<Rope<DIM>*>(
+        workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(Rope<DIM>)));
+  if (rope != nullptr) {
+    k_l3_rope<DIM><<<grid_for(num_nodes, kB), kB, 0, s>>>(nodes, num_nodes, num_leaves, rope);
+    SBH_LAUNCH_CHECK();
+    k_l3_walk_rope<DIM, KIND><<<blocks, threads, 0, s>>>(
+        static_cast<const Leaf*>(leaf_bytes->data), rope,
+        static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
+        num_nodes);
+  } else {
+    k_l3_walk<DIM, KIND><<<blocks, threads, 0, s>>>(
+        static_cast<const Leaf*>(leaf_bytes->data), nodes,
+        static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
+        num_nodes, num_leaves);
+  }
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }

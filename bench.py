@@ -211,7 +211,7 @@ def main():
                     help="time eager launches only (default: the step is also captured as a "
                          "HIP graph and the replay is the reported value)")
     ap.add_argument("--workload", default="gravity",
-                    choices=("gravity", "wn", "batched", "stochastic", "treepm"),
+                    choices=("gravity", "wn", "batched", "stochastic", "treepm", "quadtree"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
@@ -219,11 +219,15 @@ def main():
                          "classic and stochastic BH on the dense octree8 at the headline "
                          "Plummer sphere, with the seed-mean unbiasedness check; treepm: the "
                          "short-range TreePM force (periodic, erfc cutoff, prune) on 128^3 "
-                         "clustered particles")
+                         "clustered particles; quadtree: the 2-D targets over Tree2D<1,3> "
+                         "buffers (device build + vote walk, and the stochastic walk) on 2^20 "
+                         "points in a square")
     ap.add_argument("--o8-depth", type=int, default=8)
     ap.add_argument("--treepm-n1d", type=int, default=128,
                     help="treepm: particles per side (n1d^3 particles, box of n1d cells)")
     ap.add_argument("--treepm-depth", type=int, default=8)
+    ap.add_argument("--qt-n", type=int, default=1 << 20,
+                    help="quadtree: number of 2-D points (queries = the points)")
     ap.add_argument("--seeds", type=int, default=256)
     ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--sets", type=int, default=64)
@@ -238,7 +242,8 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
     bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
-             "stochastic": bench_stochastic, "treepm": bench_treepm}[args.workload]
+             "stochastic": bench_stochastic, "treepm": bench_treepm,
+             "quadtree": bench_quadtree}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -891,6 +896,104 @@ def bench_treepm(args):
                    "softening": 0.1, "cutoff_scale": 1.0, "prune_r_cut_mult": 4.0,
                    "step": "octree8 build + softened periodic walk (forward)"},
         "phases_ms": {"octree8_build": build_ms, "walk": walk_ms, "vjp_walk": vjp_ms},
+        "accuracy": acc,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
+def direct2_fp64(P, M, Q, chunk=64):
+    """FP64 2-D direct sum of gravityGrad<2> (include/kernel.h:89-94)."""
+    import torch
+
+    out = []
+    for q in Q.split(chunk):
+        d = q[:, None, :] - P[None]
+        r = torch.sqrt((d * d).sum(-1))
+        out.append((-(M[None] / (r + 0.1) ** 3)[..., None] * d).sum(1))
+    return torch.cat(out)
+
+
+def bench_quadtree(args):
+    """2-D targets over Tree2D<1,3> buffers (src/sbh_ffi.cu:1422-1680,
+    2444-3064): the tree the reference builds on the host (sbh.Quadtree3,
+    include/tree2d.h:237-407), here built on the device, then the warp-voted
+    force walk; the reference's 2-D example (python/example_jax_bh_compare.py)
+    at 2^20 points: uniform in [-1, 1]^2, masses 1/N, queries = the points in
+    input order, beta 2, 256 threads."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import ffi
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n = args.qt_n
+    rng = np.random.default_rng(7000)
+    p = rng.uniform(-1.0, 1.0, (n, 2)).astype(np.float32)
+    m = np.full(n, 1.0 / n, np.float32)
+    P, M = torch.tensor(p, device=dev), torch.tensor(m, device=dev)
+    Z = torch.zeros_like(P)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        # the build synchronizes once per level: host time is the honest clock
+        return r, max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3) / reps
+
+    (leaf, node, contrib), build_ms = timed(
+        lambda: ffi.build_quadtree_buffers_with_contrib(P, Z, M), args.steps)
+    nodes = node.numel() // 616
+    F, walk_ms = timed(lambda: ffi.barnes_hut_force(leaf, node, P, beta=2.0), args.steps)
+    _, pot_ms = timed(lambda: ffi.barnes_hut_gravity(leaf, node, P, beta=2.0, threads=64),
+                      args.steps)
+    ns = min(n, 1 << 16)
+    _, stoch_ms = timed(lambda: ffi.stochastic_barnes_hut_force(
+        leaf, node, contrib, P[:ns], samples_per_subdomain=1, seed=0), max(1, args.steps // 2))
+    ms_step = build_ms + walk_ms
+    idx = torch.tensor(np.random.default_rng(7001).choice(n, args.accuracy_queries,
+                                                          replace=False), device=dev)
+    ref = direct2_fp64(P.double(), M.double(), P[idx].double())
+    acc = {"force_rel_l2_vs_fp64_direct": rel_l2(F[idx], ref),
+           "sample_queries": args.accuracy_queries}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        ol, on, oc = O.nd_build(p, np.zeros_like(p), m, *[x.cpu().numpy() for x in
+                                                           ffi._default_corners(P, None, None)])
+        cpu_build = time.perf_counter() - t0
+        nq = 1 << 12
+        t0 = time.perf_counter()
+        O.nd_eval(ol, on, oc, p[:nq], O.L3_FORCE, beta=2.0, threads=256)
+        t_walk = (time.perf_counter() - t0) / nq
+        cpu = {"value": n / (cpu_build + t_walk * n), "unit": "forces/s (build + walk)",
+               "cores": 1, "kind": "port",
+               "sample": f"oracle Tree2D<1,3> build (the reference's host builder restated, "
+                         f"{cpu_build:.2f} s for all {n} points) + oracle vote walk (fp64, "
+                         f"same warps) on the first {nq} queries, scaled; one thread"}
+    return {
+        "metric": "2-D BH forces/sec (Tree2D<1,3> quadtree: build + warp-voted walk)",
+        "value": n / (ms_step * 1e-3),
+        "unit": "forces/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"quadtree_uniform_square_n{n}_beta2_threads256",
+                   "n": n, "nodes": nodes, "beta": 2.0, "threads": 256,
+                   "step": "Tree2D<1,3> device build + gravityGrad<2> vote walk"},
+        "phases_ms": {"tree_build": build_ms, "force_walk": walk_ms,
+                      "potential_walk_threads64": pot_ms,
+                      f"stochastic_force_walk_{ns}_queries_1_sample": stoch_ms},
         "accuracy": acc,
         "roofline": None,
         "cpu_baseline": cpu,

@@ -234,3 +234,41 @@ def test_quadtree_rejects(gpu):
     with pytest.raises(ValueError):
         ffi.build_octree_buffers(np.zeros((5, 3), np.float32), np.zeros((5, 3), np.float32),
                                  np.ones(5, np.float32), backend="gpu")
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_tree_vote_walk_rope_equals_plain_walk(gpu, dim):
+    # Trees of >= 2048 nodes are walked over the compact rope records; under
+    # stream capture (no workspace allocation) the plain walk over the node
+    # records runs instead.  Same visit order and votes: identical bits.
+    ffi = _ffi()
+    p, nv, m = _case(20000, dim, 71, "clump")
+    lo, hi = _corners(p, dim)
+    leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi)
+    rng = np.random.default_rng(4)
+    Q = torch.tensor(np.concatenate([p[:2000], rng.uniform(-1.5, 1.5, (1000, dim))]),
+                     dtype=torch.float32, device="cuda")
+    eager = {
+        "f": _np(ffi.barnes_hut_force(leaf, node, Q, beta=2.0, threads=96)),
+        "g": _np(ffi.barnes_hut_gravity(leaf, node, Q, beta=1.5)),
+        "s": _np(ffi.softened_barnes_hut_force(leaf, node, Q, beta=2.0, softening_scale=0.01,
+                                               cutoff_scale=0.5)),
+    }
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            ffi.barnes_hut_force(leaf, node, Q, beta=2.0, threads=96)
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        f = ffi.barnes_hut_force(leaf, node, Q, beta=2.0, threads=96)
+        g = ffi.barnes_hut_gravity(leaf, node, Q, beta=1.5)
+        s = ffi.softened_barnes_hut_force(leaf, node, Q, beta=2.0, softening_scale=0.01,
+                                          cutoff_scale=0.5)
+    for _ in range(2):
+        graph.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(_np(f), eager["f"])
+        np.testing.assert_array_equal(_np(g), eager["g"])
+        np.testing.assert_array_equal(_np(s), eager["s"])
