@@ -11,13 +11,17 @@
 
 namespace sbh {
 
-// LeafData<3> (include/leaf_data.h:8-16): 32 bytes.
-struct Leaf3 {
-  float pos[3];
-  float normal[3];
+// LeafData<N> (include/leaf_data.h:8-16): position, normal, mass, alpha;
+// 24 bytes in 2-D, 32 in 3-D.
+template <int DIM>
+struct LeafT {
+  float pos[DIM];
+  float normal[DIM];
   float mass;
   float alpha;
 };
+using Leaf2 = LeafT<2>;
+using Leaf3 = LeafT<3>;
 
 // OctreeNode8 (src/sbh_ffi.cu:47-65): 104 bytes, same field order.
 struct Node8 {
@@ -28,39 +32,27 @@ struct Node8 {
   Leaf3 contrib;
 };
 
-// Tree3DNode<1, 3> (include/tree3d.h:212-225; the reference's Node3Layout,
-// src/sbh_ffi.cu:123-140): 4272 bytes, same field order and padding.
-struct Node3 {
+// Tree2DNode<1, LOG> / Tree3DNode<1, LOG> (include/tree2d.h:212-225,
+// include/tree3d.h:212-225; for LOG = 3 also the reference's Node3Layout,
+// src/sbh_ffi.cu:123-140): the same members in the same order, hence the same
+// size and padding.  2^LOG cells per axis; cell b <-> x + S y (+ S^2 z),
+// S = 2^LOG; the bit mask (BitMask<LOG, DIM>, include/bitmask.h:20-27) has
+// max(1, S^DIM / 64) words.
+template <int DIM, int LOG>
+struct NodeT {
+  static constexpr int kCells = 1 << (DIM * LOG);
+  static constexpr int kWords = kCells >= 64 ? kCells / 64 : 1;
   uint64_t start, end, buffer_idx, parent_idx;
-  float bmin[3], bmax[3];  // AABB<3>
+  float bmin[DIM], bmax[DIM];  // AABB<DIM>
   float diameter;
-  Leaf3 contrib;
-  uint64_t bitmask[8];     // BitMask3D<3>: bit b <-> child cell b = x + 8 (y + 8 z)
-  uint64_t child[512];     // buffer index of the node of each present cell
+  LeafT<DIM> contrib;
+  uint64_t bitmask[kWords];
+  uint64_t child[kCells];      // buffer index of the node of each present cell
   int32_t num_children, depth, max_depth;
   uint8_t is_leaf, pad[3];
 };
-
-// LeafData<2> (include/leaf_data.h:8-16): 24 bytes.
-struct Leaf2 {
-  float pos[2];
-  float normal[2];
-  float mass;
-  float alpha;
-};
-
-// Tree2DNode<1, 3> (include/tree2d.h:212-225): 616 bytes, same field order
-// and padding; 8 x 8 cells, cell b <-> x + 8 y.
-struct Node2 {
-  uint64_t start, end, buffer_idx, parent_idx;
-  float bmin[2], bmax[2];  // AABB<2>
-  float diameter;
-  Leaf2 contrib;
-  uint64_t bitmask[1];     // BitMask2D<3>
-  uint64_t child[64];
-  int32_t num_children, depth, max_depth;
-  uint8_t is_leaf, pad[3];
-};
+using Node2 = NodeT<2, 3>;  // 616 bytes
+using Node3 = NodeT<3, 3>;  // 4272 bytes
 
 static_assert(sizeof(Leaf2) == 24, "Leaf2 layout");
 static_assert(sizeof(Node2) == 616, "Tree2DNode<1,3> layout");
@@ -75,6 +67,9 @@ static_assert(offsetof(Node8, child_mask) == 48, "OctreeNode8 layout");
 static_assert(offsetof(Node8, cx) == 52, "OctreeNode8 layout");
 static_assert(offsetof(Node8, contrib) == 72, "OctreeNode8 layout");
 static_assert(sizeof(Node3) == 4272, "Tree3DNode<1,3> layout");
+static_assert(sizeof(NodeT<2, 1>) == 136 && sizeof(NodeT<2, 2>) == 232 &&
+                  sizeof(NodeT<3, 1>) == 184 && sizeof(NodeT<3, 2>) == 632,
+              "Tree2DNode<1,1..2> / Tree3DNode<1,1..2> layouts");
 static_assert(offsetof(Node3, bmin) == 32 && offsetof(Node3, diameter) == 56 &&
                   offsetof(Node3, contrib) == 60 && offsetof(Node3, bitmask) == 96 &&
                   offsetof(Node3, child) == 160 && offsetof(Node3, num_children) == 4256 &&

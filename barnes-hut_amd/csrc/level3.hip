@@ -230,23 +230,24 @@ __global__ __launch_bounds__(kCells) void k_l3_nodes(const int32_t* __restrict__
 // ------------------------------------------------------------------ walks
 // The walks read Tree3DNode<1, 3> records (level-3 buffers, or CPU-layout
 // octrees) and Tree2DNode<1, 3> records (quadtrees) alike.
-enum L3Kind { kPotential = 0, kForce = 1, kSoftForce3 = 2 };
+enum L3Kind { kPotential = 0, kForce = 1, kSoftForce3 = 2, kWn = 3, kSmooth = 4 };
 
-template <int DIM>
-struct L3T;
-template <>
-struct L3T<3> {
-  using Node = Node3;
-  using Leaf = Leaf3;
-  static constexpr int kCellsN = 512;
-  static constexpr int kWords = 8;
-};
-template <>
-struct L3T<2> {
-  using Node = Node2;
-  using Leaf = Leaf2;
-  static constexpr int kCellsN = 64;
-  static constexpr int kWords = 1;
+// Kinds with one value per query.
+template <int KIND>
+constexpr bool kScalarL3 = KIND == kPotential || KIND == kWn || KIND == kSmooth;
+// Kinds whose terms read only a record's position and mass (the rope walk).
+template <int KIND>
+constexpr bool kRopeKind = KIND == kPotential || KIND == kForce || KIND == kSoftForce3;
+
+constexpr float kPiF = 3.14159265358979323846f;
+
+template <int DIM, int LOG>
+struct L3T {
+  using Node = NodeT<DIM, LOG>;
+  using Leaf = LeafT<DIM>;
+  static constexpr int kSplitN = 1 << LOG;  // cells per axis
+  static constexpr int kCellsN = Node::kCells;
+  static constexpr int kWords = Node::kWords;
 };
 
 struct L3Params {
@@ -261,11 +262,21 @@ template <int KIND>
 __device__ __forceinline__ void l3_term(const Leaf3& L, const float* q,
                                         const o8::WalkParams& P, float* acc) {
   const float qx = q[0], qy = q[1], qz = q[2];
-  if constexpr (KIND == kPotential) {
+  if constexpr (KIND == kPotential || KIND == kWn || KIND == kSmooth) {
     const float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
     const float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
     const float dz = min_image_floor(L.pos[2] - qz, P.periodic, P.box, P.inv_box);
-    acc[0] += -L.mass / (sqrtf(dx * dx + dy * dy + dz * dz) + 1e-1f);
+    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    if constexpr (KIND == kPotential) {
+      acc[0] += -L.mass / (dist + 1e-1f);
+    } else if constexpr (KIND == kWn) {
+      // signedSolidAngle<3> (include/kernel.h:73-80)
+      const float qn = L.normal[0] * dx + L.normal[1] * dy + L.normal[2] * dz;
+      acc[0] += L.mass * qn / (4.0f * kPiF * (dist * dist * dist + 1e-5f));
+    } else {
+      // smoothDistExp (include/kernel.h:82-87)
+      acc[0] += expf(-L.alpha * dist);
+    }
   } else if constexpr (KIND == kForce) {
     o8::contribution<o8::kGravGrad>(L, qx, qy, qz, 0.f, 0.f, 0.f, P, acc[0], acc[1], acc[2]);
   } else {
@@ -276,11 +287,20 @@ __device__ __forceinline__ void l3_term(const Leaf3& L, const float* q,
 template <int KIND>
 __device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
                                         const o8::WalkParams& P, float* acc) {
-  if constexpr (KIND == kPotential) {
-    // gravityPotential<2>: d = p - q
+  if constexpr (KIND == kPotential || KIND == kWn || KIND == kSmooth) {
+    // d = p - q
     const float dx = min_image_floor(L.pos[0] - q[0], P.periodic, P.box, P.inv_box);
     const float dy = min_image_floor(L.pos[1] - q[1], P.periodic, P.box, P.inv_box);
-    acc[0] += -L.mass / (sqrtf(dx * dx + dy * dy) + 1e-1f);
+    if constexpr (KIND == kPotential) {
+      // gravityPotential<2>
+      acc[0] += -L.mass / (sqrtf(dx * dx + dy * dy) + 1e-1f);
+    } else if constexpr (KIND == kWn) {
+      // signedSolidAngle<2> (include/kernel.h:65-71)
+      const float qn = L.normal[0] * dx + L.normal[1] * dy;
+      acc[0] += L.mass * qn / (2.0f * kPiF * (dx * dx + dy * dy + 1e-5f));
+    } else {
+      acc[0] += expf(-L.alpha * sqrtf(dx * dx + dy * dy));
+    }
   } else {
     // gravityGrad<2> and gravityGradSoftCutoff<2>: d = q - p
     const float dx = min_image_floor(q[0] - L.pos[0], P.periodic, P.box, P.inv_box);
@@ -306,8 +326,8 @@ __device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
 
 // farFieldRatio (include/tree3d.h:134-141, include/tree2d.h:132-138):
 // |q - centre of mass| / diameter, infinite for a leaf.
-template <int DIM>
-__device__ __forceinline__ float far_ratio(const typename L3T<DIM>::Node* nd, const float* q) {
+template <int DIM, int LOG>
+__device__ __forceinline__ float far_ratio(const typename L3T<DIM, LOG>::Node* nd, const float* q) {
 #pragma clang fp contract(off)
   if (nd->is_leaf) return INFINITY;
   float s = 0.0f;
@@ -321,28 +341,29 @@ __device__ __forceinline__ float far_ratio(const typename L3T<DIM>::Node* nd, co
 
 // Cell of point p in `parent` (AABB::quantize, include/aabb.h:36-42), or -1
 // when p is outside parent's bounds widened by 1e-4 (AABB::contains).
-template <int DIM>
-__device__ __forceinline__ int cell_in(const typename L3T<DIM>::Node* parent, const float* p) {
+template <int DIM, int LOG>
+__device__ __forceinline__ int cell_in(const typename L3T<DIM, LOG>::Node* parent, const float* p) {
 #pragma clang fp contract(off)
+  constexpr int kS = L3T<DIM, LOG>::kSplitN;
   int q[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) {
     if (!(parent->bmin[a] - 1e-4f <= p[a] && p[a] <= parent->bmax[a] + 1e-4f)) return -1;
     const float t =
-        static_cast<float>(kSplit) * (p[a] - parent->bmin[a]) / (parent->bmax[a] - parent->bmin[a]);
-    q[a] = min(max(static_cast<int>(t), 0), kSplit - 1);
+        static_cast<float>(kS) * (p[a] - parent->bmin[a]) / (parent->bmax[a] - parent->bmin[a]);
+    q[a] = min(max(static_cast<int>(t), 0), kS - 1);
   }
   int c = 0;
 #pragma unroll
-  for (int a = DIM - 1; a >= 0; --a) c = c * kSplit + q[a];
+  for (int a = DIM - 1; a >= 0; --a) c = c * kS + q[a];
   return c;
 }
 
 // First set bit of the node's mask at or after `from` (kCellsN if none).
-template <int DIM>
-__device__ __forceinline__ int next_bit(const typename L3T<DIM>::Node* nd, int from) {
-  constexpr int kN = L3T<DIM>::kCellsN;
-  for (int w = from >> 6; w < L3T<DIM>::kWords; ++w) {
+template <int DIM, int LOG>
+__device__ __forceinline__ int next_bit(const typename L3T<DIM, LOG>::Node* nd, int from) {
+  constexpr int kN = L3T<DIM, LOG>::kCellsN;
+  for (int w = from >> 6; w < L3T<DIM, LOG>::kWords; ++w) {
     uint64_t m = nd->bitmask[w];
     if (w == (from >> 6)) m &= ~0ull << (from & 63);
     if (m) return 64 * w + __builtin_ctzll(m);
@@ -369,7 +390,7 @@ __device__ __forceinline__ void l3_leaf_sum(const Leaf* __restrict__ leaf, uint6
 
 template <int DIM, int KIND>
 __device__ __forceinline__ void store_out(float* __restrict__ out, int q, const float* acc) {
-  if constexpr (KIND == kPotential) {
+  if constexpr (kScalarL3<KIND>) {
     out[q] = acc[0];
   } else {
 #pragma unroll
@@ -383,13 +404,13 @@ __device__ __forceinline__ void store_out(float* __restrict__ out, int q, const 
 // Launched with the caller's block size, every reference warp is one half
 // of a 64-wide wavefront here, and the vote is a ballot restricted to that
 // half.
-template <int DIM, int KIND>
-__global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
-                          const typename L3T<DIM>::Node* __restrict__ nodes,
+template <int DIM, int LOG, int KIND>
+__global__ void k_l3_walk(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
+                          const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                           const float* __restrict__ queries, int nq, float* __restrict__ out,
                           L3Params A, uint32_t num_nodes, uint32_t num_leaves) {
-  using Node = typename L3T<DIM>::Node;
-  constexpr int kN = L3T<DIM>::kCellsN;
+  using Node = typename L3T<DIM, LOG>::Node;
+  constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = q < nq;
   const int lane = __lane_id();
@@ -408,7 +429,7 @@ __global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
   for (uint64_t guard = 0; guard < guard_max; ++guard) {
     const Node* nd = &nodes[node];
     if (!from_child) {
-      const bool far = far_ratio<DIM>(nd, qv) >= A.beta;
+      const bool far = far_ratio<DIM, LOG>(nd, qv) >= A.beta;
       const bool all = (__ballot(far) & members) == members;
       if (all) {
         if (nd->is_leaf) {
@@ -418,7 +439,7 @@ __global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
           l3_term<KIND>(nd->contrib, qv, A.P, acc);
         }
       } else {
-        const int b = next_bit<DIM>(nd, 0);
+        const int b = next_bit<DIM, LOG>(nd, 0);
         if (b < kN && nd->child[b] < num_nodes) {
           node = static_cast<uint32_t>(nd->child[b]);
           continue;
@@ -431,8 +452,8 @@ __global__ void k_l3_walk(const typename L3T<DIM>::Leaf* __restrict__ leaf,
     float mid[DIM];
 #pragma unroll
     for (int a = 0; a < DIM; ++a) mid[a] = (nd->bmin[a] + nd->bmax[a]) / 2.0f;
-    const int cell = cell_in<DIM>(parent, mid);
-    const int b = cell < 0 ? kN : next_bit<DIM>(parent, cell + 1);
+    const int cell = cell_in<DIM, LOG>(parent, mid);
+    const int b = cell < 0 ? kN : next_bit<DIM, LOG>(parent, cell + 1);
     if (b >= kN || parent->child[b] >= num_nodes) {
       node = static_cast<uint32_t>(nd->parent_idx < num_nodes ? nd->parent_idx : 0);
       from_child = true;
@@ -473,28 +494,28 @@ struct Rope<3> {
 };
 static_assert(sizeof(Rope<2>) == 32 && sizeof(Rope<3>) == 48, "rope records");
 
-template <int DIM>
-__device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM>::Node* __restrict__ nodes,
+template <int DIM, int LOG>
+__device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                                                  uint32_t c, uint32_t num_nodes) {
-  using Node = typename L3T<DIM>::Node;
-  constexpr int kN = L3T<DIM>::kCellsN;
+  using Node = typename L3T<DIM, LOG>::Node;
+  constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const Node* nd = &nodes[c];
   const Node* parent = &nodes[nd->parent_idx < num_nodes ? nd->parent_idx : 0];
   float mid[DIM];
 #pragma unroll
   for (int a = 0; a < DIM; ++a) mid[a] = (nd->bmin[a] + nd->bmax[a]) / 2.0f;
-  const int cell = cell_in<DIM>(parent, mid);
-  const int b = cell < 0 ? kN : next_bit<DIM>(parent, cell + 1);
+  const int cell = cell_in<DIM, LOG>(parent, mid);
+  const int b = cell < 0 ? kN : next_bit<DIM, LOG>(parent, cell + 1);
   if (b >= kN || parent->child[b] >= num_nodes) return kNone;
   return static_cast<uint32_t>(parent->child[b]);
 }
 
-template <int DIM>
-__global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM>::Node* __restrict__ nodes,
+template <int DIM, int LOG>
+__global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                                                 uint32_t num_nodes, uint32_t num_leaves,
                                                 Rope<DIM>* __restrict__ rope) {
-  using Node = typename L3T<DIM>::Node;
-  constexpr int kN = L3T<DIM>::kCellsN;
+  using Node = typename L3T<DIM, LOG>::Node;
+  constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const uint32_t c = blockIdx.x * kB + threadIdx.x;
   if (c >= num_nodes) return;
   const Node* nd = &nodes[c];
@@ -507,13 +528,13 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM>::Node* _
   const uint64_t js = nd->start < je ? nd->start : je;
   r.start = static_cast<uint32_t>(js) | (nd->is_leaf ? kLeafBit : 0u);
   r.end = static_cast<uint32_t>(je);
-  const int b = next_bit<DIM>(nd, 0);
+  const int b = next_bit<DIM, LOG>(nd, 0);
   r.first = (b < kN && nd->child[b] < num_nodes) ? static_cast<uint32_t>(nd->child[b]) : kNone;
   // finishing node c: next sibling, else climb (the reference's FromChild
   // state) until a node with a next sibling; the root ends the walk
   uint32_t esc = kNone, cur = c;
   for (int hop = 0; hop < 256 && cur != 0; ++hop) {
-    const uint32_t s = next_sibling<DIM>(nodes, cur, num_nodes);
+    const uint32_t s = next_sibling<DIM, LOG>(nodes, cur, num_nodes);
     if (s != kNone) {
       esc = s;
       break;
@@ -525,12 +546,12 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM>::Node* _
   rope[c] = r;
 }
 
-template <int DIM, int KIND>
-__global__ void k_l3_walk_rope(const typename L3T<DIM>::Leaf* __restrict__ leaf,
+template <int DIM, int LOG, int KIND>
+__global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
                                const Rope<DIM>* __restrict__ rope,
                                const float* __restrict__ queries, int nq,
                                float* __restrict__ out, L3Params A, uint32_t num_nodes) {
-  using Leaf = typename L3T<DIM>::Leaf;
+  using Leaf = typename L3T<DIM, LOG>::Leaf;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = q < nq;
   const int lane = __lane_id();
@@ -583,12 +604,12 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM>::Leaf* __restrict__ leaf,
 // include/tree2d.h:168-179); -1 when p is outside a box on the way (the
 // reference returns nullptr there).  An absent cell reads child id 0, as the
 // reference's zero-filled child table does.
-template <int DIM>
-__device__ __forceinline__ int64_t contained_child(const typename L3T<DIM>::Node* __restrict__ nodes,
+template <int DIM, int LOG>
+__device__ __forceinline__ int64_t contained_child(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                                                    uint64_t node, const float* p,
                                                    uint32_t num_nodes) {
   for (int hop = 0; hop < 128; ++hop) {
-    const int cell = cell_in<DIM>(&nodes[node], p);
+    const int cell = cell_in<DIM, LOG>(&nodes[node], p);
     if (cell < 0) return -1;
     const uint64_t ch = nodes[node].child[cell];
     if (ch >= num_nodes) return -1;
@@ -606,15 +627,16 @@ __device__ __forceinline__ int64_t contained_child(const typename L3T<DIM>::Node
 // reference, the sample point is shared by all queries of a block (its
 // stream depends only on the block index) and the roulette stream on
 // seed + query.
-template <int DIM, int KIND>
-__global__ void k_l3_stoch(const typename L3T<DIM>::Leaf* __restrict__ leaf,
-                           const typename L3T<DIM>::Node* __restrict__ nodes,
-                           const typename L3T<DIM>::Leaf* __restrict__ contrib,
+template <int DIM, int LOG, int KIND>
+__global__ void k_l3_stoch(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
+                           const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
+                           const typename L3T<DIM, LOG>::Leaf* __restrict__ contrib,
                            const float* __restrict__ queries, int nq, float* __restrict__ out,
-                           o8::WalkParams P, int samples, int seed, uint32_t num_nodes,
+                           o8::WalkParams P, int samples, int seed, uint32_t seed_per_query,
+                           uint32_t num_nodes,
                            uint32_t num_leaves, uint32_t num_contrib) {
-  using Node = typename L3T<DIM>::Node;
-  constexpr int kN = L3T<DIM>::kCellsN;
+  using Node = typename L3T<DIM, LOG>::Node;
+  constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nq) return;
   float qv[DIM];
@@ -622,9 +644,11 @@ __global__ void k_l3_stoch(const typename L3T<DIM>::Leaf* __restrict__ leaf,
   for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
   float acc[DIM] = {};
   float mlcv[DIM] = {};
-  const uint32_t rseed = static_cast<uint32_t>(seed) + static_cast<uint32_t>(q);
+  // the FFI targets seed the roulette with seed + query; the pybind11 module
+  // gives every query seed 0 (src/sbh_gpu.cu:143-164)
+  const uint32_t rseed = static_cast<uint32_t>(seed) + seed_per_query * static_cast<uint32_t>(q);
   const Node* root = &nodes[0];
-  for (int b = next_bit<DIM>(root, 0); b < kN; b = next_bit<DIM>(root, b + 1)) {
+  for (int b = next_bit<DIM, LOG>(root, 0); b < kN; b = next_bit<DIM, LOG>(root, b + 1)) {
     const uint64_t ci = root->child[b];
     if (ci >= num_nodes) continue;
     const Node* sub = &nodes[ci];
@@ -649,21 +673,21 @@ __global__ void k_l3_stoch(const typename L3T<DIM>::Leaf* __restrict__ leaf,
       uint64_t cur = ci;
       float event_prob = 1.0f;
       float wsum[DIM] = {};
-      float cur_beta = fmaxf(1.0f, far_ratio<DIM>(sub, qv));
+      float cur_beta = fmaxf(1.0f, far_ratio<DIM, LOG>(sub, qv));
       // a path ends at a leaf; depth <= 100 in trees from the builders
       for (uint32_t step = 0; step < 1024u && !nodes[cur].is_leaf; ++step) {
         const Node* cn = &nodes[cur];
-        const float ffr = far_ratio<DIM>(cn, qv);
+        const float ffr = far_ratio<DIM, LOG>(cn, qv);
         const float tp = fminf(cur_beta / ffr, 1.0f);
         const float pu = sbh_uniform(rseed, 0u, static_cast<uint32_t>(ci),
                                      static_cast<uint32_t>(ss), step);
         if (pu > tp) break;
         event_prob *= tp;
-        const int64_t nxt = contained_child<DIM>(nodes, cur, sp, num_nodes);
+        const int64_t nxt = contained_child<DIM, LOG>(nodes, cur, sp, num_nodes);
         cur_beta = fmaxf(cur_beta, ffr);
         float prev_res[DIM] = {}, cur_res[DIM] = {};
         l3_term<KIND>(cn->contrib, qv, P, prev_res);
-        const int fb = next_bit<DIM>(cn, 0);
+        const int fb = next_bit<DIM, LOG>(cn, 0);
         const uint64_t first = fb < kN ? cn->child[fb] : 0ull;
         for (int32_t o = 0; o < cn->num_children; ++o) {
           const uint64_t c = first + static_cast<uint64_t>(o);
@@ -694,12 +718,12 @@ __global__ void k_l3_stoch(const typename L3T<DIM>::Leaf* __restrict__ leaf,
   store_out<DIM, KIND>(out, q, acc);
 }
 
-template <int DIM>
+template <int DIM, int LOG>
 int check_l3_walk(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                   const SbhBuffer* contrib_bytes, const SbhBuffer* queries,
                   const SbhBuffer* out, int threads, bool vec_out) {
-  using Node = typename L3T<DIM>::Node;
-  using Leaf = typename L3T<DIM>::Leaf;
+  using Node = typename L3T<DIM, LOG>::Node;
+  using Leaf = typename L3T<DIM, LOG>::Leaf;
   if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || !buffer_ok(node_bytes, SBH_DTYPE_U8, -1))
     return fail_invalid("leaf_bytes/node_bytes must be uint8 device buffers");
   if (byte_count(leaf_bytes) % static_cast<int64_t>(sizeof(Leaf)) != 0)
@@ -726,14 +750,14 @@ int check_l3_walk(const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
   return SBH_OK;
 }
 
-template <int DIM, int KIND>
+template <int DIM, int LOG, int KIND>
 int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                    const SbhBuffer* queries, SbhBuffer* out, float beta, float softening,
                    float cutoff, int periodic, float box, int threads) {
-  using Node = typename L3T<DIM>::Node;
-  using Leaf = typename L3T<DIM>::Leaf;
-  int rc = check_l3_walk<DIM>(leaf_bytes, node_bytes, nullptr, queries, out, threads,
-                              KIND != kPotential);
+  using Node = typename L3T<DIM, LOG>::Node;
+  using Leaf = typename L3T<DIM, LOG>::Leaf;
+  int rc = check_l3_walk<DIM, LOG>(leaf_bytes, node_bytes, nullptr, queries, out, threads,
+                              !kScalarL3<KIND>);
   if (rc != SBH_OK) return rc;
   const int nq = static_cast<int>(queries->dims[0]);
   if (nq == 0) return SBH_OK;
@@ -751,18 +775,18 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   const auto* nodes = static_cast<const Node*>(node_bytes->data);
   // the rope needs a workspace: not under stream capture (no allocation there)
   Rope<DIM>* rope = nullptr;
-  if (num_nodes >= kRopeMin && !stream_capturing(s))
+  if (kRopeKind<KIND> && num_nodes >= kRopeMin && !stream_capturing(s))
     rope = static_cast<Rope<DIM>*>(
         workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(Rope<DIM>)));
   if (rope != nullptr) {
-    k_l3_rope<DIM><<<grid_for(num_nodes, kB), kB, 0, s>>>(nodes, num_nodes, num_leaves, rope);
+    k_l3_rope<DIM, LOG><<<grid_for(num_nodes, kB), kB, 0, s>>>(nodes, num_nodes, num_leaves, rope);
     SBH_LAUNCH_CHECK();
-    k_l3_walk_rope<DIM, KIND><<<blocks, threads, 0, s>>>(
+    k_l3_walk_rope<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
         static_cast<const Leaf*>(leaf_bytes->data), rope,
         static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
         num_nodes);
   } else {
-    k_l3_walk<DIM, KIND><<<blocks, threads, 0, s>>>(
+    k_l3_walk<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
         static_cast<const Leaf*>(leaf_bytes->data), nodes,
         static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
         num_nodes, num_leaves);
@@ -771,15 +795,15 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   return SBH_OK;
 }
 
-template <int DIM, int KIND>
+template <int DIM, int LOG, int KIND>
 int launch_l3_stoch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
                     const SbhBuffer* contrib_bytes, const SbhBuffer* queries, SbhBuffer* out,
                     int samples, int seed, float softening, float cutoff, int periodic,
-                    float box, int threads) {
-  using Node = typename L3T<DIM>::Node;
-  using Leaf = typename L3T<DIM>::Leaf;
-  int rc = check_l3_walk<DIM>(leaf_bytes, node_bytes, contrib_bytes, queries, out, threads,
-                              KIND != kPotential);
+                    float box, int threads, uint32_t seed_per_query = 1u) {
+  using Node = typename L3T<DIM, LOG>::Node;
+  using Leaf = typename L3T<DIM, LOG>::Leaf;
+  int rc = check_l3_walk<DIM, LOG>(leaf_bytes, node_bytes, contrib_bytes, queries, out, threads,
+                              !kScalarL3<KIND>);
   if (rc != SBH_OK) return rc;
   if (contrib_bytes == nullptr) return fail_invalid("contrib_bytes must be given");
   const int nq = static_cast<int>(queries->dims[0]);
@@ -787,13 +811,104 @@ int launch_l3_stoch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* 
   o8::WalkParams P = o8::make_walk_params(1.0f, softening, cutoff, 0, 0.f, periodic, box);
   P.cutoff = (cutoff > 0.0f && std::isfinite(cutoff)) ? 1 : 0;
   const unsigned blocks = static_cast<unsigned>((nq + threads - 1) / threads);
-  k_l3_stoch<DIM, KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
+  k_l3_stoch<DIM, LOG, KIND><<<blocks, threads, 0, static_cast<hipStream_t>(stream)>>>(
       static_cast<const Leaf*>(leaf_bytes->data), static_cast<const Node*>(node_bytes->data),
       static_cast<const Leaf*>(contrib_bytes->data), static_cast<const float*>(queries->data),
-      nq, static_cast<float*>(out->data), P, samples, seed,
+      nq, static_cast<float*>(out->data), P, samples, seed, seed_per_query,
       static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node)),
       static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf)),
       static_cast<uint32_t>(byte_count(contrib_bytes) / sizeof(Leaf)));
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
+// --------------------------------------------------- pybind11 GPU module
+// The reference's GPU module (src/sbh_gpu.cu) evaluates gravityPotential,
+// signedSolidAngle<N> and smoothDistExp over Tree2D / Tree3D<1, L> buffers:
+// brute force (compensated sums over the leaf records in tree order,
+// include/brute_force_gpu.h:18-66), the vote walk and the control-variate
+// sampler, all with 32-thread blocks; the sampler passes seed 0 to every
+// query (src/sbh_gpu.cu:143-211, 436-550).
+constexpr int kModuleThreads = 32;
+
+template <int DIM, int KIND>
+__global__ __launch_bounds__(kB) void k_tree_brute(const LeafT<DIM>* __restrict__ leaf,
+                                                   int64_t n, const float* __restrict__ queries,
+                                                   int nq, float* __restrict__ out,
+                                                   o8::WalkParams P) {
+  __shared__ LeafT<DIM> tile[kB];
+  const int q = blockIdx.x * kB + threadIdx.x;
+  float qv[DIM] = {};
+  if (q < nq)
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
+  float total = 0.0f, comp = 0.0f;
+  for (int64_t i = 0; i < n; i += kB) {
+    if (i + threadIdx.x < n) tile[threadIdx.x] = leaf[i + threadIdx.x];
+    __syncthreads();
+    if (q < nq) {
+      const int cnt = static_cast<int>(n - i < kB ? n - i : kB);
+      for (int b = 0; b < cnt; ++b) {
+        float v[DIM] = {};
+        l3_term<KIND>(tile[b], qv, P, v);
+        {
+#pragma clang fp contract(off)
+          const float y = v[0] - comp;
+          const float t = total + y;
+          comp = (t - total) - y;
+          total = t;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (q < nq) out[q] = total;
+}
+
+template <int DIM, int LOG>
+int module_eval(void* stream, int kind, const SbhBuffer* leaf, const SbhBuffer* node,
+                const SbhBuffer* contrib, const SbhBuffer* q, SbhBuffer* out, float beta,
+                int samples) {
+  constexpr int T = kModuleThreads;
+  if (samples < 0) {
+    if (kind == 0)
+      return launch_l3_walk<DIM, LOG, kPotential>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0,
+                                                  1.f, T);
+    if (kind == 1)
+      return launch_l3_walk<DIM, LOG, kWn>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0, 1.f,
+                                           T);
+    return launch_l3_walk<DIM, LOG, kSmooth>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0,
+                                             1.f, T);
+  }
+  if (kind == 0)
+    return launch_l3_stoch<DIM, LOG, kPotential>(stream, leaf, node, contrib, q, out, samples,
+                                                 0, 0.f, 0.f, 0, 1.f, T, 0u);
+  if (kind == 1)
+    return launch_l3_stoch<DIM, LOG, kWn>(stream, leaf, node, contrib, q, out, samples, 0, 0.f,
+                                          0.f, 0, 1.f, T, 0u);
+  return launch_l3_stoch<DIM, LOG, kSmooth>(stream, leaf, node, contrib, q, out, samples, 0,
+                                            0.f, 0.f, 0, 1.f, T, 0u);
+}
+
+template <int DIM>
+int module_eval_level(void* stream, int level, int kind, const SbhBuffer* leaf,
+                      const SbhBuffer* node, const SbhBuffer* contrib, const SbhBuffer* q,
+                      SbhBuffer* out, float beta, int samples) {
+  if (level == 1) return module_eval<DIM, 1>(stream, kind, leaf, node, contrib, q, out, beta, samples);
+  if (level == 2) return module_eval<DIM, 2>(stream, kind, leaf, node, contrib, q, out, beta, samples);
+  return module_eval<DIM, 3>(stream, kind, leaf, node, contrib, q, out, beta, samples);
+}
+
+template <int DIM, int KIND>
+int launch_brute(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* queries,
+                 SbhBuffer* out) {
+  const int nq = static_cast<int>(queries->dims[0]);
+  if (nq == 0) return SBH_OK;
+  const int64_t n = byte_count(leaf_bytes) / static_cast<int64_t>(sizeof(LeafT<DIM>));
+  const o8::WalkParams P = o8::make_walk_params(1.0f, 0.f, 0.f, 0, 0.f, 0, 1.f);
+  k_tree_brute<DIM, KIND><<<grid_for(nq, kB), kB, 0, static_cast<hipStream_t>(stream)>>>(
+      static_cast<const LeafT<DIM>*>(leaf_bytes->data), n,
+      static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), P);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }
@@ -886,7 +1001,7 @@ extern "C" int sbh_barnes_hut_gravity_3d(void* stream, const SbhBuffer* leaf_byt
                                          SbhBuffer* out, float beta, int32_t periodic,
                                          float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<3, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+  return launch_l3_walk<3, 3, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
                                        0.f, periodic, box_length, threads);
 }
 
@@ -895,7 +1010,7 @@ extern "C" int sbh_barnes_hut_force_3d(void* stream, const SbhBuffer* leaf_bytes
                                        SbhBuffer* out, float beta, int32_t periodic,
                                        float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<3, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+  return launch_l3_walk<3, 3, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
                                    0.f, periodic, box_length, threads);
 }
 
@@ -906,7 +1021,7 @@ extern "C" int sbh_barnes_hut_force_soft_3d(void* stream, const SbhBuffer* leaf_
                                             float cutoff_scale, int32_t periodic,
                                             float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<3, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
+  return launch_l3_walk<3, 3, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
                                         softening_scale, cutoff_scale, periodic, box_length,
                                         threads);
 }
@@ -919,7 +1034,7 @@ extern "C" int sbh_stochastic_bh_gravity_3d(void* stream, const SbhBuffer* leaf_
                                             int32_t periodic, float box_length,
                                             int32_t threads) {
   clear_error();
-  return launch_l3_stoch<3, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+  return launch_l3_stoch<3, 3, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
                                         out, samples_per_subdomain, seed, 0.f, 0.f, periodic,
                                         box_length, threads);
 }
@@ -931,7 +1046,7 @@ extern "C" int sbh_stochastic_bh_force_3d(void* stream, const SbhBuffer* leaf_by
                                           int32_t samples_per_subdomain, int32_t seed,
                                           int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_stoch<3, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
+  return launch_l3_stoch<3, 3, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
                                     samples_per_subdomain, seed, 0.f, 0.f, periodic, box_length,
                                     threads);
 }
@@ -942,7 +1057,7 @@ extern "C" int sbh_stochastic_bh_force_soft_3d(
     int32_t samples_per_subdomain, int32_t seed, float softening_scale, float cutoff_scale,
     int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_stoch<3, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+  return launch_l3_stoch<3, 3, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
                                          out, samples_per_subdomain, seed, softening_scale,
                                          cutoff_scale, periodic, box_length, threads);
 }
@@ -954,7 +1069,7 @@ extern "C" int sbh_barnes_hut_gravity(void* stream, const SbhBuffer* leaf_bytes,
                                       SbhBuffer* out, float beta, int32_t periodic,
                                       float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<2, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+  return launch_l3_walk<2, 3, kPotential>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
                                        0.f, periodic, box_length, threads);
 }
 
@@ -963,7 +1078,7 @@ extern "C" int sbh_barnes_hut_force(void* stream, const SbhBuffer* leaf_bytes,
                                     SbhBuffer* out, float beta, int32_t periodic,
                                     float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<2, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
+  return launch_l3_walk<2, 3, kForce>(stream, leaf_bytes, node_bytes, queries, out, beta, 0.f,
                                    0.f, periodic, box_length, threads);
 }
 
@@ -973,7 +1088,7 @@ extern "C" int sbh_barnes_hut_force_soft(void* stream, const SbhBuffer* leaf_byt
                                          float cutoff_scale, int32_t periodic,
                                          float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_walk<2, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
+  return launch_l3_walk<2, 3, kSoftForce3>(stream, leaf_bytes, node_bytes, queries, out, beta,
                                         softening_scale, cutoff_scale, periodic, box_length,
                                         threads);
 }
@@ -985,7 +1100,7 @@ extern "C" int sbh_stochastic_bh_gravity(void* stream, const SbhBuffer* leaf_byt
                                          int32_t samples_per_subdomain, int32_t seed,
                                          int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_stoch<2, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+  return launch_l3_stoch<2, 3, kPotential>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
                                         out, samples_per_subdomain, seed, 0.f, 0.f, periodic,
                                         box_length, threads);
 }
@@ -997,7 +1112,7 @@ extern "C" int sbh_stochastic_bh_force(void* stream, const SbhBuffer* leaf_bytes
                                        int32_t seed, int32_t periodic, float box_length,
                                        int32_t threads) {
   clear_error();
-  return launch_l3_stoch<2, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
+  return launch_l3_stoch<2, 3, kForce>(stream, leaf_bytes, node_bytes, contrib_bytes, queries, out,
                                     samples_per_subdomain, seed, 0.f, 0.f, periodic, box_length,
                                     threads);
 }
@@ -1008,7 +1123,52 @@ extern "C" int sbh_stochastic_bh_force_soft(
     int32_t samples_per_subdomain, int32_t seed, float softening_scale, float cutoff_scale,
     int32_t periodic, float box_length, int32_t threads) {
   clear_error();
-  return launch_l3_stoch<2, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
+  return launch_l3_stoch<2, 3, kSoftForce3>(stream, leaf_bytes, node_bytes, contrib_bytes, queries,
                                          out, samples_per_subdomain, seed, softening_scale,
                                          cutoff_scale, periodic, box_length, threads);
+}
+
+// The pybind11 GPU module's evaluations (src/sbh_gpu.cu:324-550): kind 0
+// potential, 1 winding number, 2 smooth distance; samples < 0: vote walk with
+// `beta`, samples >= 0: control-variate sampler.
+extern "C" int sbh_tree_eval(void* stream, int32_t dim, int32_t level, int32_t kind,
+                             const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+                             const SbhBuffer* contrib_bytes, const SbhBuffer* queries,
+                             SbhBuffer* out, float beta, int32_t samples_per_subdomain) {
+  clear_error();
+  if (dim != 2 && dim != 3) return fail_invalid("dim must be 2 or 3");
+  if (level < 1 || level > 3) return fail_invalid("level must be 1, 2 or 3");
+  if (kind < 0 || kind > 2) return fail_invalid("kind must be 0 (gravity), 1 (wn) or 2 (smoothdist)");
+  const SbhBuffer* c = samples_per_subdomain >= 0 ? contrib_bytes : nullptr;
+  if (samples_per_subdomain >= 0 && contrib_bytes == nullptr)
+    return fail_invalid("contrib_bytes must be given for the sampler");
+  return dim == 2 ? module_eval_level<2>(stream, level, kind, leaf_bytes, node_bytes, c, queries,
+                                         out, beta, samples_per_subdomain)
+                  : module_eval_level<3>(stream, level, kind, leaf_bytes, node_bytes, c, queries,
+                                         out, beta, samples_per_subdomain);
+}
+
+// evalBruteForce_{gravity,wn,smoothdist} (src/sbh_gpu.cu:39-77, 213-322).
+extern "C" int sbh_tree_bruteforce(void* stream, int32_t dim, int32_t kind,
+                                   const SbhBuffer* leaf_bytes, const SbhBuffer* queries,
+                                   SbhBuffer* out) {
+  clear_error();
+  if (dim != 2 && dim != 3) return fail_invalid("dim must be 2 or 3");
+  if (kind < 0 || kind > 2) return fail_invalid("kind must be 0 (gravity), 1 (wn) or 2 (smoothdist)");
+  const int64_t lsz = dim == 2 ? sizeof(Leaf2) : sizeof(Leaf3);
+  if (!buffer_ok(leaf_bytes, SBH_DTYPE_U8, -1) || byte_count(leaf_bytes) % lsz != 0)
+    return fail_invalid("leaf_bytes size mismatch");
+  if (!buffer_ok(queries, SBH_DTYPE_F32, 2) || queries->dims[1] != dim)
+    return fail_invalid(dim == 2 ? "queries dim must be 2" : "queries dim must be 3");
+  if (queries->dims[0] > INT32_MAX) return fail_invalid("too many queries");
+  if (!buffer_ok(out, SBH_DTYPE_F32, 1) || out->dims[0] != queries->dims[0])
+    return fail_invalid("out shape mismatch");
+  if (dim == 2) {
+    if (kind == 0) return launch_brute<2, kPotential>(stream, leaf_bytes, queries, out);
+    if (kind == 1) return launch_brute<2, kWn>(stream, leaf_bytes, queries, out);
+    return launch_brute<2, kSmooth>(stream, leaf_bytes, queries, out);
+  }
+  if (kind == 0) return launch_brute<3, kPotential>(stream, leaf_bytes, queries, out);
+  if (kind == 1) return launch_brute<3, kWn>(stream, leaf_bytes, queries, out);
+  return launch_brute<3, kSmooth>(stream, leaf_bytes, queries, out);
 }

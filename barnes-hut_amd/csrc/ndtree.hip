@@ -41,21 +41,13 @@ namespace {
 constexpr int kB = 256;
 constexpr int kMaxDepthNd = 100;  // PyQuadtree / PyOctree (include/py_tree.h:41, 69)
 
-template <int DIM>
-struct NdT;
-template <>
-struct NdT<2> {
-  using Node = Node2;
-  using Leaf = Leaf2;
-  static constexpr int kCellBits = 6;
-  static constexpr int kWords = 1;
-};
-template <>
-struct NdT<3> {
-  using Node = Node3;
-  using Leaf = Leaf3;
-  static constexpr int kCellBits = 9;
-  static constexpr int kWords = 8;
+template <int DIM, int LOG>
+struct NdT {
+  using Node = NodeT<DIM, LOG>;
+  using Leaf = LeafT<DIM>;
+  static constexpr int kSplit = 1 << LOG;        // cells per axis
+  static constexpr int kCellBits = DIM * LOG;
+  static constexpr int kWords = Node::kWords;
 };
 
 // Node arrays of the tree under construction (structure of arrays; the
@@ -79,27 +71,29 @@ struct NodeArrays {
 };
 
 // AABB::quantize (include/aabb.h:36-42) as the reference's x86 build
-// computes it: (int)(8 (p - lo) / (hi - lo)), where the conversion of a NaN
-// or out-of-range value gives INT_MIN, then min(., 7), max(., 0).
+// computes it: (int)(S (p - lo) / (hi - lo)), where the conversion of a NaN
+// or out-of-range value gives INT_MIN, then min(., S - 1), max(., 0).
+template <int S>
 __device__ __forceinline__ int quantize_host(float p, float lo, float hi) {
 #pragma clang fp contract(off)
-  const float t = 8.0f * (p - lo) / (hi - lo);
+  const float t = static_cast<float>(S) * (p - lo) / (hi - lo);
   int c = (t >= -2147483648.0f && t < 2147483648.0f) ? static_cast<int>(t) : INT_MIN;
-  c = min(c, 7);
+  c = min(c, S - 1);
   return max(c, 0);
 }
 
-template <int DIM>
+template <int DIM, int LOG>
 __device__ __forceinline__ int point_cell(const float* __restrict__ pts, uint32_t i,
                                           const float* lo, const float* hi) {
   int c = 0;
 #pragma unroll
   for (int a = DIM - 1; a >= 0; --a)
-    c = (c << 3) | quantize_host(pts[static_cast<int64_t>(DIM) * i + a], lo[a], hi[a]);
+    c = (c << LOG) |
+        quantize_host<(1 << LOG)>(pts[static_cast<int64_t>(DIM) * i + a], lo[a], hi[a]);
   return c;
 }
 
-template <int DIM>
+template <int DIM, int LOG>
 __global__ __launch_bounds__(kB) void k_nd_init(int64_t n, const float* __restrict__ lo,
                                                 const float* __restrict__ hi,
                                                 uint32_t* __restrict__ pidx,
@@ -124,12 +118,12 @@ __global__ __launch_bounds__(kB) void k_nd_init(int64_t n, const float* __restri
       N.lo[a] = lo[a];
       N.hi[a] = hi[a];
     }
-    for (int w = 0; w < NdT<DIM>::kWords; ++w) N.mask[w] = 0ull;
+    for (int w = 0; w < NdT<DIM, LOG>::kWords; ++w) N.mask[w] = 0ull;
   }
 }
 
 // Step 1: (node, cell) keys of the active points.
-template <int DIM>
+template <int DIM, int LOG>
 __global__ __launch_bounds__(kB) void k_nd_keys(const float* __restrict__ pts,
                                                 const uint32_t* __restrict__ act, int64_t na,
                                                 const uint32_t* __restrict__ pidx,
@@ -141,9 +135,9 @@ __global__ __launch_bounds__(kB) void k_nd_keys(const float* __restrict__ pts,
   const uint32_t pos = act[k];
   const uint32_t i = pidx[pos];
   const uint32_t node = pnode[pos];
-  const int cell = point_cell<DIM>(pts, i, N.lo + static_cast<int64_t>(DIM) * node,
+  const int cell = point_cell<DIM, LOG>(pts, i, N.lo + static_cast<int64_t>(DIM) * node,
                                    N.hi + static_cast<int64_t>(DIM) * node);
-  keys[k] = ((node - a) << NdT<DIM>::kCellBits) | static_cast<uint32_t>(cell);
+  keys[k] = ((node - a) << NdT<DIM, LOG>::kCellBits) | static_cast<uint32_t>(cell);
   vals[k] = i;
 }
 
@@ -157,9 +151,9 @@ __global__ __launch_bounds__(kB) void k_nd_run_flags(const uint32_t* __restrict_
 }
 
 // Step 3b: children records and the points' new slots and nodes.  Child
-// bounds as the reference's partition loop: inc = (max - min) / 8, lo = min
+// bounds as the reference's partition loop: inc = (max - min) / S, lo = min
 // + k inc, hi = lo + inc, in fp32 without contraction.
-template <int DIM>
+template <int DIM, int LOG>
 __global__ __launch_bounds__(kB) void k_nd_children(const uint32_t* __restrict__ keys,
                                                     const uint32_t* __restrict__ vals,
                                                     const uint32_t* __restrict__ runs,
@@ -170,7 +164,8 @@ __global__ __launch_bounds__(kB) void k_nd_children(const uint32_t* __restrict__
 #pragma clang fp contract(off)
   const int64_t k = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (k >= na) return;
-  constexpr int CB = NdT<DIM>::kCellBits;
+  constexpr int CB = NdT<DIM, LOG>::kCellBits;
+  constexpr int S = NdT<DIM, LOG>::kSplit;
   const uint32_t key = keys[k];
   const bool first_of_run = k == 0 || keys[k - 1] != key;
   const bool last_of_run = k == na - 1 || keys[k + 1] != key;
@@ -187,12 +182,13 @@ __global__ __launch_bounds__(kB) void k_nd_children(const uint32_t* __restrict__
     N.depth[c] = depth + 1;
     N.num[c] = 0;
     N.first[c] = 0u;
-    for (int w = 0; w < NdT<DIM>::kWords; ++w) N.mask[static_cast<int64_t>(NdT<DIM>::kWords) * c + w] = 0ull;
+    for (int w = 0; w < NdT<DIM, LOG>::kWords; ++w)
+      N.mask[static_cast<int64_t>(NdT<DIM, LOG>::kWords) * c + w] = 0ull;
     for (int ax = 0; ax < DIM; ++ax) {
       const float plo = N.lo[static_cast<int64_t>(DIM) * parent + ax];
       const float phi = N.hi[static_cast<int64_t>(DIM) * parent + ax];
-      const float inc = (phi - plo) / 8.0f;
-      const uint32_t q = (cell >> (3 * ax)) & 7u;
+      const float inc = (phi - plo) / static_cast<float>(S);
+      const uint32_t q = (cell >> (LOG * ax)) & static_cast<uint32_t>(S - 1);
       const float lo = plo + static_cast<float>(q) * inc;
       N.lo[static_cast<int64_t>(DIM) * c + ax] = lo;
       N.hi[static_cast<int64_t>(DIM) * c + ax] = lo + inc;
@@ -215,22 +211,23 @@ __global__ __launch_bounds__(kB) void k_nd_leaf_flags(const uint32_t* __restrict
 
 // Step 4: the parents' masks and child counts (children are contiguous from
 // `first`, in cell order).
-template <int DIM>
+template <int DIM, int LOG>
 __global__ __launch_bounds__(kB) void k_nd_parents(uint32_t a, uint32_t b,
                                                    const uint32_t* __restrict__ run_count,
                                                    NodeArrays N) {
   const uint32_t p = a + blockIdx.x * kB + threadIdx.x;
   if (p >= b || N.leaf[p]) return;
   const uint32_t end = b + *run_count;
-  uint64_t m[NdT<DIM>::kWords];
-  for (int w = 0; w < NdT<DIM>::kWords; ++w) m[w] = 0ull;
+  constexpr int kW = NdT<DIM, LOG>::kWords;
+  uint64_t m[kW];
+  for (int w = 0; w < kW; ++w) m[w] = 0ull;
   int cnt = 0;
   for (uint32_t c = N.first[p]; c < end && N.parent[c] == p; ++c) {
     const uint32_t cell = N.cell[c];
     m[cell >> 6] |= 1ull << (cell & 63u);
     ++cnt;
   }
-  for (int w = 0; w < NdT<DIM>::kWords; ++w) N.mask[static_cast<int64_t>(NdT<DIM>::kWords) * p + w] = m[w];
+  for (int w = 0; w < kW; ++w) N.mask[static_cast<int64_t>(kW) * p + w] = m[w];
   N.num[p] = cnt;
 }
 
@@ -346,14 +343,15 @@ __global__ __launch_bounds__(kB) void k_nd_emit_contrib(int64_t nn, NodeArrays N
   o[2 * DIM + 1] = N.calpha[c];
 }
 
-// One 32-bit word of a node record per thread (records are 154 / 1068
-// words), so that the stores of a wave are contiguous.
-template <int DIM>
+// One 32-bit word of a node record per thread (e.g. 154 words for
+// Tree2DNode<1,3>, 1068 for Tree3DNode<1,3>), so that the stores of a wave
+// are contiguous.
+template <int DIM, int LOG>
 __global__ __launch_bounds__(kB) void k_nd_emit_nodes(int64_t nn, NodeArrays N,
                                                       uint32_t* __restrict__ out) {
-  using Node = typename NdT<DIM>::Node;
+  using Node = typename NdT<DIM, LOG>::Node;
   constexpr int64_t kW = sizeof(Node) / 4;
-  constexpr int kWords = NdT<DIM>::kWords;
+  constexpr int kWords = NdT<DIM, LOG>::kWords;
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (t >= nn * kW) return;
   const int64_t c = t / kW;
@@ -429,11 +427,13 @@ __global__ __launch_bounds__(kB) void k_nd_emit_nodes(int64_t nn, NodeArrays N,
 // node arrays grown (with copy) as levels are added.
 struct NdState {
   int dim = 0;
+  int level = 3;
   int64_t n = 0;
   int64_t num_nodes = 0;
   int64_t pt_cap = -1;
   int64_t node_cap = 0;
-  int node_dim = 0;  // dimension the node arrays were carved for
+  int node_dim = 0;  // dimension and level the node arrays were carved for
+  int node_level = 0;
   void* pt_block = nullptr;
   void* node_block = nullptr;
   uint32_t* host_counts = nullptr;  // pinned, 2 words
@@ -449,15 +449,20 @@ std::map<std::tuple<int, hipStream_t>, NdState> g_nd;
 
 size_t align256(size_t x) { return (x + 255) & ~size_t{255}; }
 
-size_t node_bytes_per(int dim) {
-  const int words = dim == 2 ? 1 : 8;
+int mask_words(int dim, int level) {
+  const int cells = 1 << (dim * level);
+  return cells >= 64 ? cells / 64 : 1;
+}
+
+size_t node_bytes_per(int dim, int level) {
+  const int words = mask_words(dim, level);
   return 4 * 8 + 4 * 2 * dim + 8 * words + 4 * (2 + 2 * dim);
 }
 
 // Carves the node arrays for `cap` nodes out of `base`.
-NodeArrays carve_nodes(char* base, int64_t cap, int dim) {
+NodeArrays carve_nodes(char* base, int64_t cap, int dim, int level) {
   NodeArrays N;
-  const int words = dim == 2 ? 1 : 8;
+  const int words = mask_words(dim, level);
   size_t o = 0;
   auto take = [&](size_t bytes) {
     char* p = base + o;
@@ -483,14 +488,14 @@ NodeArrays carve_nodes(char* base, int64_t cap, int dim) {
   return N;
 }
 
-size_t node_block_bytes(int64_t cap, int dim) {
-  return node_bytes_per(dim) * static_cast<size_t>(cap) + 16 * 256;
+size_t node_block_bytes(int64_t cap, int dim, int level) {
+  return node_bytes_per(dim, level) * static_cast<size_t>(cap) + 16 * 256;
 }
 
 // Grows the node arrays to hold at least `need` nodes, keeping the first
 // `keep` entries of every array.
 int grow_nodes(NdState& S, hipStream_t s, int64_t need, int64_t keep) {
-  if (S.node_block != nullptr && S.node_dim != S.dim) {
+  if (S.node_block != nullptr && (S.node_dim != S.dim || S.node_level != S.level)) {
     SBH_HIP_TRY(hipStreamSynchronize(s));
     SBH_HIP_TRY(hipFree(S.node_block));
     S.node_block = nullptr;
@@ -501,12 +506,12 @@ int grow_nodes(NdState& S, hipStream_t s, int64_t need, int64_t keep) {
   int64_t cap = S.node_cap > 0 ? S.node_cap : 1024;
   while (cap < need) cap *= 2;
   void* blk = nullptr;
-  SBH_HIP_TRY(hipMalloc(&blk, node_block_bytes(cap, S.dim)));
-  NodeArrays M = carve_nodes(static_cast<char*>(blk), cap, S.dim);
+  SBH_HIP_TRY(hipMalloc(&blk, node_block_bytes(cap, S.dim, S.level)));
+  NodeArrays M = carve_nodes(static_cast<char*>(blk), cap, S.dim, S.level);
   if (S.node_block != nullptr && keep > 0) {
     const NodeArrays& O = S.N;
     const size_t k = static_cast<size_t>(keep);
-    const int words = S.dim == 2 ? 1 : 8;
+    const int words = mask_words(S.dim, S.level);
     const std::pair<void*, const void*> arr[] = {
         {M.start, O.start}, {M.end, O.end},     {M.parent, O.parent}, {M.first, O.first},
         {M.cell, O.cell},   {M.num, O.num},     {M.depth, O.depth},   {M.leaf, O.leaf},
@@ -527,6 +532,7 @@ int grow_nodes(NdState& S, hipStream_t s, int64_t need, int64_t keep) {
   S.node_block = blk;
   S.node_cap = cap;
   S.node_dim = S.dim;
+  S.node_level = S.level;
   S.N = M;
   return SBH_OK;
 }
@@ -589,16 +595,16 @@ int bits_for(uint64_t count) {  // bits of the largest index count - 1
   return b;
 }
 
-template <int DIM>
+template <int DIM, int LOG>
 int build_nd(NdState& S, hipStream_t s, const float* pts, const float* nrm, const float* ms,
              const float* lo, const float* hi, int64_t n, float alpha) {
-  constexpr int CB = NdT<DIM>::kCellBits;
+  constexpr int CB = NdT<DIM, LOG>::kCellBits;
   int rc = prepare_points(S, s, n);
   if (rc != SBH_OK) return rc;
   rc = grow_nodes(S, s, 1, 0);
   if (rc != SBH_OK) return rc;
   std::vector<int64_t> level{0, 1};
-  k_nd_init<DIM><<<grid_for(n, kB), kB, 0, s>>>(n, lo, hi, S.pidx, S.pnode, S.act, S.N);
+  k_nd_init<DIM, LOG><<<grid_for(n, kB), kB, 0, s>>>(n, lo, hi, S.pidx, S.pnode, S.act, S.N);
   SBH_LAUNCH_CHECK();
   int64_t na = n > 1 ? n : 0;
   int32_t depth = 0;
@@ -609,8 +615,8 @@ int build_nd(NdState& S, hipStream_t s, const float* pts, const float* nrm, cons
     rc = grow_nodes(S, s, b + na, b);
     if (rc != SBH_OK) return rc;
     const unsigned g = grid_for(na, kB), g1 = grid_for(na + 1, kB);
-    k_nd_keys<DIM><<<g, kB, 0, s>>>(pts, S.act, na, S.pidx, S.pnode, static_cast<uint32_t>(a),
-                                    S.N, S.keys, S.vals);
+    k_nd_keys<DIM, LOG><<<g, kB, 0, s>>>(pts, S.act, na, S.pidx, S.pnode,
+                                         static_cast<uint32_t>(a), S.N, S.keys, S.vals);
     SBH_LAUNCH_CHECK();
     rc = radix_sort_pairs(s, S.keys, S.vals, S.keys2, S.vals2, na, key_bits, S.sort_temp);
     if (rc != SBH_OK) return rc;
@@ -618,13 +624,14 @@ int build_nd(NdState& S, hipStream_t s, const float* pts, const float* nrm, cons
     SBH_LAUNCH_CHECK();
     rc = exclusive_scan(s, S.runs, na + 1, S.scan_temp);
     if (rc != SBH_OK) return rc;
-    k_nd_children<DIM><<<g, kB, 0, s>>>(S.keys2, S.vals2, S.runs, S.act, na,
-                                        static_cast<uint32_t>(a), static_cast<uint32_t>(b),
-                                        depth, S.pidx, S.pnode, S.N);
+    k_nd_children<DIM, LOG><<<g, kB, 0, s>>>(S.keys2, S.vals2, S.runs, S.act, na,
+                                             static_cast<uint32_t>(a),
+                                             static_cast<uint32_t>(b), depth, S.pidx,
+                                             S.pnode, S.N);
     SBH_LAUNCH_CHECK();
     k_nd_leaf_flags<<<g, kB, 0, s>>>(S.keys2, S.runs, na, static_cast<uint32_t>(b), S.N);
     SBH_LAUNCH_CHECK();
-    k_nd_parents<DIM><<<grid_for(b - a, kB), kB, 0, s>>>(
+    k_nd_parents<DIM, LOG><<<grid_for(b - a, kB), kB, 0, s>>>(
         static_cast<uint32_t>(a), static_cast<uint32_t>(b), S.runs + na, S.N);
     SBH_LAUNCH_CHECK();
     k_nd_active_flags<<<g1, kB, 0, s>>>(S.act, na, S.pnode, S.N, S.acts);
@@ -659,7 +666,38 @@ int build_nd(NdState& S, hipStream_t s, const float* pts, const float* nrm, cons
   }
   S.n = n;
   S.dim = DIM;
+  S.level = LOG;
   return SBH_OK;
+}
+
+int64_t node_size(int dim, int level) {
+  if (dim == 2) return level == 1 ? sizeof(NodeT<2, 1>) : level == 2 ? sizeof(NodeT<2, 2>) : sizeof(NodeT<2, 3>);
+  return level == 1 ? sizeof(NodeT<3, 1>) : level == 2 ? sizeof(NodeT<3, 2>) : sizeof(NodeT<3, 3>);
+}
+
+template <int DIM>
+int build_dim(int level, NdState& S, hipStream_t s, const float* pts, const float* nrm,
+              const float* ms, const float* lo, const float* hi, int64_t n, float alpha) {
+  if (level == 1) return build_nd<DIM, 1>(S, s, pts, nrm, ms, lo, hi, n, alpha);
+  if (level == 2) return build_nd<DIM, 2>(S, s, pts, nrm, ms, lo, hi, n, alpha);
+  return build_nd<DIM, 3>(S, s, pts, nrm, ms, lo, hi, n, alpha);
+}
+
+template <int DIM, int LOG>
+void emit_nodes(hipStream_t s, const NdState& S, void* out) {
+  const int64_t words = S.num_nodes * static_cast<int64_t>(sizeof(NodeT<DIM, LOG>)) / 4;
+  k_nd_emit_nodes<DIM, LOG><<<grid_for(words, kB), kB, 0, s>>>(S.num_nodes, S.N,
+                                                              static_cast<uint32_t*>(out));
+}
+
+template <int DIM>
+void emit_nodes_dim(hipStream_t s, const NdState& S, void* out) {
+  if (S.level == 1)
+    emit_nodes<DIM, 1>(s, S, out);
+  else if (S.level == 2)
+    emit_nodes<DIM, 2>(s, S, out);
+  else
+    emit_nodes<DIM, 3>(s, S, out);
 }
 
 NdState& state_for(hipStream_t s) {
@@ -676,12 +714,13 @@ using namespace sbh;
 extern "C" size_t sbh_leaf2_size(void) { return sizeof(Leaf2); }
 extern "C" size_t sbh_node2_size(void) { return sizeof(Node2); }
 
-extern "C" int sbh_tree_build(void* stream, int32_t dim, const SbhBuffer* points,
-                              const SbhBuffer* normals, const SbhBuffer* masses,
-                              const SbhBuffer* min_corner, const SbhBuffer* max_corner,
-                              float alpha, int64_t* num_nodes) {
+extern "C" int sbh_tree_build(void* stream, int32_t dim, int32_t level,
+                              const SbhBuffer* points, const SbhBuffer* normals,
+                              const SbhBuffer* masses, const SbhBuffer* min_corner,
+                              const SbhBuffer* max_corner, float alpha, int64_t* num_nodes) {
   clear_error();
   if (dim != 2 && dim != 3) return fail_invalid("dim must be 2 or 3");
+  if (level < 1 || level > 3) return fail_invalid("level must be 1, 2 or 3");
   if (num_nodes == nullptr) return fail_invalid("num_nodes must not be null");
   if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != dim)
     return fail_invalid(dim == 2 ? "points must have shape (N, 2)" : "points must have shape (N, 3)");
@@ -703,14 +742,15 @@ extern "C" int sbh_tree_build(void* stream, int32_t dim, const SbhBuffer* points
   std::lock_guard<std::mutex> lock(g_nd_mutex);
   NdState& S = state_for(s);
   S.dim = dim;
+  S.level = level;
   S.num_nodes = 0;
   const auto* pts = static_cast<const float*>(points->data);
   const auto* nrm = static_cast<const float*>(normals->data);
   const auto* ms = static_cast<const float*>(masses->data);
   const auto* lo = static_cast<const float*>(min_corner->data);
   const auto* hi = static_cast<const float*>(max_corner->data);
-  const int rc = dim == 2 ? build_nd<2>(S, s, pts, nrm, ms, lo, hi, n, alpha)
-                          : build_nd<3>(S, s, pts, nrm, ms, lo, hi, n, alpha);
+  const int rc = dim == 2 ? build_dim<2>(level, S, s, pts, nrm, ms, lo, hi, n, alpha)
+                          : build_dim<3>(level, S, s, pts, nrm, ms, lo, hi, n, alpha);
   if (rc != SBH_OK) {
     S.num_nodes = 0;
     return rc;
@@ -727,7 +767,7 @@ extern "C" int sbh_tree_emit(void* stream, SbhBuffer* leaf_out, SbhBuffer* node_
   NdState& S = state_for(s);
   if (S.num_nodes <= 0) return fail_invalid("no tree has been built on this stream");
   const int64_t leaf_sz = S.dim == 2 ? sizeof(Leaf2) : sizeof(Leaf3);
-  const int64_t node_sz = S.dim == 2 ? sizeof(Node2) : sizeof(Node3);
+  const int64_t node_sz = node_size(S.dim, S.level);
   if (!buffer_ok(leaf_out, SBH_DTYPE_U8, -1) || byte_count(leaf_out) != S.n * leaf_sz)
     return fail_invalid("leaf_out size mismatch");
   if (!buffer_ok(node_out, SBH_DTYPE_U8, -1) || byte_count(node_out) != S.num_nodes * node_sz)
@@ -739,13 +779,10 @@ extern "C" int sbh_tree_emit(void* stream, SbhBuffer* leaf_out, SbhBuffer* node_
   if (S.n > 0)
     SBH_HIP_TRY(hipMemcpyAsync(leaf_out->data, S.leaf, static_cast<size_t>(S.n * leaf_sz),
                                hipMemcpyDeviceToDevice, s));
-  const int64_t words = S.num_nodes * node_sz / 4;
   if (S.dim == 2)
-    k_nd_emit_nodes<2><<<grid_for(words, kB), kB, 0, s>>>(S.num_nodes, S.N,
-                                                           static_cast<uint32_t*>(node_out->data));
+    emit_nodes_dim<2>(s, S, node_out->data);
   else
-    k_nd_emit_nodes<3><<<grid_for(words, kB), kB, 0, s>>>(S.num_nodes, S.N,
-                                                           static_cast<uint32_t*>(node_out->data));
+    emit_nodes_dim<3>(s, S, node_out->data);
   SBH_LAUNCH_CHECK();
   if (contrib_out != nullptr) {
     if (S.dim == 2)

@@ -13,5 +13,6 @@ from . import graph  # noqa: E402,F401
 from . import synthetic  # noqa: E402,F401
 from . import wn  # noqa: E402,F401
 from . import fields  # noqa: E402,F401
+from . import jax_sbh  # noqa: E402,F401
 
-__all__ += ["gravity", "graph", "synthetic", "wn", "fields"]
+__all__ += ["gravity", "graph", "synthetic", "wn", "fields", "jax_sbh"]

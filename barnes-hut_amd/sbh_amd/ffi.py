@@ -631,9 +631,12 @@ def _l3_queries(queries):
     return queries
 
 
-def _tree_buffers(points, normals, masses, min_corner, max_corner, alpha, dim):
-    """Tree2D<1,3> / Tree3D<1,3> bytes (the reference's sbh.Quadtree3 /
-    sbh.Octree3, include/tree2d.h:237-407, include/tree3d.h:238-413), built on
+NODE_SIZES = {(2, 1): 136, (2, 2): 232, (2, 3): 616, (3, 1): 184, (3, 2): 632, (3, 3): 4272}
+
+
+def _tree_buffers(points, normals, masses, min_corner, max_corner, alpha, dim, level=3):
+    """Tree2D<1,L> / Tree3D<1,L> bytes (the reference's sbh.QuadtreeL /
+    sbh.OctreeL, include/tree2d.h:237-407, include/tree3d.h:238-413), built on
     the device: (leaf, node, contrib) uint8 tensors."""
     points, normals, masses = _f32(points), _f32(normals), _f32(masses)
     if points.dim() != 2 or points.shape[1] != dim:
@@ -645,9 +648,9 @@ def _tree_buffers(points, normals, masses, min_corner, max_corner, alpha, dim):
     mn, mx = _default_corners(points, min_corner, max_corner)
     nodes = ctypes.c_int64(0)
     s = stream_handle()
-    call("sbh_tree_build", s, i32(dim), buf(points), buf(normals), buf(masses), buf(mn),
-         buf(mx), f32(alpha), ctypes.byref(nodes))
-    lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+    call("sbh_tree_build", s, i32(dim), i32(level), buf(points), buf(normals), buf(masses),
+         buf(mn), buf(mx), f32(alpha), ctypes.byref(nodes))
+    lsz, nsz = (24 if dim == 2 else 32), NODE_SIZES[(dim, level)]
     dev = points.device
     leaf = torch.empty((points.shape[0] * lsz,), dtype=torch.uint8, device=dev)
     node = torch.empty((nodes.value * nsz,), dtype=torch.uint8, device=dev)

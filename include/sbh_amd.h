@@ -328,15 +328,18 @@ int sbh_stochastic_bh_force_soft(void* stream, const SbhBuffer* leaf_bytes,
                                  float softening_scale, float cutoff_scale, int32_t periodic,
                                  float box_length, int32_t threads);
 
-/* ---- CPU-layout trees: Tree2D<1,3> and Tree3D<1,3> -------------------------
+/* ---- CPU-layout trees: Tree2D<1,L> and Tree3D<1,L>, L = 1, 2, 3 -------------
  *
- * Replaces the reference's host builders sbh.Quadtree3 / sbh.Octree3
+ * Replaces the reference's host builders sbh.Quadtree1..3 / sbh.Octree1..3
  * (include/tree2d.h:237-407, include/tree3d.h:238-413, include/py_tree.h:
  * 20-73; their byte exports src/sbh.cpp:227-235), used by
  * build_quadtree_buffers(_with_contrib) and build_octree_buffers(backend=
- * "cpu") (python/jax_sbh_ffi.py:658-742).  Leaf size 1, maximum depth 100,
- * 8^dim cells per split, nodes in breadth-first order with each node's
- * children contiguous in cell order, moments aggregated bottom-up in fp32.
+ * "cpu") (python/jax_sbh_ffi.py:658-742, level 3) and by the pybind11 GPU
+ * module's trees (python/jax_sbh.py:132-163, levels 1-3).  Leaf size 1,
+ * maximum depth 100, 2^level cells per axis, nodes in breadth-first order
+ * with each node's children contiguous in cell order, moments aggregated
+ * bottom-up in fp32.  Node records Tree2DNode<1,L> are 136 / 232 / 616 bytes,
+ * Tree3DNode<1,L> 184 / 632 / 4272 bytes for L = 1 / 2 / 3.
  * Built on the GPU in two calls, because the node count sizes the caller's
  * node buffer:
  *   sbh_tree_build: points (N, dim), normals (N, dim), masses (N,),
@@ -344,18 +347,39 @@ int sbh_stochastic_bh_force_soft(void* stream, const SbhBuffer* leaf_bytes,
  *     point's alpha.  Builds into state kept per (device, stream),
  *     synchronizes the stream and returns the node count.  Not capturable.
  *   sbh_tree_emit: writes the last tree built on `stream`: leaf_out N*24 /
- *     N*32 bytes, node_out num_nodes*616 / num_nodes*4272 bytes, contrib_out
- *     num_nodes*24 / num_nodes*32 bytes (may be NULL).
+ *     N*32 bytes, node_out num_nodes * node size, contrib_out num_nodes*24 /
+ *     num_nodes*32 bytes (may be NULL).
  * Points that share a cell for 100 levels form one leaf; their order there
  * is the input order (the reference's std::partition leaves it unspecified);
  * elsewhere every leaf holds one point and the bytes equal the reference's.
  * Limits: N <= 2^26 (dim 2) or 2^23 (dim 3). */
-int sbh_tree_build(void* stream, int32_t dim, const SbhBuffer* points,
+int sbh_tree_build(void* stream, int32_t dim, int32_t level, const SbhBuffer* points,
                    const SbhBuffer* normals, const SbhBuffer* masses,
                    const SbhBuffer* min_corner, const SbhBuffer* max_corner, float alpha,
                    int64_t* num_nodes);
 int sbh_tree_emit(void* stream, SbhBuffer* leaf_out, SbhBuffer* node_out,
                   SbhBuffer* contrib_out);
+
+/* The pybind11 GPU module's evaluations over these trees (python/jax_sbh.py
+ * build_gpu_tree + eval_*; src/sbh_gpu.cu:324-550): kind 0 gravityPotential
+ * -m / (|p - q| + 0.1), 1 signedSolidAngle<dim>, 2 smoothDistExp
+ * exp(-alpha |p - q|); out (M,).  samples_per_subdomain < 0: the warp-voted
+ * walk with `beta`; >= 0: the multi-level control-variate sampler (contrib
+ * required; the module's seed 0 for every query).  Both run in 32-query
+ * blocks, as the module launches them.  Queries (M, dim); leaf / node /
+ * contrib bytes of a Tree{2,3}D<1, level> (sbh_tree_build / sbh_tree_emit).
+ * Replaces evalBarnesHut_{gravity,wn,smoothdist} (src/sbh_gpu.cu:324-434) and
+ * multiLevelPrefixControlVariateSampler_{gravity,wn,smoothdist} (436-550). */
+int sbh_tree_eval(void* stream, int32_t dim, int32_t level, int32_t kind,
+                  const SbhBuffer* leaf_bytes, const SbhBuffer* node_bytes,
+                  const SbhBuffer* contrib_bytes, const SbhBuffer* queries, SbhBuffer* out,
+                  float beta, int32_t samples_per_subdomain);
+
+/* Replaces evalBruteForce_{gravity,wn,smoothdist} (src/sbh_gpu.cu:213-322,
+ * include/brute_force_gpu.h:18-66): per query the compensated (Kahan) fp32
+ * sum of the kind's kernel over the leaf records in their order. */
+int sbh_tree_bruteforce(void* stream, int32_t dim, int32_t kind, const SbhBuffer* leaf_bytes,
+                        const SbhBuffer* queries, SbhBuffer* out);
 
 /* sizeof(LeafData<2>) = 24 and sizeof(Tree2DNode<1,3>) = 616. */
 size_t sbh_leaf2_size(void);

@@ -121,17 +121,20 @@ def _declare(L: ctypes.CDLL) -> None:
                               ctypes.c_int, ctypes.c_float, _f64p]
     L.orc_num_threads.restype = ctypes.c_int
     L.orc_node2_size.restype = ctypes.c_size_t
+    L.orc_nd_node_size.restype = ctypes.c_size_t
+    L.orc_nd_node_size.argtypes = [ctypes.c_int, ctypes.c_int]
     L.orc_nd_build.restype = ctypes.c_void_p
-    L.orc_nd_build.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, _i64, _f32p, _f32p,
-                               ctypes.c_float]
+    L.orc_nd_build.argtypes = [ctypes.c_int, ctypes.c_int, _f32p, _f32p, _f32p, _i64, _f32p,
+                               _f32p, ctypes.c_float]
     L.orc_nd_num_nodes.restype = _i64
     L.orc_nd_num_nodes.argtypes = [_vp]
     L.orc_nd_emit.argtypes = [_vp, _u8p, _u8p, _u8p]
     L.orc_nd_free.argtypes = [_vp]
-    L.orc_nd_eval.argtypes = [ctypes.c_int, _u8p, _i64, _u8p, _i64, _u8p, _i64, _f32p, _i64,
-                              ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_int,
-                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
-                              ctypes.c_double, _f64p]
+    L.orc_nd_eval.argtypes = [ctypes.c_int, ctypes.c_int, _u8p, _i64, _u8p, _i64, _u8p, _i64,
+                              _f32p, _i64, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                              ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_int, ctypes.c_double, _f64p]
+    L.orc_nd_brute.argtypes = [ctypes.c_int, _u8p, _i64, _f32p, _i64, ctypes.c_int, _f64p]
     L.orc_cpu_tree_build.restype = ctypes.c_void_p
     L.orc_cpu_tree_build.argtypes = [_f32p, _vp, _f32p, _i64, _f32p, _f32p, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
@@ -367,21 +370,47 @@ NODE2_DTYPE = np.dtype({
 assert LEAF2_DTYPE.itemsize == 24
 
 
-def nd_build(points, normals, masses, min_corner, max_corner, alpha=1.0):
-    """Tree2D<1,3> (points (N, 2)) or Tree3D<1,3> (points (N, 3)) bytes:
-    (leaf, node, contrib) uint8 arrays, as the reference's sbh.Quadtree3 /
-    sbh.Octree3 leaf_data_bytes / node_buffer_bytes / contrib_data_bytes."""
+ND_WN, ND_SMOOTH = 3, 4  # signedSolidAngle<N>, smoothDistExp (kinds 0-2 as L3_*)
+
+
+def nd_node_dtype(dim, level):
+    """Tree2DNode<1, level> / Tree3DNode<1, level> record dtype."""
+    size = int(lib().orc_nd_node_size(dim, level))
+    cells = 1 << (dim * level)
+    words = max(1, cells // 64)
+    leaf = LEAF2_DTYPE if dim == 2 else LEAF3_DTYPE
+    o_contrib = 36 + 8 * dim
+    o_mask = (o_contrib + leaf.itemsize + 7) // 8 * 8
+    o_child = o_mask + 8 * words
+    o_num = o_child + 8 * cells
+    return np.dtype({
+        "names": ["start", "end", "buffer_idx", "parent_idx", "bmin", "bmax", "diameter",
+                  "contrib", "bitmask", "child", "num_children", "depth", "max_depth",
+                  "is_leaf"],
+        "formats": ["<u8", "<u8", "<u8", "<u8", ("<f4", dim), ("<f4", dim), "<f4", leaf,
+                    ("<u8", words), ("<u8", cells), "<i4", "<i4", "<i4", "u1"],
+        "offsets": [0, 8, 16, 24, 32, 32 + 4 * dim, 32 + 8 * dim, o_contrib, o_mask, o_child,
+                    o_num, o_num + 4, o_num + 8, o_num + 12],
+        "itemsize": size})
+
+
+def nd_build(points, normals, masses, min_corner, max_corner, alpha=1.0, level=3):
+    """Tree2D<1, level> (points (N, 2)) or Tree3D<1, level> (points (N, 3))
+    bytes: (leaf, node, contrib) uint8 arrays, as the reference's
+    sbh.Quadtree<level> / sbh.Octree<level> leaf_data_bytes /
+    node_buffer_bytes / contrib_data_bytes."""
     p = _c(points, np.float32)
     dim = p.shape[1]
     nv = _c(normals, np.float32)
     m = _c(masses, np.float32)
     n = p.shape[0]
     L = lib()
-    t = L.orc_nd_build(dim, p, nv, m, n, _c(min_corner, np.float32),
+    t = L.orc_nd_build(dim, int(level), p, nv, m, n, _c(min_corner, np.float32),
                        _c(max_corner, np.float32), float(alpha))
     try:
         nn = int(L.orc_nd_num_nodes(t))
-        lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+        lsz = 24 if dim == 2 else 32
+        nsz = int(L.orc_nd_node_size(dim, int(level)))
         leaf = np.zeros(max(n, 1) * lsz, np.uint8)
         node = np.zeros(nn * nsz, np.uint8)
         contrib = np.zeros(nn * lsz, np.uint8)
@@ -395,24 +424,46 @@ def decode_nodes2(node_bytes):
     return np.frombuffer(np.ascontiguousarray(node_bytes, np.uint8).tobytes(), NODE2_DTYPE)
 
 
+def decode_nodes(node_bytes, dim, level):
+    return np.frombuffer(np.ascontiguousarray(node_bytes, np.uint8).tobytes(),
+                         nd_node_dtype(dim, level))
+
+
 def nd_eval(leaf_bytes, node_bytes, contrib_bytes, queries, kind, *, beta=2.0, threads=256,
-            samples=-1, seed=0, eps=0.0, rc=0.0, periodic=False, box=1.0):
+            samples=-1, seed=0, eps=0.0, rc=0.0, periodic=False, box=1.0, level=3,
+            seed_per_query=True):
     """Vote walk (samples < 0) or control-variate sampler (samples >= 0) over
-    Tree2DNode<1,3> or Tree3DNode<1,3> buffers; the layout follows queries' width."""
+    Tree{2,3}DNode<1, level> buffers; the layout follows queries' width.
+    seed_per_query: roulette stream of seed + query (the FFI targets) or of
+    seed alone (the pybind11 module)."""
     q = _c(queries, np.float32)
     dim = q.shape[1]
-    lsz, nsz = (24, 616) if dim == 2 else (32, 4272)
+    lsz = 24 if dim == 2 else 32
+    nsz = int(lib().orc_nd_node_size(dim, int(level)))
     leaf = _c(leaf_bytes, np.uint8)
     node = _c(node_bytes, np.uint8)
     contrib = _c(contrib_bytes if contrib_bytes is not None else np.zeros(0, np.uint8), np.uint8)
     out = np.zeros((q.shape[0], dim), np.float64)
-    lib().orc_nd_eval(dim, leaf if leaf.size else np.zeros(1, np.uint8), leaf.size // lsz, node,
-                      node.size // nsz, contrib if contrib.size else np.zeros(1, np.uint8),
-                      contrib.size // lsz, q, q.shape[0], int(kind), float(beta), int(threads),
-                      int(samples), int(seed), float(eps), float(rc), int(bool(periodic)),
-                      float(box), out)
-    if kind == L3_POTENTIAL:  # written as a flat (M,) array
+    lib().orc_nd_eval(dim, int(level), leaf if leaf.size else np.zeros(1, np.uint8),
+                      leaf.size // lsz, node, node.size // nsz,
+                      contrib if contrib.size else np.zeros(1, np.uint8), contrib.size // lsz,
+                      q, q.shape[0], int(kind), float(beta), int(threads), int(samples),
+                      int(seed), int(bool(seed_per_query)), float(eps), float(rc),
+                      int(bool(periodic)), float(box), out)
+    if kind in (L3_POTENTIAL, ND_WN, ND_SMOOTH):  # written as a flat (M,) array
         return out.reshape(-1)[: q.shape[0]].copy()
+    return out
+
+
+def nd_brute(leaf_bytes, queries, kind):
+    """fp64 sum of a scalar kernel (L3_POTENTIAL, ND_WN, ND_SMOOTH) over the
+    leaf records."""
+    q = _c(queries, np.float32)
+    dim = q.shape[1]
+    leaf = _c(leaf_bytes, np.uint8)
+    out = np.zeros(q.shape[0], np.float64)
+    lib().orc_nd_brute(dim, leaf if leaf.size else np.zeros(1, np.uint8),
+                       leaf.size // (24 if dim == 2 else 32), q, q.shape[0], int(kind), out)
     return out
 
 

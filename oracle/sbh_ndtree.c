@@ -4,25 +4,29 @@
  * Nothing in the product links or calls it.
  *
  * Restated:
- *   Tree2D<1, 3> / Tree3D<1, 3> (include/tree2d.h:237-407,
+ *   Tree2D<1, L> / Tree3D<1, L>, L = 1, 2, 3 (include/tree2d.h:237-407,
  *     include/tree3d.h:238-413; include/py_tree.h:20-73 sets max_depth 100
  *     and every point's alpha): breadth-first build from a queue; a node is a
  *     leaf when depth >= 100 or it holds <= 1 point; otherwise its points are
- *     partitioned by AABB::quantize (include/aabb.h:36-42) into the 8^dim
- *     cells, visited in (z,) y, x order, and each non-empty cell becomes a
- *     child with box lo + k inc, lo + k inc + inc, inc = (hi - lo) / 8.
+ *     partitioned by AABB::quantize (include/aabb.h:36-42) into the S^dim
+ *     cells (S = 2^L), visited in (z,) y, x order, and each non-empty cell
+ *     becomes a child with box lo + k inc, lo + k inc + inc, inc = (hi - lo) / S.
  *     Moments: sequential fp32 sums over points (leaves) or children in cell
  *     order, position and normal divided by the mass.
  *   evaluateBarnesHutGPUVote (include/barnes_hut_gpu.h:19-72): recursive form
  *     of the stackless walk for one 32-query warp.
  *   multiLevelPrefixControlVariateSampleGPUQuery
  *     (include/tree_sampler_gpu.h:25-140).
+ *   Kernels (include/kernel.h:55-94): gravityPotential, gravityGrad,
+ *     signedSolidAngle<N>, smoothDistExp; gravityGradSoftCutoff
+ *     (src/sbh_ffi.cu:142-160).
  * Stated differences: the partition here is stable (the reference's
  * std::partition leaves the order inside a cell unspecified; it matters only
  * inside leaves at depth 100).  cuRAND XORWOW is replaced by Philox4x32-10
  * streams: the sample point from key = block index, counter (1, subdomain,
- * sample, 0); the roulette from key = seed + query, counter (0, subdomain,
- * sample, step).  Tree arithmetic and walk decisions are fp32 in the
+ * sample, 0); the roulette from key = seed + query (the FFI targets) or seed
+ * (the pybind11 module, which passes seed 0 to every query), counter (0,
+ * subdomain, sample, step).  Tree arithmetic and walk decisions are fp32 in the
  * reference's operation order (this file is built with -ffp-contract=off;
  * the sample index uses an explicit fmaf, as nvcc contracts it); the sums of
  * the walks are fp64.
@@ -32,7 +36,9 @@
  * partition, moments against fp64 sums; huge beta = the fp64 direct sum and
  * beta = 0 = the root monopole for the vote walk; for the sampler, one-level
  * trees give the exact sum and the mean over blocks (independent sample
- * points and paths) is the direct sum within its standard error. */
+ * points and paths) is the direct sum within its standard error; the module's
+ * fixed seed equals the FFI seeding at seed = -query; the brute force is the
+ * fp64 direct sum over the leaf records. */
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -42,43 +48,51 @@
 
 ORC_EXPORT void orc_philox4x32(const uint32_t* ctr_in, const uint32_t* key_in, uint32_t* out);
 
-typedef struct {
-  float pos[2], normal[2], mass, alpha;
-} NdLeaf2; /* LeafData<2>: 24 bytes */
-
-typedef struct {
-  float pos[3], normal[3], mass, alpha;
-} NdLeaf3; /* LeafData<3>: 32 bytes */
-
-typedef struct {
-  uint64_t start, end, buffer_idx, parent_idx;
-  float bmin[2], bmax[2];
-  float diameter;
-  NdLeaf2 contrib;
-  uint64_t bitmask[1];
-  uint64_t child[64];
-  int32_t num_children, depth, max_depth;
-  uint8_t is_leaf, pad[3];
-} NdNode2; /* Tree2DNode<1, 3>: 616 bytes */
-
-typedef struct {
-  uint64_t start, end, buffer_idx, parent_idx;
-  float bmin[3], bmax[3];
-  float diameter;
-  NdLeaf3 contrib;
-  uint64_t bitmask[8];
-  uint64_t child[512];
-  int32_t num_children, depth, max_depth;
-  uint8_t is_leaf, pad[3];
-} NdNode3; /* Tree3DNode<1, 3>: 4272 bytes */
-
-_Static_assert(sizeof(NdLeaf2) == 24, "LeafData<2>");
-_Static_assert(sizeof(NdNode2) == 616, "Tree2DNode<1,3>");
-_Static_assert(sizeof(NdNode3) == 4272, "Tree3DNode<1,3>");
-
-ORC_EXPORT size_t orc_node2_size(void) { return sizeof(NdNode2); }
-
 #define ND_MAX_DEPTH 100
+
+/* Byte layout of Tree2DNode<1, L> / Tree3DNode<1, L> (include/tree2d.h:
+ * 212-225, include/tree3d.h:212-225) under the Itanium C++ ABI: members in
+ * declaration order, each at its natural alignment, size rounded to 8. */
+typedef struct {
+  int dim, log, split, cells, words;
+  size_t o_bmin, o_bmax, o_diam, o_contrib, o_mask, o_child, o_num, o_depth, o_maxd, o_leaf;
+  size_t size, leaf_size;
+} NdLayout;
+
+static size_t nd_align(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+static NdLayout nd_layout(int dim, int log) {
+  NdLayout L;
+  L.dim = dim;
+  L.log = log;
+  L.split = 1 << log;
+  L.cells = 1 << (dim * log);
+  L.words = L.cells >= 64 ? L.cells / 64 : 1;
+  L.leaf_size = (size_t)(2 * dim + 2) * 4;
+  size_t o = 32; /* start, end, buffer_idx, parent_idx */
+  L.o_bmin = o;
+  o += 4 * (size_t)dim;
+  L.o_bmax = o;
+  o += 4 * (size_t)dim;
+  L.o_diam = o;
+  o += 4;
+  L.o_contrib = o;
+  o += L.leaf_size;
+  o = nd_align(o, 8);
+  L.o_mask = o;
+  o += 8 * (size_t)L.words;
+  L.o_child = o;
+  o += 8 * (size_t)L.cells;
+  L.o_num = o;
+  L.o_depth = o + 4;
+  L.o_maxd = o + 8;
+  L.o_leaf = o + 12;
+  L.size = nd_align(o + 13, 8);
+  return L;
+}
+
+ORC_EXPORT size_t orc_node2_size(void) { return nd_layout(2, 3).size; }
+ORC_EXPORT size_t orc_nd_node_size(int dim, int log) { return nd_layout(dim, log).size; }
 
 /* Generic in-memory node (converted to the byte layouts on output). */
 typedef struct {
@@ -90,7 +104,7 @@ typedef struct {
 } NdNode;
 
 typedef struct {
-  int dim;
+  int dim, log;
   int64_t n, nn, cap;
   float* pts; /* tree order, dim per point */
   float* nrm;
@@ -99,19 +113,19 @@ typedef struct {
   NdNode* nodes;
 } NdTree;
 
-/* AABB::quantize as the x86 build evaluates it: (int) of 8 (p - lo) / (hi -
- * lo) (INT_MIN for NaN or out of range), then min(., 7), max(., 0). */
-static int nd_quantize(float p, float lo, float hi) {
-  float t = 8.0f * (p - lo) / (hi - lo);
+/* AABB::quantize as the x86 build evaluates it: (int) of S (p - lo) / (hi -
+ * lo) (INT_MIN for NaN or out of range), then min(., S - 1), max(., 0). */
+static int nd_quantize(float p, float lo, float hi, int S) {
+  float t = (float)S * (p - lo) / (hi - lo);
   int c = (t >= -2147483648.0f && t < 2147483648.0f) ? (int)t : INT32_MIN;
-  if (c > 7) c = 7;
+  if (c > S - 1) c = S - 1;
   if (c < 0) c = 0;
   return c;
 }
 
-static int nd_cell(const float* p, const float* lo, const float* hi, int dim) {
+static int nd_cell(const float* p, const float* lo, const float* hi, int dim, int S) {
   int c = 0;
-  for (int a = dim - 1; a >= 0; a--) c = c * 8 + nd_quantize(p[a], lo[a], hi[a]);
+  for (int a = dim - 1; a >= 0; a--) c = c * S + nd_quantize(p[a], lo[a], hi[a], S);
   return c;
 }
 
@@ -133,11 +147,14 @@ ORC_EXPORT void orc_nd_free(NdTree* t) {
   free(t);
 }
 
-ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, const float* ms,
-                                int64_t n, const float* lo, const float* hi, float alpha) {
+ORC_EXPORT NdTree* orc_nd_build(int dim, int log, const float* pts, const float* nrm,
+                                const float* ms, int64_t n, const float* lo, const float* hi,
+                                float alpha) {
   NdTree* t = (NdTree*)calloc(1, sizeof(NdTree));
-  const int cells = dim == 2 ? 64 : 512;
+  const int S = 1 << log;
+  const int cells = 1 << (dim * log);
   t->dim = dim;
+  t->log = log;
   t->n = n;
   t->alpha = alpha;
   size_t np = (size_t)(n > 0 ? n : 1);
@@ -152,6 +169,7 @@ ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, con
   float* tm = (float*)malloc(sizeof(float) * np);
   int* cell = (int*)malloc(sizeof(int) * np);
   int64_t* cnt = (int64_t*)malloc(sizeof(int64_t) * (cells + 1));
+  int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * cells);
   int64_t r = nd_new_node(t);
   NdNode* root = &t->nodes[r];
   root->start = 0;
@@ -170,12 +188,11 @@ ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, con
     const int64_t s0 = nd->start, s1 = nd->end;
     memset(cnt, 0, sizeof(int64_t) * (cells + 1));
     for (int64_t j = s0; j < s1; j++) {
-      cell[j - s0] = nd_cell(&t->pts[dim * j], nd->lo, nd->hi, dim);
+      cell[j - s0] = nd_cell(&t->pts[dim * j], nd->lo, nd->hi, dim, S);
       cnt[cell[j - s0] + 1]++;
     }
     for (int c = 0; c < cells; c++) cnt[c + 1] += cnt[c];
     /* stable scatter by cell */
-    int64_t* cur = (int64_t*)malloc(sizeof(int64_t) * cells);
     for (int c = 0; c < cells; c++) cur[c] = cnt[c];
     for (int64_t j = s0; j < s1; j++) {
       int64_t d = s0 + cur[cell[j - s0]]++;
@@ -185,14 +202,13 @@ ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, con
       }
       tm[d] = t->ms[j];
     }
-    free(cur);
     memcpy(&t->pts[dim * s0], &tp[dim * s0], sizeof(float) * dim * (size_t)(s1 - s0));
     memcpy(&t->nrm[dim * s0], &tn[dim * s0], sizeof(float) * dim * (size_t)(s1 - s0));
     memcpy(&t->ms[s0], &tm[s0], sizeof(float) * (size_t)(s1 - s0));
     float inc[3], plo[3];
     int32_t depth = nd->depth;
     for (int a = 0; a < dim; a++) {
-      inc[a] = (nd->hi[a] - nd->lo[a]) / 8.0f;
+      inc[a] = (nd->hi[a] - nd->lo[a]) / (float)S;
       plo[a] = nd->lo[a];
     }
     int64_t first = -1;
@@ -206,8 +222,10 @@ ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, con
       cn->end = s0 + cnt[c + 1];
       cn->parent = id;
       cn->depth = depth + 1;
+      int rem = c;
       for (int a = 0; a < dim; a++) {
-        int k = (c >> (3 * a)) & 7;
+        int k = rem % S;
+        rem /= S;
         cn->lo[a] = plo[a] + (float)k * inc[a];
         cn->hi[a] = cn->lo[a] + inc[a];
       }
@@ -257,6 +275,7 @@ ORC_EXPORT NdTree* orc_nd_build(int dim, const float* pts, const float* nrm, con
   free(tm);
   free(cell);
   free(cnt);
+  free(cur);
   return t;
 }
 
@@ -279,138 +298,105 @@ static float nd_diameter(const NdNode* nd, int dim) {
   return sqrtf(s);
 }
 
+static void nd_put_leaf(uint8_t* dst, int dim, const float* pos, const float* nrm, float m,
+                        float alpha) {
+  float v[8];
+  for (int a = 0; a < dim; a++) {
+    v[a] = pos[a];
+    v[dim + a] = nrm[a];
+  }
+  v[2 * dim] = m;
+  v[2 * dim + 1] = alpha;
+  memcpy(dst, v, sizeof(float) * (size_t)(2 * dim + 2));
+}
+
 /* Byte output as the reference's leaf_data_bytes / node_buffer_bytes /
  * contrib_data_bytes (src/sbh.cpp:227-235); padding bytes are zero. */
 ORC_EXPORT void orc_nd_emit(const NdTree* t, uint8_t* leaf_out, uint8_t* node_out,
                             uint8_t* contrib_out) {
   const int dim = t->dim;
-  for (int64_t j = 0; j < t->n; j++) {
-    if (dim == 2) {
-      NdLeaf2* L = (NdLeaf2*)leaf_out + j;
-      for (int a = 0; a < 2; a++) {
-        L->pos[a] = t->pts[2 * j + a];
-        L->normal[a] = t->nrm[2 * j + a];
-      }
-      L->mass = t->ms[j];
-      L->alpha = t->alpha;
-    } else {
-      NdLeaf3* L = (NdLeaf3*)leaf_out + j;
-      for (int a = 0; a < 3; a++) {
-        L->pos[a] = t->pts[3 * j + a];
-        L->normal[a] = t->nrm[3 * j + a];
-      }
-      L->mass = t->ms[j];
-      L->alpha = t->alpha;
-    }
-  }
-  const size_t nsz = dim == 2 ? sizeof(NdNode2) : sizeof(NdNode3);
-  memset(node_out, 0, nsz * (size_t)t->nn);
+  const NdLayout L = nd_layout(dim, t->log);
+  for (int64_t j = 0; j < t->n; j++)
+    nd_put_leaf(leaf_out + L.leaf_size * (size_t)j, dim, &t->pts[dim * j], &t->nrm[dim * j],
+                t->ms[j], t->alpha);
+  memset(node_out, 0, L.size * (size_t)t->nn);
   for (int64_t id = 0; id < t->nn; id++) {
     const NdNode* nd = &t->nodes[id];
-    if (dim == 2) {
-      NdNode2* o = (NdNode2*)node_out + id;
-      NdLeaf2* c = contrib_out ? (NdLeaf2*)contrib_out + id : NULL;
-      o->start = (uint64_t)nd->start;
-      o->end = (uint64_t)nd->end;
-      o->buffer_idx = (uint64_t)id;
-      o->parent_idx = (uint64_t)nd->parent;
-      for (int a = 0; a < 2; a++) {
-        o->bmin[a] = nd->lo[a];
-        o->bmax[a] = nd->hi[a];
-        o->contrib.pos[a] = nd->pos[a];
-        o->contrib.normal[a] = nd->nrm[a];
-      }
-      o->diameter = nd_diameter(nd, 2);
-      o->contrib.mass = nd->m;
-      o->contrib.alpha = nd->alpha;
-      o->bitmask[0] = nd->mask[0];
-      for (int cl = 0; cl < 64; cl++) o->child[cl] = nd_child_id(nd, cl);
-      o->num_children = nd->num;
-      o->depth = nd->depth;
-      o->max_depth = ND_MAX_DEPTH;
-      o->is_leaf = (uint8_t)nd->leaf;
-      if (c) *c = o->contrib;
-    } else {
-      NdNode3* o = (NdNode3*)node_out + id;
-      NdLeaf3* c = contrib_out ? (NdLeaf3*)contrib_out + id : NULL;
-      o->start = (uint64_t)nd->start;
-      o->end = (uint64_t)nd->end;
-      o->buffer_idx = (uint64_t)id;
-      o->parent_idx = (uint64_t)nd->parent;
-      for (int a = 0; a < 3; a++) {
-        o->bmin[a] = nd->lo[a];
-        o->bmax[a] = nd->hi[a];
-        o->contrib.pos[a] = nd->pos[a];
-        o->contrib.normal[a] = nd->nrm[a];
-      }
-      o->diameter = nd_diameter(nd, 3);
-      o->contrib.mass = nd->m;
-      o->contrib.alpha = nd->alpha;
-      for (int w = 0; w < 8; w++) o->bitmask[w] = nd->mask[w];
-      for (int cl = 0; cl < 512; cl++) o->child[cl] = nd_child_id(nd, cl);
-      o->num_children = nd->num;
-      o->depth = nd->depth;
-      o->max_depth = ND_MAX_DEPTH;
-      o->is_leaf = (uint8_t)nd->leaf;
-      if (c) *c = o->contrib;
+    uint8_t* o = node_out + L.size * (size_t)id;
+    uint64_t u[4] = {(uint64_t)nd->start, (uint64_t)nd->end, (uint64_t)id, (uint64_t)nd->parent};
+    memcpy(o, u, sizeof(u));
+    memcpy(o + L.o_bmin, nd->lo, 4 * (size_t)dim);
+    memcpy(o + L.o_bmax, nd->hi, 4 * (size_t)dim);
+    float diam = nd_diameter(nd, dim);
+    memcpy(o + L.o_diam, &diam, 4);
+    nd_put_leaf(o + L.o_contrib, dim, nd->pos, nd->nrm, nd->m, nd->alpha);
+    memcpy(o + L.o_mask, nd->mask, 8 * (size_t)L.words);
+    for (int c = 0; c < L.cells; c++) {
+      uint64_t ch = nd_child_id(nd, c);
+      memcpy(o + L.o_child + 8 * (size_t)c, &ch, 8);
     }
+    int32_t iv[3] = {nd->num, nd->depth, ND_MAX_DEPTH};
+    memcpy(o + L.o_num, iv, sizeof(iv));
+    o[L.o_leaf] = (uint8_t)nd->leaf;
+    if (contrib_out)
+      memcpy(contrib_out + L.leaf_size * (size_t)id, o + L.o_contrib, L.leaf_size);
   }
 }
 
 /* ------------------------------------------------------------------ walks */
-/* Uniform view of a node record of either layout. */
 typedef struct {
   const uint8_t* base;
-  size_t size;
-  int dim;
+  NdLayout L;
 } NdView;
 
 typedef struct {
   uint64_t start, end, parent;
-  const float *bmin, *bmax, *pos;
-  float diameter, mass;
-  const uint64_t* mask;
-  const uint64_t* child;
+  float bmin[3], bmax[3], pos[3], nrm[3];
+  float diameter, mass, alpha;
+  const uint8_t* mask;
+  const uint8_t* child;
   int32_t num;
   int leaf;
 } NdRec;
 
+static uint64_t nd_u64(const uint8_t* p) {
+  uint64_t v;
+  memcpy(&v, p, 8);
+  return v;
+}
+
 static NdRec nd_rec(const NdView* v, uint64_t id) {
+  const NdLayout* L = &v->L;
+  const uint8_t* o = v->base + L->size * id;
   NdRec r;
-  if (v->dim == 2) {
-    const NdNode2* o = (const NdNode2*)(v->base) + id;
-    r.start = o->start;
-    r.end = o->end;
-    r.parent = o->parent_idx;
-    r.bmin = o->bmin;
-    r.bmax = o->bmax;
-    r.pos = o->contrib.pos;
-    r.diameter = o->diameter;
-    r.mass = o->contrib.mass;
-    r.mask = o->bitmask;
-    r.child = o->child;
-    r.num = o->num_children;
-    r.leaf = o->is_leaf;
-  } else {
-    const NdNode3* o = (const NdNode3*)(v->base) + id;
-    r.start = o->start;
-    r.end = o->end;
-    r.parent = o->parent_idx;
-    r.bmin = o->bmin;
-    r.bmax = o->bmax;
-    r.pos = o->contrib.pos;
-    r.diameter = o->diameter;
-    r.mass = o->contrib.mass;
-    r.mask = o->bitmask;
-    r.child = o->child;
-    r.num = o->num_children;
-    r.leaf = o->is_leaf;
+  float c[8];
+  r.start = nd_u64(o);
+  r.end = nd_u64(o + 8);
+  r.parent = nd_u64(o + 24);
+  memcpy(r.bmin, o + L->o_bmin, 4 * (size_t)L->dim);
+  memcpy(r.bmax, o + L->o_bmax, 4 * (size_t)L->dim);
+  memcpy(&r.diameter, o + L->o_diam, 4);
+  memcpy(c, o + L->o_contrib, L->leaf_size);
+  for (int a = 0; a < L->dim; a++) {
+    r.pos[a] = c[a];
+    r.nrm[a] = c[L->dim + a];
   }
+  r.mass = c[2 * L->dim];
+  r.alpha = c[2 * L->dim + 1];
+  r.mask = o + L->o_mask;
+  r.child = o + L->o_child;
+  memcpy(&r.num, o + L->o_num, 4);
+  r.leaf = o[L->o_leaf];
   return r;
 }
 
+static int nd_has_cell(const NdRec* r, int c) {
+  return (int)((nd_u64(r->mask + 8 * (size_t)(c >> 6)) >> (c & 63)) & 1ull);
+}
+static uint64_t nd_child_at(const NdRec* r, int c) { return nd_u64(r->child + 8 * (size_t)c); }
+
 typedef struct {
-  int kind; /* 0 gravityPotential, 1 gravityGrad, 2 gravityGradSoftCutoff */
+  int kind; /* 0 gravityPotential, 1 gravityGrad, 2 gravityGradSoftCutoff, 3 signedSolidAngle, 4 smoothDistExp */
   double eps, rc;
   int cutoff, periodic;
   double box;
@@ -421,17 +407,28 @@ static double nd_mi(double d, const NdParams* P) {
   return d - P->box * floor(d / P->box + 0.5);
 }
 
-/* Pair kernels (include/kernel.h:55-60, 89-94; src/sbh_ffi.cu:142-160) in
- * fp64 for a source at p with mass m. */
-static void nd_term(const NdParams* P, int dim, const double* q, const float* p, float m,
-                    double* acc) {
+/* Kernels (include/kernel.h:55-94; src/sbh_ffi.cu:142-160) in fp64 for a
+ * record at p with normal nv, mass m and alpha al. */
+static void nd_term(const NdParams* P, int dim, const double* q, const float* p,
+                    const float* nv, float m, float al, double* acc) {
   double d[3], r2 = 0.0;
-  if (P->kind == 0) {
+  if (P->kind == 0 || P->kind == 3 || P->kind == 4) {
+    double dn = 0.0;
     for (int a = 0; a < dim; a++) {
       d[a] = nd_mi((double)p[a] - q[a], P);
       r2 += d[a] * d[a];
+      dn += (double)nv[a] * d[a];
     }
-    acc[0] += -(double)m / (sqrt(r2) + 0.1);
+    if (P->kind == 0) {
+      acc[0] += -(double)m / (sqrt(r2) + 0.1);
+    } else if (P->kind == 3) {
+      if (dim == 2)
+        acc[0] += (double)m * dn / (2.0 * M_PI * (r2 + 1e-5));
+      else
+        acc[0] += (double)m * dn / (4.0 * M_PI * (r2 * sqrt(r2) + 1e-5));
+    } else {
+      acc[0] += exp(-(double)al * sqrt(r2));
+    }
     return;
   }
   for (int a = 0; a < dim; a++) {
@@ -454,25 +451,22 @@ static void nd_term(const NdParams* P, int dim, const double* q, const float* p,
   for (int a = 0; a < dim; a++) acc[a] += c * d[a];
 }
 
-static const float* nd_leaf_pos(const uint8_t* leaf, int dim, uint64_t j, float* mass) {
-  if (dim == 2) {
-    const NdLeaf2* L = (const NdLeaf2*)leaf + j;
-    *mass = L->mass;
-    return L->pos;
-  }
-  const NdLeaf3* L = (const NdLeaf3*)leaf + j;
-  *mass = L->mass;
-  return L->pos;
+static void nd_leaf_term(const NdParams* P, const NdLayout* L, const uint8_t* leaf, uint64_t j,
+                         const double* q, double* acc) {
+  float c[8];
+  memcpy(c, leaf + L->leaf_size * j, L->leaf_size);
+  nd_term(P, L->dim, q, c, c + L->dim, c[2 * L->dim], c[2 * L->dim + 1], acc);
 }
 
-static void nd_leaf_sum(const uint8_t* leaf, uint64_t nl, int dim, const NdRec* r,
+static void nd_leaf_sum(const uint8_t* leaf, uint64_t nl, const NdLayout* L, const NdRec* r,
                         const NdParams* P, const double* q, double* acc) {
   uint64_t je = r->end < nl ? r->end : nl;
-  for (uint64_t j = r->start; j < je; j++) {
-    float m;
-    const float* p = nd_leaf_pos(leaf, dim, j, &m);
-    nd_term(P, dim, q, p, m, acc);
-  }
+  for (uint64_t j = r->start; j < je; j++) nd_leaf_term(P, L, leaf, j, q, acc);
+}
+
+static void nd_rec_term(const NdParams* P, int dim, const NdRec* r, const double* q,
+                        double* acc) {
+  nd_term(P, dim, q, r->pos, r->nrm, r->mass, r->alpha, acc);
 }
 
 /* farFieldRatio (include/tree2d.h:132-138), fp32. */
@@ -489,7 +483,7 @@ static float nd_ratio(const NdRec* r, const float* q, int dim) {
 static void nd_vote(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t nn, uint64_t id,
                     const float* qf, int64_t g0, int64_t g1, float beta, const NdParams* P,
                     double* out, int vec) {
-  const int dim = v->dim;
+  const int dim = v->L.dim;
   NdRec r = nd_rec(v, id);
   int all = 1;
   for (int64_t i = g0; i < g1 && all; i++)
@@ -500,16 +494,17 @@ static void nd_vote(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t 
       for (int a = 0; a < dim; a++) q[a] = qf[dim * i + a];
       double* o = out + (vec ? dim * i : i);
       if (r.leaf)
-        nd_leaf_sum(leaf, nl, dim, &r, P, q, o);
+        nd_leaf_sum(leaf, nl, &v->L, &r, P, q, o);
       else
-        nd_term(P, dim, q, r.pos, r.mass, o);
+        nd_rec_term(P, dim, &r, q, o);
     }
     return;
   }
-  const int cells = dim == 2 ? 64 : 512;
-  for (int c = 0; c < cells; c++)
-    if ((r.mask[c >> 6] >> (c & 63)) & 1ull)
-      if (r.child[c] < nn) nd_vote(leaf, nl, v, nn, r.child[c], qf, g0, g1, beta, P, out, vec);
+  for (int c = 0; c < v->L.cells; c++)
+    if (nd_has_cell(&r, c)) {
+      uint64_t ch = nd_child_at(&r, c);
+      if (ch < nn) nd_vote(leaf, nl, v, nn, ch, qf, g0, g1, beta, P, out, vec);
+    }
 }
 
 static inline float nd_uniform(uint32_t key, uint32_t tag, uint32_t node, uint32_t sample,
@@ -523,27 +518,29 @@ static inline float nd_uniform(uint32_t key, uint32_t tag, uint32_t node, uint32
 
 /* AABB::contains with the 1e-4 margin, then AABB::quantize as nvcc's
  * saturating conversion; -1 outside. */
-static int nd_cell_in(const NdRec* r, const float* p, int dim) {
+static int nd_cell_in(const NdRec* r, const float* p, int dim, int S) {
   int c = 0;
   int qa[3];
   for (int a = 0; a < dim; a++) {
     if (!(r->bmin[a] - 1e-4f <= p[a] && p[a] <= r->bmax[a] + 1e-4f)) return -1;
-    float t = 8.0f * (p[a] - r->bmin[a]) / (r->bmax[a] - r->bmin[a]);
-    int k = isnan(t) ? 0 : (t >= 2147483647.0f ? 2147483647 : (t <= -2147483648.0f ? INT32_MIN : (int)t));
+    float t = (float)S * (p[a] - r->bmin[a]) / (r->bmax[a] - r->bmin[a]);
+    int k = isnan(t) ? 0
+                     : (t >= 2147483647.0f ? 2147483647
+                                           : (t <= -2147483648.0f ? INT32_MIN : (int)t));
     if (k < 0) k = 0;
-    if (k > 7) k = 7;
+    if (k > S - 1) k = S - 1;
     qa[a] = k;
   }
-  for (int a = dim - 1; a >= 0; a--) c = c * 8 + qa[a];
+  for (int a = dim - 1; a >= 0; a--) c = c * S + qa[a];
   return c;
 }
 
 static int64_t nd_contained(const NdView* v, uint64_t nn, uint64_t id, const float* p) {
   for (int hop = 0; hop < 128; hop++) {
     NdRec r = nd_rec(v, id);
-    int c = nd_cell_in(&r, p, v->dim);
+    int c = nd_cell_in(&r, p, v->L.dim, v->L.split);
     if (c < 0) return -1;
-    uint64_t ch = r.child[c];
+    uint64_t ch = nd_child_at(&r, c);
     if (ch >= nn) return -1;
     NdRec cr = nd_rec(v, ch);
     if (cr.num != 1) return (int64_t)ch;
@@ -552,38 +549,26 @@ static int64_t nd_contained(const NdView* v, uint64_t nn, uint64_t id, const flo
   return -1;
 }
 
-static void nd_contrib_term(const uint8_t* contrib, uint64_t ncontrib, const NdView* v,
-                            uint64_t c, const NdParams* P, const double* q, double* acc) {
-  const int dim = v->dim;
-  if (c < ncontrib) {
-    float m;
-    const float* p = nd_leaf_pos(contrib, dim, c, &m);
-    nd_term(P, dim, q, p, m, acc);
-  } else {
-    NdRec r = nd_rec(v, c);
-    nd_term(P, dim, q, r.pos, r.mass, acc);
-  }
-}
-
 /* One query of the sampler; `block` is the query's block index. */
 static void nd_mlpcv(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t nn,
                      const uint8_t* contrib, uint64_t ncontrib, const float* qf, int64_t qi,
-                     uint32_t block, int samples, int seed, const NdParams* P, double* out) {
-  const int dim = v->dim;
-  const int cells = dim == 2 ? 64 : 512;
+                     uint32_t block, int samples, int seed, int seed_per_query,
+                     const NdParams* P, double* out) {
+  const NdLayout* L = &v->L;
+  const int dim = L->dim;
   double q[3] = {0, 0, 0}, bh[3] = {0, 0, 0}, ml[3] = {0, 0, 0};
   for (int a = 0; a < dim; a++) q[a] = qf[a];
-  const uint32_t rseed = (uint32_t)seed + (uint32_t)qi;
+  const uint32_t rseed = (uint32_t)seed + (seed_per_query ? (uint32_t)qi : 0u);
   NdRec root = nd_rec(v, 0);
-  for (int b = 0; b < cells; b++) {
-    if (!((root.mask[b >> 6] >> (b & 63)) & 1ull)) continue;
-    uint64_t ci = root.child[b];
+  for (int b = 0; b < L->cells; b++) {
+    if (!nd_has_cell(&root, b)) continue;
+    uint64_t ci = nd_child_at(&root, b);
     if (ci >= nn) continue;
     NdRec sub = nd_rec(v, ci);
     if (sub.leaf)
-      nd_leaf_sum(leaf, nl, dim, &sub, P, q, bh);
+      nd_leaf_sum(leaf, nl, L, &sub, P, q, bh);
     else
-      nd_term(P, dim, q, sub.pos, sub.mass, bh);
+      nd_rec_term(P, dim, &sub, q, bh);
     uint64_t count = sub.end - sub.start;
     double total[3] = {0, 0, 0};
     for (int ss = 0; ss < samples; ss++) {
@@ -591,10 +576,8 @@ static void nd_mlpcv(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t
       uint64_t idx = (uint64_t)fmaf((float)count, u2, (float)sub.start);
       if (idx > sub.end - 1) idx = sub.end - 1;
       if (count == 0 || idx >= nl) continue;
-      float mm;
-      const float* sp0 = nd_leaf_pos(leaf, dim, idx, &mm);
-      float sp[3] = {0, 0, 0};
-      for (int a = 0; a < dim; a++) sp[a] = sp0[a];
+      float sp[8];
+      memcpy(sp, leaf + L->leaf_size * idx, L->leaf_size);
       uint64_t cur = ci;
       float event_prob = 1.0f;
       double w_all[3] = {0, 0, 0};
@@ -610,11 +593,11 @@ static void nd_mlpcv(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t
         int64_t nxt = nd_contained(v, nn, cur, sp);
         cur_beta = fmaxf(cur_beta, ffr);
         double prev_res[3] = {0, 0, 0}, cur_res[3] = {0, 0, 0};
-        nd_term(P, dim, q, cn.pos, cn.mass, prev_res);
+        nd_rec_term(P, dim, &cn, q, prev_res);
         uint64_t first = 0;
-        for (int c = 0; c < cells; c++)
-          if ((cn.mask[c >> 6] >> (c & 63)) & 1ull) {
-            first = cn.child[c];
+        for (int c = 0; c < L->cells; c++)
+          if (nd_has_cell(&cn, c)) {
+            first = nd_child_at(&cn, c);
             break;
           }
         for (int32_t o = 0; o < cn.num; o++) {
@@ -622,9 +605,11 @@ static void nd_mlpcv(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t
           if (c >= nn) break;
           NdRec ch = nd_rec(v, c);
           if (ch.leaf)
-            nd_leaf_sum(leaf, nl, dim, &ch, P, q, cur_res);
+            nd_leaf_sum(leaf, nl, L, &ch, P, q, cur_res);
+          else if (c < ncontrib)
+            nd_leaf_term(P, L, contrib, c, q, cur_res);
           else
-            nd_contrib_term(contrib, ncontrib, v, c, P, q, cur_res);
+            nd_rec_term(P, dim, &ch, q, cur_res);
         }
         double w = (double)((float)count / (event_prob * (float)(cn.end - cn.start)));
         for (int a = 0; a < dim; a++) w_all[a] += w * (cur_res[a] - prev_res[a]);
@@ -639,14 +624,15 @@ static void nd_mlpcv(const uint8_t* leaf, uint64_t nl, const NdView* v, uint64_t
 }
 
 /* Classic vote walk (samples < 0) or the sampler (samples >= 0) over node
- * records of either layout; kind 0 writes out[i], kinds 1, 2 out[dim i + a].
- * Reference warps: 32 consecutive queries of a block of `threads`. */
-ORC_EXPORT void orc_nd_eval(int dim, const uint8_t* leaf_bytes, int64_t nl,
+ * records Tree{2,3}DNode<1, log>; kinds 0, 3, 4 write out[i], kinds 1, 2
+ * out[dim i + a].  Reference warps: 32 consecutive queries of a block of
+ * `threads`. */
+ORC_EXPORT void orc_nd_eval(int dim, int log, const uint8_t* leaf_bytes, int64_t nl,
                             const uint8_t* node_bytes, int64_t nn, const uint8_t* contrib_bytes,
                             int64_t ncontrib, const float* queries, int64_t nq, int kind,
-                            float beta, int threads, int samples, int seed, double eps,
-                            double rc, int periodic, double box, double* out) {
-  NdView v = {node_bytes, dim == 2 ? sizeof(NdNode2) : sizeof(NdNode3), dim};
+                            float beta, int threads, int samples, int seed, int seed_per_query,
+                            double eps, double rc, int periodic, double box, double* out) {
+  NdView v = {node_bytes, nd_layout(dim, log)};
   NdParams P;
   P.kind = kind;
   P.eps = eps > 0.0 ? eps : 0.0;
@@ -654,7 +640,7 @@ ORC_EXPORT void orc_nd_eval(int dim, const uint8_t* leaf_bytes, int64_t nl,
   P.cutoff = rc > 0.0 && isfinite(rc);
   P.periodic = periodic != 0 && box > 0.0;
   P.box = box;
-  const int vec = kind != 0;
+  const int vec = kind == 1 || kind == 2;
   memset(out, 0, sizeof(double) * (size_t)(vec ? dim * nq : nq));
   if (nn <= 0) return;
   if (samples < 0) {
@@ -671,10 +657,28 @@ ORC_EXPORT void orc_nd_eval(int dim, const uint8_t* leaf_bytes, int64_t nl,
   for (int64_t i = 0; i < nq; i++) {
     double o[3] = {0, 0, 0};
     nd_mlpcv(leaf_bytes, (uint64_t)nl, &v, (uint64_t)nn, contrib_bytes, (uint64_t)ncontrib,
-             &queries[dim * i], i, (uint32_t)(i / threads), samples, seed, &P, o);
+             &queries[dim * i], i, (uint32_t)(i / threads), samples, seed, seed_per_query, &P,
+             o);
     if (vec)
       for (int a = 0; a < dim; a++) out[dim * i + a] = o[a];
     else
       out[i] = o[0];
+  }
+}
+
+/* The pybind11 module's brute force (src/sbh_gpu.cu:39-77): the sum of a
+ * scalar kernel (0 potential, 3 winding number, 4 smooth distance) over the
+ * leaf records, here in fp64. */
+ORC_EXPORT void orc_nd_brute(int dim, const uint8_t* leaf_bytes, int64_t nl,
+                             const float* queries, int64_t nq, int kind, double* out) {
+  NdLayout L = nd_layout(dim, 3);
+  NdParams P;
+  memset(&P, 0, sizeof(P));
+  P.kind = kind;
+  for (int64_t i = 0; i < nq; i++) {
+    double q[3] = {0, 0, 0}, acc[3] = {0, 0, 0};
+    for (int a = 0; a < dim; a++) q[a] = queries[dim * i + a];
+    for (int64_t j = 0; j < nl; j++) nd_leaf_term(&P, &L, leaf_bytes, (uint64_t)j, q, acc);
+    out[i] = acc[0];
   }
 }

@@ -184,9 +184,14 @@ def test_python_mirror_validates_like_reference():
 def test_tree_builder_rejects(dim, pts, nrm, ms, lo, msg):
     bufs = [_buf(*x, ptr=FAKE) for x in (pts, nrm, ms, lo, lo)]
     nodes = ctypes.c_int64(-1)
-    rc, err = _call("sbh_tree_build", None, _i(dim), *map(ctypes.byref, bufs), _f(1.0),
+    rc, err = _call("sbh_tree_build", None, _i(dim), _i(3), *map(ctypes.byref, bufs), _f(1.0),
                     ctypes.byref(nodes))
     assert rc == 3 and msg in err and nodes.value == -1
+    if dim == 2 and msg.startswith("masses"):
+        bufs = [_buf(*x, ptr=FAKE) for x in ((F32, 5, 2), (F32, 5, 2), (F32, 5), lo, lo)]
+        rc, err = _call("sbh_tree_build", None, _i(2), _i(4), *map(ctypes.byref, bufs),
+                        _f(1.0), ctypes.byref(nodes))
+        assert rc == 3 and "level must be 1, 2 or 3" in err
 
 
 def test_quadtree_walks_reject_wrong_widths():

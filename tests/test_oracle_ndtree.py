@@ -21,9 +21,10 @@ from scipy.special import erfc
 from oracle import oracle as O
 
 
-def _np_build(p, lo, hi):
+def _np_build(p, lo, hi, level=3):
     """Breadth-first reference build in numpy fp32 (stable partition)."""
     n, d = p.shape
+    S = 1 << level
     order = np.arange(n)
     nodes = [dict(start=0, end=n, parent=0, depth=0, lo=lo.astype(np.float32),
                   hi=hi.astype(np.float32))]
@@ -38,17 +39,17 @@ def _np_build(p, lo, hi):
             with np.errstate(invalid="ignore", divide="ignore"):
                 # zero-width boxes at depth ~50+ give NaN: the x86 conversion
                 # yields INT_MIN, clipped to cell 0 (as the reference)
-                t = (np.float32(8) * (p[idx] - nd["lo"])) / (nd["hi"] - nd["lo"])
-                q = np.clip(np.trunc(t).astype(np.int64), 0, 7)
+                t = (np.float32(S) * (p[idx] - nd["lo"])) / (nd["hi"] - nd["lo"])
+                q = np.clip(np.trunc(t).astype(np.int64), 0, S - 1)
             cell = np.zeros(len(idx), np.int64)
             for a in reversed(range(d)):
-                cell = cell * 8 + q[:, a]
+                cell = cell * S + q[:, a]
             s = np.argsort(cell, kind="stable")
             order[nd["start"]:nd["end"]] = idx[s]
             cs = cell[s]
-            inc = (nd["hi"] - nd["lo"]) / np.float32(8)
+            inc = (nd["hi"] - nd["lo"]) / np.float32(S)
             for c in np.unique(cs):
-                k = np.array([(c >> (3 * a)) & 7 for a in range(d)], np.float32)
+                k = np.array([(c // S ** a) % S for a in range(d)], np.float32)
                 clo = (nd["lo"] + k * inc).astype(np.float32)
                 first = np.searchsorted(cs, c, "left")
                 last = np.searchsorted(cs, c, "right")
@@ -60,10 +61,8 @@ def _np_build(p, lo, hi):
     return nodes, order
 
 
-def _decode(node_bytes, dim):
-    if dim == 2:
-        return O.decode_nodes2(node_bytes)
-    return O.decode_nodes3(node_bytes)
+def _decode(node_bytes, dim, level=3):
+    return O.decode_nodes(node_bytes, dim, level)
 
 
 def _leaves(leaf_bytes, dim):
@@ -86,14 +85,18 @@ def _data(n, dim, seed, kind="uniform"):
     return p, nv, m, lo.astype(np.float32), hi.astype(np.float32)
 
 
+@pytest.mark.parametrize("level", [1, 2, 3])
 @pytest.mark.parametrize("dim", [2, 3])
 @pytest.mark.parametrize("n,seed,kind", [(400, 1, "uniform"), (300, 2, "clump"),
                                          (120, 3, "dups"), (2, 4, "uniform")])
-def test_ndtree_build_matches_numpy_partition(dim, n, seed, kind):
+def test_ndtree_build_matches_numpy_partition(dim, n, seed, kind, level):
     p, nv, m, lo, hi = _data(n, dim, seed, kind)
-    leaf, node, contrib = O.nd_build(p, nv, m, lo, hi, alpha=0.75)
-    N = _decode(node, dim)
-    ref, order = _np_build(p, lo, hi)
+    leaf, node, contrib = O.nd_build(p, nv, m, lo, hi, alpha=0.75, level=level)
+    N = _decode(node, dim, level)
+    ref, order = _np_build(p, lo, hi, level)
+    # layout sizes of Tree2DNode<1, L> / Tree3DNode<1, L>
+    assert N.dtype.itemsize == {(2, 1): 136, (2, 2): 232, (2, 3): 616, (3, 1): 184,
+                                (3, 2): 632, (3, 3): 4272}[(dim, level)]
     assert len(N) == len(ref)
     for i, r in enumerate(ref):
         assert (N[i]["start"], N[i]["end"], N[i]["parent_idx"], N[i]["depth"]) == \
@@ -103,12 +106,12 @@ def test_ndtree_build_matches_numpy_partition(dim, n, seed, kind):
         np.testing.assert_array_equal(N[i]["bmin"], r["lo"])
         np.testing.assert_array_equal(N[i]["bmax"], r["hi"])
         assert N[i]["num_children"] == len(r["children"])
-        cells = 64 if dim == 2 else 512
+        cells = 1 << (dim * level)
         expect = np.zeros(cells, np.uint64)
         for c, ch in r["children"]:
             expect[c] = ch
         np.testing.assert_array_equal(N[i]["child"], expect)
-        mask = np.zeros(cells // 64 if dim == 3 else 1, np.uint64)
+        mask = np.zeros(max(1, cells // 64), np.uint64)
         for c, _ in r["children"]:
             mask[c >> 6] |= np.uint64(1) << np.uint64(c & 63)
         np.testing.assert_array_equal(N[i]["bitmask"], mask)
@@ -278,3 +281,67 @@ def test_sampler_determinism_and_seed():
     c = O.nd_eval(leaf, node, contrib, q, O.L3_FORCE, samples=2, seed=4, threads=32)
     np.testing.assert_array_equal(a, b)
     assert not np.array_equal(a, c)
+
+
+def _direct_scalar(p, nv, m, q, kind, alpha=1.0):
+    p, nv, q, m = (x.astype(np.float64) for x in (p, nv, q, m))
+    d = p[None] - q[:, None]
+    r2 = (d * d).sum(2)
+    if kind == O.L3_POTENTIAL:
+        return (-m[None] / (np.sqrt(r2) + 0.1)).sum(1)
+    if kind == O.ND_WN:
+        dn = (nv[None] * d).sum(2)
+        if p.shape[1] == 2:
+            return (m[None] * dn / (2 * np.pi * (r2 + 1e-5))).sum(1)
+        return (m[None] * dn / (4 * np.pi * (r2 * np.sqrt(r2) + 1e-5))).sum(1)
+    return np.exp(-alpha * np.sqrt(r2)).sum(1)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("level", [1, 3])
+@pytest.mark.parametrize("kind", [O.L3_POTENTIAL, O.ND_WN, O.ND_SMOOTH])
+def test_module_kinds_limits_and_brute_force(dim, level, kind):
+    p, nv, m, lo, hi = _data(300, dim, 23, "clump")
+    leaf, node, contrib = O.nd_build(p, nv, m, lo, hi, alpha=0.6, level=level)
+    q = np.random.default_rng(9).uniform(-1.2, 1.2, (50, dim)).astype(np.float32)
+    ref = _direct_scalar(p, nv, m, q, kind, alpha=float(np.float32(0.6)))  # the records' fp32 alpha
+    exact = O.nd_eval(leaf, node, contrib, q, kind, beta=1e30, threads=32, level=level)
+    np.testing.assert_allclose(exact, ref, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(O.nd_brute(leaf, q, kind), ref, rtol=1e-9, atol=1e-12)
+    # beta = 0: the root record alone
+    N = _decode(node, dim, level)
+    r = N[0]["contrib"]
+    mono = _direct_scalar(np.asarray(r["pos"], np.float32)[None],
+                          np.asarray(r["normal"], np.float32)[None],
+                          np.asarray([r["mass"]], np.float32), q, kind, alpha=float(r["alpha"]))
+    np.testing.assert_allclose(O.nd_eval(leaf, node, contrib, q, kind, beta=0.0, threads=32,
+                                         level=level), mono, rtol=1e-9, atol=1e-12)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_sampler_unbiased_low_levels(level):
+    p, nv, m, lo, hi = _data(500, 2, 29, "clump")
+    leaf, node, contrib = O.nd_build(p, nv, m, lo, hi, level=level)
+    K = 4000
+    q0 = np.full((1, 2), -0.2, np.float32)
+    est = O.nd_eval(leaf, node, contrib, np.repeat(q0, K, 0), O.L3_POTENTIAL, samples=1,
+                    seed=2, threads=1, level=level)
+    exact = _direct_scalar(p, nv, m, q0, O.L3_POTENTIAL)[0]
+    se = est.std(ddof=1) / math.sqrt(K)
+    assert se > 0 and abs(est.mean() - exact) < 4.5 * se, (est.mean(), exact, se)
+
+
+def test_module_seeding_is_ffi_seeding_at_minus_query():
+    # the pybind11 module passes seed 0 to every query (src/sbh_gpu.cu:143-164);
+    # the FFI targets seed + query: the two agree at seed = -query
+    p, nv, m, lo, hi = _data(400, 3, 31, "clump")
+    leaf, node, contrib = O.nd_build(p, nv, m, lo, hi, level=1)
+    q = p[:96]
+    mod = O.nd_eval(leaf, node, contrib, q, O.L3_POTENTIAL, samples=2, threads=32, level=1,
+                    seed=0, seed_per_query=False)
+    for i in (0, 5, 40, 95):
+        ffi = O.nd_eval(leaf, node, contrib, q, O.L3_POTENTIAL, samples=2, threads=32, level=1,
+                        seed=-i, seed_per_query=True)
+        assert mod[i] == ffi[i]
+    assert not np.all(mod == O.nd_eval(leaf, node, contrib, q, O.L3_POTENTIAL, samples=2,
+                                       threads=32, level=1, seed=0, seed_per_query=True))
