@@ -3,8 +3,10 @@ device tensors: potential, winding number and smooth distance, each by brute
 force, classic Barnes-Hut and the multi-level prefix control variate
 (stochastic) walk.
 
-The reference builds its trees on the CPU and returns (values, kernel ms);
-here the walks run over octree8 records (ffi.build_octree8_buffers, every
+The reference builds its trees on the CPU and returns (values, kernel ms).
+sbh_amd.jax_sbh evaluates these kernels over the reference's own trees
+(Tree2D/3D<1, L>, as python/jax_sbh.py); this module runs them over octree8
+records (ffi.build_octree8_buffers, every
 record with alpha = 1 as the reference's octree8 builder writes) and each
 function returns a float32 tensor of shape (M,).  Kernels
 (include/kernel.h): gravityPotential -m / (r + 0.1), signedSolidAngle<3>,
