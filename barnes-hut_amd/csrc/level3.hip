@@ -235,9 +235,9 @@ enum L3Kind { kPotential = 0, kForce = 1, kSoftForce3 = 2, kWn = 3, kSmooth = 4 
 // Kinds with one value per query.
 template <int KIND>
 constexpr bool kScalarL3 = KIND == kPotential || KIND == kWn || KIND == kSmooth;
-// Kinds whose terms read only a record's position and mass (the rope walk).
+// Kinds whose terms also read a record's normal and alpha (the wide rope).
 template <int KIND>
-constexpr bool kRopeKind = KIND == kPotential || KIND == kForce || KIND == kSoftForce3;
+constexpr bool kFullRope = KIND == kWn || KIND == kSmooth;
 
 constexpr float kPiF = 3.14159265358979323846f;
 
@@ -478,21 +478,41 @@ constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kRopeMin = 2048;  // fewer nodes: the plain walk
 
-template <int DIM>
+// FULL records also carry the normal and alpha (winding number, smooth
+// distance); the others only what the gravity kernels read.
+template <int DIM, bool FULL>
 struct Rope;
 template <>
-struct Rope<2> {
+struct Rope<2, false> {
   float c[2];
   float m, diam;
   uint32_t start, end, first, escape;
 };
 template <>
-struct Rope<3> {
+struct Rope<3, false> {
   float c[3];
   float m, diam;
   uint32_t start, end, first, escape, pad[3];
 };
-static_assert(sizeof(Rope<2>) == 32 && sizeof(Rope<3>) == 48, "rope records");
+template <>
+struct Rope<2, true> {
+  float c[2];
+  float m, diam;
+  float n[2];
+  float alpha;
+  uint32_t start, end, first, escape, pad;
+};
+template <>
+struct Rope<3, true> {
+  float c[3];
+  float m, diam;
+  float n[3];
+  float alpha;
+  uint32_t start, end, first, escape, pad[3];
+};
+static_assert(sizeof(Rope<2, false>) == 32 && sizeof(Rope<3, false>) == 48 &&
+                  sizeof(Rope<2, true>) == 48 && sizeof(Rope<3, true>) == 64,
+              "rope records");
 
 template <int DIM, int LOG>
 __device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
@@ -510,19 +530,24 @@ __device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM, LOG>::N
   return static_cast<uint32_t>(parent->child[b]);
 }
 
-template <int DIM, int LOG>
+template <int DIM, int LOG, bool FULL>
 __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                                                 uint32_t num_nodes, uint32_t num_leaves,
-                                                Rope<DIM>* __restrict__ rope) {
+                                                Rope<DIM, FULL>* __restrict__ rope) {
   using Node = typename L3T<DIM, LOG>::Node;
   constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const uint32_t c = blockIdx.x * kB + threadIdx.x;
   if (c >= num_nodes) return;
   const Node* nd = &nodes[c];
-  Rope<DIM> r = {};
+  Rope<DIM, FULL> r = {};
 #pragma unroll
   for (int a = 0; a < DIM; ++a) r.c[a] = nd->contrib.pos[a];
   r.m = nd->contrib.mass;
+  if constexpr (FULL) {
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) r.n[a] = nd->contrib.normal[a];
+    r.alpha = nd->contrib.alpha;
+  }
   r.diam = nd->diameter;
   const uint64_t je = nd->end < num_leaves ? nd->end : num_leaves;
   const uint64_t js = nd->start < je ? nd->start : je;
@@ -548,7 +573,7 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
 
 template <int DIM, int LOG, int KIND>
 __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
-                               const Rope<DIM>* __restrict__ rope,
+                               const Rope<DIM, kFullRope<KIND>>* __restrict__ rope,
                                const float* __restrict__ queries, int nq,
                                float* __restrict__ out, L3Params A, uint32_t num_nodes) {
   using Leaf = typename L3T<DIM, LOG>::Leaf;
@@ -565,7 +590,7 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
   uint32_t c = 0;
   const uint64_t guard_max = 4ull * num_nodes + 64ull;
   for (uint64_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
-    const Rope<DIM> r = rope[c];
+    const Rope<DIM, kFullRope<KIND>> r = rope[c];
     const bool is_leaf = (r.start & kLeafBit) != 0u;
     float ratio;
     {
@@ -589,6 +614,11 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
 #pragma unroll
         for (int a = 0; a < DIM; ++a) L.pos[a] = r.c[a];
         L.mass = r.m;
+        if constexpr (kFullRope<KIND>) {
+#pragma unroll
+          for (int a = 0; a < DIM; ++a) L.normal[a] = r.n[a];
+          L.alpha = r.alpha;
+        }
         l3_term<KIND>(L, qv, A.P, acc);
       }
       c = r.escape;
@@ -774,12 +804,13 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   const uint32_t num_leaves = static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf));
   const auto* nodes = static_cast<const Node*>(node_bytes->data);
   // the rope needs a workspace: not under stream capture (no allocation there)
-  Rope<DIM>* rope = nullptr;
-  if (kRopeKind<KIND> && num_nodes >= kRopeMin && !stream_capturing(s))
-    rope = static_cast<Rope<DIM>*>(
-        workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(Rope<DIM>)));
+  using R = Rope<DIM, kFullRope<KIND>>;
+  R* rope = nullptr;
+  if (num_nodes >= kRopeMin && !stream_capturing(s))
+    rope = static_cast<R*>(workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(R)));
   if (rope != nullptr) {
-    k_l3_rope<DIM, LOG><<<grid_for(num_nodes, kB), kB, 0, s>>>(nodes, num_nodes, num_leaves, rope);
+    k_l3_rope<DIM, LOG, kFullRope<KIND>><<<grid_for(num_nodes, kB), kB, 0, s>>>(
+        nodes, num_nodes, num_leaves, rope);
     SBH_LAUNCH_CHECK();
     k_l3_walk_rope<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
         static_cast<const Leaf*>(leaf_bytes->data), rope,

@@ -211,7 +211,8 @@ def main():
                     help="time eager launches only (default: the step is also captured as a "
                          "HIP graph and the replay is the reported value)")
     ap.add_argument("--workload", default="gravity",
-                    choices=("gravity", "wn", "batched", "stochastic", "treepm", "quadtree"),
+                    choices=("gravity", "wn", "batched", "stochastic", "treepm", "quadtree",
+                             "fields"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
@@ -221,13 +222,15 @@ def main():
                          "short-range TreePM force (periodic, erfc cutoff, prune) on 128^3 "
                          "clustered particles; quadtree: the 2-D targets over Tree2D<1,3> "
                          "buffers (device build + vote walk, and the stochastic walk) on 2^20 "
-                         "points in a square")
+                         "points in a square; fields: the reference's tree fields module "
+                         "(python/jax_sbh.py defaults, level-1 octree) on 2^20 points")
     ap.add_argument("--o8-depth", type=int, default=8)
     ap.add_argument("--treepm-n1d", type=int, default=128,
                     help="treepm: particles per side (n1d^3 particles, box of n1d cells)")
     ap.add_argument("--treepm-depth", type=int, default=8)
     ap.add_argument("--qt-n", type=int, default=1 << 20,
-                    help="quadtree: number of 2-D points (queries = the points)")
+                    help="quadtree / fields: number of points (queries = the points)")
+    ap.add_argument("--fields-level", type=int, default=1, choices=(1, 2, 3))
     ap.add_argument("--seeds", type=int, default=256)
     ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--sets", type=int, default=64)
@@ -243,7 +246,7 @@ def main():
         sys.exit(2)
     bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
              "stochastic": bench_stochastic, "treepm": bench_treepm,
-             "quadtree": bench_quadtree}[args.workload]
+             "quadtree": bench_quadtree, "fields": bench_fields}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -994,6 +997,93 @@ def bench_quadtree(args):
         "phases_ms": {"tree_build": build_ms, "force_walk": walk_ms,
                       "potential_walk_threads64": pot_ms,
                       f"stochastic_force_walk_{ns}_queries_1_sample": stoch_ms},
+        "accuracy": acc,
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_fields(args):
+    """The reference's tree fields module (python/jax_sbh.py over the pybind11
+    GPU module, src/sbh_gpu.cu) with its defaults: build_gpu_tree(level=1), a
+    Tree3D<1,1> (a true octree, leaf size 1), then eval_barnes_hut_gravity
+    (beta 2, 32-query blocks) for every point, plus the winding-number and
+    smooth-distance walks, the sampler and the compensated brute force on a
+    sample.  2^20 points uniform in [-1, 1]^3, masses 1/N, unit normals."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import jax_sbh as J
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n = args.qt_n
+    level = args.fields_level
+    rng = np.random.default_rng(8000)
+    p = rng.uniform(-1.0, 1.0, (n, 3)).astype(np.float32)
+    nv = rng.normal(size=(n, 3))
+    nv = (nv / np.linalg.norm(nv, axis=1, keepdims=True)).astype(np.float32)
+    m = np.full(n, 1.0 / n, np.float32)
+    P, NV, M = (torch.tensor(x, device=dev) for x in (p, nv, m))
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            r = fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return r, max(e0.elapsed_time(e1), (time.perf_counter() - t0) * 1e3) / reps
+
+    tree, build_ms = timed(lambda: J.build_gpu_tree(P, NV, M, level=level), args.steps)
+    nodes = tree.node.numel() // {1: 184, 2: 632, 3: 4272}[level]
+    phi, walk_ms = timed(lambda: J.eval_barnes_hut_gravity(tree, P), args.steps)
+    _, wn_ms = timed(lambda: J.eval_barnes_hut_wn(tree, P), max(1, args.steps // 2))
+    _, sd_ms = timed(lambda: J.eval_barnes_hut_smoothdist(tree, P), max(1, args.steps // 2))
+    ns = min(n, 1 << 16)
+    _, mc_ms = timed(lambda: J.eval_mlpcv_gravity(tree, P[:ns]), max(1, args.steps // 2))
+    nb = min(n, 4096)
+    bf, bf_ms = timed(lambda: J.eval_brute_force_gravity(tree, P[:nb]), 1)
+    ms_step = build_ms + walk_ms
+    acc = {"potential_rel_l2_vs_compensated_brute_force": rel_l2(phi[:nb], bf),
+           "sample_queries": nb}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        lo = (p.min(0) - np.float32(1e-3)).astype(np.float32)
+        hi = (p.max(0) + np.float32(1e-3)).astype(np.float32)
+        t0 = time.perf_counter()
+        ol, on, oc = O.nd_build(p, nv, m, lo, hi, level=level)
+        cpu_build = time.perf_counter() - t0
+        nq = 1 << 12
+        t0 = time.perf_counter()
+        O.nd_eval(ol, on, oc, p[:nq], O.L3_POTENTIAL, beta=2.0, threads=32, level=level)
+        t_walk = (time.perf_counter() - t0) / nq
+        cpu = {"value": n / (cpu_build + t_walk * n), "unit": "potentials/s (build + walk)",
+               "cores": 1, "kind": "port",
+               "sample": f"oracle Tree3D<1,{level}> build (the reference's host builder "
+                         f"restated, {cpu_build:.2f} s for all {n} points) + oracle vote walk "
+                         f"(fp64, same 32-query warps) on the first {nq} queries, scaled; one "
+                         f"thread"}
+    return {
+        "metric": "tree-module BH potentials/sec (Tree3D<1,L>: build + warp-voted walk)",
+        "value": n / (ms_step * 1e-3),
+        "unit": "potentials/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"jax_sbh_uniform_cube_n{n}_level{level}_beta2_threads32",
+                   "n": n, "level": level, "nodes": nodes, "beta": 2.0, "threads": 32,
+                   "step": "Tree3D<1,L> device build + gravityPotential vote walk"},
+        "phases_ms": {"tree_build": build_ms, "potential_walk": walk_ms,
+                      "winding_number_walk": wn_ms, "smooth_distance_walk": sd_ms,
+                      f"sampler_{ns}_queries_1_sample": mc_ms,
+                      f"brute_force_{nb}_queries": bf_ms},
         "accuracy": acc,
         "roofline": None,
         "cpu_baseline": cpu,

@@ -123,3 +123,29 @@ def test_module_rejects(gpu):
         J.eval_barnes_hut_gravity(t, np.zeros((5, 3), np.float32))
     with pytest.raises(ValueError):
         J.eval_mlpcv_gravity(t, p, samples_per_subdomain=-1)
+
+
+@pytest.mark.parametrize("level", [1, 3])
+def test_module_rope_walks_equal_plain_walks(gpu, level):
+    # Eager calls on trees of >= 2048 nodes walk the compact rope records (with
+    # normal and alpha for the winding number and the smooth distance); under
+    # stream capture the plain walk over the node records runs.  Identical bits.
+    J = _mod()
+    p, nv, m, lo, hi = _case(8000, 3, 400 + level)
+    t = J.build_gpu_tree(p, nv, m, lo, hi, alpha=1.5, level=level)
+    Q = torch.tensor(p[:3000], device="cuda")
+    names = ["gravity", "wn", "smoothdist"]
+    eager = {k: _np(getattr(J, f"eval_barnes_hut_{k}")(t, Q)) for k in names}
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        J.eval_barnes_hut_wn(t, Q)
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = {k: getattr(J, f"eval_barnes_hut_{k}")(t, Q) for k in names}
+    for _ in range(2):
+        graph.replay()
+        torch.cuda.synchronize()
+        for k in names:
+            np.testing.assert_array_equal(_np(outs[k]), eager[k])
