@@ -45,6 +45,9 @@ class SbhGpuTree:
     node: torch.Tensor
     contrib: torch.Tensor
 
+    def close(self) -> None:
+        """python/jax_sbh.py:32-47 (SbhGpuWorker.close): nothing to stop here."""
+
 
 def build_gpu_tree(points, normals, masses, min_corner=None, max_corner=None, *,
                    alpha: float = 1.0, level: int = 1) -> SbhGpuTree:
