@@ -272,3 +272,19 @@ def test_tree_vote_walk_rope_equals_plain_walk(gpu, dim):
         np.testing.assert_array_equal(_np(f), eager["f"])
         np.testing.assert_array_equal(_np(g), eager["g"])
         np.testing.assert_array_equal(_np(s), eager["s"])
+
+
+def test_tree_builder_full_size_2d(gpu):
+    # 2^20 points (the quadtree workload's size): bytes equal the oracle's
+    ffi = _ffi()
+    rng = np.random.default_rng(7000)
+    p = rng.uniform(-1.0, 1.0, (1 << 20, 2)).astype(np.float32)
+    nv = np.zeros_like(p)
+    m = np.full(1 << 20, 1.0 / (1 << 20), np.float32)
+    lo, hi = _corners(p, 2)
+    leaf, node, contrib = _build(ffi, 2, p, nv, m, lo, hi)
+    ol, on, oc = O.nd_build(p, nv, m, lo, hi)
+    assert node.numel() == on.size
+    assert np.array_equal(_np(leaf), ol)
+    assert np.array_equal(_np(contrib), oc)
+    assert np.array_equal(_np(node), on)
