@@ -282,6 +282,9 @@ def _force_vjp_finite_difference_elementwise(forward_force_fn: Callable, queries
 # ------------------------------------------------------------------ octree8 (drop-in mode)
 def _default_corners(points, min_corner, max_corner):
     # python/jax_sbh_ffi.py:879-884: fp32 min - 1e-3 / max + 1e-3.
+    if (min_corner is None or max_corner is None) and points.shape[0] == 0:
+        # numpy's min/max of an empty array raise ValueError there too
+        raise ValueError("zero-size array: pass min_corner and max_corner for empty points")
     if min_corner is None:
         min_corner = points.min(dim=0).values - np.float32(1e-3)
     if max_corner is None:
