@@ -211,3 +211,32 @@ def test_quadtree_walks_reject_wrong_widths():
                     *map(ctypes.byref, (bad_leaf, node, q2, _buf(F32, 4, ptr=FAKE))),
                     _f(2.0), _i(0), _f(1.0), _i(64))
     assert rc == 3 and "leaf_bytes size mismatch" in err
+
+
+def test_tree_module_entries_reject():
+    # sbh_tree_eval / sbh_tree_bruteforce: the pybind11 module's evaluations
+    leaf, node = _buf(U8, 24 * 5, ptr=FAKE), _buf(U8, 136 * 3, ptr=FAKE)
+    q2, out = _buf(F32, 4, 2, ptr=FAKE), _buf(F32, 4, ptr=FAKE)
+    args = (leaf, node, node, q2, out)
+    rc, err = _call("sbh_tree_eval", None, _i(4), _i(1), _i(0), *map(ctypes.byref, args),
+                    _f(2.0), _i(-1))
+    assert rc == 3 and "dim must be 2 or 3" in err
+    rc, err = _call("sbh_tree_eval", None, _i(2), _i(0), _i(0), *map(ctypes.byref, args),
+                    _f(2.0), _i(-1))
+    assert rc == 3 and "level must be 1, 2 or 3" in err
+    rc, err = _call("sbh_tree_eval", None, _i(2), _i(1), _i(5), *map(ctypes.byref, args),
+                    _f(2.0), _i(-1))
+    assert rc == 3 and "kind must be" in err
+    # level 2 records are 232 bytes: 3 * 136 is not a multiple
+    rc, err = _call("sbh_tree_eval", None, _i(2), _i(2), _i(0), *map(ctypes.byref, args),
+                    _f(2.0), _i(-1))
+    assert rc == 3 and "node_bytes size mismatch" in err
+    rc, err = _call("sbh_tree_eval", None, _i(2), _i(1), _i(0), ctypes.byref(leaf),
+                    ctypes.byref(node), None, ctypes.byref(q2), ctypes.byref(out), _f(2.0), _i(1))
+    assert rc == 3 and "contrib_bytes must be given" in err
+    rc, err = _call("sbh_tree_bruteforce", None, _i(3), _i(0), ctypes.byref(leaf),
+                    ctypes.byref(q2), ctypes.byref(out))
+    assert rc == 3 and "leaf_bytes size mismatch" in err  # 120 bytes are not LeafData<3>s
+    rc, err = _call("sbh_tree_bruteforce", None, _i(2), _i(0), ctypes.byref(leaf),
+                    ctypes.byref(_buf(F32, 4, 3, ptr=FAKE)), ctypes.byref(out))
+    assert rc == 3 and "queries dim must be 2" in err

@@ -114,7 +114,7 @@ def _walk(ffi, name, leaf, node, contrib, q, **kw):
 
 
 @pytest.mark.parametrize("dim", [2, 3])
-@pytest.mark.parametrize("threads", [32, 96, 256])
+@pytest.mark.parametrize("threads", [32, 48, 96, 256])  # 48: a 16-query warp per block
 def test_tree_vote_walks_match_oracle(gpu, dim, threads):
     ffi = _ffi()
     p, nv, m = _case(20000, dim, 31, "clump")
