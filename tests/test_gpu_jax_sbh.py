@@ -149,3 +149,39 @@ def test_module_rope_walks_equal_plain_walks(gpu, level):
         torch.cuda.synchronize()
         for k in names:
             np.testing.assert_array_equal(_np(outs[k]), eager[k])
+
+
+def test_tree_builder_random_sweep(gpu):
+    # 24 random configurations: dim, level, size (0..6000), clustering,
+    # coincident points, points outside the given box, masses with zeros
+    from sbh_amd import ffi
+
+    rng = np.random.default_rng(4242)
+    for case in range(24):
+        dim = int(rng.integers(2, 4))
+        level = int(rng.integers(1, 4))
+        n = int(rng.choice([0, 1, 2, 3, 17, 64, 500, 3000, 6000]))
+        p = rng.uniform(-1, 1, (n, dim))
+        if n > 4 and rng.random() < 0.5:
+            k = n // 2
+            p[:k] = rng.uniform(-1, 1, dim) + rng.normal(0, 10.0 ** rng.uniform(-6, -2), (k, dim))
+        if n > 4 and rng.random() < 0.3:
+            p[: n // 5] = p[-1]
+        p = p.astype(np.float32)
+        nv = rng.normal(size=(n, dim)).astype(np.float32)
+        m = rng.uniform(0, 1, n).astype(np.float32)
+        if n > 4:
+            m[rng.random(n) < 0.1] = 0.0
+        if rng.random() < 0.3 or n == 0:
+            lo, hi = np.full(dim, -0.8, np.float32), np.full(dim, 0.9, np.float32)
+        else:
+            lo = (p.min(0) - np.float32(1e-3)).astype(np.float32)
+            hi = (p.max(0) + np.float32(1e-3)).astype(np.float32)
+        alpha = float(rng.uniform(0.1, 2.0))
+        leaf, node, contrib = ffi._tree_buffers(p, nv, m, lo, hi, alpha, dim, level)
+        ol, on, oc = O.nd_build(p, nv, m, lo, hi, alpha=alpha, level=level)
+        info = (case, dim, level, n)
+        assert node.numel() == on.size, info
+        _same_words(_np(leaf), ol)
+        _same_words(_np(node), on)
+        _same_words(_np(contrib), oc)
