@@ -154,6 +154,8 @@ def test_tree_stochastic_walks_match_oracle(gpu, dim):
             ("stochastic_barnes_hut_force", O.L3_FORCE, {}, {}),
             ("softened_stochastic_barnes_hut_force", O.L3_SOFT,
              dict(softening_scale=0.02, cutoff_scale=0.5), dict(eps=0.02, rc=0.5)),
+            ("stochastic_barnes_hut_force", O.L3_FORCE, dict(periodic=True, box_length=2.0),
+             dict(periodic=True, box=2.0)),
         ]:
             out = _walk(ffi, name, leaf, node, contrib, q, samples_per_subdomain=2, seed=7,
                         threads=threads, **kw)
