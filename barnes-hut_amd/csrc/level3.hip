@@ -22,6 +22,7 @@
 // Stochastic walk: per subdomain (child of the root) its exact sum or
 // monopole plus the control-variate corrections along roulette paths to a
 // sample point; level-3 buffers have no paths (their subdomains are leaves).
+#include <cfloat>
 #include "common.hpp"
 #include "o8walk.hpp"
 
@@ -256,6 +257,14 @@ struct L3Params {
   int threads;  // the caller's block size: reference warps are 32 queries of a block
 };
 
+// The reference builds its GPU kernels with --use_fast_math
+// (CMakeLists.txt:34-39): square roots and quotients in the kernel
+// functions below are the hardware approximations (1 ulp), not the
+// correctly rounded sequences.  Decisions (the MAC, cell quantisation) keep
+// IEEE arithmetic so that they equal the oracle's.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_div(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // Contribution of one record to a query (gravityPotential, gravityGrad,
 // gravityGradSoftCutoff<N>: include/kernel.h:55-60, 89-94; src/sbh_ffi.cu:142-160).
 template <int KIND>
@@ -266,13 +275,13 @@ __device__ __forceinline__ void l3_term(const Leaf3& L, const float* q,
     const float dx = min_image_floor(L.pos[0] - qx, P.periodic, P.box, P.inv_box);
     const float dy = min_image_floor(L.pos[1] - qy, P.periodic, P.box, P.inv_box);
     const float dz = min_image_floor(L.pos[2] - qz, P.periodic, P.box, P.inv_box);
-    const float dist = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float dist = hw_sqrt(dx * dx + dy * dy + dz * dz);
     if constexpr (KIND == kPotential) {
-      acc[0] += -L.mass / (dist + 1e-1f);
+      acc[0] += hw_div(-L.mass, dist + 1e-1f);
     } else if constexpr (KIND == kWn) {
       // signedSolidAngle<3> (include/kernel.h:73-80)
       const float qn = L.normal[0] * dx + L.normal[1] * dy + L.normal[2] * dz;
-      acc[0] += L.mass * qn / (4.0f * kPiF * (dist * dist * dist + 1e-5f));
+      acc[0] += hw_div(L.mass * qn, 4.0f * kPiF * (dist * dist * dist + 1e-5f));
     } else {
       // smoothDistExp (include/kernel.h:82-87)
       acc[0] += expf(-L.alpha * dist);
@@ -293,13 +302,13 @@ __device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
     const float dy = min_image_floor(L.pos[1] - q[1], P.periodic, P.box, P.inv_box);
     if constexpr (KIND == kPotential) {
       // gravityPotential<2>
-      acc[0] += -L.mass / (sqrtf(dx * dx + dy * dy) + 1e-1f);
+      acc[0] += hw_div(-L.mass, hw_sqrt(dx * dx + dy * dy) + 1e-1f);
     } else if constexpr (KIND == kWn) {
       // signedSolidAngle<2> (include/kernel.h:65-71)
       const float qn = L.normal[0] * dx + L.normal[1] * dy;
-      acc[0] += L.mass * qn / (2.0f * kPiF * (dx * dx + dy * dy + 1e-5f));
+      acc[0] += hw_div(L.mass * qn, 2.0f * kPiF * (dx * dx + dy * dy + 1e-5f));
     } else {
-      acc[0] += expf(-L.alpha * sqrtf(dx * dx + dy * dy));
+      acc[0] += expf(-L.alpha * hw_sqrt(dx * dx + dy * dy));
     }
   } else {
     // gravityGrad<2> and gravityGradSoftCutoff<2>: d = q - p
@@ -307,8 +316,8 @@ __device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
     const float dy = min_image_floor(q[1] - L.pos[1], P.periodic, P.box, P.inv_box);
     float c;
     if constexpr (KIND == kForce) {
-      const float t = sqrtf(dx * dx + dy * dy) + 1e-1f;
-      c = -L.mass / (t * t * t);
+      const float t = hw_sqrt(dx * dx + dy * dy) + 1e-1f;
+      c = hw_div(-L.mass, t * t * t);
     } else {
       const float r2 = dx * dx + dy * dy;
       const float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
@@ -473,7 +482,11 @@ __global__ void k_l3_walk(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
 // AABB::quantize of the midpoint in the parent, else the parent's escape;
 // none past the root).  Visit order, votes and sums are those of k_l3_walk;
 // each step is one dependent load instead of the node, its parent and the
-// child tables.
+// child tables.  A leaf's centre of mass is never read (leaves always
+// accept), so a leaf of one point carries that point's record (position,
+// mass, normal, alpha) in those fields and needs no second load; and since
+// the next record is either the first child or the escape, both are fetched
+// as soon as the current one arrives, before its vote is known.
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kRopeMin = 2048;  // fewer nodes: the plain walk
@@ -532,6 +545,7 @@ __device__ __forceinline__ uint32_t next_sibling(const typename L3T<DIM, LOG>::N
 
 template <int DIM, int LOG, bool FULL>
 __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
+                                                const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
                                                 uint32_t num_nodes, uint32_t num_leaves,
                                                 Rope<DIM, FULL>* __restrict__ rope) {
   using Node = typename L3T<DIM, LOG>::Node;
@@ -553,6 +567,18 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
   const uint64_t js = nd->start < je ? nd->start : je;
   r.start = static_cast<uint32_t>(js) | (nd->is_leaf ? kLeafBit : 0u);
   r.end = static_cast<uint32_t>(je);
+  if (nd->is_leaf && je - js == 1) {
+    // the leaf's only point, as the leaf sum would read it
+    const auto& L = leaf[js];
+#pragma unroll
+    for (int a = 0; a < DIM; ++a) r.c[a] = L.pos[a];
+    r.m = L.mass;
+    if constexpr (FULL) {
+#pragma unroll
+      for (int a = 0; a < DIM; ++a) r.n[a] = L.normal[a];
+      r.alpha = L.alpha;
+    }
+  }
   const int b = next_bit<DIM, LOG>(nd, 0);
   r.first = (b < kN && nd->child[b] < num_nodes) ? static_cast<uint32_t>(nd->child[b]) : kNone;
   // finishing node c: next sibling, else climb (the reference's FromChild
@@ -571,12 +597,33 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
   rope[c] = r;
 }
 
-template <int DIM, int LOG, int KIND>
+// farFieldRatio >= beta as far_ratio() decides it (correctly rounded square
+// root and quotient), decided from the squared distance s alone where s is
+// clear of (beta diameter)^2 by a relative 2^-14: the roundings of the
+// exact expression and of (beta diameter)^2 move it by under 2^-21, so both
+// give the same answer there.  Everything else (a margin hit, beta <= 0,
+// zero, huge, infinite or NaN operands) takes the exact expression.
+__device__ __forceinline__ bool far_enough(float s, float diam, float beta) {
+#pragma clang fp contract(off)
+  const float t = beta * diam;
+  const float t2 = t * t;
+  const float hi = t2 * (1.0f + 0x1p-14f), lo = t2 * (1.0f - 0x1p-14f);
+  const bool scaled = beta > 0.0f && t2 >= FLT_MIN && t2 <= FLT_MAX;
+  if (scaled && s > hi) return true;
+  if (scaled && s < lo) return false;
+  return sqrtf(s) / diam >= beta;
+}
+
+template <int DIM, int LOG, int KIND, bool PER>
 __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
                                const Rope<DIM, kFullRope<KIND>>* __restrict__ rope,
                                const float* __restrict__ queries, int nq,
                                float* __restrict__ out, L3Params A, uint32_t num_nodes) {
   using Leaf = typename L3T<DIM, LOG>::Leaf;
+  // periodicity is a template argument here: the non-periodic walk carries
+  // no minimum-image arithmetic
+  o8::WalkParams P = A.P;
+  P.periodic = PER ? 1 : 0;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = q < nq;
   const int lane = __lane_id();
@@ -589,27 +636,38 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
   float acc[DIM] = {};
   uint32_t c = 0;
   const uint64_t guard_max = 4ull * num_nodes + 64ull;
+  Rope<DIM, kFullRope<KIND>> r = rope[0];
   for (uint64_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
-    const Rope<DIM, kFullRope<KIND>> r = rope[c];
+    // both possible successors, clamped into the array (a kNone successor
+    // ends the loop before its record is used)
+    const uint32_t open_next = r.first != kNone ? r.first : r.escape;
+    const Rope<DIM, kFullRope<KIND>> r_open = rope[min(open_next, num_nodes - 1u)];
+    const Rope<DIM, kFullRope<KIND>> r_esc = rope[min(r.escape, num_nodes - 1u)];
     const bool is_leaf = (r.start & kLeafBit) != 0u;
-    float ratio;
+    bool far;
     {
 #pragma clang fp contract(off)
-      // farFieldRatio, as far_ratio()
+      // farFieldRatio >= beta, as far_ratio() (a leaf's ratio is infinite).
+      // The difference takes the sign opposite to the kind's pair kernel
+      // (p - q for the scalar kinds, q - p for the forces): the same s, bit
+      // for bit, but no product the compiler could share with the pair
+      // kernel of a one-point leaf (whose record holds the point), so the
+      // pair kernel's contraction does not depend on the walk around it.
       float s = 0.0f;
 #pragma unroll
       for (int a = 0; a < DIM; ++a) {
-        const float d = qv[a] - r.c[a];
+        const float d = kScalarL3<KIND> ? qv[a] - r.c[a] : r.c[a] - qv[a];
         s = a == 0 ? d * d : s + d * d;
       }
-      ratio = is_leaf ? INFINITY : sqrtf(s) / r.diam;
+      far = is_leaf ? INFINITY >= A.beta : far_enough(s, r.diam, A.beta);
     }
-    const bool far = ratio >= A.beta;
     const bool all = (__ballot(far) & members) == members;
     if (all) {
-      if (is_leaf) {
-        l3_leaf_sum<KIND>(leaf, r.start & ~kLeafBit, r.end, qv, A.P, acc);
+      const uint32_t s0 = r.start & ~kLeafBit;
+      if (is_leaf && r.end - s0 != 1u) {
+        l3_leaf_sum<KIND>(leaf, s0, r.end, qv, P, acc);
       } else {
+        // an internal node's monopole, or a one-point leaf's point
         Leaf L = {};
 #pragma unroll
         for (int a = 0; a < DIM; ++a) L.pos[a] = r.c[a];
@@ -619,11 +677,13 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
           for (int a = 0; a < DIM; ++a) L.normal[a] = r.n[a];
           L.alpha = r.alpha;
         }
-        l3_term<KIND>(L, qv, A.P, acc);
+        l3_term<KIND>(L, qv, P, acc);
       }
       c = r.escape;
+      r = r_esc;
     } else {
-      c = r.first != kNone ? r.first : r.escape;
+      c = open_next;
+      r = r_open;
     }
   }
   store_out<DIM, KIND>(out, q, acc);
@@ -810,12 +870,13 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
     rope = static_cast<R*>(workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(R)));
   if (rope != nullptr) {
     k_l3_rope<DIM, LOG, kFullRope<KIND>><<<grid_for(num_nodes, kB), kB, 0, s>>>(
-        nodes, num_nodes, num_leaves, rope);
+        nodes, static_cast<const Leaf*>(leaf_bytes->data), num_nodes, num_leaves, rope);
     SBH_LAUNCH_CHECK();
-    k_l3_walk_rope<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
-        static_cast<const Leaf*>(leaf_bytes->data), rope,
-        static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
-        num_nodes);
+    auto walk = A.P.periodic ? k_l3_walk_rope<DIM, LOG, KIND, true>
+                             : k_l3_walk_rope<DIM, LOG, KIND, false>;
+    walk<<<blocks, threads, 0, s>>>(static_cast<const Leaf*>(leaf_bytes->data), rope,
+                                    static_cast<const float*>(queries->data), nq,
+                                    static_cast<float*>(out->data), A, num_nodes);
   } else {
     k_l3_walk<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
         static_cast<const Leaf*>(leaf_bytes->data), nodes,
@@ -859,8 +920,13 @@ int launch_l3_stoch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* 
 // brute force (compensated sums over the leaf records in tree order,
 // include/brute_force_gpu.h:18-66), the vote walk and the control-variate
 // sampler, all with 32-thread blocks; the sampler passes seed 0 to every
-// query (src/sbh_gpu.cu:143-211, 436-550).
+// query (src/sbh_gpu.cu:143-211, 436-550).  The sampler keeps 32-thread
+// blocks (its sample point is drawn per block).  The walk's vote groups are
+// 32 consecutive queries for any block size that is a multiple of 32 (each
+// group is one half of a wave), so it runs in 256-thread blocks: the same
+// votes and sums, with 64 lanes per wave instead of 32.
 constexpr int kModuleThreads = 32;
+constexpr int kModuleWalkThreads = 256;
 
 template <int DIM, int KIND>
 __global__ __launch_bounds__(kB) void k_tree_brute(const LeafT<DIM>* __restrict__ leaf,
@@ -901,15 +967,16 @@ int module_eval(void* stream, int kind, const SbhBuffer* leaf, const SbhBuffer* 
                 const SbhBuffer* contrib, const SbhBuffer* q, SbhBuffer* out, float beta,
                 int samples) {
   constexpr int T = kModuleThreads;
+  constexpr int W = kModuleWalkThreads;
   if (samples < 0) {
     if (kind == 0)
       return launch_l3_walk<DIM, LOG, kPotential>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0,
-                                                  1.f, T);
+                                                  1.f, W);
     if (kind == 1)
       return launch_l3_walk<DIM, LOG, kWn>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0, 1.f,
-                                           T);
+                                           W);
     return launch_l3_walk<DIM, LOG, kSmooth>(stream, leaf, node, q, out, beta, 0.f, 0.f, 0,
-                                             1.f, T);
+                                             1.f, W);
   }
   if (kind == 0)
     return launch_l3_stoch<DIM, LOG, kPotential>(stream, leaf, node, contrib, q, out, samples,
