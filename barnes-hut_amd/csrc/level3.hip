@@ -484,9 +484,12 @@ __global__ void k_l3_walk(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
 // each step is one dependent load instead of the node, its parent and the
 // child tables.  A leaf's centre of mass is never read (leaves always
 // accept), so a leaf of one point carries that point's record (position,
-// mass, normal, alpha) in those fields and needs no second load; and since
-// the next record is either the first child or the escape, both are fetched
-// as soon as the current one arrives, before its vote is known.
+// mass, normal, alpha) in those fields and needs no second load.  The walk
+// is bound by vector instruction issue (eight waves per SIMD hide the load
+// latency), so its loop keeps the exact ratio off the common path.  Measured
+// slower: fetching both possible successors ahead of the vote (the selects
+// cost more than the latency they hide), and votes kept wholly in scalar
+// masks (the scalar unit, shared by a CU's four SIMDs, becomes the bound).
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kRopeMin = 2048;  // fewer nodes: the plain walk
@@ -597,21 +600,20 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
   rope[c] = r;
 }
 
-// farFieldRatio >= beta as far_ratio() decides it (correctly rounded square
-// root and quotient), decided from the squared distance s alone where s is
-// clear of (beta diameter)^2 by a relative 2^-14: the roundings of the
-// exact expression and of (beta diameter)^2 move it by under 2^-21, so both
-// give the same answer there.  Everything else (a margin hit, beta <= 0,
-// zero, huge, infinite or NaN operands) takes the exact expression.
-__device__ __forceinline__ bool far_enough(float s, float diam, float beta) {
-#pragma clang fp contract(off)
-  const float t = beta * diam;
-  const float t2 = t * t;
-  const float hi = t2 * (1.0f + 0x1p-14f), lo = t2 * (1.0f - 0x1p-14f);
-  const bool scaled = beta > 0.0f && t2 >= FLT_MIN && t2 <= FLT_MAX;
-  if (scaled && s > hi) return true;
-  if (scaled && s < lo) return false;
-  return sqrtf(s) / diam >= beta;
+// farFieldRatio >= beta (far_ratio(): correctly rounded square root and
+// quotient) is decided from the squared distance s alone where s is clear of
+// (beta diameter)^2 by a relative 2^-14: the roundings of the exact
+// expression and of (beta diameter)^2 move it by under 2^-21, so both give
+// the same answer there.  Everything else (a margin hit, beta <= 0, zero,
+// huge, infinite or NaN operands) takes the exact expression.
+constexpr float kMarginHi = 1.0f + 0x1p-14f;
+constexpr float kMarginLo = 1.0f - 0x1p-14f;
+
+template <int DIM, bool FULL>
+__device__ __forceinline__ Rope<DIM, FULL> rope_at(const char* __restrict__ base, uint32_t c) {
+  // record offsets fit 32 bits (the launcher checks): a scalar base and a
+  // 32-bit vector offset per load
+  return *reinterpret_cast<const Rope<DIM, FULL>*>(base + c * static_cast<uint32_t>(sizeof(Rope<DIM, FULL>)));
 }
 
 template <int DIM, int LOG, int KIND, bool PER>
@@ -620,6 +622,7 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
                                const float* __restrict__ queries, int nq,
                                float* __restrict__ out, L3Params A, uint32_t num_nodes) {
   using Leaf = typename L3T<DIM, LOG>::Leaf;
+  using R = Rope<DIM, kFullRope<KIND>>;
   // periodicity is a template argument here: the non-periodic walk carries
   // no minimum-image arithmetic
   o8::WalkParams P = A.P;
@@ -634,20 +637,18 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
 #pragma unroll
   for (int a = 0; a < DIM; ++a) qv[a] = queries[static_cast<int64_t>(DIM) * q + a];
   float acc[DIM] = {};
+  const char* __restrict__ base = reinterpret_cast<const char*>(rope);
+  // a leaf's ratio is infinite: far for any beta but NaN
+  const bool leaf_far = INFINITY >= A.beta;
+  const bool fast = A.beta > 0.0f;
+  const uint32_t guard_max = num_nodes > 0x3ffffff0u ? 0xffffffffu : 4u * num_nodes + 64u;
   uint32_t c = 0;
-  const uint64_t guard_max = 4ull * num_nodes + 64ull;
-  Rope<DIM, kFullRope<KIND>> r = rope[0];
-  for (uint64_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
-    // both possible successors, clamped into the array (a kNone successor
-    // ends the loop before its record is used)
-    const uint32_t open_next = r.first != kNone ? r.first : r.escape;
-    const Rope<DIM, kFullRope<KIND>> r_open = rope[min(open_next, num_nodes - 1u)];
-    const Rope<DIM, kFullRope<KIND>> r_esc = rope[min(r.escape, num_nodes - 1u)];
+  for (uint32_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
+    const R r = rope_at<DIM, kFullRope<KIND>>(base, c);
     const bool is_leaf = (r.start & kLeafBit) != 0u;
-    bool far;
+    uint64_t far_mask;
     {
 #pragma clang fp contract(off)
-      // farFieldRatio >= beta, as far_ratio() (a leaf's ratio is infinite).
       // The difference takes the sign opposite to the kind's pair kernel
       // (p - q for the scalar kinds, q - p for the forces): the same s, bit
       // for bit, but no product the compiler could share with the pair
@@ -659,9 +660,17 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
         const float d = kScalarL3<KIND> ? qv[a] - r.c[a] : r.c[a] - qv[a];
         s = a == 0 ? d * d : s + d * d;
       }
-      far = is_leaf ? INFINITY >= A.beta : far_enough(s, r.diam, A.beta);
+      const float t = A.beta * r.diam;
+      const float t2 = t * t;
+      const bool scaled = fast && t2 >= FLT_MIN && t2 <= FLT_MAX;
+      const bool sure_far = is_leaf ? leaf_far : scaled && s > t2 * kMarginHi;
+      const bool sure_near = is_leaf ? !leaf_far : scaled && s < t2 * kMarginLo;
+      far_mask = __ballot(sure_far) & members;
+      // the exact ratio only for a vote that the sure answers leave open
+      if (far_mask != members && (__ballot(sure_near) & members) == 0)
+        far_mask |= __ballot(!sure_far && sqrtf(s) / r.diam >= A.beta) & members;
     }
-    const bool all = (__ballot(far) & members) == members;
+    const bool all = far_mask == members;
     if (all) {
       const uint32_t s0 = r.start & ~kLeafBit;
       if (is_leaf && r.end - s0 != 1u) {
@@ -679,12 +688,8 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
         }
         l3_term<KIND>(L, qv, P, acc);
       }
-      c = r.escape;
-      r = r_esc;
-    } else {
-      c = open_next;
-      r = r_open;
     }
+    c = all ? r.escape : (r.first != kNone ? r.first : r.escape);
   }
   store_out<DIM, KIND>(out, q, acc);
 }
@@ -863,10 +868,12 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   const uint32_t num_nodes = static_cast<uint32_t>(byte_count(node_bytes) / sizeof(Node));
   const uint32_t num_leaves = static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf));
   const auto* nodes = static_cast<const Node*>(node_bytes->data);
-  // the rope needs a workspace: not under stream capture (no allocation there)
+  // the rope needs a workspace (not under stream capture: no allocation
+  // there) and record offsets within 32 bits
   using R = Rope<DIM, kFullRope<KIND>>;
   R* rope = nullptr;
-  if (num_nodes >= kRopeMin && !stream_capturing(s))
+  if (num_nodes >= kRopeMin && static_cast<uint64_t>(num_nodes) * sizeof(R) <= (1ull << 32) &&
+      !stream_capturing(s))
     rope = static_cast<R*>(workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(R)));
   if (rope != nullptr) {
     k_l3_rope<DIM, LOG, kFullRope<KIND>><<<grid_for(num_nodes, kB), kB, 0, s>>>(
