@@ -492,23 +492,34 @@ __global__ void k_l3_walk(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
 // masks (the scalar unit, shared by a CU's four SIMDs, becomes the bound).
 constexpr uint32_t kNone = 0xffffffffu;
 constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kSumBit = 0x40000000u;
+constexpr uint32_t kIndexMask = 0x3fffffffu;
 constexpr uint32_t kRopeMin = 2048;  // fewer nodes: the plain walk
+// Relative margins of the far-field test's sure answers (k_l3_walk_rope).
+constexpr float kMarginHi = 1.0f + 0x1p-14f;
+constexpr float kMarginLo = 1.0f - 0x1p-13f;  // applied to the sure-far bound
 
 // FULL records also carry the normal and alpha (winding number, smooth
-// distance); the others only what the gravity kernels read.
+// distance); the others only what the gravity kernels read.  Per call (the
+// records are rebuilt for every walk): start = first point | kLeafBit for a
+// leaf | kSumBit for a leaf whose points need the leaf sum (any count but
+// one); aux = end of the point range for a leaf, else the bits of the sure-
+// far bound (beta diameter)^2 (1 + 2^-14), NaN where that bound is not
+// usable; open = the node to visit when this one is opened (its first
+// child, else its escape).
 template <int DIM, bool FULL>
 struct Rope;
 template <>
 struct Rope<2, false> {
   float c[2];
   float m, diam;
-  uint32_t start, end, first, escape;
+  uint32_t start, aux, open, escape;
 };
 template <>
 struct Rope<3, false> {
   float c[3];
   float m, diam;
-  uint32_t start, end, first, escape, pad[3];
+  uint32_t start, aux, open, escape, pad[3];
 };
 template <>
 struct Rope<2, true> {
@@ -516,7 +527,7 @@ struct Rope<2, true> {
   float m, diam;
   float n[2];
   float alpha;
-  uint32_t start, end, first, escape, pad;
+  uint32_t start, aux, open, escape, pad;
 };
 template <>
 struct Rope<3, true> {
@@ -524,7 +535,7 @@ struct Rope<3, true> {
   float m, diam;
   float n[3];
   float alpha;
-  uint32_t start, end, first, escape, pad[3];
+  uint32_t start, aux, open, escape, pad[3];
 };
 static_assert(sizeof(Rope<2, false>) == 32 && sizeof(Rope<3, false>) == 48 &&
                   sizeof(Rope<2, true>) == 48 && sizeof(Rope<3, true>) == 64,
@@ -550,7 +561,7 @@ template <int DIM, int LOG, bool FULL>
 __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                                                 const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
                                                 uint32_t num_nodes, uint32_t num_leaves,
-                                                Rope<DIM, FULL>* __restrict__ rope) {
+                                                float beta, Rope<DIM, FULL>* __restrict__ rope) {
   using Node = typename L3T<DIM, LOG>::Node;
   constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const uint32_t c = blockIdx.x * kB + threadIdx.x;
@@ -568,8 +579,18 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
   r.diam = nd->diameter;
   const uint64_t je = nd->end < num_leaves ? nd->end : num_leaves;
   const uint64_t js = nd->start < je ? nd->start : je;
-  r.start = static_cast<uint32_t>(js) | (nd->is_leaf ? kLeafBit : 0u);
-  r.end = static_cast<uint32_t>(je);
+  r.start = static_cast<uint32_t>(js) |
+            (nd->is_leaf ? (je - js == 1 ? kLeafBit : kLeafBit | kSumBit) : 0u);
+  if (nd->is_leaf) {
+    r.aux = static_cast<uint32_t>(je);
+  } else {
+    // the sure-far bound of the walk's far-field test
+#pragma clang fp contract(off)
+    const float t = beta * nd->diameter;
+    const float t2 = t * t;
+    const float hi = t2 * kMarginHi;
+    r.aux = __float_as_uint(beta > 0.0f && t2 >= FLT_MIN && hi <= FLT_MAX ? hi : NAN);
+  }
   if (nd->is_leaf && je - js == 1) {
     // the leaf's only point, as the leaf sum would read it
     const auto& L = leaf[js];
@@ -583,7 +604,8 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
     }
   }
   const int b = next_bit<DIM, LOG>(nd, 0);
-  r.first = (b < kN && nd->child[b] < num_nodes) ? static_cast<uint32_t>(nd->child[b]) : kNone;
+  const uint32_t first =
+      (b < kN && nd->child[b] < num_nodes) ? static_cast<uint32_t>(nd->child[b]) : kNone;
   // finishing node c: next sibling, else climb (the reference's FromChild
   // state) until a node with a next sibling; the root ends the walk
   uint32_t esc = kNone, cur = c;
@@ -597,6 +619,7 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
     cur = static_cast<uint32_t>(pi < num_nodes ? pi : 0);
   }
   r.escape = esc;
+  r.open = first != kNone ? first : esc;
   rope[c] = r;
 }
 
@@ -606,8 +629,6 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
 // expression and of (beta diameter)^2 move it by under 2^-21, so both give
 // the same answer there.  Everything else (a margin hit, beta <= 0, zero,
 // huge, infinite or NaN operands) takes the exact expression.
-constexpr float kMarginHi = 1.0f + 0x1p-14f;
-constexpr float kMarginLo = 1.0f - 0x1p-14f;
 
 template <int DIM, bool FULL>
 __device__ __forceinline__ Rope<DIM, FULL> rope_at(const char* __restrict__ base, uint32_t c) {
@@ -640,7 +661,6 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
   const char* __restrict__ base = reinterpret_cast<const char*>(rope);
   // a leaf's ratio is infinite: far for any beta but NaN
   const bool leaf_far = INFINITY >= A.beta;
-  const bool fast = A.beta > 0.0f;
   const uint32_t guard_max = num_nodes > 0x3ffffff0u ? 0xffffffffu : 4u * num_nodes + 64u;
   uint32_t c = 0;
   for (uint32_t guard = 0; guard < guard_max && c < num_nodes; ++guard) {
@@ -660,11 +680,11 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
         const float d = kScalarL3<KIND> ? qv[a] - r.c[a] : r.c[a] - qv[a];
         s = a == 0 ? d * d : s + d * d;
       }
-      const float t = A.beta * r.diam;
-      const float t2 = t * t;
-      const bool scaled = fast && t2 >= FLT_MIN && t2 <= FLT_MAX;
-      const bool sure_far = is_leaf ? leaf_far : scaled && s > t2 * kMarginHi;
-      const bool sure_near = is_leaf ? !leaf_far : scaled && s < t2 * kMarginLo;
+      // hi = (beta diam)^2 (1 + 2^-14), lo = hi (1 - 2^-13) ~ (beta diam)^2
+      // (1 - 2^-14); NaN (no sure answer) where the bound is not usable
+      const float hi = __uint_as_float(r.aux);
+      const bool sure_far = is_leaf ? leaf_far : s > hi;
+      const bool sure_near = is_leaf ? !leaf_far : s < hi * kMarginLo;
       far_mask = __ballot(sure_far) & members;
       // the exact ratio only for a vote that the sure answers leave open
       if (far_mask != members && (__ballot(sure_near) & members) == 0)
@@ -672,9 +692,8 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
     }
     const bool all = far_mask == members;
     if (all) {
-      const uint32_t s0 = r.start & ~kLeafBit;
-      if (is_leaf && r.end - s0 != 1u) {
-        l3_leaf_sum<KIND>(leaf, s0, r.end, qv, P, acc);
+      if (r.start & kSumBit) {
+        l3_leaf_sum<KIND>(leaf, r.start & kIndexMask, r.aux, qv, P, acc);
       } else {
         // an internal node's monopole, or a one-point leaf's point
         Leaf L = {};
@@ -689,7 +708,7 @@ __global__ void k_l3_walk_rope(const typename L3T<DIM, LOG>::Leaf* __restrict__ 
         l3_term<KIND>(L, qv, P, acc);
       }
     }
-    c = all ? r.escape : (r.first != kNone ? r.first : r.escape);
+    c = all ? r.escape : r.open;
   }
   store_out<DIM, KIND>(out, q, acc);
 }
@@ -869,15 +888,15 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
   const uint32_t num_leaves = static_cast<uint32_t>(byte_count(leaf_bytes) / sizeof(Leaf));
   const auto* nodes = static_cast<const Node*>(node_bytes->data);
   // the rope needs a workspace (not under stream capture: no allocation
-  // there) and record offsets within 32 bits
+  // there), record offsets within 32 bits and point indices within 30
   using R = Rope<DIM, kFullRope<KIND>>;
   R* rope = nullptr;
   if (num_nodes >= kRopeMin && static_cast<uint64_t>(num_nodes) * sizeof(R) <= (1ull << 32) &&
-      !stream_capturing(s))
+      num_leaves <= kIndexMask && !stream_capturing(s))
     rope = static_cast<R*>(workspace(s, kWsRope, static_cast<size_t>(num_nodes) * sizeof(R)));
   if (rope != nullptr) {
     k_l3_rope<DIM, LOG, kFullRope<KIND>><<<grid_for(num_nodes, kB), kB, 0, s>>>(
-        nodes, static_cast<const Leaf*>(leaf_bytes->data), num_nodes, num_leaves, rope);
+        nodes, static_cast<const Leaf*>(leaf_bytes->data), num_nodes, num_leaves, beta, rope);
     SBH_LAUNCH_CHECK();
     auto walk = A.P.periodic ? k_l3_walk_rope<DIM, LOG, KIND, true>
                              : k_l3_walk_rope<DIM, LOG, KIND, false>;
