@@ -290,3 +290,41 @@ def test_tree_builder_full_size_2d(gpu):
     assert np.array_equal(_np(leaf), ol)
     assert np.array_equal(_np(contrib), oc)
     assert np.array_equal(_np(node), on)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_rope_walk_far_test_edge_betas(gpu, dim):
+    # The rope walk decides most votes from a per-call bound (beta diam)^2
+    # (1 + 2^-14) and runs the exact ratio only near the threshold; beta <= 0,
+    # beta = +inf, NaN and extreme scales take the exact path.  The plain walk
+    # (run under stream capture) always evaluates the exact ratio: both must
+    # give identical bits for every beta, on a tree with coincident points
+    # (zero-diameter nodes) and a tight clump.
+    ffi = _ffi()
+    p, nv, m = _case(20000, dim, 81, "clump")
+    p[:300] = p[5000]
+    lo, hi = _corners(p, dim)
+    leaf, node, contrib = _build(ffi, dim, p, nv, m, lo, hi)
+    rng = np.random.default_rng(9)
+    Q = torch.tensor(np.concatenate([p[:1500], p[5000:5064],
+                                     rng.uniform(-1.5, 1.5, (500, dim))]),
+                     dtype=torch.float32, device="cuda")
+    betas = [2.0, 0.5, 0.0, -1.0, float("inf"), float("nan"), 1e-30, 1e30, 3e19]
+    eager = [_np(ffi.barnes_hut_force(leaf, node, Q, beta=b)) for b in betas]
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        ffi.barnes_hut_force(leaf, node, Q, beta=2.0)
+    torch.cuda.current_stream().wait_stream(side)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        outs = [ffi.barnes_hut_force(leaf, node, Q, beta=b) for b in betas]
+    for _ in range(2):
+        graph.replay()
+        torch.cuda.synchronize()
+        for b, o, e in zip(betas, outs, eager):
+            np.testing.assert_array_equal(_np(o), e, err_msg=f"beta={b}")
+    # beta = 1e30 opens every node (its bound overflows, so every vote takes
+    # the exact ratio): the sum over all leaves, finite
+    assert np.isfinite(eager[betas.index(1e30)]).all()
+    assert node.numel() // ffi.NODE_SIZES[(dim, 3)] >= 2048  # the rope path ran
