@@ -82,10 +82,13 @@ __device__ __forceinline__ void load_tile(float4* tile, const float* __restrict_
   tile[threadIdx.x] = v;
 }
 
-template <bool CUT>
+// PER: periodicity as a template argument (the non-periodic kernels carry no
+// minimum-image arithmetic; the sums are the same bits).
+template <bool CUT, bool PER>
 __global__ __launch_bounds__(kBlock) void k_direct_force(
     const float* __restrict__ pts, const float* __restrict__ ms, int n,
     const float* __restrict__ qs, int nq, float* __restrict__ out, PairParams P) {
+  P.periodic = PER ? 1 : 0;
   __shared__ float4 tile[kBlock];
   int i = blockIdx.x * kBlock + threadIdx.x;
   float qx = 0.f, qy = 0.f, qz = 0.f;
@@ -110,11 +113,12 @@ __global__ __launch_bounds__(kBlock) void k_direct_force(
   }
 }
 
-template <bool CUT>
+template <bool CUT, bool PER>
 __global__ __launch_bounds__(kBlock) void k_direct_vjp(
     const float* __restrict__ pts, const float* __restrict__ ms, int n,
     const float* __restrict__ qs, const float* __restrict__ cot, int nq,
     float* __restrict__ out, PairParams P) {
+  P.periodic = PER ? 1 : 0;
   __shared__ float4 tile[kBlock];
   int i = blockIdx.x * kBlock + threadIdx.x;
   float qx = 0.f, qy = 0.f, qz = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
@@ -150,11 +154,12 @@ __global__ __launch_bounds__(kBlock) void k_direct_vjp(
 // with d = minimage(x_k - x_j), lin = att s^3, K = 3 att s^5 + cutoff term.
 // The j == k pair is skipped: its two halves cancel in exact arithmetic but
 // not after FMA contraction, where the residual is of order m eps^-3 ulp.
-template <bool CUT>
+template <bool CUT, bool PER>
 __global__ __launch_bounds__(kBlock) void k_direct_full_vjp(
     const float* __restrict__ pts, const float* __restrict__ ms,
     const float* __restrict__ cot, int n, float* __restrict__ gx_out,
     float* __restrict__ gm_out, PairParams P) {
+  P.periodic = PER ? 1 : 0;
   __shared__ float4 tp[kBlock];
   __shared__ float4 tc[kBlock];
   int k = blockIdx.x * kBlock + threadIdx.x;
@@ -474,9 +479,9 @@ extern "C" int sbh_direct_force_soft(void* stream, const SbhBuffer* points,
   auto* pq = static_cast<const float*>(queries->data);
   auto* po = static_cast<float*>(out->data);
   if (cutoff_active(cutoff_scale))
-    k_direct_force<true><<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, (int)nq, po, P);
+    (P.periodic ? k_direct_force<true, true> : k_direct_force<true, false>)<<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, (int)nq, po, P);
   else
-    k_direct_force<false><<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, (int)nq, po, P);
+    (P.periodic ? k_direct_force<false, true> : k_direct_force<false, false>)<<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, (int)nq, po, P);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }
@@ -511,9 +516,9 @@ extern "C" int sbh_direct_force_soft_vjp(void* stream, const SbhBuffer* points,
   auto* pc = static_cast<const float*>(cotangent->data);
   auto* po = static_cast<float*>(out->data);
   if (cutoff_active(cutoff_scale))
-    k_direct_vjp<true><<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, pc, (int)nq, po, P);
+    (P.periodic ? k_direct_vjp<true, true> : k_direct_vjp<true, false>)<<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, pc, (int)nq, po, P);
   else
-    k_direct_vjp<false><<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, pc, (int)nq, po, P);
+    (P.periodic ? k_direct_vjp<false, true> : k_direct_vjp<false, false>)<<<grid_for(nq, kBlock), kBlock, 0, s>>>(pp, pm, (int)n, pq, pc, (int)nq, po, P);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }
@@ -548,9 +553,9 @@ extern "C" int sbh_direct_force_soft_full_vjp(void* stream, const SbhBuffer* poi
   auto* gx = static_cast<float*>(grad_points->data);
   auto* gm = static_cast<float*>(grad_masses->data);
   if (cutoff_active(cutoff_scale))
-    k_direct_full_vjp<true><<<grid_for(n, kBlock), kBlock, 0, s>>>(pp, pm, pc, (int)n, gx, gm, P);
+    (P.periodic ? k_direct_full_vjp<true, true> : k_direct_full_vjp<true, false>)<<<grid_for(n, kBlock), kBlock, 0, s>>>(pp, pm, pc, (int)n, gx, gm, P);
   else
-    k_direct_full_vjp<false><<<grid_for(n, kBlock), kBlock, 0, s>>>(pp, pm, pc, (int)n, gx, gm, P);
+    (P.periodic ? k_direct_full_vjp<false, true> : k_direct_full_vjp<false, false>)<<<grid_for(n, kBlock), kBlock, 0, s>>>(pp, pm, pc, (int)n, gx, gm, P);
   SBH_LAUNCH_CHECK();
   return SBH_OK;
 }

@@ -959,6 +959,8 @@ __global__ __launch_bounds__(kB) void k_tree_brute(const LeafT<DIM>* __restrict_
                                                    int64_t n, const float* __restrict__ queries,
                                                    int nq, float* __restrict__ out,
                                                    o8::WalkParams P) {
+  // the module's fields are never periodic: no minimum-image arithmetic
+  P.periodic = 0;
   __shared__ LeafT<DIM> tile[kB];
   const int q = blockIdx.x * kB + threadIdx.x;
   float qv[DIM] = {};
