@@ -238,7 +238,9 @@ constexpr int kWStack = 96;  // >= 7 * 10 + 1 entries for max_depth 10
 // bit-identical to the per-query walk, while every node and point record is
 // read once per wave through the scalar cache.  Queries run in Morton order
 // (order[t]) so that the lanes of a wave share most of their paths.
-template <int KIND>
+// PER: periodicity as a template argument (the non-periodic walk carries no
+// minimum-image arithmetic; same bits either way).
+template <int KIND, bool PER>
 __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
                                               const Node8* __restrict__ nodes,
                                               const float* __restrict__ queries,
@@ -246,6 +248,7 @@ __global__ __launch_bounds__(kB) void k_walk8(const Leaf3* __restrict__ leaf,
                                               const uint32_t* __restrict__ order,
                                               float* __restrict__ out, WalkParams P,
                                               uint32_t num_nodes, uint32_t num_leaves) {
+  P.periodic = PER ? 1 : 0;
   __shared__ uint32_t s_node[kB / 64][kWStack];
   __shared__ uint64_t s_lanes[kB / 64][kWStack];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -350,7 +353,7 @@ int launch_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node
     int rc = tree::query_order(s, static_cast<const float*>(queries->data), nq, ws, &order);
     if (rc != SBH_OK) return rc;
   }
-  k_walk8<KIND><<<grid_for(nq, kB), kB, 0, s>>>(
+  (P.periodic ? k_walk8<KIND, true> : k_walk8<KIND, false>)<<<grid_for(nq, kB), kB, 0, s>>>(
       static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node8*>(node_bytes->data),
       static_cast<const float*>(queries->data),
       cot ? static_cast<const float*>(cot->data) : nullptr, nq, order,

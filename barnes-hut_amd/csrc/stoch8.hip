@@ -21,7 +21,9 @@ using namespace o8;
 constexpr int kB = 128;
 constexpr int kControlDepth = 3;  // src/sbh_ffi.cu:2167
 
-template <int KIND>
+// PER: periodicity as a template argument (no minimum-image arithmetic in
+// the non-periodic walk; same bits either way).
+template <int KIND, bool PER>
 __global__ __launch_bounds__(kB) void k_stoch8(const Leaf3* __restrict__ leaf,
                                                const Node8* __restrict__ nodes,
                                                const float* __restrict__ queries,
@@ -30,6 +32,7 @@ __global__ __launch_bounds__(kB) void k_stoch8(const Leaf3* __restrict__ leaf,
                                                float* __restrict__ out, WalkParams P,
                                                int samples, uint32_t seed, uint32_t num_nodes,
                                                uint32_t num_leaves) {
+  P.periodic = PER ? 1 : 0;
   constexpr bool kSoft = (KIND == kSoftForce || KIND == kSoftVjp);
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (t >= nq) return;
@@ -159,7 +162,7 @@ int launch(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* node_byte
     int rc = tree::query_order(s, static_cast<const float*>(queries->data), nq, ws, &order);
     if (rc != SBH_OK) return rc;
   }
-  k_stoch8<KIND><<<grid_for(nq, kB), kB, 0, s>>>(
+  (P.periodic ? k_stoch8<KIND, true> : k_stoch8<KIND, false>)<<<grid_for(nq, kB), kB, 0, s>>>(
       static_cast<const Leaf3*>(leaf_bytes->data), static_cast<const Node8*>(node_bytes->data),
       static_cast<const float*>(queries->data),
       cot ? static_cast<const float*>(cot->data) : nullptr, nq, order,
