@@ -413,11 +413,14 @@ __device__ __forceinline__ void store_out(float* __restrict__ out, int q, const 
 // Launched with the caller's block size, every reference warp is one half
 // of a 64-wide wavefront here, and the vote is a ballot restricted to that
 // half.
-template <int DIM, int LOG, int KIND>
+// PER: periodicity as a template argument (no minimum-image arithmetic in
+// the non-periodic walk; same bits either way).
+template <int DIM, int LOG, int KIND, bool PER>
 __global__ void k_l3_walk(const typename L3T<DIM, LOG>::Leaf* __restrict__ leaf,
                           const typename L3T<DIM, LOG>::Node* __restrict__ nodes,
                           const float* __restrict__ queries, int nq, float* __restrict__ out,
                           L3Params A, uint32_t num_nodes, uint32_t num_leaves) {
+  A.P.periodic = PER ? 1 : 0;
   using Node = typename L3T<DIM, LOG>::Node;
   constexpr int kN = L3T<DIM, LOG>::kCellsN;
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -904,7 +907,8 @@ int launch_l3_walk(void* stream, const SbhBuffer* leaf_bytes, const SbhBuffer* n
                                     static_cast<const float*>(queries->data), nq,
                                     static_cast<float*>(out->data), A, num_nodes);
   } else {
-    k_l3_walk<DIM, LOG, KIND><<<blocks, threads, 0, s>>>(
+    auto plain = A.P.periodic ? k_l3_walk<DIM, LOG, KIND, true> : k_l3_walk<DIM, LOG, KIND, false>;
+    plain<<<blocks, threads, 0, s>>>(
         static_cast<const Leaf*>(leaf_bytes->data), nodes,
         static_cast<const float*>(queries->data), nq, static_cast<float*>(out->data), A,
         num_nodes, num_leaves);
