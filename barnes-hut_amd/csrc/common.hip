@@ -65,6 +65,18 @@ bool stream_capturing(hipStream_t stream) {
   return st != hipStreamCaptureStatusNone;
 }
 
+int cached_workspace(hipStream_t stream, int slot, size_t bytes, char** out) {
+  *out = nullptr;
+  // A graph would keep the cached pointer, and a later eager call that needs
+  // a larger buffer on the same stream handle frees it: replays would then
+  // write freed memory.  Captured calls therefore bring their own workspace.
+  if (stream_capturing(stream))
+    return fail_invalid("a workspace buffer must be passed when the stream is being captured");
+  *out = static_cast<char*>(workspace(stream, slot, bytes));
+  if (*out == nullptr) return fail_internal_msg("workspace allocation failed");
+  return SBH_OK;
+}
+
 }  // namespace sbh
 
 extern "C" {

@@ -128,6 +128,10 @@ inline bool buffer_ok(const SbhBuffer* b, int32_t dtype, int rank) {
 void* workspace(hipStream_t stream, int slot, size_t bytes);
 // True while `stream` is being captured into a graph (no allocation allowed).
 bool stream_capturing(hipStream_t stream);
+// workspace() for entry points whose caller passed no buffer: refuses
+// (SBH_INVALID_ARGUMENT) while `stream` is being captured, SBH_INTERNAL when
+// the allocation fails.
+int cached_workspace(hipStream_t stream, int slot, size_t bytes, char** out);
 constexpr int kWsBuild = 0;
 constexpr int kWsSort = 1;
 constexpr int kWsWalk = 2;

@@ -78,9 +78,7 @@ int resolve_workspace(hipStream_t s, SbhBuffer* workspace, const tree::Layout& L
     *ws = static_cast<char*>(workspace->data);
     return SBH_OK;
   }
-  *ws = static_cast<char*>(::sbh::workspace(s, kWsPipeline, L.total));
-  if (*ws == nullptr) return fail_internal_msg("workspace allocation failed");
-  return SBH_OK;
+  return cached_workspace(s, kWsPipeline, L.total, ws);
 }
 
 int check_stats(const SbhBuffer* stats) {
@@ -198,8 +196,9 @@ extern "C" void sbh_profile_enable(int32_t on) {
 }
 
 // out[3 k .. 3 k + 2] = {calls, build ms, walk-kernel ms} of kind k (0
-// gravity forward, 1 gravity adjoint, 2 winding numbers) summed over the
-// calls recorded since the last collect.  Waits for the recorded events.
+// gravity forward, 1 gravity adjoint, 2 winding numbers, 3 stochastic walks)
+// summed over the calls recorded since the last collect.  Waits for the
+// recorded events.
 extern "C" int sbh_profile_collect(double* out, int32_t count) {
   clear_error();
   std::vector<prof::Rec> recs;

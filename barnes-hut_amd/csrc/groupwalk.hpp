@@ -115,10 +115,14 @@ __device__ __forceinline__ Box wave_box(float x, float y, float z) {
 // non-empty nodes when flush_far(nfar) is called (at 64 entries and at the
 // end); near[0..nnear) holds rejected non-empty leaves when
 // flush_near(nnear) is called (at kNearFlush entries and at the end).  Both
-// callbacks run on the whole wave.
+// callbacks run on the whole wave.  A traversal whose stack would exceed
+// kStackCap stops early and sets bit 4 of *overflow (a header's overflow
+// word, reported by the walk's stats), so a truncated sum never passes
+// unnoticed.
 template <class FarFn, class NearFn>
-__device__ __forceinline__ void traverse(const Header* hdr, const int4* topo, const float4* geo,
-                                         const float4* mp, float beta, const Box& b, int lane,
+__device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const int4* topo,
+                                         const float4* geo, const float4* mp, float beta,
+                                         const Box& b, int lane,
                                          int root, int* stack, int* far, int* near,
                                          FarFn&& flush_far, NearFn&& flush_near) {
   const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
@@ -161,7 +165,10 @@ __device__ __forceinline__ void traverse(const Header* hdr, const int4* topo, co
     nnear += __popcll(bn);
     const int incl = wave_incl_scan(nch, lane);
     const int total = __shfl(incl, 63, 64);
-    if (top + total > kStackCap) break;  // unreachable for depth <= 10 (see kStackCap)
+    if (top + total > kStackCap) {  // unreachable for depth <= 10 (see kStackCap)
+      if (lane == 0) atomicOr(overflow, 4);
+      break;
+    }
     for (int c = 0; c < nch; ++c) stack[top + incl - nch + c] = fch + c;
     top += total;
     __builtin_amdgcn_wave_barrier();

@@ -592,7 +592,12 @@ __global__ __launch_bounds__(kB) void k_l3_rope(const typename L3T<DIM, LOG>::No
     const float t = beta * nd->diameter;
     const float t2 = t * t;
     const float hi = t2 * kMarginHi;
-    r.aux = __float_as_uint(beta > 0.0f && t2 >= FLT_MIN && hi <= FLT_MAX ? hi : NAN);
+    // a non-positive diameter (node bytes not written by this library's
+    // builders) has no usable bound: NaN sends every vote to the exact ratio
+    r.aux = __float_as_uint(beta > 0.0f && nd->diameter > 0.0f && t2 >= FLT_MIN &&
+                                    hi <= FLT_MAX
+                                ? hi
+                                : NAN);
   }
   if (nd->is_leaf && je - js == 1) {
     // the leaf's only point, as the leaf sum would read it
@@ -1087,8 +1092,8 @@ extern "C" int sbh_build_octree_level3_buffers(void* stream, const SbhBuffer* po
   const size_t o_sort = o; o = align(o + radix_sort_temp_bytes(n));
   const size_t o_offs = o; o = align(o + 4 * (kCells + 1));
   const size_t o_sums = o; o = align(o + 4 * 8 * kCells);
-  char* ws = static_cast<char*>(workspace(s, kWsBuild, o));
-  if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+  char* ws = nullptr;
+  if (int rcw = cached_workspace(s, kWsBuild, o, &ws); rcw != SBH_OK) return rcw;
   auto* keys = reinterpret_cast<uint32_t*>(ws + o_keys);
   auto* vals = reinterpret_cast<uint32_t*>(ws + o_vals);
   auto* cells = reinterpret_cast<uint32_t*>(ws + o_cells);

@@ -432,8 +432,8 @@ extern "C" int sbh_build_octree8_buffers(void* stream, const SbhBuffer* points,
   size_t level_bytes = 0;
   for (int l = 0; l <= max_depth; ++l) level_bytes += (size_t{1} << (3 * l)) * 32;
   size_t ws_bytes = off_levels + level_bytes;
-  char* ws = static_cast<char*>(workspace(s, kWsBuild, ws_bytes));
-  if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+  char* ws = nullptr;
+  if (int rcw = cached_workspace(s, kWsBuild, ws_bytes, &ws); rcw != SBH_OK) return rcw;
   auto* codes = reinterpret_cast<uint32_t*>(ws + off_codes);
   auto* idx = reinterpret_cast<uint32_t*>(ws + off_idx);
   auto* codes_sorted = reinterpret_cast<uint32_t*>(ws + off_codes2);

@@ -36,8 +36,16 @@ class Graph:
             call("sbh_graph_begin", h)
             try:
                 result = fn()
-            finally:
-                call("sbh_graph_end", h, ctypes.byref(self._exec))
+            except BaseException:
+                # end the capture and drop whatever was recorded; the error
+                # of `fn` is the one the caller sees
+                try:
+                    call("sbh_graph_end", h, ctypes.byref(self._exec))
+                except Exception:
+                    pass
+                self.close()
+                raise
+            call("sbh_graph_end", h, ctypes.byref(self._exec))
         torch.cuda.current_stream().wait_stream(side)
         return result
 

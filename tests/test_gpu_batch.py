@@ -76,6 +76,7 @@ def test_batch_matches_per_set_calls(gpu, sizes):
         scale = max(np.abs(ref_gm).max(), 1e-30)
         np.testing.assert_allclose(gm[sl], ref_gm, rtol=1e-4, atol=1e-5 * scale)
         s = st.cpu().numpy()
+        assert s[5] == 0, f"set {k}: overflow or truncated walk"
         tot += s[[0, 1, 2, 3]]
     s = sb.cpu().numpy()
     nonempty = sum(1 for n in sizes if n > 0)
@@ -84,6 +85,7 @@ def test_batch_matches_per_set_calls(gpu, sizes):
     # nodes: the per-set trees plus one empty root per empty set
     assert s[3] == tot[3] + (len(sizes) - nonempty)
     np.testing.assert_array_equal(sv.cpu().numpy()[:4], s[:4])
+    assert sv.cpu().numpy()[5] == 0
 
 
 def test_batch_rejects_bad_offsets(gpu):

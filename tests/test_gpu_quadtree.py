@@ -328,3 +328,21 @@ def test_rope_walk_far_test_edge_betas(gpu, dim):
     # the exact ratio): the sum over all leaves, finite
     assert np.isfinite(eager[betas.index(1e30)]).all()
     assert node.numel() // ffi.NODE_SIZES[(dim, 3)] >= 2048  # the rope path ran
+    # Node bytes that no builder of this library writes: internal nodes with
+    # negated diameters.  The exact ratio is then negative (the node opens);
+    # the rope has no usable sure-far bound for such a node and must agree.
+    size = ffi.NODE_SIZES[(dim, 3)]
+    off_d, off_leaf = (48, 612) if dim == 2 else (56, 4268)
+    nb = node.clone().view(-1, size)
+    internal = torch.nonzero(nb[:, off_leaf] == 0).flatten()[1::3]
+    assert internal.numel() > 10
+    d = nb[internal, off_d:off_d + 4].contiguous().view(torch.float32)
+    nb[internal, off_d:off_d + 4] = (-d).contiguous().view(torch.uint8).view(-1, 4)
+    neg = nb.view(-1)
+    e_neg = _np(ffi.barnes_hut_force(leaf, neg, Q, beta=2.0))
+    graph2 = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph2):
+        o_neg = ffi.barnes_hut_force(leaf, neg, Q, beta=2.0)
+    graph2.replay()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(_np(o_neg), e_neg)
