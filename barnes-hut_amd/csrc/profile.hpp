@@ -1,7 +1,7 @@
 // Optional phase timing of the adaptive-tree pipelines: HIP events recorded
 // on the call's stream around the tree build(s) and around the walk kernel,
 // read back by sbh_profile_collect().  Kinds: 0 gravity forward (single set
-// or batch), 1 gravity adjoint, 2 winding numbers.
+// or batch), 1 gravity adjoint, 2 winding numbers, 3 stochastic walks.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,7 +9,7 @@
 namespace sbh {
 namespace prof {
 
-constexpr int kKinds = 3;
+constexpr int kKinds = 4;
 
 struct Rec {
   hipEvent_t ev[4];  // build start, build end, walk kernel start, walk kernel end

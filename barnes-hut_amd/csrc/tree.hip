@@ -999,7 +999,8 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   };
 
   const int root = A.nsets > 1 ? set_of(A.offs, A.nsets, base) : 0;
-  traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, root, stack, far, near, flush_far,
+  traverse(A.hdr, &A.hdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, root, stack, far,
+           near, flush_far,
            flush_near);
 
   if (valid) {

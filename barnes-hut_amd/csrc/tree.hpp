@@ -32,6 +32,10 @@ struct Header {
 
 constexpr int kGroupMax = 64;     // one wavefront per query group
 constexpr int kGroupCell = 2048;  // groups are runs inside cells of at most this many points
+// Query trees (queries walked against a separate source tree): topology and
+// groups only, depth 10, leaves of at most one group.
+constexpr int kQueryTreeDepth = 10;
+constexpr int kQueryTreeLeaf = kGroupMax;
 
 // Byte offsets of every array inside the workspace.  Node arrays hold `cap`
 // entries (an upper bound on the node count, see node_capacity()).  A
@@ -113,6 +117,13 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
 
 // Copies {far, near, groups, nodes, levels, overflow, 0, 0} into a device int64[8].
 int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out);
+
+// Per-node dipoles D = sum a n of an element tree built by build() over
+// element centroids with the areas as masses and the unit normals as the
+// gathered per-point vector (sc): bottom-up, one level per launch; also
+// clears the sentinel node's entry.  wd holds L.cap + 1 float4.
+int wn_dipoles(hipStream_t s, char* ws, const Layout& L, float4* wd, int64_t n, int max_depth,
+               int leaf_size);
 
 // Cell geometry from a level-l prefix (same expression as the octree8 node
 // centres, src/sbh_ffi.cu:1828-1834): {cx, cy, cz, diagonal}.

@@ -29,8 +29,8 @@ namespace {
 using namespace gw;
 
 constexpr float kInvFourPi = 0.07957747154594767f;
-constexpr int kQueryDepth = 10;
-constexpr int kQueryLeaf = kGroupMax;
+constexpr int kQueryDepth = kQueryTreeDepth;
+constexpr int kQueryLeaf = kQueryTreeLeaf;
 
 struct WnLayout {
   Layout t;      // element tree, at the workspace base
@@ -200,7 +200,8 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
         for (int j0 = s0; j0 < s1; j0 += kChunk) w += wn_near_batch(A, q.x, q.y, q.z, j0, s1);
       }
     };
-    traverse(A.hdr, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack, far, near, flush_far,
+    traverse(A.hdr, &A.qhdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack,
+             far, near, flush_far,
              flush_near);
     if (valid) A.out[A.qperm[i]] = w;
     ++n_groups;
@@ -223,7 +224,7 @@ __global__ void k_wn_copy_stats(const Header* __restrict__ hdr, const Header* __
   for (int l = 0; l < kMaxLevels; ++l)
     if (hdr->level_count[l] > 0) levels = l + 1;
   out[4] = levels;
-  out[5] = hdr->overflow;
+  out[5] = hdr->overflow | qhdr->overflow;
   out[6] = 0;
   out[7] = 0;
 }
@@ -251,6 +252,25 @@ int check_wn(const SbhBuffer* points, const SbhBuffer* normals, const SbhBuffer*
 }
 
 }  // namespace
+
+int wn_dipoles(hipStream_t s, char* ws, const Layout& L, float4* wd, int64_t n, int max_depth,
+               int leaf_size) {
+  auto* hdr = reinterpret_cast<Header*>(ws + L.header);
+  const auto* topo = reinterpret_cast<const int4*>(ws + L.topo);
+  const auto* sp = reinterpret_cast<const float4*>(ws + L.sp);
+  const auto* sn = reinterpret_cast<const float4*>(ws + L.sc);
+  const int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+  for (int l = max_depth; l >= 0; --l) {
+    const int64_t lvl_cap = std::min<int64_t>(
+        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
+        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
+    k_wn_moments<<<grid_for(lvl_cap, gw::kB, 4096), gw::kB, 0, s>>>(hdr, l, topo, sp, sn, wd,
+                                                                    static_cast<int>(L.cap));
+    SBH_LAUNCH_CHECK();
+  }
+  return SBH_OK;
+}
+
 }  // namespace tree
 }  // namespace sbh
 
@@ -284,8 +304,7 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
       return fail_invalid("workspace too small");
     ws = static_cast<char*>(workspace->data);
   } else {
-    ws = static_cast<char*>(::sbh::workspace(s, kWsPipeline, W.total));
-    if (ws == nullptr) return fail_internal_msg("workspace allocation failed");
+    if ((rc = cached_workspace(s, kWsPipeline, W.total, &ws)) != SBH_OK) return rc;
   }
   const auto* pts = static_cast<const float*>(points->data);
   const auto* q = static_cast<const float*>(queries->data);
@@ -302,15 +321,7 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
   const auto* mp = reinterpret_cast<const float4*>(ws + W.t.mp);
   const auto* sp = reinterpret_cast<const float4*>(ws + W.t.sp);
   const auto* sn = reinterpret_cast<const float4*>(ws + W.t.sc);
-  const int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
-  for (int l = max_depth; l >= 0; --l) {
-    const int64_t lvl_cap = std::min<int64_t>(
-        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
-        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
-    tree::k_wn_moments<<<grid_for(lvl_cap, tree::gw::kB, 4096), tree::gw::kB, 0, s>>>(
-        hdr, l, topo, sp, sn, wd, static_cast<int>(W.t.cap));
-    SBH_LAUNCH_CHECK();
-  }
+  if ((rc = tree::wn_dipoles(s, ws, W.t, wd, n, max_depth, leaf_size)) != SBH_OK) return rc;
   // Query tree (topology and groups only).
   char* qws = ws + W.qbase;
   rc = tree::build(s, qws, W.qt, q, nullptr, nullptr, m, tree::kQueryDepth, tree::kQueryLeaf,

@@ -14,5 +14,6 @@ from . import synthetic  # noqa: E402,F401
 from . import wn  # noqa: E402,F401
 from . import fields  # noqa: E402,F401
 from . import jax_sbh  # noqa: E402,F401
+from . import stochastic  # noqa: E402,F401
 
-__all__ += ["gravity", "graph", "synthetic", "wn", "fields", "jax_sbh"]
+__all__ += ["gravity", "graph", "synthetic", "wn", "fields", "jax_sbh", "stochastic"]

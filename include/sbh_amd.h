@@ -550,6 +550,55 @@ int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points,
                                 int32_t walk_blocks, SbhBuffer* workspace,
                                 SbhBuffer* stats);
 
+/* ---- stochastic Barnes-Hut on the adaptive octree (new targets) --------
+ *
+ * The multi-level prefix control-variate estimator of
+ * stochastic_bh_force_octree8_kernel / its _vjp_kernel (src/sbh_ffi.cu:
+ * 2146-2291, 2293-2442; python/jax_sbh_ffi.py:1450-1535 wraps them) and of
+ * multi_level_prefix_control_variate_wn (src/sbh_gpu.cu:166-187), on the
+ * adaptive tree instead of the dense octree8 records (infeasible at
+ * max_depth 10).  Subdomains are the nodes at level 3 and the leaves above
+ * it; each contributes its node term (or exact leaf sum), and each of
+ * samples_per_subdomain samples walks from it towards a uniformly drawn
+ * source with Russian roulette (continue with probability
+ * min(beta_cur / ratio, 1), ratio = |q - cell centre| / cell diagonal),
+ * adding n_sub / (P(path) count(node)) (sum of the node's children - node).
+ * Queries are separate from the sources (as in the reference targets); they
+ * are grouped by their own Morton tree (groups of <= 64), and the queries of
+ * a group share the random draws (Philox4x32-10 keyed by seed, the group,
+ * the subdomain, the sample and the step), each keeping its own roulette
+ * probabilities.  E[estimate] = the exact sum.
+ * Gravity: softened force (eps = softening_scale, no cutoff, not periodic);
+ * multipole_order 1 (the reference's monopole node term) or 2 (+ quadrupole).
+ * The VJP is the reference's: J^T c with respect to the query positions along
+ * the same sample paths (cotangent (M, 3) -> out (M, 3)).
+ * Winding numbers: node term = dipole sum(a n) at the area-weighted centroid,
+ * leaves summed exactly (signedSolidAngle<3>, include/kernel.h:73-80).
+ * workspace: u8 buffer of sbh_stochastic_workspace_bytes() bytes, or NULL.
+ * stats: NULL or int64[8] {node terms, source pairs (both per query, summed),
+ * query groups, source-tree nodes, levels, overflow, subdomains, 0}. */
+int64_t sbh_stochastic_workspace_bytes(int64_t num_sources, int64_t num_queries,
+                                       int32_t max_depth, int32_t leaf_size);
+int sbh_stochastic_gravity_force(void* stream, const SbhBuffer* points,
+                                 const SbhBuffer* masses, const SbhBuffer* queries,
+                                 SbhBuffer* out, float beta, float softening_scale,
+                                 int32_t max_depth, int32_t leaf_size,
+                                 int32_t multipole_order, int32_t samples_per_subdomain,
+                                 int32_t seed, SbhBuffer* workspace, SbhBuffer* stats);
+int sbh_stochastic_gravity_force_vjp(void* stream, const SbhBuffer* points,
+                                     const SbhBuffer* masses, const SbhBuffer* queries,
+                                     const SbhBuffer* cotangent, SbhBuffer* out, float beta,
+                                     float softening_scale, int32_t max_depth,
+                                     int32_t leaf_size, int32_t multipole_order,
+                                     int32_t samples_per_subdomain, int32_t seed,
+                                     SbhBuffer* workspace, SbhBuffer* stats);
+int sbh_stochastic_winding_number(void* stream, const SbhBuffer* points,
+                                  const SbhBuffer* normals, const SbhBuffer* areas,
+                                  const SbhBuffer* queries, SbhBuffer* out, float beta,
+                                  int32_t max_depth, int32_t leaf_size,
+                                  int32_t samples_per_subdomain, int32_t seed,
+                                  SbhBuffer* workspace, SbhBuffer* stats);
+
 /* HIP graph capture of library calls on `stream` (no reference counterpart;
  * XLA captures its own graphs).  Between sbh_graph_begin and sbh_graph_end
  * only calls with caller-provided workspaces may be made.  sbh_graph_end
@@ -559,14 +608,14 @@ int sbh_graph_end(void* stream, void** graph_exec);
 int sbh_graph_launch(void* graph_exec, void* stream);
 int sbh_graph_destroy(void* graph_exec);
 
-/* Phase timing for the sbh_gravity_* (single set and batch) and
- * sbh_winding_number targets (no reference counterpart; the reference times
- * from Python around dispatch).  When enabled, each call records HIP events
- * on its stream around the tree build(s) and around the walk kernel.
- * sbh_profile_collect waits for the recorded events and writes up to 9
- * values, {calls, build ms, walk ms} for gravity forward, gravity adjoint and
- * winding numbers in turn, summed since the previous collect.  Not for use
- * while a graph is being captured. */
+/* Phase timing for the sbh_gravity_* (single set and batch),
+ * sbh_winding_number and sbh_stochastic_* targets (no reference counterpart;
+ * the reference times from Python around dispatch).  When enabled, each call
+ * records HIP events on its stream around the tree build(s) and around the
+ * walk kernel.  sbh_profile_collect waits for the recorded events and writes
+ * up to 12 values, {calls, build ms, walk ms} for gravity forward, gravity
+ * adjoint, winding numbers and the stochastic walks in turn, summed since
+ * the previous collect.  Not for use while a graph is being captured. */
 void sbh_profile_enable(int32_t on);
 int sbh_profile_collect(double* out, int32_t count);
 

@@ -155,6 +155,13 @@ def _declare(L: ctypes.CDLL) -> None:
     L.orc_wn_eval.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p, _f64p,
                               _f64p, _f64p, _f64p, _f32p, _f64p, _i64, _i64, _i32p, _i32p,
                               ctypes.c_float, _f64p, ctypes.POINTER(_i64)]
+    L.orc_tree_subdomains.restype = _i64
+    L.orc_tree_subdomains.argtypes = [_i64, _i32p, _i32p, _f64p, _i32p]
+    L.orc_tree_stoch.argtypes = [_i64, _i32p, _i32p, _i32p, _i32p, _i32p, _f32p, _f64p, _f64p,
+                                 _f64p, _f64p, _f64p, _f64p, _f64p, _f32p, _i64, _i64, _i32p,
+                                 _i32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_uint32, _vp, _f64p,
+                                 ctypes.POINTER(_i64)]
 
 
 def _c(a, dtype):
@@ -730,6 +737,75 @@ class Tree:
         res = np.zeros(nq, np.float64)
         res[QT.perm] = out
         return dict(wn=res, far=int(counts[0]), near=int(counts[1]), groups=int(ng))
+
+
+    # stochastic walk kinds (orc_tree_stoch)
+    STOCH_FORCE, STOCH_VJP, STOCH_WN = 0, 1, 2
+
+    def subdomains(self, masses_or_areas=None):
+        """Control subdomains of the stochastic walk: nodes at level 3 and
+        leaves above it with non-zero mass, in node order."""
+        mass = self.moments()[0] if masses_or_areas is None else masses_or_areas
+        subs = np.zeros(max(self.num_nodes, 1), np.int32)
+        ns = lib().orc_tree_subdomains(self.num_nodes, self.nchild, self.level,
+                                       _c(mass, np.float64), subs)
+        return subs[:ns].copy()
+
+    def query_groups(self, queries, group_size=64):
+        """The query tree of a stochastic or winding-number walk: (tree,
+        group starts, group counts) over the queries' own Morton order."""
+        q = _c(queries, np.float32)
+        QT = Tree(q, np.zeros(q.shape[0], np.float32), max_depth=10, leaf_size=group_size)
+        gs, gc = QT.groups(group_size)
+        return QT, gs, gc
+
+    def stoch_eval_sorted(self, qs, gs, gc, *, kind=0, beta=2.0, eps=0.0, samples=1, seed=0,
+                          use_quad=False, cot_sorted=None, normals=None):
+        """orc_tree_stoch over queries already in group order (qs float32,
+        sorted; groups gs / gc; cotangents in the same order).  Returns
+        dict(out (sorted order), node, pair)."""
+        qs = _c(qs, np.float32)
+        nq = qs.shape[0]
+        nn = self.num_nodes
+        if kind == self.STOCH_WN:
+            ns = _c(normals, np.float64)[self.perm]
+            mass, com, dip = self.wn_moments(ns)
+            quad = np.zeros(6 * max(nn, 1), np.float64)
+        else:
+            mass, com, quad = self.moments()
+            dip = np.zeros(3, np.float64)
+            ns = np.zeros(3, np.float64)
+        width = 1 if kind == self.STOCH_WN else 3
+        out = np.zeros(max(nq, 1) * width, np.float64)
+        cs = None if cot_sorted is None else _c(cot_sorted, np.float64)
+        counts = (_i64 * 2)()
+        lib().orc_tree_stoch(nn, self.start, self.count, self.first_child, self.nchild,
+                             self.level, _c(self.geo, np.float32).ravel(), _c(mass, np.float64),
+                             _c(com, np.float64).ravel(), _c(quad, np.float64).ravel(),
+                             _c(dip, np.float64).ravel(), self.points_sorted.astype(np.float64),
+                             self.masses_sorted.astype(np.float64), _c(ns, np.float64).ravel(),
+                             qs.reshape(-1) if nq else np.zeros(3, np.float32), nq, len(gs),
+                             _c(gs, np.int32), _c(gc, np.int32), int(kind), int(bool(use_quad)),
+                             float(beta), float(eps), int(samples), int(seed) & 0xFFFFFFFF,
+                             _ptr(cs), out, counts)
+        out = out[:nq * width].reshape(nq, width) if width == 3 else out[:nq]
+        return dict(out=out, node=int(counts[0]), pair=int(counts[1]))
+
+    def stoch_eval(self, queries, *, kind=0, beta=2.0, eps=0.0, samples=1, seed=0,
+                   use_quad=False, cot=None, normals=None, group_size=64):
+        """Stochastic BH estimate at `queries` (input order): forces (kind 0),
+        their query-side VJP with `cot` (kind 1) or winding numbers (kind 2,
+        this tree's points are element centroids, its masses the areas,
+        `normals` the unit normals in input order).  The queries get a tree
+        of their own (depth 10, leaf group_size) whose groups share draws."""
+        QT, gs, gc = self.query_groups(queries, group_size)
+        cs = None if cot is None else _c(cot, np.float64)[QT.perm]
+        r = self.stoch_eval_sorted(QT.points_sorted, gs, gc, kind=kind, beta=beta, eps=eps,
+                                   samples=samples, seed=seed, use_quad=use_quad,
+                                   cot_sorted=cs, normals=normals)
+        res = np.zeros_like(r["out"])
+        res[QT.perm] = r["out"]
+        return dict(out=res, node=r["node"], pair=r["pair"], groups=len(gs))
 
 
 # ---------------------------------------------------------------- Philox

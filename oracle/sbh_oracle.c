@@ -28,6 +28,11 @@
  *   orc_tree_*                            topology invariants, beta->inf == direct,
  *                                         quadrupole error order, FD of the adjoint
  *   orc_philox4x32                        Random123 known-answer vectors
+ *   orc_tree_stoch                        zero samples == sum of subdomain terms;
+ *                                         every path forced to the leaves (beta
+ *                                         huge) == exact sum; seed mean == exact
+ *                                         sum (z-test); huge beta VJP == FD of
+ *                                         the forward on the same paths
  */
 #include <math.h>
 #include <stdint.h>
@@ -1603,6 +1608,217 @@ ORC_EXPORT void orc_wn_eval(int64_t nn, const int32_t* start, const int32_t* cou
   if (counts) {
     counts[0] = nfar;
     counts[1] = nnear;
+  }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Stochastic BH over the adaptive tree: the multi-level prefix control        */
+/* variate of src/sbh_ffi.cu:2146-2442 (forward 2146-2291, query-side VJP      */
+/* 2293-2442) on the adaptive octree's nodes, with query groups sharing draws. */
+/* ------------------------------------------------------------------------- */
+
+/* Node term of one query: the node's expansion at its centre of mass
+ * (gravity: monopole, + quadrupole when use_quad; kind 1 its query-side VJP)
+ * or its dipole at the area-weighted centroid (kind 2, winding number). */
+static void orc_stoch_node(int kind, const double* q, const double* c, int64_t k,
+                           const double* mass, const double* com, const double* quad,
+                           const double* dip, int use_quad, double e2, double* acc) {
+  if (kind == 2) {
+    acc[0] += orc_wn_far(q, com + 3 * k, dip + 3 * k);
+    return;
+  }
+  double F[3] = {0, 0, 0}, G[3] = {0, 0, 0}, aX[3] = {0, 0, 0}, aM = 0.0;
+  double aS[6] = {0, 0, 0, 0, 0, 0};
+  orc_far(q, mass[k], com + 3 * k, quad + 6 * k, use_quad, e2, kind == 1 ? c : NULL, F, G, aX,
+          &aM, aS);
+  for (int a = 0; a < 3; a++) acc[a] += kind == 1 ? G[a] : F[a];
+}
+
+/* Exact sum of one query over the sorted sources [j0, j1). */
+static void orc_stoch_leaf(int kind, const double* q, const double* c, int64_t j0, int64_t j1,
+                           const double* xs, const double* ms, const double* nrm, double e2,
+                           double* acc) {
+  for (int64_t j = j0; j < j1; j++) {
+    const double* p = xs + 3 * j;
+    if (kind == 2) {
+      const double* nv = nrm + 3 * j;
+      double d[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+      double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      acc[0] += ms[j] * (nv[0] * d[0] + nv[1] * d[1] + nv[2] * d[2]) /
+                (4.0 * M_PI * (r * r * r + 1e-5));
+      continue;
+    }
+    double d[3] = {q[0] - p[0], q[1] - p[1], q[2] - p[2]};
+    double r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    double s = 1.0 / sqrt(r2 + e2);
+    double s3 = s * s * s, s5 = s3 * s * s;
+    if (kind == 0) {
+      for (int a = 0; a < 3; a++) acc[a] += -ms[j] * s3 * d[a];
+    } else {
+      double cd = c[0] * d[0] + c[1] * d[1] + c[2] * d[2];
+      for (int a = 0; a < 3; a++) acc[a] += ms[j] * (3.0 * s5 * cd * d[a] - s3 * c[a]);
+    }
+  }
+}
+
+/* Children sum minus the node's own term, and the terms evaluated for it:
+ * nodes[0] node terms, nodes[1] source pairs (per query). */
+static void orc_stoch_diff(int kind, const double* q, const double* c, int32_t prev,
+                           const int32_t* start, const int32_t* count,
+                           const int32_t* first_child, const int32_t* nchild,
+                           const double* mass, const double* com, const double* quad,
+                           const double* dip, const double* xs, const double* ms,
+                           const double* nrm, int use_quad, double e2, double* diff,
+                           int64_t* terms) {
+  double ch[3] = {0, 0, 0}, pr[3] = {0, 0, 0};
+  for (int32_t cc = 0; cc < nchild[prev]; cc++) {
+    int32_t k = first_child[prev] + cc;
+    if (nchild[k] == 0) {
+      orc_stoch_leaf(kind, q, c, start[k], start[k] + count[k], xs, ms, nrm, e2, ch);
+      terms[1] += count[k];
+    } else {
+      orc_stoch_node(kind, q, c, k, mass, com, quad, dip, use_quad, e2, ch);
+      terms[0] += 1;
+    }
+  }
+  orc_stoch_node(kind, q, c, prev, mass, com, quad, dip, use_quad, e2, pr);
+  terms[0] += 1;
+  for (int a = 0; a < 3; a++) diff[a] = ch[a] - pr[a];
+}
+
+/* Ratio |q - cell centre| / cell diagonal of node k, IEEE fp32. */
+static inline float orc_stoch_ratio(const float* q, const float* g) {
+  float dx = q[0] - g[0], dy = q[1] - g[1], dz = q[2] - g[2];
+  return sqrtf(dx * dx + dy * dy + dz * dz) / g[3];
+}
+
+/* Subdomains: nodes at level 3 and leaves above it (kControlDepth = 3,
+ * src/sbh_ffi.cu:2167-2192) with non-zero mass, in node order; returns their
+ * count (at most nn) in subs. */
+ORC_EXPORT int64_t orc_tree_subdomains(int64_t nn, const int32_t* nchild, const int32_t* level,
+                                       const double* mass, int32_t* subs) {
+  int64_t ns = 0;
+  for (int64_t k = 0; k < nn; k++)
+    if ((level[k] == 3 || (level[k] < 3 && nchild[k] == 0)) && mass[k] != 0.0)
+      subs[ns++] = (int32_t)k;
+  return ns;
+}
+
+/* Estimator per query q of a group (sorted queries g_start[g] + 0..g_count-1):
+ *   sum over subdomains of [its node term, or its exact leaf sum]
+ *   + (1/S) sum over S samples of sum over path steps of
+ *       n_sub / (P * count(prev)) * (sum of children(prev) - term(prev)),
+ * where the path starts at the subdomain with beta_cur = max(beta, ratio),
+ * continues at a node with probability t = min(beta_cur / ratio, 1) (ratio =
+ * |q - cell centre| / cell diagonal; stop when the draw exceeds t), P is the
+ * product of the continuation probabilities so far, the next node is the
+ * child holding the sampled source, and beta_cur = max(beta_cur, ratio).
+ * The group shares every draw: the sampled source (uniform over the
+ * subdomain's sorted range) and the roulette draws are
+ * sbh_uniform(seed, g_start[g], subdomain node, sample, step 0 / 1, 2, ...);
+ * each query keeps its own t and P.  Leaves are summed exactly (children
+ * that are leaves too).  kind 0: softened force (out nq x 3), 1: its
+ * query-side VJP with cot (sorted, nq x 3), 2: winding number (out nq; mass =
+ * areas, com = centroids, dip = sum a n per node, nrm = unit normals).
+ * counts[0] / counts[1]: node terms / source pairs evaluated by a group for
+ * every one of its queries (a step is evaluated while any query of the group
+ * continues), summed over the groups and multiplied by the group sizes. */
+ORC_EXPORT void orc_tree_stoch(int64_t nn, const int32_t* start, const int32_t* count,
+                               const int32_t* first_child, const int32_t* nchild,
+                               const int32_t* level, const float* geo, const double* mass,
+                               const double* com, const double* quad, const double* dip,
+                               const double* xs, const double* ms, const double* nrm,
+                               const float* qf, int64_t nq, int64_t ng, const int32_t* g_start,
+                               const int32_t* g_count, int kind, int use_quad, float beta,
+                               double eps, int samples, uint32_t seed, const double* cot,
+                               double* out, int64_t* counts) {
+  const double e2 = (eps > 0 ? eps * eps : 0.0) + 1e-12;
+  const int width = kind == 2 ? 1 : 3;
+  int32_t* subs = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nn + 1));
+  const int64_t nsub = orc_tree_subdomains(nn, nchild, level, mass, subs);
+  memset(out, 0, sizeof(double) * (size_t)nq * (size_t)width);
+  int64_t c_node = 0, c_pair = 0;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : c_node, c_pair)
+  for (int64_t g = 0; g < ng; g++) {
+    const int64_t i0 = g_start[g], i1 = g_start[g] + g_count[g];
+    const uint32_t key = (uint32_t)g_start[g];
+    int64_t wave_node = 0, wave_pair = 0; /* terms the group evaluates */
+    for (int64_t sd = 0; sd < nsub; sd++) {
+      const int32_t k = subs[sd];
+      const int leaf = nchild[k] == 0;
+      if (leaf)
+        wave_pair += count[k];
+      else
+        wave_node += 1;
+      for (int64_t i = i0; i < i1 && i < nq; i++) {
+        double qd[3] = {qf[3 * i], qf[3 * i + 1], qf[3 * i + 2]};
+        const double* c = cot ? cot + 3 * i : NULL;
+        double acc[3] = {0, 0, 0};
+        if (leaf)
+          orc_stoch_leaf(kind, qd, c, start[k], start[k] + count[k], xs, ms, nrm, e2, acc);
+        else
+          orc_stoch_node(kind, qd, c, k, mass, com, quad, dip, use_quad, e2, acc);
+        for (int a = 0; a < width; a++) out[width * i + a] += acc[a];
+      }
+      if (leaf || samples <= 0) continue;
+      const uint32_t n_sub = (uint32_t)count[k];
+      for (int s = 0; s < samples; s++) {
+        const float u0 = orc_sbh_uniform(seed, key, (uint32_t)k, (uint32_t)s, 0);
+        uint32_t sidx = (uint32_t)((float)n_sub * u0) + (uint32_t)start[k];
+        if (sidx > (uint32_t)(start[k] + count[k] - 1)) sidx = (uint32_t)(start[k] + count[k] - 1);
+        int64_t steps_max = 0;
+        int64_t step_node[64], step_pair[64];
+        for (int64_t i = i0; i < i1 && i < nq; i++) {
+          const float* qi = qf + 3 * i;
+          double qd[3] = {qi[0], qi[1], qi[2]};
+          const double* c = cot ? cot + 3 * i : NULL;
+          double wsum[3] = {0, 0, 0};
+          int32_t cur = k;
+          float ep = 1.0f;
+          float cur_beta = fmaxf(beta, orc_stoch_ratio(qi, geo + 4 * (int64_t)cur));
+          uint32_t step = 1;
+          int64_t taken = 0;
+          while (nchild[cur] > 0 && step < 64) {
+            const float ratio = orc_stoch_ratio(qi, geo + 4 * (int64_t)cur);
+            const float tp = fminf(cur_beta / ratio, 1.0f);
+            const float pu = orc_sbh_uniform(seed, key, (uint32_t)k, (uint32_t)s, step++);
+            if (pu > tp) break;
+            ep *= tp;
+            const int32_t prev = cur;
+            int32_t nxt = first_child[prev];
+            for (int32_t cc = 1; cc < nchild[prev]; cc++)
+              if ((uint32_t)start[first_child[prev] + cc] <= sidx) nxt = first_child[prev] + cc;
+            cur = nxt;
+            cur_beta = fmaxf(cur_beta, ratio);
+            double diff[3];
+            int64_t terms[2] = {0, 0};
+            orc_stoch_diff(kind, qd, c, prev, start, count, first_child, nchild, mass, com, quad,
+                           dip, xs, ms, nrm, use_quad, e2, diff, terms);
+            step_node[taken] = terms[0];
+            step_pair[taken] = terms[1];
+            const float w = (float)n_sub / (ep * (float)count[prev]);
+            for (int a = 0; a < width; a++) wsum[a] += (double)w * diff[a];
+            taken++;
+          }
+          if (taken > steps_max) {
+            for (int64_t t = steps_max; t < taken; t++) {
+              wave_node += step_node[t];
+              wave_pair += step_pair[t];
+            }
+            steps_max = taken;
+          }
+          for (int a = 0; a < width; a++) out[width * i + a] += wsum[a] / (double)samples;
+        }
+      }
+    }
+    const int64_t nv = (i1 < nq ? i1 : nq) - i0;
+    c_node += wave_node * nv;
+    c_pair += wave_pair * nv;
+  }
+  free(subs);
+  if (counts) {
+    counts[0] = c_node;
+    counts[1] = c_pair;
   }
 }
 

@@ -240,3 +240,39 @@ def test_tree_module_entries_reject():
     rc, err = _call("sbh_tree_bruteforce", None, _i(2), _i(0), ctypes.byref(leaf),
                     ctypes.byref(_buf(F32, 4, 3, ptr=FAKE)), ctypes.byref(out))
     assert rc == 3 and "queries dim must be 2" in err
+
+
+def test_stochastic_adaptive_entries_reject():
+    # argument checks of the sbh_stochastic_* targets run before any HIP call
+    p, m = _buf(F32, 5, 3, ptr=FAKE), _buf(F32, 5, ptr=FAKE)
+    q, o = _buf(F32, 4, 3, ptr=FAKE), _buf(F32, 4, 3, ptr=FAKE)
+    args = (_f(1.0), _f(0.01), _i(10), _i(32), _i(1))
+    rc, err = _call("sbh_stochastic_gravity_force", None, *map(ctypes.byref, (p, m, q, o)),
+                    *args, _i(-1), _i(0), None, None)
+    assert rc == 3 and "samples_per_subdomain must be >= 0" in err
+    rc, err = _call("sbh_stochastic_gravity_force", None, *map(ctypes.byref, (p, m, q, o)),
+                    _f(1.0), _f(0.01), _i(11), _i(32), _i(1), _i(1), _i(0), None, None)
+    assert rc == 3 and "max_depth must be in [1, 10]" in err
+    rc, err = _call("sbh_stochastic_gravity_force", None, *map(ctypes.byref, (p, m, q, o)),
+                    _f(1.0), _f(0.01), _i(10), _i(32), _i(3), _i(1), _i(0), None, None)
+    assert rc == 3 and "multipole_order must be 1 or 2" in err
+    o2 = _buf(F32, 5, 3, ptr=FAKE)
+    rc, err = _call("sbh_stochastic_gravity_force", None, *map(ctypes.byref, (p, m, q, o2)),
+                    *args, _i(1), _i(0), None, None)
+    assert rc == 3 and "out must have shape (M, 3)" in err
+    c_bad = _buf(F32, 5, 3, ptr=FAKE)
+    rc, err = _call("sbh_stochastic_gravity_force_vjp", None,
+                    *map(ctypes.byref, (p, m, q, c_bad, o)), *args, _i(1), _i(0), None, None)
+    assert rc == 3 and "cotangent must have shape (M, 3)" in err
+    n_bad, a = _buf(F32, 5, 2, ptr=FAKE), _buf(F32, 5, ptr=FAKE)
+    w = _buf(F32, 4, ptr=FAKE)
+    rc, err = _call("sbh_stochastic_winding_number", None,
+                    *map(ctypes.byref, (p, n_bad, a, q, w)), _f(2.0), _i(10), _i(16), _i(1),
+                    _i(0), None, None)
+    assert rc == 3 and "normals must have shape (N, 3)" in err
+    lib = _lib()
+    lib.sbh_stochastic_workspace_bytes.restype = ctypes.c_int64
+    assert lib.sbh_stochastic_workspace_bytes(ctypes.c_int64(-1), ctypes.c_int64(4), _i(10),
+                                              _i(32)) == -1
+    assert lib.sbh_stochastic_workspace_bytes(ctypes.c_int64(1000), ctypes.c_int64(10), _i(10),
+                                              _i(32)) > 0

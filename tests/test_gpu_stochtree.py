@@ -1,0 +1,164 @@
+"""GPU: stochastic Barnes-Hut over the adaptive octree (sbh_stochastic_*).
+
+- Path for path against the oracle (orc_tree_stoch): the same source tree,
+  query groups, subdomains and Philox draws give the same terms (node and
+  pair counts equal) and the same estimates up to fp32 rounding.  The
+  estimator multiplies (children - parent) differences by weights
+  n_sub / (P count) up to ~1e4, so fp32 rounding of the sums shows at
+  ~1e-5 relative: tolerance rel_l2 <= 2e-4.
+- Unbiasedness at the benchmark's tree parameters (max depth 10, leaf 32,
+  one sample per subdomain, theta 0.5): the mean over 256 seeds matches the
+  FP64 direct sum by a z-test (force and its query-side VJP; winding numbers
+  with 32 samples against the FP64 dipole sum).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _S():
+    from sbh_amd import stochastic
+
+    return stochastic
+
+
+def _np(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def _plummer(n, seed):
+    from sbh_amd import synthetic
+
+    return synthetic.plummer(n, seed)
+
+
+def _ztest(samples, target):
+    a = np.asarray(samples, np.float64)
+    mean = a.mean(0)
+    se = a.std(0, ddof=1) / np.sqrt(a.shape[0])
+    ok = se > 1e-12 * np.abs(target).max()
+    z = (mean - target)[ok] / se[ok]
+    return float(np.sqrt(np.mean(z ** 2))), float(np.mean(np.abs(z) < 3)), O.rel_l2(mean, target)
+
+
+@pytest.mark.parametrize("kind,order,beta,samples", [
+    (0, 1, 1.0, 1), (0, 2, 2.0, 2), (1, 1, 1.0, 1), (1, 2, 2.0, 1), (0, 1, 0.0, 1),
+])
+def test_gravity_matches_oracle_path_for_path(gpu, kind, order, beta, samples):
+    S = _S()
+    rng = np.random.default_rng(40 + kind)
+    n, eps = 20000, 0.01
+    p = _plummer(n, 41)
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32) / n
+    q = np.concatenate([p[rng.choice(n, 1500, replace=False)],
+                        rng.normal(size=(1500, 3)).astype(np.float32) * 2.0])
+    c = rng.normal(size=q.shape).astype(np.float32)
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    kw = dict(beta=beta, samples_per_subdomain=samples, seed=7, softening_scale=eps,
+              max_depth=10, leaf_size=16, multipole_order=order, stats=st)
+    if kind == 0:
+        out = _np(S.stochastic_gravity_force(p, m, q, **kw))
+    else:
+        out = _np(S.stochastic_gravity_force_vjp(p, m, q, c, **kw))
+    s = S.stats_dict(st)
+    T = O.Tree(p, m, max_depth=10, leaf_size=16)
+    r = T.stoch_eval(q, kind=kind, beta=beta, eps=eps, samples=samples, seed=7,
+                     use_quad=order == 2, cot=c if kind == 1 else None)
+    assert s["overflow"] == 0
+    assert s["subdomains"] == len(T.subdomains())
+    assert (s["node_terms"], s["source_pairs"], s["groups"]) == (r["node"], r["pair"],
+                                                                  r["groups"])
+    err = O.rel_l2(out, r["out"])
+    print(f"kind {kind} order {order} beta {beta}: rel_l2 vs oracle {err:.2e}")
+    assert err < 2e-4
+
+
+def test_winding_number_matches_oracle_path_for_path(gpu):
+    from sbh_amd import synthetic
+
+    S = _S()
+    V, F = synthetic.geodesic_sphere(16)
+    cen, nrm, area = synthetic.face_dipoles(V, F)
+    rng = np.random.default_rng(44)
+    q = rng.uniform(-1.5, 1.5, (4000, 3)).astype(np.float32)
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    out = _np(S.stochastic_winding_number(cen, nrm, area, q, beta=2.0, samples_per_subdomain=4,
+                                          seed=3, max_depth=10, leaf_size=2, stats=st))
+    s = S.stats_dict(st)
+    T = O.Tree(cen, area, max_depth=10, leaf_size=2)
+    r = T.stoch_eval(q, kind=2, beta=2.0, samples=4, seed=3, normals=nrm)
+    assert s["overflow"] == 0
+    assert (s["node_terms"], s["source_pairs"], s["groups"]) == (r["node"], r["pair"],
+                                                                  r["groups"])
+    assert O.rel_l2(out, r["out"]) < 2e-4
+
+
+def test_same_seed_same_bits_other_seed_differs(gpu):
+    S = _S()
+    n = 30000
+    p = _plummer(n, 45)
+    m = np.full(n, 1.0 / n, np.float32)
+    q = p[:4096]
+    kw = dict(beta=2.0, softening_scale=0.005, max_depth=10, leaf_size=32)
+    a = _np(S.stochastic_gravity_force(p, m, q, seed=11, **kw))
+    b = _np(S.stochastic_gravity_force(p, m, q, seed=11, **kw))
+    c = _np(S.stochastic_gravity_force(p, m, q, seed=12, **kw))
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_unbiased_over_256_seeds_depth10_leaf32(gpu, order):
+    # Plummer sphere (scale 1, truncated at 20), masses 1/N, softening 0.005,
+    # theta 0.5 (beta 2), max depth 10, leaf 32, one sample per subdomain;
+    # 512 of the points as queries (the self pair has zero force)
+    S = _S()
+    n, eps = 1 << 17, 0.005
+    p = _plummer(n, 46)
+    m = np.full(n, 1.0 / n, np.float32)
+    rng = np.random.default_rng(47)
+    q = p[rng.choice(n, 512, replace=False)]
+    c = rng.normal(size=q.shape).astype(np.float32)
+    P, M, Q, C = (torch.tensor(a, device="cuda") for a in (p, m, q, c))
+    kw = dict(beta=2.0, samples_per_subdomain=1, softening_scale=eps, max_depth=10,
+              leaf_size=32, multipole_order=order)
+    ws = S.new_workspace(n, q.shape[0], 10, 32)
+    fs, gs = [], []
+    for seed in range(256):
+        fs.append(_np(S.stochastic_gravity_force(P, M, Q, seed=seed, workspace=ws, **kw)))
+        gs.append(_np(S.stochastic_gravity_force_vjp(P, M, Q, C, seed=seed, workspace=ws,
+                                                      **kw)))
+    ref_f = O.direct_force_soft(p, m, q, eps, np.inf)
+    ref_g = O.direct_force_soft_vjp(p, m, q, c, eps, np.inf)
+    for name, a, ref in (("force", fs, ref_f), ("vjp", gs, ref_g)):
+        rms, frac, err = _ztest(a, ref)
+        one = O.rel_l2(a[0], ref)
+        print(f"order {order} {name}: one seed rel_l2 {one:.2e}, 256-seed mean rel_l2 "
+              f"{err:.2e}, rms z {rms:.3f}, |z| < 3 for {frac:.3f}")
+        assert frac > 0.95 and rms < 1.5, name
+
+
+def test_winding_number_unbiased_32_samples(gpu):
+    # perturbed geodesic sphere, 32 samples per subdomain, 128 seeds
+    from sbh_amd import synthetic
+
+    S = _S()
+    V, F = synthetic.perturbed_sphere(24, 48)
+    cen, nrm, area = synthetic.face_dipoles(V, F)
+    rng = np.random.default_rng(49)
+    q = rng.uniform(-1.5, 1.5, (1024, 3)).astype(np.float32)
+    Pc, Nn, A, Q = (torch.tensor(a, device="cuda") for a in (cen, nrm, area, q))
+    vals = [_np(S.stochastic_winding_number(Pc, Nn, A, Q, beta=2.0, samples_per_subdomain=32,
+                                            seed=s, max_depth=10, leaf_size=4))
+            for s in range(128)]
+    ref = O.bruteforce_wn(cen, nrm, area, q)
+    rms, frac, err = _ztest(vals, ref)
+    one = O.rel_l2(vals[0], ref)
+    print(f"wn: one seed rel_l2 {one:.2e}, mean rel_l2 {err:.2e}, rms z {rms:.3f}, "
+          f"|z| < 3 for {frac:.3f}")
+    assert frac > 0.95 and rms < 1.5
