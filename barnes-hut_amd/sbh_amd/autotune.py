@@ -1,8 +1,10 @@
 """Autotuner for the batched gravity pipeline.
 
 Times the captured fwd + VJP graph of the actual batch for every candidate
-(leaf size, persistent walk grid) and keeps the fastest.  Results are cached
-per workload key (total points, set count, a digest of the set sizes and the
+(leaf size, persistent walk grid) and keeps the fastest candidate whose
+forces and gradients stay within rel_l2 <= 1e-3 of an FP64 direct sum on
+sampled rows of every set.  Results are cached per workload key (device,
+total points, set sizes, a digest of the per-set bounding boxes and the
 physics parameters) for the life of the process, and optionally in a JSON
 file.
 """
@@ -22,12 +24,85 @@ _CACHE: dict = {}
 
 DEFAULT_LEAF = (8, 16, 24, 32, 48)
 DEFAULT_BLOCKS = (0, 1024, 4096)
+REL_L2_GATE = 1e-3
 
 
-def _key(offsets: torch.Tensor, kw: dict) -> str:
-    h = hashlib.sha1(offsets.cpu().numpy().tobytes()).hexdigest()[:16]
+def _set_boxes(points: torch.Tensor, offsets: torch.Tensor) -> torch.Tensor:
+    """(num_sets, 6) per-set min xyz, max xyz (empty sets: +inf / -inf)."""
+    nsets = offsets.shape[0] - 1
+    sizes = (offsets[1:] - offsets[:-1]).long()
+    sid = torch.repeat_interleave(torch.arange(nsets, device=points.device), sizes)
+    lo = torch.full((nsets, 3), float("inf"), device=points.device)
+    hi = torch.full((nsets, 3), float("-inf"), device=points.device)
+    idx = sid[:, None].expand(-1, 3)
+    lo = lo.scatter_reduce(0, idx, points, reduce="amin")
+    hi = hi.scatter_reduce(0, idx, points, reduce="amax")
+    return torch.cat([lo, hi], 1)
+
+
+def _key(points: torch.Tensor, offsets: torch.Tensor, kw: dict) -> str:
+    h = hashlib.sha1(offsets.cpu().numpy().tobytes())
+    h.update(_set_boxes(points, offsets).cpu().numpy().tobytes())
+    dev = torch.cuda.get_device_name(points.device).replace(" ", "_")
     phys = ",".join(f"{k}={kw[k]}" for k in sorted(kw))
-    return f"n{int(offsets[-1])}_s{offsets.shape[0] - 1}_{h}_{phys}"
+    return f"{dev}_n{int(offsets[-1])}_s{offsets.shape[0] - 1}_{h.hexdigest()[:16]}_{phys}"
+
+
+def direct_rows_fp64(P, M, C, rows, eps):
+    """FP64 direct-sum forces and gradients of L = sum_i c_i . F_i at `rows`
+    of one point set (softened force, self pair excluded):
+    (forces[rows], grad_points[rows], grad_masses[rows])."""
+    P, M, C = P.double(), M.double(), C.double()
+    e2 = eps * eps + 1e-12
+    d = P[rows][:, None, :] - P[None]  # x_j - x_k
+    r2 = (d * d).sum(-1) + e2
+    s3 = r2 ** -1.5
+    s5 = s3 / r2
+    self_mask = torch.arange(P.shape[0], device=P.device)[None, :] == rows[:, None]
+    s3 = s3.masked_fill(self_mask, 0.0)
+    s5 = s5.masked_fill(self_mask, 0.0)
+    f = -((M[None, :] * s3)[..., None] * d).sum(1)
+    cj = C[rows]
+    dc = (d * cj[:, None, :]).sum(-1)
+    q = -(M[None, :, None] * (s3[..., None] * cj[:, None, :] - 3.0 * (s5 * dc)[..., None] * d)).sum(1)
+    cd = (d * C[None]).sum(-1)
+    src = M[rows][:, None] * (s3[..., None] * C[None] - 3.0 * (s5 * cd)[..., None] * d).sum(1)
+    return f, q + src, (s3 * cd).sum(1)
+
+
+def _rel_l2(a: torch.Tensor, b: torch.Tensor) -> float:
+    a, b = a.double(), b.double()
+    return float(torch.linalg.norm(a - b) / torch.clamp(torch.linalg.norm(b), min=1e-300))
+
+
+class _Gate:
+    """FP64 direct-sum rows of every non-empty set (up to `rows_per_set` each)
+    and the rel_l2 of a candidate's forces and gradients against them."""
+
+    def __init__(self, P, M, C, offsets, eps, rows_per_set=4, seed=0):
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        o = offsets.cpu().tolist()
+        rows, ref_f, ref_x, ref_m = [], [], [], []
+        for o0, o1 in zip(o[:-1], o[1:]):
+            n = o1 - o0
+            if n < 2:
+                continue
+            r = torch.randperm(n, generator=g)[:rows_per_set].to(P.device)
+            f, gx, gm = direct_rows_fp64(P[o0:o1], M[o0:o1], C[o0:o1], r, eps)
+            rows.append(r + o0)
+            ref_f.append(f)
+            ref_x.append(gx)
+            ref_m.append(gm)
+        self.rows = torch.cat(rows) if rows else None
+        if self.rows is not None:
+            self.ref = (torch.cat(ref_f), torch.cat(ref_x), torch.cat(ref_m))
+
+    def errors(self, F, gx, gm) -> dict:
+        if self.rows is None:
+            return {"force": 0.0, "grad_points": 0.0, "grad_masses": 0.0}
+        r = self.rows
+        return {"force": _rel_l2(F[r], self.ref[0]), "grad_points": _rel_l2(gx[r], self.ref[1]),
+                "grad_masses": _rel_l2(gm[r], self.ref[2])}
 
 
 def build_step(P, M, C, offsets, leaf_size: int, walk_blocks: int, kw: dict):
@@ -67,11 +142,15 @@ def time_graph(step, reps: int = 3) -> float:
 
 def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
                walk_blocks: Iterable[int] = DEFAULT_BLOCKS, reps: int = 3,
-               cache_file: Optional[str] = None, **kw) -> dict:
+               cache_file: Optional[str] = None, gate: float = REL_L2_GATE,
+               rows_per_set: int = 4, **kw) -> dict:
     """Best {leaf_size, walk_blocks, ms} for this batch, plus the full table.
 
+    Candidates whose forces or gradients miss rel_l2 <= `gate` against the
+    FP64 direct sum on `rows_per_set` sampled rows of every set are dropped;
+    raises RuntimeError if none passes.
     kw: beta, softening_scale, max_depth, multipole_order (as gravity_force)."""
-    key = _key(offsets, kw)
+    key = _key(P, offsets, kw) + f"_gate{gate}"
     if key in _CACHE:
         return _CACHE[key]
     if cache_file and os.path.exists(cache_file):
@@ -80,16 +159,27 @@ def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
         if key in disk:
             _CACHE[key] = disk[key]
             return disk[key]
+    checker = _Gate(P, M, C, offsets, kw.get("softening_scale", 1e-2), rows_per_set)
     table = []
     for leaf in leaf_sizes:
         for blocks in walk_blocks:
-            step, _, ws = build_step(P, M, C, offsets, leaf, blocks, kw)
+            step, (F, gx, gm), ws = build_step(P, M, C, offsets, leaf, blocks, kw)
             ms = time_graph(step, reps)
-            table.append({"leaf_size": leaf, "walk_blocks": blocks, "ms": ms})
+            step()
+            err = checker.errors(F, gx, gm)
+            ok = max(err.values()) <= gate
+            table.append({"leaf_size": leaf, "walk_blocks": blocks, "ms": ms,
+                          "rel_l2": err, "passes_gate": ok})
             del ws
-    best = min(table, key=lambda r: r["ms"])
+    passing = [r for r in table if r["passes_gate"]]
+    if not passing:
+        raise RuntimeError(f"no candidate meets rel_l2 <= {gate}: {table}")
+    best = min(passing, key=lambda r: r["ms"])
     res = {"leaf_size": best["leaf_size"], "walk_blocks": best["walk_blocks"],
-           "ms": best["ms"], "table": table}
+           "ms": best["ms"], "table": table,
+           "gate": {"rel_l2_max": gate, "rows": 0 if checker.rows is None
+                    else int(checker.rows.numel()),
+                    "reference": "FP64 direct sum and adjoint over each row's own set"}}
     _CACHE[key] = res
     if cache_file:
         disk = {}

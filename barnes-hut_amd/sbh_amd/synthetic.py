@@ -2,6 +2,7 @@
 
 * plummer: Plummer sphere, scale radius 1, truncated at r = 20 by inverse CDF.
 * blobs: mixture of isotropic Gaussian blobs with log-normal masses.
+* blob_sets: the batched workload, many blob sets of log-uniform sizes.
 * geodesic_sphere / perturbed_sphere: closed triangle meshes (frequency-f
   geodesic subdivision of the icosahedron, 20 f^2 faces, optionally with the
   radius perturbed by real spherical harmonics) and their per-face dipoles
@@ -36,6 +37,23 @@ def blobs(n: int, k: int, seed: int, sigma: float = 0.02):
     m = rng.lognormal(0.0, 0.5, n)
     m /= m.sum()
     return p.astype(np.float32), m.astype(np.float32)
+
+
+def blob_sets(num_sets: int, seed: int = 5000, lo: int = 1024, hi: int = 524288):
+    """Sets of sizes round(exp(U(ln lo, ln hi))), each a mixture of U{8..64}
+    blobs (sigma 0.02, log-normal masses normalised to sum 1 per set), with
+    cotangents N(0, 1).  Returns (sizes, points, masses, cotangents), the last
+    three as per-set lists."""
+    rng = np.random.default_rng(seed)
+    sizes = np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), num_sets)))
+    sizes = sizes.astype(np.int64)
+    ps, ms, cs = [], [], []
+    for k, n in enumerate(sizes):
+        p, m = blobs(int(n), int(rng.integers(8, 65)), seed + 1 + k)
+        ps.append(p)
+        ms.append(m)
+        cs.append(rng.normal(size=(int(n), 3)).astype(np.float32))
+    return sizes, ps, ms, cs
 
 
 def _icosahedron():

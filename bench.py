@@ -33,10 +33,50 @@ for _p in (ROOT, os.path.join(ROOT, "barnes-hut_amd")):
 METRIC = "BH forces/sec at N=1M θ=0.5 (rel_l2≤1e-3), fwd+VJP; batched vs per-set"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
-# Algorithmic flops per lane interaction (one query x one node or source).
+FP32_PEAK_NON_PACKED_TFLOPS = 78.6  # the same VALUs without packed-FP32 instructions
+# Algorithmic flops per interaction (one query x one near source or one far
+# node), the direct-sum pair counts: 20 forward, 48 full adjoint.  A far node
+# counts as one source whatever its expansion order.
+FLOPS_ALG_FWD, FLOPS_ALG_VJP = 20, 48
+# The implementation's own count: near pairs as above; a far node costs
+# 20 / 48 with the monopole and 54 / 200 with the quadrupole terms.
 FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
 FLOPS_FAR_FWD = {1: 20, 2: 54}
 FLOPS_FAR_VJP = {1: 48, 2: 200}
+OUTLIER_REL_THRESHOLD = 0.05  # per-row relative error counted as an outlier
+
+
+def walk_flops(stats, vjp, order):
+    """(algorithmic, implementation) flops of one walk with interaction
+    counts `stats`."""
+    near, far = stats["near"], stats["far"]
+    if vjp:
+        return (FLOPS_ALG_VJP * (near + far),
+                FLOPS_NEAR_VJP * near + FLOPS_FAR_VJP[order] * far)
+    return (FLOPS_ALG_FWD * (near + far), FLOPS_NEAR_FWD * near + FLOPS_FAR_FWD[order] * far)
+
+
+def valu_roofline(kernel, flops_alg, flops_impl, ms, traffic=None, **extra):
+    """FP32-VALU roofline record of one walk launch: achieved and frac by the
+    algorithmic count, the implementation's count beside it, against the
+    157.3 TFLOP/s peak (packed FP32) and the 78.6 TFLOP/s of unpacked issue."""
+    a = flops_alg / (ms * 1e-3) / 1e12
+    b = flops_impl / (ms * 1e-3) / 1e12
+    rec = {"bound": "valu", "compute": "fp32 VALU (no matrix op in the walk)",
+           "kernel": kernel, "achieved": a, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": a / FP32_PEAK_TFLOPS, "traffic": traffic,
+           "flop_model": "20 flop per forward and 48 per adjoint interaction (near pair or "
+                         "far node)",
+           "flops_per_launch": flops_alg, "ms_per_launch": ms,
+           "frac_of_non_packed_peak": a / FP32_PEAK_NON_PACKED_TFLOPS,
+           "implementation_count": {
+               "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole) or 54 / 200 "
+                             "(quadrupole)",
+               "achieved": b, "frac": b / FP32_PEAK_TFLOPS,
+               "frac_of_non_packed_peak": b / FP32_PEAK_NON_PACKED_TFLOPS,
+               "flops_per_launch": flops_impl}}
+    rec.update(extra)
+    return rec
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
@@ -74,6 +114,43 @@ def rel_l2(a, b):
     a = a.double()
     b = b.double()
     return float(torch.linalg.norm(a - b) / torch.clamp(torch.linalg.norm(b), min=1e-30))
+
+
+def outlier_frac(pred, ref, thr=OUTLIER_REL_THRESHOLD):
+    """Fraction of rows whose relative error exceeds thr (the reference
+    benchmark's outlier metric, python/unified_force_benchmark.py:316-328)."""
+    import torch
+
+    d = torch.linalg.norm((pred.double() - ref.double()).reshape(ref.shape[0], -1), dim=1)
+    r = torch.linalg.norm(ref.double().reshape(ref.shape[0], -1), dim=1)
+    return float((d / torch.clamp(r, min=1e-12) > thr).double().mean())
+
+
+def direct_adjoint_rows_fp64(P, M, C, rows, eps, chunk=8):
+    """FP64 direct-sum gradients of L = sum_i c_i . F_i (softened force, self
+    pair excluded) with respect to the positions and masses of the points
+    `rows`: (grad_points[rows], grad_masses[rows])."""
+    import torch
+
+    e2 = eps * eps + 1e-12
+    gx, gm = [], []
+    for r in rows.split(chunk):
+        d = P[r][:, None, :] - P[None]  # x_j - x_k
+        r2 = (d * d).sum(-1) + e2
+        s3 = r2 ** -1.5
+        s5 = s3 / r2
+        self_mask = torch.arange(P.shape[0], device=P.device)[None, :] == r[:, None]
+        s3 = s3.masked_fill(self_mask, 0.0)
+        s5 = s5.masked_fill(self_mask, 0.0)
+        cj = C[r]
+        dc = (d * cj[:, None, :]).sum(-1)  # d_jk . c_j
+        q = -(M[None, :, None] * (s3[..., None] * cj[:, None, :] -
+                                  3.0 * (s5 * dc)[..., None] * d)).sum(1)
+        cd = (d * C[None]).sum(-1)  # c_k . d_jk
+        src = M[r][:, None] * (s3[..., None] * C[None] - 3.0 * (s5 * cd)[..., None] * d).sum(1)
+        gx.append(q + src)
+        gm.append((s3 * cd).sum(1))
+    return torch.cat(gx), torch.cat(gm)
 
 
 def cpu_baseline(p, m, c, args, sample_groups):
@@ -204,6 +281,12 @@ def main():
     ap.add_argument("--eps", type=float, default=0.005)
     ap.add_argument("--beta", type=float, default=2.0)
     ap.add_argument("--accuracy-queries", type=int, default=1024)
+    ap.add_argument("--grad-rows", type=int, default=256,
+                    help="rows of the FP64 direct adjoint the gradients are checked on")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="gravity: skip the 64-set batched record of the headline line")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="gravity: skip the N = 10,000 uniform-cube record")
     ap.add_argument("--cpu-sample-groups", type=int, default=10000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
@@ -212,7 +295,7 @@ def main():
                          "HIP graph and the replay is the reported value)")
     ap.add_argument("--workload", default="gravity",
                     choices=("gravity", "wn", "batched", "stochastic", "treepm", "quadtree",
-                             "fields"),
+                             "fields", "config1"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
@@ -248,7 +331,8 @@ def main():
         sys.exit(2)
     bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
              "stochastic": bench_stochastic, "treepm": bench_treepm,
-             "quadtree": bench_quadtree, "fields": bench_fields}[args.workload]
+             "quadtree": bench_quadtree, "fields": bench_fields,
+             "config1": bench_config1}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -334,16 +418,24 @@ def bench_gravity(args):
     sf, sv = G.stats_dict(sf), G.stats_dict(sv)
     walk_vjp_ms = prof[5] / max(prof[3], 1)
     walk_fwd_ms = prof[2] / max(prof[0], 1)
-    flops_vjp = FLOPS_NEAR_VJP * sv["near"] + FLOPS_FAR_VJP[args.order] * sv["far"]
-    flops_fwd = FLOPS_NEAR_FWD * sf["near"] + FLOPS_FAR_FWD[args.order] * sf["far"]
-    achieved = flops_vjp / (walk_vjp_ms * 1e-3) / 1e12
+    flops_vjp, flops_vjp_impl = walk_flops(sv, True, args.order)
+    flops_fwd, flops_fwd_impl = walk_flops(sf, False, args.order)
 
-    # accuracy against the FP64 direct sum on a query sample
+    # accuracy against the FP64 direct sum on a query sample: forces, and both
+    # gradients against the FP64 direct adjoint of the same rows
     idx = torch.tensor(np.random.default_rng(7).choice(n, args.accuracy_queries, replace=False),
                        device=dev)
     ref = direct_fp64(P.double(), M.double(), P[idx].double(), args.eps)
+    rows = idx[:args.grad_rows]
+    rgx, rgm = direct_adjoint_rows_fp64(P.double(), M.double(), C.double(), rows, args.eps)
     acc = {"force_rel_l2_vs_fp64_direct": rel_l2(F[idx], ref),
-           "force_sample_queries": args.accuracy_queries}
+           "force_outlier_frac_vs_fp64_direct": outlier_frac(F[idx], ref),
+           "outlier_rel_threshold": OUTLIER_REL_THRESHOLD,
+           "force_sample_queries": args.accuracy_queries,
+           "grad_points_rel_l2_vs_fp64_direct_adjoint": rel_l2(gx[rows], rgx),
+           "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(gm[rows], rgm),
+           "grad_points_outlier_frac_vs_fp64_direct_adjoint": outlier_frac(gx[rows], rgx),
+           "grad_sample_rows": int(rows.numel())}
     direct = None
     if not args.no_direct:
         from sbh_amd import ffi
@@ -368,6 +460,9 @@ def bench_gravity(args):
         acc["grad_points_rel_l2_vs_fp32_direct"] = rel_l2(gx, gxd)
         acc["grad_masses_rel_l2_vs_fp32_direct"] = rel_l2(gm, gmd)
         acc["force_full_rel_l2_vs_fp32_direct"] = rel_l2(F, Fd)
+
+    batched = None if args.no_batched else measure_batched(args, dev, tune=not args.no_autotune)
+    config1 = None if args.no_config1 else measure_config1(args, dev)
 
     cpu = cpu_ref = cpu_np = cpu_lit = None
     if not args.no_cpu_baseline:
@@ -396,26 +491,29 @@ def bench_gravity(args):
             "leaf_size": args.leaf, "softening": args.eps, "multipole_order": args.order,
             "step": "forward (build+walk) + recompute VJP (rebuild+adjoint walk), full "
                     "position and mass gradients",
-            "batched": "not measured (single set)",
+            "batched": "see the batched record: 64 blob sets, one graph-captured forest "
+                       "pipeline vs the 64 per-set pipelines captured as one graph",
             "launch": "HIP graph replay" if graph_ms is not None else "eager",
         },
         "eager_ms_per_step": eager_ms,
         "graph_ms_per_step": graph_ms,
         "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
                       "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
-        "interactions": {"fwd": sf, "vjp": sv, "fwd_walk_tflops":
-                         flops_fwd / (walk_fwd_ms * 1e-3) / 1e12},
+        "interactions": {"fwd": sf, "vjp": sv},
         "accuracy": acc,
         "direct_sum": direct,
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
-                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0>",
-                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS,
-                     "traffic": (profiled_traffic("k_walk<true,true,false>")
-                                 if args.order == 2 and n == 1 << 20 else None),
-                     "traffic_source": "profiles/" + os.path.basename(TRAFFIC_PROFILE),
-                     "flops_per_launch": flops_vjp},
-        "other_rooflines": other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct),
+        "roofline": valu_roofline(
+            f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0>", flops_vjp, flops_vjp_impl,
+            walk_vjp_ms,
+            traffic=(profiled_traffic("k_walk<true,true,false>")
+                     if args.order == 2 and n == 1 << 20 else None),
+            traffic_source="profiles/" + os.path.basename(TRAFFIC_PROFILE),
+            timing="library HIP events around the walk launch on its stream, mean of the "
+                   "timed eager steps"),
+        "other_rooflines": other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl,
+                                           prof, direct),
+        "batched": batched,
+        "config1": config1,
         "cpu_baseline": cpu,
         "cpu_reference_path": cpu_ref,
         "cpu_numpy_direct": cpu_np,
@@ -444,7 +542,7 @@ def build_bytes_design(n, nodes, vjp):
     return per_point * n + 84 * nodes
 
 
-def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct):
+def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl, prof, direct):
     """Build (HBM-bound), forward walk (FP32) and direct sum (FP32) against the
     chip's peaks; the VJP walk's HBM rate from the profiled traffic."""
     fwd_build_ms = prof[1] / max(prof[0], 1)
@@ -460,9 +558,10 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct):
                     "design_bytes": build_bytes_design(n, sf["nodes"], vjp),
                     "note": "about 70 dependent launches per build; launch latency, not "
                             "bandwidth, sets the time"}
-    t = flops_fwd / (walk_fwd_ms * 1e-3) / 1e12
-    out["walk_fwd"] = {"bound": "mfma", "achieved": t, "peak": FP32_PEAK_TFLOPS,
-                       "unit": "TFLOP/s", "frac": t / FP32_PEAK_TFLOPS, "ms": walk_fwd_ms}
+    out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,quad={int(args.order == 2)},forces=1>",
+                                    flops_fwd, flops_fwd_impl, walk_fwd_ms,
+                                    traffic=(profiled_traffic("k_walk<false,true,true>")
+                                             if args.order == 2 and n == 1 << 20 else None))
     # walks against HBM: profiled traffic over the kernel time, beside the
     # minimal bytes (40 B/pt + 48 B/node; the adjoint +36 B/pt)
     for tag, key, ms, extra in (("walk_fwd_hbm", "k_walk<false,true,true>", walk_fwd_ms, 0),
@@ -476,7 +575,7 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, prof, direct):
                     "minimal_bytes": (40 + extra) * n + 48 * sf["nodes"]}
     if direct is not None:
         t = direct["direct_fwd_tflops"]
-        out["direct_sum_fwd"] = {"bound": "mfma", "compute": "fp32 VALU, LDS-tiled",
+        out["direct_sum_fwd"] = {"bound": "valu", "compute": "fp32 VALU, LDS-tiled",
                                  "achieved": t, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                                  "frac": t / FP32_PEAK_TFLOPS, "flops_per_pair": 20}
     return out
@@ -1099,36 +1198,25 @@ def bench_fields(args):
 
 
 def batched_sets(num_sets, seed=5000):
-    """Set sizes round(exp(U(ln 1024, ln 524288))), each a blob mixture of
-    U{8..64} blobs (sigma 0.02, log-normal masses normalised to sum 1) with
-    cotangents N(0, 1)."""
-    import math
-
-    import numpy as np
-
+    """The 64-set workload: sizes log-uniform in [1,024, 524,288], blob
+    mixtures (sbh_amd.synthetic.blob_sets)."""
     from sbh_amd import synthetic
 
-    rng = np.random.default_rng(seed)
-    sizes = np.round(np.exp(rng.uniform(math.log(1024), math.log(524288), num_sets)))
-    sizes = sizes.astype(np.int64)
-    ps, ms, cs = [], [], []
-    for k, n in enumerate(sizes):
-        p, m = synthetic.blobs(int(n), int(rng.integers(8, 65)), seed + 1 + k)
-        ps.append(p)
-        ms.append(m)
-        cs.append(rng.normal(size=(int(n), 3)).astype(np.float32))
-    return sizes, ps, ms, cs
+    return synthetic.blob_sets(num_sets, seed)
 
 
-def bench_batched(args):
+def measure_batched(args, dev, tune=True, cpu=False):
+    """The 64-set batched fwd + VJP workload: one forest pipeline captured as
+    a HIP graph against the 64 per-set pipelines, both eager and captured as
+    one graph.  Returns a record (times, speed-ups, accuracy, walk roofline)."""
     import numpy as np
     import torch
 
+    import sbh_amd
     from sbh_amd import autotune
     from sbh_amd import gravity as G
+    from sbh_amd.graph import Graph
 
-    dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     sizes, ps, ms, cs = batched_sets(args.sets)
     P = torch.tensor(np.concatenate(ps), device=dev)
     M = torch.tensor(np.concatenate(ms), device=dev)
@@ -1137,134 +1225,288 @@ def bench_batched(args):
     n_total = int(P.shape[0])
     kw = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth,
               multipole_order=args.order)
-    if args.no_autotune:
-        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "table": None}
-    else:
+    if tune:
         tuned = autotune.tune_batch(P, M, C, offs, reps=2, **kw)
+    else:
+        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "table": None}
     leaf, blocks = tuned["leaf_size"], tuned["walk_blocks"]
     stream = torch.cuda.current_stream()
+    steps = max(1, args.steps)
 
-    def timed(fn, steps, warmup):
+    def timed(fn, nsteps, warmup):
         for _ in range(warmup):
             fn()
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
-        for _ in range(steps):
+        for _ in range(nsteps):
             fn()
         e1.record(stream)
         torch.cuda.synchronize()
-        return e0.elapsed_time(e1) / steps
+        return e0.elapsed_time(e1) / nsteps
 
     # batched: one forest pipeline, captured as a HIP graph
     step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw)
-    from sbh_amd.graph import Graph
-
     step()
     g = Graph()
     g.capture(step)
-    ms_graph = timed(g.replay, args.steps, args.warmup)
-    ms_eager = timed(step, args.steps, args.warmup)
+    ms_graph = timed(g.replay, steps, max(1, args.warmup))
+    ms_eager = timed(step, steps, max(1, args.warmup))
     # walk-kernel times of the batched step (library HIP events, eager calls)
-    import sbh_amd
-
     lib = sbh_amd.load_library()
     lib.sbh_profile_enable(ctypes.c_int32(1))
-    for _ in range(args.steps):
+    for _ in range(steps):
         step()
     torch.cuda.synchronize()
     lib.sbh_profile_enable(ctypes.c_int32(0))
     prof = (ctypes.c_double * 6)()
     lib.sbh_profile_collect(prof, ctypes.c_int32(6))
-    # per set: 64 separate calls of the single-set pipeline on one stream
-    Ps = [P[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
-    Ms = [M[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
-    Cs = [C[o0:o1] for o0, o1 in zip(offs[:-1].tolist(), offs[1:].tolist())]
+    # per set: the single-set pipeline called once per set on one stream, with
+    # the same leaf size, eagerly and captured as one graph of all 64 calls
+    bounds = list(zip(offs[:-1].tolist(), offs[1:].tolist()))
+    Ps = [P[o0:o1] for o0, o1 in bounds]
+    Ms = [M[o0:o1] for o0, o1 in bounds]
+    Cs = [C[o0:o1] for o0, o1 in bounds]
     wss = G.new_workspace(int(sizes.max()), args.depth, leaf, dev)
     Fs = [torch.empty_like(x) for x in Ps]
+    GXs = [torch.empty_like(x) for x in Ps]
+    GMs = [torch.empty_like(x) for x in Ms]
 
     def per_set():
         for k in range(len(Ps)):
             G.gravity_force(Ps[k], Ms[k], leaf_size=leaf, workspace=wss, out=Fs[k], **kw)
-            G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=leaf, workspace=wss, **kw)
+            G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=leaf, workspace=wss,
+                                grad_points_out=GXs[k], grad_masses_out=GMs[k], **kw)
 
-    ms_per_set = timed(per_set, max(1, args.steps // 2), 1)
-    per_set_default = None
-    if leaf != args.leaf:
-        wsd = G.new_workspace(int(sizes.max()), args.depth, args.leaf, dev)
-
-        def per_set_d():
-            for k in range(len(Ps)):
-                G.gravity_force(Ps[k], Ms[k], leaf_size=args.leaf, workspace=wsd, out=Fs[k], **kw)
-                G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=args.leaf, workspace=wsd,
-                                    **kw)
-        per_set_default = timed(per_set_d, max(1, args.steps // 2), 1)
-    # parity of the batch with the per-set calls (forces bit-exact)
+    ms_per_set = timed(per_set, max(1, steps // 2), 1)
+    gp = Graph()
+    gp.capture(per_set)
+    ms_per_set_graph = timed(gp.replay, max(1, steps // 2), 1)
+    # parity of the batch with the per-set calls: forces bit-exact, gradients
+    # to float-atomic order
     g.replay()
-    per_set()
+    gp.replay()
     torch.cuda.synchronize()
-    same = all(torch.equal(F[o0:o1], Fs[k]) for k, (o0, o1) in
-               enumerate(zip(offs[:-1].tolist(), offs[1:].tolist())))
-    # accuracy of the largest set against the FP64 direct sum
+    same = all(torch.equal(F[o0:o1], Fs[k]) for k, (o0, o1) in enumerate(bounds))
+    gx_err = max(rel_l2(gx[o0:o1], GXs[k]) for k, (o0, o1) in enumerate(bounds))
+    # accuracy of the largest set against the FP64 direct sum and adjoint
     kbig = int(np.argmax(sizes))
     o0 = int(offs[kbig])
-    idx = torch.tensor(np.random.default_rng(7).choice(int(sizes[kbig]), args.accuracy_queries,
+    nb = int(sizes[kbig])
+    idx = torch.tensor(np.random.default_rng(7).choice(nb, args.accuracy_queries,
                                                        replace=False), device=dev)
-    Pb, Mb = Ps[kbig].double(), Ms[kbig].double()
+    Pb, Mb, Cb = Ps[kbig].double(), Ms[kbig].double(), Cs[kbig].double()
     ref = direct_fp64(Pb, Mb, Pb[idx], args.eps)
+    rows = idx[:args.grad_rows]
+    rgx, rgm = direct_adjoint_rows_fp64(Pb, Mb, Cb, rows, args.eps)
     st = G.new_stats(dev)
     G.gravity_force_vjp_batch(P, M, C, offs, stats=st, leaf_size=leaf, walk_blocks=blocks,
-                              workspace=ws, **kw)
+                              workspace=ws, grad_points_out=gx, grad_masses_out=gm, **kw)
     sv = G.stats_dict(st)
     stf = G.new_stats(dev)
     G.gravity_force_batch(P, M, offs, stats=stf, leaf_size=leaf, walk_blocks=blocks,
                           workspace=ws, out=F, **kw)
     sf = G.stats_dict(stf)
+    torch.cuda.synchronize()
+    acc = {"largest_set": nb,
+           "largest_set_force_rel_l2_vs_fp64_direct": rel_l2(F[o0:o0 + nb][idx], ref),
+           "largest_set_force_outlier_frac": outlier_frac(F[o0:o0 + nb][idx], ref),
+           "largest_set_grad_points_rel_l2_vs_fp64_direct_adjoint":
+               rel_l2(gx[o0:o0 + nb][rows], rgx),
+           "largest_set_grad_masses_rel_l2_vs_fp64_direct_adjoint":
+               rel_l2(gm[o0:o0 + nb][rows], rgm)}
     g.close()
+    gp.close()
     walk_vjp_ms = prof[5] / max(prof[3], 1)
     walk_fwd_ms = prof[2] / max(prof[0], 1)
-    flops_vjp = FLOPS_NEAR_VJP * sv["near"] + FLOPS_FAR_VJP[args.order] * sv["far"]
-    flops_fwd = FLOPS_NEAR_FWD * sf["near"] + FLOPS_FAR_FWD[args.order] * sf["far"]
-    achieved = flops_vjp / (walk_vjp_ms * 1e-3) / 1e12
-    cpu = None
-    if not args.no_cpu_baseline:
+    flops_vjp, flops_vjp_impl = walk_flops(sv, True, args.order)
+    rec = {"workload": f"blob_sets{args.sets}_total{n_total}_theta0.5_fwd+vjp",
+           "sets": args.sets, "total_points": n_total, "min_set": int(sizes.min()),
+           "max_set": int(sizes.max()), "softening": args.eps, "beta_diag": args.beta,
+           "max_depth": args.depth, "multipole_order": args.order,
+           "leaf_size": leaf, "walk_blocks": blocks,
+           "batched_graph_ms": ms_graph, "batched_eager_ms": ms_eager,
+           "per_set_eager_ms": ms_per_set, "per_set_graph_ms": ms_per_set_graph,
+           "speedup_vs_per_set_graph": ms_per_set_graph / ms_graph,
+           "speedup_vs_per_set_eager": ms_per_set / ms_graph,
+           "forces_per_s": n_total / (ms_graph * 1e-3),
+           "batch_forces_equal_per_set": bool(same),
+           "batch_grad_points_max_rel_l2_vs_per_set": gx_err,
+           "autotune": tuned.get("table"),
+           "autotune_gate": tuned.get("gate"),
+           "interactions_vjp": sv, "interactions_fwd": sf,
+           "accuracy": acc,
+           "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
+                         "vjp_build": prof[4] / max(prof[3], 1),
+                         "vjp_walk_kernel": walk_vjp_ms},
+           "roofline": valu_roofline(
+               f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0> (all sets)", flops_vjp,
+               flops_vjp_impl, walk_vjp_ms)}
+    if cpu:
         big_args = argparse.Namespace(**vars(args))
         big_args.leaf = leaf
-        cpu = cpu_baseline(ps[kbig], ms[kbig], cs[kbig], big_args, args.cpu_sample_groups)
-        cpu["sample"] = (f"largest set ({int(sizes[kbig])} points) only; its forces/s stands "
-                         f"for the batch. " + cpu["sample"])
+        c = cpu_baseline(ps[kbig], ms[kbig], cs[kbig], big_args, args.cpu_sample_groups)
+        c["sample"] = (f"largest set ({nb} points) only; its forces/s stands for the batch. "
+                       + c["sample"])
+        rec["cpu_baseline"] = c
+    return rec
+
+
+def bench_batched(args):
+    import torch
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    b = measure_batched(args, dev, tune=not args.no_autotune, cpu=not args.no_cpu_baseline)
     return {
-        "metric": "batched fwd+VJP forces/sec, 64 sets (graph-captured forest vs per-set calls)",
-        "value": n_total / (ms_graph * 1e-3),
+        "metric": "batched fwd+VJP forces/sec, 64 sets (graph-captured forest vs per-set "
+                  "calls)",
+        "value": b["forces_per_s"],
         "unit": "forces/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_graph,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": b["batched_graph_ms"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"blob_sets{args.sets}_total{n_total}_theta0.5_fwd+vjp",
-                   "sets": args.sets, "total_points": n_total, "min_set": int(sizes.min()),
-                   "max_set": int(sizes.max()), "softening": args.eps, "beta_diag": args.beta,
-                   "max_depth": args.depth, "multipole_order": args.order,
-                   "leaf_size": leaf, "walk_blocks": blocks},
-        "batched_graph_ms": ms_graph,
-        "batched_eager_ms": ms_eager,
-        "per_set_ms": ms_per_set,
-        "per_set_default_leaf_ms": per_set_default,
-        "speedup_batched_vs_per_set": ms_per_set / ms_graph,
-        "batch_forces_equal_per_set": bool(same),
-        "autotune": tuned.get("table"),
-        "interactions_vjp": sv,
-        "accuracy": {"largest_set_force_rel_l2_vs_fp64_direct":
-                     rel_l2(F[o0:o0 + int(sizes[kbig])][idx], ref),
-                     "largest_set": int(sizes[kbig])},
-        "phases_ms": {"fwd_build": prof[1] / max(prof[0], 1), "fwd_walk_kernel": walk_fwd_ms,
-                      "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
-                     "kernel": f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0> (all sets)",
-                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "flops_per_launch": flops_vjp,
-                     "fwd_walk_tflops": flops_fwd / (walk_fwd_ms * 1e-3) / 1e12},
-        "cpu_baseline": cpu,
+        "config": {k: b[k] for k in ("workload", "sets", "total_points", "min_set", "max_set",
+                                     "softening", "beta_diag", "max_depth", "multipole_order",
+                                     "leaf_size", "walk_blocks")},
+        "batched": {k: v for k, v in b.items() if k not in ("roofline", "cpu_baseline")},
+        "roofline": b["roofline"],
+        "cpu_baseline": b.get("cpu_baseline"),
+    }
+
+
+def uniform_cube(n, seed=1000):
+    """N points uniform in [0, 1)^3 (fp32) and equal masses 1/N."""
+    import numpy as np
+
+    p = np.random.default_rng(seed).uniform(0.0, 1.0, (n, 3)).astype(np.float32)
+    return p, np.full(n, 1.0 / n, np.float32)
+
+
+def measure_config1(args, dev, n=10000, eps=0.01, leaf=16, steps=20):
+    """Plain classic BH at N = 10,000 uniform in [0, 1)^3, masses 1/N,
+    softening 0.01, theta = 0.5, leaf size 16: the GPU pipeline (forces, and
+    forces + full VJP) against the FP64 direct sum (rel_l2, outlier fraction)
+    and the reference benchmark's FP32 numpy direct sum, with the CPU legs
+    beside it: that numpy direct sum over all N queries and the reference's
+    CPU tree path (src/sbh.cpp) restated, as it ships and like-for-like."""
+    import numpy as np
+    import torch
+
+    from sbh_amd import gravity as G
+    from sbh_amd.graph import Graph
+
+    p, m = uniform_cube(n)
+    c = np.random.default_rng(1002).normal(size=(n, 3)).astype(np.float32)
+    P, M, C = (torch.tensor(a, device=dev) for a in (p, m, c))
+    kw = dict(beta=args.beta, softening_scale=eps, max_depth=10, leaf_size=leaf,
+              multipole_order=args.order)
+    ws = G.new_workspace(n, 10, leaf, dev)
+    F, GX, GM = torch.empty_like(P), torch.empty_like(P), torch.empty_like(M)
+    stream = torch.cuda.current_stream()
+
+    def fwd():
+        G.gravity_force(P, M, workspace=ws, out=F, **kw)
+
+    def fwd_vjp():
+        fwd()
+        G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX, grad_masses_out=GM, **kw)
+
+    out = {}
+    for tag, fn in (("fwd", fwd), ("fwd_vjp", fwd_vjp)):
+        fn()
+        g = Graph()
+        g.capture(fn)
+        for _ in range(3):
+            g.replay()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            g.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out[tag + "_ms"] = e0.elapsed_time(e1) / steps
+        g.close()
+    fwd_vjp()
+    torch.cuda.synchronize()
+    ref = direct_fp64(P.double(), M.double(), P.double(), eps)
+    rgx, rgm = direct_adjoint_rows_fp64(P.double(), M.double(), C.double(),
+                                        torch.arange(n, device=dev), eps, chunk=64)
+    # the reference benchmark's oracle: its FP32 numpy direct sum, timed in full
+    t0 = time.perf_counter()
+    ref_np = numpy_direct_softcut(p, m, p, eps)
+    t_np = time.perf_counter() - t0
+    ref_np_t = torch.tensor(ref_np, device=dev)
+    from oracle import oracle as O
+
+    cpu_args = argparse.Namespace(leaf=leaf, beta=args.beta, eps=eps, depth=100)
+    lit = cpu_reference_literal(p, m, cpu_args, nsample=n)
+    lfl = cpu_reference_path(p, m, cpu_args, np.arange(n), ref.cpu().numpy(), nsample=n)
+    out.update({
+        "workload": f"uniform_cube_n{n}_eps{eps}_theta0.5_leaf{leaf}",
+        "n": n, "softening": eps, "leaf_size": leaf, "beta_diag": args.beta,
+        "multipole_order": args.order,
+        "fwd_forces_per_s": n / (out["fwd_ms"] * 1e-3),
+        "fwd_vjp_forces_per_s": n / (out["fwd_vjp_ms"] * 1e-3),
+        "launch": "HIP graph replay",
+        "force_rel_l2_vs_fp64_direct": rel_l2(F, ref),
+        "force_outlier_frac_vs_fp64_direct": outlier_frac(F, ref),
+        "force_rel_l2_vs_fp32_numpy_direct": rel_l2(F, ref_np_t),
+        "force_outlier_frac_vs_fp32_numpy_direct": outlier_frac(F, ref_np_t),
+        "fp32_numpy_direct_rel_l2_vs_fp64": rel_l2(ref_np_t, ref),
+        "grad_points_rel_l2_vs_fp64_direct_adjoint": rel_l2(GX, rgx),
+        "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(GM, rgm),
+        "cpu_numpy_direct": {"value": n / t_np, "unit": "forces/s (forward only)", "cores": 1,
+                             "seconds": round(t_np, 3),
+                             "sample": f"all {n} queries, fp32 numpy with the attenuation "
+                                       "(python/unified_force_benchmark.py:208-230)"},
+        "cpu_reference_literal": lit,
+        "cpu_reference_like_for_like": lfl,
+        "host_threads_used": O.num_threads(),
+    })
+    return out
+
+
+def numpy_direct_softcut(points, masses, queries, eps, cutoff=float("inf")):
+    """The reference benchmark's numpy direct sum (python/unified_force_benchmark
+    .py:208-230): fp32, one query at a time, attenuation erfc(u) + 2u/sqrt(pi)
+    exp(-u^2) with u = r / (2 r_c) (1 at r_c = inf), non-periodic."""
+    import numpy as np
+    from scipy import special
+
+    out = np.empty((queries.shape[0], 3), dtype=np.float32)
+    eps2 = float(eps) * float(eps)
+    for i in range(queries.shape[0]):
+        disp = queries[i][None, :] - points
+        r2 = np.sum(disp * disp, axis=1)
+        inv_r = 1.0 / np.sqrt(r2 + eps2 + 1e-12)
+        inv_r3 = inv_r * inv_r * inv_r
+        dist = np.sqrt(r2 + 1e-12)
+        u = dist / (2.0 * cutoff)
+        att = (special.erfc(u) + (2.0 / np.sqrt(np.pi)) * u * np.exp(-(u * u))).astype(
+            np.float32)
+        coeff = -masses * att * inv_r3
+        out[i] = np.sum(coeff[:, None] * disp, axis=0, dtype=np.float32)
+    return out
+
+
+def bench_config1(args):
+    import torch
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    c = measure_config1(args, dev)
+    return {
+        "metric": "classic BH forces/sec, N = 10,000 uniform cube (forward; fwd+VJP beside)",
+        "value": c["fwd_forces_per_s"], "unit": "forces/s", "n_gpus": 1, "steps": 20,
+        "warmup": 3, "ms_per_step": c["fwd_ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {k: c[k] for k in ("workload", "n", "softening", "leaf_size", "beta_diag",
+                                     "multipole_order")},
+        "config1": c,
+        "cpu_baseline": {**c["cpu_numpy_direct"], "kind": "port"},
     }
 
 

@@ -246,6 +246,51 @@ def test_graph_replay_single_set_matches_eager(gpu):
     g.close()
 
 
+def test_capture_without_workspace_is_refused(gpu):
+    # A graph must not keep the library's cached per-stream workspace: a later
+    # eager call on the same stream handle that needs a larger one frees it.
+    # Entry points given no workspace therefore refuse to run under capture
+    # (before any launch), and the captured call with a workspace replays the
+    # eager result.
+    from sbh_amd import ffi, wn
+    from sbh_amd._lib import buf, call, stream_handle
+    from sbh_amd.graph import Graph
+
+    G = _G()
+    p, m, _ = _case("plummer", 5000, 13)
+    P, M = (torch.tensor(a, device="cuda") for a in (p, m))
+    F = torch.empty_like(P)
+    Fe = G.gravity_force(P, M, softening_scale=0.005).clone()  # eager: the cache is fine
+    Nrm = torch.nn.functional.normalize(torch.randn_like(P), dim=1)
+    out = torch.empty(P.shape[0], device="cuda")
+    lo = (P.min(0).values - 1e-3).contiguous()
+    hi = (P.max(0).values + 1e-3).contiguous()
+    leaf_size, node_size, max_nodes = ffi._get_octree_level3_layout()
+    leaf = torch.empty(P.shape[0] * leaf_size, dtype=torch.uint8, device="cuda")
+    node = torch.empty(max_nodes * node_size, dtype=torch.uint8, device="cuda")
+    contrib = torch.empty(max_nodes * leaf_size, dtype=torch.uint8, device="cuda")
+    calls = {
+        "gravity": lambda: G.gravity_force(P, M, softening_scale=0.005, out=F),
+        "winding_number": lambda: wn.winding_number(P, Nrm, M, P, out=out),
+        "level3_builder": lambda: call("sbh_build_octree_level3_buffers", stream_handle(),
+                                       buf(P), buf(Nrm), buf(M), buf(lo), buf(hi), buf(leaf),
+                                       buf(node), buf(contrib)),
+    }
+    for name, fn in calls.items():
+        g = Graph()
+        with pytest.raises(ValueError, match="workspace"):
+            g.capture(fn)
+        assert not g.captured, name
+    ws = G.new_workspace(P.shape[0], 10, 32)
+    g = Graph()
+    g.capture(lambda: G.gravity_force(P, M, softening_scale=0.005, out=F, workspace=ws))
+    F.zero_()
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.equal(F, Fe)
+    g.close()
+
+
 def _direct_adjoint_rows(P, M, C, rows, eps):
     """FP64 direct-sum gradients of L = sum_i c_i . F_i with respect to the
     positions and masses of the points `rows` (closed form of the softened
