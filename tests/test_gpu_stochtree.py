@@ -8,8 +8,13 @@
   ~1e-5 relative: tolerance rel_l2 <= 2e-4.
 - Unbiasedness at the benchmark's tree parameters (max depth 10, leaf 32,
   one sample per subdomain, theta 0.5): the mean over 256 seeds matches the
-  FP64 direct sum by a z-test (force and its query-side VJP; winding numbers
-  with 32 samples against the FP64 dipole sum).
+  FP64 direct sum within 3 standard errors (force and its query-side VJP;
+  winding numbers with 32 samples against the FP64 dipole sum).  The
+  per-query distributions are heavy-tailed at depth 10 (single seeds reach
+  50 standard deviations from the mean; oracle study on the same inputs), so
+  256 draws underestimate their own spread: the standard error of the
+  256-seed mean is taken from 4096 seeds, which also get a z-test of their
+  own mean.
 """
 import numpy as np
 import pytest
@@ -118,7 +123,7 @@ def test_unbiased_over_256_seeds_depth10_leaf32(gpu, order):
     # theta 0.5 (beta 2), max depth 10, leaf 32, one sample per subdomain;
     # 512 of the points as queries (the self pair has zero force)
     S = _S()
-    n, eps = 1 << 17, 0.005
+    n, eps, nseeds, batch = 1 << 17, 0.005, 4096, 256
     p = _plummer(n, 46)
     m = np.full(n, 1.0 / n, np.float32)
     rng = np.random.default_rng(47)
@@ -128,19 +133,42 @@ def test_unbiased_over_256_seeds_depth10_leaf32(gpu, order):
     kw = dict(beta=2.0, samples_per_subdomain=1, softening_scale=eps, max_depth=10,
               leaf_size=32, multipole_order=order)
     ws = S.new_workspace(n, q.shape[0], 10, 32)
-    fs, gs = [], []
-    for seed in range(256):
-        fs.append(_np(S.stochastic_gravity_force(P, M, Q, seed=seed, workspace=ws, **kw)))
-        gs.append(_np(S.stochastic_gravity_force_vjp(P, M, Q, C, seed=seed, workspace=ws,
-                                                      **kw)))
-    ref_f = O.direct_force_soft(p, m, q, eps, np.inf)
-    ref_g = O.direct_force_soft_vjp(p, m, q, c, eps, np.inf)
-    for name, a, ref in (("force", fs, ref_f), ("vjp", gs, ref_g)):
-        rms, frac, err = _ztest(a, ref)
-        one = O.rel_l2(a[0], ref)
-        print(f"order {order} {name}: one seed rel_l2 {one:.2e}, 256-seed mean rel_l2 "
-              f"{err:.2e}, rms z {rms:.3f}, |z| < 3 for {frac:.3f}")
-        assert frac > 0.95 and rms < 1.5, name
+    ref = {"force": O.direct_force_soft(p, m, q, eps, np.inf),
+           "vjp": O.direct_force_soft_vjp(p, m, q, c, eps, np.inf)}
+    # running sums of the deviations from the FP64 target, in FP64 on the device
+    dev = {k: torch.tensor(v, device="cuda", dtype=torch.float64) for k, v in ref.items()}
+    s1 = {k: torch.zeros_like(v) for k, v in dev.items()}
+    s2 = {k: torch.zeros_like(v) for k, v in dev.items()}
+    first = {}
+    one = {}
+    for seed in range(nseeds):
+        got = {"force": S.stochastic_gravity_force(P, M, Q, seed=seed, workspace=ws, **kw),
+               "vjp": S.stochastic_gravity_force_vjp(P, M, Q, C, seed=seed, workspace=ws, **kw)}
+        for k, g in got.items():
+            d = g.double() - dev[k]
+            s1[k] += d
+            s2[k] += d * d
+            if seed == 0:
+                one[k] = _np(g)
+        if seed == batch - 1:
+            first = {k: _np(v) / batch for k, v in s1.items()}
+    for k in ("force", "vjp"):
+        t1, t2 = _np(s1[k]), _np(s2[k])
+        sd = np.sqrt(np.maximum(t2 - t1 * t1 / nseeds, 0.0) / (nseeds - 1))
+        ok = sd > 1e-12 * np.abs(ref[k]).max()
+        z_all = (t1 / nseeds)[ok] / (sd[ok] / np.sqrt(nseeds))
+        z_256 = first[k][ok] / (sd[ok] / np.sqrt(batch))
+        rms_all = float(np.sqrt(np.mean(z_all ** 2)))
+        frac_all = float(np.mean(np.abs(z_all) < 3))
+        frac_256 = float(np.mean(np.abs(z_256) < 3))
+        print(f"order {order} {k}: one seed rel_l2 {O.rel_l2(one[k], ref[k]):.2e}, "
+              f"256-seed mean rel_l2 {O.rel_l2(first[k] + ref[k], ref[k]):.2e}, |z| < 3 for "
+              f"{frac_256:.4f}; {nseeds}-seed mean: rms z {rms_all:.3f}, |z| < 3 for "
+              f"{frac_all:.4f}")
+        # a normal z is inside 3 for 99.7% of the components; the oracle's
+        # 4096 seeds on these inputs give 0.991-0.995 (heavy tails converge
+        # slowly) and 0.989-0.999 for each of sixteen 256-seed means
+        assert frac_256 > 0.98 and frac_all > 0.98 and rms_all < 1.2, k
 
 
 def test_winding_number_unbiased_32_samples(gpu):
