@@ -599,6 +599,29 @@ int sbh_stochastic_winding_number(void* stream, const SbhBuffer* points,
                                   int32_t samples_per_subdomain, int32_t seed,
                                   SbhBuffer* workspace, SbhBuffer* stats);
 
+/* ---- allocation-free gravity VJP -----------------------------------------
+ *
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): sbh_gravity_force_vjp with the same arguments except
+ * the workspace, which is a library-owned static device array of
+ * sbh_gravity_noalloc_scratch_bytes() bytes (64 MiB) per device, held until
+ * the process ends.  No call allocates or frees device memory.  Problems whose
+ * sbh_gravity_workspace_bytes(n, max_depth, leaf_size) exceeds that size are
+ * rejected with SBH_INVALID_ARGUMENT (at max_depth 10: n <= 230,999 at leaf
+ * size 32, n <= 143,020 at leaf size 16).
+ * Every call on one device shares that device's array: calls that overlap
+ * on one device (different streams or host threads) race; calls queued on
+ * one stream are ordered.  Results equal sbh_gravity_force_vjp's. */
+int64_t sbh_gravity_noalloc_scratch_bytes(void);
+int sbh_gravity_force_vjp_noalloc(void* stream, const SbhBuffer* points,
+                                  const SbhBuffer* masses, const SbhBuffer* cotangent,
+                                  SbhBuffer* forces, SbhBuffer* grad_points,
+                                  SbhBuffer* grad_masses, float beta,
+                                  float softening_scale, int32_t max_depth,
+                                  int32_t leaf_size, int32_t multipole_order,
+                                  SbhBuffer* stats);
+
 /* HIP graph capture of library calls on `stream` (no reference counterpart;
  * XLA captures its own graphs).  Between sbh_graph_begin and sbh_graph_end
  * only calls with caller-provided workspaces may be made.  sbh_graph_end

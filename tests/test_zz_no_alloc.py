@@ -1,0 +1,93 @@
+"""GPU: the allocation-free gravity VJP entry point against the workspace one.
+
+sbh_gravity_force_vjp_noalloc runs the same build and adjoint walk as
+sbh_gravity_force_vjp in a library-owned static device array.  Both are
+called through the C ABI on the same inputs, twice in a row (the second call
+smaller, so the array still holds the first call's tree): forces must be
+bit-equal (the forward sums are deterministic), both gradients within
+rel_l2 1e-5 (their float-atomic adjoint sums are reassociated from run to
+run), and the interaction statistics equal.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+KW = dict(beta=2.0, softening_scale=0.01, max_depth=10, leaf_size=16, multipole_order=2)
+
+
+def _inputs(n, seed):
+    rng = np.random.default_rng(seed)
+    p = torch.tensor(rng.normal(0.0, 1.0, (n, 3)).astype(np.float32), device="cuda")
+    m = torch.tensor((rng.uniform(0.5, 1.5, n) / n).astype(np.float32), device="cuda")
+    c = torch.tensor(rng.normal(size=(n, 3)).astype(np.float32), device="cuda")
+    return p, m, c
+
+
+def _noalloc(p, m, c):
+    from sbh_amd._lib import buf, call, f32, i32, stream_handle
+
+    F, gx, gm = torch.empty_like(p), torch.empty_like(p), torch.empty_like(m)
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    call("sbh_gravity_force_vjp_noalloc", stream_handle(), buf(p), buf(m), buf(c), buf(F),
+         buf(gx), buf(gm), f32(KW["beta"]), f32(KW["softening_scale"]), i32(KW["max_depth"]),
+         i32(KW["leaf_size"]), i32(KW["multipole_order"]), buf(st))
+    torch.cuda.synchronize()
+    return F, gx, gm, st
+
+
+def _with_workspace(p, m, c):
+    from sbh_amd._lib import buf, call, f32, i32, stream_handle
+    from sbh_amd import gravity as G
+
+    n = p.shape[0]
+    ws = G.new_workspace(n, KW["max_depth"], KW["leaf_size"])
+    F, gx, gm = torch.empty_like(p), torch.empty_like(p), torch.empty_like(m)
+    st = torch.zeros(8, dtype=torch.int64, device="cuda")
+    call("sbh_gravity_force_vjp", stream_handle(), buf(p), buf(m), buf(c), buf(F), buf(gx),
+         buf(gm), f32(KW["beta"]), f32(KW["softening_scale"]), i32(KW["max_depth"]),
+         i32(KW["leaf_size"]), i32(KW["multipole_order"]), buf(ws), buf(st))
+    torch.cuda.synchronize()
+    return F, gx, gm, st
+
+
+def test_noalloc_vjp_matches_workspace_entry_twice(gpu):
+    from sbh_amd._lib import load_library
+
+    lib = load_library()
+    lib.sbh_gravity_noalloc_scratch_bytes.restype = ctypes.c_int64
+    lib.sbh_gravity_workspace_bytes.restype = ctypes.c_int64
+    cap = lib.sbh_gravity_noalloc_scratch_bytes()
+    for n, seed in ((40000, 11), (7001, 12)):
+        need = lib.sbh_gravity_workspace_bytes(ctypes.c_int64(n), KW["max_depth"], KW["leaf_size"])
+        assert 0 < need <= cap, (n, need, cap)
+        print(f"case: noalloc vs workspace entry, n={n}, seed={seed}", flush=True)
+        p, m, c = _inputs(n, seed)
+        Fa, gxa, gma, sta = _noalloc(p, m, c)
+        Fb, gxb, gmb, stb = _with_workspace(p, m, c)
+        assert torch.equal(Fa, Fb)
+        assert torch.equal(sta, stb)
+        assert int(sta[5]) == 0
+        assert O.rel_l2(gxa.cpu().numpy(), gxb.cpu().numpy()) < 1e-5
+        assert O.rel_l2(gma.cpu().numpy(), gmb.cpu().numpy()) < 1e-5
+
+
+def test_noalloc_rejects_too_large(gpu):
+    from sbh_amd._lib import load_library
+
+    lib = load_library()
+    lib.sbh_gravity_noalloc_scratch_bytes.restype = ctypes.c_int64
+    lib.sbh_gravity_workspace_bytes.restype = ctypes.c_int64
+    cap = lib.sbh_gravity_noalloc_scratch_bytes()
+    n = 1 << 17
+    while lib.sbh_gravity_workspace_bytes(ctypes.c_int64(n), KW["max_depth"],
+                                          KW["leaf_size"]) <= cap:
+        n *= 2
+    p, m, c = _inputs(n, 13)
+    with pytest.raises(ValueError, match="allocation-free scratch"):
+        _noalloc(p, m, c)
