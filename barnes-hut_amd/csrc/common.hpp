@@ -159,6 +159,18 @@ int exclusive_scan(hipStream_t stream, uint32_t* data, int64_t n, void* temp);
 int radix_sort_pairs(hipStream_t stream, uint32_t* keys_in, uint32_t* vals_in,
                      uint32_t* keys_out, uint32_t* vals_out, int64_t n,
                      int key_bits, void* temp);
+// Stable order of n rows by (set, code, row) in 2 + ceil(key_bits / 8) +
+// set-byte launches (onesweep.hip): codes (n, input order, key_bits wide),
+// set_offs the nsets + 1 row offsets of the sets (contiguous row ranges).
+// keys_a/vals_a and keys_b/vals_b are ping-pong buffers of n entries; *keys
+// and *perm point at the ones holding the sorted codes and rows.  temp holds
+// morton_sort_temp_bytes(n, nsets).  A look-back that finds a missing
+// predecessor (it cannot happen) sets bit 16 of *flag.
+size_t morton_sort_temp_bytes(int64_t n, int nsets);
+int morton_sort(hipStream_t stream, const uint32_t* codes, int64_t n, int key_bits,
+                const int32_t* set_offs, int nsets, uint32_t* keys_a, uint32_t* vals_a,
+                uint32_t* keys_b, uint32_t* vals_b, void* temp, int32_t* flag,
+                const uint32_t** keys, const uint32_t** perm);
 
 // ---------------------------------------------------------------- device math
 // Min-image as include/kernel.h:27-39 (floor(x + 0.5)); not contracted so

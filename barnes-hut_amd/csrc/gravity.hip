@@ -227,6 +227,8 @@ extern "C" int sbh_profile_collect(double* out, int32_t count) {
 // ---------------------------------------------------------------- batches
 namespace {
 
+bool walk_threads_ok(int t) { return t == 0 || t == 64 || t == 128 || t == 256 || t == 512; }
+
 int check_batch(const SbhBuffer* points, const SbhBuffer* set_offsets, int* nsets) {
   if (!buffer_ok(set_offsets, SBH_DTYPE_S32, 1) || set_offsets->dims[0] < 2)
     return fail_invalid("set_offsets must be an int32 vector of num_sets + 1 entries");
@@ -251,7 +253,8 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
                                        SbhBuffer* forces, float beta, float softening_scale,
                                        int32_t max_depth, int32_t leaf_size,
                                        int32_t multipole_order, int32_t walk_blocks,
-                                       SbhBuffer* workspace, SbhBuffer* stats) {
+                                       int32_t walk_threads, SbhBuffer* workspace,
+                                       SbhBuffer* stats) {
   clear_error();
   int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
                         multipole_order);
@@ -263,6 +266,8 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
     return fail_invalid("forces shape mismatch");
   if ((rc = check_stats(stats)) != SBH_OK) return rc;
   if (walk_blocks < 0) return fail_invalid("walk_blocks must be >= 0");
+  if (!walk_threads_ok(walk_threads))
+    return fail_invalid("walk_threads must be 0, 64, 128, 256 or 512");
   hipStream_t s = static_cast<hipStream_t>(stream);
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
@@ -275,7 +280,7 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
   if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
   rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
                   static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
-                  pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks);
+                  pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks, walk_threads);
   if (rc != SBH_OK) return rc;
   if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));
@@ -288,8 +293,8 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
                                            SbhBuffer* grad_points, SbhBuffer* grad_masses,
                                            float beta, float softening_scale, int32_t max_depth,
                                            int32_t leaf_size, int32_t multipole_order,
-                                           int32_t walk_blocks, SbhBuffer* workspace,
-                                           SbhBuffer* stats) {
+                                           int32_t walk_blocks, int32_t walk_threads,
+                                           SbhBuffer* workspace, SbhBuffer* stats) {
   clear_error();
   int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
                         multipole_order);
@@ -310,6 +315,8 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
     return fail_invalid("grad_masses shape mismatch");
   if ((rc = check_stats(stats)) != SBH_OK) return rc;
   if (walk_blocks < 0) return fail_invalid("walk_blocks must be >= 0");
+  if (!walk_threads_ok(walk_threads))
+    return fail_invalid("walk_threads must be 0, 64, 128, 256 or 512");
   hipStream_t s = static_cast<hipStream_t>(stream);
   tree::Layout L = tree::layout(n, max_depth, leaf_size, nsets);
   char* ws = nullptr;
@@ -324,7 +331,7 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
   rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
-                  stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks);
+                  stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks, walk_threads);
   if (rc != SBH_OK) return rc;
   if (n > 0) pr.commit();
   if (stats != nullptr) return tree::copy_stats(s, ws, L, static_cast<int64_t*>(stats->data));

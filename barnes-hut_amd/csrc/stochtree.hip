@@ -432,14 +432,12 @@ int run_stoch(int kind, bool quad, hipStream_t s, const float* pts, const float*
     if (rc != SBH_OK) return rc;
   }
   prof::Scope pr(s, 3);
-  int rc = build(s, ws, W.t, pts, ms, normals, n, max_depth, leaf_size, nullptr, kBuildMoments);
+  auto* wd = reinterpret_cast<float4*>(ws + W.wd);
+  int rc = build(s, ws, W.t, pts, ms, normals, n, max_depth, leaf_size, nullptr, kBuildMoments,
+                 kind == kSWn ? wd : nullptr);
   if (rc != SBH_OK) return rc;
   auto* hdr = reinterpret_cast<Header*>(ws + W.t.header);
-  auto* wd = reinterpret_cast<float4*>(ws + W.wd);
   auto* subs = reinterpret_cast<int32_t*>(ws + W.subs);
-  if (kind == kSWn) {
-    if ((rc = wn_dipoles(s, ws, W.t, wd, n, max_depth, leaf_size)) != SBH_OK) return rc;
-  }
   k_subdomains<<<1, 64, 0, s>>>(hdr, reinterpret_cast<const int4*>(ws + W.t.topo),
                                 reinterpret_cast<const float4*>(ws + W.t.mp), subs);
   SBH_LAUNCH_CHECK();

@@ -3,10 +3,13 @@
 // Build (all on the device, no host synchronisation):
 //   k_bbox_*      data bounds (min - 1e-3, max + 1e-3, as python/jax_sbh_ffi.py:879-884)
 //   k_codes       D-level Morton codes with the reference quantiser (src/sbh_ffi.cu:162-179)
-//   radix sort    stable (code, index) order
+//   morton_sort   stable (set, code, row) order (onesweep.hip)
 //   k_gather      sorted (x, y, z, m) and cotangent records
-//   per level:    k_split (children per node) -> k_scan_level -> k_emit (child records)
-//   per level:    k_moments bottom-up (mass, centre of mass, central second moment)
+//   k_topo_*      every node and query group from the sorted codes: count ->
+//                 scan -> emit (first and last points of each cell, topo.hpp)
+//   k_topo_finalize  record counts, group sizes, leaf moments (mass, centre of
+//                 mass, central second moment), one wavefront per node
+//   per level:    k_moments_internal bottom-up over the internal nodes
 // Walk (one wave = one group of 64 consecutive sorted points):
 //   a node is used as a multipole for the whole group iff the distance from its
 //   cell centre to the group's bounding box is >= beta * cell diagonal (the
@@ -22,6 +25,7 @@
 #include "common.hpp"
 #include "tree.hpp"
 #include "groupwalk.hpp"
+#include "topo.hpp"
 
 namespace sbh {
 namespace tree {
@@ -277,272 +281,491 @@ __global__ __launch_bounds__(kB) void k_forest_init(
 __global__ __launch_bounds__(kB) void k_codes(const float* __restrict__ pts, int64_t n,
                                               const CubeGeom* __restrict__ geoms,
                                               const int32_t* __restrict__ offs, int nsets,
-                                              int bits, uint32_t* __restrict__ codes_in,
-                                              uint32_t* __restrict__ keys,
-                                              uint32_t* __restrict__ idx) {
+                                              int bits, uint32_t* __restrict__ codes_in) {
   int64_t i = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (i >= n) return;
   const CubeGeom g = geoms[nsets > 1 ? set_of(offs, nsets, static_cast<int>(i)) : 0];
   uint32_t qx = quantize_axis(pts[3 * i + 0], g.min_x, g.span, bits);
   uint32_t qy = quantize_axis(pts[3 * i + 1], g.min_y, g.span, bits);
   uint32_t qz = quantize_axis(pts[3 * i + 2], g.min_z, g.span, bits);
-  const uint32_t c = morton3(qx, qy, qz);
-  codes_in[i] = c;
-  keys[i] = c;
-  idx[i] = static_cast<uint32_t>(i);
+  codes_in[i] = morton3(qx, qy, qz);
 }
 
-// Second sort pass of a forest: (set of the point, current order).
-__global__ __launch_bounds__(kB) void k_setkeys(const uint32_t* __restrict__ perm, int64_t n,
-                                                const int32_t* __restrict__ offs, int nsets,
-                                                uint32_t* __restrict__ keys,
-                                                uint32_t* __restrict__ vals) {
-  int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
-  if (r >= n) return;
-  const uint32_t i = perm[r];
-  keys[r] = static_cast<uint32_t>(set_of(offs, nsets, static_cast<int>(i)));
-  vals[r] = i;
-}
-
+// Sorted records: row r of the sorted order is input row order[r] with code
+// sorted[r]; writes the permutation and the codes where the sort did not
+// leave them, (x, y, z, m) and the cotangents.
 __global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
                                                const float* __restrict__ ms,
                                                const float* __restrict__ cot, int64_t n,
-                                               const uint32_t* __restrict__ perm,
-                                               const uint32_t* __restrict__ codes_in,
-                                               uint32_t* __restrict__ codes,
-                                               float4* __restrict__ sp, float4* __restrict__ sc) {
+                                               const uint32_t* order,
+                                               const uint32_t* sorted, uint32_t* perm,
+                                               uint32_t* codes, float4* __restrict__ sp,
+                                               float4* __restrict__ sc) {
   int64_t r = static_cast<int64_t>(blockIdx.x) * kB + threadIdx.x;
   if (r >= n) return;
-  int64_t i = perm[r];
-  codes[r] = codes_in[i];
+  const uint32_t o = order[r];
+  const int64_t i = o;
+  if (perm != order) perm[r] = o;
+  if (codes != sorted) codes[r] = sorted[r];
   sp[r] = make_float4(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2], ms != nullptr ? ms[i] : 0.0f);
   if (cot != nullptr) sc[r] = make_float4(cot[3 * i], cot[3 * i + 1], cot[3 * i + 2], 0.0f);
 }
 
 // ------------------------------------------------------------------ topology
-// Octant boundaries of node k's range at level l+1: b[0..8].  The seven
-// lower-bound searches run in lockstep (seven independent loads per step),
-// so the dependent-load chain is log2(range) long instead of seven times it.
-__device__ __forceinline__ void octant_bounds(const uint32_t* __restrict__ codes, int4 t,
-                                              uint32_t pre, int shift, int* b) {
-  const int s = t.x, e = t.x + t.y;
-  int lo[7], hi[7];
-#pragma unroll
-  for (int c = 0; c < 7; ++c) {
-    lo[c] = s;
-    hi[c] = e;
-  }
-  for (int len = e - s; len > 0; len >>= 1) {
-    // all seven probes are issued before any is compared (a finished search
-    // re-reads a valid element and ignores it)
-    int mid[7];
-    uint32_t v[7];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) mid[c] = min((lo[c] + hi[c]) >> 1, e - 1);
-#pragma unroll
-    for (int c = 0; c < 7; ++c) v[c] = codes[mid[c]];
-#pragma unroll
-    for (int c = 0; c < 7; ++c) {
-      const uint32_t key = ((pre << 3) | static_cast<uint32_t>(c + 1)) << shift;
-      const bool active = lo[c] < hi[c];
-      const bool below = v[c] < key;
-      lo[c] = (active && below) ? mid[c] + 1 : lo[c];
-      hi[c] = (active && !below) ? mid[c] : hi[c];
-    }
-  }
-  b[0] = s;
-  b[8] = e;
-#pragma unroll
-  for (int c = 0; c < 7; ++c) b[c + 1] = lo[c];
+// One pass over the sorted points (topo.hpp): point r begins the nodes of
+// levels first..top and possibly a group cell.  The node index of (l, r) is
+// level_start[l] + rank_l(r) (breadth-first, Morton order inside a level).
+// A node's END is written by its last point e: e closes the cells of levels
+// > dn(e) (the digits e shares with e + 1), and the node at level l <= lambda(e)
+// holding e has index level_start[l] + rank_l(e) + begins(e, l) - 1, where
+// lambda(e) = the deepest level of a node holding e = top of the last point
+// <= e that begins a node (or 0 at a set whose root is a leaf).  The same
+// point names the last child of each split node it closes.  Groups: a point
+// starts a group iff it lies a multiple of kGroupMax after the first point of
+// its group cell, the last group-cell first point <= r.  Kernels:
+//   k_topo_count     per tile: level begin counts, group starts after the
+//                    tile's first group cell, carries; a packed per-point record
+//   k_topo_scan      one block: per-tile level offsets, group offsets and the
+//                    carries (last group cell, last lambda) entering each tile;
+//                    the level table and the totals
+//   k_topo_emit      node records from their first points, node ends and last
+//                    children from their last points, group starts
+//   k_topo_finalize  counts and child counts of the records, group sizes, and
+//                    the moments of every leaf (one wavefront per node)
+constexpr int kTopoTile = 1024;  // one point per thread, one tile per block
+constexpr int kTopoWaves = kTopoTile / 64;
+constexpr int kTopoCols = topo::kLevels;
+// columns of the per-tile counters (1..10: level begins)
+constexpr int kColGroups = 0;   // count: group starts after the first group cell; scan: offset
+constexpr int kColFirstG = 11;  // count: first group-cell first point in the tile, or -1
+constexpr int kColLastG = 12;   // count: last one, or -1; scan: the carry entering the tile
+constexpr int kColLamPos = 13;  // count: last lambda source (a node-beginning point or a set start)
+constexpr int kColLamTop = 14;  // count: its top; scan: lambda entering the tile (-1: none)
+
+// Per-point record of k_topo_count read back by k_topo_emit:
+// first | (top + 1) << 4 | (group level + 1) << 8.
+__device__ __forceinline__ uint32_t pack_point(const topo::PointNodes& p, int gl) {
+  return static_cast<uint32_t>(p.first) | static_cast<uint32_t>(p.top + 1) << 4 |
+         static_cast<uint32_t>(gl + 1) << 8;
 }
 
-__global__ __launch_bounds__(kB) void k_split(Header* __restrict__ hdr, int level, int max_depth,
-                                              int leaf_size, const uint32_t* __restrict__ codes,
-                                              int4* __restrict__ topo,
-                                              const uint32_t* __restrict__ prefix,
-                                              uint32_t* __restrict__ nch) {
-  const int start = hdr->level_start[level];
-  const int count = hdr->level_count[level];
-  const int shift = 3 * (max_depth - level - 1);
-  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
-    int k = start + u;
-    int4 t = topo[k];
+struct SetRange {
+  int set;
+  int64_t sb, se;
+};
+
+__device__ __forceinline__ SetRange set_range(const int32_t* __restrict__ offs, int nsets,
+                                              int64_t r) {
+  SetRange q;
+  q.set = nsets > 1 ? set_of(offs, nsets, static_cast<int>(r)) : 0;
+  q.sb = offs[q.set];
+  q.se = offs[q.set + 1];
+  return q;
+}
+
+// Value of the last thread of the block at or before this one with `has`
+// set, else `fallback`.  All threads of the block must call it.
+__device__ __forceinline__ int block_last(bool has, int v, int fallback, int* s_w, int lane,
+                                          int w) {
+  const uint64_t b = __ballot(has);
+  const uint64_t le = b & (lane == 63 ? ~0ull : ((2ull << lane) - 1ull));
+  // every lane takes part in both shuffles (sources are lanes with `has`)
+  const int src = le != 0ull ? 63 - __clzll(static_cast<long long>(le)) : lane;
+  const int got = __shfl(v, src, 64);
+  int mine = le != 0ull ? got : fallback;
+  const int wsrc = b != 0ull ? 63 - __clzll(static_cast<long long>(b)) : 0;
+  const int wlast = __shfl(v, wsrc, 64);
+  if (lane == 0) s_w[2 * w] = b != 0ull ? 1 : 0;
+  if (lane == 0) s_w[2 * w + 1] = wlast;
+  __syncthreads();
+  if (le == 0ull) {
+    for (int k = w - 1; k >= 0; --k)
+      if (s_w[2 * k]) {
+        mine = s_w[2 * k + 1];
+        break;
+      }
+  }
+  __syncthreads();  // s_w is reused by the next call
+  return mine;
+}
+
+__global__ __launch_bounds__(kTopoTile) void k_topo_count(const Header* __restrict__ hdr,
+                                                          int64_t n,
+                                                   const uint32_t* __restrict__ codes,
+                                                   const int32_t* __restrict__ offs, int nsets,
+                                                   int D, int leaf_size,
+                                                   int32_t* __restrict__ tiles,
+                                                   uint32_t* __restrict__ prec) {
+  __shared__ int s_cnt[kTopoWaves][kTopoCols];
+  __shared__ int s_w[2 * kTopoWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kTopoTile + threadIdx.x;
+  const bool ok = r < n && (hdr->overflow & 2) == 0;
+  topo::PointNodes p{0, -1, 0, -1};
+  int gl = -1;
+  bool sb = false;
+  if (ok) {
+    const SetRange q = set_range(offs, nsets, r);
+    p = topo::point_nodes(codes, r, q.sb, q.se, D, leaf_size);
+    gl = topo::group_cell_level(codes, r, q.sb, q.se, D, p);
+    sb = r == q.sb;
+    prec[r] = pack_point(p, gl);
+  }
+  const int ri = static_cast<int>(r);
+  const bool has_g = ok && gl >= 0;
+  const int gs = block_last(has_g, ri, -1, s_w, lane, w);
+  const bool gstart = ok && gs >= 0 && ((ri - gs) % kGroupMax) == 0;
+  const bool lam_src = ok && (p.first > 0 || sb);
+  // tile summary: first / last group cell, last lambda source (thread 255 sees all)
+  const int lastg = block_last(has_g, ri, -1, s_w, lane, w);
+  const int lampos = block_last(lam_src, ri, -1, s_w, lane, w);
+  const int lamtop = block_last(lam_src, p.first > 0 ? p.top : 0, -1, s_w, lane, w);
+  const uint64_t bg = __ballot(has_g);
+  const int fsrc = bg != 0ull ? __ffsll(static_cast<long long>(bg)) - 1 : 0;
+  const int fval = __shfl(ri, fsrc, 64);
+  if (lane == 0) s_cnt[w][kColFirstG] = bg != 0ull ? fval : -1;
+  const int cg = __popcll(__ballot(gstart));
+  if (lane == 0) s_cnt[w][kColGroups] = cg;
+  for (int l = 1; l <= 10; ++l) {
+    const int c = __popcll(__ballot(ok && l <= D && p.first > 0 && l >= p.first && l <= p.top));
+    if (lane == 0) s_cnt[w][l] = c;
+  }
+  __syncthreads();
+  int32_t* out = tiles + static_cast<int64_t>(blockIdx.x) * kTopoCols;
+  if (threadIdx.x <= 10) {
     int c = 0;
-    if (level < max_depth && t.y > leaf_size) {
-      int b[9];
-      octant_bounds(codes, t, prefix[k], shift, b);
-      for (int o = 0; o < 8; ++o) c += (b[o + 1] > b[o]) ? 1 : 0;
-    }
-    nch[u] = static_cast<uint32_t>(c);
+    for (int k = 0; k < kTopoWaves; ++k) c += s_cnt[k][threadIdx.x];
+    out[threadIdx.x] = c;
   }
-}
-
-// In-place exclusive scan of count u32 values by one block of 1024 threads
-// (tiles of 4096: 4 consecutive values per thread, wave scans, a scan of the
-// 16 wave totals); returns the total to every thread.
-__device__ uint32_t block_exclusive_scan(uint32_t* __restrict__ data, int count) {
-  constexpr int kPer = 4;
-  __shared__ uint32_t wsum[16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  uint32_t carry = 0;
-  for (int base = 0; base < count; base += 1024 * kPer) {
-    const int i0 = base + kPer * static_cast<int>(threadIdx.x);
-    uint32_t v[kPer];
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) v[e] = i0 + e < count ? data[i0 + e] : 0u;
-    uint32_t local = 0;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) local += v[e];
-    const uint32_t incl = static_cast<uint32_t>(wave_incl_scan(static_cast<int>(local), lane));
-    if (lane == 63) wsum[wid] = incl;
-    __syncthreads();
-    if (wid == 0) {
-      const int w = lane < 16 ? static_cast<int>(wsum[lane]) : 0;
-      const int wi = wave_incl_scan(w, lane);
-      if (lane < 16) wsum[lane] = static_cast<uint32_t>(wi);
-    }
-    __syncthreads();
-    uint32_t run = carry + (wid > 0 ? wsum[wid - 1] : 0u) + incl - local;
-#pragma unroll
-    for (int e = 0; e < kPer; ++e) {
-      if (i0 + e < count) data[i0 + e] = run;
-      run += v[e];
-    }
-    carry += wsum[15];
-    __syncthreads();  // wsum is rewritten by the next tile
-  }
-  return carry;
-}
-
-// Single-block exclusive scan of the level's child counts; sets the next level.
-__global__ __launch_bounds__(1024) void k_scan_level(Header* __restrict__ hdr, int level,
-                                                     uint32_t* __restrict__ nch, int cap) {
-  const int count = hdr->level_count[level];
-  const uint32_t carry = block_exclusive_scan(nch, count);
   if (threadIdx.x == 0) {
-    int next_start = hdr->level_start[level] + count;
-    int total = static_cast<int>(carry);
-    if (next_start + total > cap) {
-      hdr->overflow = 1;
-      total = 0;
-    }
-    hdr->level_start[level + 1] = next_start;
-    hdr->level_count[level + 1] = total;
-    hdr->num_nodes = next_start + total;
+    int f = -1;
+    for (int k = 0; k < kTopoWaves && f < 0; ++k) f = s_cnt[k][kColFirstG];
+    out[kColFirstG] = f;
+  }
+  if (threadIdx.x == kTopoTile - 1) {
+    out[kColLastG] = lastg;
+    out[kColLamPos] = lampos;
+    out[kColLamTop] = lamtop;
+    out[15] = 0;
   }
 }
 
-__global__ __launch_bounds__(kB) void k_emit(Header* __restrict__ hdr, int level, int max_depth,
-                                             int leaf_size, const uint32_t* __restrict__ codes,
-                                             int4* __restrict__ topo, uint32_t* __restrict__ prefix,
-                                             float4* __restrict__ geo,
-                                             int32_t* __restrict__ parent,
-                                             const uint32_t* __restrict__ scan,
-                                             const CubeGeom* __restrict__ geoms,
-                                             const int32_t* __restrict__ offs, int nsets) {
-  const int start = hdr->level_start[level];
-  const int count = hdr->level_count[level];
-  const int next = hdr->level_start[level + 1];
-  const int next_count = hdr->level_count[level + 1];
-  const int shift = 3 * (max_depth - level - 1);
-  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
-    int k = start + u;
-    int4 t = topo[k];
-    if (!(level < max_depth && t.y > leaf_size) || next_count == 0) {
-      t.z = -1;
-      t.w = 0;
-      topo[k] = t;
-      continue;
-    }
-    uint32_t pre = prefix[k];
-    const CubeGeom g = geoms[nsets > 1 ? set_of(offs, nsets, t.x) : 0];
-    int b[9];
-    octant_bounds(codes, t, pre, shift, b);
-    int first = next + static_cast<int>(scan[u]);
-    int r = 0;
-    for (int o = 0; o < 8; ++o) {
-      if (b[o + 1] <= b[o]) continue;
-      int ch = first + r;
-      uint32_t cp = (pre << 3) | static_cast<uint32_t>(o);
-      topo[ch] = make_int4(b[o], b[o + 1] - b[o], -1, 0);
-      prefix[ch] = cp;
-      parent[ch] = k;
-      geo[ch] = cell_geometry(g, cp, level + 1);
-      ++r;
-    }
-    t.z = first;
-    t.w = r;
-    topo[k] = t;
-  }
+// Group starts in [t0, t1) of a group cell whose first point is g (-1: none).
+__device__ __forceinline__ int64_t lead_groups(int64_t g, int64_t t0, int64_t t1) {
+  if (g < 0 || t1 <= t0) return 0;
+  auto le = [&](int64_t x) { return x < g ? int64_t{0} : (x - g) / kGroupMax + 1; };
+  return le(t1 - 1) - le(t0 - 1);
 }
 
-// ------------------------------------------------------------------ moments
-__global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, int level,
-                                                const int4* __restrict__ topo,
-                                                const float4* __restrict__ geo,
-                                                const float4* __restrict__ sp,
-                                                float4* __restrict__ mp, float4* __restrict__ qd) {
-  const int start = hdr->level_start[level];
-  const int count = hdr->level_count[level];
-  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
-    int k = start + u;
-    int4 t = topo[k];
-    float M = 0.f, X = 0.f, Y = 0.f, Z = 0.f;
-    float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
-    if (t.w == 0) {
-      for (int j = t.x; j < t.x + t.y; ++j) {
-        float4 p = sp[j];
-        M += p.w;
-        X += p.w * p.x;
-        Y += p.w * p.y;
-        Z += p.w * p.z;
-      }
-      if (M != 0.f) {
-        X /= M;
-        Y /= M;
-        Z /= M;
-        for (int j = t.x; j < t.x + t.y; ++j) {
-          float4 p = sp[j];
-          float dx = p.x - X, dy = p.y - Y, dz = p.z - Z;
-          sxx += p.w * dx * dx;
-          syy += p.w * dy * dy;
-          szz += p.w * dz * dz;
-          sxy += p.w * dx * dy;
-          sxz += p.w * dx * dz;
-          syz += p.w * dy * dz;
-        }
-      }
+// Exclusive block scan (sum, or max with identity -1) of one value per
+// thread of a 1024-thread block; the block total (maximum) through *all.
+template <typename T, bool SUM>
+__device__ __forceinline__ T block_scan1(T v, T* s_w, T* all) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const T t = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = SUM ? incl + t : (incl > t ? incl : t);
+  }
+  T ex = __shfl_up(incl, 1, 64);
+  if (lane == 0) ex = SUM ? T(0) : T(-1);
+  if (lane == 63) s_w[w] = incl;
+  __syncthreads();
+  T before = SUM ? T(0) : T(-1), tot = SUM ? T(0) : T(-1);
+  for (int j = 0; j < 16; ++j) {
+    const T x = s_w[j];
+    if (SUM) {
+      if (j < w) before += x;
+      tot += x;
     } else {
-      for (int c = t.z; c < t.z + t.w; ++c) {
-        float4 m = mp[c];
-        M += m.w;
-        X += m.w * m.x;
-        Y += m.w * m.y;
-        Z += m.w * m.z;
-      }
-      if (M != 0.f) {
-        X /= M;
-        Y /= M;
-        Z /= M;
-        for (int c = t.z; c < t.z + t.w; ++c) {
-          float4 m = mp[c];
-          float4 q0 = qd[2 * c], q1 = qd[2 * c + 1];
-          float dx = m.x - X, dy = m.y - Y, dz = m.z - Z;
-          sxx += q0.x + m.w * dx * dx;
-          syy += q0.y + m.w * dy * dy;
-          szz += q0.z + m.w * dz * dz;
-          sxy += q0.w + m.w * dx * dy;
-          sxz += q1.x + m.w * dx * dz;
-          syz += q1.y + m.w * dy * dz;
-        }
-      }
+      if (j < w) before = before > x ? before : x;
+      tot = tot > x ? tot : x;
     }
-    if (M == 0.f) {
-      float4 gg = geo[k];
-      X = gg.x;
-      Y = gg.y;
-      Z = gg.z;
+  }
+  __syncthreads();
+  *all = tot;
+  return SUM ? before + ex : (before > ex ? before : ex);
+}
+
+__global__ __launch_bounds__(1024) void k_topo_scan(Header* __restrict__ hdr,
+                                                    int32_t* __restrict__ tiles, int64_t ntiles,
+                                                    int64_t n, int nsets, int D, int64_t cap) {
+  __shared__ int32_t s_w32[16];
+  __shared__ int64_t s_w64[16];
+  __shared__ int32_t s_tot[11];
+  __shared__ int32_t s_lvl[10][1024];  // exclusive level offsets entering each chunk
+  const int tid = threadIdx.x;
+  const int64_t per = (ntiles + 1023) / 1024;
+  const int64_t t0 = tid * per, t1 = t0 + per < ntiles ? t0 + per : ntiles;
+#pragma unroll 1
+  for (int l = 0; l < 10; ++l) {
+    int32_t sum = 0;
+    for (int64_t t = t0; t < t1; ++t) sum += tiles[t * kTopoCols + 1 + l];
+    int32_t tot;
+    s_lvl[l][tid] = block_scan1<int32_t, true>(sum, s_w32, &tot);
+    if (tid == 0) s_tot[l] = tot;
+  }
+  // carries entering the chunk: the last group cell (positions rise, so a
+  // max) and the last lambda source as (position << 4 | top)
+  int64_t cg = -1, cl = -1;
+  for (int64_t t = t0; t < t1; ++t) {
+    const int32_t* c = tiles + t * kTopoCols;
+    if (c[kColLastG] >= 0) cg = c[kColLastG];
+    if (c[kColLamPos] >= 0) cl = (static_cast<int64_t>(c[kColLamPos]) << 4) | c[kColLamTop];
+  }
+  int64_t dummy;
+  const int64_t g_in = block_scan1<int64_t, false>(cg, s_w64, &dummy);
+  const int64_t l_in = block_scan1<int64_t, false>(cl, s_w64, &dummy);
+  // group starts per chunk (the leading ones need the carried cell)
+  int32_t gsum = 0;
+  {
+    int64_t g = g_in;
+    for (int64_t t = t0; t < t1; ++t) {
+      const int32_t* c = tiles + t * kTopoCols;
+      const int64_t x = t * kTopoTile, e = x + kTopoTile < n ? x + kTopoTile : n;
+      gsum += static_cast<int32_t>(lead_groups(g, x, c[kColFirstG] >= 0 ? c[kColFirstG] : e)) +
+              c[kColGroups];
+      if (c[kColLastG] >= 0) g = c[kColLastG];
     }
+  }
+  int32_t gtot;
+  int32_t goff = block_scan1<int32_t, true>(gsum, s_w32, &gtot);
+  // per-tile outputs
+  {
+    int64_t g = g_in;
+    int32_t lam = l_in >= 0 ? static_cast<int32_t>(l_in & 15) : -1;
+    for (int64_t t = t0; t < t1; ++t) {
+      int32_t* c = tiles + t * kTopoCols;
+      const int64_t x = t * kTopoTile, e = x + kTopoTile < n ? x + kTopoTile : n;
+      const int32_t first_g = c[kColFirstG], last_g = c[kColLastG];
+      const int32_t lam_pos = c[kColLamPos], lam_top = c[kColLamTop];
+      const int32_t here =
+          static_cast<int32_t>(lead_groups(g, x, first_g >= 0 ? first_g : e)) + c[kColGroups];
+      c[kColGroups] = goff;
+      goff += here;
+      for (int l = 0; l < 10; ++l) {
+        const int32_t v = c[1 + l];
+        c[1 + l] = s_lvl[l][tid];
+        s_lvl[l][tid] += v;
+      }
+      c[kColLastG] = static_cast<int32_t>(g);
+      c[kColLamTop] = lam;
+      if (last_g >= 0) g = last_g;
+      if (lam_pos >= 0) lam = lam_top;
+    }
+  }
+  if (tid == 0) {
+    int64_t start = nsets;
+    for (int l = 1; l < kMaxLevels; ++l) {
+      const int64_t cnt = l <= D && l <= 10 ? s_tot[l - 1] : 0;
+      hdr->level_start[l] = static_cast<int32_t>(start);
+      hdr->level_count[l] = static_cast<int32_t>(cnt);
+      start += cnt;
+    }
+    if (start > cap) {
+      for (int l = 1; l < kMaxLevels; ++l) {
+        hdr->level_start[l] = nsets;
+        hdr->level_count[l] = 0;
+      }
+      hdr->overflow |= 1;
+      start = nsets;
+    }
+    hdr->num_nodes = static_cast<int32_t>(start);
+    hdr->num_groups = gtot;
+    hdr->queue = 0;
+    for (int x = 0; x < 8; ++x) hdr->xq[x] = 0;
+  }
+}
+
+__global__ __launch_bounds__(kTopoTile) void k_topo_emit(
+    const Header* __restrict__ hdr, int64_t n, const uint32_t* __restrict__ codes,
+    const int32_t* __restrict__ offs, int nsets, int D, const int32_t* __restrict__ tiles,
+    const uint32_t* __restrict__ prec, const CubeGeom* __restrict__ geoms,
+    int4* __restrict__ topo, uint32_t* __restrict__ prefix, float4* __restrict__ geo,
+    int32_t* __restrict__ parent, int2* __restrict__ ends, int2* __restrict__ groups) {
+  __shared__ int s_cnt[kTopoWaves][kTopoCols];
+  __shared__ int s_w[2 * kTopoWaves];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * kTopoTile + threadIdx.x;
+  const bool nodes_ok = (hdr->overflow & 3) == 0;
+  const bool ok = r < n && (hdr->overflow & 2) == 0;
+  const int32_t* base = tiles + static_cast<int64_t>(blockIdx.x) * kTopoCols;
+  topo::PointNodes p{0, -1, 0, -1};
+  int gl = -1;
+  SetRange q{0, 0, 0};
+  if (ok) {
+    const uint32_t v = prec[r];
+    p.first = static_cast<int>(v & 15u);
+    p.top = static_cast<int>((v >> 4) & 15u) - 1;
+    gl = static_cast<int>((v >> 8) & 15u) - 1;
+    q = set_range(offs, nsets, r);
+  }
+  const int ri = static_cast<int>(r);
+  const bool sb = ok && r == q.sb;
+  const int gs = block_last(ok && gl >= 0, ri, base[kColLastG], s_w, lane, w);
+  const int lam = block_last(ok && (p.first > 0 || sb), p.first > 0 ? p.top : 0,
+                             base[kColLamTop], s_w, lane, w);
+  const bool gstart = ok && gs >= 0 && ((ri - gs) % kGroupMax) == 0;
+  int rank[kTopoCols];
+  {
+    const uint64_t b = __ballot(gstart);
+    rank[0] = __popcll(b & lt);
+    if (lane == 0) s_cnt[w][0] = __popcll(b);
+  }
+  for (int l = 1; l <= 10; ++l) {
+    const uint64_t b = __ballot(ok && l <= D && p.first > 0 && l >= p.first && l <= p.top);
+    rank[l] = __popcll(b & lt);
+    if (lane == 0) s_cnt[w][l] = __popcll(b);
+  }
+  __syncthreads();
+  if (!ok) return;
+  auto rank_of = [&](int l) {
+    int v = base[l] + rank[l];
+    for (int k = 0; k < w; ++k) v += s_cnt[k][l];
+    return v;
+  };
+  auto begins = [&](int l) { return p.first > 0 && l >= p.first && l <= p.top ? 1 : 0; };
+  if (gstart) groups[rank_of(0)] = make_int2(ri, 0);
+  if (!nodes_ok) return;
+  // records of the nodes r begins
+  if (sb && p.first == 1) topo[q.set].z = hdr->level_start[1] + rank_of(1);
+  if (p.first > 0) {
+    const CubeGeom gm = geoms[q.set];
+    for (int l = p.first; l <= p.top; ++l) {
+      const int idx = hdr->level_start[l] + rank_of(l);
+      const uint32_t pre = codes[r] >> (3 * (D - l));
+      const int fc = l < p.top ? hdr->level_start[l + 1] + rank_of(l + 1) : -1;
+      topo[idx] = make_int4(ri, 0, fc, 0);
+      prefix[idx] = pre;
+      parent[idx] = l == 1 ? q.set
+                           : hdr->level_start[l - 1] + rank_of(l - 1) - (l - 1 >= p.first ? 0 : 1);
+      geo[idx] = cell_geometry(gm, pre, l);
+    }
+  }
+  // ends and last children of the nodes r closes
+  const int dn = r + 1 < q.se ? topo::common_digits(codes[r], codes[r + 1], D) : -1;
+  for (int l = dn + 1 > 1 ? dn + 1 : 1; l <= lam; ++l) {
+    const int idx = hdr->level_start[l] + rank_of(l) + begins(l) - 1;
+    ends[idx].x = ri + 1;
+    if (l < lam) ends[idx].y = hdr->level_start[l + 1] + rank_of(l + 1) + begins(l + 1) - 1;
+  }
+  if (r + 1 == q.se && lam >= 1) ends[q.set].y = hdr->level_start[1] + rank_of(1) + begins(1) - 1;
+}
+
+// Sum over a row of 16 lanes (every lane of the row active), by DPP moves:
+// partners l ^ 8 (row rotate by 8), 7 - l within each half row, then the
+// quad permutations l ^ 2 and l ^ 1; each stage adds a disjoint set of lanes.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float row_sum16(float v) {
+  v += dpp_move<0x128>(v);  // row_ror:8
+  v += dpp_move<0x141>(v);  // row_half_mirror
+  v += dpp_move<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+  return v;
+}
+
+// Moments of node k by a row of 16 lanes (sl = lane within the row): a leaf
+// sums its points 16 at a time (mass and mass-weighted position, then the
+// central second moment about the centre of mass: two passes, as the FP64
+// oracle), an internal node its <= 8 children (parallel-axis shift of their
+// second moments).  The row sums in a fixed order: deterministic.
+__device__ __forceinline__ void node_moments(int k, int4 t, int sl,
+                                             const float4* __restrict__ geo,
+                                             const float4* __restrict__ sp,
+                                             float4* __restrict__ mp, float4* __restrict__ qd,
+                                             const float4* __restrict__ sv,
+                                             float4* __restrict__ wd) {
+  if (wd != nullptr) {
+    // dipole sum m v of the node (winding numbers: areas times unit normals)
+    float vx = 0.f, vy = 0.f, vz = 0.f;
+    if (t.w == 0) {
+      for (int j = t.x + sl; j < t.x + t.y; j += 16) {
+        const float a = sp[j].w;
+        const float4 nv = sv[j];
+        vx = fmaf(a, nv.x, vx);
+        vy = fmaf(a, nv.y, vy);
+        vz = fmaf(a, nv.z, vz);
+      }
+    } else if (sl < t.w) {
+      const float4 dc = wd[t.z + sl];
+      vx = dc.x;
+      vy = dc.y;
+      vz = dc.z;
+    }
+    vx = row_sum16(vx);
+    vy = row_sum16(vy);
+    vz = row_sum16(vz);
+    if (sl == 0) wd[k] = make_float4(vx, vy, vz, 0.f);
+  }
+  float M = 0.f, X = 0.f, Y = 0.f, Z = 0.f;
+  if (t.w == 0) {
+    for (int j = t.x + sl; j < t.x + t.y; j += 16) {
+      const float4 p = sp[j];
+      M += p.w;
+      X = fmaf(p.w, p.x, X);
+      Y = fmaf(p.w, p.y, Y);
+      Z = fmaf(p.w, p.z, Z);
+    }
+  } else if (sl < t.w) {
+    const float4 m = mp[t.z + sl];
+    M = m.w;
+    X = m.w * m.x;
+    Y = m.w * m.y;
+    Z = m.w * m.z;
+  }
+  M = row_sum16(M);
+  X = row_sum16(X);
+  Y = row_sum16(Y);
+  Z = row_sum16(Z);
+  if (M != 0.f) {
+    X /= M;
+    Y /= M;
+    Z /= M;
+  } else {
+    const float4 gg = geo[k];
+    X = gg.x;
+    Y = gg.y;
+    Z = gg.z;
+  }
+  float sxx = 0.f, syy = 0.f, szz = 0.f, sxy = 0.f, sxz = 0.f, syz = 0.f;
+  if (t.w == 0) {
+    for (int j = t.x + sl; j < t.x + t.y; j += 16) {
+      const float4 p = sp[j];
+      const float dx = p.x - X, dy = p.y - Y, dz = p.z - Z;
+      const float wx = p.w * dx, wy = p.w * dy;
+      sxx = fmaf(wx, dx, sxx);
+      syy = fmaf(wy, dy, syy);
+      szz = fmaf(p.w * dz, dz, szz);
+      sxy = fmaf(wx, dy, sxy);
+      sxz = fmaf(wx, dz, sxz);
+      syz = fmaf(wy, dz, syz);
+    }
+  } else if (sl < t.w) {
+    const int c = t.z + sl;
+    const float4 m = mp[c];
+    const float4 q0 = qd[2 * c], q1 = qd[2 * c + 1];
+    const float dx = m.x - X, dy = m.y - Y, dz = m.z - Z;
+    const float wx = m.w * dx, wy = m.w * dy;
+    sxx = fmaf(wx, dx, q0.x);
+    syy = fmaf(wy, dy, q0.y);
+    szz = fmaf(m.w * dz, dz, q0.z);
+    sxy = fmaf(wx, dy, q0.w);
+    sxz = fmaf(wx, dz, q1.x);
+    syz = fmaf(wy, dz, q1.y);
+  }
+  sxx = row_sum16(sxx);
+  syy = row_sum16(syy);
+  szz = row_sum16(szz);
+  sxy = row_sum16(sxy);
+  sxz = row_sum16(sxz);
+  syz = row_sum16(syz);
+  if (M == 0.f) sxx = syy = szz = sxy = sxz = syz = 0.f;
+  if (sl == 0) {
     mp[k] = make_float4(X, Y, Z, M);
     qd[2 * k] = make_float4(sxx, syy, szz, sxy);
     // + 1.5 tr(S) and -2.5 tr(S), constants of the walk's far-field terms
@@ -551,58 +774,64 @@ __global__ __launch_bounds__(kB) void k_moments(const Header* __restrict__ hdr, 
   }
 }
 
-// ------------------------------------------------------------------ query groups
-// Partition of the sorted points into contiguous groups of <= kGroupMax
-// points that stay inside one cell: every maximal node holding at most
-// kGroupCell points (its parent holds more, or it is a root), and every
-// over-full leaf at the depth limit, is cut into consecutive runs of
-// kGroupMax points (the last run shorter).  Runs cross the child cells of
-// such a node, which keeps wavefronts ~95% full, while the node bounds the
-// group's box.
-template <bool EMIT>
-__device__ __forceinline__ int node_groups(const int4* __restrict__ topo,
-                                           const int32_t* __restrict__ parent, int k,
-                                           int2* __restrict__ out) {
-  const int4 t = topo[k];
-  if (t.y == 0) return 0;
-  const int p = parent[k];
-  const bool maximal = t.y <= kGroupCell && (p < 0 || topo[p].y > kGroupCell);
-  const bool overfull_leaf = t.y > kGroupCell && t.w == 0;
-  if (!maximal && !overfull_leaf) return 0;
-  int ng = 0;
-  for (int s0 = t.x; s0 < t.x + t.y; s0 += kGroupMax) {
-    if (EMIT) out[ng] = make_int2(s0, min(kGroupMax, t.x + t.y - s0));
-    ++ng;
+// Counts and child counts of every record, group sizes (groups tile the
+// sorted points), and with `moments` the moments of every leaf (a row of 16
+// lanes per node).
+__global__ __launch_bounds__(kB) void k_topo_finalize(const Header* __restrict__ hdr, int64_t n,
+                                                      int nsets, int moments,
+                                                      int4* __restrict__ topo,
+                                                      const int2* __restrict__ ends,
+                                                      int2* __restrict__ groups,
+                                                      const float4* __restrict__ geo,
+                                                      const float4* __restrict__ sp,
+                                                      float4* __restrict__ mp,
+                                                      float4* __restrict__ qd,
+                                                      const float4* __restrict__ sv,
+                                                      float4* __restrict__ wd, int sentinel) {
+  if (wd != nullptr && blockIdx.x == 0 && threadIdx.x == 0)
+    wd[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int ng = hdr->num_groups;
+  for (int g = blockIdx.x * kB + threadIdx.x; g < ng; g += gridDim.x * kB) {
+    const int s0 = groups[g].x;
+    const int s1 = g + 1 < ng ? groups[g + 1].x : static_cast<int>(n);
+    groups[g].y = s1 - s0;
   }
-  return ng;
+  const int sl = threadIdx.x & 15;
+  const int row = (blockIdx.x * kB + threadIdx.x) >> 4;
+  const int nrows = (gridDim.x * kB) >> 4;
+  const int nn = hdr->num_nodes;
+  for (int k = row; k < nn; k += nrows) {
+    int4 t = topo[k];
+    const int2 e = ends[k];
+    if (k >= nsets) t.y = e.x - t.x;
+    t.w = t.z >= 0 ? e.y - t.z + 1 : 0;
+    if (sl == 0) {
+      topo[k].y = t.y;
+      topo[k].w = t.w;
+    }
+    if (moments && t.w == 0) node_moments(k, t, sl, geo, sp, mp, qd, sv, wd);
+  }
 }
 
-// Per-node group counts over the whole capacity `cap` (zero past the node
-// count), for a device-wide exclusive scan of host-known length.
-__global__ __launch_bounds__(kB) void k_group_count(const Header* __restrict__ hdr,
-                                                    const int4* __restrict__ topo,
-                                                    const int32_t* __restrict__ parent,
-                                                    uint32_t* __restrict__ goff, int cap) {
-  const int nn = hdr->num_nodes;
-  for (int k = blockIdx.x * kB + threadIdx.x; k < cap; k += gridDim.x * kB)
-    goff[k] = k < nn ? static_cast<uint32_t>(node_groups<false>(topo, parent, k, nullptr)) : 0u;
-}
-
-// Writes the groups at their scanned offsets; publishes the group count
-// (the exclusive sum at the node count) and resets the walk's work queues.
-__global__ __launch_bounds__(kB) void k_group_emit(Header* __restrict__ hdr,
-                                                   const int4* __restrict__ topo,
-                                                   const int32_t* __restrict__ parent,
-                                                   const uint32_t* __restrict__ goff,
-                                                   int2* __restrict__ groups) {
-  const int nn = hdr->num_nodes;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    hdr->num_groups = static_cast<int>(goff[nn]);
-    hdr->queue = 0;
-    for (int x = 0; x < 8; ++x) hdr->xq[x] = 0;
+// Moments of the internal nodes of one level (children done).
+__global__ __launch_bounds__(kB) void k_moments_internal(const Header* __restrict__ hdr,
+                                                         int level, const int4* __restrict__ topo,
+                                                         const float4* __restrict__ geo,
+                                                         const float4* __restrict__ sp,
+                                                         float4* __restrict__ mp,
+                                                         float4* __restrict__ qd,
+                                                         const float4* __restrict__ sv,
+                                                         float4* __restrict__ wd) {
+  const int sl = threadIdx.x & 15;
+  const int row = (blockIdx.x * kB + threadIdx.x) >> 4;
+  const int nrows = (gridDim.x * kB) >> 4;
+  const int start = hdr->level_start[level];
+  const int count = hdr->level_count[level];
+  for (int u = row; u < count; u += nrows) {
+    const int k = start + u;
+    const int4 t = topo[k];
+    if (t.w > 0) node_moments(k, t, sl, geo, sp, mp, qd, sv, wd);
   }
-  for (int k = blockIdx.x * kB + threadIdx.x; k < nn; k += gridDim.x * kB)
-    node_groups<true>(topo, parent, k, groups + goff[k]);
 }
 
 // ------------------------------------------------------------------ wave helpers
@@ -1018,12 +1247,14 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 // until none are left (heavy groups do not hold up a static schedule).
 // Registers are capped at 128 VGPRs (4 waves per SIMD): the adjoint variant
 // would take ~143 and drop to 3 waves; the cap costs a few spills outside
-// the interaction batches and is ~4% faster.
-template <bool VJP, bool QUAD, bool FORCES>
-__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(WalkArgs A) {
-  __shared__ int s_stack[kWavesPerBlock][kStackCap];
-  __shared__ int s_far[kWavesPerBlock][kFarCap];
-  __shared__ int s_near[kWavesPerBlock][kNearCap];
+// the interaction batches and is ~4% faster.  WPB waves per block (the
+// block size is a tuning dimension; every wave walks on its own).
+template <bool VJP, bool QUAD, bool FORCES, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
+    WalkArgs A) {
+  __shared__ int s_stack[WPB][kStackCap];
+  __shared__ int s_far[WPB][kFarCap];
+  __shared__ int s_near[WPB][kNearCap];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int num_groups = A.hdr->num_groups;
@@ -1174,7 +1405,7 @@ Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   L.vals_tmp = o; o = align(o + 4 * nn);
   // radix sort temporaries, reused by the scan of the group counts
   L.sort_tmp = o;
-  o = align(o + std::max(radix_sort_temp_bytes(n), exclusive_scan_temp_bytes(L.cap + 1)));
+  o = align(o + morton_sort_temp_bytes(n, nsets));
   L.bbox_tmp = o; o = align(o + 6 * 4 * (kBboxBlocks + 1));
   L.sp = o; o = align(o + 16 * nn);
   L.sc = o; o = align(o + 16 * nn);
@@ -1184,13 +1415,13 @@ Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   L.geo = o; o = align(o + 16 * cap);
   L.mp = o; o = align(o + 16 * cap);
   L.qd = o; o = align(o + 32 * cap);
-  L.nch = o; o = align(o + 4 * cap);
   L.adj = o; o = align(o + 16 * cap);
   L.adjq = o; o = align(o + 32 * cap);
   L.qgrad = o; o = align(o + 16 * nn);
   L.sgrad = o; o = align(o + 16 * nn);
   L.groups = o; o = align(o + 8 * nn);
-  L.goff = o; o = align(o + 4 * cap);
+  L.tiles = o;
+  o = align(o + 4 * static_cast<size_t>(kTopoCols) * ((nn + kTopoTile - 1) / kTopoTile));
   L.geoms = o; o = align(o + sizeof(CubeGeom) * static_cast<size_t>(nsets));
   L.offs = o; o = align(o + 4 * (static_cast<size_t>(nsets) + 1));
   L.setbox = o; o = align(o + 24 * static_cast<size_t>(nsets));
@@ -1215,7 +1446,7 @@ namespace {
 // Everything after the roots exist: codes, sort, gather, levels, moments, groups.
 int build_rest(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
                const float* cot, int64_t n, const int32_t* offs, int max_depth,
-               int leaf_size, int parts) {
+               int leaf_size, int parts, float4* dipoles) {
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
   auto* codes = reinterpret_cast<uint32_t*>(ws + L.codes);
   auto* codes_in = reinterpret_cast<uint32_t*>(ws + L.codes_in);
@@ -1228,62 +1459,60 @@ int build_rest(hipStream_t s, char* ws, const Layout& L, const float* pts, const
   auto* geo = reinterpret_cast<float4*>(ws + L.geo);
   auto* mp = reinterpret_cast<float4*>(ws + L.mp);
   auto* qd = reinterpret_cast<float4*>(ws + L.qd);
-  auto* nch = reinterpret_cast<uint32_t*>(ws + L.nch);
   auto* sp = reinterpret_cast<float4*>(ws + L.sp);
   auto* sc = reinterpret_cast<float4*>(ws + L.sc);
   auto* geoms = reinterpret_cast<CubeGeom*>(ws + L.geoms);
   const int nsets = L.nsets;
   if (n > 0) {
-    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, geoms, offs, nsets, max_depth, codes_in, keys,
-                                           vals);
+    k_codes<<<grid_for(n, kB), kB, 0, s>>>(pts, n, geoms, offs, nsets, max_depth, codes_in);
     SBH_LAUNCH_CHECK();
-    int rc = radix_sort_pairs(s, keys, vals, codes, perm, n, 3 * max_depth, ws + L.sort_tmp);
+    // stable order by (set, code, input row); the permutation lands in perm
+    // or in vals (then k_gather copies it)
+    const uint32_t* order = nullptr;
+    const uint32_t* sorted = nullptr;
+    int rc = morton_sort(s, codes_in, n, 3 * max_depth, offs, nsets, keys, vals, codes, perm,
+                         ws + L.sort_tmp, &hdr->overflow, &sorted, &order);
     if (rc != SBH_OK) return rc;
-    if (nsets > 1) {
-      // stable pass on the set id: order (set, code, input index)
-      int set_bits = 0;
-      while ((1 << set_bits) < nsets) ++set_bits;
-      k_setkeys<<<grid_for(n, kB), kB, 0, s>>>(perm, n, offs, nsets, keys, vals);
-      SBH_LAUNCH_CHECK();
-      rc = radix_sort_pairs(s, keys, vals, codes, perm, n, set_bits, ws + L.sort_tmp);
-      if (rc != SBH_OK) return rc;
-    }
-    k_gather<<<grid_for(n, kB), kB, 0, s>>>(pts, ms, cot, n, perm, codes_in, codes, sp, sc);
+    k_gather<<<grid_for(n, kB), kB, 0, s>>>(pts, ms, cot, n, order, sorted, perm, codes, sp, sc);
     SBH_LAUNCH_CHECK();
   }
-  int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
-  auto level_cap = [&](int l) {
-    return std::min<int64_t>(
-        l < 18 ? static_cast<int64_t>(nsets) << (3 * l) : INT64_MAX,
-        std::max<int64_t>(nsets, std::min<int64_t>(n, 8 * internal_max)));
-  };
-  for (int l = 0; l < max_depth; ++l) {
-    unsigned g = grid_for(level_cap(l), kB, 4096);
-    k_split<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, nch);
-    SBH_LAUNCH_CHECK();
-    k_scan_level<<<1, 1024, 0, s>>>(hdr, l, nch, static_cast<int>(L.cap));
-    SBH_LAUNCH_CHECK();
-    k_emit<<<g, kB, 0, s>>>(hdr, l, max_depth, leaf_size, codes, topo, prefix, geo, parent, nch,
-                            geoms, offs, nsets);
-    SBH_LAUNCH_CHECK();
-  }
-  for (int l = max_depth; l >= 0 && (parts & kBuildMoments); --l) {
-    k_moments<<<grid_for(level_cap(l), kB, 4096), kB, 0, s>>>(hdr, l, topo, geo, sp, mp, qd);
-    SBH_LAUNCH_CHECK();
-  }
-  if (!(parts & kBuildGroups)) return SBH_OK;
-  auto* goff = reinterpret_cast<uint32_t*>(ws + L.goff);
+  // topology, query groups and moments from the sorted codes
+  auto* tiles = reinterpret_cast<int32_t*>(ws + L.tiles);
+  auto* prec = reinterpret_cast<uint32_t*>(ws + L.qgrad);  // per-point scratch of the build
+  auto* ends = reinterpret_cast<int2*>(ws + L.adj);        // per-node scratch of the build
   auto* groups = reinterpret_cast<int2*>(ws + L.groups);
-  // counts over cap + 1 entries: the node count is at most cap, so the
-  // exclusive sum at index num_nodes is the group total
-  const int64_t span = L.cap + 1;
-  unsigned gg = grid_for(span, kB, 4096);
-  k_group_count<<<gg, kB, 0, s>>>(hdr, topo, parent, goff, static_cast<int>(span));
+  const int64_t ntiles = (n + kTopoTile - 1) / kTopoTile;
+  if (ntiles > 0) {
+    k_topo_count<<<static_cast<unsigned>(ntiles), kTopoTile, 0, s>>>(hdr, n, codes, offs, nsets,
+                                                              max_depth, leaf_size, tiles, prec);
+    SBH_LAUNCH_CHECK();
+  }
+  k_topo_scan<<<1, 1024, 0, s>>>(hdr, tiles, ntiles, n, nsets, max_depth, L.cap);
   SBH_LAUNCH_CHECK();
-  int rc = exclusive_scan(s, goff, span, ws + L.sort_tmp);
-  if (rc != SBH_OK) return rc;
-  k_group_emit<<<gg, kB, 0, s>>>(hdr, topo, parent, goff, groups);
+  if (ntiles > 0) {
+    k_topo_emit<<<static_cast<unsigned>(ntiles), kTopoTile, 0, s>>>(hdr, n, codes, offs, nsets,
+                                                             max_depth, tiles, prec, geoms, topo,
+                                                             prefix, geo, parent, ends, groups);
+    SBH_LAUNCH_CHECK();
+  }
+  const bool moments = (parts & kBuildMoments) != 0;
+  float4* wd = moments ? dipoles : nullptr;
+  k_topo_finalize<<<grid_for(L.cap, kB / 16, 2048), kB, 0, s>>>(
+      hdr, n, nsets, moments ? 1 : 0, topo, ends, groups, geo, sp, mp, qd, sc, wd,
+      static_cast<int>(L.cap));
   SBH_LAUNCH_CHECK();
+  if (moments) {
+    int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
+    for (int l = max_depth - 1; l >= 0; --l) {
+      // internal nodes of level l: at most nsets 8^l and at most internal_max
+      const int64_t cap_l = std::min<int64_t>(
+          l < 18 ? static_cast<int64_t>(nsets) << (3 * l) : INT64_MAX,
+          std::max<int64_t>(nsets, internal_max));
+      k_moments_internal<<<grid_for(cap_l, kB / 16, 1024), kB, 0, s>>>(hdr, l, topo, geo, sp,
+                                                                        mp, qd, sc, wd);
+      SBH_LAUNCH_CHECK();
+    }
+  }
   return SBH_OK;
 }
 
@@ -1304,7 +1533,7 @@ int init_roots(hipStream_t s, char* ws, const Layout& L, int64_t n, const float*
 
 int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
           const float* cot, int64_t n, int max_depth, int leaf_size,
-          const float* user_bounds, int parts) {
+          const float* user_bounds, int parts, float4* dipoles) {
   if (L.nsets != 1) return fail_internal_msg("build: layout is a forest");
   if (user_bounds == nullptr) {
     float* b = nullptr;
@@ -1315,7 +1544,7 @@ int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const floa
   int rc = init_roots(s, ws, L, n, user_bounds, nullptr);
   if (rc != SBH_OK) return rc;
   return build_rest(s, ws, L, pts, ms, cot, n, reinterpret_cast<const int32_t*>(ws + L.offs),
-                    max_depth, leaf_size, parts);
+                    max_depth, leaf_size, parts, dipoles);
 }
 
 int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
@@ -1339,12 +1568,31 @@ int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, con
   }
   int rc = init_roots(s, ws, L, n, nullptr, box);
   if (rc != SBH_OK) return rc;
-  return build_rest(s, ws, L, pts, ms, cot, n, ofs, max_depth, leaf_size, parts);
+  return build_rest(s, ws, L, pts, ms, cot, n, ofs, max_depth, leaf_size, parts, nullptr);
 }
+
+namespace {
+template <int WPB>
+void launch_walk(hipStream_t s, unsigned grid, bool vjp, bool quad, bool f, const WalkArgs& A) {
+  const unsigned block = 64 * WPB;
+  if (vjp && quad && f)
+    k_walk<true, true, true, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp && quad)
+    k_walk<true, true, false, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp && f)
+    k_walk<true, false, true, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp)
+    k_walk<true, false, false, WPB><<<grid, block, 0, s>>>(A);
+  else if (quad)
+    k_walk<false, true, true, WPB><<<grid, block, 0, s>>>(A);
+  else
+    k_walk<false, false, true, WPB><<<grid, block, 0, s>>>(A);
+}
+}  // namespace
 
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
          bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
-         bool want_stats, hipEvent_t* ev_walk, int walk_blocks) {
+         bool want_stats, hipEvent_t* ev_walk, int walk_blocks, int walk_threads) {
   if (n == 0) return SBH_OK;
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
   WalkArgs A;
@@ -1375,23 +1623,21 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
                                                                         A.adjq);
     SBH_LAUNCH_CHECK();
   }
-  // Persistent waves drain the group queue: enough blocks to fill every CU
-  // several times over, never more than one wave per point.
-  unsigned grid = grid_for(n, 64 * kWavesPerBlock, walk_blocks > 0 ? walk_blocks : 256 * 8);
+  // Persistent waves drain the group queue: 8192 waves by default (8 per
+  // SIMD, twice what fits at once, so that every CU stays busy to the end),
+  // never more than one wave per point.
+  const int threads = walk_threads > 0 ? walk_threads : kB;
+  const int wpb = threads / 64;
+  const int64_t waves_cap = walk_blocks > 0 ? static_cast<int64_t>(walk_blocks) * wpb : 8192;
+  unsigned grid = grid_for(n, 64 * wpb, static_cast<unsigned>((waves_cap + wpb - 1) / wpb));
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[0], s));
   const bool f = forces != nullptr;
-  if (vjp && quad && f)
-    k_walk<true, true, true><<<grid, kB, 0, s>>>(A);
-  else if (vjp && quad)
-    k_walk<true, true, false><<<grid, kB, 0, s>>>(A);
-  else if (vjp && f)
-    k_walk<true, false, true><<<grid, kB, 0, s>>>(A);
-  else if (vjp)
-    k_walk<true, false, false><<<grid, kB, 0, s>>>(A);
-  else if (quad)
-    k_walk<false, true, true><<<grid, kB, 0, s>>>(A);
-  else
-    k_walk<false, false, true><<<grid, kB, 0, s>>>(A);
+  switch (wpb) {
+    case 1: launch_walk<1>(s, grid, vjp, quad, f, A); break;
+    case 2: launch_walk<2>(s, grid, vjp, quad, f, A); break;
+    case 8: launch_walk<8>(s, grid, vjp, quad, f, A); break;
+    default: launch_walk<4>(s, grid, vjp, quad, f, A); break;
+  }
   SBH_LAUNCH_CHECK();
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[1], s));
   if (vjp) {

@@ -48,9 +48,9 @@ struct Layout {
   size_t header, codes, perm, keys_tmp, vals_tmp, sort_tmp, bbox_tmp;
   size_t sp, sc;                       // sorted (x,y,z,m) and cotangent records
   size_t topo, prefix, parent, geo;    // int4 {start,count,first_child,nchild}, cell prefix
-  size_t mp, qd, nch;                  // float4 {com, mass}, 2 x float4 second moment
+  size_t mp, qd;                       // float4 {com, mass}, 2 x float4 second moment
   size_t adj, adjq, qgrad, sgrad;      // adjoint accumulators
-  size_t groups, goff;                 // int2 {start, count} per group; per-node offsets
+  size_t groups, tiles;                // int2 {start, count} per group; per-tile counters
   size_t total;
 };
 
@@ -92,9 +92,12 @@ constexpr int kBuildAll = kBuildMoments | kBuildGroups;
 // `user_bounds` (device, 6 floats: min xyz, max xyz) may be null to use the
 // data bounds expanded by 1e-3.  Without kBuildMoments the node masses
 // stay unset (zero only for the sentinel).
+// dipoles (may be null): with kBuildMoments, also the per-node sums of the
+// masses times the gathered vectors (winding numbers: areas times unit
+// normals), L.cap + 1 float4 (the sentinel's zero included).
 int build(hipStream_t s, char* ws, const Layout& L, const float* pts, const float* ms,
           const float* cot, int64_t n, int max_depth, int leaf_size,
-          const float* user_bounds, int parts = kBuildAll);
+          const float* user_bounds, int parts = kBuildAll, float4* dipoles = nullptr);
 
 // Forest build: points are the concatenation of L.nsets sets, set k holding
 // input rows [offs[k], offs[k+1]) (device int32 array of nsets + 1 entries,
@@ -110,20 +113,15 @@ int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, con
 // ev_walk (may be null): two events recorded immediately before and after the
 // walk kernel on stream s.
 // Works on forests: each group walks the tree of its own set.  walk_blocks
-// (0: default) caps the persistent grid.
+// (0: default) caps the persistent grid; walk_threads (0: 256, else 64, 128,
+// 256 or 512) sets its block size.
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
          bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
-         bool want_stats, hipEvent_t* ev_walk = nullptr, int walk_blocks = 0);
+         bool want_stats, hipEvent_t* ev_walk = nullptr, int walk_blocks = 0,
+         int walk_threads = 0);
 
 // Copies {far, near, groups, nodes, levels, overflow, 0, 0} into a device int64[8].
 int copy_stats(hipStream_t s, char* ws, const Layout& L, int64_t* out);
-
-// Per-node dipoles D = sum a n of an element tree built by build() over
-// element centroids with the areas as masses and the unit normals as the
-// gathered per-point vector (sc): bottom-up, one level per launch; also
-// clears the sentinel node's entry.  wd holds L.cap + 1 float4.
-int wn_dipoles(hipStream_t s, char* ws, const Layout& L, float4* wd, int64_t n, int max_depth,
-               int leaf_size);
 
 // Cell geometry from a level-l prefix (same expression as the octree8 node
 // centres, src/sbh_ffi.cu:1828-1834): {cx, cy, cz, diagonal}.

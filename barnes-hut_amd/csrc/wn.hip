@@ -56,39 +56,6 @@ WnLayout wn_layout(int64_t n, int64_t m, int max_depth, int leaf_size) {
 // ------------------------------------------------------------------ moments
 // Bottom-up, one level per launch (children are finished before parents).
 // The level-0 launch also clears the sentinel node's dipole.
-__global__ __launch_bounds__(kB) void k_wn_moments(const Header* __restrict__ hdr, int level,
-                                                   const int4* __restrict__ topo,
-                                                   const float4* __restrict__ sp,
-                                                   const float4* __restrict__ sn,
-                                                   float4* __restrict__ wd, int sentinel) {
-  if (level == 0 && blockIdx.x == 0 && threadIdx.x == 0)
-    wd[sentinel] = make_float4(0.f, 0.f, 0.f, 0.f);
-  const int start = hdr->level_start[level];
-  const int count = hdr->level_count[level];
-  for (int u = blockIdx.x * kB + threadIdx.x; u < count; u += gridDim.x * kB) {
-    const int k = start + u;
-    const int4 t = topo[k];
-    float dx = 0.f, dy = 0.f, dz = 0.f;
-    if (t.w == 0) {
-      for (int j = t.x; j < t.x + t.y; ++j) {
-        const float a = sp[j].w;
-        const float4 nv = sn[j];
-        dx += a * nv.x;
-        dy += a * nv.y;
-        dz += a * nv.z;
-      }
-    } else {
-      for (int ch = t.z; ch < t.z + t.w; ++ch) {
-        const float4 dc = wd[ch];
-        dx += dc.x;
-        dy += dc.y;
-        dz += dc.z;
-      }
-    }
-    wd[k] = make_float4(dx, dy, dz, 0.f);
-  }
-}
-
 // ------------------------------------------------------------------ walk
 struct WnArgs {
   const Header* hdr;  // element tree
@@ -253,24 +220,6 @@ int check_wn(const SbhBuffer* points, const SbhBuffer* normals, const SbhBuffer*
 
 }  // namespace
 
-int wn_dipoles(hipStream_t s, char* ws, const Layout& L, float4* wd, int64_t n, int max_depth,
-               int leaf_size) {
-  auto* hdr = reinterpret_cast<Header*>(ws + L.header);
-  const auto* topo = reinterpret_cast<const int4*>(ws + L.topo);
-  const auto* sp = reinterpret_cast<const float4*>(ws + L.sp);
-  const auto* sn = reinterpret_cast<const float4*>(ws + L.sc);
-  const int64_t internal_max = n / (static_cast<int64_t>(leaf_size) + 1);
-  for (int l = max_depth; l >= 0; --l) {
-    const int64_t lvl_cap = std::min<int64_t>(
-        l < 21 ? (int64_t{1} << (3 * l)) : INT64_MAX,
-        std::max<int64_t>(1, std::min<int64_t>(n, 8 * internal_max)));
-    k_wn_moments<<<grid_for(lvl_cap, gw::kB, 4096), gw::kB, 0, s>>>(hdr, l, topo, sp, sn, wd,
-                                                                    static_cast<int>(L.cap));
-    SBH_LAUNCH_CHECK();
-  }
-  return SBH_OK;
-}
-
 }  // namespace tree
 }  // namespace sbh
 
@@ -312,16 +261,16 @@ extern "C" int sbh_winding_number(void* stream, const SbhBuffer* points, const S
   const bool profiled = pr.on() && m > 0;
   // Source tree over the centroids with areas as weights; the normals ride
   // along as the gathered per-point vector.
+  auto* wd = reinterpret_cast<float4*>(ws + W.wd);
   rc = tree::build(s, ws, W.t, pts, static_cast<const float*>(areas->data),
-                   static_cast<const float*>(normals->data), n, max_depth, leaf_size, nullptr);
+                   static_cast<const float*>(normals->data), n, max_depth, leaf_size, nullptr,
+                   tree::kBuildAll, wd);
   if (rc != SBH_OK) return rc;
   auto* hdr = reinterpret_cast<tree::Header*>(ws + W.t.header);
-  auto* wd = reinterpret_cast<float4*>(ws + W.wd);
   const auto* topo = reinterpret_cast<const int4*>(ws + W.t.topo);
   const auto* mp = reinterpret_cast<const float4*>(ws + W.t.mp);
   const auto* sp = reinterpret_cast<const float4*>(ws + W.t.sp);
   const auto* sn = reinterpret_cast<const float4*>(ws + W.t.sc);
-  if ((rc = tree::wn_dipoles(s, ws, W.t, wd, n, max_depth, leaf_size)) != SBH_OK) return rc;
   // Query tree (topology and groups only).
   char* qws = ws + W.qbase;
   rc = tree::build(s, qws, W.qt, q, nullptr, nullptr, m, tree::kQueryDepth, tree::kQueryLeaf,

@@ -1,12 +1,14 @@
 """Autotuner for the batched gravity pipeline.
 
-Times the captured fwd + VJP graph of the actual batch for every candidate
-(leaf size, persistent walk grid) and keeps the fastest candidate whose
-forces and gradients stay within rel_l2 <= 1e-3 of an FP64 direct sum on
-sampled rows of every set.  Results are cached per workload key (device,
-total points, set sizes, a digest of the per-set bounding boxes and the
-physics parameters) for the life of the process, and optionally in a JSON
-file.
+Times the captured fwd + VJP graph of the actual batch for candidate (leaf
+size, persistent walk grid, walk block size) triples and keeps the fastest
+candidate whose forces and gradients stay within rel_l2 <= 1e-3 of an FP64
+direct sum on sampled rows of every set.  The search is coordinate-wise by
+default (leaf sizes at the default schedule, then grids at the best leaf,
+then block sizes at the best pair: 5 + 2 + 3 captures) or the full product.
+Results are cached per workload key (device, total points, set sizes, a
+digest of the per-set bounding boxes and the physics parameters) for the
+life of the process, and optionally in a JSON file.
 """
 from __future__ import annotations
 
@@ -24,6 +26,7 @@ _CACHE: dict = {}
 
 DEFAULT_LEAF = (8, 16, 24, 32, 48)
 DEFAULT_BLOCKS = (0, 1024, 4096)
+DEFAULT_THREADS = (256, 64, 128, 512)  # walk block size; 256 is the library default
 REL_L2_GATE = 1e-3
 
 
@@ -105,15 +108,16 @@ class _Gate:
                 "grad_masses": _rel_l2(gm[r], self.ref[2])}
 
 
-def build_step(P, M, C, offsets, leaf_size: int, walk_blocks: int, kw: dict):
+def build_step(P, M, C, offsets, leaf_size: int, walk_blocks: int, kw: dict,
+               walk_threads: int = 0):
     """Preallocated buffers and a closure running one batched fwd + VJP."""
     nsets = offsets.shape[0] - 1
     ws = G.new_batch_workspace(P.shape[0], nsets, kw.get("max_depth", 10), leaf_size)
     F = torch.empty_like(P)
     gx = torch.empty_like(P)
     gm = torch.empty_like(M)
-    args = dict(kw, leaf_size=leaf_size, walk_blocks=walk_blocks, workspace=ws,
-                check_offsets=False)
+    args = dict(kw, leaf_size=leaf_size, walk_blocks=walk_blocks, walk_threads=walk_threads,
+                workspace=ws, check_offsets=False)
 
     def step():
         G.gravity_force_batch(P, M, offsets, out=F, **args)
@@ -141,16 +145,23 @@ def time_graph(step, reps: int = 3) -> float:
 
 
 def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
-               walk_blocks: Iterable[int] = DEFAULT_BLOCKS, reps: int = 3,
-               cache_file: Optional[str] = None, gate: float = REL_L2_GATE,
+               walk_blocks: Iterable[int] = DEFAULT_BLOCKS,
+               walk_threads: Iterable[int] = DEFAULT_THREADS, search: str = "coordinate",
+               reps: int = 3, cache_file: Optional[str] = None, gate: float = REL_L2_GATE,
                rows_per_set: int = 4, **kw) -> dict:
-    """Best {leaf_size, walk_blocks, ms} for this batch, plus the full table.
+    """Best {leaf_size, walk_blocks, walk_threads, ms} for this batch, plus the
+    table of every candidate timed.
 
     Candidates whose forces or gradients miss rel_l2 <= `gate` against the
     FP64 direct sum on `rows_per_set` sampled rows of every set are dropped;
-    raises RuntimeError if none passes.
+    raises RuntimeError if none passes.  search: "coordinate" (each dimension
+    in turn, the others at their best so far) or "grid" (every combination).
     kw: beta, softening_scale, max_depth, multipole_order (as gravity_force)."""
-    key = _key(P, offsets, kw) + f"_gate{gate}"
+    leaf_sizes, walk_blocks, walk_threads = list(leaf_sizes), list(walk_blocks), list(walk_threads)
+    if search not in ("coordinate", "grid"):
+        raise ValueError("search must be 'coordinate' or 'grid'")
+    key = (_key(P, offsets, kw) + f"_gate{gate}_{search}_l{leaf_sizes}_b{walk_blocks}"
+           f"_t{walk_threads}").replace(" ", "")
     if key in _CACHE:
         return _CACHE[key]
     if cache_file and os.path.exists(cache_file):
@@ -161,22 +172,45 @@ def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
             return disk[key]
     checker = _Gate(P, M, C, offsets, kw.get("softening_scale", 1e-2), rows_per_set)
     table = []
-    for leaf in leaf_sizes:
-        for blocks in walk_blocks:
-            step, (F, gx, gm), ws = build_step(P, M, C, offsets, leaf, blocks, kw)
-            ms = time_graph(step, reps)
-            step()
-            err = checker.errors(F, gx, gm)
-            ok = max(err.values()) <= gate
-            table.append({"leaf_size": leaf, "walk_blocks": blocks, "ms": ms,
-                          "rel_l2": err, "passes_gate": ok})
-            del ws
-    passing = [r for r in table if r["passes_gate"]]
-    if not passing:
+    done = {}
+
+    def run(leaf, blocks, threads):
+        if (leaf, blocks, threads) in done:
+            return done[(leaf, blocks, threads)]
+        step, (F, gx, gm), ws = build_step(P, M, C, offsets, leaf, blocks, kw, threads)
+        ms = time_graph(step, reps)
+        step()
+        err = checker.errors(F, gx, gm)
+        ok = max(err.values()) <= gate
+        row = {"leaf_size": leaf, "walk_blocks": blocks, "walk_threads": threads, "ms": ms,
+               "rel_l2": err, "passes_gate": ok}
+        table.append(row)
+        done[(leaf, blocks, threads)] = row
+        del ws
+        return row
+
+    def best_of(rows):
+        passing = [r for r in rows if r["passes_gate"]]
+        return min(passing, key=lambda r: r["ms"]) if passing else None
+
+    if search == "grid":
+        for leaf in leaf_sizes:
+            for blocks in walk_blocks:
+                for threads in walk_threads:
+                    run(leaf, blocks, threads)
+    else:
+        b0, t0 = walk_blocks[0], walk_threads[0]
+        cur = best_of([run(leaf, b0, t0) for leaf in leaf_sizes])
+        if cur is not None:
+            cur = best_of([run(cur["leaf_size"], b, t0) for b in walk_blocks])
+        if cur is not None:
+            cur = best_of([run(cur["leaf_size"], cur["walk_blocks"], t) for t in walk_threads])
+    best = best_of(table)
+    if best is None:
         raise RuntimeError(f"no candidate meets rel_l2 <= {gate}: {table}")
-    best = min(passing, key=lambda r: r["ms"])
     res = {"leaf_size": best["leaf_size"], "walk_blocks": best["walk_blocks"],
-           "ms": best["ms"], "table": table,
+           "walk_threads": best["walk_threads"], "ms": best["ms"], "table": table,
+           "search": search,
            "gate": {"rel_l2_max": gate, "rows": 0 if checker.rows is None
                     else int(checker.rows.numel()),
                     "reference": "FP64 direct sum and adjoint over each row's own set"}}

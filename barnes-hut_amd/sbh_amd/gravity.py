@@ -184,7 +184,7 @@ def _check_batch(points, masses, offsets, check_offsets):
 def gravity_force_batch(points, masses, offsets, *, beta: float = 2.0,
                         softening_scale: float = 1e-2, max_depth: int = 10,
                         leaf_size: int = 32, multipole_order: int = 2, walk_blocks: int = 0,
-                        workspace: Optional[torch.Tensor] = None,
+                        walk_threads: int = 0, workspace: Optional[torch.Tensor] = None,
                         stats: Optional[torch.Tensor] = None,
                         out: Optional[torch.Tensor] = None,
                         check_offsets: bool = True) -> torch.Tensor:
@@ -198,14 +198,15 @@ def gravity_force_batch(points, masses, offsets, *, beta: float = 2.0,
     forces = out if out is not None else torch.empty_like(points)
     call("sbh_gravity_batch_force", stream_handle(), buf(points), buf(masses), buf(offsets),
          buf(forces), f32(beta), f32(softening_scale), i32(max_depth), i32(leaf_size),
-         i32(multipole_order), i32(walk_blocks), buf(workspace), buf(stats))
+         i32(multipole_order), i32(walk_blocks), i32(walk_threads), buf(workspace), buf(stats))
     return forces
 
 
 def gravity_force_vjp_batch(points, masses, cotangent, offsets, *, beta: float = 2.0,
                             softening_scale: float = 1e-2, max_depth: int = 10,
                             leaf_size: int = 32, multipole_order: int = 2,
-                            walk_blocks: int = 0, workspace: Optional[torch.Tensor] = None,
+                            walk_blocks: int = 0, walk_threads: int = 0,
+                            workspace: Optional[torch.Tensor] = None,
                             stats: Optional[torch.Tensor] = None,
                             forces_out: Optional[torch.Tensor] = None,
                             grad_points_out: Optional[torch.Tensor] = None,
@@ -223,7 +224,7 @@ def gravity_force_vjp_batch(points, masses, cotangent, offsets, *, beta: float =
     call("sbh_gravity_batch_force_vjp", stream_handle(), buf(points), buf(masses),
          buf(cotangent), buf(offsets), buf(forces_out), buf(gx), buf(gm), f32(beta),
          f32(softening_scale), i32(max_depth), i32(leaf_size), i32(multipole_order),
-         i32(walk_blocks), buf(workspace), buf(stats))
+         i32(walk_blocks), i32(walk_threads), buf(workspace), buf(stats))
     return gx, gm
 
 

@@ -530,8 +530,9 @@ int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
  * set (the persistent walk drains the groups of all sets from one queue).
  * Per set, results equal sbh_gravity_force / sbh_gravity_force_vjp on that
  * set alone.  Offsets that do not rise from 0 to N leave every set empty and
- * set stats[5] (overflow) to 2.  walk_blocks: persistent walk grid (0 =
- * default).  No reference counterpart (the reference runs one launch per
+ * set stats[5] (overflow) to 2.  walk_blocks: persistent walk grid in
+ * blocks (0 = default, 8192 waves); walk_threads: its block size, 64, 128,
+ * 256 or 512 (0 = 256).  No reference counterpart (the reference runs one launch per
  * call, python/benchmark_octree8.py). */
 int64_t sbh_gravity_batch_workspace_bytes(int64_t total_points, int32_t num_sets,
                                           int32_t max_depth, int32_t leaf_size);
@@ -540,15 +541,15 @@ int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
                             SbhBuffer* forces, float beta, float softening_scale,
                             int32_t max_depth, int32_t leaf_size,
                             int32_t multipole_order, int32_t walk_blocks,
-                            SbhBuffer* workspace, SbhBuffer* stats);
+                            int32_t walk_threads, SbhBuffer* workspace, SbhBuffer* stats);
 int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points,
                                 const SbhBuffer* masses, const SbhBuffer* cotangent,
                                 const SbhBuffer* set_offsets, SbhBuffer* forces,
                                 SbhBuffer* grad_points, SbhBuffer* grad_masses,
                                 float beta, float softening_scale, int32_t max_depth,
                                 int32_t leaf_size, int32_t multipole_order,
-                                int32_t walk_blocks, SbhBuffer* workspace,
-                                SbhBuffer* stats);
+                                int32_t walk_blocks, int32_t walk_threads,
+                                SbhBuffer* workspace, SbhBuffer* stats);
 
 /* ---- stochastic Barnes-Hut on the adaptive octree (new targets) --------
  *

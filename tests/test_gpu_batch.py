@@ -139,3 +139,26 @@ def test_graph_replay_matches_eager(gpu):
         torch.testing.assert_close(gx, gxe, rtol=1e-4, atol=1e-5 * float(gxe.abs().max()))
         torch.testing.assert_close(gm, gme, rtol=1e-4, atol=1e-5 * float(gme.abs().max()))
     g.close()
+
+
+@pytest.mark.parametrize("threads", [64, 128, 512])
+def test_walk_block_sizes_give_the_same_results(gpu, threads):
+    # every query group is walked by one wave whatever the block size, so
+    # forces and statistics are bit-equal to the default 256-thread walk
+    G = _G()
+    sizes = [9000, 0, 2500, 17, 30000]
+    ps, ms, cs = _sets(sizes, 9)
+    P, M, C = _dev(ps, ms, cs)
+    offs = G.set_offsets(sizes)
+    sa, sb = G.new_stats(), G.new_stats()
+    Fa = G.gravity_force_batch(P, M, offs, stats=sa, **KW)
+    Fb = G.gravity_force_batch(P, M, offs, stats=sb, walk_threads=threads, walk_blocks=97, **KW)
+    ga, gma = G.gravity_force_vjp_batch(P, M, C, offs, **KW)
+    gb, gmb = G.gravity_force_vjp_batch(P, M, C, offs, walk_threads=threads, **KW)
+    torch.cuda.synchronize()
+    assert torch.equal(Fa, Fb)
+    assert torch.equal(sa, sb) and int(sa[5]) == 0
+    assert float(torch.linalg.norm(ga - gb) / torch.linalg.norm(ga)) < 1e-5
+    assert float(torch.linalg.norm(gma - gmb) / torch.linalg.norm(gma)) < 1e-5
+    with pytest.raises(ValueError, match="walk_threads"):
+        G.gravity_force_batch(P, M, offs, walk_threads=96, **KW)
