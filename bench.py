@@ -39,10 +39,12 @@ FP32_PEAK_NON_PACKED_TFLOPS = 78.6  # the same VALUs without packed-FP32 instruc
 # counts as one source whatever its expansion order.
 FLOPS_ALG_FWD, FLOPS_ALG_VJP = 20, 48
 # The implementation's own count: near pairs as above; a far node costs
-# 20 / 48 with the monopole and 54 / 200 with the quadrupole terms.
+# 20 / 48 with the monopole and 54 / 142 with the quadrupole terms (the FP32
+# operations, FMA = 2, that the walks' SQ_INSTS_VALU_FLOPS_FP32 counter
+# reports per far node: profiles/gravity_walk_plummer_n1048576_quad_valu_counters.md).
 FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
 FLOPS_FAR_FWD = {1: 20, 2: 54}
-FLOPS_FAR_VJP = {1: 48, 2: 200}
+FLOPS_FAR_VJP = {1: 48, 2: 142}
 OUTLIER_REL_THRESHOLD = 0.05  # per-row relative error counted as an outlier
 
 
@@ -70,7 +72,7 @@ def valu_roofline(kernel, flops_alg, flops_impl, ms, traffic=None, **extra):
            "flops_per_launch": flops_alg, "ms_per_launch": ms,
            "frac_of_non_packed_peak": a / FP32_PEAK_NON_PACKED_TFLOPS,
            "implementation_count": {
-               "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole) or 54 / 200 "
+               "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole) or 54 / 142 "
                              "(quadrupole)",
                "achieved": b, "frac": b / FP32_PEAK_TFLOPS,
                "frac_of_non_packed_peak": b / FP32_PEAK_NON_PACKED_TFLOPS,
@@ -105,6 +107,24 @@ def direct_fp64(P, M, Q, eps, chunk=16):
         d = q[:, None, :] - P[None]
         s3 = ((d * d).sum(-1) + e2) ** -1.5
         out.append(-((M[None] * s3)[..., None] * d).sum(1))
+    return torch.cat(out)
+
+
+def direct_vjp_fp64(P, M, Q, C, eps, chunk=16):
+    """FP64 query-side VJP of the softened direct force at queries Q with
+    cotangents C: sum_k m_k (3 s^5 (c . d) d - s^3 c), d = q - x_k."""
+    import torch
+
+    out = []
+    e2 = eps * eps + 1e-12
+    for q, c in zip(Q.split(chunk), C.split(chunk)):
+        d = q[:, None, :] - P[None]
+        r2 = (d * d).sum(-1) + e2
+        s3 = r2 ** -1.5
+        s5 = s3 / r2
+        cd = (d * c[:, None, :]).sum(-1)
+        out.append((M[None, :, None] * (3.0 * (s5 * cd)[..., None] * d -
+                                        s3[..., None] * c[:, None, :])).sum(1))
     return torch.cat(out)
 
 
@@ -315,6 +335,11 @@ def main():
                     help="quadtree / fields: number of points (queries = the points)")
     ap.add_argument("--fields-level", type=int, default=1, choices=(1, 2, 3))
     ap.add_argument("--seeds", type=int, default=256)
+    ap.add_argument("--stoch-seeds", type=int, default=4096,
+                    help="stochastic: seeds whose spread gives the standard error of the "
+                         "--seeds mean")
+    ap.add_argument("--no-octree8", action="store_true",
+                    help="stochastic: skip the dense octree8 record")
     ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--l3-queries", type=int, default=65536,
                     help="queries of the timed level-3 walk (stochastic workload)")
@@ -556,8 +581,9 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl, prof, d
         out[tag] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": gbs / HBM_PEAK_GBS, "bytes": b, "ms": ms,
                     "design_bytes": build_bytes_design(n, sf["nodes"], vjp),
-                    "note": "about 70 dependent launches per build; launch latency, not "
-                            "bandwidth, sets the time"}
+                    "note": "24 dependent launches per build (10 of them the internal "
+                            "moment levels); launch latency and the radix passes' "
+                            "look-back set the time, not bandwidth"}
     out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,quad={int(args.order == 2)},forces=1>",
                                     flops_fwd, flops_fwd_impl, walk_fwd_ms,
                                     traffic=(profiled_traffic("k_walk<false,true,true>")
@@ -735,6 +761,36 @@ def bench_wn(args):
     others["octree8_stochastic_depth7"] = {"ms": sb_ms, "samples_per_subdomain":
                                            args.wn_samples,
                                            "rel_l2_vs_fp64": rel_l2(sb[idx], ref)}
+    # the same estimator on the adaptive element tree (depth 10), one wave per
+    # query group sharing each sampled path
+    from sbh_amd import stochastic as S
+
+    wss = S.new_workspace(c.shape[0], m, 10, leaf, dev)
+    sto = torch.zeros(8, dtype=torch.int64, device=dev)
+    kws = dict(beta=args.beta, samples_per_subdomain=args.wn_samples, max_depth=10,
+               leaf_size=leaf, workspace=wss)
+    sa, sa_ms = timed(lambda: S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws))
+    S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, stats=sto, **kws)
+    ss = S.stats_dict(sto)
+    lib.sbh_profile_enable(ctypes.c_int32(1))
+    S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws)
+    torch.cuda.synchronize()
+    lib.sbh_profile_enable(ctypes.c_int32(0))
+    prof_s = (ctypes.c_double * 12)()
+    lib.sbh_profile_collect(prof_s, ctypes.c_int32(12))
+    s_walk_ms = prof_s[11] / max(prof_s[9], 1)
+    s_flops = FLOPS_WN * (ss["node_terms"] + ss["source_pairs"])
+    others["adaptive_stochastic"] = {
+        "ms": sa_ms, "samples_per_subdomain": args.wn_samples, "beta_diag": args.beta,
+        "max_depth": 10, "leaf_size": leaf, "rel_l2_vs_fp64": rel_l2(sa[idx], ref),
+        "terms": ss, "build_ms": prof_s[10] / max(prof_s[9], 1), "walk_kernel_ms": s_walk_ms,
+        "speedup_vs_octree8_stochastic": sb_ms / sa_ms,
+        "roofline": {"bound": "valu", "kernel": "k_stoch_walk (winding numbers)",
+                     "achieved": s_flops / (s_walk_ms * 1e-3) / 1e12,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": s_flops / (s_walk_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                     "flops_per_launch": s_flops,
+                     "flop_model": f"{FLOPS_WN} flop per lane node term or element pair"}}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_wn(c, nrm, a, q, leaf, m)
@@ -755,7 +811,7 @@ def bench_wn(args):
         "phases_ms": {"build": wn_build_ms, "walk_kernel": wn_walk_ms},
         "accuracy": acc,
         "others": others,
-        "roofline": {"bound": "mfma", "compute": "fp32 VALU (no matrix op in the walk)",
+        "roofline": {"bound": "valu", "compute": "fp32 VALU (no matrix op in the walk)",
                      "kernel": "k_wn_walk", "achieved": achieved,
                      "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
@@ -764,7 +820,195 @@ def bench_wn(args):
     }
 
 
+FLOPS_STOCH_TERM = {0: 20, 1: 34}  # per lane node term or source pair: force, query-side VJP
+
+
+def stochastic_zstats(sum1, sum2, first, nseeds, batch, ref_scale):
+    """z statistics of seed means against a target from running sums of the
+    per-seed deviations (sum1, sum2) and the `batch`-seed sum `first`: the
+    standard error of a mean comes from all `nseeds` seeds (heavy tails make
+    `batch` draws underestimate their own spread)."""
+    import numpy as np
+
+    sd = np.sqrt(np.maximum(sum2 - sum1 * sum1 / nseeds, 0.0) / (nseeds - 1))
+    ok = sd > 1e-12 * ref_scale
+    z_all = (sum1 / nseeds)[ok] / (sd[ok] / np.sqrt(nseeds))
+    z_b = (first / batch)[ok] / (sd[ok] / np.sqrt(batch))
+    return {f"frac_abs_z_lt_3_{batch}_seed_mean": float(np.mean(np.abs(z_b) < 3)),
+            f"rms_z_{batch}_seed_mean": float(np.sqrt(np.mean(z_b ** 2))),
+            f"frac_abs_z_lt_3_{nseeds}_seed_mean": float(np.mean(np.abs(z_all) < 3)),
+            f"rms_z_{nseeds}_seed_mean": float(np.sqrt(np.mean(z_all ** 2))),
+            "standard_error_from_seeds": nseeds}
+
+
 def bench_stochastic(args):
+    """Stochastic BH gravity on the adaptive octree at the Plummer workload of
+    the headline (N = 2^20, scale radius 1 truncated at 20, masses 2^-20,
+    softening 0.005, theta 0.5, max depth 10, leaf 32, monopole node terms,
+    one sample per subdomain): the reference's multi-level prefix control
+    variate (src/sbh_ffi.cu:2146-2442) with every point as a query, forward
+    and query-side VJP timed; unbiasedness on a query sample against the FP64
+    direct sum (the 256-seed mean, its standard error from --stoch-seeds
+    seeds).  The reference's dense octree8 targets at depth --o8-depth on the
+    same points are recorded beside it (the depth-10 octree8 does not fit)."""
+    import numpy as np
+    import torch
+
+    import sbh_amd
+    from sbh_amd import gravity as G
+    from sbh_amd import stochastic as S
+    from sbh_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    stream = torch.cuda.current_stream()
+    n, eps, depth, leaf = args.n, args.eps, args.depth, args.leaf
+    p = synthetic.plummer(n, 2000)
+    m = np.full(n, 1.0 / n, np.float32)
+    c = np.random.default_rng(2002).normal(size=(n, 3)).astype(np.float32)
+    P, M, C = (torch.tensor(a, device=dev) for a in (p, m, c))
+    kw = dict(beta=args.beta, samples_per_subdomain=1, softening_scale=eps, max_depth=depth,
+              leaf_size=leaf, multipole_order=1)
+    ws = S.new_workspace(n, n, depth, leaf, dev)
+    out = torch.empty_like(P)
+    lib = sbh_amd.load_library()
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    steps = max(1, args.steps)
+    fwd = lambda: S.stochastic_gravity_force(P, M, P, seed=1, workspace=ws, out=out, **kw)
+    vjp = lambda: S.stochastic_gravity_force_vjp(P, M, P, C, seed=1, workspace=ws, out=out,
+                                                 **kw)
+    fwd_ms = timed(fwd, steps)
+    vjp_ms = timed(vjp, steps)
+    phases = {}
+    for name, fn in (("fwd", fwd), ("vjp", vjp)):
+        lib.sbh_profile_enable(ctypes.c_int32(1))
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        lib.sbh_profile_enable(ctypes.c_int32(0))
+        prof = (ctypes.c_double * 12)()
+        lib.sbh_profile_collect(prof, ctypes.c_int32(12))
+        phases[f"{name}_build"] = prof[10] / max(prof[9], 1)
+        phases[f"{name}_walk_kernel"] = prof[11] / max(prof[9], 1)
+    stf, stv = torch.zeros(8, dtype=torch.int64, device=dev), torch.zeros(8, dtype=torch.int64,
+                                                                          device=dev)
+    F1 = S.stochastic_gravity_force(P, M, P, seed=1, workspace=ws, stats=stf, **kw).clone()
+    S.stochastic_gravity_force_vjp(P, M, P, C, seed=1, workspace=ws, stats=stv, out=out, **kw)
+    sf, sv = S.stats_dict(stf), S.stats_dict(stv)
+    flops_f = FLOPS_STOCH_TERM[0] * (sf["node_terms"] + sf["source_pairs"])
+    flops_v = FLOPS_STOCH_TERM[1] * (sv["node_terms"] + sv["source_pairs"])
+
+    # accuracy of one seed and of the classic walk on a query sample
+    rng = np.random.default_rng(7)
+    idx = torch.tensor(rng.choice(n, args.accuracy_queries, replace=False), device=dev)
+    ref = direct_fp64(P.double(), M.double(), P[idx].double(), eps)
+    Fc = G.gravity_force(P, M, beta=args.beta, softening_scale=eps, max_depth=depth,
+                         leaf_size=leaf, multipole_order=1)
+    acc = {"stochastic_one_seed_rel_l2_vs_fp64_direct": rel_l2(F1[idx], ref),
+           "classic_rel_l2_vs_fp64_direct": rel_l2(Fc[idx], ref),
+           "sample_queries": args.accuracy_queries}
+
+    # unbiasedness: 512 of the points (force) and 512 independent draws from
+    # the same sphere (VJP; at a query on a source the self pair's m c / eps^3
+    # is sampled too rarely to estimate its standard error), seeds
+    # 0..nseeds-1 of the same tree
+    nseeds, batch = max(args.stoch_seeds, args.seeds), args.seeds
+    q = p[rng.choice(n, 512, replace=False)]
+    qv = synthetic.plummer(512, 2003)
+    cq = rng.normal(size=(512, 3)).astype(np.float32)
+    Q, QV, CQ = (torch.tensor(a, device=dev) for a in (q, qv, cq))
+    wsq = S.new_workspace(n, 512, depth, leaf, dev)
+    refs = {"force": direct_fp64(P.double(), M.double(), Q.double(), eps)}
+    refs["vjp"] = direct_vjp_fp64(P.double(), M.double(), QV.double(), CQ.double(), eps)
+    s1 = {k: torch.zeros_like(v) for k, v in refs.items()}
+    s2 = {k: torch.zeros_like(v) for k, v in refs.items()}
+    first = {}
+    t0 = time.perf_counter()
+    for seed in range(nseeds):
+        got = {"force": S.stochastic_gravity_force(P, M, Q, seed=seed, workspace=wsq, **kw),
+               "vjp": S.stochastic_gravity_force_vjp(P, M, QV, CQ, seed=seed, workspace=wsq,
+                                                     **kw)}
+        for k, g in got.items():
+            d = g.double() - refs[k]
+            s1[k] += d
+            s2[k] += d * d
+        if seed == batch - 1:
+            first = {k: v.cpu().numpy().copy() for k, v in s1.items()}
+    torch.cuda.synchronize()
+    unbiased = {}
+    for k in ("force", "vjp"):
+        r = refs[k].cpu().numpy()
+        unbiased[k] = stochastic_zstats(s1[k].cpu().numpy(), s2[k].cpu().numpy(), first[k],
+                                        nseeds, batch, float(np.abs(r).max()))
+        unbiased[k][f"{batch}_seed_mean_rel_l2_vs_fp64_direct"] = float(
+            np.linalg.norm(first[k] / batch) / np.linalg.norm(r))
+    unbiased["queries"] = {"force": "512 of the points", "vjp": "512 independent draws from "
+                           "the same Plummer sphere"}
+    unbiased["seconds"] = round(time.perf_counter() - t0, 2)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        T = O.Tree(p, m, max_depth=depth, leaf_size=leaf)
+        ns = 1 << 14
+        qs = p[rng.choice(n, ns, replace=False)]
+        t0 = time.perf_counter()
+        T.stoch_eval(qs, kind=0, beta=args.beta, eps=eps, samples=1, seed=1)
+        t = (time.perf_counter() - t0) / ns
+        cpu = {"value": 1.0 / t, "unit": "forces/s", "cores": O.num_threads(), "kind": "port",
+               "sample": f"oracle stochastic walk over the same adaptive tree (same paths, "
+                         f"fp64, OpenMP over query groups) on {ns} of {n} queries, query "
+                         f"tree included, source tree build excluded, scaled"}
+    octree8 = None if args.no_octree8 else measure_octree8_stochastic(args)
+    achieved = flops_f / (phases["fwd_walk_kernel"] * 1e-3) / 1e12
+    return {
+        "metric": "stochastic BH forces/sec at N=1M (adaptive octree, depth 10, leaf 32, 1 "
+                  "sample per subdomain)",
+        "value": n / (fwd_ms * 1e-3),
+        "unit": "forces/s",
+        "n_gpus": 1, "steps": steps, "warmup": 1, "ms_per_step": fwd_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plummer_sphere_n{n}_theta{1.0 / args.beta:g}_depth{depth}"
+                               f"_leaf{leaf}_eps{eps}_stochastic_s1",
+                   "n": n, "queries": n, "max_depth": depth, "leaf_size": leaf,
+                   "beta_diag": args.beta, "softening": eps, "samples_per_subdomain": 1,
+                   "multipole_order": 1,
+                   "step": "source tree build + query tree build + stochastic walk"},
+        "vjp_ms_per_step": vjp_ms,
+        "phases_ms": phases,
+        "terms": {"fwd": sf, "vjp": sv},
+        "accuracy": acc,
+        "unbiasedness": unbiased,
+        "roofline": {"bound": "valu", "compute": "fp32 VALU (no matrix op in the walk)",
+                     "kernel": "k_stoch_walk (forces)", "achieved": achieved,
+                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS,
+                     "frac_of_non_packed_peak": achieved / FP32_PEAK_NON_PACKED_TFLOPS,
+                     "traffic": None, "flops_per_launch": flops_f,
+                     "flop_model": "20 flop per lane node term or source pair (forward), 34 "
+                                   "(query-side VJP)",
+                     "vjp": {"kernel": "k_stoch_walk (query-side VJP)",
+                             "achieved": flops_v / (phases["vjp_walk_kernel"] * 1e-3) / 1e12,
+                             "flops_per_launch": flops_v}},
+        "cpu_baseline": cpu,
+        "octree8_drop_in": octree8,
+    }
+
+
+def measure_octree8_stochastic(args):
     """Classic and stochastic BH gravity on the reference's dense octree8 at
     the headline's Plummer sphere (the octree8 handlers of src/sbh_ffi.cu:
     3183-3248 and 3375-3443): one classic launch, one stochastic launch with
@@ -1228,8 +1472,8 @@ def measure_batched(args, dev, tune=True, cpu=False):
     if tune:
         tuned = autotune.tune_batch(P, M, C, offs, reps=2, **kw)
     else:
-        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "table": None}
-    leaf, blocks = tuned["leaf_size"], tuned["walk_blocks"]
+        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "walk_threads": 0, "table": None}
+    leaf, blocks, threads = tuned["leaf_size"], tuned["walk_blocks"], tuned["walk_threads"]
     stream = torch.cuda.current_stream()
     steps = max(1, args.steps)
 
@@ -1246,7 +1490,7 @@ def measure_batched(args, dev, tune=True, cpu=False):
         return e0.elapsed_time(e1) / nsteps
 
     # batched: one forest pipeline, captured as a HIP graph
-    step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw)
+    step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw, threads)
     step()
     g = Graph()
     g.capture(step)
@@ -1301,10 +1545,12 @@ def measure_batched(args, dev, tune=True, cpu=False):
     rgx, rgm = direct_adjoint_rows_fp64(Pb, Mb, Cb, rows, args.eps)
     st = G.new_stats(dev)
     G.gravity_force_vjp_batch(P, M, C, offs, stats=st, leaf_size=leaf, walk_blocks=blocks,
+                              walk_threads=threads,
                               workspace=ws, grad_points_out=gx, grad_masses_out=gm, **kw)
     sv = G.stats_dict(st)
     stf = G.new_stats(dev)
     G.gravity_force_batch(P, M, offs, stats=stf, leaf_size=leaf, walk_blocks=blocks,
+                          walk_threads=threads,
                           workspace=ws, out=F, **kw)
     sf = G.stats_dict(stf)
     torch.cuda.synchronize()
@@ -1324,7 +1570,7 @@ def measure_batched(args, dev, tune=True, cpu=False):
            "sets": args.sets, "total_points": n_total, "min_set": int(sizes.min()),
            "max_set": int(sizes.max()), "softening": args.eps, "beta_diag": args.beta,
            "max_depth": args.depth, "multipole_order": args.order,
-           "leaf_size": leaf, "walk_blocks": blocks,
+           "leaf_size": leaf, "walk_blocks": blocks, "walk_threads": threads,
            "batched_graph_ms": ms_graph, "batched_eager_ms": ms_eager,
            "per_set_eager_ms": ms_per_set, "per_set_graph_ms": ms_per_set_graph,
            "speedup_vs_per_set_graph": ms_per_set_graph / ms_graph,
@@ -1369,7 +1615,7 @@ def bench_batched(args):
         "data": "synthetic",
         "config": {k: b[k] for k in ("workload", "sets", "total_points", "min_set", "max_set",
                                      "softening", "beta_diag", "max_depth", "multipole_order",
-                                     "leaf_size", "walk_blocks")},
+                                     "leaf_size", "walk_blocks", "walk_threads")},
         "batched": {k: v for k, v in b.items() if k not in ("roofline", "cpu_baseline")},
         "roofline": b["roofline"],
         "cpu_baseline": b.get("cpu_baseline"),
