@@ -53,9 +53,8 @@ WnLayout wn_layout(int64_t n, int64_t m, int max_depth, int leaf_size) {
   return W;
 }
 
-// ------------------------------------------------------------------ moments
-// Bottom-up, one level per launch (children are finished before parents).
-// The level-0 launch also clears the sentinel node's dipole.
+// The dipoles are computed by the tree build's moment pass (tree::build with
+// a dipole array), which also clears the sentinel node's entry.
 // ------------------------------------------------------------------ walk
 struct WnArgs {
   const Header* hdr;  // element tree

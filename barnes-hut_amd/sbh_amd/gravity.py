@@ -109,31 +109,42 @@ def gravity_force_vjp(points, masses, cotangent, *, beta: float = 2.0,
 
 class _GravityRecompute(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, points, masses, kw):
+    def forward(ctx, points, masses, kw, vjp_order):
         ctx.save_for_backward(points, masses)
         ctx.kw = kw
+        ctx.vjp_order = vjp_order
         return gravity_force(points, masses, **kw)
 
     @staticmethod
     def backward(ctx, cot):
         points, masses = ctx.saved_tensors
-        gx, gm = gravity_force_vjp(points, masses, cot.contiguous(), **ctx.kw)
-        return gx, gm, None
+        kw = dict(ctx.kw, multipole_order=ctx.vjp_order)
+        gx, gm = gravity_force_vjp(points, masses, cot.contiguous(), **kw)
+        return gx, gm, None, None
 
 
 def gravity_force_recompute_vjp(points, masses, *, beta: float = 2.0,
                                 softening_scale: float = 1e-2, max_depth: int = 10,
                                 leaf_size: int = 32, multipole_order: int = 2,
+                                vjp_multipole_order: Optional[int] = None,
                                 workspace: Optional[torch.Tensor] = None) -> torch.Tensor:
     """Differentiable forces: backward rebuilds the tree and runs the adjoint
-    walk (no interaction lists are stored)."""
+    walk (no interaction lists are stored).
+
+    multipole_order is the far-field expansion of the forward walk;
+    vjp_multipole_order (default: the same) that of the adjoint walk.  Both
+    approximate the same softened sum, so a monopole forward (the expansion
+    of the reference's octree8 node records) may be paired with a quadrupole
+    adjoint when the gradients need the higher order."""
+    vjp_order = multipole_order if vjp_multipole_order is None else vjp_multipole_order
     _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
+    _check(beta, softening_scale, max_depth, leaf_size, vjp_order)
     p = points if isinstance(points, torch.Tensor) and points.is_cuda else _f32(points)
     m = masses if isinstance(masses, torch.Tensor) and masses.is_cuda else _f32(masses)
     kw = dict(beta=float(beta), softening_scale=float(softening_scale), max_depth=int(max_depth),
               leaf_size=int(leaf_size), multipole_order=int(multipole_order),
               workspace=workspace)
-    return _GravityRecompute.apply(p, m, kw)
+    return _GravityRecompute.apply(p, m, kw, int(vjp_order))
 
 
 # ------------------------------------------------------------------ batches

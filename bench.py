@@ -188,7 +188,7 @@ def cpu_baseline(p, m, c, args, sample_groups):
     quad = args.order == 2
     cs = c[T.perm]
     t0 = time.perf_counter()
-    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=quad)
+    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=args.fwd_order == 2)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
     T.eval_groups(0, 0, beta=args.beta, eps=args.eps, use_quad=quad, cot_sorted=cs)
@@ -291,8 +291,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--order", type=int, default=2, choices=(1, 2),
-                    help="1 = monopole, 2 = monopole + quadrupole (default: the monopole "
-                         "misses rel_l2 <= 1e-3 on the gradients at N = 2^20)")
+                    help="far-field expansion of the adjoint walk: 1 = monopole, 2 = "
+                         "monopole + quadrupole (default: the monopole adjoint misses "
+                         "rel_l2 <= 1e-3 on the gradients at N = 2^20)")
+    ap.add_argument("--fwd-order", type=int, default=1, choices=(1, 2),
+                    help="far-field expansion of the forward walk (default 1, the monopole "
+                         "of the reference's octree8 node records: forces at rel_l2 ~1e-4 "
+                         "at N = 2^20)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one device-to-device copy of this many MiB before the timed "
                          "region (reference kernel for HBM byte counters)")
@@ -315,13 +320,16 @@ def main():
                          "HIP graph and the replay is the reported value)")
     ap.add_argument("--workload", default="gravity",
                     choices=("gravity", "wn", "batched", "stochastic", "treepm", "quadtree",
-                             "fields", "config1"),
+                             "fields", "config1", "vjp_blobs"),
                     help="gravity: the headline fwd+VJP step; wn: winding numbers of a "
                          "204,020-face perturbed geodesic sphere at 2^20 queries; batched: "
                          "fwd+VJP of 64 blob sets (log-uniform sizes 1,024..524,288), one "
                          "graph-captured forest pipeline vs 64 per-set calls; stochastic: "
-                         "classic and stochastic BH on the dense octree8 at the headline "
-                         "Plummer sphere, with the seed-mean unbiasedness check; treepm: the "
+                         "stochastic BH (forces and query-side VJP) on the adaptive tree at the "
+                         "headline Plummer sphere, with the seed-mean unbiasedness check, and "
+                         "on the dense octree8 beside it; vjp_blobs: the custom-VJP path on "
+                         "2^18 points in 64 blobs (eps 0.002) through torch.autograd; "
+                         "config1: 10,000 uniform points with the CPU legs; treepm: the "
                          "short-range TreePM force (periodic, erfc cutoff, prune) on 128^3 "
                          "clustered particles; quadtree: the 2-D targets over Tree2D<1,3> "
                          "buffers (device build + vote walk, and the stochastic walk) on 2^20 "
@@ -339,7 +347,9 @@ def main():
                     help="stochastic: seeds whose spread gives the standard error of the "
                          "--seeds mean")
     ap.add_argument("--no-octree8", action="store_true",
-                    help="stochastic: skip the dense octree8 record")
+                    help="stochastic, wn: skip the dense octree8 legs")
+    ap.add_argument("--no-stochastic", action="store_true",
+                    help="wn: skip the stochastic estimate on the adaptive tree")
     ap.add_argument("--stoch-queries", type=int, default=4096)
     ap.add_argument("--l3-queries", type=int, default=65536,
                     help="queries of the timed level-3 walk (stochastic workload)")
@@ -351,13 +361,14 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of "
-                                   "it (queries would shard by Morton range, see DESIGN.md)",
+                                   "it (the query groups would shard by Morton range, one "
+                                   "range per GPU, over a replicated tree)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     bench = {"gravity": bench_gravity, "wn": bench_wn, "batched": bench_batched,
              "stochastic": bench_stochastic, "treepm": bench_treepm,
              "quadtree": bench_quadtree, "fields": bench_fields,
-             "config1": bench_config1}[args.workload]
+             "config1": bench_config1, "vjp_blobs": bench_vjp_blobs}[args.workload]
     rec = bench(args)
     print(json.dumps(rec))
 
@@ -381,12 +392,13 @@ def bench_gravity(args):
     C = torch.tensor(c, device=dev)
     kw = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth,
               leaf_size=args.leaf, multipole_order=args.order)
+    kw_f = dict(kw, multipole_order=args.fwd_order)
     ws = G.new_workspace(n, args.depth, args.leaf, dev)
     F = torch.empty_like(P)
     GX, GM = torch.empty_like(P), torch.empty_like(M)
 
     def step():
-        G.gravity_force(P, M, workspace=ws, out=F, **kw)
+        G.gravity_force(P, M, workspace=ws, out=F, **kw_f)
         return G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX,
                                    grad_masses_out=GM, **kw)
 
@@ -438,13 +450,13 @@ def bench_gravity(args):
 
     # interaction counts (deterministic: identical every step)
     sf, sv = G.new_stats(dev), G.new_stats(dev)
-    G.gravity_force(P, M, workspace=ws, stats=sf, **kw)
+    G.gravity_force(P, M, workspace=ws, stats=sf, **kw_f)
     G.gravity_force_vjp(P, M, C, workspace=ws, stats=sv, **kw)
     sf, sv = G.stats_dict(sf), G.stats_dict(sv)
     walk_vjp_ms = prof[5] / max(prof[3], 1)
     walk_fwd_ms = prof[2] / max(prof[0], 1)
     flops_vjp, flops_vjp_impl = walk_flops(sv, True, args.order)
-    flops_fwd, flops_fwd_impl = walk_flops(sf, False, args.order)
+    flops_fwd, flops_fwd_impl = walk_flops(sf, False, args.fwd_order)
 
     # accuracy against the FP64 direct sum on a query sample: forces, and both
     # gradients against the FP64 direct adjoint of the same rows
@@ -461,6 +473,35 @@ def bench_gravity(args):
            "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(gm[rows], rgm),
            "grad_points_outlier_frac_vs_fp64_direct_adjoint": outlier_frac(gx[rows], rgx),
            "grad_sample_rows": int(rows.numel())}
+    # the same step with the other forward expansion (graph replay), beside
+    alt = None
+    if not args.no_graph:
+        from sbh_amd.graph import Graph
+
+        kw_alt = dict(kw, multipole_order=3 - args.fwd_order)
+        F_alt = torch.empty_like(P)
+
+        def step_alt():
+            G.gravity_force(P, M, workspace=ws, out=F_alt, **kw_alt)
+            G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX,
+                                grad_masses_out=GM, **kw)
+
+        g = Graph()
+        g.capture(step_alt)
+        g.replay()
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            g.replay()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        alt_ms = ev0.elapsed_time(ev1) / args.steps
+        g.close()
+        alt = {"forward_order": 3 - args.fwd_order, "vjp_order": args.order,
+               "graph_ms_per_step": alt_ms, "forces_per_s": n / (alt_ms * 1e-3),
+               "force_rel_l2_vs_fp64_direct": rel_l2(F_alt[idx], ref),
+               "force_outlier_frac_vs_fp64_direct": outlier_frac(F_alt[idx], ref)}
+        del F_alt
     direct = None
     if not args.no_direct:
         from sbh_amd import ffi
@@ -513,7 +554,8 @@ def bench_gravity(args):
             "workload": f"plummer_sphere_n{n}_theta0.5_depth{args.depth}_leaf{args.leaf}"
                         f"_eps{args.eps}_fwd+vjp",
             "n": n, "theta": 1.0 / args.beta, "beta_diag": args.beta, "max_depth": args.depth,
-            "leaf_size": args.leaf, "softening": args.eps, "multipole_order": args.order,
+            "leaf_size": args.leaf, "softening": args.eps,
+            "multipole_order": {"forward": args.fwd_order, "vjp": args.order},
             "step": "forward (build+walk) + recompute VJP (rebuild+adjoint walk), full "
                     "position and mass gradients",
             "batched": "see the batched record: 64 blob sets, one graph-captured forest "
@@ -526,6 +568,7 @@ def bench_gravity(args):
                       "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
         "interactions": {"fwd": sf, "vjp": sv},
         "accuracy": acc,
+        "other_forward_order": alt,
         "direct_sum": direct,
         "roofline": valu_roofline(
             f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0>", flops_vjp, flops_vjp_impl,
@@ -584,16 +627,18 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl, prof, d
                     "note": "24 dependent launches per build (10 of them the internal "
                             "moment levels); launch latency and the radix passes' "
                             "look-back set the time, not bandwidth"}
-    out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,quad={int(args.order == 2)},forces=1>",
+    out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,quad={int(args.fwd_order == 2)},forces=1>",
                                     flops_fwd, flops_fwd_impl, walk_fwd_ms,
                                     traffic=(profiled_traffic("k_walk<false,true,true>")
-                                             if args.order == 2 and n == 1 << 20 else None))
+                                             if args.fwd_order == 2 and n == 1 << 20
+                                             else None))
     # walks against HBM: profiled traffic over the kernel time, beside the
     # minimal bytes (40 B/pt + 48 B/node; the adjoint +36 B/pt)
-    for tag, key, ms, extra in (("walk_fwd_hbm", "k_walk<false,true,true>", walk_fwd_ms, 0),
-                                ("walk_vjp_hbm", "k_walk<true,true,false>", vjp_walk_ms, 36)):
+    for tag, key, ms, extra, order in (
+            ("walk_fwd_hbm", "k_walk<false,true,true>", walk_fwd_ms, 0, args.fwd_order),
+            ("walk_vjp_hbm", "k_walk<true,true,false>", vjp_walk_ms, 36, args.order)):
         traffic = profiled_traffic(key)
-        if traffic is None or args.order != 2 or n != 1 << 20:
+        if traffic is None or order != 2 or n != 1 << 20:
             continue
         gbs = traffic / (ms * 1e-3) / 1e9
         out[tag] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -753,44 +798,47 @@ def bench_wn(args):
         others["brute_force"] = {"ms": bf_ms, "speedup_tree_vs_brute": bf_ms / ms_step,
                                  "rel_l2_vs_fp64": rel_l2(bf[idx], ref),
                                  "tflops": FLOPS_WN * m * c.shape[0] / (bf_ms * 1e-3) / 1e12}
-    lb, nb = ffi.build_octree8_buffers(Pc, Nn, A, max_depth=7)
-    cl, cl_ms = timed(lambda: wn.eval_barnes_hut_wn(lb, nb, Q, beta=args.beta))
-    others["octree8_classic_depth7"] = {"ms": cl_ms, "rel_l2_vs_fp64": rel_l2(cl[idx], ref)}
-    sb, sb_ms = timed(lambda: wn.multi_level_prefix_control_variate_wn(
-        lb, nb, Q, samples_per_subdomain=args.wn_samples, seed=0, beta=1.0), reps=1)
-    others["octree8_stochastic_depth7"] = {"ms": sb_ms, "samples_per_subdomain":
-                                           args.wn_samples,
-                                           "rel_l2_vs_fp64": rel_l2(sb[idx], ref)}
-    # the same estimator on the adaptive element tree (depth 10), one wave per
-    # query group sharing each sampled path
-    from sbh_amd import stochastic as S
+    sb_ms = None
+    if not args.no_octree8:
+        lb, nb = ffi.build_octree8_buffers(Pc, Nn, A, max_depth=7)
+        cl, cl_ms = timed(lambda: wn.eval_barnes_hut_wn(lb, nb, Q, beta=args.beta))
+        others["octree8_classic_depth7"] = {"ms": cl_ms, "rel_l2_vs_fp64": rel_l2(cl[idx], ref)}
+        sb, sb_ms = timed(lambda: wn.multi_level_prefix_control_variate_wn(
+            lb, nb, Q, samples_per_subdomain=args.wn_samples, seed=0, beta=1.0), reps=1)
+        others["octree8_stochastic_depth7"] = {"ms": sb_ms, "samples_per_subdomain":
+                                               args.wn_samples,
+                                               "rel_l2_vs_fp64": rel_l2(sb[idx], ref)}
+    if not args.no_stochastic:
+        # the same estimator on the adaptive element tree (depth 10), one wave per
+        # query group sharing each sampled path
+        from sbh_amd import stochastic as S
 
-    wss = S.new_workspace(c.shape[0], m, 10, leaf, dev)
-    sto = torch.zeros(8, dtype=torch.int64, device=dev)
-    kws = dict(beta=args.beta, samples_per_subdomain=args.wn_samples, max_depth=10,
-               leaf_size=leaf, workspace=wss)
-    sa, sa_ms = timed(lambda: S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws))
-    S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, stats=sto, **kws)
-    ss = S.stats_dict(sto)
-    lib.sbh_profile_enable(ctypes.c_int32(1))
-    S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws)
-    torch.cuda.synchronize()
-    lib.sbh_profile_enable(ctypes.c_int32(0))
-    prof_s = (ctypes.c_double * 12)()
-    lib.sbh_profile_collect(prof_s, ctypes.c_int32(12))
-    s_walk_ms = prof_s[11] / max(prof_s[9], 1)
-    s_flops = FLOPS_WN * (ss["node_terms"] + ss["source_pairs"])
-    others["adaptive_stochastic"] = {
-        "ms": sa_ms, "samples_per_subdomain": args.wn_samples, "beta_diag": args.beta,
-        "max_depth": 10, "leaf_size": leaf, "rel_l2_vs_fp64": rel_l2(sa[idx], ref),
-        "terms": ss, "build_ms": prof_s[10] / max(prof_s[9], 1), "walk_kernel_ms": s_walk_ms,
-        "speedup_vs_octree8_stochastic": sb_ms / sa_ms,
-        "roofline": {"bound": "valu", "kernel": "k_stoch_walk (winding numbers)",
-                     "achieved": s_flops / (s_walk_ms * 1e-3) / 1e12,
-                     "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": s_flops / (s_walk_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-                     "flops_per_launch": s_flops,
-                     "flop_model": f"{FLOPS_WN} flop per lane node term or element pair"}}
+        wss = S.new_workspace(c.shape[0], m, 10, leaf, dev)
+        sto = torch.zeros(8, dtype=torch.int64, device=dev)
+        kws = dict(beta=args.beta, samples_per_subdomain=args.wn_samples, max_depth=10,
+                   leaf_size=leaf, workspace=wss)
+        sa, sa_ms = timed(lambda: S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws))
+        S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, stats=sto, **kws)
+        ss = S.stats_dict(sto)
+        lib.sbh_profile_enable(ctypes.c_int32(1))
+        S.stochastic_winding_number(Pc, Nn, A, Q, seed=0, **kws)
+        torch.cuda.synchronize()
+        lib.sbh_profile_enable(ctypes.c_int32(0))
+        prof_s = (ctypes.c_double * 12)()
+        lib.sbh_profile_collect(prof_s, ctypes.c_int32(12))
+        s_walk_ms = prof_s[11] / max(prof_s[9], 1)
+        s_flops = FLOPS_WN * (ss["node_terms"] + ss["source_pairs"])
+        others["adaptive_stochastic"] = {
+            "ms": sa_ms, "samples_per_subdomain": args.wn_samples, "beta_diag": args.beta,
+            "max_depth": 10, "leaf_size": leaf, "rel_l2_vs_fp64": rel_l2(sa[idx], ref),
+            "terms": ss, "build_ms": prof_s[10] / max(prof_s[9], 1), "walk_kernel_ms": s_walk_ms,
+            "speedup_vs_octree8_stochastic": sb_ms / sa_ms if sb_ms else None,
+            "roofline": {"bound": "valu", "kernel": "k_stoch_walk (winding numbers)",
+                         "achieved": s_flops / (s_walk_ms * 1e-3) / 1e12,
+                         "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": s_flops / (s_walk_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                         "flops_per_launch": s_flops,
+                         "flop_model": f"{FLOPS_WN} flop per lane node term or element pair"}}
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline_wn(c, nrm, a, q, leaf, m)
@@ -1736,6 +1784,95 @@ def numpy_direct_softcut(points, masses, queries, eps, cutoff=float("inf")):
         coeff = -masses * att * inv_r3
         out[i] = np.sum(coeff[:, None] * disp, axis=0, dtype=np.float32)
     return out
+
+
+def bench_vjp_blobs(args):
+    """Custom-VJP path at 2^18 points in 64 Gaussian blobs (sigma 0.02,
+    log-normal(0, 0.5) masses normalised to 1, eps = 0.002, cotangent N(0, 1)):
+    forward through `gravity_force_recompute_vjp` and backward through
+    torch.autograd (rebuild + adjoint walk), as the reference's recompute test
+    python/test_jax_custom_vjp_recompute.py:105-151 drives its custom VJP.
+    Forces and both gradients against the FP64 direct sum and adjoint on
+    sampled rows; the GPU direct sum fwd + full VJP timed beside."""
+    import copy
+
+    import numpy as np
+    import torch
+
+    from sbh_amd import ffi
+    from sbh_amd import gravity as G
+    from sbh_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    a = copy.copy(args)
+    a.eps = 0.002
+    n = 1 << 18
+    p, m = synthetic.blobs(n, 64, 3000)
+    c = np.random.default_rng(3002).normal(size=(n, 3)).astype(np.float32)
+    P = torch.tensor(p, device=dev, requires_grad=True)
+    M = torch.tensor(m, device=dev, requires_grad=True)
+    C = torch.tensor(c, device=dev)
+    ws = G.new_workspace(n, a.depth, a.leaf, dev)
+    kw = dict(beta=a.beta, softening_scale=a.eps, max_depth=a.depth, leaf_size=a.leaf,
+              multipole_order=a.fwd_order, vjp_multipole_order=a.order, workspace=ws)
+
+    def step():
+        F = G.gravity_force_recompute_vjp(P, M, **kw)
+        gx, gm = torch.autograd.grad(F, (P, M), grad_outputs=C)
+        return F, gx, gm
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        F, gx, gm = step()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    rows = torch.tensor(np.random.default_rng(3003).choice(n, args.grad_rows, replace=False),
+                        device=dev)
+    Pd, Md, Cd = (t.detach().double() for t in (P, M, C))
+    ref = direct_fp64(Pd, Md, Pd[rows], a.eps)
+    rgx, rgm = direct_adjoint_rows_fp64(Pd, Md, Cd, rows, a.eps)
+    acc = {"force_rel_l2_vs_fp64_direct": rel_l2(F[rows], ref),
+           "force_outlier_frac_vs_fp64_direct": outlier_frac(F[rows], ref),
+           "grad_points_rel_l2_vs_fp64_direct_adjoint": rel_l2(gx[rows], rgx),
+           "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(gm[rows], rgm),
+           "grad_points_outlier_frac_vs_fp64_direct_adjoint": outlier_frac(gx[rows], rgx),
+           "sample_rows": int(rows.numel())}
+    Pn, Mn = P.detach(), M.detach()
+    t0, t1, t2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    t0.record(stream)
+    ffi.softened_direct_force(Pn, Mn, Pn, softening_scale=a.eps, cutoff_scale=float("inf"))
+    t1.record(stream)
+    ffi.softened_direct_force_full_vjp(Pn, Mn, C, softening_scale=a.eps,
+                                       cutoff_scale=float("inf"))
+    t2.record(stream)
+    torch.cuda.synchronize()
+    d_ms = t0.elapsed_time(t2)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, m, c, a, min(args.cpu_sample_groups, 4000))
+    return {
+        "metric": "BH forces/sec, custom-VJP path (torch.autograd), N = 2^18 blobs, fwd+VJP",
+        "value": n / (ms * 1e-3), "unit": "forces/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"blobs64_n{n}_sigma0.02_eps0.002_theta0.5_depth{a.depth}"
+                               f"_leaf{a.leaf}_custom_vjp",
+                   "n": n, "blobs": 64, "softening": a.eps, "beta_diag": a.beta,
+                   "max_depth": a.depth, "leaf_size": a.leaf,
+                   "multipole_order": {"forward": a.fwd_order, "vjp": a.order},
+                   "launch": "eager, torch.autograd"},
+        "accuracy": acc,
+        "direct_sum": {"fwd_plus_vjp_ms": d_ms, "fwd_ms": t0.elapsed_time(t1),
+                       "speedup_bh_vs_direct": d_ms / ms},
+        "cpu_baseline": cpu,
+    }
 
 
 def bench_config1(args):

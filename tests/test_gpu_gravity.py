@@ -202,6 +202,29 @@ def test_recompute_custom_vjp(gpu):
     np.testing.assert_allclose(_np(mt.grad), _np(gm), rtol=1e-4, atol=1e-4 * np.abs(_np(gm)).max())
 
 
+def test_recompute_custom_vjp_with_its_own_adjoint_order(gpu):
+    # monopole forward, quadrupole adjoint: the forward equals the order-1
+    # forces bit for bit, the gradients the order-2 VJP
+    G = _G()
+    n = 6000
+    p, m, c = _case("plummer", n, 19)
+    x = torch.tensor(p, device="cuda", requires_grad=True)
+    mt = torch.tensor(m, device="cuda", requires_grad=True)
+    ct = torch.tensor(c, device="cuda")
+    F = G.gravity_force_recompute_vjp(x, mt, softening_scale=0.005, multipole_order=1,
+                                      vjp_multipole_order=2)
+    (F * ct).sum().backward()
+    np.testing.assert_array_equal(_np(F), _np(G.gravity_force(p, m, softening_scale=0.005,
+                                                               multipole_order=1)))
+    gx, gm = G.gravity_force_vjp(p, m, c, softening_scale=0.005, multipole_order=2)
+    np.testing.assert_allclose(_np(x.grad), _np(gx), rtol=1e-4, atol=1e-4 * np.abs(_np(gx)).max())
+    np.testing.assert_allclose(_np(mt.grad), _np(gm), rtol=1e-4, atol=1e-4 * np.abs(_np(gm)).max())
+    g1x, _ = G.gravity_force_vjp(p, m, c, softening_scale=0.005, multipole_order=1)
+    assert O.rel_l2(_np(x.grad), _np(g1x)) > 1e-6  # the adjoint order did change
+    with pytest.raises(ValueError):
+        G.gravity_force_recompute_vjp(x, mt, multipole_order=1, vjp_multipole_order=3)
+
+
 def test_determinism_of_forces(gpu):
     G = _G()
     p, m, _ = _case("plummer", 30000, 4)
