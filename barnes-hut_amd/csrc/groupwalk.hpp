@@ -43,6 +43,9 @@ __device__ __forceinline__ int4 sload4(const int4* p, int i) {
 // 1/sqrt(x) with the hardware instruction; x >= 1e-12 wherever it is used,
 // far from the denormal range that rsqrtf's extra scaling handles.
 __device__ __forceinline__ float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+// Square root by the hardware instruction (about 1 ulp); sqrtf wraps it in a
+// correctly rounding sequence of about a dozen more VALU operations.
+__device__ __forceinline__ float sqrt_hw(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ float wave_min(float v) {
   for (int d = 32; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));

@@ -137,8 +137,8 @@ __device__ __forceinline__ void node_term(const StochArgs& A, int k, const Q& L,
     const float4 d = sload4(A.wd, k);
     const float Xx = m.x - L.x, Xy = m.y - L.y, Xz = m.z - L.z;
     const float R2 = Xx * Xx + Xy * Xy + Xz * Xz;
-    const float R = __builtin_sqrtf(R2);
-    ax += (d.x * Xx + d.y * Xy + d.z * Xz) * kInvFourPi * rcp(R * R2 + 1e-5f);
+    const float R = sqrt_hw(R2);
+    ax += (d.x * Xx + d.y * Xy + d.z * Xz) * kInvFourPi * rcp(fmaf(R, R2, 1e-5f));
   } else {
     const float Dx = L.x - m.x, Dy = L.y - m.y, Dz = L.z - m.z;
     const float s = rsq(fmaf(Dz, Dz, fmaf(Dy, Dy, fmaf(Dx, Dx, A.e2))));
@@ -209,8 +209,8 @@ __device__ __forceinline__ void leaf_sum(const StochArgs& A, int j0, int j1, con
         const float4 nv = sload4(A.sn, jj);
         const float dx = p.x - L.x, dy = p.y - L.y, dz = p.z - L.z;
         const float r2 = dx * dx + dy * dy + dz * dz;
-        const float r = __builtin_sqrtf(r2);
-        ax += w * (nv.x * dx + nv.y * dy + nv.z * dz) * kInvFourPi * rcp(r * r2 + 1e-5f);
+        const float r = sqrt_hw(r2);
+        ax += w * (nv.x * dx + nv.y * dy + nv.z * dz) * kInvFourPi * rcp(fmaf(r, r2, 1e-5f));
       } else {
         const float dx = L.x - p.x, dy = L.y - p.y, dz = L.z - p.z;
         const float s = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, A.e2))));

@@ -77,7 +77,8 @@ struct WnArgs {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// C far nodes (sentinel-padded) against the lane's query.
+// C far nodes (sentinel-padded) against the lane's query; like the near batch
+// below it returns 4 pi times the contribution (the caller scales once).
 template <int C>
 __device__ __forceinline__ float wn_far_batch(const WnArgs& A, float qx, float qy, float qz,
                                               const int* far) {
@@ -89,8 +90,8 @@ __device__ __forceinline__ float wn_far_batch(const WnArgs& A, float qx, float q
     const float4 d = sload4(A.wd, k);
     const float Xx = c.x - qx, Xy = c.y - qy, Xz = c.z - qz;
     const float R2 = Xx * Xx + Xy * Xy + Xz * Xz;
-    const float R = __builtin_sqrtf(R2);
-    w += (d.x * Xx + d.y * Xy + d.z * Xz) * kInvFourPi * rcp(R * R2 + 1e-5f);
+    const float R = sqrt_hw(R2);
+    w += (d.x * Xx + d.y * Xy + d.z * Xz) * rcp(fmaf(R, R2, 1e-5f));
   }
   return w;
 }
@@ -109,8 +110,8 @@ __device__ __forceinline__ float wn_near_batch(const WnArgs& A, float qx, float 
     const float a = (j < s1) ? p.w : 0.0f;
     const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    const float r = __builtin_sqrtf(r2);
-    w += a * (nv.x * dx + nv.y * dy + nv.z * dz) * kInvFourPi * rcp(r * r2 + 1e-5f);
+    const float r = sqrt_hw(r2);
+    w += a * (nv.x * dx + nv.y * dy + nv.z * dz) * rcp(fmaf(r, r2, 1e-5f));
   }
   return w;
 }
@@ -169,7 +170,7 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
     traverse(A.hdr, &A.qhdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack,
              far, near, flush_far,
              flush_near);
-    if (valid) A.out[A.qperm[i]] = w;
+    if (valid) A.out[A.qperm[i]] = w * kInvFourPi;
     ++n_groups;
   }
   if (A.stats && lane == 0) {
