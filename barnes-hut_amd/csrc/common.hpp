@@ -237,6 +237,11 @@ __device__ __forceinline__ uint32_t morton3(uint32_t x, uint32_t y, uint32_t z) 
 constexpr float kTwoOverSqrtPi = 2.0f / 1.772453850905516f;
 constexpr float kTwoSqrtPi = 2.0f * 1.772453850905516f;
 
+// 1/sqrt(x) by the hardware instruction (about 1 ulp).  Callers pass
+// r^2 + eps^2 + 1e-12 >= 1e-12, far from the denormal range that rsqrtf's
+// extra compare-and-scale (three more VALU operations) handles.
+__device__ __forceinline__ float rsq_softened(float x) { return __builtin_amdgcn_rsqf(x); }
+
 // Short-range attenuation erfc(u) + 2u/sqrt(pi) exp(-u^2), u = r / (2 r_c)
 // (src/sbh_ffi.cu:151-157).  Also returns exp(-u^2) and r.
 __device__ __forceinline__ float attenuation(float r2, float rc, float* exp_u2,

@@ -29,7 +29,7 @@ __device__ __forceinline__ void pair_force(float qx, float qy, float qz, float4 
   float ry = min_image_rint(qy - s.y, P.periodic, P.box, P.inv_box);
   float rz = min_image_rint(qz - s.z, P.periodic, P.box, P.inv_box);
   float r2 = rx * rx + ry * ry + rz * rz;
-  float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+  float inv_r = rsq_softened(r2 + P.eps2 + 1e-12f);
   float inv_r3 = inv_r * inv_r * inv_r;
   float att = 1.0f;
   if constexpr (CUT) {
@@ -50,7 +50,7 @@ __device__ __forceinline__ void pair_vjp(float qx, float qy, float qz, float gx,
   float ry = min_image_rint(qy - s.y, P.periodic, P.box, P.inv_box);
   float rz = min_image_rint(qz - s.z, P.periodic, P.box, P.inv_box);
   float r2 = rx * rx + ry * ry + rz * rz;
-  float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+  float inv_r = rsq_softened(r2 + P.eps2 + 1e-12f);
   float inv_r3 = inv_r * inv_r * inv_r;
   float inv_r5 = inv_r3 * inv_r * inv_r;
   float att = 1.0f, cc = 0.0f;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_full_vjp(
       float dy = min_image_rint(yk - s.y, P.periodic, P.box, P.inv_box);
       float dz = min_image_rint(zk - s.z, P.periodic, P.box, P.inv_box);
       float r2 = dx * dx + dy * dy + dz * dz;
-      float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+      float inv_r = rsq_softened(r2 + P.eps2 + 1e-12f);
       float s3 = inv_r * inv_r * inv_r;
       float s5 = s3 * inv_r * inv_r;
       float att = 1.0f, cc = 0.0f;

@@ -320,7 +320,7 @@ __device__ __forceinline__ void l3_term(const Leaf2& L, const float* q,
       c = hw_div(-L.mass, t * t * t);
     } else {
       const float r2 = dx * dx + dy * dy;
-      const float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+      const float inv_r = rsq_softened(r2 + P.eps2 + 1e-12f);
       float att = 1.0f;
       if (P.cutoff) {
         float e, dist;

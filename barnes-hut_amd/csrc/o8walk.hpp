@@ -154,7 +154,7 @@ __device__ __forceinline__ void contribution(float px, float py, float pz, float
     az += -m * (i3 * gz - k * dz);
   } else {
     float r2 = dx * dx + dy * dy + dz * dz;
-    float inv_r = rsqrtf(r2 + P.eps2 + 1e-12f);
+    float inv_r = rsq_softened(r2 + P.eps2 + 1e-12f);
     float s3 = inv_r * inv_r * inv_r;
     float att = 1.0f, cc = 0.0f;
     if (P.cutoff) {
