@@ -55,8 +55,10 @@ void destroy(Rec& r) {
 
 namespace {
 
+// order: 1 or 2; the VJP entry points also take 3 (monopole terms with the
+// quadrupole term's second-moment adjoint), passed as max_order.
 int check_common(const SbhBuffer* points, const SbhBuffer* masses, float beta, float softening,
-                 int max_depth, int leaf_size, int order) {
+                 int max_depth, int leaf_size, int order, int max_order = 2) {
   if (!buffer_ok(points, SBH_DTYPE_F32, 2) || points->dims[1] != 3)
     return fail_invalid("points must have shape (N, 3)");
   if (!buffer_ok(masses, SBH_DTYPE_F32, 1) || masses->dims[0] != points->dims[0])
@@ -66,7 +68,9 @@ int check_common(const SbhBuffer* points, const SbhBuffer* masses, float beta, f
   if (softening < 0.0f) return fail_invalid("softening_scale must be >= 0");
   if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
   if (leaf_size < 1) return fail_invalid("leaf_size must be >= 1");
-  if (order != 1 && order != 2) return fail_invalid("multipole_order must be 1 or 2");
+  if (order < 1 || order > max_order)
+    return fail_invalid(max_order == 2 ? "multipole_order must be 1 or 2"
+                                       : "multipole_order must be 1, 2 or 3");
   return SBH_OK;
 }
 
@@ -137,7 +141,7 @@ extern "C" int sbh_gravity_force(void* stream, const SbhBuffer* points, const Sb
                    nullptr);
   if (rc != SBH_OK) return rc;
   if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
-  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order,
                   static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
                   pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr);
   if (rc != SBH_OK) return rc;
@@ -155,7 +159,7 @@ extern "C" int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
                                      SbhBuffer* stats) {
   clear_error();
   int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
-                        multipole_order);
+                        multipole_order, 3);
   if (rc != SBH_OK) return rc;
   int64_t n = points->dims[0];
   if (!buffer_ok(cotangent, SBH_DTYPE_F32, 2) || cotangent->dims[0] != n ||
@@ -180,7 +184,7 @@ extern "C" int sbh_gravity_force_vjp(void* stream, const SbhBuffer* points,
                    static_cast<const float*>(cotangent->data), n, max_depth, leaf_size, nullptr);
   if (rc != SBH_OK) return rc;
   if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
-  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
                   stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr);
@@ -278,7 +282,7 @@ extern "C" int sbh_gravity_batch_force(void* stream, const SbhBuffer* points,
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
   if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
-  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order == 2,
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, false, multipole_order,
                   static_cast<float*>(forces->data), nullptr, nullptr, stats != nullptr,
                   pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks, walk_threads);
   if (rc != SBH_OK) return rc;
@@ -297,7 +301,7 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
                                            SbhBuffer* workspace, SbhBuffer* stats) {
   clear_error();
   int rc = check_common(points, masses, beta, softening_scale, max_depth, leaf_size,
-                        multipole_order);
+                        multipole_order, 3);
   if (rc != SBH_OK) return rc;
   int nsets = 0;
   if ((rc = check_batch(points, set_offsets, &nsets)) != SBH_OK) return rc;
@@ -328,7 +332,7 @@ extern "C" int sbh_gravity_batch_force_vjp(void* stream, const SbhBuffer* points
                           static_cast<const int32_t*>(set_offsets->data), max_depth, leaf_size);
   if (rc != SBH_OK) return rc;
   if (pr.on()) SBH_HIP_TRY(hipEventRecord(pr.rec().ev[1], s));
-  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
                   stats != nullptr, pr.on() && n > 0 ? pr.rec().ev + 2 : nullptr, walk_blocks, walk_threads);

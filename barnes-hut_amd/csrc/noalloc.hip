@@ -68,8 +68,8 @@ extern "C" int sbh_gravity_force_vjp_noalloc(void* stream, const SbhBuffer* poin
   if (softening_scale < 0.0f) return fail_invalid("softening_scale must be >= 0");
   if (max_depth < 1 || max_depth > 10) return fail_invalid("max_depth must be in [1, 10]");
   if (leaf_size < 1) return fail_invalid("leaf_size must be >= 1");
-  if (multipole_order != 1 && multipole_order != 2)
-    return fail_invalid("multipole_order must be 1 or 2");
+  if (multipole_order < 1 || multipole_order > 3)
+    return fail_invalid("multipole_order must be 1, 2 or 3");
   const int64_t n = points->dims[0];
   if (n > (int64_t{1} << 30)) return fail_invalid("too many points");
   if (!buffer_ok(cotangent, SBH_DTYPE_F32, 2) || cotangent->dims[0] != n ||
@@ -98,7 +98,7 @@ extern "C" int sbh_gravity_force_vjp_noalloc(void* stream, const SbhBuffer* poin
                    static_cast<const float*>(masses->data),
                    static_cast<const float*>(cotangent->data), n, max_depth, leaf_size, nullptr);
   if (rc != SBH_OK) return rc;
-  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order == 2,
+  rc = tree::walk(s, ws, L, n, beta, softening_scale, true, multipole_order,
                   forces ? static_cast<float*>(forces->data) : nullptr,
                   static_cast<float*>(grad_points->data), static_cast<float*>(grad_masses->data),
                   stats != nullptr);

@@ -969,11 +969,17 @@ __device__ __forceinline__ void scatter_batch(float (&v)[V], int lane, float* ds
 }
 
 // Far-field batch of C nodes (sentinel-padded): forces (if FORCES),
-// query-side gradient and the node adjoint partials (if VJP).
-template <bool VJP, bool QUAD, bool FORCES, int C>
+// query-side gradient and the node adjoint partials (if VJP).  ORDER: 1
+// monopole, 2 monopole + quadrupole, 3 (adjoint only) monopole terms plus
+// the quadrupole term's dL/dS, which does not depend on S: pushed down to
+// the sources it adds the first-order term of their position gradients and
+// the second-order term of their mass gradients.
+template <bool VJP, int ORDER, bool FORCES, int C>
 __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, const int* far,
                                           int lane, float& fx, float& fy, float& fz, float& gx,
                                           float& gy, float& gz) {
+  constexpr bool QUAD = ORDER == 2;
+  constexpr bool SMOM = VJP && ORDER == 3;  // second-moment adjoint with monopole terms
   int idx[C];
 #pragma unroll
   for (int t = 0; t < C; ++t) idx[t] = __builtin_amdgcn_readfirstlane(far[t]);
@@ -1004,10 +1010,31 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         gx += dDx;
         gy += dDy;
         gz += dDz;
-        part[4 * t + 0] = dDx;
-        part[4 * t + 1] = dDy;
-        part[4 * t + 2] = dDz;
-        part[4 * t + 3] = s3 * u;
+        if constexpr (SMOM) {
+          // dL/dS as in the quadrupole branch below, with b = 3 s^5 and
+          // b15 / 2 = -7.5 s^7 u
+          const float s5 = s3 * s2;
+          const float b = 3.0f * s5;
+          const float hb15 = (-7.5f * (s5 * s2)) * u, hu = (0.5f * b) * u;
+          const float Px = fmaf(hb15, Dx, b * L.cx);
+          const float Py = fmaf(hb15, Dy, b * L.cy);
+          const float Pz = fmaf(hb15, Dz, b * L.cz);
+          q8[8 * t + 0] = fmaf(Px, Dx, hu);
+          q8[8 * t + 1] = fmaf(Py, Dy, hu);
+          q8[8 * t + 2] = fmaf(Pz, Dz, hu);
+          q8[8 * t + 3] = fmaf(Px, Dy, Dx * Py);
+          q8[8 * t + 4] = fmaf(Px, Dz, Dx * Pz);
+          q8[8 * t + 5] = fmaf(Py, Dz, Dy * Pz);
+          q8[8 * t + 6] = s3 * u;
+          q8[8 * t + 7] = dDx;
+          q2[2 * t + 0] = dDy;
+          q2[2 * t + 1] = dDz;
+        } else {
+          part[4 * t + 0] = dDx;
+          part[4 * t + 1] = dDy;
+          part[4 * t + 2] = dDz;
+          part[4 * t + 3] = s3 * u;
+        }
       }
     } else {
       const float4 q0 = sload4(A.qd, 2 * idx[t]);
@@ -1067,11 +1094,11 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       }
     }
   }
-  if constexpr (VJP && !QUAD) {
+  if constexpr (VJP && !QUAD && !SMOM) {
     // node adjoints (dL/dX, dL/dM)
     scatter_batch<4 * C, 4, 4, true>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
   }
-  if constexpr (VJP && QUAD) {
+  if constexpr (VJP && (QUAD || SMOM)) {
     static_assert(C == 4, "the packed quadrupole adjoint assumes 4 nodes per batch");
     {
       // q8: value j = lane >> 1 is slot j & 7 of node j >> 3
@@ -1144,7 +1171,7 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
-template <bool VJP, bool QUAD, bool FORCES>
+template <bool VJP, int ORDER, bool FORCES>
 __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt, int lane,
                                            int* stack, int* far, int* near,
                                            unsigned long long& n_far,
@@ -1173,15 +1200,16 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float gx = 0.f, gy = 0.f, gz = 0.f;
 
-  // The quadrupole adjoint holds 12 partials per node: batch 4 nodes, not 8.
-  constexpr int C = (VJP && QUAD) ? kChunk / 2 : kChunk;
+  // The quadrupole and second-moment adjoints hold 12 partials per node:
+  // batch 4 nodes, not 8.
+  constexpr int C = (VJP && ORDER != 1) ? kChunk / 2 : kChunk;
   auto flush_far = [&](int nfar) {
     n_far += static_cast<unsigned long long>(nfar) * nvalid;
     const int padded = (nfar + C - 1) & ~(C - 1);
     if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
     __builtin_amdgcn_wave_barrier();
     for (int f0 = 0; f0 < padded; f0 += C)
-      far_batch<VJP, QUAD, FORCES, C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
+      far_batch<VJP, ORDER, FORCES, C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
     __builtin_amdgcn_wave_barrier();
   };
 
@@ -1249,7 +1277,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 // would take ~143 and drop to 3 waves; the cap costs a few spills outside
 // the interaction batches and is ~4% faster.  WPB waves per block (the
 // block size is a tuning dimension; every wave walks on its own).
-template <bool VJP, bool QUAD, bool FORCES, int WPB>
+template <bool VJP, int ORDER, bool FORCES, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
     WalkArgs A) {
   __shared__ int s_stack[WPB][kStackCap];
@@ -1267,8 +1295,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
     if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.n) continue;  // corrupt entry
-    walk_group<VJP, QUAD, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far,
-                                  n_near);
+    walk_group<VJP, ORDER, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far,
+                                   n_near);
     ++n_groups;
   }
   if (A.stats != nullptr && lane == 0) {
@@ -1573,25 +1601,29 @@ int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, con
 
 namespace {
 template <int WPB>
-void launch_walk(hipStream_t s, unsigned grid, bool vjp, bool quad, bool f, const WalkArgs& A) {
+void launch_walk(hipStream_t s, unsigned grid, bool vjp, int order, bool f, const WalkArgs& A) {
   const unsigned block = 64 * WPB;
-  if (vjp && quad && f)
-    k_walk<true, true, true, WPB><<<grid, block, 0, s>>>(A);
-  else if (vjp && quad)
-    k_walk<true, true, false, WPB><<<grid, block, 0, s>>>(A);
+  if (vjp && order == 2 && f)
+    k_walk<true, 2, true, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp && order == 2)
+    k_walk<true, 2, false, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp && order == 3 && f)
+    k_walk<true, 3, true, WPB><<<grid, block, 0, s>>>(A);
+  else if (vjp && order == 3)
+    k_walk<true, 3, false, WPB><<<grid, block, 0, s>>>(A);
   else if (vjp && f)
-    k_walk<true, false, true, WPB><<<grid, block, 0, s>>>(A);
+    k_walk<true, 1, true, WPB><<<grid, block, 0, s>>>(A);
   else if (vjp)
-    k_walk<true, false, false, WPB><<<grid, block, 0, s>>>(A);
-  else if (quad)
-    k_walk<false, true, true, WPB><<<grid, block, 0, s>>>(A);
+    k_walk<true, 1, false, WPB><<<grid, block, 0, s>>>(A);
+  else if (order == 2)
+    k_walk<false, 2, true, WPB><<<grid, block, 0, s>>>(A);
   else
-    k_walk<false, false, true, WPB><<<grid, block, 0, s>>>(A);
+    k_walk<false, 1, true, WPB><<<grid, block, 0, s>>>(A);
 }
 }  // namespace
 
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
-         bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
+         bool vjp, int order, float* forces, float* grad_points, float* grad_masses,
          bool want_stats, hipEvent_t* ev_walk, int walk_blocks, int walk_threads) {
   if (n == 0) return SBH_OK;
   auto* hdr = reinterpret_cast<Header*>(ws + L.header);
@@ -1633,15 +1665,15 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[0], s));
   const bool f = forces != nullptr;
   switch (wpb) {
-    case 1: launch_walk<1>(s, grid, vjp, quad, f, A); break;
-    case 2: launch_walk<2>(s, grid, vjp, quad, f, A); break;
-    case 8: launch_walk<8>(s, grid, vjp, quad, f, A); break;
-    default: launch_walk<4>(s, grid, vjp, quad, f, A); break;
+    case 1: launch_walk<1>(s, grid, vjp, order, f, A); break;
+    case 2: launch_walk<2>(s, grid, vjp, order, f, A); break;
+    case 8: launch_walk<8>(s, grid, vjp, order, f, A); break;
+    default: launch_walk<4>(s, grid, vjp, order, f, A); break;
   }
   SBH_LAUNCH_CHECK();
   if (ev_walk != nullptr) SBH_HIP_TRY(hipEventRecord(ev_walk[1], s));
   if (vjp) {
-    if (quad)
+    if (order != 1)  // the node adjoints carry dL/dS
       k_finalize<true><<<grid_for(n, kB), kB, 0, s>>>(hdr, n, A.offs, A.nsets, A.topo, A.mp,
                                                        A.sp, A.perm, A.qgrad, A.sgrad, A.adj,
                                                        A.adjq, grad_points, grad_masses);

@@ -109,14 +109,16 @@ int build_forest(hipStream_t s, char* ws, const Layout& L, const float* pts, con
                  int leaf_size, int parts = kBuildAll);
 
 // Walk; forces (input order, N x 3) may be null; with vjp, writes grad_points
-// (N x 3) and grad_masses (N) in input order.
+// (N x 3) and grad_masses (N) in input order.  order: 1 monopole, 2 monopole
+// + quadrupole, 3 (with vjp) monopole terms with the quadrupole term's
+// second-moment adjoint (forces, if requested, are the monopole's).
 // ev_walk (may be null): two events recorded immediately before and after the
 // walk kernel on stream s.
 // Works on forests: each group walks the tree of its own set.  walk_blocks
 // (0: default) caps the persistent grid; walk_threads (0: 256, else 64, 128,
 // 256 or 512) sets its block size.
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
-         bool vjp, bool quad, float* forces, float* grad_points, float* grad_masses,
+         bool vjp, int order, float* forces, float* grad_points, float* grad_masses,
          bool want_stats, hipEvent_t* ev_walk = nullptr, int walk_blocks = 0,
          int walk_threads = 0);
 

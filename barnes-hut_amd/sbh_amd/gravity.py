@@ -24,8 +24,15 @@ LAYOUT_FIELDS = ("header", "codes", "perm", "sp", "sc", "topo", "prefix", "paren
 
 STATS_FIELDS = ("far", "near", "groups", "nodes", "levels", "overflow")
 
+# multipole_order of the VJP functions beyond 1 (monopole) and 2 (monopole +
+# quadrupole): monopole terms plus the quadrupole term's adjoint with respect
+# to each node's second moment.  Pushed down to the sources it gives their
+# position gradients to first and mass gradients to second order in (cell
+# size / distance), as 2 does, at about 3/4 of its adjoint work.
+SECOND_MOMENT_ADJOINT = 3
 
-def _check(beta, softening_scale, max_depth, leaf_size, multipole_order):
+
+def _check(beta, softening_scale, max_depth, leaf_size, multipole_order, vjp=False):
     if beta < 0:
         raise ValueError("beta must be >= 0")
     if softening_scale < 0:
@@ -34,7 +41,9 @@ def _check(beta, softening_scale, max_depth, leaf_size, multipole_order):
         raise ValueError("max_depth must be in [1, 10]")
     if leaf_size < 1:
         raise ValueError("leaf_size must be >= 1")
-    if multipole_order not in (1, 2):
+    if vjp and multipole_order not in (1, 2, SECOND_MOMENT_ADJOINT):
+        raise ValueError("multipole_order must be 1, 2 or 3")
+    if not vjp and multipole_order not in (1, 2):
         raise ValueError("multipole_order must be 1 or 2")
 
 
@@ -88,7 +97,7 @@ def gravity_force_vjp(points, masses, cotangent, *, beta: float = 2.0,
 
     A graph capture passes a preallocated `workspace`, `grad_points_out` and
     `grad_masses_out` (return_forces allocates)."""
-    _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
+    _check(beta, softening_scale, max_depth, leaf_size, multipole_order, vjp=True)
     points, masses, cotangent = _f32(points), _f32(masses), _f32(cotangent)
     if points.dim() != 2 or points.shape[1] != 3:
         raise ValueError("points must have shape (N, 3)")
@@ -132,13 +141,14 @@ def gravity_force_recompute_vjp(points, masses, *, beta: float = 2.0,
     walk (no interaction lists are stored).
 
     multipole_order is the far-field expansion of the forward walk;
-    vjp_multipole_order (default: the same) that of the adjoint walk.  Both
-    approximate the same softened sum, so a monopole forward (the expansion
-    of the reference's octree8 node records) may be paired with a quadrupole
-    adjoint when the gradients need the higher order."""
+    vjp_multipole_order (default: the same) that of the adjoint walk (1, 2
+    or SECOND_MOMENT_ADJOINT).  Both approximate the same softened sum, so a
+    monopole forward (the expansion of the reference's octree8 node records)
+    may be paired with an adjoint of higher order when the gradients need
+    it."""
     vjp_order = multipole_order if vjp_multipole_order is None else vjp_multipole_order
     _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
-    _check(beta, softening_scale, max_depth, leaf_size, vjp_order)
+    _check(beta, softening_scale, max_depth, leaf_size, vjp_order, vjp=True)
     p = points if isinstance(points, torch.Tensor) and points.is_cuda else _f32(points)
     m = masses if isinstance(masses, torch.Tensor) and masses.is_cuda else _f32(masses)
     kw = dict(beta=float(beta), softening_scale=float(softening_scale), max_depth=int(max_depth),
@@ -225,7 +235,7 @@ def gravity_force_vjp_batch(points, masses, cotangent, offsets, *, beta: float =
                             check_offsets: bool = True):
     """(grad_points, grad_masses) of sum_i c_i . F_i over all sets; also fills
     forces_out when given."""
-    _check(beta, softening_scale, max_depth, leaf_size, multipole_order)
+    _check(beta, softening_scale, max_depth, leaf_size, multipole_order, vjp=True)
     points, masses, cotangent = _f32(points), _f32(masses), _f32(cotangent)
     _check_batch(points, masses, offsets, check_offsets)
     if cotangent.shape != points.shape:

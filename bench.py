@@ -39,12 +39,13 @@ FP32_PEAK_NON_PACKED_TFLOPS = 78.6  # the same VALUs without packed-FP32 instruc
 # counts as one source whatever its expansion order.
 FLOPS_ALG_FWD, FLOPS_ALG_VJP = 20, 48
 # The implementation's own count: near pairs as above; a far node costs
-# 20 / 48 with the monopole and 54 / 142 with the quadrupole terms (the FP32
-# operations, FMA = 2, that the walks' SQ_INSTS_VALU_FLOPS_FP32 counter
-# reports per far node: profiles/gravity_walk_plummer_n1048576_quad_valu_counters.md).
+# 20 / 48 with the monopole, 54 / 142 with the quadrupole terms and 76 in the
+# adjoint with monopole terms plus the second-moment adjoint (order 3): the
+# FP32 operations, FMA = 2, that the walks' SQ_INSTS_VALU_FLOPS_FP32 counter
+# reports per far node (profiles/gravity_walk_plummer_n1048576_valu_counters.md).
 FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
 FLOPS_FAR_FWD = {1: 20, 2: 54}
-FLOPS_FAR_VJP = {1: 48, 2: 142}
+FLOPS_FAR_VJP = {1: 48, 2: 142, 3: 76}
 OUTLIER_REL_THRESHOLD = 0.05  # per-row relative error counted as an outlier
 
 
@@ -72,8 +73,8 @@ def valu_roofline(kernel, flops_alg, flops_impl, ms, traffic=None, **extra):
            "flops_per_launch": flops_alg, "ms_per_launch": ms,
            "frac_of_non_packed_peak": a / FP32_PEAK_NON_PACKED_TFLOPS,
            "implementation_count": {
-               "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole) or 54 / 142 "
-                             "(quadrupole)",
+               "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole), 54 / 142 "
+                             "(quadrupole), - / 76 (second-moment adjoint)",
                "achieved": b, "frac": b / FP32_PEAK_TFLOPS,
                "frac_of_non_packed_peak": b / FP32_PEAK_NON_PACKED_TFLOPS,
                "flops_per_launch": flops_impl}}
@@ -185,16 +186,17 @@ def cpu_baseline(p, m, c, args, sample_groups):
     ng = len(T.groups(64)[0])
     gsel = min(sample_groups, ng)
     g0 = (ng - gsel) // 2
-    quad = args.order == 2
+    vjp_mode = {1: 0, 2: 1, 3: 2}[args.order]  # the oracle's expansion modes
     cs = c[T.perm]
     t0 = time.perf_counter()
     T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=args.fwd_order == 2)
     t_fwd = time.perf_counter() - t0
     t0 = time.perf_counter()
-    T.eval_groups(0, 0, beta=args.beta, eps=args.eps, use_quad=quad, cot_sorted=cs)
+    T.eval_groups(0, 0, beta=args.beta, eps=args.eps, use_quad=vjp_mode, cot_sorted=cs)
     t_push = time.perf_counter() - t0
     t0 = time.perf_counter()
-    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=quad, cot_sorted=cs)
+    T.eval_groups(g0, g0 + gsel, beta=args.beta, eps=args.eps, use_quad=vjp_mode,
+                  cot_sorted=cs)
     t_vjp = time.perf_counter() - t0
     scale = ng / gsel
     t_step = 2 * t_build + t_fwd * scale + t_push + max(t_vjp - t_push, 0.0) * scale
@@ -290,10 +292,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--order", type=int, default=2, choices=(1, 2),
+    ap.add_argument("--order", type=int, default=3, choices=(1, 2, 3),
                     help="far-field expansion of the adjoint walk: 1 = monopole, 2 = "
-                         "monopole + quadrupole (default: the monopole adjoint misses "
-                         "rel_l2 <= 1e-3 on the gradients at N = 2^20)")
+                         "monopole + quadrupole, 3 = monopole terms with the quadrupole "
+                         "term's second-moment adjoint (default; the monopole adjoint "
+                         "misses rel_l2 <= 1e-3 on the gradients at N = 2^20)")
     ap.add_argument("--fwd-order", type=int, default=1, choices=(1, 2),
                     help="far-field expansion of the forward walk (default 1, the monopole "
                          "of the reference's octree8 node records: forces at rel_l2 ~1e-4 "
@@ -473,35 +476,41 @@ def bench_gravity(args):
            "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(gm[rows], rgm),
            "grad_points_outlier_frac_vs_fp64_direct_adjoint": outlier_frac(gx[rows], rgx),
            "grad_sample_rows": int(rows.numel())}
-    # the same step with the other forward expansion (graph replay), beside
-    alt = None
+    # the same step with other expansion orders (graph replay), beside: all
+    # quadrupole, and the headline's forward with the full quadrupole adjoint
+    alts = []
     if not args.no_graph:
         from sbh_amd.graph import Graph
 
-        kw_alt = dict(kw, multipole_order=3 - args.fwd_order)
-        F_alt = torch.empty_like(P)
+        F_a, GX_a, GM_a = torch.empty_like(P), torch.empty_like(P), torch.empty_like(M)
+        for fo, vo in ((2, 2), (args.fwd_order, 2)):
+            if (fo, vo) == (args.fwd_order, args.order) or (fo, vo) in [
+                    (x["forward_order"], x["vjp_order"]) for x in alts]:
+                continue
+            kw_fa, kw_va = dict(kw, multipole_order=fo), dict(kw, multipole_order=vo)
 
-        def step_alt():
-            G.gravity_force(P, M, workspace=ws, out=F_alt, **kw_alt)
-            G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX,
-                                grad_masses_out=GM, **kw)
+            def step_alt():
+                G.gravity_force(P, M, workspace=ws, out=F_a, **kw_fa)
+                G.gravity_force_vjp(P, M, C, workspace=ws, grad_points_out=GX_a,
+                                    grad_masses_out=GM_a, **kw_va)
 
-        g = Graph()
-        g.capture(step_alt)
-        g.replay()
-        torch.cuda.synchronize()
-        ev0.record(stream)
-        for _ in range(args.steps):
+            g = Graph()
+            g.capture(step_alt)
             g.replay()
-        ev1.record(stream)
-        torch.cuda.synchronize()
-        alt_ms = ev0.elapsed_time(ev1) / args.steps
-        g.close()
-        alt = {"forward_order": 3 - args.fwd_order, "vjp_order": args.order,
-               "graph_ms_per_step": alt_ms, "forces_per_s": n / (alt_ms * 1e-3),
-               "force_rel_l2_vs_fp64_direct": rel_l2(F_alt[idx], ref),
-               "force_outlier_frac_vs_fp64_direct": outlier_frac(F_alt[idx], ref)}
-        del F_alt
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                g.replay()
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            alt_ms = ev0.elapsed_time(ev1) / args.steps
+            g.close()
+            alts.append({"forward_order": fo, "vjp_order": vo, "graph_ms_per_step": alt_ms,
+                         "forces_per_s": n / (alt_ms * 1e-3),
+                         "force_rel_l2_vs_fp64_direct": rel_l2(F_a[idx], ref),
+                         "grad_points_rel_l2_vs_fp64_direct_adjoint": rel_l2(GX_a[rows], rgx),
+                         "grad_masses_rel_l2_vs_fp64_direct_adjoint": rel_l2(GM_a[rows], rgm)})
+        del F_a, GX_a, GM_a
     direct = None
     if not args.no_direct:
         from sbh_amd import ffi
@@ -568,10 +577,10 @@ def bench_gravity(args):
                       "vjp_build": prof[4] / max(prof[3], 1), "vjp_walk_kernel": walk_vjp_ms},
         "interactions": {"fwd": sf, "vjp": sv},
         "accuracy": acc,
-        "other_forward_order": alt,
+        "other_orders": alts,
         "direct_sum": direct,
         "roofline": valu_roofline(
-            f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0>", flops_vjp, flops_vjp_impl,
+            f"k_walk<vjp=1,order={args.order},forces=0>", flops_vjp, flops_vjp_impl,
             walk_vjp_ms,
             traffic=(profiled_traffic("k_walk<true,true,false>")
                      if args.order == 2 and n == 1 << 20 else None),
@@ -627,7 +636,7 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl, prof, d
                     "note": "24 dependent launches per build (10 of them the internal "
                             "moment levels); launch latency and the radix passes' "
                             "look-back set the time, not bandwidth"}
-    out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,quad={int(args.fwd_order == 2)},forces=1>",
+    out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,order={args.fwd_order},forces=1>",
                                     flops_fwd, flops_fwd_impl, walk_fwd_ms,
                                     traffic=(profiled_traffic("k_walk<false,true,true>")
                                              if args.fwd_order == 2 and n == 1 << 20
@@ -1515,12 +1524,21 @@ def measure_batched(args, dev, tune=True, cpu=False):
     C = torch.tensor(np.concatenate(cs), device=dev)
     offs = G.set_offsets(sizes, dev)
     n_total = int(P.shape[0])
-    kw = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth,
-              multipole_order=args.order)
+    # the forward walk keeps the quadrupole here: on these dense blobs at
+    # eps = 0.005 the monopole's largest-set force error (6e-4) leaves little
+    # margin; the autotuner picks the adjoint expansion with the leaf size
+    fwd_order = 2
+    base = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth)
     if tune:
-        tuned = autotune.tune_batch(P, M, C, offs, reps=2, **kw)
+        tuned = autotune.tune_batch(P, M, C, offs, reps=2, fwd_multipole_order=fwd_order,
+                                    on_fail="relax", **base)
     else:
-        tuned = {"leaf_size": args.leaf, "walk_blocks": 0, "walk_threads": 0, "table": None}
+        tuned = {"vjp_order": args.order, "leaf_size": args.leaf, "walk_blocks": 0,
+                 "walk_threads": 0, "table": None}
+    vjp_order = tuned["vjp_order"]
+    kw = dict(base, multipole_order=vjp_order)
+    kw_f = dict(base, multipole_order=fwd_order)
+    kw_t = dict(kw, fwd_multipole_order=fwd_order)  # the autotuner's step
     leaf, blocks, threads = tuned["leaf_size"], tuned["walk_blocks"], tuned["walk_threads"]
     stream = torch.cuda.current_stream()
     steps = max(1, args.steps)
@@ -1538,7 +1556,7 @@ def measure_batched(args, dev, tune=True, cpu=False):
         return e0.elapsed_time(e1) / nsteps
 
     # batched: one forest pipeline, captured as a HIP graph
-    step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw, threads)
+    step, (F, gx, gm), ws = autotune.build_step(P, M, C, offs, leaf, blocks, kw_t, threads)
     step()
     g = Graph()
     g.capture(step)
@@ -1566,7 +1584,7 @@ def measure_batched(args, dev, tune=True, cpu=False):
 
     def per_set():
         for k in range(len(Ps)):
-            G.gravity_force(Ps[k], Ms[k], leaf_size=leaf, workspace=wss, out=Fs[k], **kw)
+            G.gravity_force(Ps[k], Ms[k], leaf_size=leaf, workspace=wss, out=Fs[k], **kw_f)
             G.gravity_force_vjp(Ps[k], Ms[k], Cs[k], leaf_size=leaf, workspace=wss,
                                 grad_points_out=GXs[k], grad_masses_out=GMs[k], **kw)
 
@@ -1599,7 +1617,7 @@ def measure_batched(args, dev, tune=True, cpu=False):
     stf = G.new_stats(dev)
     G.gravity_force_batch(P, M, offs, stats=stf, leaf_size=leaf, walk_blocks=blocks,
                           walk_threads=threads,
-                          workspace=ws, out=F, **kw)
+                          workspace=ws, out=F, **kw_f)
     sf = G.stats_dict(stf)
     torch.cuda.synchronize()
     acc = {"largest_set": nb,
@@ -1613,11 +1631,12 @@ def measure_batched(args, dev, tune=True, cpu=False):
     gp.close()
     walk_vjp_ms = prof[5] / max(prof[3], 1)
     walk_fwd_ms = prof[2] / max(prof[0], 1)
-    flops_vjp, flops_vjp_impl = walk_flops(sv, True, args.order)
+    flops_vjp, flops_vjp_impl = walk_flops(sv, True, vjp_order)
     rec = {"workload": f"blob_sets{args.sets}_total{n_total}_theta0.5_fwd+vjp",
            "sets": args.sets, "total_points": n_total, "min_set": int(sizes.min()),
            "max_set": int(sizes.max()), "softening": args.eps, "beta_diag": args.beta,
-           "max_depth": args.depth, "multipole_order": args.order,
+           "max_depth": args.depth,
+           "multipole_order": {"forward": fwd_order, "vjp": vjp_order},
            "leaf_size": leaf, "walk_blocks": blocks, "walk_threads": threads,
            "batched_graph_ms": ms_graph, "batched_eager_ms": ms_eager,
            "per_set_eager_ms": ms_per_set, "per_set_graph_ms": ms_per_set_graph,
@@ -1634,11 +1653,12 @@ def measure_batched(args, dev, tune=True, cpu=False):
                          "vjp_build": prof[4] / max(prof[3], 1),
                          "vjp_walk_kernel": walk_vjp_ms},
            "roofline": valu_roofline(
-               f"k_walk<vjp=1,quad={int(args.order == 2)},forces=0> (all sets)", flops_vjp,
+               f"k_walk<vjp=1,order={vjp_order},forces=0> (all sets)", flops_vjp,
                flops_vjp_impl, walk_vjp_ms)}
     if cpu:
         big_args = argparse.Namespace(**vars(args))
         big_args.leaf = leaf
+        big_args.order, big_args.fwd_order = vjp_order, fwd_order
         c = cpu_baseline(ps[kbig], ms[kbig], cs[kbig], big_args, args.cpu_sample_groups)
         c["sample"] = (f"largest set ({nb} points) only; its forces/s stands for the batch. "
                        + c["sample"])
@@ -1696,12 +1716,13 @@ def measure_config1(args, dev, n=10000, eps=0.01, leaf=16, steps=20):
     P, M, C = (torch.tensor(a, device=dev) for a in (p, m, c))
     kw = dict(beta=args.beta, softening_scale=eps, max_depth=10, leaf_size=leaf,
               multipole_order=args.order)
+    kw_f = dict(kw, multipole_order=args.fwd_order)
     ws = G.new_workspace(n, 10, leaf, dev)
     F, GX, GM = torch.empty_like(P), torch.empty_like(P), torch.empty_like(M)
     stream = torch.cuda.current_stream()
 
     def fwd():
-        G.gravity_force(P, M, workspace=ws, out=F, **kw)
+        G.gravity_force(P, M, workspace=ws, out=F, **kw_f)
 
     def fwd_vjp():
         fwd()
@@ -1741,7 +1762,7 @@ def measure_config1(args, dev, n=10000, eps=0.01, leaf=16, steps=20):
     out.update({
         "workload": f"uniform_cube_n{n}_eps{eps}_theta0.5_leaf{leaf}",
         "n": n, "softening": eps, "leaf_size": leaf, "beta_diag": args.beta,
-        "multipole_order": args.order,
+        "multipole_order": {"forward": args.fwd_order, "vjp": args.order},
         "fwd_forces_per_s": n / (out["fwd_ms"] * 1e-3),
         "fwd_vjp_forces_per_s": n / (out["fwd_vjp_ms"] * 1e-3),
         "launch": "HIP graph replay",

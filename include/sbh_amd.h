@@ -489,7 +489,15 @@ int sbh_winding_number(void* stream, const SbhBuffer* points,
  *
  * Self-gravity only (queries are the points; the i == j pair is skipped).
  * Kernel: F_i = sum -m (r^2 + eps^2 + 1e-12)^(-3/2) d, no cutoff, not periodic.
- * multipole_order: 1 = monopole, 2 = monopole + quadrupole.
+ * multipole_order: 1 = monopole, 2 = monopole + quadrupole.  The VJP entry
+ *            points also take 3: monopole terms (forces, the query-side
+ *            gradient and the node adjoints dL/dX, dL/dM) plus the
+ *            quadrupole term's adjoint with respect to the node's second
+ *            moment, which does not depend on that moment.  Pushed down to
+ *            the sources it adds the first-order term of their position
+ *            gradients and the second-order term of their mass gradients
+ *            (the same orders as 2, at about 3/4 of its adjoint work);
+ *            forces returned with 3 are the monopole's.
  * workspace: u8 device buffer of sbh_gravity_workspace_bytes() bytes, or NULL
  *            for a library-owned cached buffer (allocated when it must grow).
  * stats:     NULL or int64[8] device buffer receiving {far (lane x node)

@@ -648,6 +648,8 @@ class Tree:
              xs=None, ms=None, boxes=None):
         """Self-gravity walk over the sorted sources (queries == sources).
 
+        use_quad: False / 0 monopole, True / 1 monopole + quadrupole, 2 monopole
+        terms with the quadrupole's second-moment adjoint (orc_far).
         group_size is the largest query group (see orc_tree_groups).  xs/ms
         override the source positions/masses (fp64, sorted) while the
         topology and the group boxes stay frozen -- used for finite differences.
@@ -669,7 +671,7 @@ class Tree:
         lib().orc_tree_eval(self.num_nodes, self.start, self.count, self.first_child,
                             self.nchild, _c(self.geo, np.float32).ravel(), mass,
                             com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n, len(gs),
-                            gs, gc, float(beta), int(bool(use_quad)), float(eps), 1,
+                            gs, gc, float(beta), int(use_quad), float(eps), 1,
                             _ptr(cot), force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts)
         return dict(force=force, gq=gq, gsx=gsx, gsm=gsm, far=int(counts[0]),
                     near=int(counts[1]), groups=len(gs))
@@ -693,7 +695,7 @@ class Tree:
         lib().orc_tree_eval_range(self.num_nodes, self.start, self.count, self.first_child,
                                   self.nchild, _c(self.geo, np.float32).ravel(), mass,
                                   com.ravel(), quad.ravel(), xs, ms, n, boxes, xs, n, gs, gc,
-                                  float(beta), int(bool(use_quad)), float(eps), 1, _ptr(cot),
+                                  float(beta), int(use_quad), float(eps), 1, _ptr(cot),
                                   force, _ptr(gq), _ptr(gsx), _ptr(gsm), counts,
                                   int(g_begin), int(g_end))
         return dict(force=force, far=int(counts[0]), near=int(counts[1]), groups=len(gs))

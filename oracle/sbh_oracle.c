@@ -1270,7 +1270,13 @@ static inline int orc_group_accept(const float* box, const float* geo, float bet
 
 /* Far-field (monopole [+ quadrupole]) force and its adjoint for one query.
  * F = -M s^3 D + 3 s^5 (S D) + 1.5 tr(S) s^5 D - 7.5 s^7 (D.S.D) D,
- * s = (|D|^2 + eps^2 + 1e-12)^(-1/2), D = q - X. */
+ * s = (|D|^2 + eps^2 + 1e-12)^(-1/2), D = q - X.
+ * use_quad: 0 monopole; 1 monopole + quadrupole; 2 monopole terms (force,
+ * query-side gradient, dL/dX, dL/dM) plus the quadrupole term's dL/dS, which
+ * does not depend on S: pushed down to the sources it adds the first-order
+ * term of their position gradients and the second-order term of their mass
+ * gradients (the Taylor expansion of the exact source-side gradient about
+ * the centre of mass). */
 static void orc_far(const double* q, double M, const double* X, const double* Q,
                     int use_quad, double e2, const double* c, double* F, double* gq,
                     double* adjX, double* adjM, double* adjS) {
@@ -1279,7 +1285,7 @@ static void orc_far(const double* q, double M, const double* X, const double* Q,
   double s = 1.0 / sqrt(r2 + e2);
   double s2 = s * s, s3 = s2 * s, s5 = s3 * s2, s7 = s5 * s2, s9 = s7 * s2;
   double SD[3] = {0, 0, 0}, t = 0.0, T = 0.0;
-  if (use_quad) {
+  if (use_quad == 1) {
     SD[0] = Q[0] * D[0] + Q[3] * D[1] + Q[4] * D[2];
     SD[1] = Q[3] * D[0] + Q[1] * D[1] + Q[5] * D[2];
     SD[2] = Q[4] * D[0] + Q[5] * D[1] + Q[2] * D[2];
@@ -1291,7 +1297,7 @@ static void orc_far(const double* q, double M, const double* X, const double* Q,
   if (!c) return;
   double u = c[0] * D[0] + c[1] * D[1] + c[2] * D[2];
   double Sc[3] = {0, 0, 0}, cw = 0.0;
-  if (use_quad) {
+  if (use_quad == 1) {
     Sc[0] = Q[0] * c[0] + Q[3] * c[1] + Q[4] * c[2];
     Sc[1] = Q[3] * c[0] + Q[1] * c[1] + Q[5] * c[2];
     Sc[2] = Q[4] * c[0] + Q[5] * c[1] + Q[2] * c[2];

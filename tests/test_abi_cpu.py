@@ -276,3 +276,23 @@ def test_stochastic_adaptive_entries_reject():
                                               _i(32)) == -1
     assert lib.sbh_stochastic_workspace_bytes(ctypes.c_int64(1000), ctypes.c_int64(10), _i(10),
                                               _i(32)) > 0
+
+
+def test_gravity_expansion_orders_reject():
+    # forces take orders 1 and 2; the VJP entry points also take 3 (the
+    # second-moment adjoint); checks run before any HIP call
+    p, m, f = _buf(F32, 5, 3, ptr=FAKE), _buf(F32, 5, ptr=FAKE), _buf(F32, 5, 3, ptr=FAKE)
+    c, gx, gm = _buf(F32, 5, 3, ptr=FAKE), _buf(F32, 5, 3, ptr=FAKE), _buf(F32, 5, ptr=FAKE)
+    rc, err = _call("sbh_gravity_force", None, *map(ctypes.byref, (p, m, f)), _f(2.0),
+                    _f(0.01), _i(10), _i(32), _i(3), None, None)
+    assert rc == 3 and "multipole_order must be 1 or 2" in err
+    for name, extra in (("sbh_gravity_force_vjp", (None,)), ("sbh_gravity_force_vjp_noalloc", ())):
+        rc, err = _call(name, None, *map(ctypes.byref, (p, m, c)), None,
+                        *map(ctypes.byref, (gx, gm)), _f(2.0), _f(0.01), _i(10), _i(32), _i(4),
+                        *extra, None)
+        assert rc == 3 and "multipole_order must be 1, 2 or 3" in err, (name, err)
+    bad_gm = _buf(F32, 4, ptr=FAKE)
+    rc, err = _call("sbh_gravity_force_vjp", None, *map(ctypes.byref, (p, m, c)), None,
+                    *map(ctypes.byref, (gx, bad_gm)), _f(2.0), _f(0.01), _i(10), _i(32), _i(3),
+                    None, None)
+    assert rc == 3 and "grad_masses shape mismatch" in err

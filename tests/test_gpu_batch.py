@@ -162,3 +162,46 @@ def test_walk_block_sizes_give_the_same_results(gpu, threads):
     assert float(torch.linalg.norm(gma - gmb) / torch.linalg.norm(gma)) < 1e-5
     with pytest.raises(ValueError, match="walk_threads"):
         G.gravity_force_batch(P, M, offs, walk_threads=96, **KW)
+
+
+def test_autotuner_gate_and_search(gpu):
+    # a small batch: every candidate of the coordinate search is timed once,
+    # the winner passes the FP64 gate and is the fastest that does, the
+    # largest sets are checked alone, and the result is cached
+    from sbh_amd import autotune
+
+    G = _G()
+    sizes = [20000, 1500, 0, 6000, 300, 12000, 2]
+    ps, ms, cs = _sets(sizes, 13)
+    P, M, C = _dev(ps, ms, cs)
+    offs = G.set_offsets(sizes)
+    kw = dict(beta=2.0, softening_scale=0.005, max_depth=10, fwd_multipole_order=2)
+    res = autotune.tune_batch(P, M, C, offs, leaf_sizes=(16, 32), walk_blocks=(0, 512),
+                              walk_threads=(256, 128), reps=1, **kw)
+    table = res["table"]
+    assert len(table) == 4 + 1 + 1  # orders x leaves, then one more grid, one more block size
+    assert {r["vjp_order"] for r in table} == {2, G.SECOND_MOMENT_ADJOINT}
+    passing = [r for r in table if r["passes_gate"]]
+    assert passing and res["ms"] == min(r["ms"] for r in passing)
+    best = [r for r in passing if r["ms"] == res["ms"]][0]
+    assert (res["vjp_order"], res["leaf_size"], res["walk_blocks"], res["walk_threads"]) == (
+        best["vjp_order"], best["leaf_size"], best["walk_blocks"], best["walk_threads"])
+    assert res["fwd_order"] == 2 and res["gate"]["rows"] > 0
+    assert res["gate"]["sets_checked_alone"] >= 1
+    for r in table:
+        assert r["passes_gate"] == (max(r["rel_l2"].values()) <= 1e-3)
+    assert autotune.tune_batch(P, M, C, offs, leaf_sizes=(16, 32), walk_blocks=(0, 512),
+                               walk_threads=(256, 128), reps=1, **kw) is res
+    with pytest.raises(RuntimeError, match="no candidate"):
+        autotune.tune_batch(P, M, C, offs, leaf_sizes=(16,), walk_blocks=(0,),
+                            walk_threads=(256,), reps=1, gate=1e-12, **kw)
+    # relaxed: the gate becomes 1.25 x the best accuracy any candidate reached
+    with pytest.warns(UserWarning, match="relaxed"):
+        rel = autotune.tune_batch(P, M, C, offs, leaf_sizes=(16, 32), walk_blocks=(0,),
+                                  walk_threads=(256,), reps=1, gate=1e-12, on_fail="relax",
+                                  **kw)
+    errs = [max(r["rel_l2"].values()) for r in rel["table"]]
+    assert rel["gate"]["relaxed"] and rel["gate"]["requested_rel_l2_max"] == 1e-12
+    assert rel["gate"]["rel_l2_max"] == pytest.approx(autotune.RELAX_FACTOR * min(errs))
+    chosen = [r for r in rel["table"] if r["ms"] == rel["ms"]][0]
+    assert chosen["passes_gate"] and max(chosen["rel_l2"].values()) <= rel["gate"]["rel_l2_max"]

@@ -95,8 +95,10 @@ def test_tree_matches_oracle(gpu, kind, n, depth, leaf):
 
 
 @pytest.mark.parametrize("kind,n", [("plummer", 20000), ("blobs", 16384), ("uniform", 3001)])
-@pytest.mark.parametrize("order", [1, 2])
+@pytest.mark.parametrize("order", [1, 2, 3])
 def test_forces_and_adjoint_match_oracle(gpu, kind, n, order):
+    # order 3 (second-moment adjoint) is the oracle's mode 2; its forces are
+    # the monopole's
     G = _G()
     p, m, c = _case(kind, n, 11)
     eps = 0.005
@@ -107,7 +109,7 @@ def test_forces_and_adjoint_match_oracle(gpu, kind, n, order):
     gx, gm, f = _np(gx), _np(gm), _np(f)
     st = G.stats_dict(stats)
     T = O.Tree(p, m, max_depth=10, leaf_size=32)
-    r = T.eval(beta=2.0, eps=eps, use_quad=order == 2, cot_sorted=c[T.perm])
+    r = T.eval(beta=2.0, eps=eps, use_quad={1: 0, 2: 1, 3: 2}[order], cot_sorted=c[T.perm])
     # identical interaction sets
     assert st["far"] == r["far"] and st["near"] == r["near"], (st, r["far"], r["near"])
     assert st["groups"] == r["groups"] and st["overflow"] == 0
@@ -116,7 +118,8 @@ def test_forces_and_adjoint_match_oracle(gpu, kind, n, order):
     assert O.rel_l2(gx[perm], r["gq"] + r["gsx"]) < 1e-4
     assert O.rel_l2(gm[perm], r["gsm"]) < 1e-4
     # forward-only entry point gives the same forces
-    f2 = _np(G.gravity_force(p, m, beta=2.0, softening_scale=eps, multipole_order=order))
+    f2 = _np(G.gravity_force(p, m, beta=2.0, softening_scale=eps,
+                             multipole_order=2 if order == 2 else 1))
     np.testing.assert_allclose(f2, f, rtol=1e-6, atol=1e-6 * np.abs(f).max())
 
 
@@ -141,12 +144,12 @@ def test_gradients_vs_fp64_direct_adjoint(gpu):
     p, m, c = _case("blobs", n, 5)
     eps = 0.002
     rgx, rgm = O.direct_full_vjp(p, m, c, eps)
-    for order in (1, 2):
+    for order in (1, 2, 3):
         gx, gm = G.gravity_force_vjp(p, m, c, beta=2.0, softening_scale=eps,
                                      multipole_order=order)
         ex, em = O.rel_l2(_np(gx), rgx), O.rel_l2(_np(gm), rgm)
         print(f"order={order} grad_x rel_l2={ex:.3e} grad_m rel_l2={em:.3e}")
-        assert ex < 1e-3 and em < (1e-3 if order == 2 else 3e-3)
+        assert ex < 1e-3 and em < (3e-3 if order == 1 else 1e-3)
 
 
 def test_huge_beta_is_direct_sum(gpu):
@@ -222,7 +225,9 @@ def test_recompute_custom_vjp_with_its_own_adjoint_order(gpu):
     g1x, _ = G.gravity_force_vjp(p, m, c, softening_scale=0.005, multipole_order=1)
     assert O.rel_l2(_np(x.grad), _np(g1x)) > 1e-6  # the adjoint order did change
     with pytest.raises(ValueError):
-        G.gravity_force_recompute_vjp(x, mt, multipole_order=1, vjp_multipole_order=3)
+        G.gravity_force_recompute_vjp(x, mt, multipole_order=1, vjp_multipole_order=4)
+    with pytest.raises(ValueError):
+        G.gravity_force_recompute_vjp(x, mt, multipole_order=3)  # not a forward order
 
 
 def test_determinism_of_forces(gpu):

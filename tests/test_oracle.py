@@ -557,6 +557,59 @@ def test_quadrupole_error_order():
     assert -3.3 < slope_quad < -2.7
 
 
+def test_source_gradient_error_orders():
+    # The same distant cluster with a cotangent at the query: the sources'
+    # gradients reach them through the root node's adjoint.  Against their
+    # closed form m (s^3 c - 3 s^5 (c.d) d) and -s^3 (c.d), the monopole
+    # push-down errs at first order in a/d for positions and second for
+    # masses; adding the second-moment adjoint (mode 2, also part of the full
+    # quadrupole, mode 1) lifts both by one order.  Mode 2's force and
+    # query-side gradient are the monopole's.
+    import ctypes
+
+    rng = _rng(9)
+    n = 400
+    p = (rng.normal(size=(n, 3)) * 0.1).astype(np.float32)
+    m = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    T = O.Tree(p, m, max_depth=10, leaf_size=8)
+    mass, com, quad = T.moments()
+    xs, ms = T.points_sorted.astype(np.float64), T.masses_sorted.astype(np.float64)
+    c = np.array([[0.3, -1.0, 0.7]])
+    ds = [4.0, 8.0, 16.0]
+    slopes, fq = {}, {}
+    for mode in (0, 1, 2):
+        ex, em = [], []
+        for d in ds:
+            q = np.array([[d, 0.3 * d, -0.2 * d]])
+            F, gq = np.zeros((1, 3)), np.zeros((1, 3))
+            gsx, gsm = np.zeros((n, 3)), np.zeros(n)
+            boxes = np.array([q[0, 0], q[0, 1], q[0, 2]] * 2, np.float32)
+            cnt = (ctypes.c_int64 * 2)()
+            vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+            O.lib().orc_tree_eval(T.num_nodes, T.start, T.count, T.first_child, T.nchild,
+                                  np.ascontiguousarray(T.geo.ravel()), mass, com.ravel().copy(),
+                                  quad.ravel().copy(), xs, ms, T.n, boxes, q, 1, 1,
+                                  np.zeros(1, np.int32), np.ones(1, np.int32), 0.0, mode, 0.0,
+                                  0, vp(c), F, vp(gq), vp(gsx), vp(gsm), cnt)
+            assert cnt[0] == 1
+            dd = q - xs
+            s = (dd * dd).sum(1) ** -0.5
+            cd = dd @ c[0]
+            rx = ms[:, None] * ((s ** 3)[:, None] * c - 3 * (s ** 5 * cd)[:, None] * dd)
+            rm = -(s ** 3) * cd
+            ex.append(np.linalg.norm(gsx - rx) / np.linalg.norm(rx))
+            em.append(np.linalg.norm(gsm - rm) / np.linalg.norm(rm))
+            fq[(mode, d)] = (F.copy(), gq.copy())
+        slopes[mode] = (np.polyfit(np.log(ds), np.log(ex), 1)[0],
+                        np.polyfit(np.log(ds), np.log(em), 1)[0])
+    assert -1.2 < slopes[0][0] < -0.8 and -2.2 < slopes[0][1] < -1.8
+    for mode in (1, 2):
+        assert -2.2 < slopes[mode][0] < -1.8 and -3.2 < slopes[mode][1] < -2.8, slopes
+    for d in ds:
+        for a, b in zip(fq[(2, d)], fq[(0, d)]):
+            np.testing.assert_array_equal(a, b)
+
+
 @pytest.mark.parametrize("use_quad", [False, True])
 def test_tree_adjoint_matches_central_differences(use_quad):
     n = 700
