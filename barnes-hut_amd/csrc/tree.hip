@@ -1232,10 +1232,11 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       // self pairs are skipped
       n_near += static_cast<unsigned long long>(s1 - s0) * nvalid -
                 static_cast<unsigned long long>(max(0, min(s1, base + nvalid) - max(s0, base)));
-      // batches of 8, then in the forward walk the rest exactly (4, 2, 1); the
-      // adjoint pads its last batch of 8 (tail variants there push the kernel
-      // past its register budget and cost more than the padding saves)
-      if constexpr (VJP) {
+      // batches of 8, then the rest exactly (4, 2, 1); the quadrupole adjoint
+      // and the adjoint that also returns forces pad their last batch of 8
+      // instead (tail variants push them past their register budget and cost
+      // more than the padding saves)
+      if constexpr (VJP && (ORDER == 2 || FORCES)) {
         for (int j0 = s0; j0 < s1; j0 += kChunk)
           near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
       } else {
