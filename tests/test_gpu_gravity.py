@@ -173,12 +173,14 @@ def test_small_and_ragged(gpu, n):
     assert np.isfinite(f).all()
     if n > 1:
         assert O.rel_l2(f, ref) < 1e-3
-    gx, gm = G.gravity_force_vjp(p, m, c, softening_scale=0.05, leaf_size=4)
     rgx, rgm = O.direct_full_vjp(p, m, c, 0.05)
-    if n > 1:
-        assert O.rel_l2(_np(gx), rgx) < 2e-3 and O.rel_l2(_np(gm), rgm) < 5e-3
-    else:
-        assert np.all(_np(gx) == 0) and np.all(_np(gm) == 0)
+    for order in (2, 3):
+        gx, gm = G.gravity_force_vjp(p, m, c, softening_scale=0.05, leaf_size=4,
+                                     multipole_order=order)
+        if n > 1:
+            assert O.rel_l2(_np(gx), rgx) < 2e-3 and O.rel_l2(_np(gm), rgm) < 5e-3
+        else:
+            assert np.all(_np(gx) == 0) and np.all(_np(gm) == 0)
 
 
 def test_identical_points_deep_leaf(gpu):

@@ -18,7 +18,7 @@ from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-KW = dict(beta=2.0, softening_scale=0.01, max_depth=10, leaf_size=16, multipole_order=2)
+KW = dict(beta=2.0, softening_scale=0.01, max_depth=10, leaf_size=16)
 
 
 def _inputs(n, seed):
@@ -29,19 +29,19 @@ def _inputs(n, seed):
     return p, m, c
 
 
-def _noalloc(p, m, c):
+def _noalloc(p, m, c, order=2):
     from sbh_amd._lib import buf, call, f32, i32, stream_handle
 
     F, gx, gm = torch.empty_like(p), torch.empty_like(p), torch.empty_like(m)
     st = torch.zeros(8, dtype=torch.int64, device="cuda")
     call("sbh_gravity_force_vjp_noalloc", stream_handle(), buf(p), buf(m), buf(c), buf(F),
          buf(gx), buf(gm), f32(KW["beta"]), f32(KW["softening_scale"]), i32(KW["max_depth"]),
-         i32(KW["leaf_size"]), i32(KW["multipole_order"]), buf(st))
+         i32(KW["leaf_size"]), i32(order), buf(st))
     torch.cuda.synchronize()
     return F, gx, gm, st
 
 
-def _with_workspace(p, m, c):
+def _with_workspace(p, m, c, order=2):
     from sbh_amd._lib import buf, call, f32, i32, stream_handle
     from sbh_amd import gravity as G
 
@@ -51,7 +51,7 @@ def _with_workspace(p, m, c):
     st = torch.zeros(8, dtype=torch.int64, device="cuda")
     call("sbh_gravity_force_vjp", stream_handle(), buf(p), buf(m), buf(c), buf(F), buf(gx),
          buf(gm), f32(KW["beta"]), f32(KW["softening_scale"]), i32(KW["max_depth"]),
-         i32(KW["leaf_size"]), i32(KW["multipole_order"]), buf(ws), buf(st))
+         i32(KW["leaf_size"]), i32(order), buf(ws), buf(st))
     torch.cuda.synchronize()
     return F, gx, gm, st
 
@@ -63,13 +63,15 @@ def test_noalloc_vjp_matches_workspace_entry_twice(gpu):
     lib.sbh_gravity_noalloc_scratch_bytes.restype = ctypes.c_int64
     lib.sbh_gravity_workspace_bytes.restype = ctypes.c_int64
     cap = lib.sbh_gravity_noalloc_scratch_bytes()
-    for n, seed in ((40000, 11), (7001, 12)):
+    # the second-moment adjoint (order 3) then the quadrupole one, each on a
+    # larger then a smaller problem
+    for order, n, seed in ((3, 40000, 11), (3, 7001, 12), (2, 40000, 11), (2, 7001, 12)):
         need = lib.sbh_gravity_workspace_bytes(ctypes.c_int64(n), KW["max_depth"], KW["leaf_size"])
         assert 0 < need <= cap, (n, need, cap)
-        print(f"case: noalloc vs workspace entry, n={n}, seed={seed}", flush=True)
+        print(f"case: noalloc vs workspace entry, order={order}, n={n}, seed={seed}", flush=True)
         p, m, c = _inputs(n, seed)
-        Fa, gxa, gma, sta = _noalloc(p, m, c)
-        Fb, gxb, gmb, stb = _with_workspace(p, m, c)
+        Fa, gxa, gma, sta = _noalloc(p, m, c, order)
+        Fb, gxb, gmb, stb = _with_workspace(p, m, c, order)
         assert torch.equal(Fa, Fb)
         assert torch.equal(sta, stb)
         assert int(sta[5]) == 0
