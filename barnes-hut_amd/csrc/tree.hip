@@ -109,11 +109,8 @@ __global__ __launch_bounds__(kB) void k_setbox_init(uint32_t* __restrict__ box, 
     box[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
 }
 
-// Per-set data bounds of the input rows (sets are contiguous row ranges):
-// a block whose rows all belong to one set reduces in LDS and issues six
-// atomics; otherwise every wave with a single set reduces across lanes, and
-// mixed waves fall back to per-lane atomics.
-// Per-set boxes, pass 1: one block per tile of kB consecutive points.  A tile
+// Per-set data bounds of the input rows (sets are contiguous row ranges).
+// Pass 1: one block per tile of kB consecutive points.  A tile
 // inside one set stores its box in part[6 t ..]; a tile that straddles set
 // boundaries (at most nsets - 1 of them) stores the neutral box and adds its
 // points to the set boxes with ordered-integer atomics.
