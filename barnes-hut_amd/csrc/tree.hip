@@ -1200,12 +1200,21 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   // The quadrupole and second-moment adjoints hold 12 partials per node:
   // batch 4 nodes, not 8.
   constexpr int C = (VJP && ORDER != 1) ? kChunk / 2 : kChunk;
+  // The monopole forward walk reads 16 nodes / sources per batch while the
+  // loads of a batch are in flight together (its registers allow it); the
+  // rest of a list or range is done in batches of 8 as the others.
+  constexpr bool kWide = !VJP && ORDER == 1;
   auto flush_far = [&](int nfar) {
     n_far += static_cast<unsigned long long>(nfar) * nvalid;
     const int padded = (nfar + C - 1) & ~(C - 1);
     if (lane < padded - nfar) far[nfar + lane] = A.sentinel;
     __builtin_amdgcn_wave_barrier();
-    for (int f0 = 0; f0 < padded; f0 += C)
+    int f0 = 0;
+    if constexpr (kWide) {
+      for (; f0 + 2 * C <= padded; f0 += 2 * C)
+        far_batch<VJP, ORDER, FORCES, 2 * C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
+    }
+    for (; f0 < padded; f0 += C)
       far_batch<VJP, ORDER, FORCES, C>(A, L, far + f0, lane, fx, fy, fz, gx, gy, gz);
     __builtin_amdgcn_wave_barrier();
   };
@@ -1238,6 +1247,10 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
           near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
       } else {
         int j0 = s0;
+        if constexpr (kWide) {
+          for (; j0 + 2 * kChunk <= s1; j0 += 2 * kChunk)
+            near_batch<VJP, FORCES, 2 * kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+        }
         for (; j0 + kChunk <= s1; j0 += kChunk)
           near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
         if (j0 + 4 <= s1) {
