@@ -328,7 +328,9 @@ __global__ __launch_bounds__(kB) void k_gather(const float* __restrict__ pts,
 //   k_topo_emit      node records from their first points, node ends and last
 //                    children from their last points, group starts
 //   k_topo_finalize  counts and child counts of the records, group sizes, and
-//                    the moments of every leaf (one wavefront per node)
+//                    the moments of every leaf (a row of 16 lanes per node);
+//                    k_moments_internal then does the internal levels, one
+//                    launch per level, deepest first
 constexpr int kTopoTile = 1024;  // one point per thread, one tile per block
 constexpr int kTopoWaves = kTopoTile / 64;
 constexpr int kTopoCols = topo::kLevels;
