@@ -1,17 +1,17 @@
 """Autotuner for the batched gravity pipeline.
 
-Times the captured fwd + VJP graph of the actual batch for candidate (adjoint
-expansion order, leaf size, persistent walk grid, walk block size) and keeps
-the fastest candidate whose forces and gradients stay within rel_l2 <= 1e-3
-of an FP64 direct sum on sampled rows: pooled over all sets and, separately,
-on every set sampled with at least 16 rows (rows are dealt to the sets in
-proportion to their size, 4 to 64 each, so the largest sets, where the
-error is largest, are checked on their own).  The search is coordinate-wise
-by default (adjoint orders x leaf sizes, then grids at the best pair, then
-block sizes: 10 + 2 + 3 captures) or the full product.  Results are cached
-per workload key (device, total points, set sizes, a digest of the per-set
-bounding boxes and the physics parameters) for the life of the process, and
-optionally in a JSON file.
+Times the captured fwd + VJP graph of the actual batch for candidate
+(forward and adjoint expansion orders, leaf size, persistent walk grid, walk
+block size) and keeps the fastest candidate whose forces and gradients stay
+within rel_l2 <= 1e-3 of an FP64 direct sum on sampled rows: pooled over all
+sets and, separately, on every set sampled with at least 16 rows (rows are
+dealt to the sets in proportion to their size, 4 to 64 each, so the largest
+sets, where the error is largest, are checked on their own).  The search is
+coordinate-wise by default (adjoint orders x leaf sizes, then forward orders,
+grids and block sizes at the best so far: 10 + 1 + 2 + 3 captures) or the
+full product.  Results are cached per workload key (device, total points,
+set sizes, a digest of the per-set bounding boxes and the physics
+parameters) for the life of the process, and optionally in a JSON file.
 """
 from __future__ import annotations
 
@@ -33,6 +33,8 @@ DEFAULT_BLOCKS = (0, 1024, 4096)
 DEFAULT_THREADS = (256, 64, 128, 512)  # walk block size; 256 is the library default
 # adjoint expansions: the second-moment adjoint (cheaper), then the full quadrupole
 DEFAULT_VJP_ORDERS = (G.SECOND_MOMENT_ADJOINT, 2)
+# forward expansions: the quadrupole first (the more accurate), then the monopole
+DEFAULT_FWD_ORDERS = (2, 1)
 REL_L2_GATE = 1e-3
 RELAX_FACTOR = 1.25  # on_fail="relax": gate = this x the best accuracy reached
 GATE_ROWS = 512        # FP64 rows over the whole batch
@@ -178,38 +180,40 @@ def time_graph(step, reps: int = 3) -> float:
 def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
                walk_blocks: Iterable[int] = DEFAULT_BLOCKS,
                walk_threads: Iterable[int] = DEFAULT_THREADS,
-               vjp_orders: Optional[Iterable[int]] = None, search: str = "coordinate",
+               vjp_orders: Optional[Iterable[int]] = None,
+               fwd_orders: Optional[Iterable[int]] = None, search: str = "coordinate",
                reps: int = 3, cache_file: Optional[str] = None, gate: float = REL_L2_GATE,
                gate_rows: int = GATE_ROWS, on_fail: str = "raise", **kw) -> dict:
-    """Best {vjp_order, leaf_size, walk_blocks, walk_threads, ms} for this
-    batch, plus the table of every candidate timed.
+    """Best {fwd_order, vjp_order, leaf_size, walk_blocks, walk_threads, ms}
+    for this batch, plus the table of every candidate timed.
 
     Candidates whose forces or gradients miss rel_l2 <= `gate` against the
     FP64 direct sum on `gate_rows` sampled rows (see _Gate) are dropped.  If
     none of the first stage passes: on_fail="raise" raises RuntimeError,
     on_fail="relax" sets the gate to RELAX_FACTOR x the smallest error any
     candidate reached (the timings are kept) and says so in the result's
-    "gate".  vjp_orders: adjoint expansions to
-    try (default DEFAULT_VJP_ORDERS, or kw["multipole_order"] alone when
-    given).  search: "coordinate" (adjoint order x leaf size, then grid, then
-    block size, the others at their best so far) or "grid" (every
-    combination).  kw: beta, softening_scale, max_depth, multipole_order (the
-    adjoint's, as gravity_force_vjp), fwd_multipole_order (the forward
-    walk's; default the adjoint's if that is 1 or 2, else 2)."""
+    "gate".  vjp_orders / fwd_orders: adjoint and forward expansions to try
+    (defaults DEFAULT_VJP_ORDERS / DEFAULT_FWD_ORDERS, or kw["multipole_order"]
+    / kw["fwd_multipole_order"] alone when given).  search: "coordinate"
+    (adjoint order x leaf size at the first forward order, then the other
+    forward orders, then grid, then block size, the others at their best so
+    far) or "grid" (every combination).  kw: beta, softening_scale,
+    max_depth."""
     kw = dict(kw)
     given = kw.pop("multipole_order", None)
+    given_fwd = kw.pop("fwd_multipole_order", None)
     if vjp_orders is None:
         vjp_orders = DEFAULT_VJP_ORDERS if given is None else (given,)
+    if fwd_orders is None:
+        fwd_orders = DEFAULT_FWD_ORDERS if given_fwd is None else (given_fwd,)
     leaf_sizes, walk_blocks, walk_threads = list(leaf_sizes), list(walk_blocks), list(walk_threads)
-    vjp_orders = list(vjp_orders)
+    vjp_orders, fwd_orders = list(vjp_orders), list(fwd_orders)
     if search not in ("coordinate", "grid"):
         raise ValueError("search must be 'coordinate' or 'grid'")
     if on_fail not in ("raise", "relax"):
         raise ValueError("on_fail must be 'raise' or 'relax'")
-    if kw.get("fwd_multipole_order") is None:
-        kw["fwd_multipole_order"] = given if given in (1, 2) else 2
     key = (_key(P, offsets, kw) + f"_gate{gate}x{gate_rows}{on_fail}_{search}_o{vjp_orders}"
-           f"_l{leaf_sizes}_b{walk_blocks}_t{walk_threads}").replace(" ", "")
+           f"_f{fwd_orders}_l{leaf_sizes}_b{walk_blocks}_t{walk_threads}").replace(" ", "")
     if key in _CACHE:
         return _CACHE[key]
     if cache_file and os.path.exists(cache_file):
@@ -223,17 +227,18 @@ def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
     done = {}
     requested = gate
 
-    def run(order, leaf, blocks, threads):
-        cand = (order, leaf, blocks, threads)
+    def run(fwd, order, leaf, blocks, threads):
+        cand = (fwd, order, leaf, blocks, threads)
         if cand in done:
             return done[cand]
-        step, (F, gx, gm), ws = build_step(P, M, C, offsets, leaf, blocks,
-                                           dict(kw, multipole_order=order), threads)
+        step, (F, gx, gm), ws = build_step(
+            P, M, C, offsets, leaf, blocks,
+            dict(kw, multipole_order=order, fwd_multipole_order=fwd), threads)
         ms = time_graph(step, reps)
         step()
         err = checker.errors(F, gx, gm)
         ok = max(err.values()) <= gate  # the gate in force when timed (see relax below)
-        row = {"vjp_order": order, "leaf_size": leaf, "walk_blocks": blocks,
+        row = {"fwd_order": fwd, "vjp_order": order, "leaf_size": leaf, "walk_blocks": blocks,
                "walk_threads": threads, "ms": ms, "rel_l2": err, "passes_gate": ok}
         table.append(row)
         done[cand] = row
@@ -255,29 +260,34 @@ def tune_batch(P, M, C, offsets, *, leaf_sizes: Iterable[int] = DEFAULT_LEAF,
                 r["passes_gate"] = max(r["rel_l2"].values()) <= gate
 
     if search == "grid":
-        for order in vjp_orders:
-            for leaf in leaf_sizes:
-                for blocks in walk_blocks:
-                    for threads in walk_threads:
-                        run(order, leaf, blocks, threads)
+        for fwd in fwd_orders:
+            for order in vjp_orders:
+                for leaf in leaf_sizes:
+                    for blocks in walk_blocks:
+                        for threads in walk_threads:
+                            run(fwd, order, leaf, blocks, threads)
         relax(table)
     else:
-        b0, t0 = walk_blocks[0], walk_threads[0]
-        first = [run(o, leaf, b0, t0) for o in vjp_orders for leaf in leaf_sizes]
+        f0, b0, t0 = fwd_orders[0], walk_blocks[0], walk_threads[0]
+        first = [run(f0, o, leaf, b0, t0) for o in vjp_orders for leaf in leaf_sizes]
         relax(first)
         cur = best_of(first)
         if cur is not None:
-            cur = best_of([run(cur["vjp_order"], cur["leaf_size"], b, t0) for b in walk_blocks])
+            cur = best_of([run(f, cur["vjp_order"], cur["leaf_size"], b0, t0)
+                           for f in fwd_orders])
         if cur is not None:
-            cur = best_of([run(cur["vjp_order"], cur["leaf_size"], cur["walk_blocks"], t)
-                           for t in walk_threads])
+            cur = best_of([run(cur["fwd_order"], cur["vjp_order"], cur["leaf_size"], b, t0)
+                           for b in walk_blocks])
+        if cur is not None:
+            cur = best_of([run(cur["fwd_order"], cur["vjp_order"], cur["leaf_size"],
+                               cur["walk_blocks"], t) for t in walk_threads])
     best = best_of(table)
     if best is None:
         raise RuntimeError(f"no candidate meets rel_l2 <= {gate}: {table}")
     if gate != requested:
         warnings.warn(f"autotune: no candidate met rel_l2 <= {requested}; gate relaxed to "
                       f"{gate}")
-    res = {"vjp_order": best["vjp_order"], "fwd_order": kw["fwd_multipole_order"],
+    res = {"vjp_order": best["vjp_order"], "fwd_order": best["fwd_order"],
            "leaf_size": best["leaf_size"], "walk_blocks": best["walk_blocks"],
            "walk_threads": best["walk_threads"], "ms": best["ms"], "table": table,
            "search": search,

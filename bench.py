@@ -1524,18 +1524,15 @@ def measure_batched(args, dev, tune=True, cpu=False):
     C = torch.tensor(np.concatenate(cs), device=dev)
     offs = G.set_offsets(sizes, dev)
     n_total = int(P.shape[0])
-    # the forward walk keeps the quadrupole here: on these dense blobs at
-    # eps = 0.005 the monopole's largest-set force error (6e-4) leaves little
-    # margin; the autotuner picks the adjoint expansion with the leaf size
-    fwd_order = 2
+    # the autotuner picks both expansion orders with the leaf size, grid and
+    # block size
     base = dict(beta=args.beta, softening_scale=args.eps, max_depth=args.depth)
     if tune:
-        tuned = autotune.tune_batch(P, M, C, offs, reps=2, fwd_multipole_order=fwd_order,
-                                    on_fail="relax", **base)
+        tuned = autotune.tune_batch(P, M, C, offs, reps=2, on_fail="relax", **base)
     else:
-        tuned = {"vjp_order": args.order, "leaf_size": args.leaf, "walk_blocks": 0,
-                 "walk_threads": 0, "table": None}
-    vjp_order = tuned["vjp_order"]
+        tuned = {"fwd_order": args.fwd_order, "vjp_order": args.order, "leaf_size": args.leaf,
+                 "walk_blocks": 0, "walk_threads": 0, "table": None}
+    fwd_order, vjp_order = tuned["fwd_order"], tuned["vjp_order"]
     kw = dict(base, multipole_order=vjp_order)
     kw_f = dict(base, multipole_order=fwd_order)
     kw_t = dict(kw, fwd_multipole_order=fwd_order)  # the autotuner's step

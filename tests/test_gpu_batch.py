@@ -175,18 +175,22 @@ def test_autotuner_gate_and_search(gpu):
     ps, ms, cs = _sets(sizes, 13)
     P, M, C = _dev(ps, ms, cs)
     offs = G.set_offsets(sizes)
-    kw = dict(beta=2.0, softening_scale=0.005, max_depth=10, fwd_multipole_order=2)
+    kw = dict(beta=2.0, softening_scale=0.005, max_depth=10)
     res = autotune.tune_batch(P, M, C, offs, leaf_sizes=(16, 32), walk_blocks=(0, 512),
                               walk_threads=(256, 128), reps=1, **kw)
     table = res["table"]
-    assert len(table) == 4 + 1 + 1  # orders x leaves, then one more grid, one more block size
+    # adjoint orders x leaves, then the monopole forward, one more grid, one
+    # more block size
+    assert len(table) == 4 + 1 + 1 + 1
     assert {r["vjp_order"] for r in table} == {2, G.SECOND_MOMENT_ADJOINT}
+    assert {r["fwd_order"] for r in table} == {1, 2}
     passing = [r for r in table if r["passes_gate"]]
     assert passing and res["ms"] == min(r["ms"] for r in passing)
     best = [r for r in passing if r["ms"] == res["ms"]][0]
-    assert (res["vjp_order"], res["leaf_size"], res["walk_blocks"], res["walk_threads"]) == (
-        best["vjp_order"], best["leaf_size"], best["walk_blocks"], best["walk_threads"])
-    assert res["fwd_order"] == 2 and res["gate"]["rows"] > 0
+    assert (res["fwd_order"], res["vjp_order"], res["leaf_size"], res["walk_blocks"],
+            res["walk_threads"]) == (best["fwd_order"], best["vjp_order"], best["leaf_size"],
+                                     best["walk_blocks"], best["walk_threads"])
+    assert res["gate"]["rows"] > 0
     assert res["gate"]["sets_checked_alone"] >= 1
     for r in table:
         assert r["passes_gate"] == (max(r["rel_l2"].values()) <= 1e-3)
@@ -198,8 +202,8 @@ def test_autotuner_gate_and_search(gpu):
     # relaxed: the gate becomes 1.25 x the best accuracy any candidate reached
     with pytest.warns(UserWarning, match="relaxed"):
         rel = autotune.tune_batch(P, M, C, offs, leaf_sizes=(16, 32), walk_blocks=(0,),
-                                  walk_threads=(256,), reps=1, gate=1e-12, on_fail="relax",
-                                  **kw)
+                                  walk_threads=(256,), fwd_orders=(2,), reps=1, gate=1e-12,
+                                  on_fail="relax", **kw)
     errs = [max(r["rel_l2"].values()) for r in rel["table"]]
     assert rel["gate"]["relaxed"] and rel["gate"]["requested_rel_l2_max"] == 1e-12
     assert rel["gate"]["rel_l2_max"] == pytest.approx(autotune.RELAX_FACTOR * min(errs))
