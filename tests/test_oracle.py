@@ -406,6 +406,55 @@ def test_octree8_walk_beta_zero_is_root_monopole():
     np.testing.assert_allclose(F, -np.float32(M) * d / (r + 0.1)[:, None] ** 3, rtol=1e-6)
 
 
+def _octree8_walk_numpy(p_leaf, m_leaf, nodes, q, beta, denom="diameter"):
+    """Independent restatement of the softened octree8 walk's decisions
+    (src/sbh_ffi.cu:1878-1893, 1995-2042) over the decoded node records:
+    empty nodes are skipped, a leaf is summed exactly, an internal node is
+    its monopole iff |q - centre| / node.<denom> >= beta (fp32 as the
+    reference), otherwise its children are visited.  Returns the source list
+    (positions, masses) the walk sums for q."""
+    pos, mass = [], []
+    stack = [0]
+    while stack:
+        nd = nodes[stack.pop()]
+        c = nd["contrib"]
+        if c["mass"] == 0.0:
+            continue
+        d = (q - np.array([nd["cx"], nd["cy"], nd["cz"]], np.float32)).astype(np.float32)
+        dist = np.sqrt(np.float32((d * d).sum(dtype=np.float32)), dtype=np.float32)
+        if nd["is_leaf"]:
+            pos.append(p_leaf[nd["start"]:nd["end"]])
+            mass.append(m_leaf[nd["start"]:nd["end"]])
+        elif np.float32(dist / nd[denom]) >= np.float32(beta):
+            pos.append(c["pos"][None])
+            mass.append(np.array([c["mass"]], np.float32))
+        else:
+            stack.extend(int(nd["child"][k]) for k in range(8) if nd["child_mask"] >> k & 1)
+    return np.concatenate(pos).astype(np.float32), np.concatenate(mass).astype(np.float32)
+
+
+def test_octree8_walk_finite_beta_decisions():
+    # At beta = 2 the oracle's walk equals the direct sum over the sources an
+    # independent numpy walk selects with the reference MAC (distance to the
+    # cell centre over the cell diagonal); the same walk with the half size
+    # as denominator selects other sources and misses it, so the check pins
+    # the MAC itself, not only its beta -> inf and beta = 0 limits.
+    p, m, leaf, node = _octree8_case(seed=5, n=600, D=4)
+    lf, nd = O.decode_leaf3(leaf), O.decode_nodes8(node)
+    q = np.concatenate([p[:24], _rng(6).uniform(-3, 3, (24, 3)).astype(np.float32)])
+    eps, beta = 0.03, 2.0
+    F, _ = O.bh_octree8(leaf, node, q, O.KIND_SOFT, beta, eps=eps, rc=np.inf)
+    differs = 0
+    for k in range(q.shape[0]):
+        sp, sm = _octree8_walk_numpy(lf["pos"], lf["mass"], nd, q[k], beta)
+        ref = O.direct_force_soft(sp, sm, q[k:k + 1], eps, np.inf)
+        np.testing.assert_allclose(F[k:k + 1], ref, rtol=1e-9, atol=1e-12 * np.abs(ref).max())
+        hp, hm = _octree8_walk_numpy(lf["pos"], lf["mass"], nd, q[k], beta, denom="half_size")
+        alt = O.direct_force_soft(hp, hm, q[k:k + 1], eps, np.inf)
+        differs += int(np.abs(alt - F[k:k + 1]).max() > 1e-6 * np.abs(ref).max())
+    assert differs >= q.shape[0] // 2
+
+
 def test_octree8_prune_huge_cutoff_matches_disabled():
     p, m, leaf, node = _octree8_case(seed=11)
     q = p[:96]
