@@ -523,6 +523,7 @@ def test_tree_topology_invariants():
             for a in range(3):
                 ijk[a] |= ((int(pre[0]) >> (3 * b + a)) & 1) << b
         np.testing.assert_allclose(T.geo[k, :3], T.min_corner + (np.array(ijk) + 0.5) * side, atol=1e-5)
+        np.testing.assert_allclose(T.geo[k, 3], side * np.sqrt(3.0), rtol=1e-6)  # diagonal
 
 
 @pytest.mark.parametrize("gmax", [64, 32])
@@ -557,6 +558,43 @@ def test_tree_groups_partition_and_stay_in_one_cell(gmax):
     assert diag.max() <= np.float32(T.span) * np.sqrt(3) + 1e-3
     default = T.groups(gmax)
     assert (default[1] <= gmax).all() and default[1].sum() == n
+
+
+def test_tree_group_mac_counts_match_numpy_walk():
+    # The group walk's decisions at beta = 2: a node is far for a group iff
+    # the distance from its cell centre to the group's box is >= beta x the
+    # cell diagonal (fp32, no contraction), a rejected leaf is near, and
+    # empty nodes are skipped.  A numpy traversal with that rule reproduces
+    # the oracle's far and near counts group by group.
+    n = 4000
+    p = _plummer(n, 4)
+    m = np.full(n, 1.0 / n, np.float32)
+    T = O.Tree(p, m, max_depth=10, leaf_size=16)
+    gs, gc = T.groups(64)
+    boxes = T.group_boxes(T.points_sorted, 64).reshape(-1, 6)
+    beta = np.float32(2.0)
+    for g in np.linspace(0, len(gs) - 1, 12).astype(int):
+        b = boxes[g]
+        far = near = 0
+        stack = [0]
+        while stack:
+            k = stack.pop()
+            if T.count[k] == 0:
+                continue
+            c = T.geo[k]
+            d = np.maximum(np.maximum(b[:3] - c[:3], c[:3] - b[3:]), np.float32(0))
+            d2 = np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+            t = np.float32(beta * c[3])
+            if d2 >= np.float32(t * t):
+                far += int(gc[g])
+            elif T.nchild[k] == 0:
+                s0, s1 = T.start[k], T.start[k] + T.count[k]
+                own = max(0, min(s1, gs[g] + gc[g]) - max(s0, gs[g]))
+                near += int(T.count[k]) * int(gc[g]) - own
+            else:
+                stack.extend(range(T.first_child[k], T.first_child[k] + T.nchild[k]))
+        r = T.eval_groups(g, g + 1, beta=2.0, eps=0.01)
+        assert (r["far"], r["near"]) == (far, near), (g, r["far"], r["near"], far, near)
 
 
 def test_tree_huge_beta_is_direct_sum():
