@@ -1125,7 +1125,9 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
 
 // Near-field batch: CH consecutive sources starting at j0 (entries at or
 // beyond s1 carry zero weight).
-template <bool VJP, bool FORCES, int CH>
+// SELF: the batch may hold the lane's own point. Batches of a range that
+// lies wholly outside the group's points pass false and skip that test.
+template <bool VJP, bool FORCES, int CH, bool SELF = true>
 __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
                                            int s1, int lane, float& fx, float& fy, float& fz,
                                            float& gx, float& gy, float& gz) {
@@ -1143,7 +1145,7 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
     const float s2 = s * s, s3 = s * s2;
     float ws3 = w * s3;  // w s^3
     // the self pair: its force term vanishes with d = 0, its adjoint terms do not
-    if constexpr (VJP) ws3 = (j == i) ? 0.0f : ws3;
+    if constexpr (VJP && SELF) ws3 = (j == i) ? 0.0f : ws3;
     if constexpr (FORCES) {
       fx = fmaf(-ws3, dx, fx);
       fy = fmaf(-ws3, dy, fy);
@@ -1249,6 +1251,14 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
           near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
       } else {
         int j0 = s0;
+        if constexpr (VJP) {
+          // ranges away from the group's own points skip the self-pair test
+          if (s1 <= base || s0 >= base + nvalid) {
+            for (; j0 + kChunk <= s1; j0 += kChunk)
+              near_batch<VJP, FORCES, kChunk, false>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy,
+                                                     gz);
+          }
+        }
         if constexpr (kWide) {
           for (; j0 + 2 * kChunk <= s1; j0 += 2 * kChunk)
             near_batch<VJP, FORCES, 2 * kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
