@@ -987,13 +987,48 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   // (dL/dS xx yy zz xy xz yz, minus dL/dM, minus dL/dX x) and (minus dL/dX y, z)
   float q8[8 * C];
   float q2[2 * C];
+  float mw[C];  // node masses (second-moment adjoint)
 #pragma unroll
   for (int t = 0; t < C; ++t) {
     const float4 m = sload4(A.mp, idx[t]);
     const float Dx = L.x - m.x, Dy = L.y - m.y, Dz = L.z - m.z;
     const float s = rsq(fmaf(Dz, Dz, fmaf(Dy, Dy, fmaf(Dx, Dx, A.e2))));
     const float s2 = s * s, s3 = s2 * s;
-    if constexpr (!QUAD) {
+    if constexpr (SMOM) {
+      // d(c.F)/dD = M s^3 e with e = 3 s^2 u D - c; the lanes' s^3 e are
+      // summed and scaled by the node's M in the scatter.  dL/dS as in the
+      // quadrupole branch below, doubled: with p = s^5 (c - 5 e), whose half
+      // is P2 there, the diagonal is p_a D_a + 3 s^5 u and the off-diagonal
+      // p_a D_b + D_a p_b (halved in the scatter).
+      if constexpr (FORCES) {
+        const float a = -m.w * s3;
+        fx = fmaf(a, Dx, fx);
+        fy = fmaf(a, Dy, fy);
+        fz = fmaf(a, Dz, fz);
+      }
+      mw[t] = m.w;
+      const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
+      const float h = 3.0f * (s2 * u);
+      const float ex = fmaf(h, Dx, -L.cx), ey = fmaf(h, Dy, -L.cy), ez = fmaf(h, Dz, -L.cz);
+      const float sx = s3 * ex, sy = s3 * ey, sz = s3 * ez;
+      gx = fmaf(m.w, sx, gx);
+      gy = fmaf(m.w, sy, gy);
+      gz = fmaf(m.w, sz, gz);
+      const float s5 = s3 * s2, w5 = s3 * h;  // s^5, 3 s^5 u
+      const float px = s5 * fmaf(-5.0f, ex, L.cx);
+      const float py = s5 * fmaf(-5.0f, ey, L.cy);
+      const float pz = s5 * fmaf(-5.0f, ez, L.cz);
+      q8[8 * t + 0] = fmaf(px, Dx, w5);
+      q8[8 * t + 1] = fmaf(py, Dy, w5);
+      q8[8 * t + 2] = fmaf(pz, Dz, w5);
+      q8[8 * t + 3] = fmaf(px, Dy, Dx * py);
+      q8[8 * t + 4] = fmaf(px, Dz, Dx * pz);
+      q8[8 * t + 5] = fmaf(py, Dz, Dy * pz);
+      q8[8 * t + 6] = s3 * u;
+      q8[8 * t + 7] = sx;
+      q2[2 * t + 0] = sy;
+      q2[2 * t + 1] = sz;
+    } else if constexpr (!QUAD) {
       const float a = -m.w * s3;
       if constexpr (FORCES) {
         fx = fmaf(a, Dx, fx);
@@ -1009,31 +1044,10 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         gx += dDx;
         gy += dDy;
         gz += dDz;
-        if constexpr (SMOM) {
-          // dL/dS as in the quadrupole branch below, with b = 3 s^5 and
-          // b15 / 2 = -7.5 s^7 u
-          const float s5 = s3 * s2;
-          const float b = 3.0f * s5;
-          const float hb15 = (-7.5f * (s5 * s2)) * u, hu = (0.5f * b) * u;
-          const float Px = fmaf(hb15, Dx, b * L.cx);
-          const float Py = fmaf(hb15, Dy, b * L.cy);
-          const float Pz = fmaf(hb15, Dz, b * L.cz);
-          q8[8 * t + 0] = fmaf(Px, Dx, hu);
-          q8[8 * t + 1] = fmaf(Py, Dy, hu);
-          q8[8 * t + 2] = fmaf(Pz, Dz, hu);
-          q8[8 * t + 3] = fmaf(Px, Dy, Dx * Py);
-          q8[8 * t + 4] = fmaf(Px, Dz, Dx * Pz);
-          q8[8 * t + 5] = fmaf(Py, Dz, Dy * Pz);
-          q8[8 * t + 6] = s3 * u;
-          q8[8 * t + 7] = dDx;
-          q2[2 * t + 0] = dDy;
-          q2[2 * t + 1] = dDz;
-        } else {
-          part[4 * t + 0] = dDx;
-          part[4 * t + 1] = dDy;
-          part[4 * t + 2] = dDz;
-          part[4 * t + 3] = s3 * u;
-        }
+        part[4 * t + 0] = dDx;
+        part[4 * t + 1] = dDy;
+        part[4 * t + 2] = dDz;
+        part[4 * t + 3] = s3 * u;
       }
     } else {
       const float4 q0 = sload4(A.qd, 2 * idx[t]);
@@ -1099,6 +1113,13 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   }
   if constexpr (VJP && (QUAD || SMOM)) {
     static_assert(C == 4, "the packed quadrupole adjoint assumes 4 nodes per batch");
+    // both layouts below put node lane >> 4's values on lane; the
+    // second-moment adjoint's dL/dX partials are scaled by the node's mass
+    float mx = -1.0f, sq = 1.0f;
+    if constexpr (SMOM) {
+      mx = -(lane < 32 ? (lane < 16 ? mw[0] : mw[1]) : (lane < 48 ? mw[2] : mw[3]));
+      sq = 0.5f;
+    }
     {
       // q8: value j = lane >> 1 is slot j & 7 of node j >> 3
       const float sum = wave_transpose_sum<8 * C>(q8, lane);
@@ -1107,7 +1128,7 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         const int e = far[j >> 3];
         if (e != A.sentinel) {
           float* dst = slot < 6 ? A.adjq + 8 * e + slot : A.adj + 4 * e + (slot == 6 ? 3 : 0);
-          atomicAdd(dst, slot < 6 ? sum : -sum);
+          atomicAdd(dst, sum * (slot < 6 ? sq : slot == 6 ? -1.0f : mx));
         }
       }
     }
@@ -1117,7 +1138,7 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       const int j = lane >> 3;
       if ((lane & 7) == 0) {
         const int e = far[j >> 1];
-        if (e != A.sentinel) atomicAdd(A.adj + 4 * e + 1 + (j & 1), -sum);
+        if (e != A.sentinel) atomicAdd(A.adj + 4 * e + 1 + (j & 1), mx * sum);
       }
     }
   }
