@@ -19,8 +19,8 @@
 //   lanes sharing each node / source record, which is read once per wave
 //   through the scalar cache (the records are read-only during the walk).
 // Adjoint: query-side terms stay in registers; source-side terms of each
-//   interaction batch are summed across the wave with a transposed butterfly
-//   and added with one float atomic per value; node adjoints are pushed down to
+//   interaction batch are summed across the wave through an LDS slab and
+//   added with one float atomic per value; node adjoints are pushed down to
 //   the points in k_finalize.
 #include "common.hpp"
 #include "tree.hpp"
@@ -848,74 +848,43 @@ __device__ __forceinline__ float dpp_partner(float v) {
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
-// One butterfly stage of a transposed wave reduction: lanes with bit `D` set
-// keep the upper half of the first 2H values (adding the partner's copy),
-// the others keep the lower half.  The partner is any lane whose bit D
-// differs while the higher bits agree (so it holds the same value indices
-// summed over a disjoint set of lanes); the DPP patterns above qualify.
-template <int H, int N, int D, int CTRL>
-__device__ __forceinline__ void butterfly(float (&v)[N], int lane) {
-  const bool hi = lane & D;
-#pragma unroll
-  for (int k = 0; k < H; ++k) {
-    float send = hi ? v[k] : v[H + k];
-    float keep = hi ? v[H + k] : v[k];
-    v[k] = keep + dpp_partner<CTRL>(send);
-  }
-}
+// Wave sums through LDS, used by the adjoint batches.  The V values of each
+// lane are written to the wave's slab as V rows of 64 (one float per lane,
+// rows kRedRow floats apart); 64 / V lanes per value then each add a run of
+// V consecutive floats of that value's row, and DPP adds combine the runs.
+// Lane l ends with the sum of value l / (64 / V) (all lanes of its run hold
+// it).  A value costs about one VALU add; a transposed butterfly in
+// registers (permlane swaps, selects and DPP adds) takes about two VALU
+// operations per value, and the walk is VALU bound, so the LDS pipe does
+// the transposition.  The slab is private to the wave, and the LDS
+// operations of one wave complete in issue order, so passes may reuse it.
+constexpr int kRedRow = 68;  // 64 lanes + 4 floats: staggers the runs over the banks
+constexpr int kRedMax = 16;  // values per pass (the slab holds kRedMax rows)
 
-// The d = 32 and d = 16 stages with the gfx950 half-wave / row swaps:
-// v_permlane32_swap exchanges lanes 32-63 of its first operand with lanes
-// 0-31 of the second (v_permlane16_swap: odd 16-lane rows of the first with
-// even rows of the second), so the sum of the two results is the butterfly
-// output with no select or LDS permute.
-template <int H, int N>
-__device__ __forceinline__ void butterfly_swap32(float (&v)[N]) {
-#pragma unroll
-  for (int k = 0; k < H; ++k) {
-    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[k]), __float_as_uint(v[H + k]),
-                                              false, false);
-    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  }
-}
-template <int H, int N>
-__device__ __forceinline__ void butterfly_swap16(float (&v)[N]) {
-#pragma unroll
-  for (int k = 0; k < H; ++k) {
-    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(v[k]), __float_as_uint(v[H + k]),
-                                              false, false);
-    v[k] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
-  }
-}
-
-// Transposed reduction of V = 2^k values per lane over the 64 lanes: returns
-// the 64-lane sum of v[lane >> (6 - k)] (each value is held by 64/V lanes).
 template <int V>
-__device__ __forceinline__ float wave_transpose_sum(float (&v)[V], int lane) {
-  static_assert(V == 4 || V == 8 || V == 16 || V == 32 || V == 64, "unsupported width");
-  butterfly_swap32<V / 2>(v);
-  butterfly_swap16<V / 4>(v);
-  if constexpr (V >= 8) {
-    butterfly<V / 8, V, 8, kRowRor8>(v, lane);
-  } else {
-    v[0] += dpp_partner<kRowRor8>(v[0]);
+__device__ __forceinline__ float lds_wave_sum(const float* v, float* red, int lane) {
+  static_assert(V == 4 || V == 8 || V == 16, "unsupported width");
+  constexpr int kLanes = 64 / V;  // lanes per value
+#pragma unroll
+  for (int k = 0; k < V; ++k) red[k * kRedRow + lane] = v[k];
+  const float4* p =
+      reinterpret_cast<const float4*>(red + (lane / kLanes) * kRedRow + (lane % kLanes) * V);
+  float r[V / 4];
+#pragma unroll
+  for (int k = 0; k < V / 4; ++k) {
+    const float4 q = p[k];
+    r[k] = (q.x + q.y) + (q.z + q.w);
   }
-  if constexpr (V >= 16) {
-    butterfly<V / 16, V, 4, kHalfMirror>(v, lane);
-  } else {
-    v[0] += dpp_partner<kHalfMirror>(v[0]);
-  }
-  if constexpr (V >= 32) {
-    butterfly<V / 32, V, 2, kQuadXor2>(v, lane);
-  } else {
-    v[0] += dpp_partner<kQuadXor2>(v[0]);
-  }
-  if constexpr (V >= 64) {
-    butterfly<1, V, 1, kQuadXor1>(v, lane);
-  } else {
-    v[0] += dpp_partner<kQuadXor1>(v[0]);
-  }
-  return v[0];
+#pragma unroll
+  for (int h = V / 8; h >= 1; h /= 2)
+#pragma unroll
+    for (int k = 0; k < h; ++k) r[k] += r[k + h];
+  float sum = r[0];
+  sum += dpp_partner<kQuadXor1>(sum);
+  sum += dpp_partner<kQuadXor2>(sum);
+  if constexpr (kLanes >= 8) sum += dpp_partner<kHalfMirror>(sum);
+  if constexpr (kLanes >= 16) sum += dpp_partner<kRowRor8>(sum);
+  return sum;
 }
 
 struct WalkArgs {
@@ -945,25 +914,29 @@ struct WalkArgs {
 struct Lane {
   float x, y, z;     // query position
   float cx, cy, cz;  // cotangent
+  float* red;        // the wave's LDS slab for the adjoint sums (kRedMax rows)
 };
-
-constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
 // Adds the wave-summed batch partials v[0..V) (PER values per entry, the
 // first USED of them meaningful; negated when NEG) to dst: value j goes to
 // dst[stride * e + j % PER] with e = entry[j / PER] (or base + j / PER when
-// entry is null); e == skip and e >= limit are dropped.  One lane per value
-// issues the atomic.
+// entry is null); e == skip and e >= limit are dropped.  The sums run
+// through the wave's slab in passes of up to kRedMax values; one lane per
+// value issues the atomic.
 template <int V, int PER, int USED, bool NEG>
-__device__ __forceinline__ void scatter_batch(float (&v)[V], int lane, float* dst, int stride,
-                                              const int* entry, int skip, int base, int limit) {
-  constexpr int sh = 6 - ilog2(V);
-  const float sum = wave_transpose_sum<V>(v, lane);
-  const float s = NEG ? -sum : sum;
-  const int j = lane >> sh;
-  if ((lane & ((1 << sh) - 1)) == 0 && (j % PER) < USED) {
-    const int e = entry != nullptr ? entry[j / PER] : base + j / PER;
-    if (e != skip && e < limit) atomicAdd(&dst[stride * e + (j % PER)], s);
+__device__ __forceinline__ void scatter_batch(const float (&v)[V], float* red, int lane,
+                                              float* dst, int stride, const int* entry, int skip,
+                                              int base, int limit) {
+  constexpr int P = V < kRedMax ? V : kRedMax;  // values per pass
+  constexpr int kLanes = 64 / P;
+#pragma unroll
+  for (int c = 0; c < V; c += P) {
+    const float sum = lds_wave_sum<P>(v + c, red, lane);
+    const int j = c + lane / kLanes;
+    if (lane % kLanes == 0 && (j % PER) < USED) {
+      const int e = entry != nullptr ? entry[j / PER] : base + j / PER;
+      if (e != skip && e < limit) atomicAdd(&dst[stride * e + (j % PER)], NEG ? -sum : sum);
+    }
   }
 }
 
@@ -1109,23 +1082,22 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   }
   if constexpr (VJP && !QUAD && !SMOM) {
     // node adjoints (dL/dX, dL/dM)
-    scatter_batch<4 * C, 4, 4, true>(part, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
+    scatter_batch<4 * C, 4, 4, true>(part, L.red, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
   }
   if constexpr (VJP && (QUAD || SMOM)) {
     static_assert(C == 4, "the packed quadrupole adjoint assumes 4 nodes per batch");
-    // both layouts below put node lane >> 4's values on lane; the
-    // second-moment adjoint's dL/dX partials are scaled by the node's mass
-    float mx = -1.0f, sq = 1.0f;
-    if constexpr (SMOM) {
-      mx = -(lane < 32 ? (lane < 16 ? mw[0] : mw[1]) : (lane < 48 ? mw[2] : mw[3]));
-      sq = 0.5f;
-    }
-    {
-      // q8: value j = lane >> 1 is slot j & 7 of node j >> 3
-      const float sum = wave_transpose_sum<8 * C>(q8, lane);
-      const int j = lane >> 1, slot = j & 7;
-      if ((lane & 1) == 0) {
-        const int e = far[j >> 3];
+    // the second-moment adjoint's dL/dX partials are scaled by the node's
+    // mass, its dL/dS partials (doubled) by 1/2
+    constexpr float sq = SMOM ? 0.5f : 1.0f;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      // q8 of nodes 2c, 2c + 1: value j = lane >> 2 of the pass is slot j & 7
+      // of node 2c + (j >> 3)
+      const float sum = lds_wave_sum<16>(q8 + 16 * c, L.red, lane);
+      const int j = lane >> 2, slot = j & 7;
+      const float mx = SMOM ? -(lane < 32 ? mw[2 * c] : mw[2 * c + 1]) : -1.0f;
+      if ((lane & 3) == 0) {
+        const int e = far[2 * c + (j >> 3)];
         if (e != A.sentinel) {
           float* dst = slot < 6 ? A.adjq + 8 * e + slot : A.adj + 4 * e + (slot == 6 ? 3 : 0);
           atomicAdd(dst, sum * (slot < 6 ? sq : slot == 6 ? -1.0f : mx));
@@ -1134,8 +1106,11 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
     }
     {
       // q2: value j = lane >> 3 is component 1 + (j & 1) of node j >> 1's dL/dX
-      const float sum = wave_transpose_sum<2 * C>(q2, lane);
+      const float sum = lds_wave_sum<8>(q2, L.red, lane);
       const int j = lane >> 3;
+      float mx = -1.0f;
+      if constexpr (SMOM)
+        mx = -(lane < 32 ? (lane < 16 ? mw[0] : mw[1]) : (lane < 48 ? mw[2] : mw[3]));
       if ((lane & 7) == 0) {
         const int e = far[j >> 1];
         if (e != A.sentinel) atomicAdd(A.adj + 4 * e + 1 + (j & 1), mx * sum);
@@ -1189,18 +1164,19 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
     }
   }
   if constexpr (VJP)
-    scatter_batch<4 * CH, 4, 4, true>(part, lane, A.sgrad, 4, nullptr, -1, j0, s1);
+    scatter_batch<4 * CH, 4, 4, true>(part, L.red, lane, A.sgrad, 4, nullptr, -1, j0, s1);
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
 template <bool VJP, int ORDER, bool FORCES>
 __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt, int lane,
-                                           int* stack, int* far, int* near,
+                                           int* stack, int* far, int* near, float* red,
                                            unsigned long long& n_far,
                                            unsigned long long& n_near) {
   const int i = base + lane;
   const bool valid = lane < cnt;
   Lane L;
+  L.red = red;
   {
     const float4 me = A.sp[valid ? i : base];
     L.x = me.x;
@@ -1327,6 +1303,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
   __shared__ int s_stack[WPB][kStackCap];
   __shared__ int s_far[WPB][kFarCap];
   __shared__ int s_near[WPB][kNearCap];
+  // the adjoint's wave sums (the forward walk needs none)
+  __shared__ __align__(16) float s_red[WPB][VJP ? kRedMax * kRedRow : 4];
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int num_groups = A.hdr->num_groups;
@@ -1339,8 +1317,8 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
     if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.n) continue;  // corrupt entry
-    walk_group<VJP, ORDER, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w], n_far,
-                                   n_near);
+    walk_group<VJP, ORDER, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w],
+                                   s_red[w], n_far, n_near);
     ++n_groups;
   }
   if (A.stats != nullptr && lane == 0) {
