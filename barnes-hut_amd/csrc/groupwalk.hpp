@@ -35,6 +35,14 @@ __device__ __forceinline__ float4 sload4(const float4* p, int i) {
   cfloat* q = (cfloat*)(p) + 4 * static_cast<int64_t>(i);  // generic -> constant space
   return make_float4(q[0], q[1], q[2], q[3]);
 }
+// Record u of a run starting at q (constant offsets from one base, so the
+// loads of consecutive records can merge into wider scalar loads).
+__device__ __forceinline__ float4 sload4_run(cfloat* q, int u) {
+  return make_float4(q[4 * u], q[4 * u + 1], q[4 * u + 2], q[4 * u + 3]);
+}
+__device__ __forceinline__ cfloat* run_base(const float4* p, int i0) {
+  return (cfloat*)(p) + 4 * static_cast<int64_t>(i0);
+}
 __device__ __forceinline__ int4 sload4(const int4* p, int i) {
   cint* q = (cint*)(p) + 4 * static_cast<int64_t>(i);
   return make_int4(q[0], q[1], q[2], q[3]);

@@ -914,16 +914,18 @@ struct WalkArgs {
 struct Lane {
   float x, y, z;     // query position
   float cx, cy, cz;  // cotangent
-  float* red;        // the wave's LDS slab for the adjoint sums (kRedMax rows)
+  float c3x, c3y, c3z;  // 3 x cotangent (adjoint): the 3 of 3 s^2 (c.D) comes free
+  float* red;      // the wave's LDS slab for the adjoint sums (kRedMax rows)
 };
 
 // Adds the wave-summed batch partials v[0..V) (PER values per entry, the
 // first USED of them meaningful; negated when NEG) to dst: value j goes to
 // dst[stride * e + j % PER] with e = entry[j / PER] (or base + j / PER when
-// entry is null); e == skip and e >= limit are dropped.  The sums run
-// through the wave's slab in passes of up to kRedMax values; one lane per
-// value issues the atomic.
-template <int V, int PER, int USED, bool NEG>
+// entry is null); e == skip and e >= limit are dropped.  THIRD: value 3 of
+// each entry arrives tripled and is scaled by 1/3.  The sums run through the
+// wave's slab in passes of up to kRedMax values; one lane per value issues
+// the atomic.
+template <int V, int PER, int USED, bool NEG, bool THIRD = false>
 __device__ __forceinline__ void scatter_batch(const float (&v)[V], float* red, int lane,
                                               float* dst, int stride, const int* entry, int skip,
                                               int base, int limit) {
@@ -933,9 +935,10 @@ __device__ __forceinline__ void scatter_batch(const float (&v)[V], float* red, i
   for (int c = 0; c < V; c += P) {
     const float sum = lds_wave_sum<P>(v + c, red, lane);
     const int j = c + lane / kLanes;
+    const float scale = (THIRD && j % PER == 3 ? 1.0f / 3.0f : 1.0f) * (NEG ? -1.0f : 1.0f);
     if (lane % kLanes == 0 && (j % PER) < USED) {
       const int e = entry != nullptr ? entry[j / PER] : base + j / PER;
-      if (e != skip && e < limit) atomicAdd(&dst[stride * e + (j % PER)], NEG ? -sum : sum);
+      if (e != skip && e < limit) atomicAdd(&dst[stride * e + (j % PER)], scale * sum);
     }
   }
 }
@@ -980,8 +983,8 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         fz = fmaf(a, Dz, fz);
       }
       mw[t] = m.w;
-      const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
-      const float h = 3.0f * (s2 * u);
+      const float u3 = L.c3x * Dx + L.c3y * Dy + L.c3z * Dz;  // 3 u
+      const float h = s2 * u3;
       const float ex = fmaf(h, Dx, -L.cx), ey = fmaf(h, Dy, -L.cy), ez = fmaf(h, Dz, -L.cz);
       const float sx = s3 * ex, sy = s3 * ey, sz = s3 * ez;
       gx = fmaf(m.w, sx, gx);
@@ -997,7 +1000,7 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
       q8[8 * t + 3] = fmaf(px, Dy, Dx * py);
       q8[8 * t + 4] = fmaf(px, Dz, Dx * pz);
       q8[8 * t + 5] = fmaf(py, Dz, Dy * pz);
-      q8[8 * t + 6] = s3 * u;
+      q8[8 * t + 6] = s3 * u3;  // scaled by 1/3 in the scatter
       q8[8 * t + 7] = sx;
       q2[2 * t + 0] = sy;
       q2[2 * t + 1] = sz;
@@ -1009,8 +1012,8 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         fz = fmaf(a, Dz, fz);
       }
       if constexpr (VJP) {
-        const float u = L.cx * Dx + L.cy * Dy + L.cz * Dz;
-        const float kD = -3.0f * a * s2 * u;  // 3 M s^5 u
+        const float u3 = L.c3x * Dx + L.c3y * Dy + L.c3z * Dz;  // 3 u
+        const float kD = -a * s2 * u3;  // 3 M s^5 u
         const float dDx = kD * Dx + a * L.cx;
         const float dDy = kD * Dy + a * L.cy;
         const float dDz = kD * Dz + a * L.cz;
@@ -1020,7 +1023,7 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         part[4 * t + 0] = dDx;
         part[4 * t + 1] = dDy;
         part[4 * t + 2] = dDz;
-        part[4 * t + 3] = s3 * u;
+        part[4 * t + 3] = s3 * u3;  // scaled by 1/3 in the scatter
       }
     } else {
       const float4 q0 = sload4(A.qd, 2 * idx[t]);
@@ -1082,13 +1085,16 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
   }
   if constexpr (VJP && !QUAD && !SMOM) {
     // node adjoints (dL/dX, dL/dM)
-    scatter_batch<4 * C, 4, 4, true>(part, L.red, lane, A.adj, 4, far, A.sentinel, 0, INT_MAX);
+    scatter_batch<4 * C, 4, 4, true, true>(part, L.red, lane, A.adj, 4, far, A.sentinel, 0,
+                                           INT_MAX);
   }
   if constexpr (VJP && (QUAD || SMOM)) {
     static_assert(C == 4, "the packed quadrupole adjoint assumes 4 nodes per batch");
     // the second-moment adjoint's dL/dX partials are scaled by the node's
-    // mass, its dL/dS partials (doubled) by 1/2
+    // mass, its dL/dS partials (doubled) by 1/2 and its dL/dM partials
+    // (tripled) by 1/3
     constexpr float sq = SMOM ? 0.5f : 1.0f;
+    constexpr float s6 = SMOM ? -1.0f / 3.0f : -1.0f;
 #pragma unroll
     for (int c = 0; c < 2; ++c) {
       // q8 of nodes 2c, 2c + 1: value j = lane >> 2 of the pass is slot j & 7
@@ -1100,7 +1106,7 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
         const int e = far[2 * c + (j >> 3)];
         if (e != A.sentinel) {
           float* dst = slot < 6 ? A.adjq + 8 * e + slot : A.adj + 4 * e + (slot == 6 ? 3 : 0);
-          atomicAdd(dst, sum * (slot < 6 ? sq : slot == 6 ? -1.0f : mx));
+          atomicAdd(dst, sum * (slot < 6 ? sq : slot == 6 ? s6 : mx));
         }
       }
     }
@@ -1123,19 +1129,29 @@ __device__ __forceinline__ void far_batch(const WalkArgs& A, const Lane& L, cons
 // beyond s1 carry zero weight).
 // SELF: the batch may hold the lane's own point. Batches of a range that
 // lies wholly outside the group's points pass false and skip that test.
-template <bool VJP, bool FORCES, int CH, bool SELF = true>
+// PAD: the batch may run past s1; batches that fit read CH consecutive
+// records with no clamp.
+template <bool VJP, bool FORCES, int CH, bool SELF = true, bool PAD = false>
 __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int i, int j0,
                                            int s1, int lane, float& fx, float& fy, float& fz,
                                            float& gx, float& gy, float& gz) {
   float part[4 * CH];  // minus the source-side partials (negated in the scatter)
+  cfloat* run = run_base(A.sp, j0);
 #pragma unroll
   for (int u = 0; u < CH; ++u) {
     const int j = j0 + u;
-    const float4 src = sload4(A.sp, min(j, s1 - 1));
+    // the adjoint reads runs of 4 records with one wide load (its batches
+    // are long enough to hide the wait); the forward walk's own schedule of
+    // single-record loads overlaps them better with its shorter batches
+    const float4 src = PAD    ? sload4(A.sp, min(j, s1 - 1))
+                       : VJP ? sload4_run(run, u)
+                             : sload4(A.sp, j);
     // zero weight past the range end; j, s1 and the record are wave-uniform,
     // so the mask stays on the scalar unit
-    const float w = __uint_as_float(
-        __builtin_amdgcn_readfirstlane(__float_as_uint(src.w) & (j < s1 ? ~0u : 0u)));
+    const float w =
+        PAD ? __uint_as_float(__builtin_amdgcn_readfirstlane(__float_as_uint(src.w) &
+                                                             (j < s1 ? ~0u : 0u)))
+            : src.w;
     const float dx = L.x - src.x, dy = L.y - src.y, dz = L.z - src.z;
     const float s = rsq(fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, A.e2))));
     const float s2 = s * s, s3 = s * s2;
@@ -1148,9 +1164,9 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
       fz = fmaf(-ws3, dz, fz);
     }
     if constexpr (VJP) {
-      const float cd = L.cx * dx + L.cy * dy + L.cz * dz;
+      const float cd3 = L.c3x * dx + L.c3y * dy + L.c3z * dz;  // 3 (c.d)
       // v = 3 w s^5 (c.d) d - w s^3 c = w s^3 (3 s^2 (c.d) d - c)
-      const float k = (3.0f * s2) * cd;
+      const float k = s2 * cd3;
       const float vx = ws3 * fmaf(k, dx, -L.cx);
       const float vy = ws3 * fmaf(k, dy, -L.cy);
       const float vz = ws3 * fmaf(k, dz, -L.cz);
@@ -1160,11 +1176,11 @@ __device__ __forceinline__ void near_batch(const WalkArgs& A, const Lane& L, int
       part[4 * u + 0] = vx;
       part[4 * u + 1] = vy;
       part[4 * u + 2] = vz;
-      part[4 * u + 3] = s3 * cd;  // s^3 (c.d); exactly 0 for the self pair (d = 0)
+      part[4 * u + 3] = s3 * cd3;  // 3 s^3 (c.d); exactly 0 for the self pair (d = 0)
     }
   }
   if constexpr (VJP)
-    scatter_batch<4 * CH, 4, 4, true>(part, L.red, lane, A.sgrad, 4, nullptr, -1, j0, s1);
+    scatter_batch<4 * CH, 4, 4, true, true>(part, L.red, lane, A.sgrad, 4, nullptr, -1, j0, s1);
 }
 
 // One query group (sorted points [base, base + cnt)) processed by one wave.
@@ -1191,6 +1207,9 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
         L.cz = c.z;
       }
     }
+    L.c3x = 3.0f * L.cx;
+    L.c3y = 3.0f * L.cy;
+    L.c3z = 3.0f * L.cz;
   }
   const Box box = wave_box(L.x, L.y, L.z);
   const int nvalid = cnt;
@@ -1245,7 +1264,8 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
       // more than the padding saves)
       if constexpr (VJP && (ORDER == 2 || FORCES)) {
         for (int j0 = s0; j0 < s1; j0 += kChunk)
-          near_batch<VJP, FORCES, kChunk>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy, gz);
+          near_batch<VJP, FORCES, kChunk, true, true>(A, L, i, j0, s1, lane, fx, fy, fz, gx, gy,
+                                                      gz);
       } else {
         int j0 = s0;
         if constexpr (VJP) {
