@@ -80,8 +80,11 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
 // current eighth exhausted moves on to the next one (work stealing); after 8
 // exhausted eighths it gets -1.  `tries` is the calling wave's state (used by
 // lane 0 only), starting at 0.  Each wave makes at most 8 failing claims, so
-// every wave reaches the -1 exit.
-__device__ __forceinline__ int next_group(Header* hdr, int num_groups, int lane, int& tries) {
+// every wave reaches the -1 exit.  With `order` (a permutation of the groups
+// that keeps each eighth's groups inside it) claim t of an eighth takes
+// group order[t] instead of group t.
+__device__ __forceinline__ int next_group(Header* hdr, int num_groups, int lane, int& tries,
+                                          const int* order = nullptr) {
   int g = -1;
   if (lane == 0) {
     const int home = static_cast<int>(blockIdx.x & 7u);
@@ -91,13 +94,19 @@ __device__ __forceinline__ int next_group(Header* hdr, int num_groups, int lane,
       const int hi = static_cast<int>(static_cast<int64_t>(num_groups) * (q + 1) / 8);
       const int t = lo + atomicAdd(&hdr->xq[q], 1);
       if (t < hi) {
-        g = t;
+        g = order != nullptr ? order[t] : t;
         break;
       }
       ++tries;
     }
   }
   return __builtin_amdgcn_readfirstlane(g);
+}
+
+// Eighth q of `num_groups` groups: [lo, hi), as next_group deals them.
+__device__ __forceinline__ int2 eighth_range(int num_groups, int q) {
+  return make_int2(static_cast<int>(static_cast<int64_t>(num_groups) * q / 8),
+                   static_cast<int>(static_cast<int64_t>(num_groups) * (q + 1) / 8));
 }
 
 // Group MAC, fp32 without contraction (identical expression in the oracle).

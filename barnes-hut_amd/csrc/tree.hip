@@ -909,6 +909,8 @@ struct WalkArgs {
   float* adj;    // 4 floats per node: dL/dX (3), dL/dM
   float* adjq;   // 8 floats per node: dL/dS (6 used)
   unsigned long long* stats;
+  const int* order;  // claim order of the groups (k_group_order)
+  float* gcost;      // measured cost of each group, for the next walk's order
 };
 
 struct Lane {
@@ -1311,6 +1313,63 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   }
 }
 
+// Claim order of the next walk: within each eighth of the groups (one block
+// each), heaviest first by the costs the previous walk over this workspace
+// measured, when the tag says it walked as many groups; otherwise the
+// groups' own order.  The persistent waves drain the queue unevenly at the
+// end; dealing the cheap groups last shortens that tail.  Costs are binned
+// (256 bins, heaviest first) and the order inside a bin is that of the
+// atomics.  Only the schedule depends on the costs, never the results.
+constexpr int kOrderBins = 256;
+__global__ __launch_bounds__(1024) void k_group_order(const Header* __restrict__ hdr,
+                                                      const float* __restrict__ gcost,
+                                                      int* __restrict__ gorder, Sched* sched) {
+  __shared__ float s_max[16];
+  __shared__ int s_bin[kOrderBins];
+  const int ng = hdr->num_groups;
+  const int2 r = gw::eighth_range(ng, static_cast<int>(blockIdx.x));
+  // block 0 rewrites the tag below; a block that reads the new tag early
+  // orders by stale costs, which changes only the schedule
+  const bool valid = sched->magic == kSchedMagic && sched->groups == ng;
+  if (!valid) {
+    for (int t = r.x + static_cast<int>(threadIdx.x); t < r.y; t += 1024) gorder[t] = t;
+  } else {
+    auto clean = [](float c) { return c > 0.0f && c < 3.0e38f ? c : 0.0f; };  // NaN -> 0
+    float m = 0.0f;
+    for (int t = r.x + static_cast<int>(threadIdx.x); t < r.y; t += 1024)
+      m = fmaxf(m, clean(gcost[t]));
+    m = gw::wave_max(m);
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    for (int b = threadIdx.x; b < kOrderBins; b += 1024) s_bin[b] = 0;
+    __syncthreads();
+    m = 0.0f;
+    for (int k = 0; k < 16; ++k) m = fmaxf(m, s_max[k]);
+    const float scale = m > 0.0f ? (kOrderBins - 0.5f) / m : 0.0f;
+    auto bin_of = [&](int t) {
+      const int b = static_cast<int>(clean(gcost[t]) * scale);
+      return kOrderBins - 1 - min(max(b, 0), kOrderBins - 1);  // heaviest in bin 0
+    };
+    for (int t = r.x + static_cast<int>(threadIdx.x); t < r.y; t += 1024)
+      atomicAdd(&s_bin[bin_of(t)], 1);
+    __syncthreads();
+    if (threadIdx.x == 0) {  // exclusive scan of the bins (256 entries)
+      int run = r.x;
+      for (int b = 0; b < kOrderBins; ++b) {
+        const int c = s_bin[b];
+        s_bin[b] = run;
+        run += c;
+      }
+    }
+    __syncthreads();
+    for (int t = r.x + static_cast<int>(threadIdx.x); t < r.y; t += 1024)
+      gorder[atomicAdd(&s_bin[bin_of(t)], 1)] = t;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sched->magic = kSchedMagic;  // the walk after this order measures every group
+    sched->groups = ng;
+  }
+}
+
 // Persistent walk: each wave takes query groups from a device work queue
 // until none are left (heavy groups do not hold up a static schedule).
 // Registers are capped at 128 VGPRs (4 waves per SIMD): the adjoint variant
@@ -1318,7 +1377,7 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 // the interaction batches and is ~4% faster.  WPB waves per block (the
 // block size is a tuning dimension; every wave walks on its own).
 template <bool VJP, int ORDER, bool FORCES, int WPB>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) void k_walk(
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(VJP ? 4 : 7))) void k_walk(
     WalkArgs A) {
   __shared__ int s_stack[WPB][kStackCap];
   __shared__ int s_far[WPB][kFarCap];
@@ -1331,14 +1390,17 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(4))) v
   unsigned long long n_far = 0, n_near = 0, n_groups = 0;
   int tries = 0;
   while (true) {
-    const int g = next_group(A.hdr, num_groups, lane, tries);
+    const int g = next_group(A.hdr, num_groups, lane, tries, A.order);
     if (g < 0) break;  // every wave reaches this exit
     const int2 grp = A.groups[g];
     const int base = __builtin_amdgcn_readfirstlane(grp.x);
     const int cnt = __builtin_amdgcn_readfirstlane(grp.y);
     if (cnt < 1 || cnt > 64 || base < 0 || base + cnt > A.n) continue;  // corrupt entry
+    const uint64_t t0 = __builtin_amdgcn_s_memtime();
     walk_group<VJP, ORDER, FORCES>(A, base, cnt, lane, s_stack[w], s_far[w], s_near[w],
                                    s_red[w], n_far, n_near);
+    const uint64_t t1 = __builtin_amdgcn_s_memtime();
+    if (lane == 0) A.gcost[g] = static_cast<float>(t1 - t0);
     ++n_groups;
   }
   if (A.stats != nullptr && lane == 0) {
@@ -1490,6 +1552,9 @@ Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   L.qgrad = o; o = align(o + 16 * nn);
   L.sgrad = o; o = align(o + 16 * nn);
   L.groups = o; o = align(o + 8 * nn);
+  L.gcost = o; o = align(o + 4 * nn);
+  L.gorder = o; o = align(o + 4 * nn);
+  L.sched = o; o = align(o + sizeof(Sched));
   L.tiles = o;
   o = align(o + 4 * static_cast<size_t>(kTopoCols) * ((nn + kTopoTile - 1) / kTopoTile));
   L.geoms = o; o = align(o + sizeof(CubeGeom) * static_cast<size_t>(nsets));
@@ -1692,6 +1757,11 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   A.adj = reinterpret_cast<float*>(ws + L.adj);
   A.adjq = reinterpret_cast<float*>(ws + L.adjq);
   A.stats = want_stats ? hdr->stats : nullptr;
+  A.order = reinterpret_cast<const int*>(ws + L.gorder);
+  A.gcost = reinterpret_cast<float*>(ws + L.gcost);
+  k_group_order<<<8, 1024, 0, s>>>(hdr, A.gcost, reinterpret_cast<int*>(ws + L.gorder),
+                                   reinterpret_cast<Sched*>(ws + L.sched));
+  SBH_LAUNCH_CHECK();
   if (vjp) {
     k_zero_adjoint<<<grid_for(4 * n + 12 * L.cap, kB, 8192), kB, 0, s>>>(hdr, n, A.sgrad, A.adj,
                                                                         A.adjq);

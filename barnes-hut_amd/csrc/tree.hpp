@@ -30,6 +30,15 @@ struct Header {
   unsigned long long stats[8];  // far interactions, near pairs, groups
 };
 
+// Tag of the walk schedule: the group costs in the workspace were measured
+// by a walk over `groups` groups (the tag is checked, never trusted: any
+// cost values give a valid claim order).
+constexpr uint32_t kSchedMagic = 0x5c4ed01eu;
+struct Sched {
+  uint32_t magic;
+  int32_t groups;
+};
+
 constexpr int kGroupMax = 64;     // one wavefront per query group
 constexpr int kGroupCell = 2048;  // groups are runs inside cells of at most this many points
 // Query trees (queries walked against a separate source tree): topology and
@@ -51,6 +60,7 @@ struct Layout {
   size_t mp, qd;                       // float4 {com, mass}, 2 x float4 second moment
   size_t adj, adjq, qgrad, sgrad;      // adjoint accumulators
   size_t groups, tiles;                // int2 {start, count} per group; per-tile counters
+  size_t gcost, gorder, sched;         // walk schedule: cost per group, claim order, its tag
   size_t total;
 };
 
