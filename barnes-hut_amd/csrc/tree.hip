@@ -1314,24 +1314,29 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
 }
 
 // Claim order of the next walk: within each eighth of the groups (one block
-// each), heaviest first by the costs the previous walk over this workspace
-// measured, when the tag says it walked as many groups; otherwise the
-// groups' own order.  The persistent waves drain the queue unevenly at the
-// end; dealing the cheap groups last shortens that tail.  Costs are binned
-// (256 bins, heaviest first) and the order inside a bin is that of the
-// atomics.  Only the schedule depends on the costs, never the results.
+// each), heaviest first by the costs that the previous walk of the same kind
+// (forward or adjoint) over this workspace measured, when its tag says it
+// walked as many groups; else by the other kind's costs under the same test;
+// else the groups' own order.  The persistent waves drain the queue unevenly
+// at the end, and dealing the cheap groups last shortens that tail.  Costs
+// are binned (256 bins, heaviest first) and the order inside a bin is that
+// of the atomics.  Only the schedule depends on the costs, never a result.
 constexpr int kOrderBins = 256;
 __global__ __launch_bounds__(1024) void k_group_order(const Header* __restrict__ hdr,
-                                                      const float* __restrict__ gcost,
-                                                      int* __restrict__ gorder, Sched* sched) {
+                                                      const float* __restrict__ cost_own,
+                                                      const float* __restrict__ cost_other,
+                                                      int* __restrict__ gorder, Sched* sched_own,
+                                                      const Sched* sched_other) {
   __shared__ float s_max[16];
   __shared__ int s_bin[kOrderBins];
   const int ng = hdr->num_groups;
   const int2 r = gw::eighth_range(ng, static_cast<int>(blockIdx.x));
-  // block 0 rewrites the tag below; a block that reads the new tag early
+  // block 0 rewrites the own tag below; a block that reads the new tag early
   // orders by stale costs, which changes only the schedule
-  const bool valid = sched->magic == kSchedMagic && sched->groups == ng;
-  if (!valid) {
+  const bool own = sched_own->magic == kSchedMagic && sched_own->groups == ng;
+  const bool other = sched_other->magic == kSchedMagic && sched_other->groups == ng;
+  const float* gcost = own ? cost_own : cost_other;
+  if (!own && !other) {
     for (int t = r.x + static_cast<int>(threadIdx.x); t < r.y; t += 1024) gorder[t] = t;
   } else {
     auto clean = [](float c) { return c > 0.0f && c < 3.0e38f ? c : 0.0f; };  // NaN -> 0
@@ -1365,8 +1370,8 @@ __global__ __launch_bounds__(1024) void k_group_order(const Header* __restrict__
       gorder[atomicAdd(&s_bin[bin_of(t)], 1)] = t;
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    sched->magic = kSchedMagic;  // the walk after this order measures every group
-    sched->groups = ng;
+    sched_own->magic = kSchedMagic;  // the walk after this order measures every group
+    sched_own->groups = ng;
   }
 }
 
@@ -1552,9 +1557,9 @@ Layout layout(int64_t n, int max_depth, int leaf_size, int nsets) {
   L.qgrad = o; o = align(o + 16 * nn);
   L.sgrad = o; o = align(o + 16 * nn);
   L.groups = o; o = align(o + 8 * nn);
-  L.gcost = o; o = align(o + 4 * nn);
+  L.gcost = o; o = align(o + 8 * nn);  // forward, then adjoint walks
   L.gorder = o; o = align(o + 4 * nn);
-  L.sched = o; o = align(o + sizeof(Sched));
+  L.sched = o; o = align(o + 2 * sizeof(Sched));
   L.tiles = o;
   o = align(o + 4 * static_cast<size_t>(kTopoCols) * ((nn + kTopoTile - 1) / kTopoTile));
   L.geoms = o; o = align(o + sizeof(CubeGeom) * static_cast<size_t>(nsets));
@@ -1757,10 +1762,16 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   A.adj = reinterpret_cast<float*>(ws + L.adj);
   A.adjq = reinterpret_cast<float*>(ws + L.adjq);
   A.stats = want_stats ? hdr->stats : nullptr;
-  A.order = reinterpret_cast<const int*>(ws + L.gorder);
-  A.gcost = reinterpret_cast<float*>(ws + L.gcost);
-  k_group_order<<<8, 1024, 0, s>>>(hdr, A.gcost, reinterpret_cast<int*>(ws + L.gorder),
-                                   reinterpret_cast<Sched*>(ws + L.sched));
+  {
+    auto* costs = reinterpret_cast<float*>(ws + L.gcost);
+    auto* tags = reinterpret_cast<Sched*>(ws + L.sched);
+    const int64_t nn = n > 0 ? n : 1, own = vjp ? 1 : 0;
+    A.order = reinterpret_cast<const int*>(ws + L.gorder);
+    A.gcost = costs + own * nn;
+    k_group_order<<<8, 1024, 0, s>>>(hdr, A.gcost, costs + (1 - own) * nn,
+                                     reinterpret_cast<int*>(ws + L.gorder), tags + own,
+                                     tags + (1 - own));
+  }
   SBH_LAUNCH_CHECK();
   if (vjp) {
     k_zero_adjoint<<<grid_for(4 * n + 12 * L.cap, kB, 8192), kB, 0, s>>>(hdr, n, A.sgrad, A.adj,

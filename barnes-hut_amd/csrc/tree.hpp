@@ -60,7 +60,7 @@ struct Layout {
   size_t mp, qd;                       // float4 {com, mass}, 2 x float4 second moment
   size_t adj, adjq, qgrad, sgrad;      // adjoint accumulators
   size_t groups, tiles;                // int2 {start, count} per group; per-tile counters
-  size_t gcost, gorder, sched;         // walk schedule: cost per group, claim order, its tag
+  size_t gcost, gorder, sched;         // walk schedule: group costs (forward, adjoint), order, tags
   size_t total;
 };
 
