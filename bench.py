@@ -39,13 +39,13 @@ FP32_PEAK_NON_PACKED_TFLOPS = 78.6  # the same VALUs without packed-FP32 instruc
 # counts as one source whatever its expansion order.
 FLOPS_ALG_FWD, FLOPS_ALG_VJP = 20, 48
 # The implementation's own count: near pairs as above; a far node costs
-# 20 / 48 with the monopole, 54 / 142 with the quadrupole terms and 76 in the
+# 20 / 48 with the monopole, 54 / 142 with the quadrupole terms and 68 in the
 # adjoint with monopole terms plus the second-moment adjoint (order 3): the
 # FP32 operations, FMA = 2, that the walks' SQ_INSTS_VALU_FLOPS_FP32 counter
 # reports per far node (profiles/gravity_walk_plummer_n1048576_valu_counters.md).
 FLOPS_NEAR_FWD, FLOPS_NEAR_VJP = 20, 48
 FLOPS_FAR_FWD = {1: 20, 2: 54}
-FLOPS_FAR_VJP = {1: 48, 2: 142, 3: 76}
+FLOPS_FAR_VJP = {1: 48, 2: 142, 3: 68}
 OUTLIER_REL_THRESHOLD = 0.05  # per-row relative error counted as an outlier
 
 
@@ -74,7 +74,7 @@ def valu_roofline(kernel, flops_alg, flops_impl, ms, traffic=None, **extra):
            "frac_of_non_packed_peak": a / FP32_PEAK_NON_PACKED_TFLOPS,
            "implementation_count": {
                "flop_model": "near pair 20 / 48, far node 20 / 48 (monopole), 54 / 142 "
-                             "(quadrupole), - / 76 (second-moment adjoint)",
+                             "(quadrupole), - / 68 (second-moment adjoint)",
                "achieved": b, "frac": b / FP32_PEAK_TFLOPS,
                "frac_of_non_packed_peak": b / FP32_PEAK_NON_PACKED_TFLOPS,
                "flops_per_launch": flops_impl}}
