@@ -617,8 +617,8 @@ int sbh_stochastic_winding_number(void* stream, const SbhBuffer* points,
  * sbh_gravity_noalloc_scratch_bytes() bytes (64 MiB) per device, held until
  * the process ends.  No call allocates or frees device memory.  Problems whose
  * sbh_gravity_workspace_bytes(n, max_depth, leaf_size) exceeds that size are
- * rejected with SBH_INVALID_ARGUMENT (at max_depth 10: n <= 230,999 at leaf
- * size 32, n <= 143,020 at leaf size 16).
+ * rejected with SBH_INVALID_ARGUMENT (at max_depth 10: n <= 224,912 at leaf
+ * size 32, n <= 143,008 at leaf size 16).
  * Every call on one device shares that device's array: calls that overlap
  * on one device (different streams or host threads) race; calls queued on
  * one stream are ordered.  Results equal sbh_gravity_force_vjp's. */
