@@ -1734,6 +1734,20 @@ void launch_walk(hipStream_t s, unsigned grid, bool vjp, int order, bool f, cons
 }
 }  // namespace
 
+int schedule_groups(hipStream_t s, char* ws, const Layout& L, int64_t n, int kind,
+                    const int** order, float** cost) {
+  auto* costs = reinterpret_cast<float*>(ws + L.gcost);
+  auto* tags = reinterpret_cast<Sched*>(ws + L.sched);
+  const int64_t nn = n > 0 ? n : 1, own = kind != 0 ? 1 : 0;
+  *order = reinterpret_cast<const int*>(ws + L.gorder);
+  *cost = costs + own * nn;
+  k_group_order<<<8, 1024, 0, s>>>(reinterpret_cast<const Header*>(ws + L.header), *cost,
+                                   costs + (1 - own) * nn, reinterpret_cast<int*>(ws + L.gorder),
+                                   tags + own, tags + (1 - own));
+  SBH_LAUNCH_CHECK();
+  return SBH_OK;
+}
+
 int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float softening,
          bool vjp, int order, float* forces, float* grad_points, float* grad_masses,
          bool want_stats, hipEvent_t* ev_walk, int walk_blocks, int walk_threads) {
@@ -1763,14 +1777,12 @@ int walk(hipStream_t s, char* ws, const Layout& L, int64_t n, float beta, float 
   A.adjq = reinterpret_cast<float*>(ws + L.adjq);
   A.stats = want_stats ? hdr->stats : nullptr;
   {
-    auto* costs = reinterpret_cast<float*>(ws + L.gcost);
-    auto* tags = reinterpret_cast<Sched*>(ws + L.sched);
-    const int64_t nn = n > 0 ? n : 1, own = vjp ? 1 : 0;
-    A.order = reinterpret_cast<const int*>(ws + L.gorder);
-    A.gcost = costs + own * nn;
-    k_group_order<<<8, 1024, 0, s>>>(hdr, A.gcost, costs + (1 - own) * nn,
-                                     reinterpret_cast<int*>(ws + L.gorder), tags + own,
-                                     tags + (1 - own));
+    const int* order = nullptr;
+    float* cost = nullptr;
+    const int rc = schedule_groups(s, ws, L, n, vjp ? 1 : 0, &order, &cost);
+    if (rc != SBH_OK) return rc;
+    A.order = order;
+    A.gcost = cost;
   }
   SBH_LAUNCH_CHECK();
   if (vjp) {

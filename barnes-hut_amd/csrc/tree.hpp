@@ -81,6 +81,15 @@ __device__ __forceinline__ int set_of(const int32_t* __restrict__ offs, int nset
   return lo;
 }
 
+// Claim order of the next walk over the query groups of the tree in ws
+// (n points): the groups' own order, or heaviest first within each eighth by
+// the costs the previous walk of the same kind (0: forward-type walks, 1:
+// adjoint walks; else the other kind) measured in this workspace.  *order is
+// the permutation for gw::next_group, *cost where the walk stores each
+// group's measured cost.  Only the schedule depends on them.
+int schedule_groups(hipStream_t s, char* ws, const Layout& L, int64_t n, int kind,
+                    const int** order, float** cost);
+
 // Data bounds of n points on the device (no host synchronisation): min - 1e-3
 // (3 floats) then max + 1e-3 (3 floats), written to tmp + 6 * kBboxBlocks and
 // returned in *bounds; tmp holds 6 * (kBboxBlocks + 1) floats.
