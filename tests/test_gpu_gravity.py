@@ -240,6 +240,32 @@ def test_determinism_of_forces(gpu):
     np.testing.assert_array_equal(a, b)
 
 
+def test_walk_schedule_changes_no_result(gpu):
+    # A fresh (zeroed) workspace walks the groups in their own order; the next
+    # walks on it are ordered by the costs the previous ones measured.  Every
+    # order must walk each group once: same interaction counts, bitwise equal
+    # forces, and gradients equal to atomics' rounding.
+    G = _G()
+    n = 60000
+    p, m, c = _case("plummer", n, 21)
+    P, M, C = (torch.tensor(a, device="cuda") for a in (p, m, c))
+    kw = dict(softening_scale=0.005, max_depth=10, leaf_size=32, multipole_order=1)
+    ws = torch.zeros(G.workspace_bytes(n, 10, 32), dtype=torch.uint8, device="cuda")
+    runs = []
+    for _ in range(3):
+        sf, sv = G.new_stats(), G.new_stats()
+        f = G.gravity_force(P, M, workspace=ws, stats=sf, **kw).clone()
+        gx, gm = (t.clone() for t in G.gravity_force_vjp(P, M, C, workspace=ws, stats=sv,
+                                                          **dict(kw, multipole_order=3)))
+        runs.append((_np(f), _np(gx), _np(gm), G.stats_dict(sf), G.stats_dict(sv)))
+    f0, gx0, gm0, sf0, sv0 = runs[0]
+    assert sf0["groups"] > 8 and sf0["far"] > 0 and sf0["near"] > 0
+    for f, gx, gm, sf, sv in runs[1:]:
+        np.testing.assert_array_equal(f, f0)
+        assert sf == sf0 and sv == sv0
+        assert O.rel_l2(gx, gx0) < 1e-5 and O.rel_l2(gm, gm0) < 1e-5
+
+
 def test_graph_replay_single_set_matches_eager(gpu):
     # one set through the single-set entry points, captured once and replayed
     # with unrelated eager work in between (the bench's timed step)
