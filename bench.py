@@ -82,21 +82,27 @@ def valu_roofline(kernel, flops_alg, flops_impl, ms, traffic=None, **extra):
     return rec
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles",
-                               "gravity_walk_plummer_n1048576_quad_hbm_traffic.md")
+# HBM bytes per launch of the 2^20 Plummer walks from rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes: the headline orders, then the quadrupole walks
+TRAFFIC_PROFILES = [os.path.join(ROOT, "profiles", f) for f in (
+    "gravity_walk_plummer_n1048576_hbm_traffic.md",
+    "gravity_walk_plummer_n1048576_quad_hbm_traffic.md")]
 
 
-def profiled_traffic(kernel_key):
-    """HBM bytes per launch of `kernel_key` measured by the two rocprofv3
-    --pmc passes recorded in profiles/ (None if not profiled)."""
-    try:
-        with open(TRAFFIC_PROFILE) as fh:
-            for line in fh:
-                if line.startswith(f"traffic_bytes_per_launch {kernel_key}:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+def profiled_traffic(vjp, order, forces):
+    """HBM bytes per launch of k_walk<vjp, order, forces> at N = 2^20 measured
+    by the rocprofv3 --pmc passes recorded in profiles/, and that file (None,
+    None if not profiled)."""
+    key = f"k_walk<{str(vjp).lower()},{order},{str(forces).lower()}>"
+    for path in TRAFFIC_PROFILES:
+        try:
+            with open(path) as fh:
+                for line in fh:
+                    if line.startswith(f"traffic_bytes_per_launch {key}:"):
+                        return float(line.split(":", 1)[1]), "profiles/" + os.path.basename(path)
+        except OSError:
+            pass
+    return None, None
 
 
 def direct_fp64(P, M, Q, eps, chunk=16):
@@ -567,6 +573,7 @@ def bench_gravity(args):
         cpu_np = cpu_numpy_direct(p, m, args)
         cpu_lit = cpu_reference_literal(p, m, args)
 
+    vjp_traffic = profiled_traffic(True, args.order, False) if n == 1 << 20 else (None, None)
     rec = {
         "metric": METRIC,
         "value": value,
@@ -604,9 +611,8 @@ def bench_gravity(args):
         "roofline": valu_roofline(
             f"k_walk<vjp=1,order={args.order},forces=0>", flops_vjp, flops_vjp_impl,
             walk_vjp_ms,
-            traffic=(profiled_traffic("k_walk<true,true,false>")
-                     if args.order == 2 and n == 1 << 20 else None),
-            traffic_source="profiles/" + os.path.basename(TRAFFIC_PROFILE),
+            traffic=vjp_traffic[0],
+            traffic_source=vjp_traffic[1],
             timing="library HIP events around the walk launch on its stream, mean of the "
                    "timed eager steps"),
         "other_rooflines": other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl,
@@ -660,20 +666,19 @@ def other_rooflines(args, n, sf, walk_fwd_ms, flops_fwd, flops_fwd_impl, prof, d
                             "look-back set the time, not bandwidth"}
     out["walk_fwd"] = valu_roofline(f"k_walk<vjp=0,order={args.fwd_order},forces=1>",
                                     flops_fwd, flops_fwd_impl, walk_fwd_ms,
-                                    traffic=(profiled_traffic("k_walk<false,true,true>")
-                                             if args.fwd_order == 2 and n == 1 << 20
-                                             else None))
+                                    traffic=(profiled_traffic(False, args.fwd_order, True)[0]
+                                             if n == 1 << 20 else None))
     # walks against HBM: profiled traffic over the kernel time, beside the
     # minimal bytes (40 B/pt + 48 B/node; the adjoint +36 B/pt)
-    for tag, key, ms, extra, order in (
-            ("walk_fwd_hbm", "k_walk<false,true,true>", walk_fwd_ms, 0, args.fwd_order),
-            ("walk_vjp_hbm", "k_walk<true,true,false>", vjp_walk_ms, 36, args.order)):
-        traffic = profiled_traffic(key)
-        if traffic is None or order != 2 or n != 1 << 20:
+    for tag, vjp, ms, extra, order in (
+            ("walk_fwd_hbm", False, walk_fwd_ms, 0, args.fwd_order),
+            ("walk_vjp_hbm", True, vjp_walk_ms, 36, args.order)):
+        traffic, source = profiled_traffic(vjp, order, not vjp)
+        if traffic is None or n != 1 << 20:
             continue
         gbs = traffic / (ms * 1e-3) / 1e9
         out[tag] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": gbs / HBM_PEAK_GBS, "traffic": traffic,
+                    "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "traffic_source": source,
                     "minimal_bytes": (40 + extra) * n + 48 * sf["nodes"]}
     if direct is not None:
         t = direct["direct_fwd_tflops"]
