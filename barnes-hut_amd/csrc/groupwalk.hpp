@@ -63,12 +63,25 @@ __device__ __forceinline__ float wave_max(float v) {
   for (int d = 32; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
   return v;
 }
-__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
-  for (int d = 1; d < 64; d <<= 1) {
-    int t = __shfl_up(v, d, 64);
-    if (lane >= d) v += t;
-  }
+// Inclusive prefix sum over the 64 lanes by DPP moves (no LDS round trips):
+// Hillis-Steele inside each row of 16 (row_shr 1, 2, 4, 8; lanes whose
+// source falls outside the row add 0), then the last lane of row 0 added to
+// rows 1 and 3 (row_bcast:15) and the last lane of row 1 to rows 2 and 3
+// (row_bcast:31).  All lanes must be active.
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
+  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
+  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
+  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
+  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
   return v;
+}
+
+// Number of lanes below this one whose bit is set in the wave mask m.
+__device__ __forceinline__ int lanes_below(uint64_t m) {
+  return static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+      static_cast<uint32_t>(m >> 32), __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u)));
 }
 
 // XCD-aware work queue of a persistent walk.  The groups (in Morton order)
@@ -145,7 +158,6 @@ __device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const
                                          const Box& b, int lane,
                                          int root, int* stack, int* far, int* near,
                                          FarFn&& flush_far, NearFn&& flush_near) {
-  const uint64_t lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   int top = 1, nfar = 0, nnear = 0;
   if (lane == 0) stack[0] = root;
   __builtin_amdgcn_wave_barrier();
@@ -178,13 +190,13 @@ __device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const
       }
     }
     const uint64_t bf = __ballot(kind == 1);
-    if (kind == 1) far[nfar + __popcll(bf & lt_mask)] = node;
+    if (kind == 1) far[nfar + lanes_below(bf)] = node;
     nfar += __popcll(bf);
     const uint64_t bn = __ballot(kind == 2);
-    if (kind == 2) near[nnear + __popcll(bn & lt_mask)] = node;
+    if (kind == 2) near[nnear + lanes_below(bn)] = node;
     nnear += __popcll(bn);
-    const int incl = wave_incl_scan(nch, lane);
-    const int total = __shfl(incl, 63, 64);
+    const int incl = wave_incl_scan(nch);
+    const int total = __builtin_amdgcn_readlane(incl, 63);
     if (top + total > kStackCap) {  // unreachable for depth <= 10 (see kStackCap)
       if (lane == 0) atomicOr(overflow, 4);
       break;
