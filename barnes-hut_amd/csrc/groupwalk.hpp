@@ -67,15 +67,26 @@ __device__ __forceinline__ float wave_max(float v) {
 // Hillis-Steele inside each row of 16 (row_shr 1, 2, 4, 8; lanes whose
 // source falls outside the row add 0), then the last lane of row 0 added to
 // rows 1 and 3 (row_bcast:15) and the last lane of row 1 to rows 2 and 3
-// (row_bcast:31).  All lanes must be active.
+// (row_bcast:31).  The sums run in fp32, where the compiler fuses each move
+// into its add; they are exact for non-negative totals below 2^24 (the
+// traversal's child counts sum to at most 512).  All lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ float dpp_shr(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dpp_bcast(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, ROWS, 0xf, false));
+}
 __device__ __forceinline__ int wave_incl_scan(int v) {
-  v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);   // row_shr:1
-  v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);   // row_shr:2
-  v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);   // row_shr:4
-  v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);   // row_shr:8
-  v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15
-  v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31
-  return v;
+  float x = static_cast<float>(v);
+  x += dpp_shr<0x111>(x);        // row_shr:1
+  x += dpp_shr<0x112>(x);        // row_shr:2
+  x += dpp_shr<0x114>(x);        // row_shr:4
+  x += dpp_shr<0x118>(x);        // row_shr:8
+  x += dpp_bcast<0x142, 0xa>(x);  // row_bcast:15 into rows 1 and 3
+  x += dpp_bcast<0x143, 0xc>(x);  // row_bcast:31 into rows 2 and 3
+  return static_cast<int>(x);
 }
 
 // Number of lanes below this one whose bit is set in the wave mask m.
@@ -201,7 +212,14 @@ __device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const
       if (lane == 0) atomicOr(overflow, 4);
       break;
     }
-    for (int c = 0; c < nch; ++c) stack[top + incl - nch + c] = fch + c;
+    // children fch .. fch + nch - 1, two per store (ds_write2)
+    {
+      int* dst = stack + (top + incl - nch);
+      for (int c = 0; c < nch; c += 2) {
+        dst[c] = fch + c;
+        if (c + 1 < nch) dst[c + 1] = fch + c + 1;
+      }
+    }
     top += total;
     __builtin_amdgcn_wave_barrier();
     if (nfar >= 64) {
