@@ -96,18 +96,22 @@ __device__ __forceinline__ float wn_far_batch(const WnArgs& A, float qx, float q
   return w;
 }
 
-// kChunk consecutive elements of one leaf from j0 (entries at or beyond s1
-// carry zero area).
+// kChunk consecutive elements of one leaf from j0 (with PAD, entries at or
+// beyond s1 carry zero area; without, the batch lies inside the range and
+// reads its records as runs, which the scalar loads fetch 4 at a time).
+template <bool PAD>
 __device__ __forceinline__ float wn_near_batch(const WnArgs& A, float qx, float qy, float qz,
                                                int j0, int s1) {
   float w = 0.f;
+  cfloat* rp = run_base(A.sp, j0);
+  cfloat* rn = run_base(A.sn, j0);
 #pragma unroll
   for (int u = 0; u < kChunk; ++u) {
     const int j = j0 + u;
     const int jj = min(j, s1 - 1);
-    const float4 p = sload4(A.sp, jj);
-    const float4 nv = sload4(A.sn, jj);
-    const float a = (j < s1) ? p.w : 0.0f;
+    const float4 p = PAD ? sload4(A.sp, jj) : sload4_run(rp, u);
+    const float4 nv = PAD ? sload4(A.sn, jj) : sload4_run(rn, u);
+    const float a = (!PAD || j < s1) ? p.w : 0.0f;
     const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float r = sqrt_hw(r2);
@@ -164,7 +168,10 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
         }
         s1 = __builtin_amdgcn_readfirstlane(s1);  // wave-uniform: keeps element loads scalar
         n_near += static_cast<unsigned long long>(s1 - s0) * cnt;
-        for (int j0 = s0; j0 < s1; j0 += kChunk) w += wn_near_batch(A, q.x, q.y, q.z, j0, s1);
+        int j0 = s0;
+        for (; j0 + kChunk <= s1; j0 += kChunk)
+          w += wn_near_batch<false>(A, q.x, q.y, q.z, j0, s1);
+        if (j0 < s1) w += wn_near_batch<true>(A, q.x, q.y, q.z, j0, s1);
       }
     };
     traverse(A.hdr, &A.qhdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack,
