@@ -1219,8 +1219,8 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   float fx = 0.f, fy = 0.f, fz = 0.f;
   float gx = 0.f, gy = 0.f, gz = 0.f;
 
-  // The quadrupole and second-moment adjoints hold 12 partials per node:
-  // batch 4 nodes, not 8.
+  // The quadrupole and second-moment adjoints hold 10 partials per node:
+  // batches of 4 nodes, not 8 (8 spill).
   constexpr int C = (VJP && ORDER != 1) ? kChunk / 2 : kChunk;
   // The monopole forward walk reads 16 nodes / sources per batch while the
   // loads of a batch are in flight together (its registers allow it); the
@@ -1377,10 +1377,11 @@ __global__ __launch_bounds__(1024) void k_group_order(const Header* __restrict__
 
 // Persistent walk: each wave takes query groups from a device work queue
 // until none are left (heavy groups do not hold up a static schedule).
-// Registers are capped at 128 VGPRs (4 waves per SIMD): the adjoint variant
-// would take ~143 and drop to 3 waves; the cap costs a few spills outside
-// the interaction batches and is ~4% faster.  WPB waves per block (the
-// block size is a tuning dimension; every wave walks on its own).
+// Occupancy targets: the adjoint walks at 4 waves per SIMD (at most 128
+// VGPRs; their 40 KB of LDS per 4-wave block allow no more), the forward
+// walks at 7 (at most 72 VGPRs; 22 KB of LDS per block, 7 blocks per CU;
+// 6 waves were 5% slower).  WPB waves per block (the block size is a tuning
+// dimension; every wave walks on its own).
 template <bool VJP, int ORDER, bool FORCES, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(VJP ? 4 : 7))) void k_walk(
     WalkArgs A) {
