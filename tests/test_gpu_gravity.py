@@ -264,6 +264,16 @@ def test_walk_schedule_changes_no_result(gpu):
         np.testing.assert_array_equal(f, f0)
         assert sf == sf0 and sv == sv0
         assert O.rel_l2(gx, gx0) < 1e-5 and O.rel_l2(gm, gm0) < 1e-5
+    # a workspace full of random bytes (tags, costs) and one whose last walk
+    # had another group count: the order is still a permutation of the groups
+    junk = torch.randint(0, 256, ws.shape, dtype=torch.uint8, device="cuda")
+    sf = G.new_stats()
+    np.testing.assert_array_equal(_np(G.gravity_force(P, M, workspace=junk, stats=sf, **kw)), f0)
+    assert G.stats_dict(sf) == sf0
+    G.gravity_force(P[: n // 3], M[: n // 3], workspace=ws, **kw)
+    sf = G.new_stats()
+    np.testing.assert_array_equal(_np(G.gravity_force(P, M, workspace=ws, stats=sf, **kw)), f0)
+    assert G.stats_dict(sf) == sf0
 
 
 def test_graph_replay_single_set_matches_eager(gpu):
