@@ -19,6 +19,18 @@ namespace gw {
 
 constexpr int kB = 256;
 constexpr int kWavesPerBlock = kB / 64;
+// Run stack (traverse<true>): one entry per opened node, holding the run of
+// its children.  An iteration pops whole entries from the top (at most 64
+// nodes) and pushes at most 64 entries, each one level below the node that
+// made it.  The entries left from one push lie below every later push, and
+// each push is at least one level deeper than the one beneath it, so a tree
+// of depth <= 10 holds at most 11 pushes of 64 entries (704) on the stack.
+constexpr int kRunStackCap = 768;
+// first-child words, the wave's 64 spread slots, then one count byte per entry
+constexpr int kRunStackWords = kRunStackCap + 64 + kRunStackCap / 4;
+// Node stack (traverse<false>): one entry per node; a pop takes at most
+// (kStackSoft - top) / 7 nodes near the soft limit, so a push of <= 8 children
+// each stays under kStackCap.
 constexpr int kStackSoft = 1024;
 constexpr int kStackCap = kStackSoft + 8 * 24;
 constexpr int kFarCap = 128;
@@ -160,28 +172,66 @@ __device__ __forceinline__ Box wave_box(float x, float y, float z) {
 // end); near[0..nnear) holds rejected non-empty leaves when
 // flush_near(nnear) is called (at kNearFlush entries and at the end).  Both
 // callbacks run on the whole wave.  A traversal whose stack would exceed
-// kStackCap stops early and sets bit 4 of *overflow (a header's overflow
+// its capacity stops early and sets bit 4 of *overflow (a header's overflow
 // word, reported by the walk's stats), so a truncated sum never passes
-// unnoticed.
-template <class FarFn, class NearFn>
+// unnoticed.  RUNS selects the run stack (kRunStackWords ints per wave) or the
+// node stack (kStackCap ints per wave); both visit the same nodes.
+template <bool RUNS, class FarFn, class NearFn>
 __device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const int4* topo,
                                          const float4* geo, const float4* mp, float beta,
                                          const Box& b, int lane,
                                          int root, int* stack, int* far, int* near,
                                          FarFn&& flush_far, NearFn&& flush_near) {
+  // run stack entry e: nodes stack[e] .. stack[e] + count[e] - 1 (count 1 to
+  // 8, one byte); spread[l] marks the lane where a popped entry's first node
+  // lands
+  int* spread = stack + kRunStackCap;
+  unsigned char* count = reinterpret_cast<unsigned char*>(spread + 64);
+  const int cap = RUNS ? kRunStackCap : kStackCap;
   int top = 1, nfar = 0, nnear = 0;
-  if (lane == 0) stack[0] = root;
+  if (lane == 0) {
+    stack[0] = root;
+    if (RUNS) count[0] = 1;
+  }
+  if (RUNS) spread[lane] = 0;
   __builtin_amdgcn_wave_barrier();
   // Every node is pushed at most once per group and each iteration pops at
   // least one, so num_nodes iterations bound the loop even for a corrupt tree.
   const int num_nodes = hdr->num_nodes;
   int iters = 0;
   while (top > 0 && iters++ <= num_nodes) {
-    int k = min(64, top);
-    k = min(k, max(1, (kStackSoft - top) / 7));
-    const int from = top - k;
-    const int node = lane < k ? stack[from + lane] : -1;
-    top = from;
+    int node;
+    if constexpr (RUNS) {
+      // lane r holds the r-th entry from the top; the whole entries whose
+      // nodes fit in 64 lanes are popped (at least one: an entry holds <= 8)
+      const int ne = min(64, top);
+      const int ent = lane < ne ? stack[top - 1 - lane] : 0;
+      const int cnt = lane < ne ? count[top - 1 - lane] : 0;
+      const int incl = wave_incl_scan(cnt);
+      const int npop = __popcll(__ballot(lane < ne && incl <= 64));
+      if (npop == 0) {  // an entry of more than 64 nodes: not a tree of ours
+        if (lane == 0) atomicOr(overflow, 4);
+        break;
+      }
+      const int nodes = __builtin_amdgcn_readlane(incl, npop - 1);
+      top -= npop;
+      // lane l < nodes takes node l: entry r's nodes occupy lanes
+      // [incl_r - cnt_r, incl_r); mark each popped entry's first lane
+      if (lane < npop) spread[incl - cnt] = lane + 1;
+      __builtin_amdgcn_wave_barrier();
+      const bool first = spread[lane] != 0;
+      spread[lane] = 0;
+      const int r = lanes_below(__ballot(first)) + (first ? 1 : 0) - 1;
+      const int er = __shfl(ent, max(r, 0), 64);
+      const int sr = __shfl(incl - cnt, max(r, 0), 64);
+      node = lane < nodes ? er + (lane - sr) : -1;
+    } else {
+      int k = min(64, top);
+      k = min(k, max(1, (kStackSoft - top) / 7));
+      const int from = top - k;
+      node = lane < k ? stack[from + lane] : -1;
+      top = from;
+    }
     __builtin_amdgcn_wave_barrier();
     const bool have = node >= 0 && node < num_nodes;
     const int nd = have ? node : 0;
@@ -206,15 +256,29 @@ __device__ __forceinline__ void traverse(const Header* hdr, int* overflow, const
     const uint64_t bn = __ballot(kind == 2);
     if (kind == 2) near[nnear + lanes_below(bn)] = node;
     nnear += __popcll(bn);
-    const int incl = wave_incl_scan(nch);
-    const int total = __builtin_amdgcn_readlane(incl, 63);
-    if (top + total > kStackCap) {  // unreachable for depth <= 10 (see kStackCap)
+    // children fch .. fch + nch - 1: one run entry per opened node, or one
+    // entry per child
+    int total, incl = 0;
+    uint64_t bo = 0;
+    if constexpr (RUNS) {
+      bo = __ballot(kind == 3);
+      total = __popcll(bo);
+    } else {
+      incl = wave_incl_scan(nch);
+      total = __builtin_amdgcn_readlane(incl, 63);
+    }
+    if (top + total > cap) {  // unreachable for depth <= 10 (see the capacities)
       if (lane == 0) atomicOr(overflow, 4);
       break;
     }
-    // children fch .. fch + nch - 1, two per store (ds_write2)
-    {
-      int* dst = stack + (top + incl - nch);
+    if constexpr (RUNS) {
+      if (kind == 3) {
+        const int e = top + lanes_below(bo);
+        stack[e] = fch;
+        count[e] = static_cast<unsigned char>(nch);
+      }
+    } else {
+      int* dst = stack + (top + incl - nch);  // two per store (ds_write2)
       for (int c = 0; c < nch; c += 2) {
         dst[c] = fch + c;
         if (c + 1 < nch) dst[c + 1] = fch + c + 1;

@@ -1298,9 +1298,8 @@ __device__ __forceinline__ void walk_group(const WalkArgs& A, int base, int cnt,
   };
 
   const int root = A.nsets > 1 ? set_of(A.offs, A.nsets, base) : 0;
-  traverse(A.hdr, &A.hdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, root, stack, far,
-           near, flush_far,
-           flush_near);
+  traverse<true>(A.hdr, &A.hdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, root, stack,
+                 far, near, flush_far, flush_near);
 
   if (valid) {
     const int64_t o = static_cast<int64_t>(A.perm[i]);
@@ -1385,7 +1384,7 @@ __global__ __launch_bounds__(1024) void k_group_order(const Header* __restrict__
 template <bool VJP, int ORDER, bool FORCES, int WPB>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(VJP ? 4 : 7))) void k_walk(
     WalkArgs A) {
-  __shared__ int s_stack[WPB][kStackCap];
+  __shared__ int s_stack[WPB][kRunStackWords];
   __shared__ int s_far[WPB][kFarCap];
   __shared__ int s_near[WPB][kNearCap];
   // the adjoint's wave sums (the forward walk needs none)

@@ -174,9 +174,8 @@ __global__ __launch_bounds__(kB) void k_wn_walk(WnArgs A) {
         if (j0 < s1) w += wn_near_batch<true>(A, q.x, q.y, q.z, j0, s1);
       }
     };
-    traverse(A.hdr, &A.qhdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, 0, stack,
-             far, near, flush_far,
-             flush_near);
+    traverse<false>(A.hdr, &A.qhdr->overflow, A.topo, A.geo, A.mp, A.beta, box, lane, 0,
+                    stack, far, near, flush_far, flush_near);
     if (valid) A.out[A.qperm[i]] = w * kInvFourPi;
     ++n_groups;
   }

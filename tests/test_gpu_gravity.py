@@ -164,6 +164,26 @@ def test_huge_beta_is_direct_sum(gpu):
     assert O.rel_l2(_np(gx), rgx) < 1e-5 and O.rel_l2(_np(gm), rgm) < 1e-5
 
 
+@pytest.mark.parametrize("kind", ["uniform", "plummer"])
+def test_full_traversal_of_a_deep_tree(gpu, kind):
+    # beta = 1e30 accepts no node, so every group opens every node of a
+    # depth-10 tree with leaves of 1 point: the traversal stack's worst case
+    # (every pop of 64 nodes opens all of them); it must neither overflow
+    # nor change the sum
+    G = _G()
+    n = 1 << 15
+    p, m, c = _case(kind, n, 21)
+    stats = G.new_stats()
+    gx, gm, f = G.gravity_force_vjp(p, m, c, beta=1e30, softening_scale=0.01, max_depth=10,
+                                    leaf_size=1, multipole_order=3, stats=stats,
+                                    return_forces=True)
+    st = G.stats_dict(stats)
+    assert st["overflow"] == 0
+    rows = np.random.default_rng(5).choice(n, 512, replace=False)
+    ref = O.direct_force_soft(p, m, p[rows], 0.01, np.inf)
+    assert O.rel_l2(_np(f)[rows], ref) < 1e-5
+
+
 @pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129])
 def test_small_and_ragged(gpu, n):
     G = _G()
